@@ -1,0 +1,916 @@
+// HIP kernels of the FFS pseudo-spectral time step (gfx950).
+//
+// One 3D real transform is three pencil passes.  For the vorticity-form step
+// (models.jl:121-214) the passes are fused with the pointwise work around them:
+//   inverse z pass  : reads the retained modes of u, forms w = ik x u on the fly
+//                     (fourierspacep1!, models.jl:53-119), writes 6 fields
+//   inverse y pass  : plain column transform, zero-padding the dealiased rows
+//   x pass          : per (y,z) row, C2R of u and w, u x w (realspacecalculation!,
+//                     models.jl:236-280), 1/2|u|^2 on stats steps (calc_k!,
+//                     models.jl:216-226), max|u_i| for the CFL (find_max,
+//                     utils.jl:247-258), R2C of the products -- physical-space
+//                     fields never reach HBM
+//   forward y pass  : keeps only the retained rows (dealias!, utils.jl:114-120)
+//   forward z pass  : keeps retained rows, scales by 1/(nx ny nz) (myscale!,
+//                     utils.jl:41-51), zeroes the k=0 mode (models.jl:174)
+// The spectral update (fourierspacep2_velocity! models.jl:292-341, AB3
+// Adams_Bashforth.jl:242-256, projection models.jl:443-456) is one
+// elementwise kernel over the retained modes.
+#include "ffs_kernels.h"
+#include "fft_device.h"
+
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+namespace fsk {
+
+template <typename T> __host__ __device__ constexpr int col_tx(int N) {
+  return sizeof(T) == 8 ? 16 : (N <= 256 ? 32 : 16);
+}
+template <int N> __host__ __device__ constexpr int tper() { return N / default_ppt(N); }
+template <int N> __host__ __device__ constexpr int xrows() { return (64 / tper<N>()) > 0 ? (64 / tper<N>()) : 1; }
+
+template <typename T> __device__ __forceinline__ T nanmax(T a, T b) { return (a > b || a != a) ? a : b; }
+
+// wave64 + cross-wave max, then one atomic per block on the IEEE bit pattern
+// (monotone for non-negative values; NaN sorts above +inf so it propagates).
+__device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) {
+  __shared__ double red[16];
+  for (int off = 32; off > 0; off >>= 1) v = nanmax(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 1; i < nw; ++i) m = nanmax(m, red[i]);
+    atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// ===========================================================================
+// Column (y / z) passes.  Block = TX consecutive x columns x TPER threads per
+// column; LDS holds one [N][TX] tile.
+enum ColKind { INV_Z = 0, INV_Y = 1, FWD_Y = 2, FWD_Z = 3 };
+
+template <typename T, int N, int KIND, bool CURL>
+__global__ void __launch_bounds__(512) k_col(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ out,
+                                            C<T>* __restrict__ out2, const C<T>* __restrict__ tw, int nzero,
+                                            T scale) {
+  constexpr int TX = col_tx<T>(N);
+  constexpr int TP = tper<N>();
+  extern __shared__ unsigned char smem_raw[];
+  C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
+  const int c = threadIdx.x % TX;
+  const int q = threadIdx.x / TX;
+  const int cx = blockIdx.x * TX + c;
+  const bool valid = cx < g.mx;
+  const int b = blockIdx.y;
+  const int f = blockIdx.z;
+  const size_t mx = g.mx;
+  const size_t Ssp = mx * g.my * g.mz;  // compact spectral
+  const size_t S1 = mx * g.my * g.nz;   // (mx, my, nz)
+  const size_t S2 = mx * g.ny * g.nz;   // (mx, ny, nz)
+  auto ldsf = [&](int p) -> C<T>& { return lds[p * TX + c]; };
+
+  if constexpr (KIND == INV_Z) {
+    const int cy = b;
+    auto ld = [&](int p) -> C<T> {
+      const int cz = g.zinv[p];
+      if (!valid || cz < 0) return mk<T>(0, 0);
+      const size_t idx = cx + mx * (cy + (size_t)g.my * cz);
+      if (!CURL || f < 3) return in[f * Ssp + idx];
+      const T kx = (T)g.kx[cx], ky = (T)g.ky[cy], kz = (T)g.kz[cz];
+      const C<T> ux = in[idx], uy = in[Ssp + idx], uz = in[2 * Ssp + idx];
+      C<T> a;  // w = i k x u
+      if (f == 3) a = mk<T>(ky * uz.x - kz * uy.x, ky * uz.y - kz * uy.y);
+      else if (f == 4) a = mk<T>(kz * ux.x - kx * uz.x, kz * ux.y - kx * uz.y);
+      else a = mk<T>(kx * uy.x - ky * ux.x, kx * uy.y - ky * ux.y);
+      return mk<T>(-a.y, a.x);
+    };
+    auto st = [&](int p, C<T> v) {
+      if (valid) out[f * S1 + cx + mx * (cy + (size_t)g.my * p)] = v;
+    };
+    Fft<T, N, +1, TP>::run(q, tw, ld, st, ldsf);
+  } else if constexpr (KIND == INV_Y) {
+    const int z = b;
+    auto ld = [&](int p) -> C<T> {
+      const int cy = g.yinv[p];
+      if (!valid || cy < 0) return mk<T>(0, 0);
+      return in[f * S1 + cx + mx * (cy + (size_t)g.my * z)];
+    };
+    auto st = [&](int p, C<T> v) {
+      if (valid) out[f * S2 + cx + mx * (p + (size_t)g.ny * z)] = v;
+    };
+    Fft<T, N, +1, TP>::run(q, tw, ld, st, ldsf);
+  } else if constexpr (KIND == FWD_Y) {
+    const int z = b;
+    auto ld = [&](int p) -> C<T> {
+      if (!valid) return mk<T>(0, 0);
+      return in[f * S2 + cx + mx * (p + (size_t)g.ny * z)];
+    };
+    auto st = [&](int p, C<T> v) {
+      const int cy = g.yinv[p];
+      if (valid && cy >= 0) out[f * S1 + cx + mx * (cy + (size_t)g.my * z)] = v;
+    };
+    Fft<T, N, -1, TP>::run(q, tw, ld, st, ldsf);
+  } else {  // FWD_Z
+    const int cy = b;
+    auto ld = [&](int p) -> C<T> {
+      if (!valid) return mk<T>(0, 0);
+      return in[f * S1 + cx + mx * (cy + (size_t)g.my * p)];
+    };
+    auto st = [&](int p, C<T> v) {
+      const int cz = g.zinv[p];
+      if (valid && cz >= 0) {
+        C<T> r = cscale<T>(v, scale);
+        if (f < nzero && cx == 0 && cy == 0 && cz == 0) r = mk<T>(0, 0);
+        const size_t o = cx + mx * (cy + (size_t)g.my * cz);
+        if (f < 3) out[f * Ssp + o] = r;   // velocity-like fields
+        else out2[(f - 3) * Ssp + o] = r;  // extra fields (1/2|u|^2, ...)
+      }
+    };
+    Fft<T, N, -1, TP>::run(q, tw, ld, st, ldsf);
+  }
+}
+
+template <typename T, int N, int KIND, bool CURL>
+static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, const void* tw, int nzero, double scale,
+                    hipStream_t s, void* out2 = nullptr) {
+  constexpr int TX = col_tx<T>(N);
+  constexpr int TP = tper<N>();
+  const size_t lds = (size_t)N * TX * sizeof(C<T>);
+  static bool attr = false;
+  if (!attr) {
+    hipFuncSetAttribute((const void*)&k_col<T, N, KIND, CURL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
+  }
+  dim3 grid((g.mx + TX - 1) / TX, nb, nf);
+  k_col<T, N, KIND, CURL><<<grid, TX * TP, lds, s>>>(g, (const C<T>*)in, (C<T>*)out, (C<T>*)out2, (const C<T>*)tw, nzero,
+                                                    (T)scale);
+}
+
+#define FFS_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(512)
+
+[[noreturn]] static void bad_size(const char* what, int n) {
+  throw std::runtime_error(std::string("unsupported transform length ") + std::to_string(n) + " in " + what);
+}
+
+template <typename T>
+void launch_inv_z(const Geo& g, const void* u, void* a1, int nf, bool curl, const void* tw, hipStream_t s) {
+  switch (g.nz) {
+#define X(n)                                                                              \
+  case n:                                                                                 \
+    if (curl) run_col<T, n, INV_Z, true>(g, u, a1, nf, g.my, tw, 0, 1.0, s);              \
+    else run_col<T, n, INV_Z, false>(g, u, a1, nf, g.my, tw, 0, 1.0, s);                  \
+    break;
+    FFS_SIZES(X)
+#undef X
+    default: bad_size("inverse z pass", g.nz);
+  }
+}
+
+template <typename T>
+void launch_inv_y(const Geo& g, const void* a1, void* a2, int nf, const void* tw, hipStream_t s) {
+  switch (g.ny) {
+#define X(n)                                                            \
+  case n: run_col<T, n, INV_Y, false>(g, a1, a2, nf, g.nz, tw, 0, 1.0, s); break;
+    FFS_SIZES(X)
+#undef X
+    default: bad_size("inverse y pass", g.ny);
+  }
+}
+
+template <typename T>
+void launch_fwd_y(const Geo& g, const void* b2, void* b1, int nf, const void* tw, hipStream_t s) {
+  switch (g.ny) {
+#define X(n)                                                            \
+  case n: run_col<T, n, FWD_Y, false>(g, b2, b1, nf, g.nz, tw, 0, 1.0, s); break;
+    FFS_SIZES(X)
+#undef X
+    default: bad_size("forward y pass", g.ny);
+  }
+}
+
+template <typename T>
+void launch_fwd_z(const Geo& g, const void* b1, void* out, void* out2, int nf, int nzero, double scale,
+                  const void* tw, hipStream_t s) {
+  switch (g.nz) {
+#define X(n)                                                                        \
+  case n: run_col<T, n, FWD_Z, false>(g, b1, out, nf, g.my, tw, nzero, scale, s, out2); break;
+    FFS_SIZES(X)
+#undef X
+    default: bad_size("forward z pass", g.nz);
+  }
+}
+
+// ===========================================================================
+// x passes.  LDS slots of N+1 complex values; TPER threads per slot.  Two real
+// rows travel as one complex sequence z = a + i b: the inverse builds
+// Z[k] = A[k] + i B[k], Z[N-k] = conj(A[k]) + i conj(B[k]) from the retained
+// half spectra, the forward separates A = (Z[k] + conj Z[N-k])/2,
+// B = (Z[k] - conj Z[N-k])/(2i).
+
+template <typename T, int N>
+__device__ __forceinline__ void build_pair(C<T>* Z, int k, C<T> A, C<T> B) {
+  if (k == 0 || k == N / 2) {
+    Z[k] = mk<T>(A.x, B.x);  // DC / Nyquist of a real row: imaginary parts are not representable
+  } else {
+    Z[k] = mk<T>(A.x - B.y, A.y + B.x);
+    Z[N - k] = mk<T>(A.x + B.y, B.x - A.y);
+  }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void split_pair(const C<T>* Z, int k, C<T>& A, C<T>& B) {
+  const C<T> zk = Z[k], zn = Z[(N - k) % N];
+  A = mk<T>((T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y));
+  B = mk<T>((T)0.5 * (zk.y + zn.y), (T)0.5 * (zn.x - zk.x));
+}
+
+// Fused nonlinear x pass: a2 holds (ux, uy, uz, wx, wy, wz) y/z-transformed,
+// b2 receives the x-forward transform of (u x w)_{x,y,z} and, when KE, 1/2|u|^2.
+template <typename T, int N, bool KE>
+__global__ void __launch_bounds__(256) k_x_nl(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
+                                             const C<T>* __restrict__ tw, unsigned long long* __restrict__ umax) {
+  constexpr int TP = tper<N>();
+  constexpr int ROWS = xrows<N>();
+  constexpr int NSLOT = 3 * ROWS;
+  constexpr int LD = N + 1;
+  constexpr int NTHR = NSLOT * TP;
+  extern __shared__ unsigned char smem_raw[];
+  C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int nrows = g.ny * g.nz;
+  const int row0 = blockIdx.x * ROWS;
+  const int mx = g.mx;
+  const size_t S2 = (size_t)mx * nrows;
+
+  // A: gather retained half spectra into Hermitian-packed pairs (ux,uy) (uz,wx) (wy,wz)
+  for (int w = tid; w < NSLOT * (N / 2 + 1); w += NTHR) {
+    const int sl = w / (N / 2 + 1), k = w % (N / 2 + 1);
+    const int r = sl / 3, pr = sl % 3;
+    const int row = row0 + r;
+    C<T> A = mk<T>(0, 0), B = mk<T>(0, 0);
+    if (row < nrows && k < mx) {
+      A = a2[(2 * pr) * S2 + (size_t)row * mx + k];
+      B = a2[(2 * pr + 1) * S2 + (size_t)row * mx + k];
+    }
+    build_pair<T, N>(lds + sl * LD, k, A, B);
+  }
+  __syncthreads();
+  const int sl = tid / TP, q = tid % TP;
+  C<T>* Zs = lds + sl * LD;
+  // B: inverse transforms (all slots)
+  Fft<T, N, +1, TP, true, true>::run(
+      q, tw, [&](int p) { return Zs[p]; }, [&](int p, C<T> v) { Zs[p] = v; }, [&](int p) -> C<T>& { return Zs[p]; });
+  __syncthreads();
+  // C: pointwise u x w (and 1/2|u|^2), max |u_i|
+  double lmax = 0.0;
+  for (int w = tid; w < ROWS * N; w += NTHR) {
+    const int r = w / N, n = w % N;
+    C<T>* Z0 = lds + (3 * r) * LD;
+    C<T>* Z1 = Z0 + LD;
+    C<T>* Z2 = Z1 + LD;
+    const C<T> z0 = Z0[n], z1 = Z1[n], z2 = Z2[n];
+    const T ux = z0.x, uy = z0.y, uz = z1.x, wx = z1.y, wy = z2.x, wz = z2.y;
+    const T px = uy * wz - uz * wy;
+    const T py = uz * wx - ux * wz;
+    const T pz = ux * wy - uy * wx;
+    T ke = 0;
+    if constexpr (KE) ke = (T)0.5 * (ux * ux + uy * uy + uz * uz);
+    Z0[n] = mk<T>(px, py);
+    Z1[n] = mk<T>(pz, ke);
+    if (row0 + r < nrows) {
+      const T m = nanmax(fabs(ux), nanmax(fabs(uy), fabs(uz)));
+      lmax = nanmax(lmax, (double)m);
+    }
+  }
+  __syncthreads();
+  // D: forward transforms (the third slot of each row is idle work)
+  Fft<T, N, -1, TP, true, true>::run(
+      q, tw, [&](int p) { return Zs[p]; }, [&](int p, C<T> v) { Zs[p] = v; }, [&](int p) -> C<T>& { return Zs[p]; });
+  __syncthreads();
+  // E: separate the two real transforms of each slot, keep k < mx
+  for (int w = tid; w < ROWS * 2 * mx; w += NTHR) {
+    const int k = w % mx, rj = w / mx, r = rj >> 1, j = rj & 1;
+    const int row = row0 + r;
+    if (row >= nrows) continue;
+    C<T> A, B;
+    split_pair<T, N>(lds + (3 * r + j) * LD, k, A, B);
+    b2[(2 * j) * S2 + (size_t)row * mx + k] = A;
+    if (j == 0 || KE) b2[(2 * j + 1) * S2 + (size_t)row * mx + k] = B;
+  }
+  if (umax) block_max_to(lmax, umax);
+}
+
+// C2R of nf fields (rows paired 2m, 2m+1) into the padded real layout
+// (2*nxh, ny, nz), padding entries zeroed.
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_x_c2r(Geo g, const C<T>* __restrict__ a2, T* __restrict__ real,
+                                              const C<T>* __restrict__ tw, int nf) {
+  constexpr int TP = tper<N>();
+  constexpr int NSLOT = xrows<N>() * 3;
+  constexpr int LD = N + 1;
+  constexpr int NTHR = NSLOT * TP;
+  extern __shared__ unsigned char smem_raw[];
+  C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int nrows = g.ny * g.nz;
+  const int npairs = (nrows + 1) / 2;
+  const int mx = g.mx;
+  const size_t S2 = (size_t)mx * nrows;
+  const int rl = 2 * g.nxh;
+  const size_t RS = (size_t)rl * nrows;
+  const int p0 = blockIdx.x * NSLOT;
+
+  for (int w = tid; w < NSLOT * (N / 2 + 1); w += NTHR) {
+    const int sl = w / (N / 2 + 1), k = w % (N / 2 + 1);
+    const int pidx = p0 + sl;
+    C<T> A = mk<T>(0, 0), B = mk<T>(0, 0);
+    if (pidx < nf * npairs && k < mx) {
+      const int f = pidx / npairs, row = 2 * (pidx % npairs);
+      A = a2[f * S2 + (size_t)row * mx + k];
+      if (row + 1 < nrows) B = a2[f * S2 + (size_t)(row + 1) * mx + k];
+    }
+    build_pair<T, N>(lds + sl * LD, k, A, B);
+  }
+  __syncthreads();
+  const int sl = tid / TP, q = tid % TP;
+  C<T>* Zs = lds + sl * LD;
+  Fft<T, N, +1, TP, true, true>::run(
+      q, tw, [&](int p) { return Zs[p]; }, [&](int p, C<T> v) { Zs[p] = v; }, [&](int p) -> C<T>& { return Zs[p]; });
+  __syncthreads();
+  for (int w = tid; w < NSLOT * rl; w += NTHR) {
+    const int sl2 = w / rl, n = w % rl;
+    const int pidx = p0 + sl2;
+    if (pidx >= nf * npairs) continue;
+    const int f = pidx / npairs, row = 2 * (pidx % npairs);
+    const C<T> z = (n < N) ? lds[sl2 * LD + n] : mk<T>(0, 0);
+    real[f * RS + (size_t)row * rl + n] = z.x;
+    if (row + 1 < nrows) real[f * RS + (size_t)(row + 1) * rl + n] = z.y;
+  }
+}
+
+// R2C of nf padded real fields into (mx, ny, nz) half spectra (k < mx kept).
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_x_r2c(Geo g, const T* __restrict__ real, C<T>* __restrict__ b2,
+                                              const C<T>* __restrict__ tw, int nf) {
+  constexpr int TP = tper<N>();
+  constexpr int NSLOT = xrows<N>() * 3;
+  constexpr int LD = N + 1;
+  constexpr int NTHR = NSLOT * TP;
+  extern __shared__ unsigned char smem_raw[];
+  C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int nrows = g.ny * g.nz;
+  const int npairs = (nrows + 1) / 2;
+  const int mx = g.mx;
+  const size_t S2 = (size_t)mx * nrows;
+  const int rl = 2 * g.nxh;
+  const size_t RS = (size_t)rl * nrows;
+  const int p0 = blockIdx.x * NSLOT;
+
+  for (int w = tid; w < NSLOT * N; w += NTHR) {
+    const int sl = w / N, n = w % N;
+    const int pidx = p0 + sl;
+    T a = 0, b = 0;
+    if (pidx < nf * npairs) {
+      const int f = pidx / npairs, row = 2 * (pidx % npairs);
+      a = real[f * RS + (size_t)row * rl + n];
+      if (row + 1 < nrows) b = real[f * RS + (size_t)(row + 1) * rl + n];
+    }
+    lds[sl * LD + n] = mk<T>(a, b);
+  }
+  __syncthreads();
+  const int sl = tid / TP, q = tid % TP;
+  C<T>* Zs = lds + sl * LD;
+  Fft<T, N, -1, TP, true, true>::run(
+      q, tw, [&](int p) { return Zs[p]; }, [&](int p, C<T> v) { Zs[p] = v; }, [&](int p) -> C<T>& { return Zs[p]; });
+  __syncthreads();
+  for (int w = tid; w < NSLOT * mx; w += NTHR) {
+    const int sl2 = w / mx, k = w % mx;
+    const int pidx = p0 + sl2;
+    if (pidx >= nf * npairs) continue;
+    const int f = pidx / npairs, row = 2 * (pidx % npairs);
+    C<T> A, B;
+    split_pair<T, N>(lds + sl2 * LD, k, A, B);
+    b2[f * S2 + (size_t)row * mx + k] = A;
+    if (row + 1 < nrows) b2[f * S2 + (size_t)(row + 1) * mx + k] = B;
+  }
+}
+
+template <typename T, int N>
+static size_t x_lds_bytes() {
+  return (size_t)3 * xrows<N>() * (N + 1) * sizeof(C<T>);
+}
+
+template <typename T>
+void launch_x_nl(const Geo& g, const void* a2, void* b2, bool with_ke, unsigned long long* umax, const void* tw,
+                 hipStream_t s) {
+  const int nrows = g.ny * g.nz;
+  switch (g.nrx) {
+#define X(n)                                                                                              \
+  case n: {                                                                                               \
+    constexpr int R = xrows<n>();                                                                         \
+    const size_t lds = x_lds_bytes<T, n>();                                                               \
+    dim3 grid((nrows + R - 1) / R);                                                                       \
+    if (with_ke)                                                                                          \
+      k_x_nl<T, n, true><<<grid, 3 * R * tper<n>(), lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2, (const C<T>*)tw, umax); \
+    else                                                                                                  \
+      k_x_nl<T, n, false><<<grid, 3 * R * tper<n>(), lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2, (const C<T>*)tw, umax); \
+    break;                                                                                                \
+  }
+    FFS_SIZES(X)
+#undef X
+    default: bad_size("nonlinear x pass", g.nrx);
+  }
+}
+
+template <typename T>
+void launch_x_c2r(const Geo& g, const void* a2, T* real, int nf, const void* tw, hipStream_t s) {
+  const int npairs = (g.ny * g.nz + 1) / 2;
+  switch (g.nrx) {
+#define X(n)                                                                                      \
+  case n: {                                                                                       \
+    constexpr int NS = 3 * xrows<n>();                                                            \
+    dim3 grid((nf * npairs + NS - 1) / NS);                                                       \
+    k_x_c2r<T, n><<<grid, NS * tper<n>(), x_lds_bytes<T, n>(), s>>>(g, (const C<T>*)a2, real, (const C<T>*)tw, nf); \
+    break;                                                                                        \
+  }
+    FFS_SIZES(X)
+#undef X
+    default: bad_size("x C2R pass", g.nrx);
+  }
+}
+
+template <typename T>
+void launch_x_r2c(const Geo& g, const T* real, void* b2, int nf, const void* tw, hipStream_t s) {
+  const int npairs = (g.ny * g.nz + 1) / 2;
+  switch (g.nrx) {
+#define X(n)                                                                                      \
+  case n: {                                                                                       \
+    constexpr int NS = 3 * xrows<n>();                                                            \
+    dim3 grid((nf * npairs + NS - 1) / NS);                                                       \
+    k_x_r2c<T, n><<<grid, NS * tper<n>(), x_lds_bytes<T, n>(), s>>>(g, real, (C<T>*)b2, (const C<T>*)tw, nf); \
+    break;                                                                                        \
+  }
+    FFS_SIZES(X)
+#undef X
+    default: bad_size("x R2C pass", g.nrx);
+  }
+}
+
+// ===========================================================================
+// Spectral update: rhs <- P(rhs - nu k^2 u) (fourierspacep2_velocity!,
+// models.jl:292-341); AB3 (Adams_Bashforth.jl:242-256) with the step-2
+// history extrapolation fix_fm2! (ETD.jl:136-139) done in registers; then the
+// final projection (time_step_functions.jl:78).  rhs is written back: it is
+// the next step's fm1 (history rotated by pointer on the host).
+template <typename T>
+__global__ void __launch_bounds__(256) k_update_ab3(Geo g, C<T>* __restrict__ u, C<T>* __restrict__ rhs,
+                                                   const C<T>* __restrict__ fm1, const C<T>* __restrict__ fm2,
+                                                   const DevState* __restrict__ st, UpdateArgs a) {
+  const size_t n = (size_t)g.mx * g.my * g.mz;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int cx = idx % g.mx;
+  const int cy = (idx / g.mx) % g.my;
+  const int cz = idx / ((size_t)g.mx * g.my);
+  const double kv3[3] = {g.kx[cx], g.ky[cy], g.kz[cz]};
+  const double K2 = kv3[0] * kv3[0] + kv3[1] * kv3[1] + kv3[2] * kv3[2];
+  const double kvis = K2 * (-a.nu);
+  const bool origin = (idx == 0);
+  double U[3][2], R[3][2], M1[3][2], M2[3][2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const C<T> v = u[c * n + idx], r = rhs[c * n + idx], h1 = fm1[c * n + idx], h2 = fm2[c * n + idx];
+    U[c][0] = v.x; U[c][1] = v.y;
+    R[c][0] = r.x; R[c][1] = r.y;
+    M1[c][0] = h1.x; M1[c][1] = h1.y;
+    M2[c][0] = h2.x; M2[c][1] = h2.y;
+  }
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {  // real, imaginary parts
+#pragma unroll
+    for (int c = 0; c < 3; ++c) R[c][p] += kvis * U[c][p];
+    const double p1 = origin ? 0.0 : -(kv3[0] * R[0][p] + kv3[1] * R[1][p] + kv3[2] * R[2][p]) / K2;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) R[c][p] = p1 * kv3[c] + R[c][p];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) rhs[c * n + idx] = mk<T>((T)R[c][0], (T)R[c][1]);
+  if (a.mode == 3) return;  // initialisation: final rhs only (initialize!, FluidFlowSimulation.jl:82)
+  const double At = st->At, Bt = st->Bt, Ct = st->Ct;
+  if (a.mode == 1) {  // fix_fm2!: fm2 = -(dt3+dt2)(rhs - fm1)/dt2 + rhs
+    const double dt2 = st->dt2, dt3 = st->dt3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) M2[c][p] = -(dt3 + dt2) * (R[c][p] - M1[c][p]) / dt2 + R[c][p];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int p = 0; p < 2; ++p) U[c][p] = fma(At, R[c][p], fma(Bt, M1[c][p], fma(Ct, M2[c][p], U[c][p])));
+  // constant-power isotropic shell forcing, added after the integrator like add_forcing!
+  // (time_step_functions.jl:77): u += dt * (P / sum_band w|u|^2) u on 0 < |k|^2 <= kf2
+  if (a.forcing == 1 && K2 <= a.kf2 && !origin) {
+    const double fac = st->dt * st->force_factor;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) U[c][p] += fac * U[c][p];
+  }
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const double p1 = origin ? 0.0 : -(kv3[0] * U[0][p] + kv3[1] * U[1][p] + kv3[2] * U[2][p]) / K2;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) U[c][p] = p1 * kv3[c] + U[c][p];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) u[c * n + idx] = mk<T>((T)U[c][0], (T)U[c][1]);
+}
+
+template <typename T>
+void launch_update_ab3(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
+                       const UpdateArgs& a, hipStream_t s) {
+  const size_t n = (size_t)g.mx * g.my * g.mz;
+  k_update_ab3<T><<<(n + 255) / 256, 256, 0, s>>>(g, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1, (const C<T>*)fm2, st, a);
+}
+
+// Leray projection of a spectral vector field (pressure_projection!, models.jl:443-456)
+template <typename T>
+__global__ void k_project(Geo g, C<T>* __restrict__ u) {
+  const size_t n = (size_t)g.mx * g.my * g.mz;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n || idx == 0) return;
+  const int cx = idx % g.mx;
+  const int cy = (idx / g.mx) % g.my;
+  const int cz = idx / ((size_t)g.mx * g.my);
+  const double kx = g.kx[cx], ky = g.ky[cy], kz = g.kz[cz];
+  const double K2 = kx * kx + ky * ky + kz * kz;
+  const C<T> v0 = u[idx], v1 = u[n + idx], v2 = u[2 * n + idx];
+  double ux[2] = {v0.x, v0.y}, uy[2] = {v1.x, v1.y}, uz[2] = {v2.x, v2.y};
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const double p1 = -(kx * ux[c] + ky * uy[c] + kz * uz[c]) / K2;
+    ux[c] = p1 * kx + ux[c];
+    uy[c] = p1 * ky + uy[c];
+    uz[c] = p1 * kz + uz[c];
+  }
+  u[idx] = mk<T>((T)ux[0], (T)ux[1]);
+  u[n + idx] = mk<T>((T)uy[0], (T)uy[1]);
+  u[2 * n + idx] = mk<T>((T)uz[0], (T)uz[1]);
+}
+
+template <typename T>
+void launch_project(const Geo& g, void* u, hipStream_t s) {
+  const size_t n = (size_t)g.mx * g.my * g.mz;
+  k_project<T><<<(n + 255) / 256, 256, 0, s>>>(g, (C<T>*)u);
+}
+
+// ===========================================================================
+// Statistics (stats.jl velocity_stats :168-219, squared_mean :289-307,
+// viscosity_stats :608-635, d?_squared_mean :334-473, pressure_stats
+// :750-814, non_linear_stats :897-917, integral_lenght_? :952-980).
+// Weight (1 + (i>1)) = 2 for kx > 0.  Each block writes kStatsSlots partial
+// sums; one block then adds the partials in a fixed order, so results are
+// bit-reproducible run to run.
+//  0..2  sum w|u_c|^2            3  sum w k^2 (|ux|^2+|uy|^2)   4  sum w k^2 |uz|^2
+//  5,6   nlh, nlv                7,8 pressh, pressv
+//  9 sum w kx^2|ux|^2  10 sum w ky^2|ux|^2  11 sum w kx^2|uy|^2  12 sum w ky^2|uy|^2
+//  13+3c+{0,1,2}: integral-length plane sums of component c over kx=0 (no weight), ky=0, kz=0
+enum { kStatsN = 22, kStatsBlocks = 512 };
+
+__device__ __forceinline__ double proj2(double ax, double ay, double bx, double by) { return ax * bx + ay * by; }
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_stats(Geo g, const C<T>* __restrict__ u, const C<T>* __restrict__ nl,
+                                              const C<T>* __restrict__ ke, double* __restrict__ partials) {
+  const size_t n = (size_t)g.mx * g.my * g.mz;
+  double acc[kStatsN];
+#pragma unroll
+  for (int i = 0; i < kStatsN; ++i) acc[i] = 0.0;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const int cx = idx % g.mx;
+    const int cy = (idx / g.mx) % g.my;
+    const int cz = idx / ((size_t)g.mx * g.my);
+    const double kx = g.kx[cx], ky = g.ky[cy], kz = g.kz[cz];
+    const double w = cx > 0 ? 2.0 : 1.0;
+    const double K2 = kx * kx + ky * ky + kz * kz;
+    const C<T> a0 = u[idx], a1 = u[n + idx], a2 = u[2 * n + idx];
+    const double ux[2] = {a0.x, a0.y}, uy[2] = {a1.x, a1.y}, uz[2] = {a2.x, a2.y};
+    const double ex = ux[0] * ux[0] + ux[1] * ux[1];
+    const double ey = uy[0] * uy[0] + uy[1] * uy[1];
+    const double ez = uz[0] * uz[0] + uz[1] * uz[1];
+    acc[0] += w * ex;
+    acc[1] += w * ey;
+    acc[2] += w * ez;
+    acc[3] += w * K2 * (ex + ey);
+    acc[4] += w * K2 * ez;
+    acc[9] += w * kx * kx * ex;
+    acc[10] += w * ky * ky * ex;
+    acc[11] += w * kx * kx * ey;
+    acc[12] += w * ky * ky * ey;
+    const double e3[3] = {ex, ey, ez};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (cx == 0) acc[13 + 3 * c] += e3[c];
+      if (ky == 0.0) acc[14 + 3 * c] += w * e3[c];
+      if (kz == 0.0) acc[15 + 3 * c] += w * e3[c];
+    }
+    if (nl) {
+      const C<T> n0 = nl[idx], n1 = nl[n + idx], n2 = nl[2 * n + idx];
+      const C<T> kk = ke[idx];
+      const double kr = kk.x, ki = kk.y;
+      // nl - (i k) ke  (== rhs - (i ke) k of pressure_stats)
+      const double qx[2] = {n0.x + kx * ki, n0.y - kx * kr};
+      const double qy[2] = {n1.x + ky * ki, n1.y - ky * kr};
+      const double qz[2] = {n2.x + kz * ki, n2.y - kz * kr};
+      acc[5] += w * (proj2(qx[0], qx[1], ux[0], ux[1]) + proj2(qy[0], qy[1], uy[0], uy[1]));
+      acc[6] += w * proj2(qz[0], qz[1], uz[0], uz[1]);
+      if (idx != 0) {
+        const double p1r = -(kx * qx[0] + ky * qy[0] + kz * qz[0]) / K2;
+        const double p1i = -(kx * qx[1] + ky * qy[1] + kz * qz[1]) / K2;
+        acc[7] += w * (proj2(ux[0], ux[1], p1r * kx, p1i * kx) + proj2(uy[0], uy[1], p1r * ky, p1i * ky));
+        acc[8] += w * proj2(uz[0], uz[1], p1r * kz, p1i * kz);
+      }
+    }
+  }
+  // block reduction in a fixed tree
+  __shared__ double red[kStatsN][256];
+#pragma unroll
+  for (int i = 0; i < kStatsN; ++i) red[i][threadIdx.x] = acc[i];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+#pragma unroll
+      for (int i = 0; i < kStatsN; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < kStatsN) partials[(size_t)blockIdx.x * kStatsSlots + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// sums nblk rows of `partials` (each kStatsSlots wide) column-wise in a fixed order
+__global__ void k_reduce_partials(const double* __restrict__ partials, int nblk, int ncols, double* __restrict__ out) {
+  __shared__ double red[256];
+  for (int col = 0; col < ncols; ++col) {
+    double a = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) a += partials[(size_t)b * kStatsSlots + col];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[col] = red[0];
+    __syncthreads();
+  }
+}
+
+template <typename T>
+int launch_stats(const Geo& g, const void* u, const void* nl, const void* ke, double* partials, double* out,
+                 hipStream_t s) {
+  const size_t n = (size_t)g.mx * g.my * g.mz;
+  int nblk = (int)((n + 255) / 256);
+  if (nblk > kStatsBlocks) nblk = kStatsBlocks;
+  k_stats<T><<<nblk, 256, 0, s>>>(g, (const C<T>*)u, (const C<T>*)nl, (const C<T>*)ke, partials);
+  k_reduce_partials<<<1, 256, 0, s>>>(partials, nblk, kStatsN, out);
+  return kStatsN;
+}
+
+// ===========================================================================
+// Spectrum marginals for calculate_specHV (spec.jl:424-450) of spectrum_u
+// (spec.jl:316-327): per mode h = (|ux|^2+|uy|^2)/2, v = |uz|^2/2.  The
+// binned spectra are sums of marginals along one axis, so the device
+// produces, for h and v alike,
+//   X[cx]   = sum_{cy,cz} e           Y0[cy] = sum_cz e(cx=0)
+//   Y[cy]   = sum_{cz,cx>0} e         Z0[cz] = sum_cy e(cx=0)
+//   Z[cz]   = sum_{cy,cx>0} e
+// and the host bins them (a few hundred values).  Plane partials go through
+// `partials` (mz x (mx + my) x 2) and are reduced in a fixed order.
+template <typename T>
+__global__ void k_spec_planes(Geo g, const C<T>* __restrict__ u, double* __restrict__ px, double* __restrict__ py,
+                              double* __restrict__ pz) {
+  // one block per z plane; layouts: px[(cz*2 + hv)*mx + cx], py[(cz*2+hv)*my + cy] (cx>0 part),
+  // y0 values are read directly by the second kernel; pz[cz*4 + {h0,v0,h,v}]
+  const int cz = blockIdx.x;
+  const size_t n = (size_t)g.mx * g.my * g.mz;
+  __shared__ double sh[4][256];
+  double z0h = 0, z0v = 0, zh = 0, zv = 0;
+  // X partials: each thread owns cx values, loops over cy
+  for (int cx = threadIdx.x; cx < g.mx; cx += blockDim.x) {
+    double sh_ = 0, sv_ = 0;
+    for (int cy = 0; cy < g.my; ++cy) {
+      const size_t idx = cx + (size_t)g.mx * (cy + (size_t)g.my * cz);
+      const C<T> a0 = u[idx], a1 = u[n + idx], a2 = u[2 * n + idx];
+      const double h = ((double)a0.x * a0.x + (double)a0.y * a0.y + (double)a1.x * a1.x + (double)a1.y * a1.y) / 2;
+      const double v = ((double)a2.x * a2.x + (double)a2.y * a2.y) / 2;
+      sh_ += h;
+      sv_ += v;
+    }
+    px[((size_t)cz * 2 + 0) * g.mx + cx] = sh_;
+    px[((size_t)cz * 2 + 1) * g.mx + cx] = sv_;
+  }
+  // Y (cx > 0) partials and Z sums: each thread owns cy values, loops over cx
+  for (int cy = threadIdx.x; cy < g.my; cy += blockDim.x) {
+    double sh_ = 0, sv_ = 0;
+    for (int cx = 0; cx < g.mx; ++cx) {
+      const size_t idx = cx + (size_t)g.mx * (cy + (size_t)g.my * cz);
+      const C<T> a0 = u[idx], a1 = u[n + idx], a2 = u[2 * n + idx];
+      const double h = ((double)a0.x * a0.x + (double)a0.y * a0.y + (double)a1.x * a1.x + (double)a1.y * a1.y) / 2;
+      const double v = ((double)a2.x * a2.x + (double)a2.y * a2.y) / 2;
+      if (cx == 0) {
+        z0h += h;
+        z0v += v;
+      } else {
+        sh_ += h;
+        sv_ += v;
+      }
+    }
+    py[((size_t)cz * 2 + 0) * g.my + cy] = sh_;
+    py[((size_t)cz * 2 + 1) * g.my + cy] = sv_;
+    zh += sh_;
+    zv += sv_;
+  }
+  sh[0][threadIdx.x] = z0h;
+  sh[1][threadIdx.x] = z0v;
+  sh[2][threadIdx.x] = zh;
+  sh[3][threadIdx.x] = zv;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      for (int i = 0; i < 4; ++i) sh[i][threadIdx.x] += sh[i][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) pz[(size_t)cz * 4 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+template <typename T>
+__global__ void k_spec_finish(Geo g, const C<T>* __restrict__ u, const double* __restrict__ px,
+                              const double* __restrict__ py, double* __restrict__ ox, double* __restrict__ oy0,
+                              double* __restrict__ oy) {
+  // ox[hv*mx + cx] = sum_cz px ; oy[hv*my + cy] = sum_cz py ; oy0[hv*my+cy] = sum_cz e(0,cy,cz)
+  const size_t n = (size_t)g.mx * g.my * g.mz;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 2 * g.mx) {
+    const int hv = t / g.mx, cx = t % g.mx;
+    double a = 0;
+    for (int cz = 0; cz < g.mz; ++cz) a += px[((size_t)cz * 2 + hv) * g.mx + cx];
+    ox[t] = a;
+  } else if (t < 2 * g.mx + 2 * g.my) {
+    const int tt = t - 2 * g.mx;
+    const int hv = tt / g.my, cy = tt % g.my;
+    double a = 0, a0 = 0;
+    for (int cz = 0; cz < g.mz; ++cz) {
+      a += py[((size_t)cz * 2 + hv) * g.my + cy];
+      const size_t idx = (size_t)g.mx * (cy + (size_t)g.my * cz);
+      double e;
+      if (hv == 0) {
+        const C<T> a0v = u[idx], a1v = u[n + idx];
+        e = ((double)a0v.x * a0v.x + (double)a0v.y * a0v.y + (double)a1v.x * a1v.x + (double)a1v.y * a1v.y) / 2;
+      } else {
+        const C<T> a2v = u[2 * n + idx];
+        e = ((double)a2v.x * a2v.x + (double)a2v.y * a2v.y) / 2;
+      }
+      a0 += e;
+    }
+    oy[tt] = a;
+    oy0[tt] = a0;
+  }
+}
+
+template <typename T>
+void launch_spectrum(const Geo& g, const void* u, double* out_x, double* out_y0, double* out_y, double* out_z0,
+                     double* out_z, double* partials, hipStream_t s) {
+  // partials layout: px [mz*2*mx] | py [mz*2*my] | pz [mz*4]
+  double* px = partials;
+  double* py = px + (size_t)g.mz * 2 * g.mx;
+  double* pz = py + (size_t)g.mz * 2 * g.my;
+  k_spec_planes<T><<<g.mz, 256, 0, s>>>(g, (const C<T>*)u, px, py, pz);
+  const int nt = 2 * g.mx + 2 * g.my;
+  k_spec_finish<T><<<(nt + 255) / 256, 256, 0, s>>>(g, (const C<T>*)u, px, py, out_x, out_y0, out_y);
+  (void)out_z0;
+  (void)out_z;  // z marginals are the pz plane sums (copied by the host)
+}
+
+// ===========================================================================
+// isotropic constant-power forcing: factor = P / sum_{0<|k|^2<=kf2} w |u|^2 over
+// the (few) band modes listed by the host, summed in list order by one block.
+template <typename T>
+__global__ void k_forcing_sum(Geo g, const C<T>* __restrict__ u, const int* __restrict__ band, int nband,
+                              DevState* st, double power) {
+  const size_t n = (size_t)g.mx * g.my * g.mz;
+  double a = 0;
+  for (int b = threadIdx.x; b < nband; b += blockDim.x) {
+    const int idx = band[b];
+    const double w = (idx % g.mx) > 0 ? 2.0 : 1.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const C<T> v = u[c * n + idx];
+      a += w * ((double)v.x * v.x + (double)v.y * v.y);
+    }
+  }
+  __shared__ double red[64];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int off = 32; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) st->force_factor = red[0] > 0 ? power / red[0] : 0.0;
+}
+
+template <typename T>
+void launch_forcing_sum(const Geo& g, const void* u, const int* band, int nband, DevState* st, double power,
+                        hipStream_t s) {
+  k_forcing_sum<T><<<1, 64, 0, s>>>(g, (const C<T>*)u, band, nband, st, power);
+}
+
+// ===========================================================================
+// Variable time step (set_dt!, time_step_functions.jl:86-129) with the AB3
+// history shift (Adams_Bashforth.jl:166-176) and coefficients (:178-218).
+// `it` is s.iteration[], `tit` the integrator's own iteration counter.
+__device__ void ab3_coefficients(DevState* st) {
+  const double dt = st->dt, dt2 = st->dt2, dt3 = st->dt3;
+  double At = 1. + (dt * (2 * dt + 6 * dt2 + 3 * dt3)) / (6 * dt2 * (dt2 + dt3));
+  At *= dt;
+  double Bt = -dt * dt * (2 * dt + 3 * dt2 + 3 * dt3) / (6 * dt2 * dt3 * (dt + dt2));
+  Bt *= (dt + dt2);
+  double Ct = dt * dt * (2 * dt + 3 * dt2) / (6 * dt3 * (dt + dt2 + dt3) * (dt2 + dt3));
+  Ct *= (dt + dt2 + dt3);
+  st->At = At;
+  st->Bt = Bt;
+  st->Ct = Ct;
+}
+
+__global__ void k_set_dt_ab3(DevState* st, int it, int tit, double cfl, double dx) {
+  const double umax = __longlong_as_double((long long)st->umax);
+  st->umax_d = umax;
+  st->umax = 0ull;  // cleared for the next step's atomicMax
+  if (it == 0) cfl *= 0.1;
+  else if (it == 1) cfl *= 0.2;
+  else if (it == 2) cfl *= 0.4;
+  else if (it == 3) cfl *= 0.8;
+  double newdt = cfl * dx / umax;
+  if (it > 3) {
+    const double dt = st->dt;
+    if (newdt < dt) newdt = fmax(dt / 1.25, newdt);
+    else newdt = fmin(1.25 * dt, newdt);
+  }
+  if (tit == 0) {
+    st->dt3 = newdt / 2;
+    st->dt2 = newdt / 2;
+    st->dt = newdt / 2;
+  } else {
+    st->dt3 = st->dt2;
+    st->dt2 = st->dt;
+    st->dt = newdt;
+  }
+  ab3_coefficients(st);
+}
+
+void launch_set_dt_ab3(DevState* st, int it, int tit, double cfl, double dx, hipStream_t s) {
+  k_set_dt_ab3<<<1, 1, 0, s>>>(st, it, tit, cfl, dx);
+}
+
+__global__ void k_end_step(DevState* st) { st->time += st->dt; }
+void launch_end_step(DevState* st, hipStream_t s) { k_end_step<<<1, 1, 0, s>>>(st); }
+
+template <typename S, typename D>
+__global__ void k_cast_copy(const S* __restrict__ a, D* __restrict__ b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (D)a[i];
+}
+
+// copies n reals from double to T
+template <typename T>
+void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
+  k_cast_copy<double, T><<<(n + 255) / 256, 256, 0, s>>>((const double*)src, (T*)dst, n);
+}
+
+// ---- explicit instantiations ----
+#define FFS_INST(T)                                                                                                   \
+  template void launch_inv_z<T>(const Geo&, const void*, void*, int, bool, const void*, hipStream_t);                 \
+  template void launch_inv_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t);                       \
+  template void launch_x_nl<T>(const Geo&, const void*, void*, bool, unsigned long long*, const void*, hipStream_t);  \
+  template void launch_x_c2r<T>(const Geo&, const void*, T*, int, const void*, hipStream_t);                          \
+  template void launch_x_r2c<T>(const Geo&, const T*, void*, int, const void*, hipStream_t);                          \
+  template void launch_fwd_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t);                       \
+  template void launch_fwd_z<T>(const Geo&, const void*, void*, void*, int, int, double, const void*, hipStream_t);          \
+  template void launch_update_ab3<T>(const Geo&, void*, void*, const void*, const void*, const DevState*,             \
+                                     const UpdateArgs&, hipStream_t);                                                 \
+  template void launch_project<T>(const Geo&, void*, hipStream_t);                                                   \
+  template int launch_stats<T>(const Geo&, const void*, const void*, const void*, double*, double*, hipStream_t);     \
+  template void launch_spectrum<T>(const Geo&, const void*, double*, double*, double*, double*, double*, double*,     \
+                                   hipStream_t);                                                                      \
+  template void launch_forcing_sum<T>(const Geo&, const void*, const int*, int, DevState*, double, hipStream_t);               \
+  template void launch_cast_copy<T>(const void*, void*, size_t, hipStream_t);
+
+FFS_INST(double)
+FFS_INST(float)
+
+}  // namespace fsk

@@ -1,0 +1,947 @@
+// Host driver of libffs_hip.so: one `ffs_sim` per run, owning its device
+// buffers, twiddle tables, HIP stream and a queue of statistics records.
+//
+// The step follows advance_in_time! (models.jl:3-17):
+//   calculate_rhs!  -> inv_z (+curl) / inv_y / fused x pass / fwd_y / fwd_z
+//   set_dt!         -> one-thread kernel on device-resident dt (variable dt)
+//   forcing         -> band reduction (isotropic shell forcing)
+//   writestats      -> deterministic reductions, copied asynchronously
+//   time_step!      -> fused update kernel, history rotated by pointer
+#include "ffs_kernels.h"
+#include "ffs/ffs.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+using namespace fsk;
+
+struct FfsError : std::runtime_error {
+  int code;
+  FfsError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(x)                                                                                     \
+  do {                                                                                                  \
+    hipError_t e_ = (x);                                                                                \
+    if (e_ != hipSuccess)                                                                               \
+      throw FfsError(FFS_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + ":" + \
+                                      std::to_string(__LINE__) + ")");                                  \
+  } while (0)
+
+std::mutex g_err_mu;
+std::string g_last_create_error;
+
+// rfftfreq / fftfreq of Globals.jl:95-104 (index -> wavenumber)
+std::vector<double> rfftfreq(int n, double s) {
+  std::vector<double> v(n / 2 + 1);
+  for (int i = 0; i <= n / 2; ++i) v[i] = i / s;
+  return v;
+}
+std::vector<double> fftfreq(int n, double s) {
+  std::vector<double> v(n);
+  const int h = (n % 2 == 0) ? n / 2 : (n + 1) / 2;  // non-negative count
+  for (int i = 0; i < n; ++i) v[i] = (i < h ? i : i - n) / s;
+  return v;
+}
+
+template <typename T>
+std::vector<T> twiddles(int n) {
+  std::vector<T> t(2 * (size_t)n);
+  const long double tau = 6.283185307179586476925286766559005768L;
+  for (int m = 0; m < n; ++m) {
+    const long double a = -tau * (long double)m / (long double)n;
+    t[2 * m] = (T)cosl(a);
+    t[2 * m + 1] = (T)sinl(a);
+  }
+  return t;
+}
+
+struct StatsSlot {
+  double* red = nullptr;   // pinned: reduction results
+  DevState* st = nullptr;  // pinned: state snapshot
+  hipEvent_t ev = nullptr;
+  int64_t iteration = 0;
+  bool busy = false;
+};
+
+struct Record {
+  std::vector<double> stats, scales, numbers;
+};
+
+}  // namespace
+
+struct ffs_sim {
+  ffs_params p{};
+  bool fp32 = false;
+  size_t csz = 16;  // bytes per complex element
+  Geo g{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+
+  // grid (host)
+  std::vector<int> ymap, zmap, yinv, zinv;
+  std::vector<double> KX, KY, KZ;  // full FFT-order wavenumbers
+  std::vector<double> kxc, kyc, kzc;
+  double scale = 1.0;
+
+  // device buffers
+  std::vector<void*> allocs;
+  size_t dev_bytes = 0;
+  size_t S = 0;  // compact spectral elements per component
+  void* u = nullptr;
+  void* rhs = nullptr;
+  void* fm1 = nullptr;
+  void* fm2 = nullptr;
+  void* ke = nullptr;
+  void* w1 = nullptr;
+  void* w2 = nullptr;
+  void *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr;
+  DevState* st = nullptr;
+  double* partials = nullptr;
+  double* red = nullptr;
+  int* band = nullptr;
+  int nband = 0;
+  double kf2 = 0.0;
+
+  // run state (host mirror of s.iteration[] and the integrator's iteration)
+  int64_t iteration = 0;
+  int64_t tit = 0;
+  bool initialized = false;
+  int64_t output_served = -1;
+  double host_time = 0.0;
+
+  // stats
+  std::vector<StatsSlot> slots;
+  std::deque<int> pending;
+  std::deque<Record> ready;
+  int nstats = 0;
+
+  // profiling
+  bool prof = false;
+  struct ProfEv {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<ProfEv> prof_pending;
+  std::vector<hipEvent_t> ev_pool;
+  std::map<std::string, std::pair<double, int64_t>> prof_acc;
+  std::vector<std::string> prof_names;
+
+  std::string err;
+
+  void* dalloc(size_t bytes) {
+    void* ptr = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(&ptr, bytes);
+    if (e != hipSuccess)
+      throw FfsError(FFS_ERR_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    allocs.push_back(ptr);
+    dev_bytes += bytes;
+    return ptr;
+  }
+
+  template <typename V>
+  V* upload(const std::vector<V>& v) {
+    V* d = (V*)dalloc(v.size() * sizeof(V));
+    HIPCHECK(hipMemcpy(d, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice));
+    return d;
+  }
+
+  hipEvent_t get_ev() {
+    if (!ev_pool.empty()) {
+      hipEvent_t e = ev_pool.back();
+      ev_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return e;
+  }
+
+  template <class F>
+  void launch(const char* name, F&& f) {
+    if (!prof) {
+      f();
+      return;
+    }
+    hipEvent_t a = get_ev(), b = get_ev();
+    HIPCHECK(hipEventRecord(a, stream));
+    f();
+    HIPCHECK(hipEventRecord(b, stream));
+    prof_pending.push_back({name, a, b});
+    if (prof_pending.size() > 4096) drain_prof();
+  }
+
+  void drain_prof() {
+    for (auto& e : prof_pending) {
+      HIPCHECK(hipEventSynchronize(e.b));
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, e.a, e.b));
+      auto& acc = prof_acc[e.name];
+      acc.first += ms;
+      acc.second += 1;
+      ev_pool.push_back(e.a);
+      ev_pool.push_back(e.b);
+    }
+    prof_pending.clear();
+  }
+
+  // ---------------------------------------------------------------- setup
+  void setup_grid() {
+    const int nrx = p.nx, ny = p.ny, nz = p.nz;
+    if (nrx <= 0 || ny <= 0 || nz <= 0) throw FfsError(FFS_ERR_ARG, "nx, ny, nz must be positive");
+    if (nrx % 2) throw FfsError(FFS_ERR_ARG, "nx must be even");
+    const int nxh = nrx / 2 + 1;
+    KX = rfftfreq(nrx, p.x_domain);
+    KY = fftfreq(ny, p.y_domain);
+    KZ = fftfreq(nz, p.z_domain);
+    // DEALIAS = (kx^2 > (c kx_max)^2) | (ky^2 > (c max|ky|)^2) | (kz^2 > (c max|kz|)^2)   Globals.jl:132-142
+    double mky = 0, mkz = 0;
+    for (double v : KY) mky = std::max(mky, std::fabs(v));
+    for (double v : KZ) mkz = std::max(mkz, std::fabs(v));
+    const double cx2 = std::pow(p.cutoff * KX.back(), 2);
+    const double cy2 = std::pow(p.cutoff * mky, 2);
+    const double cz2 = std::pow(p.cutoff * mkz, 2);
+    int mx = 0;
+    while (mx < nxh && !(KX[mx] * KX[mx] > cx2)) ++mx;
+    ymap.clear();
+    zmap.clear();
+    yinv.assign(ny, -1);
+    zinv.assign(nz, -1);
+    for (int j = 0; j < ny; ++j)
+      if (!(KY[j] * KY[j] > cy2)) {
+        yinv[j] = (int)ymap.size();
+        ymap.push_back(j);
+      }
+    for (int k = 0; k < nz; ++k)
+      if (!(KZ[k] * KZ[k] > cz2)) {
+        zinv[k] = (int)zmap.size();
+        zmap.push_back(k);
+      }
+    if (mx == 0 || ymap.empty() || zmap.empty()) throw FfsError(FFS_ERR_ARG, "dealiasing removes every mode");
+    kxc.assign(KX.begin(), KX.begin() + mx);
+    kyc.resize(ymap.size());
+    kzc.resize(zmap.size());
+    for (size_t j = 0; j < ymap.size(); ++j) kyc[j] = KY[ymap[j]];
+    for (size_t k = 0; k < zmap.size(); ++k) kzc[k] = KZ[zmap[k]];
+    g.nrx = nrx;
+    g.ny = ny;
+    g.nz = nz;
+    g.nxh = nxh;
+    g.mx = mx;
+    g.my = (int)ymap.size();
+    g.mz = (int)zmap.size();
+    g.ymap = upload(ymap);
+    g.zmap = upload(zmap);
+    g.yinv = upload(yinv);
+    g.zinv = upload(zinv);
+    g.kx = upload(kxc);
+    g.ky = upload(kyc);
+    g.kz = upload(kzc);
+    scale = 1.0 / ((double)nrx * ny * nz);
+    S = (size_t)g.mx * g.my * g.mz;
+  }
+
+  template <typename T>
+  void* tw_table(int n) {
+    return upload(twiddles<T>(n));
+  }
+
+  void setup_buffers() {
+    csz = fp32 ? 8 : 16;
+    u = dalloc(3 * S * csz);
+    rhs = dalloc(3 * S * csz);
+    fm1 = dalloc(3 * S * csz);
+    fm2 = dalloc(3 * S * csz);
+    ke = dalloc(S * csz);
+    HIPCHECK(hipMemset(u, 0, 3 * S * csz));
+    HIPCHECK(hipMemset(rhs, 0, 3 * S * csz));
+    HIPCHECK(hipMemset(fm1, 0, 3 * S * csz));
+    HIPCHECK(hipMemset(fm2, 0, 3 * S * csz));
+    HIPCHECK(hipMemset(ke, 0, S * csz));
+    const size_t a1 = 6 * (size_t)g.mx * g.my * g.nz, b2 = 4 * (size_t)g.mx * g.ny * g.nz;
+    const size_t a2 = 6 * (size_t)g.mx * g.ny * g.nz, b1 = 4 * (size_t)g.mx * g.my * g.nz;
+    w1 = dalloc(std::max(a1, b2) * csz);
+    w2 = dalloc(std::max(a2, b1) * csz);
+    if (fp32) {
+      tw_x = tw_table<float>(g.nrx);
+      tw_y = tw_table<float>(g.ny);
+      tw_z = tw_table<float>(g.nz);
+    } else {
+      tw_x = tw_table<double>(g.nrx);
+      tw_y = tw_table<double>(g.ny);
+      tw_z = tw_table<double>(g.nz);
+    }
+    st = (DevState*)dalloc(sizeof(DevState));
+    const size_t npart = std::max<size_t>((size_t)512 * kStatsSlots,
+                                          (size_t)g.mz * 2 * (g.mx + g.my) + (size_t)g.mz * 4 + 64);
+    partials = (double*)dalloc(npart * sizeof(double));
+    red = (double*)dalloc(64 * sizeof(double) + 4 * (size_t)(g.mx + g.my) * sizeof(double));
+    // forcing band (0 < |k|^2 <= kf^2), listed in index order
+    if (p.forcing == FFS_FORCING_SHELL) {
+      kf2 = p.forcing_kf * p.forcing_kf;
+      std::vector<int> b;
+      for (int cz = 0; cz < g.mz; ++cz)
+        for (int cy = 0; cy < g.my; ++cy)
+          for (int cx = 0; cx < g.mx; ++cx) {
+            const double K2 = kxc[cx] * kxc[cx] + kyc[cy] * kyc[cy] + kzc[cz] * kzc[cz];
+            if (K2 > 0 && K2 <= kf2) b.push_back(cx + g.mx * (cy + g.my * cz));
+          }
+      nband = (int)b.size();
+      if (nband) band = upload(b);
+    }
+  }
+
+  void setup_state() {
+    DevState h{};
+    h.dt = h.dt2 = h.dt3 = p.dt;
+    h.time = p.start_time;
+    set_ab3_coefficients(h);
+    HIPCHECK(hipMemcpy(st, &h, sizeof(h), hipMemcpyHostToDevice));
+    host_time = p.start_time;
+    iteration = p.start;
+    tit = 0;
+  }
+
+  // get_At/get_Bt/get_Ct of Adams_Bashforth.jl:178-218 (variable-step formulas;
+  // with dt = dt2 = dt3 they give 23/12, -16/12, 5/12 times dt)
+  static void set_ab3_coefficients(DevState& h) {
+    const double dt = h.dt, dt2 = h.dt2, dt3 = h.dt3;
+    double At = 1. + (dt * (2 * dt + 6 * dt2 + 3 * dt3)) / (6 * dt2 * (dt2 + dt3));
+    At *= dt;
+    double Bt = -dt * dt * (2 * dt + 3 * dt2 + 3 * dt3) / (6 * dt2 * dt3 * (dt + dt2));
+    Bt *= (dt + dt2);
+    double Ct = dt * dt * (2 * dt + 3 * dt2) / (6 * dt3 * (dt + dt2 + dt3) * (dt2 + dt3));
+    Ct *= (dt + dt2 + dt3);
+    h.At = At;
+    h.Bt = Bt;
+    h.Ct = Ct;
+  }
+
+  void create(const ffs_params& in) {
+    p = in;
+    if (p.size != sizeof(ffs_params)) throw FfsError(FFS_ERR_ARG, "ffs_params.size mismatch (ABI version)");
+    if (p.integrator != FFS_AB3)
+      throw FfsError(FFS_ERR_ARG, "only the Adams_Bashforth3rdO integrator is available in this build");
+    if (p.les_model != FFS_LES_NONE) throw FfsError(FFS_ERR_ARG, "LES models are not available in this build");
+    if (p.density) throw FfsError(FFS_ERR_ARG, "density stratification is not available in this build");
+    if (p.dt_stat <= 0 || p.dt_spec <= 0 || p.write_time <= 0)
+      throw FfsError(FFS_ERR_ARG, "dtStat, dtSpec and writeTime must be positive");
+    if (!p.variable_dt && !(p.dt > 0)) throw FfsError(FFS_ERR_ARG, "a fixed time step needs dt > 0");
+    fp32 = (p.precision == FFS_FP32);
+    HIPCHECK(hipGetDevice(&device));
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    setup_grid();
+    setup_buffers();
+    setup_state();
+    slots.resize(64);
+    for (auto& sl : slots) {
+      HIPCHECK(hipHostMalloc((void**)&sl.red, 64 * sizeof(double), hipHostMallocDefault));
+      HIPCHECK(hipHostMalloc((void**)&sl.st, sizeof(DevState), hipHostMallocDefault));
+      HIPCHECK(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+    }
+  }
+
+  void destroy() {
+    if (stream) hipStreamSynchronize(stream);
+    for (auto& e : prof_pending) {
+      hipEventDestroy(e.a);
+      hipEventDestroy(e.b);
+    }
+    for (auto e : ev_pool) hipEventDestroy(e);
+    for (auto& sl : slots) {
+      if (sl.red) hipHostFree(sl.red);
+      if (sl.st) hipHostFree(sl.st);
+      if (sl.ev) hipEventDestroy(sl.ev);
+    }
+    for (void* a : allocs) hipFree(a);
+    allocs.clear();
+    if (stream) hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+
+  // ---------------------------------------------------------------- transforms
+#define FFS_DISPATCH(fn, ...)           \
+  do {                                  \
+    if (fp32) fn<float>(__VA_ARGS__);   \
+    else fn<double>(__VA_ARGS__);       \
+  } while (0)
+
+  char* cptr(void* base, size_t elems) { return (char*)base + elems * csz; }
+
+  // compact spectral (nf components at `spec`) -> padded real (nf fields at `real`, precision T)
+  void inverse_to_real(void* spec, int nf, void* real) {
+    launch("inv_z", [&] { FFS_DISPATCH(launch_inv_z, g, spec, w1, nf, false, tw_z, stream); });
+    launch("inv_y", [&] { FFS_DISPATCH(launch_inv_y, g, w1, w2, nf, tw_y, stream); });
+    if (fp32) launch("x_c2r", [&] { launch_x_c2r<float>(g, w2, (float*)real, nf, tw_x, stream); });
+    else launch("x_c2r", [&] { launch_x_c2r<double>(g, w2, (double*)real, nf, tw_x, stream); });
+  }
+
+  // padded real -> compact spectral, dealiased and scaled (myfourier!, utils.jl:101-106)
+  void forward_from_real(void* real, int nf, void* spec) {
+    if (fp32) launch("x_r2c", [&] { launch_x_r2c<float>(g, (const float*)real, w1, nf, tw_x, stream); });
+    else launch("x_r2c", [&] { launch_x_r2c<double>(g, (const double*)real, w1, nf, tw_x, stream); });
+    launch("fwd_y", [&] { FFS_DISPATCH(launch_fwd_y, g, w1, w2, nf, tw_y, stream); });
+    launch("fwd_z", [&] { FFS_DISPATCH(launch_fwd_z, g, w2, spec, nullptr, nf, 0, scale, tw_z, stream); });
+  }
+
+  // ---------------------------------------------------------------- the step
+  bool is_stats_time(int64_t it) const { return it % p.dt_stat == 0; }
+  bool is_spec_time(int64_t it) const { return it % p.dt_spec == 0; }
+  bool is_output_time(int64_t it) const { return tit != 0 && it % p.write_time == 0; }
+
+  // calculate_rhs! up to (not including) fourierspacep2!: rhs <- dealias(F(u x w))/N^3,
+  // ke <- F(1/2|u|^2) on stats/spectra steps, max|u_i| into st->umax
+  void nonlinear(bool need_ke) {
+    launch("inv_z_curl", [&] { FFS_DISPATCH(launch_inv_z, g, u, w1, 6, true, tw_z, stream); });
+    launch("inv_y", [&] { FFS_DISPATCH(launch_inv_y, g, w1, w2, 6, tw_y, stream); });
+    unsigned long long* um = p.variable_dt ? &st->umax : nullptr;
+    launch("x_nl", [&] { FFS_DISPATCH(launch_x_nl, g, w2, w1, need_ke, um, tw_x, stream); });
+    const int nf = need_ke ? 4 : 3;
+    launch("fwd_y", [&] { FFS_DISPATCH(launch_fwd_y, g, w1, w2, nf, tw_y, stream); });
+    launch("fwd_z", [&] { FFS_DISPATCH(launch_fwd_z, g, w2, rhs, ke, nf, 3, scale, tw_z, stream); });
+  }
+
+  void set_dt() {
+    if (!p.variable_dt) return;
+    const double dx = 2 * M_PI * p.y_domain / p.ny;
+    launch("set_dt", [&] { launch_set_dt_ab3(st, (int)std::min<int64_t>(iteration, 1000), (int)std::min<int64_t>(tit, 1000), p.cfl, dx, stream); });
+  }
+
+  void forcing() {
+    if (p.forcing == FFS_FORCING_SHELL && nband > 0)
+      launch("forcing_sum", [&] { FFS_DISPATCH(launch_forcing_sum, g, u, band, nband, st, p.forcing_power, stream); });
+  }
+
+  void queue_stats() {
+    // find a free slot (complete the oldest pending record if all are busy)
+    int slot = -1;
+    for (size_t i = 0; i < slots.size(); ++i)
+      if (!slots[i].busy) {
+        slot = (int)i;
+        break;
+      }
+    if (slot < 0) {
+      finish_oldest();
+      for (size_t i = 0; i < slots.size(); ++i)
+        if (!slots[i].busy) {
+          slot = (int)i;
+          break;
+        }
+    }
+    auto& sl = slots[slot];
+    launch("stats", [&] { FFS_DISPATCH(launch_stats, g, u, rhs, ke, partials, red, stream); });
+    HIPCHECK(hipMemcpyAsync(sl.red, red, 32 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipMemcpyAsync(sl.st, st, sizeof(DevState), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipEventRecord(sl.ev, stream));
+    sl.iteration = iteration;
+    sl.busy = true;
+    pending.push_back(slot);
+  }
+
+  void finish_oldest() {
+    if (pending.empty()) return;
+    const int slot = pending.front();
+    pending.pop_front();
+    auto& sl = slots[slot];
+    HIPCHECK(hipEventSynchronize(sl.ev));
+    ready.push_back(make_record(sl.red, *sl.st, sl.iteration));
+    sl.busy = false;
+  }
+
+  // stats(s) of stats.jl:64-165 from the device sums
+  Record make_record(const double* a, const DevState& sd, int64_t it) const {
+    const double nu = p.nu;
+    const double u1 = 0, u2 = 0, u3 = 0;  // real(u[1,1,1]): the k=0 mode is zero (dealiased mean flow)
+    const double u12 = a[0], u22 = a[1], u32 = a[2];
+    const double kh = (u12 + u22) / 2, kv = u32 / 2, k = kh + kv, kaniso = 2 * kv / kh;
+    const double eh = nu * a[3], ev = nu * a[4], e = eh + ev;
+    const double nlh = a[5], nlv = a[6], pressh = a[7], pressv = a[8];
+    Record r;
+    const double time = sd.time;
+    r.stats = {(double)it, time, u1, u2, u3, u12, u22, u32, kh, kv, kaniso, k, eh, ev, e,
+               nlh, nlv, pressh, pressv, -(nlh + pressh), (nlv + pressv)};
+    const double u1p2 = u12, u2p2 = u22, u3p2 = u32;
+    const double uh = std::sqrt((u1p2 + u2p2) / 2);
+    const double urms = std::sqrt((u1p2 + u2p2 + u3p2) / 3);
+    const double LX = p.x_domain, LY = p.y_domain, LZ = p.z_domain;
+    const double L11 = LX * a[13] * M_PI / u1p2, L12 = LY * a[14] * M_PI / u1p2, L13 = LZ * a[15] * M_PI / u1p2;
+    const double du1dyp2 = a[10], du1dxp2 = a[9], du2dyp2 = a[12], du2dxp2 = a[11];
+    const double l12 = std::sqrt(2 * u1p2 / du1dyp2);
+    const double L21 = LX * a[16] * M_PI / u2p2, L22 = LY * a[17] * M_PI / u2p2, L23 = LZ * a[18] * M_PI / u2p2;
+    const double l21 = std::sqrt(2 * u2p2 / du2dxp2);
+    const double L31 = LX * a[19] * M_PI / u3p2, L32 = LY * a[20] * M_PI / u3p2, L33 = LZ * a[21] * M_PI / u3p2;
+    const double Lhh = (L11 + L22) / 2, Thh = Lhh / urms;
+    const double ll = std::sqrt(2 * (u1p2 + u2p2) / (du1dxp2 + du2dyp2));
+    const double lt = std::sqrt(2 * (u1p2 + u2p2) / (du1dyp2 + du2dxp2));
+    const double Lhv = (L13 + L23) / 2, Lvv = L33, Lvh = (L31 + L32) / 2;
+    const double tdiss = e;
+    r.stats.push_back(tdiss);
+    const double Lt = std::sqrt(k * k * k) / tdiss, Tt = k / tdiss;
+    const double Lk = std::sqrt(std::sqrt(nu * nu * nu / tdiss)), Tk = std::sqrt(nu / tdiss);
+    const double lg = std::sqrt(15 * nu / tdiss) * urms;
+    r.scales = {(double)it, time, Lk, Lt, lg, ll, lt, Lhh, Lvv, Lhv, Lvh, l12, l21, L11, L12, L13, L21, L22, L23,
+                L31, L32, L33, Tk, Tt, Thh};
+    const double Reh = uh * Lhh / nu, Reh1 = std::pow(Lhh / Lk, 4.0 / 3);
+    const double Rel = urms * ll / nu, Rel1 = std::pow(ll / Lk, 4.0 / 3);
+    const double Ret = urms * lt / nu, Ret1 = std::pow(lt / Lk, 4.0 / 3);
+    r.numbers = {(double)it, time, Reh, Reh1, Rel, Ret, Rel1, Ret1};
+    return r;
+  }
+
+  void time_step() {
+    UpdateArgs a{};
+    a.mode = (tit == 1) ? 1 : 0;
+    a.nu = p.nu;
+    a.forcing = (p.forcing == FFS_FORCING_SHELL && nband > 0) ? 1 : 0;
+    a.kf2 = kf2;
+    launch("update_ab3", [&] { FFS_DISPATCH(launch_update_ab3, g, u, rhs, fm1, fm2, st, a, stream); });
+    // history rotation: fm2 <- fm1, fm1 <- rhs, old fm2 becomes the next rhs buffer
+    void* old2 = fm2;
+    fm2 = fm1;
+    fm1 = rhs;
+    rhs = old2;
+    tit += 1;
+    iteration += 1;
+    launch("end_step", [&] { launch_end_step(st, stream); });
+    if (!p.variable_dt) host_time += p.dt;
+  }
+
+  void advance_one() {
+    const bool stats_t = is_stats_time(iteration);
+    const bool spec_t = is_spec_time(iteration);
+    nonlinear(stats_t || spec_t);
+    set_dt();
+    forcing();
+    if (stats_t) queue_stats();
+    time_step();
+  }
+
+  void initialize() {
+    if (initialized) throw FfsError(FFS_ERR_STATE, "already initialized");
+    // calculate_rhs!(s) at iteration `start`: nonlinear term from the (dealiased) initial field,
+    // then the projection of u done by realspace! (models.jl:192-193), then fourierspacep2!
+    nonlinear(false);
+    set_dt();
+    launch("project", [&] { FFS_DISPATCH(launch_project, g, u, stream); });
+    UpdateArgs a{};
+    a.mode = 3;
+    a.nu = p.nu;
+    launch("update_rhs", [&] { FFS_DISPATCH(launch_update_ab3, g, u, rhs, fm1, fm2, st, a, stream); });
+    // initialize!(timestep): fm1 = fm2 = rhs; dt2 = dt3 = dt (Adams_Bashforth.jl:157-164)
+    HIPCHECK(hipMemcpyAsync(fm1, rhs, 3 * S * csz, hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(fm2, rhs, 3 * S * csz, hipMemcpyDeviceToDevice, stream));
+    initialized = true;
+  }
+
+  void writestats_now() {
+    nonlinear(true);
+    forcing();
+    queue_stats();
+  }
+};
+
+namespace {
+
+int fail(ffs_sim* s, const FfsError& e) {
+  if (s) s->err = e.what();
+  return e.code;
+}
+
+template <class F>
+int guard(ffs_sim* s, F&& f) {
+  try {
+    f();
+    return FFS_OK;
+  } catch (const FfsError& e) {
+    return fail(s, e);
+  } catch (const std::exception& e) {
+    if (s) s->err = e.what();
+    return FFS_ERR_ARG;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffs_abi_version(void) { return FFS_ABI_VERSION; }
+
+int ffs_params_default(ffs_params* p) {
+  if (!p) return FFS_ERR_ARG;
+  std::memset(p, 0, sizeof(*p));
+  p->size = sizeof(ffs_params);
+  p->x_domain = p->y_domain = p->z_domain = 1.0;
+  p->cutoff = 2.0 / 3.0;
+  p->integrator = FFS_ETD3;
+  p->variable_dt = 1;
+  p->cfl = 0.18;
+  p->dt = 0.0;
+  p->dt_stat = p->dt_spec = p->write_time = 1 << 30;
+  p->precision = FFS_FP64;
+  p->smagorinsky_c = 0.17;
+  p->filter_width = 0.0;
+  p->pr = 1.0;
+  p->reference_density = 1.0;
+  p->forcing_kf = 2.5;
+  p->forcing_power = 0.1;
+  return FFS_OK;
+}
+
+int ffs_create(const ffs_params* p, ffs_sim** out) {
+  if (!p || !out) return FFS_ERR_ARG;
+  *out = nullptr;
+  ffs_sim* s = new ffs_sim();
+  try {
+    s->create(*p);
+  } catch (const FfsError& e) {
+    {
+      std::lock_guard<std::mutex> lk(g_err_mu);
+      g_last_create_error = e.what();
+    }
+    s->destroy();
+    delete s;
+    return e.code;
+  } catch (const std::exception& e) {
+    {
+      std::lock_guard<std::mutex> lk(g_err_mu);
+      g_last_create_error = e.what();
+    }
+    s->destroy();
+    delete s;
+    return FFS_ERR_ARG;
+  }
+  *out = s;
+  return FFS_OK;
+}
+
+int ffs_set_field(ffs_sim* s, int field, int space, const double* host, size_t n) {
+  if (!s || !host) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    if (field < FFS_FIELD_U1 || field > FFS_FIELD_U3) throw FfsError(FFS_ERR_ARG, "unknown field");
+    if (s->initialized) throw FfsError(FFS_ERR_STATE, "fields are set before ffs_initialize");
+    const size_t nxh = s->g.nxh, ny = s->g.ny, nz = s->g.nz, nrx = s->g.nrx;
+    const size_t padded = 2 * nxh * ny * nz, unpadded = nrx * ny * nz;
+    void* dst = s->cptr(s->u, (size_t)field * s->S);
+    if (space == FFS_SPACE_REAL) {
+      std::vector<double> buf;
+      const double* src = host;
+      if (n == unpadded && n != padded) {  // FluidFields reader: accept the unpadded layout too
+        buf.assign(padded, 0.0);
+        for (size_t r = 0; r < ny * nz; ++r) std::memcpy(&buf[r * 2 * nxh], host + r * nrx, nrx * sizeof(double));
+        src = buf.data();
+      } else if (n != padded) {
+        throw FfsError(FFS_ERR_IO, "field size " + std::to_string(n) + " matches neither the padded (" +
+                                       std::to_string(padded) + ") nor the unpadded (" + std::to_string(unpadded) +
+                                       ") layout");
+      }
+      double* d64 = (double*)s->dalloc(padded * sizeof(double));
+      HIPCHECK(hipMemcpy(d64, src, padded * sizeof(double), hipMemcpyHostToDevice));
+      void* real = d64;
+      if (s->fp32) {
+        real = s->dalloc(padded * sizeof(float));
+        launch_cast_copy<float>(d64, real, padded, s->stream);
+      }
+      s->forward_from_real(real, 1, dst);
+      HIPCHECK(hipStreamSynchronize(s->stream));
+      // release the staging buffers
+      for (void* b : {real, (void*)d64}) {
+        for (size_t i = 0; i < s->allocs.size(); ++i)
+          if (s->allocs[i] == b) {
+            hipFree(b);
+            s->allocs.erase(s->allocs.begin() + i);
+            break;
+          }
+      }
+      s->dev_bytes -= padded * sizeof(double) + (s->fp32 ? padded * sizeof(float) : 0);
+    } else if (space == FFS_SPACE_FOURIER) {
+      if (n != padded) throw FfsError(FFS_ERR_IO, "half-spectrum size mismatch");
+      // gather the retained modes
+      std::vector<double> c(2 * s->S);
+      for (int cz = 0; cz < s->g.mz; ++cz)
+        for (int cy = 0; cy < s->g.my; ++cy)
+          for (int cx = 0; cx < s->g.mx; ++cx) {
+            const size_t src = cx + nxh * (s->ymap[cy] + ny * (size_t)s->zmap[cz]);
+            const size_t d = cx + (size_t)s->g.mx * (cy + (size_t)s->g.my * cz);
+            c[2 * d] = host[2 * src];
+            c[2 * d + 1] = host[2 * src + 1];
+          }
+      if (s->fp32) {
+        std::vector<float> f(c.begin(), c.end());
+        HIPCHECK(hipMemcpy(dst, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+      } else {
+        HIPCHECK(hipMemcpy(dst, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice));
+      }
+    } else {
+      throw FfsError(FFS_ERR_ARG, "unknown space");
+    }
+  });
+}
+
+int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n) {
+  if (!s || !host) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    if (field < FFS_FIELD_U1 || field > FFS_FIELD_U3) throw FfsError(FFS_ERR_ARG, "unknown field");
+    const size_t nxh = s->g.nxh, ny = s->g.ny, nz = s->g.nz;
+    const size_t padded = 2 * nxh * ny * nz;
+    if (n != padded) throw FfsError(FFS_ERR_IO, "output buffer must hold 2*(nx/2+1)*ny*nz doubles");
+    void* src = s->cptr(s->u, (size_t)field * s->S);
+    if (space == FFS_SPACE_FOURIER) {
+      std::vector<double> c(2 * s->S);
+      if (s->fp32) {
+        std::vector<float> f(2 * s->S);
+        HIPCHECK(hipMemcpyAsync(f.data(), src, f.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+        for (size_t i = 0; i < f.size(); ++i) c[i] = f[i];
+      } else {
+        HIPCHECK(hipMemcpyAsync(c.data(), src, c.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+      }
+      std::memset(host, 0, n * sizeof(double));
+      for (int cz = 0; cz < s->g.mz; ++cz)
+        for (int cy = 0; cy < s->g.my; ++cy)
+          for (int cx = 0; cx < s->g.mx; ++cx) {
+            const size_t dsti = cx + nxh * (s->ymap[cy] + ny * (size_t)s->zmap[cz]);
+            const size_t d = cx + (size_t)s->g.mx * (cy + (size_t)s->g.my * cz);
+            host[2 * dsti] = c[2 * d];
+            host[2 * dsti + 1] = c[2 * d + 1];
+          }
+    } else if (space == FFS_SPACE_REAL) {
+      const size_t tb = s->fp32 ? sizeof(float) : sizeof(double);
+      void* real = s->dalloc(padded * tb);
+      s->inverse_to_real(src, 1, real);
+      if (s->fp32) {
+        std::vector<float> f(padded);
+        HIPCHECK(hipMemcpyAsync(f.data(), real, padded * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+        for (size_t i = 0; i < padded; ++i) host[i] = f[i];
+      } else {
+        HIPCHECK(hipMemcpyAsync(host, real, padded * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+      }
+      for (size_t i = 0; i < s->allocs.size(); ++i)
+        if (s->allocs[i] == real) {
+          hipFree(real);
+          s->allocs.erase(s->allocs.begin() + i);
+          break;
+        }
+      s->dev_bytes -= padded * tb;
+    } else {
+      throw FfsError(FFS_ERR_ARG, "unknown space");
+    }
+  });
+}
+
+int ffs_initialize(ffs_sim* s) {
+  if (!s) return FFS_ERR_ARG;
+  return guard(s, [&] { s->initialize(); });
+}
+
+int ffs_advance(ffs_sim* s, int64_t nsteps, int64_t* done, int* event) {
+  if (!s) return FFS_ERR_ARG;
+  if (done) *done = 0;
+  if (event) *event = FFS_EVENT_NONE;
+  return guard(s, [&] {
+    if (!s->initialized) throw FfsError(FFS_ERR_STATE, "ffs_initialize must be called first");
+    for (int64_t i = 0; i < nsteps; ++i) {
+      if (s->is_output_time(s->iteration) && s->output_served != s->iteration) {
+        s->output_served = s->iteration;
+        if (event) *event = FFS_EVENT_OUTPUT;
+        return;
+      }
+      s->advance_one();
+      if (done) *done += 1;
+    }
+  });
+}
+
+int ffs_writestats_now(ffs_sim* s) {
+  if (!s) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    if (!s->initialized) throw FfsError(FFS_ERR_STATE, "ffs_initialize must be called first");
+    s->writestats_now();
+  });
+}
+
+int ffs_stats_pop(ffs_sim* s, double* stats, int cap_stats, int* n_stats, double* scales, int cap_scales,
+                  int* n_scales, double* numbers, int cap_numbers, int* n_numbers) {
+  if (!s) return FFS_ERR_ARG;
+  int got = 0;
+  int rc = guard(s, [&] {
+    if (s->ready.empty()) s->finish_oldest();
+    if (s->ready.empty()) return;
+    Record r = std::move(s->ready.front());
+    s->ready.pop_front();
+    auto put = [](const std::vector<double>& v, double* dst, int cap, int* n) {
+      if (n) *n = (int)v.size();
+      if (dst)
+        for (int i = 0; i < (int)v.size() && i < cap; ++i) dst[i] = v[i];
+    };
+    put(r.stats, stats, cap_stats, n_stats);
+    put(r.scales, scales, cap_scales, n_scales);
+    put(r.numbers, numbers, cap_numbers, n_numbers);
+    got = 1;
+  });
+  return rc < 0 ? rc : got;
+}
+
+int ffs_stats_header(const ffs_sim* s, int table, char* buf, size_t cap) {
+  if (!s || !buf || cap == 0) return FFS_ERR_ARG;
+  std::string h;
+  if (table == FFS_TABLE_STATS)
+    h = "iteration,time,u1,u2,u3,u1p2,u2p2,u3p2,kh,kv,kaniso,k,dissh,dissv,diss,nlh,nlv,pressh,pressv,tnlh,tnlv,tdiss";
+  else if (table == FFS_TABLE_SCALES)
+    h = "iteration,time,Lk,Lt,lamg,laml,lamt,Lhh,Lvv,Lhv,Lvh,lam12,lam21,L11,L12,L13,L21,L22,L23,L31,L32,L33,Tk,Tt,Thh";
+  else if (table == FFS_TABLE_NUMBERS)
+    h = "iteration,time,Reh,Reh1,Relamg,Relaml,Relamt,Relamg1,Relaml1,Relamt1";
+  else
+    return FFS_ERR_ARG;
+  std::snprintf(buf, cap, "%s", h.c_str());
+  return h.size() < cap ? FFS_OK : FFS_ERR_ARG;
+}
+
+int ffs_spectrum(ffs_sim* s, double* h, int cap_h, int* n_h, double* v, int cap_v, int* n_v) {
+  if (!s) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    const Geo& g = s->g;
+    double* ox = s->red + 64;
+    double* oy0 = ox + 2 * g.mx;
+    double* oy = oy0 + 2 * g.my;
+    s->launch("spectrum", [&] {
+      if (s->fp32) launch_spectrum<float>(g, s->u, ox, oy0, oy, nullptr, nullptr, s->partials, s->stream);
+      else launch_spectrum<double>(g, s->u, ox, oy0, oy, nullptr, nullptr, s->partials, s->stream);
+    });
+    std::vector<double> X(2 * g.mx), Y0(2 * g.my), Y(2 * g.my), PZ(4 * (size_t)g.mz);
+    HIPCHECK(hipMemcpyAsync(X.data(), ox, X.size() * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipMemcpyAsync(Y0.data(), oy0, Y0.size() * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipMemcpyAsync(Y.data(), oy, Y.size() * 8, hipMemcpyDeviceToHost, s->stream));
+    const double* pz = s->partials + (size_t)g.mz * 2 * g.mx + (size_t)g.mz * 2 * g.my;
+    HIPCHECK(hipMemcpyAsync(PZ.data(), pz, PZ.size() * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    // KH / MAXDKH / KRZ / DKZ of Globals.jl:148-152
+    std::vector<double> KH = (s->KX.size() > 1 && s->KY.size() > 1 && s->KX[1] >= s->KY[1]) ? s->KX
+                                                                                           : rfftfreq(g.ny, s->p.y_domain);
+    const double dkh = KH.size() > 1 ? KH[1] : 1.0;
+    std::vector<double> KRZ = rfftfreq(g.nz, s->p.z_domain);
+    const double dkz = KRZ.size() > 1 ? KRZ[1] : 1.0;
+    const int nh = (int)KH.size(), nv = (int)KRZ.size();
+    // hv = 0: horizontal-energy quantity (|ux|^2+|uy|^2)/2, hv = 1: vertical |uz|^2/2
+    std::vector<double> H(2 * (size_t)nh, 0.0), V(2 * (size_t)nv, 0.0);
+    auto bin = [](double k, double dk) { return (int)std::nearbyint(std::fabs(k) / dk); };
+    for (int hv = 0; hv < 2; ++hv) {
+      double* Hh = H.data() + (size_t)hv * nh;
+      double* Vh = V.data() + (size_t)hv * nv;
+      const double* Xh = X.data() + (size_t)hv * g.mx;
+      const double* Y0h = Y0.data() + (size_t)hv * g.my;
+      const double* Yh = Y.data() + (size_t)hv * g.my;
+      // h[1] += 1/2 sum e(i=1); h[nx] += dkh X[cx] (cx>0); h[nj] += 1/2 Y0 + dkh Y; v[nk] += Z0 + 2 dkz Z
+      Hh[0] += 0.5 * Xh[0];
+      for (int cx = 1; cx < g.mx; ++cx) {
+        const int b = bin(s->kxc[cx], dkh);
+        if (b < nh) Hh[b] += dkh * Xh[cx];
+      }
+      for (int cy = 0; cy < g.my; ++cy) {
+        const int b = bin(s->kyc[cy], dkh);
+        if (b < nh) Hh[b] += 0.5 * Y0h[cy] + dkh * Yh[cy];
+      }
+      for (int cz = 0; cz < g.mz; ++cz) {
+        const int b = bin(s->kzc[cz], dkz);
+        if (b < nv) Vh[b] += PZ[4 * cz + hv] + 2 * dkz * PZ[4 * cz + 2 + hv];
+      }
+    }
+    if (n_h) *n_h = nh;
+    if (n_v) *n_v = nv;
+    if (h)
+      for (int i = 0; i < 2 * nh && i < cap_h; ++i) h[i] = H[i];
+    if (v)
+      for (int i = 0; i < 2 * nv && i < cap_v; ++i) v[i] = V[i];
+  });
+}
+
+int ffs_get_scalar(const ffs_sim* cs, int what, double* out) {
+  ffs_sim* s = const_cast<ffs_sim*>(cs);
+  if (!s || !out) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    if (what == FFS_SCALAR_ITERATION) {
+      *out = (double)s->iteration;
+      return;
+    }
+    DevState h;
+    HIPCHECK(hipMemcpyAsync(&h, s->st, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    if (what == FFS_SCALAR_TIME) *out = h.time;
+    else if (what == FFS_SCALAR_DT) *out = h.dt;
+    else if (what == FFS_SCALAR_UMAX) *out = h.umax_d;
+    else throw FfsError(FFS_ERR_ARG, "unknown scalar");
+  });
+}
+
+int ffs_profile_enable(ffs_sim* s, int on) {
+  if (!s) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    if (!on) s->drain_prof();
+    s->prof = on != 0;
+  });
+}
+
+int ffs_profile_reset(ffs_sim* s) {
+  if (!s) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    s->drain_prof();
+    s->prof_acc.clear();
+  });
+}
+
+int ffs_profile_get(ffs_sim* s, const char** names, double* total_ms, int64_t* launches, int cap, int* n) {
+  if (!s) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    s->drain_prof();
+    s->prof_names.clear();
+    for (auto& kv : s->prof_acc) s->prof_names.push_back(kv.first);
+    int i = 0;
+    for (auto& kv : s->prof_acc) {
+      if (i >= cap) break;
+      if (names) names[i] = s->prof_names[i].c_str();
+      if (total_ms) total_ms[i] = kv.second.first;
+      if (launches) launches[i] = kv.second.second;
+      ++i;
+    }
+    if (n) *n = (int)s->prof_acc.size();
+  });
+}
+
+int ffs_device_bytes(const ffs_sim* s, size_t* bytes) {
+  if (!s || !bytes) return FFS_ERR_ARG;
+  *bytes = s->dev_bytes;
+  return FFS_OK;
+}
+
+int ffs_synchronize(ffs_sim* s) {
+  if (!s) return FFS_ERR_ARG;
+  return guard(s, [&] { HIPCHECK(hipStreamSynchronize(s->stream)); });
+}
+
+const char* ffs_last_error(const ffs_sim* s) {
+  if (!s) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_last_create_error.c_str();
+  }
+  return s->err.c_str();
+}
+
+void ffs_destroy(ffs_sim* s) {
+  if (!s) return;
+  s->destroy();
+  delete s;
+}
+
+}  // extern "C"

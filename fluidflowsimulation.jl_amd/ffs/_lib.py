@@ -1,0 +1,134 @@
+"""ctypes binding of the C ABI declared in include/ffs/ffs.h.
+
+The product path has no fallback: if libffs_hip.so is missing or fails to
+load, every call raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libffs_hip.so")
+
+FFS_OK = 0
+ERRORS = {
+    -1: "FFS_ERR_ARG",
+    -2: "FFS_ERR_MEMORY",
+    -3: "FFS_ERR_IO",
+    -4: "FFS_ERR_NONFINITE",
+    -5: "FFS_ERR_HIP",
+    -6: "FFS_ERR_STATE",
+}
+FFS_AB3, FFS_ETD3 = 0, 1
+FFS_FP64, FFS_FP32 = 0, 1
+FFS_LES_NONE, FFS_LES_SMAGORINSKY = 0, 1
+FFS_FORCING_NONE, FFS_FORCING_SHELL = 0, 1
+FFS_FIELD_U1, FFS_FIELD_U2, FFS_FIELD_U3, FFS_FIELD_RHO = 0, 1, 2, 3
+FFS_SPACE_REAL, FFS_SPACE_FOURIER = 0, 1
+FFS_SCALAR_ITERATION, FFS_SCALAR_TIME, FFS_SCALAR_DT, FFS_SCALAR_UMAX = 0, 1, 2, 3
+FFS_TABLE_STATS, FFS_TABLE_SCALES, FFS_TABLE_NUMBERS = 0, 1, 2
+FFS_EVENT_NONE, FFS_EVENT_OUTPUT = 0, 1
+
+
+class FfsParams(ctypes.Structure):
+    """Mirror of `ffs_params` (include/ffs/ffs.h)."""
+
+    _fields_ = [
+        ("size", ctypes.c_uint32),
+        ("nx", ctypes.c_int32),
+        ("ny", ctypes.c_int32),
+        ("nz", ctypes.c_int32),
+        ("x_domain", ctypes.c_double),
+        ("y_domain", ctypes.c_double),
+        ("z_domain", ctypes.c_double),
+        ("nu", ctypes.c_double),
+        ("cutoff", ctypes.c_double),
+        ("integrator", ctypes.c_int32),
+        ("variable_dt", ctypes.c_int32),
+        ("cfl", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("start", ctypes.c_int64),
+        ("start_time", ctypes.c_double),
+        ("dt_stat", ctypes.c_int32),
+        ("dt_spec", ctypes.c_int32),
+        ("write_time", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("les_model", ctypes.c_int32),
+        ("smagorinsky_c", ctypes.c_double),
+        ("filter_width", ctypes.c_double),
+        ("density", ctypes.c_int32),
+        ("pr", ctypes.c_double),
+        ("density_gradient", ctypes.c_double),
+        ("z_acceleration", ctypes.c_double),
+        ("reference_density", ctypes.c_double),
+        ("forcing", ctypes.c_int32),
+        ("forcing_kf", ctypes.c_double),
+        ("forcing_power", ctypes.c_double),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/ffs/ffs.h
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+SIGNATURES = {
+    "ffs_abi_version": (ctypes.c_int, []),
+    "ffs_params_default": (ctypes.c_int, [ctypes.POINTER(FfsParams)]),
+    "ffs_create": (ctypes.c_int, [ctypes.POINTER(FfsParams), ctypes.POINTER(_P)]),
+    "ffs_set_field": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D, ctypes.c_size_t]),
+    "ffs_get_field": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D, ctypes.c_size_t]),
+    "ffs_initialize": (ctypes.c_int, [_P]),
+    "ffs_advance": (ctypes.c_int, [_P, ctypes.c_int64, _I64, _I]),
+    "ffs_writestats_now": (ctypes.c_int, [_P]),
+    "ffs_stats_pop": (ctypes.c_int, [_P, _D, ctypes.c_int, _I, _D, ctypes.c_int, _I, _D, ctypes.c_int, _I]),
+    "ffs_stats_header": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
+    "ffs_spectrum": (ctypes.c_int, [_P, _D, ctypes.c_int, _I, _D, ctypes.c_int, _I]),
+    "ffs_get_scalar": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+    "ffs_profile_enable": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ffs_profile_get": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_char_p), _D, _I64, ctypes.c_int, _I]),
+    "ffs_profile_reset": (ctypes.c_int, [_P]),
+    "ffs_device_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    "ffs_synchronize": (ctypes.c_int, [_P]),
+    "ffs_last_error": (ctypes.c_char_p, [_P]),
+    "ffs_destroy": (None, [_P]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libffs_hip.so once; raises if it is absent (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"libffs_hip.so not found at {LIB_PATH}; build it with __graft_entry__.build() "
+            "(there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class FfsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def check(rc, handle=None):
+    if rc is not None and rc < 0:
+        lib = load()
+        msg = lib.ffs_last_error(handle)
+        raise FfsError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def default_params():
+    p = FfsParams()
+    check(load().ffs_params_default(ctypes.byref(p)))
+    return p

@@ -1,0 +1,238 @@
+"""The `global` parameter file: reader and conversion to run parameters.
+
+Mirrors ReadGlobal.readglobal() (un-vendored; format as in
+test/Scalar_Advection/global:1-26) and parameters(d) of
+simulationtypes.jl:242-365 together with the module-load constants of
+Globals.jl:106-182.  Required keys raise KeyError exactly where the reference
+does (`d[:nx]` ...); optional keys take the reference defaults.
+"""
+import math
+import re
+from dataclasses import dataclass, field, fields
+
+__all__ = ["readglobal", "parse_expr", "Params", "parameters"]
+
+
+def readglobal(path="global"):
+    """Reads `key value` lines; `//` starts a comment; returns {key: value-string}."""
+    d = {}
+    with open(path) as f:
+        for line in f:
+            line = line.split("//", 1)[0].strip()
+            if not line:
+                continue
+            parts = line.split(None, 1)
+            d[parts[0]] = parts[1].strip() if len(parts) > 1 else ""
+    return d
+
+
+_TOK = re.compile(r"\s*(?:(\d+\.\d*(?:[eE][-+]?\d+)?|\.\d+(?:[eE][-+]?\d+)?|\d+(?:[eE][-+]?\d+)?)|(π|pi|Inf|inf)|(.))")
+
+
+def parse_expr(s):
+    """Evaluates the numeric expressions the reference accepts through
+    `Float64(eval(Meta.parse(v)))` (e.g. `2π`, `1/3`, `2^-4`, `0.03333333`):
+    numbers, + - * / ^, parentheses, π/pi, Inf and implicit multiplication of a
+    number by π or a parenthesis."""
+    toks = []
+    for m in _TOK.finditer(s.strip()):
+        num, const, op = m.groups()
+        if num is not None:
+            toks.append(("n", float(num)))
+        elif const is not None:
+            toks.append(("n", math.pi if const in ("π", "pi") else math.inf))
+        elif op is not None and not op.isspace():
+            toks.append(("o", op))
+    pos = 0
+
+    def peek():
+        return toks[pos] if pos < len(toks) else None
+
+    def take():
+        nonlocal pos
+        t = toks[pos]
+        pos += 1
+        return t
+
+    def expr():
+        v = term()
+        while peek() and peek() in (("o", "+"), ("o", "-")):
+            op = take()[1]
+            r = term()
+            v = v + r if op == "+" else v - r
+        return v
+
+    def term():
+        v = unary()
+        while True:
+            t = peek()
+            if t in (("o", "*"), ("o", "/")):
+                op = take()[1]
+                r = unary()
+                v = v * r if op == "*" else v / r
+            elif t is not None and (t[0] == "n" or t == ("o", "(")):
+                v = v * unary()  # juxtaposition: 2π, 2(1+x)
+            else:
+                return v
+
+    def unary():
+        t = peek()
+        if t == ("o", "-"):
+            take()
+            return -unary()
+        if t == ("o", "+"):
+            take()
+            return unary()
+        return power()
+
+    def power():
+        b = atom()
+        if peek() == ("o", "^"):
+            take()
+            return b ** unary()
+        return b
+
+    def atom():
+        t = take() if peek() else None
+        if t is None:
+            raise ValueError(f"incomplete expression: {s!r}")
+        if t[0] == "n":
+            return t[1]
+        if t == ("o", "("):
+            v = expr()
+            if take() != ("o", ")"):
+                raise ValueError(f"missing ')' in {s!r}")
+            return v
+        raise ValueError(f"unexpected {t[1]!r} in {s!r}")
+
+    v = expr()
+    if pos != len(toks):
+        raise ValueError(f"trailing input in {s!r}")
+    return float(v)
+
+
+def _parse_bool(s):
+    s = s.strip().lower()
+    if s in ("true", "1"):
+        return True
+    if s in ("false", "0"):
+        return False
+    raise ValueError(f"not a Bool: {s!r}")
+
+
+@dataclass
+class Params:
+    """Run parameters (the `ffs_params` struct of include/ffs/ffs.h)."""
+
+    nx: int
+    ny: int
+    nz: int
+    x_domain: float = 1.0
+    y_domain: float = 1.0
+    z_domain: float = 1.0
+    nu: float = 0.0
+    cutoff: float = 2.0 / 3.0
+    integrator: str = "ETD3rdO"
+    variable_dt: bool = True
+    cfl: float = 0.18
+    dt: float = 0.0
+    start: int = 0
+    start_time: float = 0.0
+    dt_stat: int = 1 << 30
+    dt_spec: int = 1 << 30
+    write_time: int = 1 << 30
+    nt: int = None
+    time_duration: float = math.inf
+    threaded: bool = True
+    precision: str = "fp64"
+    les_model: str = None
+    smagorinsky_c: float = 0.17
+    filter_width: float = 0.0
+    density: bool = False
+    pr: float = 1.0
+    density_gradient: float = 0.0
+    z_acceleration: float = 0.0
+    reference_density: float = 1.0
+    forcing: str = None
+    forcing_kf: float = 2.5
+    forcing_power: float = 0.1
+    extra: dict = field(default_factory=dict)
+
+    def to_c(self):
+        from . import _lib
+
+        p = _lib.default_params()
+        p.nx, p.ny, p.nz = self.nx, self.ny, self.nz
+        p.x_domain, p.y_domain, p.z_domain = self.x_domain, self.y_domain, self.z_domain
+        p.nu = self.nu
+        p.cutoff = self.cutoff
+        p.integrator = _lib.FFS_AB3 if self.integrator == "Adams_Bashforth3rdO" else _lib.FFS_ETD3
+        p.variable_dt = int(bool(self.variable_dt))
+        p.cfl = self.cfl
+        p.dt = self.dt
+        p.start = self.start
+        p.start_time = self.start_time
+        p.dt_stat, p.dt_spec, p.write_time = self.dt_stat, self.dt_spec, self.write_time
+        p.precision = _lib.FFS_FP32 if self.precision == "fp32" else _lib.FFS_FP64
+        p.les_model = {None: _lib.FFS_LES_NONE, "Smagorinsky": _lib.FFS_LES_SMAGORINSKY}.get(self.les_model, -1)
+        p.smagorinsky_c = self.smagorinsky_c
+        p.filter_width = self.filter_width
+        p.density = int(bool(self.density))
+        p.pr, p.density_gradient = self.pr, self.density_gradient
+        p.z_acceleration, p.reference_density = self.z_acceleration, self.reference_density
+        p.forcing = {None: _lib.FFS_FORCING_NONE, "shellForcing": _lib.FFS_FORCING_SHELL}.get(self.forcing, -1)
+        p.forcing_kf, p.forcing_power = self.forcing_kf, self.forcing_power
+        return p
+
+
+def parameters(d):
+    """parameters(d) of simulationtypes.jl:242-365 (plus nt/timeDuration of
+    FluidFlowSimulation.jl:26-27).  Raises KeyError for a missing required key."""
+    known = set()
+
+    def req(k):
+        known.add(k)
+        return d[k]
+
+    def opt(k, conv, default):
+        known.add(k)
+        return conv(d[k]) if k in d else default
+
+    p = Params(nx=int(req("nx")), ny=int(req("ny")), nz=int(req("nz")))
+    p.x_domain = parse_expr(req("xDomainSize"))
+    p.y_domain = parse_expr(req("yDomainSize"))
+    p.z_domain = parse_expr(req("zDomainSize"))
+    p.nu = parse_expr(req("kinematicViscosity"))
+    p.start = opt("start", int, 0)
+    p.start_time = opt("startTime", float, 0.0)
+    p.dt_stat = int(req("dtStat"))
+    p.dt_spec = int(req("dtSpec"))
+    p.write_time = int(req("writeTime"))
+    p.threaded = opt("threaded", _parse_bool, True)
+    p.cutoff = opt("cutoff", parse_expr, 2.0 / 3.0)
+    p.integrator = opt("velocityTimeStep", str, "ETD3rdO")
+    p.variable_dt = opt("variableTimestepFlag", _parse_bool, True)
+    p.cfl = opt("cfl", float, 0.18)
+    p.dt = opt("dt", parse_expr, 0.0)
+    p.nt = opt("nt", int, None)
+    p.time_duration = opt("timeDuration", float, math.inf)
+    p.precision = opt("precision", str, "fp64")
+    if "lesModel" in d:
+        p.les_model = d["lesModel"]
+        p.smagorinsky_c = opt("smagorinskyConstant", parse_expr, 0.17)
+        p.filter_width = opt("filterWidth", parse_expr, 0.0)
+    known.add("lesModel")
+    if "densityStratification" in d:
+        known.add("densityStratification")
+        p.density = True
+        p.pr = parse_expr(req("Pr"))
+        p.density_gradient = parse_expr(req("densityGradient"))
+        p.z_acceleration = parse_expr(req("zAcceleration"))
+        p.reference_density = parse_expr(req("referenceDensity"))
+    if "forcing" in d:
+        known.add("forcing")
+        p.forcing = d["forcing"]
+        p.forcing_kf = opt("forcingKf", parse_expr, 2.5)
+        p.forcing_power = opt("forcingPower", parse_expr, 0.1)
+    p.extra = {k: v for k, v in d.items() if k not in known}
+    return p

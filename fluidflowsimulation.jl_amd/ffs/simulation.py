@@ -1,0 +1,258 @@
+"""Host-side mirror of the reference's driver API over the C ABI.
+
+Names follow FluidFlowSimulation.jl: `parameters`, `initialize` (initialize!),
+`advance_in_time` (advance_in_time!), `writestats`, `writeoutput`,
+`run_simulation` (FluidFlowSimulation.jl:23-91).  All computation happens in
+libffs_hip.so; this module only moves files and rows.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+from . import _lib
+from .config import Params, parameters, readglobal
+from .fields import padded_shape, read_field, write_field
+
+__all__ = ["Simulation", "run_simulation"]
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Simulation:
+    """One run on the GPU (an `ffs_sim` handle)."""
+
+    def __init__(self, params: Params):
+        self.params = params
+        self._lib = _lib.load()
+        cp = params.to_c()
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.ffs_create(ctypes.byref(cp), ctypes.byref(h)))
+        self._h = h
+        self.nx, self.ny, self.nz = params.nx, params.ny, params.nz
+        self.stats_rows = []
+
+    # ---------------------------------------------------------------- lifetime
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ffs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        return _lib.check(rc, self._h)
+
+    # ---------------------------------------------------------------- fields
+    @property
+    def padded_size(self):
+        return 2 * (self.nx // 2 + 1) * self.ny * self.nz
+
+    def set_field(self, field, data, space=_lib.FFS_SPACE_REAL):
+        """data: real (nx,ny,nz) / padded real, or complex (nx//2+1, ny, nz) half spectrum."""
+        a = np.asarray(data)
+        if np.iscomplexobj(a):
+            space = _lib.FFS_SPACE_FOURIER
+            buf = np.empty(a.size * 2, dtype=np.float64)
+            flat = np.asfortranarray(a).ravel(order="F")
+            buf[0::2] = flat.real
+            buf[1::2] = flat.imag
+        else:
+            buf = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+        self._check(self._lib.ffs_set_field(self._h, field, space, _dptr(buf), buf.size))
+
+    def set_velocity(self, u1, u2, u3):
+        for f, a in enumerate((u1, u2, u3)):
+            self.set_field(f, a)
+
+    def get_field(self, field, space=_lib.FFS_SPACE_REAL):
+        """Real space: padded Fortran array (2*(nx//2+1), ny, nz).
+        Fourier space: complex (nx//2+1, ny, nz) (the retained modes; zeros elsewhere)."""
+        buf = np.empty(self.padded_size, dtype=np.float64)
+        self._check(self._lib.ffs_get_field(self._h, field, space, _dptr(buf), buf.size))
+        if space == _lib.FFS_SPACE_REAL:
+            return buf.reshape(padded_shape(self.nx, self.ny, self.nz), order="F")
+        c = buf[0::2] + 1j * buf[1::2]
+        return c.reshape((self.nx // 2 + 1, self.ny, self.nz), order="F")
+
+    def velocity_hat(self):
+        return [self.get_field(f, _lib.FFS_SPACE_FOURIER) for f in range(3)]
+
+    def velocity(self):
+        return [self.get_field(f, _lib.FFS_SPACE_REAL)[: self.nx] for f in range(3)]
+
+    # ---------------------------------------------------------------- the step
+    def initialize(self):
+        """initialize!(s) (FluidFlowSimulation.jl:67-91)."""
+        self._check(self._lib.ffs_initialize(self._h))
+
+    def advance(self, nsteps=1):
+        """advance_in_time! nsteps times; returns (steps done, event)."""
+        done = ctypes.c_int64()
+        ev = ctypes.c_int()
+        self._check(self._lib.ffs_advance(self._h, nsteps, ctypes.byref(done), ctypes.byref(ev)))
+        return done.value, ev.value
+
+    def advance_in_time(self, nsteps=1, on_output=None):
+        """Runs nsteps steps, calling on_output(self) at output iterations."""
+        left = nsteps
+        while left > 0:
+            done, ev = self.advance(left)
+            left -= done
+            if ev == _lib.FFS_EVENT_OUTPUT and on_output is not None:
+                on_output(self)
+        return self
+
+    def writestats_now(self):
+        self._check(self._lib.ffs_writestats_now(self._h))
+
+    def synchronize(self):
+        self._check(self._lib.ffs_synchronize(self._h))
+
+    # ---------------------------------------------------------------- diagnostics
+    def pop_stats(self):
+        """Returns the queued stats records as a list of (stats, scales, numbers) rows."""
+        out = []
+        a = np.zeros(64)
+        b = np.zeros(64)
+        c = np.zeros(64)
+        na, nb, nc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        while True:
+            r = self._check(self._lib.ffs_stats_pop(self._h, _dptr(a), 64, ctypes.byref(na), _dptr(b), 64,
+                                                    ctypes.byref(nb), _dptr(c), 64, ctypes.byref(nc)))
+            if r == 0:
+                return out
+            out.append((a[: na.value].copy(), b[: nb.value].copy(), c[: nc.value].copy()))
+
+    def header(self, table):
+        buf = ctypes.create_string_buffer(4096)
+        self._check(self._lib.ffs_stats_header(self._h, table, buf, 4096))
+        return buf.value.decode()
+
+    def spectrum(self):
+        """spectrum_u + calculate_specHV (spec.jl:316-327,424-450):
+        returns ((u_h.specH, u_h.specV), (u_v.specH, u_v.specV))."""
+        h = np.zeros(8192)
+        v = np.zeros(8192)
+        nh, nv = ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.ffs_spectrum(self._h, _dptr(h), h.size, ctypes.byref(nh), _dptr(v), v.size,
+                                           ctypes.byref(nv)))
+        a, b = nh.value, nv.value
+        return (h[:a].copy(), v[:b].copy()), (h[a:2 * a].copy(), v[b:2 * b].copy())
+
+    def scalar(self, what):
+        out = ctypes.c_double()
+        self._check(self._lib.ffs_get_scalar(self._h, what, ctypes.byref(out)))
+        return out.value
+
+    @property
+    def iteration(self):
+        return int(self.scalar(_lib.FFS_SCALAR_ITERATION))
+
+    @property
+    def time(self):
+        return self.scalar(_lib.FFS_SCALAR_TIME)
+
+    @property
+    def dt(self):
+        return self.scalar(_lib.FFS_SCALAR_DT)
+
+    def device_bytes(self):
+        b = ctypes.c_size_t()
+        self._check(self._lib.ffs_device_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    # ---------------------------------------------------------------- profiling
+    def profile(self, on=True):
+        self._check(self._lib.ffs_profile_enable(self._h, int(on)))
+
+    def profile_reset(self):
+        self._check(self._lib.ffs_profile_reset(self._h))
+
+    def profile_get(self):
+        cap = 64
+        names = (ctypes.c_char_p * cap)()
+        ms = np.zeros(cap)
+        cnt = np.zeros(cap, dtype=np.int64)
+        n = ctypes.c_int()
+        self._check(self._lib.ffs_profile_get(self._h, names, _dptr(ms),
+                                              cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), cap,
+                                              ctypes.byref(n)))
+        return {names[i].decode(): (ms[i], int(cnt[i])) for i in range(min(n.value, cap))}
+
+    # ---------------------------------------------------------------- files
+    def writeheader(self, directory="."):
+        """writeheader (stats.jl:9-37)."""
+        for name, t in (("Stats.txt", _lib.FFS_TABLE_STATS), ("Scales.txt", _lib.FFS_TABLE_SCALES),
+                        ("Numbers.txt", _lib.FFS_TABLE_NUMBERS)):
+            with open(os.path.join(directory, name), "w") as f:
+                f.write(self.header(t) + "\n")
+
+    def writestats(self, directory="."):
+        """Appends every queued record (writestats, stats.jl:39-62)."""
+        recs = self.pop_stats()
+        for st, sc, nu in recs:
+            for name, row in (("Stats.txt", st), ("Scales.txt", sc), ("Numbers.txt", nu)):
+                with open(os.path.join(directory, name), "a+") as f:
+                    f.write(",".join(_fmt(v, i) for i, v in enumerate(row)) + "\n")
+        self.stats_rows.extend(recs)
+        return recs
+
+    def writeoutput(self, directory="."):
+        """writeoutput (output.jl:1-8): u1.$it u2.$it u3.$it, padded real."""
+        it = self.iteration
+        for f in range(3):
+            write_field(os.path.join(directory, f"u{f + 1}.{it}"), self.get_field(f, _lib.FFS_SPACE_REAL))
+
+
+def _fmt(v, i):
+    if i == 0:
+        return str(int(v))
+    return repr(float(v))
+
+
+def run_simulation(directory=".", params=None, nt=None):
+    """run_simulation() of FluidFlowSimulation.jl:23-65: reads `global` and
+    u1.$start..u3.$start from `directory`, runs, and writes Stats/Scales/Numbers
+    rows and field dumps there."""
+    if params is None:
+        params = parameters(readglobal(os.path.join(directory, "global")))
+    if nt is None:
+        nt = params.nt if params.nt is not None else (1 << 62) - params.start
+    s = Simulation(params)
+    nx, ny, nz = params.nx, params.ny, params.nz
+    for f in range(3):
+        s.set_field(f, read_field(os.path.join(directory, f"u{f + 1}.{params.start}"), nx, ny, nz))
+    if params.start == 0:
+        s.writeheader(directory)
+    s.initialize()
+    assert nt + params.start >= params.write_time
+    assert nt + params.start >= params.dt_stat
+    final_step = params.start + nt
+    final_time = params.start_time + params.time_duration
+    while s.iteration < final_step and (math.isinf(final_time) or s.time < final_time):
+        left = final_step - s.iteration
+        if not math.isinf(final_time):
+            left = 1
+        done, ev = s.advance(left)
+        if ev == _lib.FFS_EVENT_OUTPUT:
+            s.writeoutput(directory)
+        s.writestats(directory)
+    s.writestats_now()
+    s.writestats(directory)
+    if s.iteration % params.write_time != 0:
+        s.writeoutput(directory)
+    return s

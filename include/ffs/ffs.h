@@ -1,0 +1,176 @@
+/*
+ * ffs.h -- C ABI of libffs_hip.so, the MI355X (gfx950) time-step engine that
+ * stands in for the hot path of favba/FluidFlowSimulation.jl.
+ *
+ * Every entry point takes plain pointers and sizes; no C++ or torch types
+ * cross this boundary.  All functions return 0 on success or a negative
+ * FFS_ERR_* code; the message of the last failure of a handle is available
+ * from ffs_last_error(handle) (or ffs_last_error(NULL) for ffs_create).
+ * Nothing aborts or throws across the ABI.
+ *
+ * Threading: a handle is not re-entrant and owns one HIP stream; distinct
+ * handles may be driven from distinct host threads.  The library never
+ * retains a host pointer beyond the call that received it.
+ *
+ * Each declaration names the reference function it replaces (paths relative
+ * to the reference's src/ directory).
+ */
+#ifndef FFS_FFS_H
+#define FFS_FFS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFS_ABI_VERSION 1
+
+/* error codes */
+#define FFS_OK 0
+#define FFS_ERR_ARG -1        /* bad argument / unsupported configuration */
+#define FFS_ERR_MEMORY -2     /* device allocation failed or budget exceeded */
+#define FFS_ERR_IO -3         /* file or size mismatch */
+#define FFS_ERR_NONFINITE -4  /* the state contains NaN/Inf */
+#define FFS_ERR_HIP -5        /* HIP runtime error */
+#define FFS_ERR_STATE -6      /* call out of order (e.g. advance before initialize) */
+
+/* integrators: `velocityTimeStep` key (simulationtypes.jl:308) */
+#define FFS_AB3 0  /* Adams_Bashforth3rdO */
+#define FFS_ETD3 1 /* ETD3rdO (reference default) */
+
+/* precision of the bulk arithmetic */
+#define FFS_FP64 0
+#define FFS_FP32 1 /* FP32 fields/FFTs; FP64 coefficients and reductions */
+
+/* LES models (`lesModel` key, LESmodels/les_types.jl:360) */
+#define FFS_LES_NONE 0
+#define FFS_LES_SMAGORINSKY 1
+
+/* forcing (`forcing` key); FFS_FORCING_SHELL is the constant-power isotropic
+ * low-wavenumber forcing f = (P / sum_{0<|k|<=kf} |u_k|^2) u_k, which has no
+ * counterpart in the reference's kz=0-plane schemes */
+#define FFS_FORCING_NONE 0
+#define FFS_FORCING_SHELL 1
+
+/* fields */
+#define FFS_FIELD_U1 0
+#define FFS_FIELD_U2 1
+#define FFS_FIELD_U3 2
+#define FFS_FIELD_RHO 3
+
+/* spaces for get/set_field */
+#define FFS_SPACE_REAL 0    /* padded real (2*(nx/2+1), ny, nz), column major; or unpadded (nx,ny,nz) on input */
+#define FFS_SPACE_FOURIER 1 /* half spectrum (nx/2+1, ny, nz) complex, interleaved re/im */
+
+/* scalars for ffs_get_scalar */
+#define FFS_SCALAR_ITERATION 0
+#define FFS_SCALAR_TIME 1
+#define FFS_SCALAR_DT 2
+#define FFS_SCALAR_UMAX 3
+
+/* stats tables (stats.jl:1-62) */
+#define FFS_TABLE_STATS 0   /* Stats.txt   */
+#define FFS_TABLE_SCALES 1  /* Scales.txt  */
+#define FFS_TABLE_NUMBERS 2 /* Numbers.txt */
+
+/* events returned through ffs_advance's *event */
+#define FFS_EVENT_NONE 0
+#define FFS_EVENT_OUTPUT 1 /* an output step (writeoutput, models.jl:158) is due at the current iteration */
+
+/* Parameters: the `global` file keys of simulationtypes.jl:242-365 and
+ * Globals.jl:106-182 as plain fields.  `size` must be sizeof(ffs_params). */
+typedef struct ffs_params {
+  uint32_t size;
+  int32_t nx, ny, nz;                       /* nx ny nz */
+  double x_domain, y_domain, z_domain;      /* xDomainSize ...: the box is 2 pi L per side */
+  double nu;                                /* kinematicViscosity */
+  double cutoff;                            /* cutoff (2/3): box dealias mask, Globals.jl:132-142 */
+  int32_t integrator;                       /* FFS_AB3 / FFS_ETD3 */
+  int32_t variable_dt;                      /* variableTimestepFlag */
+  double cfl;                               /* cfl (0.18) */
+  double dt;                                /* dt (initial / fixed step) */
+  int64_t start;                            /* start: initial iteration number */
+  double start_time;                        /* startTime */
+  int32_t dt_stat, dt_spec, write_time;     /* dtStat, dtSpec, writeTime */
+  int32_t precision;                        /* FFS_FP64 / FFS_FP32 */
+  int32_t les_model;                        /* FFS_LES_* */
+  double smagorinsky_c;                     /* smagorinskyConstant (0.17) */
+  double filter_width;                      /* filterWidth; <= 0 selects (2 pi L/nx)/cutoff */
+  int32_t density;                          /* densityStratification present */
+  double pr, density_gradient, z_acceleration, reference_density; /* Pr, densityGradient, zAcceleration, referenceDensity */
+  int32_t forcing;                          /* FFS_FORCING_* */
+  double forcing_kf;                        /* forced band |k| <= kf */
+  double forcing_power;                     /* injected power P */
+} ffs_params;
+
+typedef struct ffs_sim ffs_sim;
+
+/* Fills p with the reference defaults (parameters(), simulationtypes.jl:242-365). */
+int ffs_params_default(ffs_params* p);
+
+/* Replaces parameters(d) (simulationtypes.jl:242-365) + the module-load grid
+ * constants (Globals.jl:106-182): allocates the device state of one run. */
+int ffs_create(const ffs_params* p, ffs_sim** out);
+
+/* Replaces VectorField{Float64}("u1.$start",...) (simulationtypes.jl:276): loads
+ * one component.  n_doubles selects the layout: 2*(nx/2+1)*ny*nz (padded real or
+ * half spectrum) or nx*ny*nz (unpadded real, space must be REAL). */
+int ffs_set_field(ffs_sim* s, int field, int space, const double* host, size_t n_doubles);
+
+/* Replaces real!(s.u) + write("u1.$it", s.u.rr.x) (output.jl:6-8) and
+ * complex views of the state: copies one component out in the padded layout. */
+int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n_doubles);
+
+/* Replaces initialize!(s) (FluidFlowSimulation.jl:67-91). */
+int ffs_initialize(ffs_sim* s);
+
+/* Replaces nsteps calls of advance_in_time!(s) (models.jl:3-17).  Stops early
+ * with *event = FFS_EVENT_OUTPUT before a step whose iteration is an output
+ * time; the next call performs that step.  *done receives the steps taken. */
+int ffs_advance(ffs_sim* s, int64_t nsteps, int64_t* done, int* event);
+
+/* Replaces the tail of run_simulation (FluidFlowSimulation.jl:46-49):
+ * calculate_rhs!(s), forcing, writestats(s) at the current iteration. */
+int ffs_writestats_now(ffs_sim* s);
+
+/* Pops the oldest queued statistics record (one per stats step, written by
+ * writestats, stats.jl:39-62).  Returns 1 and fills the three rows if a record
+ * was pending, 0 if none, <0 on error.  Rows start with iteration, time. */
+int ffs_stats_pop(ffs_sim* s, double* stats, int cap_stats, int* n_stats, double* scales, int cap_scales,
+                  int* n_scales, double* numbers, int cap_numbers, int* n_numbers);
+
+/* Header line of a stats table (statsheader / writeheader, stats.jl:1-37). */
+int ffs_stats_header(const ffs_sim* s, int table, char* buf, size_t cap);
+
+/* Replaces spectrum_u + calculate_specHV (spec.jl:316-327,424-450) for the
+ * current state.  *n_h = length(KH), *n_v = length(KRZ) = nz/2+1; h receives
+ * 2 * n_h values (specH of (|u1|^2+|u2|^2)/2, then specH of |u3|^2/2) and v
+ * receives 2 * n_v values (the matching specV arrays): the u_h.specH,
+ * u_v.specH, u_h.specV and u_v.specV files of writespectrum (spec.jl:173-194). */
+int ffs_spectrum(ffs_sim* s, double* h, int cap_h, int* n_h, double* v, int cap_v, int* n_v);
+
+/* Scalars: s.iteration[], s.time[], get_dt(s), last max|u_i|. */
+int ffs_get_scalar(const ffs_sim* s, int what, double* out);
+
+/* Per-kernel device time accumulation with HIP events on the handle's stream. */
+int ffs_profile_enable(ffs_sim* s, int on);
+/* Fills up to cap entries; names point into storage owned by the handle. */
+int ffs_profile_get(ffs_sim* s, const char** names, double* total_ms, int64_t* launches, int cap, int* n);
+int ffs_profile_reset(ffs_sim* s);
+
+/* Device bytes held by the handle. */
+int ffs_device_bytes(const ffs_sim* s, size_t* bytes);
+
+/* Blocks until all queued work of the handle is done. */
+int ffs_synchronize(ffs_sim* s);
+
+const char* ffs_last_error(const ffs_sim* s);
+void ffs_destroy(ffs_sim* s);
+int ffs_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FFS_FFS_H */

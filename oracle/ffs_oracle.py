@@ -1,0 +1,395 @@
+"""CPU restatement of FluidFlowSimulation.jl's time step (FP64, numpy/scipy).
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product path imports this module:
+it is the parity checker of tests/, the checker of __graft_entry__.smoke()
+and the CPU baseline leg of bench.py.
+
+It follows the reference's structure literally, on full FFTW-layout arrays
+(half spectrum (nx/2+1, ny, nz), x fastest), with the 12 real transforms of a
+plain step, separate dealias/scale sweeps, the u round trip and the
+projection after it:
+  advance_in_time!      models.jl:3-17
+  calculate_rhs!        models.jl:19-51
+  fourierspacep1!       models.jl:53-119       (w = ik x u)
+  realspace!            models.jl:121-214      (u x w, 1/2|u|^2, u round trip)
+  fourierspacep2_velocity! models.jl:292-341   (-nu k^2 u, Leray)
+  pressure_projection!  models.jl:443-456
+  myfourier!/dealias!/myscale!  utils.jl:41-51,101-137
+  AB3                   Adams_Bashforth.jl:142-256, fix_fm2! ETD.jl:130-140
+  set_dt!               time_step_functions.jl:86-129
+  stats                 stats.jl:64-219,289-331,608-635,750-814,897-917,952-980
+  spectrum_u + calculate_specHV  spec.jl:316-327,424-450
+The FFT is scipy.fft (pocketfft), standing in for FFTW with the same
+definitions (unnormalised forward e^{-ikx}, unnormalised backward).
+
+Parity status of each function is recorded in its docstring ("pinned by"):
+the checks live in tests/test_oracle.py.
+"""
+import math
+import os
+
+import numpy as np
+import scipy.fft as sfft
+
+__all__ = ["Grid", "OracleSim", "taylor_green", "workers"]
+
+
+def workers():
+    return int(os.environ.get("FFS_ORACLE_WORKERS", os.cpu_count() or 1))
+
+
+def rfftfreq(n, s):
+    """Globals.jl:95-97."""
+    return np.array([(n / 2 - i) / s for i in np.arange(n / 2, -1, -1)], dtype=np.float64)
+
+
+def fftfreq(n, s):
+    """Globals.jl:99-104."""
+    if n % 2 == 0:
+        a = [(n / 2 - i) / s for i in np.arange(n / 2, 0, -1)]
+        b = [-i / s for i in np.arange(n / 2, 0, -1)]
+    else:
+        a = [(n / 2 - i) / s for i in np.arange(n / 2, -0.5, -1)]
+        b = [-i / s for i in np.arange((n - 1) / 2, 0, -1)]
+    return np.array(a + b, dtype=np.float64)
+
+
+class Grid:
+    """Module-load constants of Globals.jl:106-182.
+
+    pinned by: closed forms of the wavenumbers, mask and loop ranges
+    (tests/test_oracle.py::test_grid_*)."""
+
+    def __init__(self, nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, cutoff=2 / 3):
+        self.NRX, self.NY, self.NZ = nx, ny, nz
+        self.NX = nx // 2 + 1
+        self.LX, self.LY, self.LZ = lx, ly, lz
+        kxp = rfftfreq(nx, lx)
+        kyp = fftfreq(ny, ly)
+        kzp = fftfreq(nz, lz)
+        self.KX, self.KY, self.KZ = kxp, kyp, kzp
+        cutoffx = (cutoff * kxp[-1]) ** 2
+        cutoffy = (cutoff * np.max(np.abs(kyp))) ** 2
+        cutoffz = (cutoff * np.max(np.abs(kzp))) ** 2
+        KXa = kxp[:, None, None]
+        KYa = kyp[None, :, None]
+        KZa = kzp[None, None, :]
+        self.DEALIAS = (KXa**2 > cutoffx) | (KYa**2 > cutoffy) | (KZa**2 > cutoffz)
+        self.kx, self.ky, self.kz = KXa, KYa, KZa
+        self.K2 = KXa * KXa + KYa * KYa + KZa * KZa
+        # loop ranges (Globals.jl:163-169), 0-based
+        fx = int(np.argmax(self.DEALIAS[:, 0, 0])) if self.DEALIAS[:, 0, 0].any() else self.NX
+        fy = int(np.argmax(self.DEALIAS[0, :, 0])) if self.DEALIAS[0, :, 0].any() else ny
+        fz = int(np.argmax(self.DEALIAS[0, 0, :])) if self.DEALIAS[0, 0, :].any() else nz
+        self.XRANGE = np.arange(0, fx)
+        self.YRANGE = np.concatenate([np.arange(0, fy), np.arange(ny - fy - 1, ny)]) if fy < ny else np.arange(ny)
+        self.ZRANGE = np.concatenate([np.arange(0, fz), np.arange(nz - fz - 1, nz)]) if fz < nz else np.arange(nz)
+        # retained loop set as a mask (the ranges cover every unmasked mode)
+        R = np.zeros((self.NX, ny, nz), dtype=bool)
+        R[np.ix_(self.XRANGE, self.YRANGE, self.ZRANGE)] = True
+        self.RANGE = R
+        self.W = np.where(np.arange(self.NX) > 0, 2.0, 1.0)[:, None, None]  # (1 + (i>1))
+        self.scale = 1.0 / (nx * ny * nz)
+        # spectra bins (Globals.jl:148-152)
+        self.KH = kxp if kxp[1] >= rfftfreq(ny, ly)[1] else rfftfreq(ny, ly)
+        self.MAXDKH = self.KH[1]
+        self.KRZ = rfftfreq(nz, lz)
+        self.DKZ = self.KRZ[1]
+
+    def coords(self):
+        """Physical grid points x_i = 2 pi L i / n."""
+        x = 2 * np.pi * self.LX * np.arange(self.NRX) / self.NRX
+        y = 2 * np.pi * self.LY * np.arange(self.NY) / self.NY
+        z = 2 * np.pi * self.LZ * np.arange(self.NZ) / self.NZ
+        return np.meshgrid(x, y, z, indexing="ij")
+
+    # transforms on (nx, ny, nz) real arrays, x = axis 0
+    def rfft(self, a):
+        """Unnormalised R2C (FFTW forward).  pinned by: brute-force DFT (test_oracle)."""
+        return sfft.rfftn(a, axes=(2, 1, 0), workers=workers())
+
+    def irfft(self, c):
+        """Unnormalised C2R (FFTW backward).  pinned by: brute-force DFT (test_oracle)."""
+        return sfft.irfftn(c, s=(self.NZ, self.NY, self.NRX), axes=(2, 1, 0), norm="forward", workers=workers())
+
+    def myfourier(self, a):
+        """rfft! + dealias! + myscale! (utils.jl:101-106)."""
+        c = self.rfft(a)
+        c[self.DEALIAS] = 0
+        c[self.RANGE] *= self.scale
+        return c
+
+    def project(self, v):
+        """pressure_projection! (models.jl:443-456) on the retained range."""
+        kx, ky, kz = self.kx, self.ky, self.kz
+        with np.errstate(invalid="ignore", divide="ignore"):
+            p1 = -(kx * v[0] + ky * v[1] + kz * v[2]) / self.K2
+        p1[0, 0, 0] = 0
+        p1 = np.where(self.RANGE, p1, 0)
+        return np.array([p1 * kx + v[0], p1 * ky + v[1], p1 * kz + v[2]])
+
+
+def taylor_green(grid):
+    """u1 = cos x sin y sin z, u2 = -sin x cos y sin z, u3 = 0 (the Taylor-Green case)."""
+    x, y, z = grid.coords()
+    return np.array([np.cos(x) * np.sin(y) * np.sin(z), -np.sin(x) * np.cos(y) * np.sin(z), np.zeros_like(x)])
+
+
+def proj(a, b):
+    """proj(a,b) = re(a) re(b) + im(a) im(b) (Forcing_methods/forcing.jl:4)."""
+    return a.real * b.real + a.imag * b.imag
+
+
+class OracleSim:
+    """Plain vorticity-form run with AB3 (fixed or variable dt) and optional
+    isotropic constant-power shell forcing.
+
+    pinned by: Taylor-Green t=0 closed forms (k=1/8, eps=6 nu k, pressure),
+    single-mode curl/projection closed forms, divergence-free state, energy
+    budget dk/dt = -eps, AB3 constant-step limit and third-order convergence
+    (tests/test_oracle.py)."""
+
+    def __init__(self, grid, nu, dt=0.0, variable_dt=False, cfl=0.18, dt_stat=1 << 30, dt_spec=1 << 30,
+                 forcing_kf=None, forcing_power=0.1, start=0, start_time=0.0):
+        self.g = grid
+        self.nu = nu
+        self.variable_dt = variable_dt
+        self.cfl = cfl
+        self.dt = self.dt2 = self.dt3 = dt
+        self.dt_stat = dt_stat
+        self.dt_spec = dt_spec
+        self.iteration = start
+        self.time = start_time
+        self.tit = 0
+        self.forcing_kf = forcing_kf
+        self.forcing_power = forcing_power
+        self.force_factor = 0.0
+        self.umax = 0.0
+        self.records = []
+        self.spectra = []
+
+    # -------------------------------------------------------------- setup
+    def set_velocity(self, u):
+        """Real (3, nx, ny, nz) field -> myfourier! of initialize! (FluidFlowSimulation.jl:69)."""
+        self.uh = np.array([self.g.myfourier(np.asarray(c, dtype=np.float64)) for c in u])
+
+    def initialize(self):
+        """initialize!(s) (FluidFlowSimulation.jl:67-91) for the AB3 velocity path."""
+        self.calculate_rhs(collect=False)
+        self.fm1 = self.rhs.copy()
+        self.fm2 = self.fm1.copy()
+        self.dt2 = self.dt
+        self.dt3 = self.dt
+
+    # -------------------------------------------------------------- the step
+    def is_stats_time(self):
+        return self.iteration % self.dt_stat == 0
+
+    def is_spec_time(self):
+        return self.iteration % self.dt_spec == 0
+
+    def calculate_rhs(self, collect=True):
+        g = self.g
+        u = self.uh
+        kx, ky, kz = g.kx, g.ky, g.kz
+        # fourierspacep1!: rhs = (ik) x u on the retained range
+        w = np.array([1j * (ky * u[2] - kz * u[1]), 1j * (kz * u[0] - kx * u[2]), 1j * (kx * u[1] - ky * u[0])])
+        w = np.where(g.RANGE, w, 0)
+        # realspace!
+        ur = np.array([g.irfft(c) for c in u])
+        wr = np.array([g.irfft(c) for c in w])
+        need_k = self.is_spec_time() or self.is_stats_time()
+        if need_k:
+            self.kspec = g.myfourier(0.5 * (ur[0] * ur[0] + ur[1] * ur[1] + ur[2] * ur[2]))
+        cross = np.array([ur[1] * wr[2] - ur[2] * wr[1], ur[2] * wr[0] - ur[0] * wr[2], ur[0] * wr[1] - ur[1] * wr[0]])
+        if self.variable_dt:
+            self.umax = float(np.max(np.abs(ur)))
+        rhs = np.array([g.myfourier(c) for c in cross])
+        rhs[:, 0, 0, 0] = 0
+        self.uh = g.project(np.array([g.myfourier(c) for c in ur]))
+        if self.variable_dt:
+            self.set_dt()
+        if collect and self.is_spec_time():
+            self.spectra.append((self.iteration, *self.spectrum()))
+        if collect and self.is_stats_time():
+            self.nlstats = self.non_linear_stats(rhs)
+            self.pressstats = self.pressure_stats(rhs)
+        # fourierspacep2_velocity!
+        rhs = rhs + (g.K2 * (-self.nu)) * self.uh
+        rhs = np.where(g.RANGE, rhs, 0)
+        self.rhs = g.project(rhs)
+
+    def set_dt(self):
+        """set_dt! (time_step_functions.jl:86-129) with the intended fresh max|u_i|."""
+        g = self.g
+        dx = 2 * math.pi * g.LY / g.NY
+        cfl = self.cfl
+        it = self.iteration
+        cfl *= {0: 0.1, 1: 0.2, 2: 0.4, 3: 0.8}.get(it, 1.0)
+        newdt = cfl * dx / self.umax
+        if it > 3:
+            dt = self.dt
+            newdt = max(dt / 1.25, newdt) if newdt < dt else min(1.25 * dt, newdt)
+        # Adams_Bashforth3rdO set_dt! (Adams_Bashforth.jl:166-176)
+        if self.tit == 0:
+            self.dt3 = self.dt2 = self.dt = newdt / 2
+        else:
+            self.dt3, self.dt2, self.dt = self.dt2, self.dt, newdt
+
+    def ab3_coefficients(self):
+        """get_At/get_Bt/get_Ct, Adams_Bashforth.jl:178-218 (variable-step form)."""
+        dt, dt2, dt3 = self.dt, self.dt2, self.dt3
+        At = 1.0 + (dt * (2 * dt + 6 * dt2 + 3 * dt3)) / (6 * dt2 * (dt2 + dt3))
+        At *= dt
+        Bt = -dt * dt * (2 * dt + 3 * dt2 + 3 * dt3) / (6 * dt2 * dt3 * (dt + dt2))
+        Bt *= dt + dt2
+        Ct = dt * dt * (2 * dt + 3 * dt2) / (6 * dt3 * (dt + dt2 + dt3) * (dt2 + dt3))
+        Ct *= dt + dt2 + dt3
+        return At, Bt, Ct
+
+    def forcing(self):
+        """Constant-power isotropic shell forcing (no reference counterpart):
+        factor = P / sum_{0<|k|<=kf} w |u_k|^2."""
+        if self.forcing_kf is None:
+            return
+        band = (self.g.K2 <= self.forcing_kf**2) & (self.g.K2 > 0) & self.g.RANGE
+        e = np.sum(self.g.W * band * (np.abs(self.uh) ** 2).sum(axis=0))
+        self.force_factor = self.forcing_power / e if e > 0 else 0.0
+
+    def time_step(self):
+        """VectorTimeStep functor (time_step_functions.jl:72-80) with AB3."""
+        g = self.g
+        self.tit += 1
+        if self.tit == 2:
+            self.fm2 = -(self.dt3 + self.dt2) * (self.rhs - self.fm1) / self.dt2 + self.rhs
+        At, Bt, Ct = self.ab3_coefficients()
+        u = At * self.rhs + (Bt * self.fm1 + (Ct * self.fm2 + self.uh))
+        u = np.where(g.RANGE, u, 0)
+        self.fm2 = self.fm1
+        self.fm1 = self.rhs
+        if self.forcing_kf is not None:
+            band = (g.K2 <= self.forcing_kf**2) & (g.K2 > 0) & g.RANGE
+            u = u + np.where(band, self.dt * self.force_factor, 0.0) * u
+        self.uh = g.project(u)
+
+    def advance_in_time(self, n=1):
+        for _ in range(n):
+            self.calculate_rhs()
+            self.forcing()
+            if self.is_stats_time():
+                self.records.append(self.stats())
+            self.time_step()
+            self.iteration += 1
+            self.time += self.dt
+
+    # -------------------------------------------------------------- stats
+    def non_linear_stats(self, nl):
+        """non_linear_stats (stats.jl:897-917)."""
+        g = self.g
+        kv = (g.kx, g.ky, g.kz)
+        out = [proj(nl[c] - 1j * kv[c] * self.kspec, self.uh[c]) for c in range(3)]
+        nlh = np.sum(np.where(g.RANGE, g.W * (out[0] + out[1]), 0))
+        nlv = np.sum(np.where(g.RANGE, g.W * out[2], 0))
+        return nlh, nlv
+
+    def pressure_stats(self, nl):
+        """pressure_stats (stats.jl:750-814) without LES or density."""
+        g = self.g
+        kv = (g.kx, g.ky, g.kz)
+        r = [nl[c] - (1j * self.kspec) * kv[c] for c in range(3)]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            p1 = -(g.kx * r[0] + g.ky * r[1] + g.kz * r[2]) / g.K2
+        p1[0, 0, 0] = 0
+        out = [proj(self.uh[c], p1 * kv[c]) for c in range(3)]
+        ph = np.sum(np.where(g.RANGE, g.W * (out[0] + out[1]), 0))
+        pv = np.sum(np.where(g.RANGE, g.W * out[2], 0))
+        return ph, pv
+
+    def stats(self):
+        """stats(s) (stats.jl:64-165): returns (Stats row, Scales row, Numbers row)."""
+        g = self.g
+        u = self.uh
+        nu = self.nu
+        W = np.where(g.RANGE, g.W, 0.0)
+        e = [np.abs(c) ** 2 for c in u]
+        u12, u22, u32 = (float(np.sum(W * c)) for c in e)
+        u1, u2, u3 = (float(c[0, 0, 0].real) for c in u)
+        kh = (u12 + u22) / 2
+        kv = u32 / 2
+        k = kh + kv
+        kaniso = 2 * kv / kh
+        eh = nu * float(np.sum(W * g.K2 * (e[0] + e[1])))
+        ev = nu * float(np.sum(W * g.K2 * e[2]))
+        eps = eh + ev
+        nlh, nlv = self.nlstats
+        pressh, pressv = self.pressstats
+        vstats = [u1, u2, u3, u12, u22, u32, kh, kv, kaniso, k, eh, ev, eps, nlh, nlv, pressh, pressv,
+                  -(nlh + pressh), (nlv + pressv)]
+        tdiss = eps
+        R = g.RANGE
+
+        def lx(c, p2):
+            return g.LX * float(np.sum(np.where(R[0], e[c][0], 0))) * math.pi / p2
+
+        def ly(c, p2):
+            return g.LY * float(np.sum(np.where(R[:, 0, :], (g.W[:, 0, :] * e[c][:, 0, :]), 0))) * math.pi / p2
+
+        def lz(c, p2):
+            return g.LZ * float(np.sum(np.where(R[:, :, 0], (g.W[:, :, 0] * e[c][:, :, 0]), 0))) * math.pi / p2
+
+        def dsq(c, kk):
+            return float(np.sum(W * kk * kk * e[c]))
+
+        L11, L12, L13 = lx(0, u12), ly(0, u12), lz(0, u12)
+        du1dyp2, du1dxp2 = dsq(0, g.ky), dsq(0, g.kx)
+        l12 = math.sqrt(2 * u12 / du1dyp2)
+        L21, L22, L23 = lx(1, u22), ly(1, u22), lz(1, u22)
+        du2dyp2, du2dxp2 = dsq(1, g.ky), dsq(1, g.kx)
+        l21 = math.sqrt(2 * u22 / du2dxp2)
+        L31, L32, L33 = lx(2, u32), ly(2, u32), lz(2, u32)
+        uh = math.sqrt((u12 + u22) / 2)
+        urms = math.sqrt((u12 + u22 + u32) / 3)
+        Lhh = (L11 + L22) / 2
+        Thh = Lhh / urms
+        ll = math.sqrt(2 * (u12 + u22) / (du1dxp2 + du2dyp2))
+        lt = math.sqrt(2 * (u12 + u22) / (du1dyp2 + du2dxp2))
+        Lhv = (L13 + L23) / 2
+        Lvv = L33
+        Lvh = (L31 + L32) / 2
+        Lt = math.sqrt(k**3) / tdiss
+        Tt = k / tdiss
+        Lk = math.sqrt(math.sqrt(nu**3 / tdiss))
+        Tk = math.sqrt(nu / tdiss)
+        lg = math.sqrt(15 * nu / tdiss) * urms
+        stats_row = [self.iteration, self.time, *vstats, tdiss]
+        scales = [self.iteration, self.time, Lk, Lt, lg, ll, lt, Lhh, Lvv, Lhv, Lvh, l12, l21, L11, L12, L13, L21,
+                  L22, L23, L31, L32, L33, Tk, Tt, Thh]
+        numbers = [self.iteration, self.time, uh * Lhh / nu, (Lhh / Lk) ** (4 / 3), urms * ll / nu, urms * lt / nu,
+                   (ll / Lk) ** (4 / 3), (lt / Lk) ** (4 / 3)]
+        return np.array(stats_row), np.array(scales), np.array(numbers)
+
+    def spectrum(self):
+        """spectrum_u + calculate_specHV (spec.jl:316-327,424-450), loop for loop."""
+        g = self.g
+        u = self.uh
+        hs = np.where(g.RANGE, (proj(u[0], u[0]) + proj(u[1], u[1])) / 2, 0.0)
+        vs = np.where(g.RANGE, proj(u[2], u[2]) / 2, 0.0)
+        out = []
+        for arr in (hs, vs):
+            h = np.zeros(len(g.KH))
+            v = np.zeros(len(g.KRZ))
+            for l in g.ZRANGE:
+                nk = int(round(abs(g.KZ[l]) / g.DKZ))
+                for j in g.YRANGE:
+                    nj = int(round(abs(g.KY[j]) / g.MAXDKH))
+                    ee = arr[0, j, l]
+                    h[0] += 0.5 * ee
+                    h[nj] += 0.5 * ee
+                    v[nk] += ee
+                    ee2 = 2 * arr[1:, j, l]
+                    nx = np.rint(np.abs(g.KX[1:]) / g.MAXDKH).astype(int)
+                    np.add.at(h, nx, 0.5 * ee2 * g.MAXDKH)
+                    h[nj] += float(np.sum(0.5 * ee2 * g.MAXDKH))
+                    v[nk] += float(np.sum(ee2 * g.DKZ))
+            out.append((h, v))
+        return out

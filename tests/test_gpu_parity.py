@@ -1,0 +1,190 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on the
+same inputs.  FP64 tolerance: fields 1e-12 relative (max-norm over the
+component, relative to the component's max), statistics 1e-10 relative."""
+import numpy as np
+import pytest
+
+import ffs
+from ffs import _lib
+from ffs.initial import random_spectrum, taylor_green
+from oracle.ffs_oracle import Grid, OracleSim
+
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-12
+STATS_TOL = 1e-10
+
+
+def make(n, nu, dt, u0, variable_dt=False, shape=None, dom=(1.0, 1.0, 1.0), dt_stat=1, forcing_kf=None,
+         cfl=0.5, write_time=1 << 30):
+    nx, ny, nz = shape or (n, n, n)
+    g = Grid(nx, ny, nz, *dom)
+    ref = OracleSim(g, nu, dt=dt, variable_dt=variable_dt, cfl=cfl, dt_stat=dt_stat, forcing_kf=forcing_kf)
+    ref.set_velocity(u0)
+    p = ffs.Params(nx=nx, ny=ny, nz=nz, x_domain=dom[0], y_domain=dom[1], z_domain=dom[2], nu=nu, dt=dt,
+                   integrator="Adams_Bashforth3rdO", variable_dt=variable_dt, cfl=cfl, dt_stat=dt_stat,
+                   dt_spec=1 << 30, write_time=write_time,
+                   forcing="shellForcing" if forcing_kf else None, forcing_kf=forcing_kf or 2.5)
+    s = ffs.Simulation(p)
+    s.set_velocity(*u0)
+    return ref, s
+
+
+def rel_field_err(a, b):
+    return max(np.abs(a[c] - b[c]).max() / max(np.abs(b[c]).max(), 1e-300) for c in range(3))
+
+
+def test_roundtrip_transform():
+    for shape in [(16, 16, 16), (32, 24, 48), (64, 64, 64)]:
+        print_case(f"transform round trip {shape}")
+        nx, ny, nz = shape
+        g = Grid(nx, ny, nz)
+        rng = np.random.default_rng(7)
+        a = rng.standard_normal((3, nx, ny, nz))
+        p = ffs.Params(nx=nx, ny=ny, nz=nz, nu=0.01, dt=0.01, integrator="Adams_Bashforth3rdO", variable_dt=False,
+                       dt_stat=1 << 20, dt_spec=1 << 20, write_time=1 << 20)
+        with ffs.Simulation(p) as s:
+            s.set_velocity(*a)
+            for c in range(3):
+                ref = g.myfourier(a[c])
+                got = s.get_field(c, _lib.FFS_SPACE_FOURIER)
+                err = np.abs(got - ref).max() / np.abs(ref).max()
+                assert err < 1e-13, (shape, c, err)
+                back = s.get_field(c, _lib.FFS_SPACE_REAL)
+                assert np.all(back[nx:] == 0)
+                refr = g.irfft(ref)
+                err_r = np.abs(back[:nx] - refr).max() / np.abs(refr).max()
+                assert err_r < 1e-13, (shape, c, err_r)
+
+
+def test_unpadded_and_padded_inputs_agree():
+    n = 16
+    u0 = random_spectrum(n, seed=3)
+    p = ffs.Params(nx=n, ny=n, nz=n, nu=0.01, dt=0.01, integrator="Adams_Bashforth3rdO", variable_dt=False,
+                   dt_stat=1 << 20, dt_spec=1 << 20, write_time=1 << 20)
+    with ffs.Simulation(p) as a, ffs.Simulation(p) as b:
+        a.set_velocity(*u0)
+        b.set_velocity(*[ffs.pad(c) for c in u0])
+        for c in range(3):
+            assert np.array_equal(a.get_field(c, _lib.FFS_SPACE_FOURIER), b.get_field(c, _lib.FFS_SPACE_FOURIER))
+
+
+@pytest.mark.parametrize("n", [16, 24, 32, 48, 64])
+def test_taylor_green_one_step(n):
+    print_case(f"taylor-green one step n={n}")
+    nu, dt = 0.0333333, 0.05
+    g = Grid(n, n, n)
+    ref, s = make(n, nu, dt, taylor_green(n))
+    with s:
+        ref.initialize()
+        s.initialize()
+        ref.advance_in_time(1)
+        s.advance_in_time(1)
+        err = rel_field_err(s.velocity_hat(), ref.uh)
+        assert err < FIELD_TOL, err
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 48, 48), (32, 24, 48), (64, 32, 16)])
+def test_random_field_steps(shape):
+    print_case(f"random field 5 steps {shape}")
+    nx, ny, nz = shape
+    u0 = random_spectrum(nx, ny, nz, seed=11)
+    ref, s = make(nx, 0.01, 0.01, u0, shape=shape)
+    with s:
+        ref.initialize()
+        s.initialize()
+        for step in range(5):
+            ref.advance_in_time(1)
+            s.advance_in_time(1)
+            err = rel_field_err(s.velocity_hat(), ref.uh)
+            assert err < FIELD_TOL * (step + 1), (step, err)
+
+
+def test_noncubic_domain():
+    shape = (32, 16, 24)
+    dom = (1.0, 0.5, 2.0)
+    print_case(f"non-cubic domain {shape} {dom}")
+    u0 = random_spectrum(*shape, seed=5, lx=dom[0], ly=dom[1], lz=dom[2])
+    ref, s = make(32, 0.02, 0.005, u0, shape=shape, dom=dom)
+    with s:
+        ref.initialize()
+        s.initialize()
+        ref.advance_in_time(3)
+        s.advance_in_time(3)
+        assert rel_field_err(s.velocity_hat(), ref.uh) < 3 * FIELD_TOL
+
+
+def test_taylor_green_stats_40_steps():
+    n, nu, dt = 32, 0.0333333, 0.05
+    print_case("taylor-green 32^3 stats every step, 40 steps")
+    ref, s = make(n, nu, dt, taylor_green(n), dt_stat=1)
+    with s:
+        ref.initialize()
+        s.initialize()
+        ref.advance_in_time(40)
+        s.advance_in_time(40)
+        rows = s.pop_stats()
+    assert len(rows) == 40
+    for (st, sc, nm), (ost, osc, onm) in zip(rows, ref.records):
+        assert st[0] == ost[0]
+        assert abs(st[1] - ost[1]) <= 1e-14 * max(1.0, abs(ost[1]))
+        for col in (5, 6, 7, 8, 9, 11, 12, 13, 14, 21):  # u1p2..k, dissh, dissv, diss, tdiss
+            assert abs(st[col] - ost[col]) <= STATS_TOL * abs(ost[col]), (st[0], col, st[col], ost[col])
+        # transfer terms: absolute tolerance on the scale of the dissipation
+        for col in (15, 16, 17, 18):
+            assert abs(st[col] - ost[col]) <= 1e-12 * abs(ost[14]) + STATS_TOL * abs(ost[col]), (col, st[col], ost[col])
+        for a, b in zip(sc[2:], osc[2:]):
+            if np.isfinite(b):
+                assert abs(a - b) <= 1e-9 * abs(b), (sc, osc)
+
+
+def test_variable_dt_and_forcing():
+    n = 32
+    print_case("random field, variable dt (cfl 0.5) + shell forcing, 12 steps")
+    u0 = random_spectrum(n, seed=2)
+    ref, s = make(n, 0.005, 0.0, u0, variable_dt=True, cfl=0.5, forcing_kf=2.5, dt_stat=3)
+    with s:
+        ref.initialize()
+        s.initialize()
+        for step in range(12):
+            ref.advance_in_time(1)
+            s.advance_in_time(1)
+            assert abs(s.dt - ref.dt) <= 1e-13 * ref.dt, (step, s.dt, ref.dt)
+            assert abs(s.time - ref.time) <= 1e-13 * ref.time
+            err = rel_field_err(s.velocity_hat(), ref.uh)
+            assert err < FIELD_TOL * (step + 1), (step, err)
+        rows = s.pop_stats()
+    assert [int(r[0][0]) for r in rows] == [int(r[0][0]) for r in ref.records]
+
+
+def test_spectrum_matches_oracle():
+    n = 32
+    print_case("spectrum_u + calculate_specHV")
+    u0 = random_spectrum(n, seed=4)
+    ref, s = make(n, 0.01, 0.01, u0)
+    with s:
+        ref.initialize()
+        s.initialize()
+        ref.advance_in_time(2)
+        s.advance_in_time(2)
+        got = s.spectrum()
+    exp = ref.spectrum()
+    for (gh, gv), (eh, ev) in zip(got, exp):
+        np.testing.assert_allclose(gh, eh, rtol=1e-11, atol=1e-14 * np.abs(eh).max())
+        np.testing.assert_allclose(gv, ev, rtol=1e-11, atol=1e-14 * np.abs(ev).max())
+
+
+def test_output_event():
+    n = 16
+    print_case("output event cadence")
+    ref, s = make(n, 0.01, 0.01, taylor_green(n), write_time=4)
+    with s:
+        s.initialize()
+        done, ev = s.advance(10)
+        assert (done, ev) == (4, _lib.FFS_EVENT_OUTPUT) and s.iteration == 4
+        done, ev = s.advance(6)
+        assert (done, ev) == (4, _lib.FFS_EVENT_OUTPUT) and s.iteration == 8
+        done, ev = s.advance(2)
+        assert (done, ev) == (2, _lib.FFS_EVENT_NONE) and s.iteration == 10
