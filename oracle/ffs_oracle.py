@@ -82,8 +82,8 @@ class Grid:
         fy = int(np.argmax(self.DEALIAS[0, :, 0])) if self.DEALIAS[0, :, 0].any() else ny
         fz = int(np.argmax(self.DEALIAS[0, 0, :])) if self.DEALIAS[0, 0, :].any() else nz
         self.XRANGE = np.arange(0, fx)
-        self.YRANGE = np.concatenate([np.arange(0, fy), np.arange(ny - fy - 1, ny)]) if fy < ny else np.arange(ny)
-        self.ZRANGE = np.concatenate([np.arange(0, fz), np.arange(nz - fz - 1, nz)]) if fz < nz else np.arange(nz)
+        self.YRANGE = np.concatenate([np.arange(0, fy), np.arange(ny - fy - 2, ny)]) if fy < ny else np.arange(ny)
+        self.ZRANGE = np.concatenate([np.arange(0, fz), np.arange(nz - fz - 2, nz)]) if fz < nz else np.arange(nz)
         # retained loop set as a mask (the ranges cover every unmasked mode)
         R = np.zeros((self.NX, ny, nz), dtype=bool)
         R[np.ix_(self.XRANGE, self.YRANGE, self.ZRANGE)] = True
@@ -317,7 +317,8 @@ class OracleSim:
         kh = (u12 + u22) / 2
         kv = u32 / 2
         k = kh + kv
-        kaniso = 2 * kv / kh
+        with np.errstate(all="ignore"):
+            kaniso = np.float64(2 * kv) / np.float64(kh)
         eh = nu * float(np.sum(W * g.K2 * (e[0] + e[1])))
         ev = nu * float(np.sum(W * g.K2 * e[2]))
         eps = eh + ev
@@ -329,43 +330,46 @@ class OracleSim:
         R = g.RANGE
 
         def lx(c, p2):
-            return g.LX * float(np.sum(np.where(R[0], e[c][0], 0))) * math.pi / p2
+            return g.LX * np.float64(np.sum(np.where(R[0], e[c][0], 0))) * math.pi / p2
 
         def ly(c, p2):
-            return g.LY * float(np.sum(np.where(R[:, 0, :], (g.W[:, 0, :] * e[c][:, 0, :]), 0))) * math.pi / p2
+            return g.LY * np.float64(np.sum(np.where(R[:, 0, :], (g.W[:, 0, :] * e[c][:, 0, :]), 0))) * math.pi / p2
 
         def lz(c, p2):
-            return g.LZ * float(np.sum(np.where(R[:, :, 0], (g.W[:, :, 0] * e[c][:, :, 0]), 0))) * math.pi / p2
+            return g.LZ * np.float64(np.sum(np.where(R[:, :, 0], (g.W[:, :, 0] * e[c][:, :, 0]), 0))) * math.pi / p2
 
         def dsq(c, kk):
             return float(np.sum(W * kk * kk * e[c]))
 
-        L11, L12, L13 = lx(0, u12), ly(0, u12), lz(0, u12)
-        du1dyp2, du1dxp2 = dsq(0, g.ky), dsq(0, g.kx)
-        l12 = math.sqrt(2 * u12 / du1dyp2)
-        L21, L22, L23 = lx(1, u22), ly(1, u22), lz(1, u22)
-        du2dyp2, du2dxp2 = dsq(1, g.ky), dsq(1, g.kx)
-        l21 = math.sqrt(2 * u22 / du2dxp2)
-        L31, L32, L33 = lx(2, u32), ly(2, u32), lz(2, u32)
-        uh = math.sqrt((u12 + u22) / 2)
-        urms = math.sqrt((u12 + u22 + u32) / 3)
-        Lhh = (L11 + L22) / 2
-        Thh = Lhh / urms
-        ll = math.sqrt(2 * (u12 + u22) / (du1dxp2 + du2dyp2))
-        lt = math.sqrt(2 * (u12 + u22) / (du1dyp2 + du2dxp2))
-        Lhv = (L13 + L23) / 2
-        Lvv = L33
-        Lvh = (L31 + L32) / 2
-        Lt = math.sqrt(k**3) / tdiss
-        Tt = k / tdiss
-        Lk = math.sqrt(math.sqrt(nu**3 / tdiss))
-        Tk = math.sqrt(nu / tdiss)
-        lg = math.sqrt(15 * nu / tdiss) * urms
+        with np.errstate(all="ignore"):  # Julia float semantics: x/0 -> Inf, 0/0 -> NaN
+            u12, u22, u32, k, tdiss, nu = (np.float64(v) for v in (u12, u22, u32, k, tdiss, nu))
+            L11, L12, L13 = lx(0, u12), ly(0, u12), lz(0, u12)
+            du1dyp2, du1dxp2 = dsq(0, g.ky), dsq(0, g.kx)
+            l12 = np.sqrt(2 * u12 / du1dyp2)
+            L21, L22, L23 = lx(1, u22), ly(1, u22), lz(1, u22)
+            du2dyp2, du2dxp2 = dsq(1, g.ky), dsq(1, g.kx)
+            l21 = np.sqrt(2 * u22 / du2dxp2)
+            L31, L32, L33 = lx(2, u32), ly(2, u32), lz(2, u32)
+            uh = np.sqrt((u12 + u22) / 2)
+            urms = np.sqrt((u12 + u22 + u32) / 3)
+            Lhh = (L11 + L22) / 2
+            Thh = Lhh / urms
+            ll = np.sqrt(2 * (u12 + u22) / (du1dxp2 + du2dyp2))
+            lt = np.sqrt(2 * (u12 + u22) / (du1dyp2 + du2dxp2))
+            Lhv = (L13 + L23) / 2
+            Lvv = L33
+            Lvh = (L31 + L32) / 2
+            Lt = np.sqrt(k**3) / tdiss
+            Tt = k / tdiss
+            Lk = np.sqrt(np.sqrt(nu**3 / tdiss))
+            Tk = np.sqrt(nu / tdiss)
+            lg = np.sqrt(15 * nu / tdiss) * urms
         stats_row = [self.iteration, self.time, *vstats, tdiss]
         scales = [self.iteration, self.time, Lk, Lt, lg, ll, lt, Lhh, Lvv, Lhv, Lvh, l12, l21, L11, L12, L13, L21,
                   L22, L23, L31, L32, L33, Tk, Tt, Thh]
-        numbers = [self.iteration, self.time, uh * Lhh / nu, (Lhh / Lk) ** (4 / 3), urms * ll / nu, urms * lt / nu,
-                   (ll / Lk) ** (4 / 3), (lt / Lk) ** (4 / 3)]
+        with np.errstate(all="ignore"):
+            numbers = [self.iteration, self.time, uh * Lhh / nu, (Lhh / Lk) ** (4 / 3), urms * ll / nu,
+                       urms * lt / nu, (ll / Lk) ** (4 / 3), (lt / Lk) ** (4 / 3)]
         return np.array(stats_row), np.array(scales), np.array(numbers)
 
     def spectrum(self):
