@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--fp32", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first time a plain 1 GiB device copy (HBM calibration for the traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -91,8 +93,36 @@ def reference_ranges(n, cutoff=2 / 3):
     return mx, 2 * f + 2, 2 * f + 2
 
 
+def copy_probe():
+    """Plain device-to-device copy of 1 GiB (torch's copy kernel): reads 1 GiB and
+    writes 1 GiB; its FETCH_SIZE / WRITE_SIZE calibrate the traffic counters and
+    its rate is the measured copy bandwidth of the box."""
+    import torch
+
+    n = (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 10
+    gbs = 2 * (1 << 30) / (ms * 1e-3) / 1e9
+    log(f"copy probe: 1 GiB copy {ms:.3f} ms -> {gbs:.0f} GB/s (read+write)")
+    del a, b
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def bench(args):
     import numpy as np
+
+    copy_gbs = copy_probe() if args.copy_probe else None
 
     import ffs
     from ffs.initial import random_spectrum
@@ -201,6 +231,7 @@ def bench(args):
             "pruned_GBs": step_actual / (ms_per_step * 1e-3) / 1e9,
         },
         "kernels": kernels,
+        "copy_probe_GBs": copy_gbs,
     }
     s.close()
     if not args.no_cpu:
