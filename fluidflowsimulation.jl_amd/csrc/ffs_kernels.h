@@ -12,6 +12,7 @@ struct Geo {
   int nrx, ny, nz;   // physical grid (nx, ny, nz of the `global` file)
   int nxh;           // nrx/2 + 1  (NX of Globals.jl:111)
   int mx, my, mz;    // retained modes per direction
+  int sx;            // x stride of spectral/pencil arrays: mx rounded up to a 128-byte line
   const int* ymap;   // compact y -> FFT index       [my]
   const int* zmap;   // compact z -> FFT index       [mz]
   const int* yinv;   // FFT y index -> compact or -1 [ny]

@@ -25,11 +25,16 @@
 
 namespace fsk {
 
-template <typename T> __host__ __device__ constexpr int col_tx(int N) {
-  return sizeof(T) == 8 ? 16 : (N <= 256 ? 32 : 16);
-}
+// columns per tile of the y/z passes: one 128-byte line of x-consecutive values
+template <typename T> __host__ __device__ constexpr int col_tx(int) { return 128 / (2 * (int)sizeof(T)); }
 template <int N> __host__ __device__ constexpr int tper() { return N / default_ppt(N); }
 template <int N> __host__ __device__ constexpr int xrows() { return (64 / tper<N>()) > 0 ? (64 / tper<N>()) : 1; }
+
+// points per lane of the register-resident x pass and whether it applies
+template <int N> __host__ __device__ constexpr int xreg_ppt() { return 8; }
+template <int N> __host__ __device__ constexpr bool xreg_ok() {
+  return N % 8 == 0 && 64 % (N / 8) == 0 && plan_even(N, N / 8, 8);
+}
 
 template <typename T> __device__ __forceinline__ T nanmax(T a, T b) { return (a > b || a != a) ? a : b; }
 
@@ -67,66 +72,79 @@ __global__ void __launch_bounds__(512) k_col(Geo g, const C<T>* __restrict__ in,
   const int cx = blockIdx.x * TX + c;
   const bool valid = cx < g.mx;
   const int b = blockIdx.y;
-  const int f = blockIdx.z;
-  const size_t mx = g.mx;
-  const size_t Ssp = mx * g.my * g.mz;  // compact spectral
-  const size_t S1 = mx * g.my * g.nz;   // (mx, my, nz)
-  const size_t S2 = mx * g.ny * g.nz;   // (mx, ny, nz)
+  const size_t sx = g.sx;
+  const size_t Ssp = sx * g.my * g.mz;  // compact spectral
+  const size_t S1 = sx * g.my * g.nz;   // (sx, my, nz)
+  const size_t S2 = sx * g.ny * g.nz;   // (sx, ny, nz)
   auto ldsf = [&](int p) -> C<T>& { return lds[p * TX + c]; };
 
   if constexpr (KIND == INV_Z) {
     const int cy = b;
-    auto ld = [&](int p) -> C<T> {
-      const int cz = g.zinv[p];
-      if (!valid || cz < 0) return mk<T>(0, 0);
-      const size_t idx = cx + mx * (cy + (size_t)g.my * cz);
-      if (!CURL || f < 3) return in[f * Ssp + idx];
-      const T kx = (T)g.kx[cx], ky = (T)g.ky[cy], kz = (T)g.kz[cz];
-      const C<T> ux = in[idx], uy = in[Ssp + idx], uz = in[2 * Ssp + idx];
-      C<T> a;  // w = i k x u
-      if (f == 3) a = mk<T>(ky * uz.x - kz * uy.x, ky * uz.y - kz * uy.y);
-      else if (f == 4) a = mk<T>(kz * ux.x - kx * uz.x, kz * ux.y - kx * uz.y);
-      else a = mk<T>(kx * uy.x - ky * ux.x, kx * uy.y - ky * ux.y);
-      return mk<T>(-a.y, a.x);
-    };
-    auto st = [&](int p, C<T> v) {
-      if (valid) out[f * S1 + cx + mx * (cy + (size_t)g.my * p)] = v;
-    };
-    Fft<T, N, +1, TP>::run(q, tw, ld, st, ldsf);
+    // the curl variant loops over its 6 output fields inside the block, so the
+    // u tile is fetched from HBM once and re-read from L2
+    const int f0 = CURL ? 0 : blockIdx.z;
+    const int f1 = CURL ? 6 : blockIdx.z + 1;
+    const T kx = valid ? (T)g.kx[cx] : (T)0, ky = (T)g.ky[cy];
+    for (int f = f0; f < f1; ++f) {
+      if (f != f0) __syncthreads();
+      auto ld = [&](int p, int, int) -> C<T> {
+        const int cz = g.zinv[p];
+        if (!valid || cz < 0) return mk<T>(0, 0);
+        const size_t idx = cx + sx * (cy + (size_t)g.my * cz);
+        if (!CURL || f < 3) return in[f * Ssp + idx];
+        const T kz = (T)g.kz[cz];
+        C<T> a;  // w = i k x u
+        if (f == 3) {
+          const C<T> uy = in[Ssp + idx], uz = in[2 * Ssp + idx];
+          a = mk<T>(ky * uz.x - kz * uy.x, ky * uz.y - kz * uy.y);
+        } else if (f == 4) {
+          const C<T> ux = in[idx], uz = in[2 * Ssp + idx];
+          a = mk<T>(kz * ux.x - kx * uz.x, kz * ux.y - kx * uz.y);
+        } else {
+          const C<T> ux = in[idx], uy = in[Ssp + idx];
+          a = mk<T>(kx * uy.x - ky * ux.x, kx * uy.y - ky * ux.y);
+        }
+        return mk<T>(-a.y, a.x);
+      };
+      auto st = [&](int p, int, int, C<T> v) {
+        if (valid) out[f * S1 + cx + sx * (cy + (size_t)g.my * p)] = v;
+      };
+      Fft<T, N, +1, TP>::run(q, tw, ld, st, ldsf);
+    }
   } else if constexpr (KIND == INV_Y) {
-    const int z = b;
-    auto ld = [&](int p) -> C<T> {
+    const int z = b, f = blockIdx.z;
+    auto ld = [&](int p, int, int) -> C<T> {
       const int cy = g.yinv[p];
       if (!valid || cy < 0) return mk<T>(0, 0);
-      return in[f * S1 + cx + mx * (cy + (size_t)g.my * z)];
+      return in[f * S1 + cx + sx * (cy + (size_t)g.my * z)];
     };
-    auto st = [&](int p, C<T> v) {
-      if (valid) out[f * S2 + cx + mx * (p + (size_t)g.ny * z)] = v;
+    auto st = [&](int p, int, int, C<T> v) {
+      if (valid) out[f * S2 + cx + sx * (p + (size_t)g.ny * z)] = v;
     };
     Fft<T, N, +1, TP>::run(q, tw, ld, st, ldsf);
   } else if constexpr (KIND == FWD_Y) {
-    const int z = b;
-    auto ld = [&](int p) -> C<T> {
+    const int z = b, f = blockIdx.z;
+    auto ld = [&](int p, int, int) -> C<T> {
       if (!valid) return mk<T>(0, 0);
-      return in[f * S2 + cx + mx * (p + (size_t)g.ny * z)];
+      return in[f * S2 + cx + sx * (p + (size_t)g.ny * z)];
     };
-    auto st = [&](int p, C<T> v) {
+    auto st = [&](int p, int, int, C<T> v) {
       const int cy = g.yinv[p];
-      if (valid && cy >= 0) out[f * S1 + cx + mx * (cy + (size_t)g.my * z)] = v;
+      if (valid && cy >= 0) out[f * S1 + cx + sx * (cy + (size_t)g.my * z)] = v;
     };
     Fft<T, N, -1, TP>::run(q, tw, ld, st, ldsf);
   } else {  // FWD_Z
-    const int cy = b;
-    auto ld = [&](int p) -> C<T> {
+    const int cy = b, f = blockIdx.z;
+    auto ld = [&](int p, int, int) -> C<T> {
       if (!valid) return mk<T>(0, 0);
-      return in[f * S1 + cx + mx * (cy + (size_t)g.my * p)];
+      return in[f * S1 + cx + sx * (cy + (size_t)g.my * p)];
     };
-    auto st = [&](int p, C<T> v) {
+    auto st = [&](int p, int, int, C<T> v) {
       const int cz = g.zinv[p];
       if (valid && cz >= 0) {
         C<T> r = cscale<T>(v, scale);
         if (f < nzero && cx == 0 && cy == 0 && cz == 0) r = mk<T>(0, 0);
-        const size_t o = cx + mx * (cy + (size_t)g.my * cz);
+        const size_t o = cx + sx * (cy + (size_t)g.my * cz);
         if (f < 3) out[f * Ssp + o] = r;   // velocity-like fields
         else out2[(f - 3) * Ssp + o] = r;  // extra fields (1/2|u|^2, ...)
       }
@@ -146,7 +164,7 @@ static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, con
     hipFuncSetAttribute((const void*)&k_col<T, N, KIND, CURL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr = true;
   }
-  dim3 grid((g.mx + TX - 1) / TX, nb, nf);
+  dim3 grid((g.mx + TX - 1) / TX, nb, CURL ? 1 : nf);
   k_col<T, N, KIND, CURL><<<grid, TX * TP, lds, s>>>(g, (const C<T>*)in, (C<T>*)out, (C<T>*)out2, (const C<T>*)tw, nzero,
                                                     (T)scale);
 }
@@ -245,7 +263,7 @@ __global__ void __launch_bounds__(256) k_x_nl(Geo g, const C<T>* __restrict__ a2
   const int nrows = g.ny * g.nz;
   const int row0 = blockIdx.x * ROWS;
   const int mx = g.mx;
-  const size_t S2 = (size_t)mx * nrows;
+  const size_t S2 = (size_t)g.sx * nrows;
 
   // A: gather retained half spectra into Hermitian-packed pairs (ux,uy) (uz,wx) (wy,wz)
   for (int w = tid; w < NSLOT * (N / 2 + 1); w += NTHR) {
@@ -254,8 +272,8 @@ __global__ void __launch_bounds__(256) k_x_nl(Geo g, const C<T>* __restrict__ a2
     const int row = row0 + r;
     C<T> A = mk<T>(0, 0), B = mk<T>(0, 0);
     if (row < nrows && k < mx) {
-      A = a2[(2 * pr) * S2 + (size_t)row * mx + k];
-      B = a2[(2 * pr + 1) * S2 + (size_t)row * mx + k];
+      A = a2[(2 * pr) * S2 + (size_t)row * g.sx + k];
+      B = a2[(2 * pr + 1) * S2 + (size_t)row * g.sx + k];
     }
     build_pair<T, N>(lds + sl * LD, k, A, B);
   }
@@ -264,7 +282,8 @@ __global__ void __launch_bounds__(256) k_x_nl(Geo g, const C<T>* __restrict__ a2
   C<T>* Zs = lds + sl * LD;
   // B: inverse transforms (all slots)
   Fft<T, N, +1, TP, true, true>::run(
-      q, tw, [&](int p) { return Zs[p]; }, [&](int p, C<T> v) { Zs[p] = v; }, [&](int p) -> C<T>& { return Zs[p]; });
+      q, tw, [&](int p, int, int) { return Zs[p]; }, [&](int p, int, int, C<T> v) { Zs[p] = v; },
+      [&](int p) -> C<T>& { return Zs[p]; });
   __syncthreads();
   // C: pointwise u x w (and 1/2|u|^2), max |u_i|
   double lmax = 0.0;
@@ -290,7 +309,8 @@ __global__ void __launch_bounds__(256) k_x_nl(Geo g, const C<T>* __restrict__ a2
   __syncthreads();
   // D: forward transforms (the third slot of each row is idle work)
   Fft<T, N, -1, TP, true, true>::run(
-      q, tw, [&](int p) { return Zs[p]; }, [&](int p, C<T> v) { Zs[p] = v; }, [&](int p) -> C<T>& { return Zs[p]; });
+      q, tw, [&](int p, int, int) { return Zs[p]; }, [&](int p, int, int, C<T> v) { Zs[p] = v; },
+      [&](int p) -> C<T>& { return Zs[p]; });
   __syncthreads();
   // E: separate the two real transforms of each slot, keep k < mx
   for (int w = tid; w < ROWS * 2 * mx; w += NTHR) {
@@ -299,8 +319,97 @@ __global__ void __launch_bounds__(256) k_x_nl(Geo g, const C<T>* __restrict__ a2
     if (row >= nrows) continue;
     C<T> A, B;
     split_pair<T, N>(lds + (3 * r + j) * LD, k, A, B);
-    b2[(2 * j) * S2 + (size_t)row * mx + k] = A;
-    if (j == 0 || KE) b2[(2 * j + 1) * S2 + (size_t)row * mx + k] = B;
+    b2[(2 * j) * S2 + (size_t)row * g.sx + k] = A;
+    if (j == 0 || KE) b2[(2 * j + 1) * S2 + (size_t)row * g.sx + k] = B;
+  }
+  if (umax) block_max_to(lmax, umax);
+}
+
+// Register-resident variant of the fused x pass (used when every FFT stage
+// splits evenly over the TPER lanes of a row, i.e. power-of-two nx): TPER
+// consecutive lanes own one row; lane q keeps the row's values at the
+// positions p = q (mod TPER) in registers from the first load to the last
+// store, so LDS carries only the one exchange inside each transform (plus the
+// k <-> N-k pairing at the end), synchronised per wavefront.
+template <typename T, int N, bool KE>
+__global__ void __launch_bounds__(128) k_x_nl_reg(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
+                                                 const C<T>* __restrict__ tw, unsigned long long* __restrict__ umax) {
+  constexpr int PPT = xreg_ppt<N>();
+  constexpr int TP = N / PPT;
+  constexpr int GPW = 64 / TP;  // rows per wavefront
+  constexpr int LDX = N + 1;
+  constexpr int NSTG = plan_ns(N, PPT);
+  constexpr int M0 = (N / plan_r(N, 0, PPT)) / TP;         // register stride of first-stage legs
+  constexpr int ML = (N / plan_r(N, NSTG - 1, PPT)) / TP;  // register stride of last-stage legs
+  static_assert(xreg_ok<N>(), "register-resident x pass needs an even plan");
+  extern __shared__ unsigned char smem_raw[];
+  C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int grp = lane / TP, q = lane % TP;
+  const int nrows = g.ny * g.nz;
+  const int row = (blockIdx.x * (int)(blockDim.x >> 6) + wave) * GPW + grp;
+  const bool row_ok = row < nrows;
+  const int mx = g.mx;
+  const size_t S2 = (size_t)g.sx * nrows;
+  const size_t roff = (size_t)(row_ok ? row : 0) * g.sx;
+  C<T>* X = lds + (size_t)(wave * GPW + grp) * LDX;
+  auto xref = [&](int p) -> C<T>& { return X[p]; };
+
+  C<T> R[3][PPT];
+  // inverse transforms of the pairs (ux,uy) (uz,wx) (wy,wz); Z built from the half spectra on load
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const C<T>* A = a2 + (size_t)(2 * j) * S2 + roff;
+    const C<T>* B = a2 + (size_t)(2 * j + 1) * S2 + roff;
+    auto ld = [&](int p, int, int) -> C<T> {
+      if (!row_ok) return mk<T>(0, 0);
+      if (p < mx) {
+        const C<T> a = A[p], b = B[p];
+        if (p == 0 || 2 * p == N) return mk<T>(a.x, b.x);
+        return mk<T>(a.x - b.y, a.y + b.x);
+      }
+      const int k = N - p;
+      if (k < mx && 2 * p != N) {
+        const C<T> a = A[k], b = B[k];
+        return mk<T>(a.x + b.y, b.x - a.y);
+      }
+      return mk<T>(0, 0);
+    };
+    auto st = [&](int, int t, int r, C<T> v) { R[j][t + r * ML] = v; };
+    if (j > 0) __syncwarp();
+    Fft<T, N, +1, TP, false, false, SYNC_WAVE, PPT>::run(q, tw, ld, st, xref);
+  }
+  // u x w, 1/2|u|^2, max|u_i|
+  double lmax = 0.0;
+#pragma unroll
+  for (int m = 0; m < PPT; ++m) {
+    const T ux = R[0][m].x, uy = R[0][m].y, uz = R[1][m].x, wx = R[1][m].y, wy = R[2][m].x, wz = R[2][m].y;
+    T ke = 0;
+    if constexpr (KE) ke = (T)0.5 * (ux * ux + uy * uy + uz * uz);
+    R[0][m] = mk<T>(uy * wz - uz * wy, uz * wx - ux * wz);
+    R[1][m] = mk<T>(ux * wy - uy * wx, ke);
+    if (row_ok) lmax = nanmax(lmax, (double)nanmax(fabs(ux), nanmax(fabs(uy), fabs(uz))));
+  }
+  // forward transforms of (px,py) and (pz,ke), split into the half spectra k < mx
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    auto ld = [&](int, int t, int r) { return R[j][t + r * M0]; };
+    auto st = [&](int, int t, int r, C<T> v) { R[j][t + r * ML] = v; };
+    __syncwarp();
+    Fft<T, N, -1, TP, false, false, SYNC_WAVE, PPT>::run(q, tw, ld, st, xref);
+    __syncwarp();
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) X[q + TP * m] = R[j][m];
+    __syncwarp();
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+      const int k = q + TP * m;
+      if (row_ok && k < mx) {
+        const C<T> zk = R[j][m], zn = X[(N - k) % N];
+        b2[(size_t)(2 * j) * S2 + roff + k] = mk<T>((T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y));
+        if (j == 0 || KE) b2[(size_t)(2 * j + 1) * S2 + roff + k] = mk<T>((T)0.5 * (zk.y + zn.y), (T)0.5 * (zn.x - zk.x));
+      }
+    }
   }
   if (umax) block_max_to(lmax, umax);
 }
@@ -320,7 +429,7 @@ __global__ void __launch_bounds__(256) k_x_c2r(Geo g, const C<T>* __restrict__ a
   const int nrows = g.ny * g.nz;
   const int npairs = (nrows + 1) / 2;
   const int mx = g.mx;
-  const size_t S2 = (size_t)mx * nrows;
+  const size_t S2 = (size_t)g.sx * nrows;
   const int rl = 2 * g.nxh;
   const size_t RS = (size_t)rl * nrows;
   const int p0 = blockIdx.x * NSLOT;
@@ -331,8 +440,8 @@ __global__ void __launch_bounds__(256) k_x_c2r(Geo g, const C<T>* __restrict__ a
     C<T> A = mk<T>(0, 0), B = mk<T>(0, 0);
     if (pidx < nf * npairs && k < mx) {
       const int f = pidx / npairs, row = 2 * (pidx % npairs);
-      A = a2[f * S2 + (size_t)row * mx + k];
-      if (row + 1 < nrows) B = a2[f * S2 + (size_t)(row + 1) * mx + k];
+      A = a2[f * S2 + (size_t)row * g.sx + k];
+      if (row + 1 < nrows) B = a2[f * S2 + (size_t)(row + 1) * g.sx + k];
     }
     build_pair<T, N>(lds + sl * LD, k, A, B);
   }
@@ -340,7 +449,8 @@ __global__ void __launch_bounds__(256) k_x_c2r(Geo g, const C<T>* __restrict__ a
   const int sl = tid / TP, q = tid % TP;
   C<T>* Zs = lds + sl * LD;
   Fft<T, N, +1, TP, true, true>::run(
-      q, tw, [&](int p) { return Zs[p]; }, [&](int p, C<T> v) { Zs[p] = v; }, [&](int p) -> C<T>& { return Zs[p]; });
+      q, tw, [&](int p, int, int) { return Zs[p]; }, [&](int p, int, int, C<T> v) { Zs[p] = v; },
+      [&](int p) -> C<T>& { return Zs[p]; });
   __syncthreads();
   for (int w = tid; w < NSLOT * rl; w += NTHR) {
     const int sl2 = w / rl, n = w % rl;
@@ -367,7 +477,7 @@ __global__ void __launch_bounds__(256) k_x_r2c(Geo g, const T* __restrict__ real
   const int nrows = g.ny * g.nz;
   const int npairs = (nrows + 1) / 2;
   const int mx = g.mx;
-  const size_t S2 = (size_t)mx * nrows;
+  const size_t S2 = (size_t)g.sx * nrows;
   const int rl = 2 * g.nxh;
   const size_t RS = (size_t)rl * nrows;
   const int p0 = blockIdx.x * NSLOT;
@@ -387,7 +497,8 @@ __global__ void __launch_bounds__(256) k_x_r2c(Geo g, const T* __restrict__ real
   const int sl = tid / TP, q = tid % TP;
   C<T>* Zs = lds + sl * LD;
   Fft<T, N, -1, TP, true, true>::run(
-      q, tw, [&](int p) { return Zs[p]; }, [&](int p, C<T> v) { Zs[p] = v; }, [&](int p) -> C<T>& { return Zs[p]; });
+      q, tw, [&](int p, int, int) { return Zs[p]; }, [&](int p, int, int, C<T> v) { Zs[p] = v; },
+      [&](int p) -> C<T>& { return Zs[p]; });
   __syncthreads();
   for (int w = tid; w < NSLOT * mx; w += NTHR) {
     const int sl2 = w / mx, k = w % mx;
@@ -396,8 +507,8 @@ __global__ void __launch_bounds__(256) k_x_r2c(Geo g, const T* __restrict__ real
     const int f = pidx / npairs, row = 2 * (pidx % npairs);
     C<T> A, B;
     split_pair<T, N>(lds + sl2 * LD, k, A, B);
-    b2[f * S2 + (size_t)row * mx + k] = A;
-    if (row + 1 < nrows) b2[f * S2 + (size_t)(row + 1) * mx + k] = B;
+    b2[f * S2 + (size_t)row * g.sx + k] = A;
+    if (row + 1 < nrows) b2[f * S2 + (size_t)(row + 1) * g.sx + k] = B;
   }
 }
 
@@ -406,22 +517,39 @@ static size_t x_lds_bytes() {
   return (size_t)3 * xrows<N>() * (N + 1) * sizeof(C<T>);
 }
 
+template <typename T, int N, bool KE>
+static void run_x_nl(const Geo& g, const void* a2, void* b2, unsigned long long* umax, const void* tw, hipStream_t s) {
+  const int nrows = g.ny * g.nz;
+  if constexpr (xreg_ok<N>()) {
+    constexpr int TP = N / xreg_ppt<N>();
+    constexpr int WPB = 2;  // wavefronts per block
+    constexpr int RPB = WPB * (64 / TP);
+    const size_t lds = (size_t)RPB * (N + 1) * sizeof(C<T>);
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute((const void*)&k_x_nl_reg<T, N, KE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+      attr = true;
+    }
+    k_x_nl_reg<T, N, KE><<<(nrows + RPB - 1) / RPB, 64 * WPB, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
+                                                                      (const C<T>*)tw, umax);
+  } else {
+    constexpr int R = xrows<N>();
+    constexpr int TP = tper<N>();
+    k_x_nl<T, N, KE><<<(nrows + R - 1) / R, 3 * R * TP, x_lds_bytes<T, N>(), s>>>(g, (const C<T>*)a2, (C<T>*)b2,
+                                                                                (const C<T>*)tw, umax);
+  }
+}
+
 template <typename T>
 void launch_x_nl(const Geo& g, const void* a2, void* b2, bool with_ke, unsigned long long* umax, const void* tw,
                  hipStream_t s) {
-  const int nrows = g.ny * g.nz;
   switch (g.nrx) {
-#define X(n)                                                                                              \
-  case n: {                                                                                               \
-    constexpr int R = xrows<n>();                                                                         \
-    const size_t lds = x_lds_bytes<T, n>();                                                               \
-    dim3 grid((nrows + R - 1) / R);                                                                       \
-    if (with_ke)                                                                                          \
-      k_x_nl<T, n, true><<<grid, 3 * R * tper<n>(), lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2, (const C<T>*)tw, umax); \
-    else                                                                                                  \
-      k_x_nl<T, n, false><<<grid, 3 * R * tper<n>(), lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2, (const C<T>*)tw, umax); \
-    break;                                                                                                \
-  }
+#define X(n)                                                    \
+  case n:                                                       \
+    if (with_ke) run_x_nl<T, n, true>(g, a2, b2, umax, tw, s);  \
+    else run_x_nl<T, n, false>(g, a2, b2, umax, tw, s);         \
+    break;
     FFS_SIZES(X)
 #undef X
     default: bad_size("nonlinear x pass", g.nrx);
@@ -472,12 +600,13 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_update_ab3(Geo g, C<T>* __restrict__ u, C<T>* __restrict__ rhs,
                                                    const C<T>* __restrict__ fm1, const C<T>* __restrict__ fm2,
                                                    const DevState* __restrict__ st, UpdateArgs a) {
-  const size_t n = (size_t)g.mx * g.my * g.mz;
+  const size_t n = (size_t)g.sx * g.my * g.mz;
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n) return;
-  const int cx = idx % g.mx;
-  const int cy = (idx / g.mx) % g.my;
-  const int cz = idx / ((size_t)g.mx * g.my);
+  const int cx = idx % g.sx;
+  if (cx >= g.mx) return;  // row padding
+  const int cy = (idx / g.sx) % g.my;
+  const int cz = idx / ((size_t)g.sx * g.my);
   const double kv3[3] = {g.kx[cx], g.ky[cy], g.kz[cz]};
   const double K2 = kv3[0] * kv3[0] + kv3[1] * kv3[1] + kv3[2] * kv3[2];
   const double kvis = K2 * (-a.nu);
@@ -536,19 +665,20 @@ __global__ void __launch_bounds__(256) k_update_ab3(Geo g, C<T>* __restrict__ u,
 template <typename T>
 void launch_update_ab3(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
                        const UpdateArgs& a, hipStream_t s) {
-  const size_t n = (size_t)g.mx * g.my * g.mz;
+  const size_t n = (size_t)g.sx * g.my * g.mz;
   k_update_ab3<T><<<(n + 255) / 256, 256, 0, s>>>(g, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1, (const C<T>*)fm2, st, a);
 }
 
 // Leray projection of a spectral vector field (pressure_projection!, models.jl:443-456)
 template <typename T>
 __global__ void k_project(Geo g, C<T>* __restrict__ u) {
-  const size_t n = (size_t)g.mx * g.my * g.mz;
+  const size_t n = (size_t)g.sx * g.my * g.mz;
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n || idx == 0) return;
-  const int cx = idx % g.mx;
-  const int cy = (idx / g.mx) % g.my;
-  const int cz = idx / ((size_t)g.mx * g.my);
+  const int cx = idx % g.sx;
+  if (cx >= g.mx) return;
+  const int cy = (idx / g.sx) % g.my;
+  const int cz = idx / ((size_t)g.sx * g.my);
   const double kx = g.kx[cx], ky = g.ky[cy], kz = g.kz[cz];
   const double K2 = kx * kx + ky * ky + kz * kz;
   const C<T> v0 = u[idx], v1 = u[n + idx], v2 = u[2 * n + idx];
@@ -567,7 +697,7 @@ __global__ void k_project(Geo g, C<T>* __restrict__ u) {
 
 template <typename T>
 void launch_project(const Geo& g, void* u, hipStream_t s) {
-  const size_t n = (size_t)g.mx * g.my * g.mz;
+  const size_t n = (size_t)g.sx * g.my * g.mz;
   k_project<T><<<(n + 255) / 256, 256, 0, s>>>(g, (C<T>*)u);
 }
 
@@ -589,14 +719,15 @@ __device__ __forceinline__ double proj2(double ax, double ay, double bx, double 
 template <typename T>
 __global__ void __launch_bounds__(256) k_stats(Geo g, const C<T>* __restrict__ u, const C<T>* __restrict__ nl,
                                               const C<T>* __restrict__ ke, double* __restrict__ partials) {
-  const size_t n = (size_t)g.mx * g.my * g.mz;
+  const size_t n = (size_t)g.sx * g.my * g.mz;
   double acc[kStatsN];
 #pragma unroll
   for (int i = 0; i < kStatsN; ++i) acc[i] = 0.0;
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
-    const int cx = idx % g.mx;
-    const int cy = (idx / g.mx) % g.my;
-    const int cz = idx / ((size_t)g.mx * g.my);
+    const int cx = idx % g.sx;
+    if (cx >= g.mx) continue;
+    const int cy = (idx / g.sx) % g.my;
+    const int cz = idx / ((size_t)g.sx * g.my);
     const double kx = g.kx[cx], ky = g.ky[cy], kz = g.kz[cz];
     const double w = cx > 0 ? 2.0 : 1.0;
     const double K2 = kx * kx + ky * ky + kz * kz;
@@ -674,7 +805,7 @@ __global__ void k_reduce_partials(const double* __restrict__ partials, int nblk,
 template <typename T>
 int launch_stats(const Geo& g, const void* u, const void* nl, const void* ke, double* partials, double* out,
                  hipStream_t s) {
-  const size_t n = (size_t)g.mx * g.my * g.mz;
+  const size_t n = (size_t)g.sx * g.my * g.mz;
   int nblk = (int)((n + 255) / 256);
   if (nblk > kStatsBlocks) nblk = kStatsBlocks;
   k_stats<T><<<nblk, 256, 0, s>>>(g, (const C<T>*)u, (const C<T>*)nl, (const C<T>*)ke, partials);
@@ -698,14 +829,14 @@ __global__ void k_spec_planes(Geo g, const C<T>* __restrict__ u, double* __restr
   // one block per z plane; layouts: px[(cz*2 + hv)*mx + cx], py[(cz*2+hv)*my + cy] (cx>0 part),
   // y0 values are read directly by the second kernel; pz[cz*4 + {h0,v0,h,v}]
   const int cz = blockIdx.x;
-  const size_t n = (size_t)g.mx * g.my * g.mz;
+  const size_t n = (size_t)g.sx * g.my * g.mz;
   __shared__ double sh[4][256];
   double z0h = 0, z0v = 0, zh = 0, zv = 0;
   // X partials: each thread owns cx values, loops over cy
   for (int cx = threadIdx.x; cx < g.mx; cx += blockDim.x) {
     double sh_ = 0, sv_ = 0;
     for (int cy = 0; cy < g.my; ++cy) {
-      const size_t idx = cx + (size_t)g.mx * (cy + (size_t)g.my * cz);
+      const size_t idx = cx + (size_t)g.sx * (cy + (size_t)g.my * cz);
       const C<T> a0 = u[idx], a1 = u[n + idx], a2 = u[2 * n + idx];
       const double h = ((double)a0.x * a0.x + (double)a0.y * a0.y + (double)a1.x * a1.x + (double)a1.y * a1.y) / 2;
       const double v = ((double)a2.x * a2.x + (double)a2.y * a2.y) / 2;
@@ -719,7 +850,7 @@ __global__ void k_spec_planes(Geo g, const C<T>* __restrict__ u, double* __restr
   for (int cy = threadIdx.x; cy < g.my; cy += blockDim.x) {
     double sh_ = 0, sv_ = 0;
     for (int cx = 0; cx < g.mx; ++cx) {
-      const size_t idx = cx + (size_t)g.mx * (cy + (size_t)g.my * cz);
+      const size_t idx = cx + (size_t)g.sx * (cy + (size_t)g.my * cz);
       const C<T> a0 = u[idx], a1 = u[n + idx], a2 = u[2 * n + idx];
       const double h = ((double)a0.x * a0.x + (double)a0.y * a0.y + (double)a1.x * a1.x + (double)a1.y * a1.y) / 2;
       const double v = ((double)a2.x * a2.x + (double)a2.y * a2.y) / 2;
@@ -754,7 +885,7 @@ __global__ void k_spec_finish(Geo g, const C<T>* __restrict__ u, const double* _
                               const double* __restrict__ py, double* __restrict__ ox, double* __restrict__ oy0,
                               double* __restrict__ oy) {
   // ox[hv*mx + cx] = sum_cz px ; oy[hv*my + cy] = sum_cz py ; oy0[hv*my+cy] = sum_cz e(0,cy,cz)
-  const size_t n = (size_t)g.mx * g.my * g.mz;
+  const size_t n = (size_t)g.sx * g.my * g.mz;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < 2 * g.mx) {
     const int hv = t / g.mx, cx = t % g.mx;
@@ -767,7 +898,7 @@ __global__ void k_spec_finish(Geo g, const C<T>* __restrict__ u, const double* _
     double a = 0, a0 = 0;
     for (int cz = 0; cz < g.mz; ++cz) {
       a += py[((size_t)cz * 2 + hv) * g.my + cy];
-      const size_t idx = (size_t)g.mx * (cy + (size_t)g.my * cz);
+      const size_t idx = (size_t)g.sx * (cy + (size_t)g.my * cz);
       double e;
       if (hv == 0) {
         const C<T> a0v = u[idx], a1v = u[n + idx];
@@ -803,11 +934,11 @@ void launch_spectrum(const Geo& g, const void* u, double* out_x, double* out_y0,
 template <typename T>
 __global__ void k_forcing_sum(Geo g, const C<T>* __restrict__ u, const int* __restrict__ band, int nband,
                               DevState* st, double power) {
-  const size_t n = (size_t)g.mx * g.my * g.mz;
+  const size_t n = (size_t)g.sx * g.my * g.mz;
   double a = 0;
   for (int b = threadIdx.x; b < nband; b += blockDim.x) {
     const int idx = band[b];
-    const double w = (idx % g.mx) > 0 ? 2.0 : 1.0;
+    const double w = (idx % g.sx) > 0 ? 2.0 : 1.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const C<T> v = u[c * n + idx];

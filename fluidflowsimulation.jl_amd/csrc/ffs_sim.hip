@@ -240,6 +240,8 @@ struct ffs_sim {
     g.mx = mx;
     g.my = (int)ymap.size();
     g.mz = (int)zmap.size();
+    const int line = fp32 ? 16 : 8;  // complex values per 128-byte line
+    g.sx = (mx + line - 1) / line * line;
     g.ymap = upload(ymap);
     g.zmap = upload(zmap);
     g.yinv = upload(yinv);
@@ -248,7 +250,7 @@ struct ffs_sim {
     g.ky = upload(kyc);
     g.kz = upload(kzc);
     scale = 1.0 / ((double)nrx * ny * nz);
-    S = (size_t)g.mx * g.my * g.mz;
+    S = (size_t)g.sx * g.my * g.mz;
   }
 
   template <typename T>
@@ -268,8 +270,8 @@ struct ffs_sim {
     HIPCHECK(hipMemset(fm1, 0, 3 * S * csz));
     HIPCHECK(hipMemset(fm2, 0, 3 * S * csz));
     HIPCHECK(hipMemset(ke, 0, S * csz));
-    const size_t a1 = 6 * (size_t)g.mx * g.my * g.nz, b2 = 4 * (size_t)g.mx * g.ny * g.nz;
-    const size_t a2 = 6 * (size_t)g.mx * g.ny * g.nz, b1 = 4 * (size_t)g.mx * g.my * g.nz;
+    const size_t a1 = 6 * (size_t)g.sx * g.my * g.nz, b2 = 4 * (size_t)g.sx * g.ny * g.nz;
+    const size_t a2 = 6 * (size_t)g.sx * g.ny * g.nz, b1 = 4 * (size_t)g.sx * g.my * g.nz;
     w1 = dalloc(std::max(a1, b2) * csz);
     w2 = dalloc(std::max(a2, b1) * csz);
     if (fp32) {
@@ -294,7 +296,7 @@ struct ffs_sim {
         for (int cy = 0; cy < g.my; ++cy)
           for (int cx = 0; cx < g.mx; ++cx) {
             const double K2 = kxc[cx] * kxc[cx] + kyc[cy] * kyc[cy] + kzc[cz] * kzc[cz];
-            if (K2 > 0 && K2 <= kf2) b.push_back(cx + g.mx * (cy + g.my * cz));
+            if (K2 > 0 && K2 <= kf2) b.push_back(cx + g.sx * (cy + g.my * cz));
           }
       nband = (int)b.size();
       if (nband) band = upload(b);
@@ -671,7 +673,7 @@ int ffs_set_field(ffs_sim* s, int field, int space, const double* host, size_t n
         for (int cy = 0; cy < s->g.my; ++cy)
           for (int cx = 0; cx < s->g.mx; ++cx) {
             const size_t src = cx + nxh * (s->ymap[cy] + ny * (size_t)s->zmap[cz]);
-            const size_t d = cx + (size_t)s->g.mx * (cy + (size_t)s->g.my * cz);
+            const size_t d = cx + (size_t)s->g.sx * (cy + (size_t)s->g.my * cz);
             c[2 * d] = host[2 * src];
             c[2 * d + 1] = host[2 * src + 1];
           }
@@ -711,7 +713,7 @@ int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n) {
         for (int cy = 0; cy < s->g.my; ++cy)
           for (int cx = 0; cx < s->g.mx; ++cx) {
             const size_t dsti = cx + nxh * (s->ymap[cy] + ny * (size_t)s->zmap[cz]);
-            const size_t d = cx + (size_t)s->g.mx * (cy + (size_t)s->g.my * cz);
+            const size_t d = cx + (size_t)s->g.sx * (cy + (size_t)s->g.my * cz);
             host[2 * dsti] = c[2 * d];
             host[2 * dsti + 1] = c[2 * d + 1];
           }

@@ -184,45 +184,44 @@ template <int R, int S, typename T> __device__ __forceinline__ void dftR(C<T>* a
 }
 
 // ---------------------------------------------------------------------------
-// Radix plans.  N = 2^a 3^b 5^c; powers of two are grouped into radix-16
-// stages (a remainder of one becomes 8*4 in place of 16*2), then 3s and 5s.
+// Radix plans.  N = 2^a 3^b 5^c; powers of two are grouped into radix-MAXR
+// stages (MAXR = 16 or 8; a remainder of one turns a MAXR*2 into (MAXR/2)*4),
+// then 5s and 3s.
 struct Plan {
   int n;
   int ns;  // number of stages
-  int r[8];
+  int r[10];
 };
 
-__host__ __device__ constexpr Plan make_plan(int N) {
-  Plan p{N, 0, {0, 0, 0, 0, 0, 0, 0, 0}};
+__host__ __device__ constexpr Plan make_plan(int N, int maxr = 16) {
+  Plan p{N, 0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  int L = 0;
+  while ((1 << (L + 1)) <= maxr) ++L;  // log2(maxr)
   int m = N, a = 0;
   while (m % 2 == 0) { m /= 2; ++a; }
-  int n16 = a / 4, rem = a % 4;
-  if (rem == 1 && n16 > 0) {
-    --n16;
-    p.r[p.ns++] = 8;
+  int nbig = a / L, rem = a % L;
+  if (rem == 1 && nbig > 0) {
+    --nbig;
+    p.r[p.ns++] = 1 << (L - 1);
     p.r[p.ns++] = 4;
-  } else if (rem == 1) {
-    p.r[p.ns++] = 2;
-  } else if (rem == 2) {
-    p.r[p.ns++] = 4;
-  } else if (rem == 3) {
-    p.r[p.ns++] = 8;
+  } else if (rem > 0) {
+    p.r[p.ns++] = 1 << rem;
   }
-  for (int i = 0; i < n16; ++i) p.r[p.ns++] = 16;
+  for (int i = 0; i < nbig; ++i) p.r[p.ns++] = 1 << L;
   while (m % 5 == 0) { m /= 5; p.r[p.ns++] = 5; }
   while (m % 3 == 0) { m /= 3; p.r[p.ns++] = 3; }
   if (m != 1) p.ns = -1;  // unsupported length
   return p;
 }
 
-__host__ __device__ constexpr int plan_ns(int N) { return make_plan(N).ns; }
-__host__ __device__ constexpr int plan_r(int N, int s) { return make_plan(N).r[s]; }
-__host__ __device__ constexpr int plan_nsprev(int N, int s) {
+__host__ __device__ constexpr int plan_ns(int N, int maxr = 16) { return make_plan(N, maxr).ns; }
+__host__ __device__ constexpr int plan_r(int N, int s, int maxr = 16) { return make_plan(N, maxr).r[s]; }
+__host__ __device__ constexpr int plan_nsprev(int N, int s, int maxr = 16) {
   int v = 1;
-  for (int i = 0; i < s; ++i) v *= make_plan(N).r[i];
+  for (int i = 0; i < s; ++i) v *= make_plan(N, maxr).r[i];
   return v;
 }
-// points handled per thread: 16 if it divides N, else 8, 4, ... (always >= the largest radix of
+// points handled per thread: 16 if it divides N, else 8, 12, ... (always >= the largest radix of
 // the plan's mixed part, so every stage fits in registers)
 __host__ __device__ constexpr int default_ppt(int N) {
   return (N % 16 == 0) ? 16 : (N % 8 == 0) ? 8 : (N % 12 == 0) ? 12 : (N % 6 == 0) ? 6 : (N % 5 == 0) ? 5 : (N % 4 == 0) ? 4 : (N % 3 == 0) ? 3 : 2;
@@ -233,11 +232,13 @@ __host__ __device__ constexpr int default_ppt(int N) {
 //   stage s: radix R, Ns = product of the previous radices
 //   butterfly b: inputs x[b + r N/R], twiddle W_{Ns R}^{r (b mod Ns)}, outputs
 //   y[(b/Ns) Ns R + (b mod Ns) + r Ns]
-// Load/Store functors take the position along the transform axis.
-template <typename T, int N, int DIR, int STAGE, int TPER>
+// Load/Store functors receive (position, t, r): t indexes the thread's
+// butterflies, r the radix leg; both are compile-time after unrolling, which
+// lets register-resident callers index register arrays with them.
+template <typename T, int N, int DIR, int STAGE, int TPER, int MAXR = 16>
 struct Stage {
-  static constexpr int R = plan_r(N, STAGE);
-  static constexpr int NS = plan_nsprev(N, STAGE);
+  static constexpr int R = plan_r(N, STAGE, MAXR);
+  static constexpr int NS = plan_nsprev(N, STAGE, MAXR);
   static constexpr int NBF = N / R;                       // butterflies
   static constexpr int NB = (NBF + TPER - 1) / TPER;      // per thread
 
@@ -248,7 +249,7 @@ struct Stage {
       const int b = q + t * TPER;
       if (NB * TPER == NBF || b < NBF) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[t][r] = ld(b + r * NBF);
+        for (int r = 0; r < R; ++r) v[t][r] = ld(b + r * NBF, t, r);
       }
     }
   }
@@ -279,11 +280,19 @@ struct Stage {
       if (NB * TPER == NBF || b < NBF) {
         const int base = (b / NS) * NS * R + (b % NS);
 #pragma unroll
-        for (int r = 0; r < R; ++r) st(base + r * NS, v[t][r]);
+        for (int r = 0; r < R; ++r) st(base + r * NS, t, r, v[t][r]);
       }
     }
   }
 };
+
+// Exchange synchronisation: the whole block, or only the wavefront (when all
+// TPER threads of a transform live in one wavefront).
+enum { SYNC_BLOCK = 0, SYNC_WAVE = 1 };
+template <int SYNC> __device__ __forceinline__ void fft_sync() {
+  if constexpr (SYNC == SYNC_BLOCK) __syncthreads();
+  else __syncwarp();
+}
 
 // Runs stages [STAGE, ns) with stage STAGE reading through `ld` and the last
 // stage writing through `st`; intermediate exchanges go through LDS via
@@ -292,32 +301,43 @@ struct Stage {
 //            every thread must finish its first loads before anyone stores.
 //   OUT_LDS: `st` writes the same LDS region, so the last stage must wait for
 //            every thread's loads before storing.
-// The caller issues a __syncthreads() before the call if the LDS region was
-// in use by a previous transform, and after it before reading OUT_LDS results.
-// Every thread of the block must call run() (barriers are block wide).
-template <typename T, int N, int DIR, int TPER, bool IN_LDS = false, bool OUT_LDS = false, int STAGE = 0>
+// The caller synchronises before the call if the LDS region was in use by a
+// previous transform, and after it before reading OUT_LDS results.  Every
+// thread of the synchronisation scope must call run().
+template <typename T, int N, int DIR, int TPER, bool IN_LDS = false, bool OUT_LDS = false, int SYNC = SYNC_BLOCK,
+          int MAXR = 16, int STAGE = 0>
 struct Fft {
-  static constexpr int NSTG = plan_ns(N);
+  static constexpr int NSTG = plan_ns(N, MAXR);
   static_assert(NSTG > 0, "FFT length must be 2^a 3^b 5^c");
 
   template <class Load, class Store, class Lds>
   static __device__ __forceinline__ void run(int q, const C<T>* __restrict__ tw, const Load& ld, const Store& st,
                                              const Lds& lds) {
-    using S = Stage<T, N, DIR, STAGE, TPER>;
+    using S = Stage<T, N, DIR, STAGE, TPER, MAXR>;
     C<T> v[S::NB][S::R];
     S::load(v, q, ld);
     S::compute(v, q, tw);
     constexpr bool reads_lds = (STAGE > 0) || IN_LDS;
     if constexpr (STAGE + 1 == NSTG) {
-      if constexpr (OUT_LDS && reads_lds) __syncthreads();
+      if constexpr (OUT_LDS && reads_lds) fft_sync<SYNC>();
       S::store(v, q, st);
     } else {
-      if constexpr (reads_lds) __syncthreads();  // everyone has read the previous contents
-      S::store(v, q, [&](int p, C<T> x) { lds(p) = x; });
-      __syncthreads();
-      Fft<T, N, DIR, TPER, IN_LDS, OUT_LDS, STAGE + 1>::run(q, tw, [&](int p) { return lds(p); }, st, lds);
+      if constexpr (reads_lds) fft_sync<SYNC>();  // everyone has read the previous contents
+      S::store(v, q, [&](int p, int, int, C<T> x) { lds(p) = x; });
+      fft_sync<SYNC>();
+      Fft<T, N, DIR, TPER, IN_LDS, OUT_LDS, SYNC, MAXR, STAGE + 1>::run(
+          q, tw, [&](int p, int, int) { return lds(p); }, st, lds);
     }
   }
 };
+
+// position sets of the first and the last stage coincide ({p = q mod TPER})
+// when every stage's butterflies divide evenly over the TPER threads; register
+// resident callers rely on it and check it with this predicate.
+__host__ __device__ constexpr bool plan_even(int N, int tper, int maxr = 16) {
+  for (int s = 0; s < plan_ns(N, maxr); ++s)
+    if ((N / plan_r(N, s, maxr)) % tper != 0) return false;
+  return true;
+}
 
 }  // namespace fsk
