@@ -32,6 +32,7 @@ template <int N> __host__ __device__ constexpr int xrows() { return (64 / tper<N
 
 // points per lane of the register-resident x pass and whether it applies
 template <int N> __host__ __device__ constexpr int xreg_ppt() { return 8; }
+template <int N> __host__ __device__ constexpr int xreg_ldx() { return N + N / 8 + 1; }
 template <int N> __host__ __device__ constexpr bool xreg_ok() {
   return N % 8 == 0 && 64 % (N / 8) == 0 && plan_even(N, N / 8, 8);
 }
@@ -337,7 +338,7 @@ __global__ void __launch_bounds__(128) k_x_nl_reg(Geo g, const C<T>* __restrict_
   constexpr int PPT = xreg_ppt<N>();
   constexpr int TP = N / PPT;
   constexpr int GPW = 64 / TP;  // rows per wavefront
-  constexpr int LDX = N + 1;
+  constexpr int LDX = xreg_ldx<N>();
   constexpr int NSTG = plan_ns(N, PPT);
   constexpr int M0 = (N / plan_r(N, 0, PPT)) / TP;         // register stride of first-stage legs
   constexpr int ML = (N / plan_r(N, NSTG - 1, PPT)) / TP;  // register stride of last-stage legs
@@ -353,7 +354,8 @@ __global__ void __launch_bounds__(128) k_x_nl_reg(Geo g, const C<T>* __restrict_
   const size_t S2 = (size_t)g.sx * nrows;
   const size_t roff = (size_t)(row_ok ? row : 0) * g.sx;
   C<T>* X = lds + (size_t)(wave * GPW + grp) * LDX;
-  auto xref = [&](int p) -> C<T>& { return X[p]; };
+  // one padding slot per 8 values: the radix-4/8 Stockham exchange becomes bank-conflict free
+  auto xref = [&](int p) -> C<T>& { return X[p + (p >> 3)]; };
 
   C<T> R[3][PPT];
   // inverse transforms of the pairs (ux,uy) (uz,wx) (wy,wz); Z built from the half spectra on load
@@ -399,13 +401,13 @@ __global__ void __launch_bounds__(128) k_x_nl_reg(Geo g, const C<T>* __restrict_
     Fft<T, N, -1, TP, false, false, SYNC_WAVE, PPT>::run(q, tw, ld, st, xref);
     __syncwarp();
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) X[q + TP * m] = R[j][m];
+    for (int m = 0; m < PPT; ++m) xref(q + TP * m) = R[j][m];
     __syncwarp();
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
       const int k = q + TP * m;
       if (row_ok && k < mx) {
-        const C<T> zk = R[j][m], zn = X[(N - k) % N];
+        const C<T> zk = R[j][m], zn = xref((N - k) % N);
         b2[(size_t)(2 * j) * S2 + roff + k] = mk<T>((T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y));
         if (j == 0 || KE) b2[(size_t)(2 * j + 1) * S2 + roff + k] = mk<T>((T)0.5 * (zk.y + zn.y), (T)0.5 * (zn.x - zk.x));
       }
@@ -524,7 +526,7 @@ static void run_x_nl(const Geo& g, const void* a2, void* b2, unsigned long long*
     constexpr int TP = N / xreg_ppt<N>();
     constexpr int WPB = 2;  // wavefronts per block
     constexpr int RPB = WPB * (64 / TP);
-    const size_t lds = (size_t)RPB * (N + 1) * sizeof(C<T>);
+    const size_t lds = (size_t)RPB * xreg_ldx<N>() * sizeof(C<T>);
     static bool attr = false;
     if (!attr) {
       (void)hipFuncSetAttribute((const void*)&k_x_nl_reg<T, N, KE>, hipFuncAttributeMaxDynamicSharedMemorySize,

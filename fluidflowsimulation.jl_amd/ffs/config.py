@@ -50,6 +50,8 @@ def parse_expr(s):
 
     def take():
         nonlocal pos
+        if pos >= len(toks):
+            raise ValueError(f"incomplete expression: {s!r}")
         t = toks[pos]
         pos += 1
         return t

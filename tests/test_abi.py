@@ -1,0 +1,89 @@
+"""The C ABI: libffs_hip.so loads without a GPU, exports every symbol that
+include/ffs/ffs.h declares, and the ctypes mirror of ffs_params matches the
+C layout (checked against a gcc-compiled probe of the header)."""
+import ctypes
+import os
+import re
+import subprocess
+
+from conftest import REPO
+
+HEADER = os.path.join(REPO, "include", "ffs", "ffs.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(ffs_\w+)\s*\(", src, flags=re.M)))
+
+
+def test_header_declares_the_api():
+    names = declared_functions()
+    for n in ("ffs_create", "ffs_set_field", "ffs_get_field", "ffs_initialize", "ffs_advance", "ffs_stats_pop",
+              "ffs_spectrum", "ffs_get_scalar", "ffs_last_error", "ffs_destroy"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol(lib):
+    for n in declared_functions():
+        assert hasattr(lib, n), n
+
+
+def test_ctypes_signatures_cover_header(lib):
+    from ffs import _lib
+
+    assert set(_lib.SIGNATURES) == set(declared_functions())
+
+
+def test_params_layout_matches_c(tmp_path):
+    from ffs._lib import FfsParams
+
+    fields = [f[0] for f in FfsParams._fields_]
+    probe = tmp_path / "probe.c"
+    body = "\n".join(f'  printf("{f} %zu\\n", offsetof(ffs_params, {f}));' for f in fields)
+    probe.write_text(f"""#include <stdio.h>
+#include <stddef.h>
+#include "ffs/ffs.h"
+int main(void) {{
+  printf("__sizeof %zu\\n", sizeof(ffs_params));
+{body}
+  return 0;
+}}
+""")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c99", f"-I{os.path.join(REPO, 'include')}", str(probe), "-o", str(exe)], check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    assert int(out["__sizeof"]) == ctypes.sizeof(FfsParams)
+    for f in fields:
+        assert int(out[f]) == getattr(FfsParams, f).offset, f
+
+
+def test_defaults_and_version(lib):
+    from ffs import _lib
+
+    assert lib.ffs_abi_version() == 1
+    p = _lib.default_params()
+    assert p.size == ctypes.sizeof(_lib.FfsParams)
+    assert abs(p.cutoff - 2 / 3) < 1e-16 and p.cfl == 0.18 and p.integrator == _lib.FFS_ETD3
+    assert p.variable_dt == 1 and p.smagorinsky_c == 0.17
+
+
+def test_null_arguments_are_rejected(lib):
+    assert lib.ffs_create(None, None) == -1
+    assert lib.ffs_initialize(None) == -1
+    assert lib.ffs_advance(None, 1, None, None) == -1
+    lib.ffs_destroy(None)  # no-op
+
+
+def test_bad_params_fail_cleanly_without_device(lib):
+    """Validation happens before any device call, so it runs here too."""
+    from ffs import _lib
+
+    p = _lib.default_params()
+    p.nx = p.ny = p.nz = 16
+    p.size = 12  # wrong ABI size
+    h = ctypes.c_void_p()
+    rc = lib.ffs_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc == -1 and not h.value
+    assert b"size" in lib.ffs_last_error(None)

@@ -1,0 +1,100 @@
+"""The `global` file: parsing, expression values, defaults and required keys
+(simulationtypes.jl:242-365, Globals.jl:106-182; format of
+test/Scalar_Advection/global)."""
+import math
+
+import numpy as np
+import pytest
+
+from ffs import config, fields
+
+# the key/value content of the reference's example case (test/Scalar_Advection/global)
+EXAMPLE = """// Set physical parameters
+kinematicViscosity 0.03333333
+
+// Set domain parameters
+xDomainSize 1.0
+yDomainSize 1.0
+zDomainSize 1.0
+nx 128
+ny 128
+nz 128
+
+passiveScalar true
+scalarGradient 0.0
+scalarPr Inf
+
+// Set simulation control variables
+dtStat 125
+nt 500
+dt 0.002
+variableTimestepFlag false
+velocityTimeStep Adams_Bashforth3rdO
+dealias sphere
+threaded true
+
+// Times to write output files
+writeTime 125
+"""
+
+
+def test_readglobal_example(tmp_path):
+    f = tmp_path / "global"
+    f.write_text(EXAMPLE)
+    d = config.readglobal(str(f))
+    assert d["nx"] == "128" and d["kinematicViscosity"] == "0.03333333" and d["scalarPr"] == "Inf"
+    assert d["velocityTimeStep"] == "Adams_Bashforth3rdO" and not any(k.startswith("//") for k in d)
+
+
+def test_missing_required_key_raises_keyerror(tmp_path):
+    """dtSpec is required (simulationtypes.jl:262) and the example omits it."""
+    f = tmp_path / "global"
+    f.write_text(EXAMPLE)
+    with pytest.raises(KeyError, match="dtSpec"):
+        config.parameters(config.readglobal(str(f)))
+
+
+def test_parameters_defaults_and_values(tmp_path):
+    f = tmp_path / "global"
+    f.write_text(EXAMPLE + "dtSpec 1000000\n")
+    p = config.parameters(config.readglobal(str(f)))
+    assert (p.nx, p.ny, p.nz) == (128, 128, 128)
+    assert p.nu == 0.03333333 and p.dt == 0.002 and p.dt_stat == 125 and p.write_time == 125 and p.nt == 500
+    assert p.integrator == "Adams_Bashforth3rdO" and p.variable_dt is False
+    assert p.cfl == 0.18 and abs(p.cutoff - 2 / 3) < 1e-16  # defaults
+    assert "passiveScalar" in p.extra and "dealias" in p.extra  # read nowhere by the velocity path
+
+
+@pytest.mark.parametrize("s,v", [("2π", 2 * math.pi), ("1/3", 1 / 3), ("2^-4", 2**-4), ("0.5e-3", 5e-4),
+                                 ("(1+2)*3", 9.0), ("-2", -2.0), ("Inf", math.inf), ("2pi/3", 2 * math.pi / 3),
+                                 ("3(1+1)", 6.0), ("1.0", 1.0)])
+def test_parse_expr(s, v):
+    assert config.parse_expr(s) == pytest.approx(v, rel=1e-15)
+
+
+def test_parse_expr_rejects_garbage():
+    for bad in ("", "2+", "(1", "abc"):
+        with pytest.raises(ValueError):
+            config.parse_expr(bad)
+
+
+def test_field_files_padded_and_unpadded(tmp_path):
+    nx, ny, nz = 6, 4, 5
+    a = np.random.default_rng(0).standard_normal((nx, ny, nz))
+    p = fields.pad(a)
+    assert p.shape == (8, 4, 5) and np.all(p[6:] == 0)
+    fields.write_field(str(tmp_path / "u1.0"), p)
+    assert (tmp_path / "u1.0").stat().st_size == 8 * 8 * 4 * 5  # 8 * 2(nx/2+1) ny nz bytes
+    back = fields.read_field(str(tmp_path / "u1.0"), nx, ny, nz)
+    assert np.array_equal(back[:nx], a)
+    a.ravel(order="F").astype("<f8").tofile(tmp_path / "u2.0")  # unpadded
+    assert np.array_equal(fields.read_field(str(tmp_path / "u2.0"), nx, ny, nz)[:nx], a)
+    (tmp_path / "u3.0").write_bytes(b"\0" * 16)
+    with pytest.raises(ValueError):
+        fields.read_field(str(tmp_path / "u3.0"), nx, ny, nz)
+
+
+def test_padded_file_size_256():
+    """Padded 256^3 field is 135,266,304 bytes; 64^3 is 2,162,688 bytes."""
+    assert 8 * np.prod(fields.padded_shape(256, 256, 256)) == 135266304
+    assert 8 * np.prod(fields.padded_shape(64, 64, 64)) == 2162688
