@@ -30,14 +30,51 @@ template <typename T> __host__ __device__ constexpr int col_tx(int) { return 128
 template <int N> __host__ __device__ constexpr int tper() { return N / default_ppt(N); }
 template <int N> __host__ __device__ constexpr int xrows() { return (64 / tper<N>()) > 0 ? (64 / tper<N>()) : 1; }
 
+// Tunables of the fused x pass: points per lane, minimum waves per SIMD the
+// register allocation must allow, and the LDS padding of the exchange.
+#ifndef FFS_XNL_PPT
+#define FFS_XNL_PPT 8
+#endif
+#ifndef FFS_XNL_WAVES
+#define FFS_XNL_WAVES 1
+#endif
+#ifndef FFS_XNL_PAD_SHIFT
+#define FFS_XNL_PAD_SHIFT 0  // 0: no padding; s > 0: one slot per 2^s values
+#endif
+#ifndef FFS_COL_WAVES
+#define FFS_COL_WAVES 1
+#endif
+
 // points per lane of the register-resident x pass and whether it applies
-template <int N> __host__ __device__ constexpr int xreg_ppt() { return 8; }
-template <int N> __host__ __device__ constexpr int xreg_ldx() { return N + N / 8 + 1; }
+template <int N> __host__ __device__ constexpr int xreg_ppt() { return FFS_XNL_PPT; }
+template <int N> __host__ __device__ constexpr int xreg_ldx() {
+  return FFS_XNL_PAD_SHIFT > 0 ? N + (N >> FFS_XNL_PAD_SHIFT) + 1 : N + 1;
+}
 template <int N> __host__ __device__ constexpr bool xreg_ok() {
-  return N % 8 == 0 && 64 % (N / 8) == 0 && plan_even(N, N / 8, 8);
+  constexpr int P = FFS_XNL_PPT;
+  return N % P == 0 && 64 % (N / P) == 0 && plan_even(N, N / P, P);
 }
 
 template <typename T> __device__ __forceinline__ T nanmax(T a, T b) { return (a > b || a != a) ? a : b; }
+
+// Copies the length-N twiddle table into LDS (one copy per block): the FFT
+// stages then read their twiddles with LDS latency instead of an L2 round
+// trip inside the dependent butterfly chain.  The caller synchronises.
+#ifndef FFS_LDS_TW64
+#define FFS_LDS_TW64 0
+#endif
+#ifndef FFS_LDS_TW32
+#define FFS_LDS_TW32 0
+#endif
+template <typename T> __host__ __device__ constexpr bool lds_twiddles() {
+  return sizeof(T) == 8 ? FFS_LDS_TW64 : FFS_LDS_TW32;
+}
+template <typename T, int N>
+__device__ __forceinline__ const C<T>* stage_twiddles(C<T>* __restrict__ twl, const C<T>* __restrict__ tw) {
+  if constexpr (!lds_twiddles<T>()) return tw;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
+  return twl;
+}
 
 // wave64 + cross-wave max, then one atomic per block on the IEEE bit pattern
 // (monotone for non-negative values; NaN sorts above +inf so it propagates).
@@ -61,13 +98,15 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) 
 enum ColKind { INV_Z = 0, INV_Y = 1, FWD_Y = 2, FWD_Z = 3 };
 
 template <typename T, int N, int KIND, bool CURL>
-__global__ void __launch_bounds__(512) k_col(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ out,
-                                            C<T>* __restrict__ out2, const C<T>* __restrict__ tw, int nzero,
+__global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ out,
+                                            C<T>* __restrict__ out2, const C<T>* __restrict__ tw_g, int nzero,
                                             T scale) {
   constexpr int TX = col_tx<T>(N);
   constexpr int TP = tper<N>();
   extern __shared__ unsigned char smem_raw[];
   C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
+  const C<T>* tw = stage_twiddles<T, N>(lds + N * TX, tw_g);
+  __syncthreads();
   const int c = threadIdx.x % TX;
   const int q = threadIdx.x / TX;
   const int cx = blockIdx.x * TX + c;
@@ -159,7 +198,7 @@ static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, con
                     hipStream_t s, void* out2 = nullptr) {
   constexpr int TX = col_tx<T>(N);
   constexpr int TP = tper<N>();
-  const size_t lds = (size_t)N * TX * sizeof(C<T>);
+  const size_t lds = (size_t)(N * TX + N) * sizeof(C<T>);
   static bool attr = false;
   if (!attr) {
     hipFuncSetAttribute((const void*)&k_col<T, N, KIND, CURL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -252,7 +291,7 @@ __device__ __forceinline__ void split_pair(const C<T>* Z, int k, C<T>& A, C<T>& 
 // b2 receives the x-forward transform of (u x w)_{x,y,z} and, when KE, 1/2|u|^2.
 template <typename T, int N, bool KE>
 __global__ void __launch_bounds__(256) k_x_nl(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
-                                             const C<T>* __restrict__ tw, unsigned long long* __restrict__ umax) {
+                                             const C<T>* __restrict__ tw_g, unsigned long long* __restrict__ umax) {
   constexpr int TP = tper<N>();
   constexpr int ROWS = xrows<N>();
   constexpr int NSLOT = 3 * ROWS;
@@ -260,6 +299,7 @@ __global__ void __launch_bounds__(256) k_x_nl(Geo g, const C<T>* __restrict__ a2
   constexpr int NTHR = NSLOT * TP;
   extern __shared__ unsigned char smem_raw[];
   C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
+  const C<T>* tw = stage_twiddles<T, N>(lds + NSLOT * LD, tw_g);  // published by the first __syncthreads below
   const int tid = threadIdx.x;
   const int nrows = g.ny * g.nz;
   const int row0 = blockIdx.x * ROWS;
@@ -333,8 +373,8 @@ __global__ void __launch_bounds__(256) k_x_nl(Geo g, const C<T>* __restrict__ a2
 // store, so LDS carries only the one exchange inside each transform (plus the
 // k <-> N-k pairing at the end), synchronised per wavefront.
 template <typename T, int N, bool KE>
-__global__ void __launch_bounds__(128) k_x_nl_reg(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
-                                                 const C<T>* __restrict__ tw, unsigned long long* __restrict__ umax) {
+__global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
+                                                 const C<T>* __restrict__ tw_g, unsigned long long* __restrict__ umax) {
   constexpr int PPT = xreg_ppt<N>();
   constexpr int TP = N / PPT;
   constexpr int GPW = 64 / TP;  // rows per wavefront
@@ -344,7 +384,10 @@ __global__ void __launch_bounds__(128) k_x_nl_reg(Geo g, const C<T>* __restrict_
   constexpr int ML = (N / plan_r(N, NSTG - 1, PPT)) / TP;  // register stride of last-stage legs
   static_assert(xreg_ok<N>(), "register-resident x pass needs an even plan");
   extern __shared__ unsigned char smem_raw[];
-  C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
+  C<T>* twl = reinterpret_cast<C<T>*>(smem_raw);
+  C<T>* lds = twl + N;
+  const C<T>* tw = stage_twiddles<T, N>(twl, tw_g);
+  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int grp = lane / TP, q = lane % TP;
   const int nrows = g.ny * g.nz;
@@ -354,8 +397,10 @@ __global__ void __launch_bounds__(128) k_x_nl_reg(Geo g, const C<T>* __restrict_
   const size_t S2 = (size_t)g.sx * nrows;
   const size_t roff = (size_t)(row_ok ? row : 0) * g.sx;
   C<T>* X = lds + (size_t)(wave * GPW + grp) * LDX;
-  // one padding slot per 8 values: the radix-4/8 Stockham exchange becomes bank-conflict free
-  auto xref = [&](int p) -> C<T>& { return X[p + (p >> 3)]; };
+  auto xref = [&](int p) -> C<T>& {
+    if constexpr (FFS_XNL_PAD_SHIFT > 0) return X[p + (p >> FFS_XNL_PAD_SHIFT)];
+    else return X[p];
+  };
 
   C<T> R[3][PPT];
   // inverse transforms of the pairs (ux,uy) (uz,wx) (wy,wz); Z built from the half spectra on load
@@ -420,13 +465,14 @@ __global__ void __launch_bounds__(128) k_x_nl_reg(Geo g, const C<T>* __restrict_
 // (2*nxh, ny, nz), padding entries zeroed.
 template <typename T, int N>
 __global__ void __launch_bounds__(256) k_x_c2r(Geo g, const C<T>* __restrict__ a2, T* __restrict__ real,
-                                              const C<T>* __restrict__ tw, int nf) {
+                                              const C<T>* __restrict__ tw_g, int nf) {
   constexpr int TP = tper<N>();
   constexpr int NSLOT = xrows<N>() * 3;
   constexpr int LD = N + 1;
   constexpr int NTHR = NSLOT * TP;
   extern __shared__ unsigned char smem_raw[];
   C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
+  const C<T>* tw = stage_twiddles<T, N>(lds + NSLOT * LD, tw_g);  // published by the first __syncthreads below
   const int tid = threadIdx.x;
   const int nrows = g.ny * g.nz;
   const int npairs = (nrows + 1) / 2;
@@ -468,13 +514,14 @@ __global__ void __launch_bounds__(256) k_x_c2r(Geo g, const C<T>* __restrict__ a
 // R2C of nf padded real fields into (mx, ny, nz) half spectra (k < mx kept).
 template <typename T, int N>
 __global__ void __launch_bounds__(256) k_x_r2c(Geo g, const T* __restrict__ real, C<T>* __restrict__ b2,
-                                              const C<T>* __restrict__ tw, int nf) {
+                                              const C<T>* __restrict__ tw_g, int nf) {
   constexpr int TP = tper<N>();
   constexpr int NSLOT = xrows<N>() * 3;
   constexpr int LD = N + 1;
   constexpr int NTHR = NSLOT * TP;
   extern __shared__ unsigned char smem_raw[];
   C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
+  const C<T>* tw = stage_twiddles<T, N>(lds + NSLOT * LD, tw_g);  // published by the first __syncthreads below
   const int tid = threadIdx.x;
   const int nrows = g.ny * g.nz;
   const int npairs = (nrows + 1) / 2;
@@ -516,7 +563,7 @@ __global__ void __launch_bounds__(256) k_x_r2c(Geo g, const T* __restrict__ real
 
 template <typename T, int N>
 static size_t x_lds_bytes() {
-  return (size_t)3 * xrows<N>() * (N + 1) * sizeof(C<T>);
+  return ((size_t)3 * xrows<N>() * (N + 1) + N) * sizeof(C<T>);
 }
 
 template <typename T, int N, bool KE>
@@ -526,7 +573,7 @@ static void run_x_nl(const Geo& g, const void* a2, void* b2, unsigned long long*
     constexpr int TP = N / xreg_ppt<N>();
     constexpr int WPB = 2;  // wavefronts per block
     constexpr int RPB = WPB * (64 / TP);
-    const size_t lds = (size_t)RPB * xreg_ldx<N>() * sizeof(C<T>);
+    const size_t lds = ((size_t)RPB * xreg_ldx<N>() + N) * sizeof(C<T>);
     static bool attr = false;
     if (!attr) {
       (void)hipFuncSetAttribute((const void*)&k_x_nl_reg<T, N, KE>, hipFuncAttributeMaxDynamicSharedMemorySize,

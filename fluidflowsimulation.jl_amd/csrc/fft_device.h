@@ -235,7 +235,11 @@ __host__ __device__ constexpr int default_ppt(int N) {
 // Load/Store functors receive (position, t, r): t indexes the thread's
 // butterflies, r the radix leg; both are compile-time after unrolling, which
 // lets register-resident callers index register arrays with them.
-template <typename T, int N, int DIR, int STAGE, int TPER, int MAXR = 16>
+#ifndef FFS_TWIDDLE_POWERS
+#define FFS_TWIDDLE_POWERS 1
+#endif
+// TWIDDLE_POWERS: derive W^r (r >= 2) from W^1 instead of reading every leg's twiddle
+template <typename T, int N, int DIR, int STAGE, int TPER, int MAXR = 16, bool TWIDDLE_POWERS = FFS_TWIDDLE_POWERS>
 struct Stage {
   static constexpr int R = plan_r(N, STAGE, MAXR);
   static constexpr int NS = plan_nsprev(N, STAGE, MAXR);
@@ -261,10 +265,21 @@ struct Stage {
       if (NB * TPER == NBF || b < NBF) {
         if constexpr (NS > 1) {
           const int j = b % NS;
+          if constexpr (TWIDDLE_POWERS) {
+            // one table read per butterfly; W^r by multiplication (at most 3 rounding steps deep)
+            const C<T> w1 = tw[j * (N / (NS * R))];
+            C<T> w[R];
+            w[1] = w1;
 #pragma unroll
-          for (int r = 1; r < R; ++r) {
-            const C<T> w = tw[(r * j) * (N / (NS * R))];
-            v[t][r] = (DIR < 0) ? cmul<T>(v[t][r], w) : cmulc<T>(v[t][r], w);
+            for (int r = 2; r < R; ++r) w[r] = (r & 1) ? cmul<T>(w[r - 1], w1) : cmul<T>(w[r / 2], w[r / 2]);
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[t][r] = (DIR < 0) ? cmul<T>(v[t][r], w[r]) : cmulc<T>(v[t][r], w[r]);
+          } else {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+              const C<T> w = tw[(r * j) * (N / (NS * R))];
+              v[t][r] = (DIR < 0) ? cmul<T>(v[t][r], w) : cmulc<T>(v[t][r], w);
+            }
           }
         }
         dftR<R, DIR, T>(v[t]);

@@ -7,7 +7,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libffs_hip.so")
+# FFS_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("FFS_LIB", os.path.join(HERE, "libffs_hip.so"))
 
 FFS_OK = 0
 ERRORS = {

@@ -191,3 +191,41 @@ def test_output_event():
         assert (done, ev) == (4, _lib.FFS_EVENT_OUTPUT) and s.iteration == 8
         done, ev = s.advance(2)
         assert (done, ev) == (2, _lib.FFS_EVENT_NONE) and s.iteration == 10
+
+
+def shell_spectrum(uh, g):
+    """3D shell spectrum E(k) binned by round(|k|/dk) with Parseval weights."""
+    k = np.sqrt(g.K2)
+    dk = g.KX[1]
+    bins = np.rint(k / dk).astype(int)
+    e = 0.5 * g.W * sum(np.abs(c) ** 2 for c in uh)
+    return np.bincount(bins.ravel(), weights=np.where(g.RANGE, e, 0).ravel())
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_fp32_bulk_against_fp64_oracle(n):
+    """FP32 fields/FFTs with FP64 coefficients and reductions: spectrum error <= 1e-5."""
+    print_case(f"fp32 bulk vs fp64 oracle n={n}")
+    u0 = random_spectrum(n, seed=8)
+    g = Grid(n, n, n)
+    ref = OracleSim(g, 0.01, dt=0.002, dt_stat=5)
+    ref.set_velocity(u0)
+    p = ffs.Params(nx=n, ny=n, nz=n, nu=0.01, dt=0.002, integrator="Adams_Bashforth3rdO", variable_dt=False,
+                   dt_stat=5, dt_spec=1 << 30, write_time=1 << 30, precision="fp32")
+    with ffs.Simulation(p) as s:
+        s.set_velocity(*u0)
+        ref.initialize()
+        s.initialize()
+        ref.advance_in_time(20)
+        s.advance_in_time(20)
+        uh = s.velocity_hat()
+        rows = s.pop_stats()
+    err = rel_field_err(uh, ref.uh)
+    assert err < 2e-5, err
+    e32, e64 = shell_spectrum(uh, g), shell_spectrum(ref.uh, g)
+    sel = e64 >= 1e-6 * e64.max()
+    spec_err = np.max(np.abs(e32[sel] - e64[sel]) / e64[sel])
+    print(f"fp32 field err {err:.2e} spectrum err {spec_err:.2e}")
+    assert spec_err <= 1e-5, spec_err
+    for (st, _, _), (ost, _, _) in zip(rows, ref.records):
+        assert abs(st[11] - ost[11]) <= 1e-6 * ost[11] and abs(st[14] - ost[14]) <= 1e-5 * ost[14]
