@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 namespace fsk {
 
@@ -32,7 +33,9 @@ struct DevState {
   unsigned long long umax; // bit pattern of max |u_i| (atomicMax on non-negative doubles)
   unsigned long long rhomax;
   double force_factor;     // isotropic forcing: P / sum |u_lowk|^2
-  double pad[4];
+  double rhomax_d;         // last max|rho|
+  double etd[20];          // ETD3 scalars a0..a3 b0..b2 c0..c2 e1..e9 of set_ABCt! (ETD.jl:142-227)
+  double pad[3];
 };
 
 enum : int { kStatsSlots = 32 };
@@ -41,40 +44,74 @@ struct Tables {  // per-length twiddle tables (device pointers), by precision
   const void* tw;
 };
 
+enum : int { kMaxFields = 16 };
+
+// extra operands of the column kernels
+template <typename T> struct ColAux {
+  const typename std::conditional<sizeof(T) == 8, double2, float2>::type* rho = nullptr;  // inverse z: density
+  typename std::conditional<sizeof(T) == 8, double2, float2>::type* dst[kMaxFields] = {};  // forward z outputs
+};
+
 // ---- launchers (T = double or float) ----
+// inverse z pass; curl = true: the nf physical-space inputs of the nonlinear term
+// (u, w, [rho], [S]) are formed on the fly from u (and rho)
 template <typename T>
-void launch_inv_z(const Geo& g, const void* u, void* a1, int nf, bool curl, const void* tw, hipStream_t s);
+void launch_inv_z(const Geo& g, const void* u, void* a1, int nf, bool curl, const void* tw, hipStream_t s,
+                  const void* rho = nullptr);
 template <typename T>
 void launch_inv_y(const Geo& g, const void* a1, void* a2, int nf, const void* tw, hipStream_t s);
+// field set of the fused x pass
+struct XOps {
+  int nin, nout;      // physical inputs (6 [+1 rho] [+5 S]) and forward outputs (3 [+3] [+5] [+1 ke])
+  int rho, les, ke;   // flags, in that order after u x w
+  double les_2alpha;  // 2 (c Delta)^2 of the Smagorinsky stress
+};
 template <typename T>
-void launch_x_nl(const Geo& g, const void* a2, void* b2, bool with_ke, unsigned long long* umax, const void* tw,
-                 hipStream_t s);
+void launch_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
+                 unsigned long long* rhomax, const void* tw, hipStream_t s);
 template <typename T>
 void launch_x_c2r(const Geo& g, const void* a2, T* real, int nf, const void* tw, hipStream_t s);
 template <typename T>
 void launch_x_r2c(const Geo& g, const T* real, void* b2, int nf, const void* tw, hipStream_t s);
 template <typename T>
 void launch_fwd_y(const Geo& g, const void* b2, void* b1, int nf, const void* tw, hipStream_t s);
+// forward z pass: field f goes to dst[f] (compact spectral), scaled; the k=0 mode
+// of the first nzero_origin fields is zeroed
 template <typename T>
-void launch_fwd_z(const Geo& g, const void* b1, void* out, void* out2, int nf, int nzero_origin, double scale,
+void launch_fwd_z(const Geo& g, const void* b1, void* const* dst, int nf, int nzero_origin, double scale,
                   const void* tw, hipStream_t s);
 
 struct UpdateArgs {
-  int mode;         // 0 = AB3 step, 1 = AB3 step with fm2 extrapolation (fix_fm2!), 3 = final rhs only
-  double nu;        // kinematic viscosity (explicit term, AB3)
-  int forcing;      // 0 none, 1 isotropic constant-power shell forcing
-  double kf2;       // forced shell |k|^2 bound
+  int integrator;      // 0 = Adams_Bashforth3rdO (explicit viscosity), 1 = ETD3rdO (ETD1/ETD2 start)
+  int step;            // integrator iteration after this step's increment (1, 2, ...); 0 = final rhs only
+  double nu;           // kinematic viscosity
+  int forcing;         // 0 none, 1 isotropic constant-power shell forcing
+  double kf2;          // forced shell |k|^2 bound
+  const void* tau;     // Smagorinsky stress, 5 compact components (or null)
+  void* rho;           // density (updated in place) or null
+  void* rrhs;          // density rhs out
+  const void* rfm1;    // density history
+  const void* rfm2;
+  const void* flux;    // F(-rho u), 3 compact components
+  double alpha;        // density diffusivity nu / Pr (ETD only, as in the reference)
+  double drhodz;       // mean density gradient
+  double gz;           // g_z = zAcceleration / referenceDensity
 };
 template <typename T>
-void launch_update_ab3(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
-                       const UpdateArgs& a, hipStream_t s);
+void launch_update(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
+                   const UpdateArgs& a, hipStream_t s);
 template <typename T>
 void launch_project(const Geo& g, void* u, hipStream_t s);
 
 // stats: partial sums per block then a deterministic final reduction
+struct StatsAux {
+  const void* tau;  // Smagorinsky stress (5 compact components) or null
+  const void* rho;  // density or null
+  double gz;        // g_z for the buoyancy term of the pressure statistics
+};
 template <typename T>
-int launch_stats(const Geo& g, const void* u, const void* nl, const void* ke, double* partials, double* out,
-                 hipStream_t s);
+int launch_stats(const Geo& g, const void* u, const void* nl, const void* ke, const StatsAux& aux, double* partials,
+                 double* out, hipStream_t s);
 template <typename T>
 void launch_spectrum(const Geo& g, const void* u, double* out_x, double* out_y0, double* out_y, double* out_z0,
                      double* out_z, double* partials, hipStream_t s);
@@ -82,7 +119,10 @@ template <typename T>
 void launch_forcing_sum(const Geo& g, const void* u, const int* band, int nband, DevState* st, double power,
                         hipStream_t s);
 
-void launch_set_dt_ab3(DevState* st, int it, int tstep_it, double cfl, double dx, hipStream_t s);
+// set_dt! on device; etd selects the ETD3rdO history rule; gnorm > 0 adds the density limit
+void launch_set_dt(DevState* st, int it, int tit, double cfl, double dx, int etd, double gnorm, hipStream_t s);
+// AB3 coefficients and ETD3 scalars of the current dt, dt2, dt3 (fixed-step runs)
+void host_step_coefficients(DevState* st);
 void launch_end_step(DevState* st, hipStream_t s);
 
 template <typename T>

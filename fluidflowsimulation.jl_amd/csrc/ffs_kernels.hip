@@ -19,6 +19,7 @@
 #include "ffs_kernels.h"
 #include "fft_device.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -29,6 +30,10 @@ namespace fsk {
 template <typename T> __host__ __device__ constexpr int col_tx(int) { return 128 / (2 * (int)sizeof(T)); }
 template <int N> __host__ __device__ constexpr int tper() { return N / default_ppt(N); }
 template <int N> __host__ __device__ constexpr int xrows() { return (64 / tper<N>()) > 0 ? (64 / tper<N>()) : 1; }
+// rows per block of the general x pass: at most 256 threads
+template <int N, int NSL> __host__ __device__ constexpr int xgen_rows() {
+  return (256 / (NSL * tper<N>())) > 0 ? (256 / (NSL * tper<N>())) : 1;
+}
 
 // Tunables of the fused x pass: points per lane, minimum waves per SIMD the
 // register allocation must allow, and the LDS padding of the exchange.
@@ -99,8 +104,7 @@ enum ColKind { INV_Z = 0, INV_Y = 1, FWD_Y = 2, FWD_Z = 3 };
 
 template <typename T, int N, int KIND, bool CURL>
 __global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ out,
-                                            C<T>* __restrict__ out2, const C<T>* __restrict__ tw_g, int nzero,
-                                            T scale) {
+                                            ColAux<T> aux, const C<T>* __restrict__ tw_g, int nzero, T scale) {
   constexpr int TX = col_tx<T>(N);
   constexpr int TP = tper<N>();
   extern __shared__ unsigned char smem_raw[];
@@ -120,10 +124,12 @@ __global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* _
 
   if constexpr (KIND == INV_Z) {
     const int cy = b;
-    // the curl variant loops over its 6 output fields inside the block, so the
-    // u tile is fetched from HBM once and re-read from L2
+    // SRC variant: the block loops over the nin physical-space inputs of the
+    // nonlinear term -- u (0..2), w = ik x u (3..5) (fourierspacep1!,
+    // models.jl:88-95), rho (if aux.rho), S = sym(ik (x) u) (xx,xy,xz,yy,yz; if
+    // aux.les) -- so the u tile is fetched from HBM once and re-read from L2
     const int f0 = CURL ? 0 : blockIdx.z;
-    const int f1 = CURL ? 6 : blockIdx.z + 1;
+    const int f1 = CURL ? nzero : blockIdx.z + 1;
     const T kx = valid ? (T)g.kx[cx] : (T)0, ky = (T)g.ky[cy];
     for (int f = f0; f < f1; ++f) {
       if (f != f0) __syncthreads();
@@ -133,18 +139,33 @@ __global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* _
         const size_t idx = cx + sx * (cy + (size_t)g.my * cz);
         if (!CURL || f < 3) return in[f * Ssp + idx];
         const T kz = (T)g.kz[cz];
-        C<T> a;  // w = i k x u
-        if (f == 3) {
-          const C<T> uy = in[Ssp + idx], uz = in[2 * Ssp + idx];
-          a = mk<T>(ky * uz.x - kz * uy.x, ky * uz.y - kz * uy.y);
-        } else if (f == 4) {
-          const C<T> ux = in[idx], uz = in[2 * Ssp + idx];
-          a = mk<T>(kz * ux.x - kx * uz.x, kz * ux.y - kx * uz.y);
-        } else {
-          const C<T> ux = in[idx], uy = in[Ssp + idx];
-          a = mk<T>(kx * uy.x - ky * ux.x, kx * uy.y - ky * ux.y);
+        if (f < 6) {
+          C<T> a;  // w = i k x u
+          if (f == 3) {
+            const C<T> uy = in[Ssp + idx], uz = in[2 * Ssp + idx];
+            a = mk<T>(ky * uz.x - kz * uy.x, ky * uz.y - kz * uy.y);
+          } else if (f == 4) {
+            const C<T> ux = in[idx], uz = in[2 * Ssp + idx];
+            a = mk<T>(kz * ux.x - kx * uz.x, kz * ux.y - kx * uz.y);
+          } else {
+            const C<T> ux = in[idx], uy = in[Ssp + idx];
+            a = mk<T>(kx * uy.x - ky * ux.x, kx * uy.y - ky * ux.y);
+          }
+          return mk<T>(-a.y, a.x);
         }
-        return mk<T>(-a.y, a.x);
+        int e = f - 6;
+        if (aux.rho) {
+          if (e == 0) return aux.rho[idx];
+          --e;
+        }
+        // S_ab = (i k_a u_b + i k_b u_a) / 2 for ab = xx, xy, xz, yy, yz (symouter, models.jl:98)
+        const int a_ = (e < 3) ? 0 : 1;
+        const int b_ = (e < 3) ? e : e - 2;
+        const T ka = a_ == 0 ? kx : ky;
+        const T kb = b_ == 0 ? kx : (b_ == 1 ? ky : kz);
+        const C<T> ua = in[a_ * Ssp + idx], ub = in[b_ * Ssp + idx];
+        const C<T> sum = mk<T>((T)0.5 * (ka * ub.x + kb * ua.x), (T)0.5 * (ka * ub.y + kb * ua.y));
+        return mk<T>(-sum.y, sum.x);
       };
       auto st = [&](int p, int, int, C<T> v) {
         if (valid) out[f * S1 + cx + sx * (cy + (size_t)g.my * p)] = v;
@@ -185,8 +206,7 @@ __global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* _
         C<T> r = cscale<T>(v, scale);
         if (f < nzero && cx == 0 && cy == 0 && cz == 0) r = mk<T>(0, 0);
         const size_t o = cx + sx * (cy + (size_t)g.my * cz);
-        if (f < 3) out[f * Ssp + o] = r;   // velocity-like fields
-        else out2[(f - 3) * Ssp + o] = r;  // extra fields (1/2|u|^2, ...)
+        aux.dst[f][o] = r;  // rhs components, density flux, tau, 1/2|u|^2 ...
       }
     };
     Fft<T, N, -1, TP>::run(q, tw, ld, st, ldsf);
@@ -195,7 +215,7 @@ __global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* _
 
 template <typename T, int N, int KIND, bool CURL>
 static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, const void* tw, int nzero, double scale,
-                    hipStream_t s, void* out2 = nullptr) {
+                    hipStream_t s, const ColAux<T>& aux = ColAux<T>{}) {
   constexpr int TX = col_tx<T>(N);
   constexpr int TP = tper<N>();
   const size_t lds = (size_t)(N * TX + N) * sizeof(C<T>);
@@ -205,7 +225,7 @@ static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, con
     attr = true;
   }
   dim3 grid((g.mx + TX - 1) / TX, nb, CURL ? 1 : nf);
-  k_col<T, N, KIND, CURL><<<grid, TX * TP, lds, s>>>(g, (const C<T>*)in, (C<T>*)out, (C<T>*)out2, (const C<T>*)tw, nzero,
+  k_col<T, N, KIND, CURL><<<grid, TX * TP, lds, s>>>(g, (const C<T>*)in, (C<T>*)out, aux, (const C<T>*)tw, nzero,
                                                     (T)scale);
 }
 
@@ -216,12 +236,15 @@ static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, con
 }
 
 template <typename T>
-void launch_inv_z(const Geo& g, const void* u, void* a1, int nf, bool curl, const void* tw, hipStream_t s) {
+void launch_inv_z(const Geo& g, const void* u, void* a1, int nf, bool curl, const void* tw, hipStream_t s,
+                  const void* rho) {
+  ColAux<T> aux{};
+  aux.rho = (const C<T>*)rho;
   switch (g.nz) {
-#define X(n)                                                                              \
-  case n:                                                                                 \
-    if (curl) run_col<T, n, INV_Z, true>(g, u, a1, nf, g.my, tw, 0, 1.0, s);              \
-    else run_col<T, n, INV_Z, false>(g, u, a1, nf, g.my, tw, 0, 1.0, s);                  \
+#define X(n)                                                                                    \
+  case n:                                                                                       \
+    if (curl) run_col<T, n, INV_Z, true>(g, u, a1, 1, g.my, tw, nf, 1.0, s, aux);               \
+    else run_col<T, n, INV_Z, false>(g, u, a1, nf, g.my, tw, 0, 1.0, s, aux);                   \
     break;
     FFS_SIZES(X)
 #undef X
@@ -252,11 +275,13 @@ void launch_fwd_y(const Geo& g, const void* b2, void* b1, int nf, const void* tw
 }
 
 template <typename T>
-void launch_fwd_z(const Geo& g, const void* b1, void* out, void* out2, int nf, int nzero, double scale,
-                  const void* tw, hipStream_t s) {
+void launch_fwd_z(const Geo& g, const void* b1, void* const* dst, int nf, int nzero, double scale, const void* tw,
+                  hipStream_t s) {
+  ColAux<T> aux{};
+  for (int f = 0; f < nf && f < kMaxFields; ++f) aux.dst[f] = (C<T>*)dst[f];
   switch (g.nz) {
 #define X(n)                                                                        \
-  case n: run_col<T, n, FWD_Z, false>(g, b1, out, nf, g.my, tw, nzero, scale, s, out2); break;
+  case n: run_col<T, n, FWD_Z, false>(g, b1, nullptr, nf, g.my, tw, nzero, scale, s, aux); break;
     FFS_SIZES(X)
 #undef X
     default: bad_size("forward z pass", g.nz);
@@ -287,83 +312,110 @@ __device__ __forceinline__ void split_pair(const C<T>* Z, int k, C<T>& A, C<T>& 
   B = mk<T>((T)0.5 * (zk.y + zn.y), (T)0.5 * (zn.x - zk.x));
 }
 
-// Fused nonlinear x pass: a2 holds (ux, uy, uz, wx, wy, wz) y/z-transformed,
-// b2 receives the x-forward transform of (u x w)_{x,y,z} and, when KE, 1/2|u|^2.
-template <typename T, int N, bool KE>
-__global__ void __launch_bounds__(256) k_x_nl(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
-                                             const C<T>* __restrict__ tw_g, unsigned long long* __restrict__ umax) {
+// Fused nonlinear x pass, general field set (LDS slots; any transform length).
+// Inputs (a2, y/z-transformed): u (0..2), w (3..5), [rho], [S xx xy xz yy yz].
+// Outputs (b2, x-forward transformed): u x w (0..2), [-rho u (3)], [tau =
+// 2 (c Delta)^2 |S| S (5), calc_les! les_real_calc.jl:77-79 with
+// |S| = sqrt(2 S:S)], [1/2|u|^2].  Each row holds NSL slots of two real
+// sequences; outputs overwrite inputs slot by slot after the pointwise phase.
+template <typename T, int N, int NSL>
+__global__ void __launch_bounds__(256) k_x_gen(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
+                                              const C<T>* __restrict__ tw_g, XOps ops,
+                                              unsigned long long* __restrict__ umax,
+                                              unsigned long long* __restrict__ rhomax) {
   constexpr int TP = tper<N>();
-  constexpr int ROWS = xrows<N>();
-  constexpr int NSLOT = 3 * ROWS;
+  constexpr int ROWS = xgen_rows<N, NSL>();
   constexpr int LD = N + 1;
-  constexpr int NTHR = NSLOT * TP;
+  constexpr int NTHR = ROWS * NSL * TP;
   extern __shared__ unsigned char smem_raw[];
   C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
-  const C<T>* tw = stage_twiddles<T, N>(lds + NSLOT * LD, tw_g);  // published by the first __syncthreads below
+  const C<T>* tw = stage_twiddles<T, N>(lds + ROWS * NSL * LD, tw_g);  // published by the barrier below
   const int tid = threadIdx.x;
   const int nrows = g.ny * g.nz;
   const int row0 = blockIdx.x * ROWS;
   const int mx = g.mx;
   const size_t S2 = (size_t)g.sx * nrows;
+  const int nin = ops.nin, nout = ops.nout;
 
-  // A: gather retained half spectra into Hermitian-packed pairs (ux,uy) (uz,wx) (wy,wz)
-  for (int w = tid; w < NSLOT * (N / 2 + 1); w += NTHR) {
+  // A: gather retained half spectra into Hermitian-packed pairs
+  for (int w = tid; w < ROWS * NSL * (N / 2 + 1); w += NTHR) {
     const int sl = w / (N / 2 + 1), k = w % (N / 2 + 1);
-    const int r = sl / 3, pr = sl % 3;
+    const int r = sl / NSL, pr = sl % NSL;
     const int row = row0 + r;
     C<T> A = mk<T>(0, 0), B = mk<T>(0, 0);
     if (row < nrows && k < mx) {
-      A = a2[(2 * pr) * S2 + (size_t)row * g.sx + k];
-      B = a2[(2 * pr + 1) * S2 + (size_t)row * g.sx + k];
+      if (2 * pr < nin) A = a2[(size_t)(2 * pr) * S2 + (size_t)row * g.sx + k];
+      if (2 * pr + 1 < nin) B = a2[(size_t)(2 * pr + 1) * S2 + (size_t)row * g.sx + k];
     }
     build_pair<T, N>(lds + sl * LD, k, A, B);
   }
   __syncthreads();
   const int sl = tid / TP, q = tid % TP;
   C<T>* Zs = lds + sl * LD;
-  // B: inverse transforms (all slots)
   Fft<T, N, +1, TP, true, true>::run(
       q, tw, [&](int p, int, int) { return Zs[p]; }, [&](int p, int, int, C<T> v) { Zs[p] = v; },
       [&](int p) -> C<T>& { return Zs[p]; });
   __syncthreads();
-  // C: pointwise u x w (and 1/2|u|^2), max |u_i|
-  double lmax = 0.0;
+  // C: pointwise
+  double lmax = 0.0, rmax = 0.0;
   for (int w = tid; w < ROWS * N; w += NTHR) {
     const int r = w / N, n = w % N;
-    C<T>* Z0 = lds + (3 * r) * LD;
-    C<T>* Z1 = Z0 + LD;
-    C<T>* Z2 = Z1 + LD;
-    const C<T> z0 = Z0[n], z1 = Z1[n], z2 = Z2[n];
-    const T ux = z0.x, uy = z0.y, uz = z1.x, wx = z1.y, wy = z2.x, wz = z2.y;
-    const T px = uy * wz - uz * wy;
-    const T py = uz * wx - ux * wz;
-    const T pz = ux * wy - uy * wx;
-    T ke = 0;
-    if constexpr (KE) ke = (T)0.5 * (ux * ux + uy * uy + uz * uz);
-    Z0[n] = mk<T>(px, py);
-    Z1[n] = mk<T>(pz, ke);
-    if (row0 + r < nrows) {
-      const T m = nanmax(fabs(ux), nanmax(fabs(uy), fabs(uz)));
-      lmax = nanmax(lmax, (double)m);
+    C<T>* Z = lds + (r * NSL) * LD;
+    T in[2 * NSL], out[2 * NSL];
+#pragma unroll
+    for (int j = 0; j < NSL; ++j) {
+      const C<T> z = Z[j * LD + n];
+      in[2 * j] = z.x;
+      in[2 * j + 1] = z.y;
+      out[2 * j] = 0;
+      out[2 * j + 1] = 0;
     }
+    const T ux = in[0], uy = in[1], uz = in[2], wx = in[3], wy = in[4], wz = in[5];
+    out[0] = uy * wz - uz * wy;
+    out[1] = uz * wx - ux * wz;
+    out[2] = ux * wy - uy * wx;
+    int o = 3, i = 6;
+    if (ops.rho) {
+      const T rho = in[i++];
+      out[o++] = (-rho) * ux;
+      out[o++] = (-rho) * uy;
+      out[o++] = (-rho) * uz;
+      if (row0 + r < nrows) rmax = nanmax(rmax, (double)fabs(rho));
+    }
+    if (ops.les) {
+      const T sxx = in[i], sxy = in[i + 1], sxz = in[i + 2], syy = in[i + 3], syz = in[i + 4];
+      const T szz = -sxx - syy;
+      const T ss = sxx * sxx + syy * syy + szz * szz + (T)2 * (sxy * sxy + sxz * sxz + syz * syz);
+      const T f = (T)ops.les_2alpha * sqrt((T)2 * ss);
+      out[o++] = f * sxx;
+      out[o++] = f * sxy;
+      out[o++] = f * sxz;
+      out[o++] = f * syy;
+      out[o++] = f * syz;
+    }
+    if (ops.ke) out[o++] = (T)0.5 * (ux * ux + uy * uy + uz * uz);
+#pragma unroll
+    for (int j = 0; j < NSL; ++j) Z[j * LD + n] = mk<T>(out[2 * j], out[2 * j + 1]);
+    if (row0 + r < nrows) lmax = nanmax(lmax, (double)nanmax(fabs(ux), nanmax(fabs(uy), fabs(uz))));
   }
   __syncthreads();
-  // D: forward transforms (the third slot of each row is idle work)
+  // D: forward transforms
   Fft<T, N, -1, TP, true, true>::run(
       q, tw, [&](int p, int, int) { return Zs[p]; }, [&](int p, int, int, C<T> v) { Zs[p] = v; },
       [&](int p) -> C<T>& { return Zs[p]; });
   __syncthreads();
-  // E: separate the two real transforms of each slot, keep k < mx
-  for (int w = tid; w < ROWS * 2 * mx; w += NTHR) {
-    const int k = w % mx, rj = w / mx, r = rj >> 1, j = rj & 1;
+  // E: separate the pairs, keep k < mx
+  for (int w = tid; w < ROWS * NSL * mx; w += NTHR) {
+    const int k = w % mx, rj = w / mx, r = rj / NSL, j = rj % NSL;
     const int row = row0 + r;
-    if (row >= nrows) continue;
+    if (row >= nrows || 2 * j >= nout) continue;
     C<T> A, B;
-    split_pair<T, N>(lds + (3 * r + j) * LD, k, A, B);
-    b2[(2 * j) * S2 + (size_t)row * g.sx + k] = A;
-    if (j == 0 || KE) b2[(2 * j + 1) * S2 + (size_t)row * g.sx + k] = B;
+    split_pair<T, N>(lds + (r * NSL + j) * LD, k, A, B);
+    b2[(size_t)(2 * j) * S2 + (size_t)row * g.sx + k] = A;
+    if (2 * j + 1 < nout) b2[(size_t)(2 * j + 1) * S2 + (size_t)row * g.sx + k] = B;
   }
   if (umax) block_max_to(lmax, umax);
+  if (rhomax) block_max_to(rmax, rhomax);
 }
 
 // Register-resident variant of the fused x pass (used when every FFT stage
@@ -566,39 +618,62 @@ static size_t x_lds_bytes() {
   return ((size_t)3 * xrows<N>() * (N + 1) + N) * sizeof(C<T>);
 }
 
-template <typename T, int N, bool KE>
-static void run_x_nl(const Geo& g, const void* a2, void* b2, unsigned long long* umax, const void* tw, hipStream_t s) {
-  const int nrows = g.ny * g.nz;
-  if constexpr (xreg_ok<N>()) {
-    constexpr int TP = N / xreg_ppt<N>();
-    constexpr int WPB = 2;  // wavefronts per block
-    constexpr int RPB = WPB * (64 / TP);
-    const size_t lds = ((size_t)RPB * xreg_ldx<N>() + N) * sizeof(C<T>);
-    static bool attr = false;
-    if (!attr) {
-      (void)hipFuncSetAttribute((const void*)&k_x_nl_reg<T, N, KE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
-      attr = true;
-    }
-    k_x_nl_reg<T, N, KE><<<(nrows + RPB - 1) / RPB, 64 * WPB, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
-                                                                      (const C<T>*)tw, umax);
-  } else {
-    constexpr int R = xrows<N>();
-    constexpr int TP = tper<N>();
-    k_x_nl<T, N, KE><<<(nrows + R - 1) / R, 3 * R * TP, x_lds_bytes<T, N>(), s>>>(g, (const C<T>*)a2, (C<T>*)b2,
-                                                                                (const C<T>*)tw, umax);
+template <typename T, int N, int NSL>
+static void run_x_gen(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
+                      unsigned long long* rhomax, const void* tw, hipStream_t s) {
+  constexpr int ROWS = xgen_rows<N, NSL>();
+  const size_t lds = ((size_t)ROWS * NSL * (N + 1) + N) * sizeof(C<T>);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)&k_x_gen<T, N, NSL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    attr = true;
   }
+  const int nrows = g.ny * g.nz;
+  k_x_gen<T, N, NSL><<<(nrows + ROWS - 1) / ROWS, ROWS * NSL * tper<N>(), lds, s>>>(
+      g, (const C<T>*)a2, (C<T>*)b2, (const C<T>*)tw, ops, umax, rhomax);
+}
+
+template <typename T, int N>
+static void run_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
+                     unsigned long long* rhomax, const void* tw, hipStream_t s) {
+  const int nrows = g.ny * g.nz;
+  const bool plain = !ops.rho && !ops.les;
+  if constexpr (xreg_ok<N>()) {
+    if (plain) {
+      constexpr int TP = N / xreg_ppt<N>();
+      constexpr int WPB = 2;  // wavefronts per block
+      constexpr int RPB = WPB * (64 / TP);
+      const size_t lds = ((size_t)RPB * xreg_ldx<N>() + N) * sizeof(C<T>);
+      static bool attr = false;
+      if (!attr) {
+        (void)hipFuncSetAttribute((const void*)&k_x_nl_reg<T, N, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        (void)hipFuncSetAttribute((const void*)&k_x_nl_reg<T, N, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        attr = true;
+      }
+      if (ops.ke)
+        k_x_nl_reg<T, N, true><<<(nrows + RPB - 1) / RPB, 64 * WPB, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
+                                                                            (const C<T>*)tw, umax);
+      else
+        k_x_nl_reg<T, N, false><<<(nrows + RPB - 1) / RPB, 64 * WPB, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
+                                                                             (const C<T>*)tw, umax);
+      return;
+    }
+  }
+  const int nsl = (std::max(ops.nin, ops.nout) + 1) / 2;
+  if (nsl <= 3) run_x_gen<T, N, 3>(g, a2, b2, ops, umax, rhomax, tw, s);
+  else if (nsl <= 4) run_x_gen<T, N, 4>(g, a2, b2, ops, umax, rhomax, tw, s);
+  else run_x_gen<T, N, 6>(g, a2, b2, ops, umax, rhomax, tw, s);
 }
 
 template <typename T>
-void launch_x_nl(const Geo& g, const void* a2, void* b2, bool with_ke, unsigned long long* umax, const void* tw,
-                 hipStream_t s) {
+void launch_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
+                 unsigned long long* rhomax, const void* tw, hipStream_t s) {
   switch (g.nrx) {
-#define X(n)                                                    \
-  case n:                                                       \
-    if (with_ke) run_x_nl<T, n, true>(g, a2, b2, umax, tw, s);  \
-    else run_x_nl<T, n, false>(g, a2, b2, umax, tw, s);         \
-    break;
+#define X(n)                                                     \
+  case n: run_x_nl<T, n>(g, a2, b2, ops, umax, rhomax, tw, s); break;
     FFS_SIZES(X)
 #undef X
     default: bad_size("nonlinear x pass", g.nrx);
@@ -640,15 +715,66 @@ void launch_x_r2c(const Geo& g, const T* real, void* b2, int nf, const void* tw,
 }
 
 // ===========================================================================
-// Spectral update: rhs <- P(rhs - nu k^2 u) (fourierspacep2_velocity!,
-// models.jl:292-341); AB3 (Adams_Bashforth.jl:242-256) with the step-2
-// history extrapolation fix_fm2! (ETD.jl:136-139) done in registers; then the
-// final projection (time_step_functions.jl:78).  rhs is written back: it is
-// the next step's fm1 (history rotated by pointer on the host).
+// Spectral update of one mode, fused: fourierspacep2! (models.jl:282-364:
+// -nu k^2 u for AB3, + (ik).tau, + rho g, Leray; rhs.c[1] = 0 was applied by
+// the forward z pass) then VectorTimeStep (time_step_functions.jl:72-80: AB3
+// Adams_Bashforth.jl:231-256 with fix_fm2! ETD.jl:130-140, or ETD1/ETD2/ETD3
+// ETD1stand2ndO.jl:264-327 / ETD.jl:229-262, forcing, projection) and the
+// density integrator (models.jl:471) with rho rhs = ik.F(-rho u) - drhodz w
+// (div!, vectorfunctions.jl:93-103) from the pre-step w.  rhs / rho rhs are
+// written back: they are the next step's fm1 (history rotated by pointer).
+
+// ETD coefficients of one mode (FP64): the closed forms with expm1 and the
+// series below |c dt| = 1e-4 exactly as set_ABCt! / ETD1stO! / ETD2ndO!.
+struct EtdCoef {
+  double E, A, B, Cc;  // exp(c dt) and the multipliers of rhs, fm1, fm2
+};
+__device__ __forceinline__ EtdCoef etd_coef(double c, int step, const DevState* __restrict__ st) {
+  // no FMA contraction: the closed forms cancel to O((c dt)^3), so every rounding step has to be
+  // the one the reference's plain * and + perform
+#pragma clang fp contract(off)
+  const double dt = st->dt;
+  EtdCoef r;
+  r.E = exp(c * dt);
+  r.A = r.B = r.Cc = 0.0;
+  if (step == 1) {
+    const double cdt = c * dt, cdt2 = cdt * dt, c2dt3 = cdt2 * cdt, c3dt4 = c2dt3 * cdt;
+    r.A = (-cdt <= 1e-4) ? dt + 0.5 * cdt2 + c2dt3 / 6 + c3dt4 / 24 : expm1(cdt) / c;
+  } else if (step == 2) {
+    const double dt2 = st->dt2;
+    const double cp2 = c * c, cdt = c * dt, cdtp2 = cdt * dt, c2dtp3 = cdtp2 * cdt, c3dtp4 = c2dtp3 * cdt;
+    const double em = expm1(cdt), c2dt2 = c * c * dt2, dtp2 = dt * dt, dtp3 = dtp2 * dt, dtp4 = dtp3 * dt;
+    if (-cdt <= 1e-4) {
+      r.A = dt + cdtp2 / 2 + (c2dtp3) / 6 + (c3dtp4) / 24 + dtp2 / (2 * dt2) + (c * dtp3) / (6 * dt2) +
+            (cp2 * dtp4) / (24 * dt2);
+      r.B = (-(dtp2 / (2 * dt2)) - (c * dtp3) / (6 * dt2) - (cp2 * dtp4) / (24 * dt2));
+    } else {
+      r.A = (em - c * (dt - dt2 * em)) / c2dt2;
+      r.B = (cdt - em) / c2dt2;
+    }
+  } else {
+    const double* e = st->etd;  // a0 a1 a2 a3 b0 b1 b2 c0 c1 c2 e1 .. e9
+    const double l1 = c, l2 = l1 * l1, l3 = l2 * l1, ldt = l1 * dt;
+    if (-l1 * dt <= 1e-4) {
+      r.A = fma(e[1], l1, fma(e[2], l2, fma(e[3], l3, e[0])));
+      r.B = fma(e[5], l1, fma(e[6], l2, e[4]));
+      r.Cc = fma(e[8], l1, fma(e[9], l2, e[7]));
+    } else {
+      const double em = expm1(ldt), dt2 = st->dt2;
+      const double e1 = e[10], e2 = e[11], e3 = e[12], e4 = e[13], e5 = e[14], e6 = e[15], e7 = e[16], e8 = e[17],
+                   e9 = e[18];
+      r.A = (2 * em - l2 * (e1 - e2 * em) - l1 * (e3 - e4 * em)) / (l3 * e2);
+      r.B = (e5 * l2 - 2 * em + l1 * (e3 - e6 * em)) / (l3 * e7);
+      r.Cc = (2 * em - e8 * l2 - l1 * (e3 - dt2 * em)) / (l3 * e9);
+    }
+  }
+  return r;
+}
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_update_ab3(Geo g, C<T>* __restrict__ u, C<T>* __restrict__ rhs,
-                                                   const C<T>* __restrict__ fm1, const C<T>* __restrict__ fm2,
-                                                   const DevState* __restrict__ st, UpdateArgs a) {
+__global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T>* __restrict__ rhs,
+                                               const C<T>* __restrict__ fm1, const C<T>* __restrict__ fm2,
+                                               const DevState* __restrict__ st, UpdateArgs a) {
   const size_t n = (size_t)g.sx * g.my * g.mz;
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n) return;
@@ -658,40 +784,115 @@ __global__ void __launch_bounds__(256) k_update_ab3(Geo g, C<T>* __restrict__ u,
   const int cz = idx / ((size_t)g.sx * g.my);
   const double kv3[3] = {g.kx[cx], g.ky[cy], g.kz[cz]};
   const double K2 = kv3[0] * kv3[0] + kv3[1] * kv3[1] + kv3[2] * kv3[2];
-  const double kvis = K2 * (-a.nu);
   const bool origin = (idx == 0);
-  double U[3][2], R[3][2], M1[3][2], M2[3][2];
+  double U[3][2], R[3][2];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const C<T> v = u[c * n + idx], r = rhs[c * n + idx], h1 = fm1[c * n + idx], h2 = fm2[c * n + idx];
+    const C<T> v = u[c * n + idx], r = rhs[c * n + idx];
     U[c][0] = v.x; U[c][1] = v.y;
     R[c][0] = r.x; R[c][1] = r.y;
-    M1[c][0] = h1.x; M1[c][1] = h1.y;
-    M2[c][0] = h2.x; M2[c][1] = h2.y;
+  }
+  if (a.integrator == 0) {  // explicit viscosity (is_implicit false)
+    const double kvis = K2 * (-a.nu);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) R[c][p] += kvis * U[c][p];
+  }
+  if (a.tau) {  // rhs += (i k) . tau, tau symmetric traceless (xx xy xz yy yz)
+    const C<T>* t = (const C<T>*)a.tau;
+    const C<T> txx = t[idx], txy = t[n + idx], txz = t[2 * n + idx], tyy = t[3 * n + idx], tyz = t[4 * n + idx];
+    const double T6[6][2] = {{txx.x, txx.y}, {txy.x, txy.y}, {txz.x, txz.y},
+                             {tyy.x, tyy.y}, {tyz.x, tyz.y}, {-(double)txx.x - tyy.x, -(double)txx.y - tyy.y}};
+    // rows of the symmetric tensor: x (xx xy xz), y (xy yy yz), z (xz yz zz)
+    const int row[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double sr = kv3[0] * T6[row[c][0]][0] + kv3[1] * T6[row[c][1]][0] + kv3[2] * T6[row[c][2]][0];
+      const double si = kv3[0] * T6[row[c][0]][1] + kv3[1] * T6[row[c][1]][1] + kv3[2] * T6[row[c][2]][1];
+      R[c][0] += -si;
+      R[c][1] += sr;
+    }
+  }
+  double RH[2] = {0, 0};
+  if (a.rho) {  // buoyancy rhs += rho g, g = (0, 0, g_z)
+    const C<T> r0 = ((const C<T>*)a.rho)[idx];
+    RH[0] = r0.x;
+    RH[1] = r0.y;
+    R[2][0] += RH[0] * a.gz;
+    R[2][1] += RH[1] * a.gz;
   }
 #pragma unroll
-  for (int p = 0; p < 2; ++p) {  // real, imaginary parts
-#pragma unroll
-    for (int c = 0; c < 3; ++c) R[c][p] += kvis * U[c][p];
+  for (int p = 0; p < 2; ++p) {
     const double p1 = origin ? 0.0 : -(kv3[0] * R[0][p] + kv3[1] * R[1][p] + kv3[2] * R[2][p]) / K2;
 #pragma unroll
     for (int c = 0; c < 3; ++c) R[c][p] = p1 * kv3[c] + R[c][p];
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) rhs[c * n + idx] = mk<T>((T)R[c][0], (T)R[c][1]);
-  if (a.mode == 3) return;  // initialisation: final rhs only (initialize!, FluidFlowSimulation.jl:82)
-  const double At = st->At, Bt = st->Bt, Ct = st->Ct;
-  if (a.mode == 1) {  // fix_fm2!: fm2 = -(dt3+dt2)(rhs - fm1)/dt2 + rhs
-    const double dt2 = st->dt2, dt3 = st->dt3;
+  // density rhs from the pre-step vertical velocity
+  double RR[2] = {0, 0};
+  if (a.rho) {
+    const C<T>* f = (const C<T>*)a.flux;
+    const C<T> fx = f[idx], fy = f[n + idx], fz = f[2 * n + idx];
+    const double kf_r = fma(kv3[0], (double)fx.x, fma(kv3[1], (double)fy.x, kv3[2] * fz.x));
+    const double kf_i = fma(kv3[0], (double)fx.y, fma(kv3[1], (double)fy.y, kv3[2] * fz.y));
+    RR[0] = -kf_i + (-a.drhodz) * U[2][0];
+    RR[1] = kf_r + (-a.drhodz) * U[2][1];
+    ((C<T>*)a.rrhs)[idx] = mk<T>((T)RR[0], (T)RR[1]);
+  }
+  if (a.step == 0) return;  // initialisation: final rhs only (initialize!, FluidFlowSimulation.jl:82)
+
+  double M1[3][2], M2[3][2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const C<T> h1 = fm1[c * n + idx], h2 = fm2[c * n + idx];
+    M1[c][0] = h1.x; M1[c][1] = h1.y;
+    M2[c][0] = h2.x; M2[c][1] = h2.y;
+  }
+  double HR1[2] = {0, 0}, HR2[2] = {0, 0};
+  if (a.rho) {
+    const C<T> h1 = ((const C<T>*)a.rfm1)[idx], h2 = ((const C<T>*)a.rfm2)[idx];
+    HR1[0] = h1.x; HR1[1] = h1.y;
+    HR2[0] = h2.x; HR2[1] = h2.y;
+  }
+  if (a.integrator == 0) {
+    const double At = st->At, Bt = st->Bt, Ct = st->Ct;
+    if (a.step == 2) {  // fix_fm2!: fm2 = -(dt3+dt2)(rhs - fm1)/dt2 + rhs
+      const double dt2 = st->dt2, dt3 = st->dt3;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) M2[c][p] = -(dt3 + dt2) * (R[c][p] - M1[c][p]) / dt2 + R[c][p];
+#pragma unroll
+      for (int p = 0; p < 2; ++p) HR2[p] = -(dt3 + dt2) * (RR[p] - HR1[p]) / dt2 + RR[p];
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int p = 0; p < 2; ++p) M2[c][p] = -(dt3 + dt2) * (R[c][p] - M1[c][p]) / dt2 + R[c][p];
+      for (int p = 0; p < 2; ++p) U[c][p] = fma(At, R[c][p], fma(Bt, M1[c][p], fma(Ct, M2[c][p], U[c][p])));
+#pragma unroll
+    for (int p = 0; p < 2; ++p) RH[p] = fma(At, RR[p], fma(Bt, HR1[p], fma(Ct, HR2[p], RH[p])));
+  } else {
+    const EtdCoef e = etd_coef(K2 * (-a.nu), a.step, st);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        if (a.step == 1) U[c][p] = fma(e.E, U[c][p], e.A * R[c][p]);
+        else if (a.step == 2) U[c][p] = fma(e.E, U[c][p], fma(e.A, R[c][p], e.B * M1[c][p]));
+        else U[c][p] = fma(e.A, R[c][p], fma(e.B, M1[c][p], fma(e.Cc, M2[c][p], e.E * U[c][p])));
+      }
+    if (a.rho) {
+      const EtdCoef d = etd_coef(K2 * (-a.alpha), a.step, st);
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        if (a.step == 1) RH[p] = fma(d.E, RH[p], d.A * RR[p]);
+        else if (a.step == 2) RH[p] = fma(d.E, RH[p], fma(d.A, RR[p], d.B * HR1[p]));
+        else RH[p] = fma(d.A, RR[p], fma(d.B, HR1[p], fma(d.Cc, HR2[p], d.E * RH[p])));
+      }
+    }
   }
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int p = 0; p < 2; ++p) U[c][p] = fma(At, R[c][p], fma(Bt, M1[c][p], fma(Ct, M2[c][p], U[c][p])));
   // constant-power isotropic shell forcing, added after the integrator like add_forcing!
   // (time_step_functions.jl:77): u += dt * (P / sum_band w|u|^2) u on 0 < |k|^2 <= kf2
   if (a.forcing == 1 && K2 <= a.kf2 && !origin) {
@@ -709,13 +910,14 @@ __global__ void __launch_bounds__(256) k_update_ab3(Geo g, C<T>* __restrict__ u,
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) u[c * n + idx] = mk<T>((T)U[c][0], (T)U[c][1]);
+  if (a.rho) ((C<T>*)a.rho)[idx] = mk<T>((T)RH[0], (T)RH[1]);
 }
 
 template <typename T>
-void launch_update_ab3(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
-                       const UpdateArgs& a, hipStream_t s) {
+void launch_update(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
+                   const UpdateArgs& a, hipStream_t s) {
   const size_t n = (size_t)g.sx * g.my * g.mz;
-  k_update_ab3<T><<<(n + 255) / 256, 256, 0, s>>>(g, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1, (const C<T>*)fm2, st, a);
+  k_update<T><<<(n + 255) / 256, 256, 0, s>>>(g, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1, (const C<T>*)fm2, st, a);
 }
 
 // Leray projection of a spectral vector field (pressure_projection!, models.jl:443-456)
@@ -761,17 +963,23 @@ void launch_project(const Geo& g, void* u, hipStream_t s) {
 //  5,6   nlh, nlv                7,8 pressh, pressv
 //  9 sum w kx^2|ux|^2  10 sum w ky^2|ux|^2  11 sum w kx^2|uy|^2  12 sum w ky^2|uy|^2
 //  13+3c+{0,1,2}: integral-length plane sums of component c over kx=0 (no weight), ky=0, kz=0
-enum { kStatsN = 22, kStatsBlocks = 512 };
+//  density (scalar_stats :221-248, buoyancy_flux density_stratification_types.jl:192-211):
+//  22 sum w|rho|^2  23..25 sum w k_i^2 |rho|^2  26 sum w proj(uz, rho)  29 Re rho(k=0)
+//  LES (les_stats :679-705): 27 sum w proj((ik.tau)_{x,y}, u_{x,y})  28 the same for z
+enum { kStatsN = 30, kStatsBlocks = 512, kStatsThreads = 128 };
 
 __device__ __forceinline__ double proj2(double ax, double ay, double bx, double by) { return ax * bx + ay * by; }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_stats(Geo g, const C<T>* __restrict__ u, const C<T>* __restrict__ nl,
-                                              const C<T>* __restrict__ ke, double* __restrict__ partials) {
+__global__ void __launch_bounds__(kStatsThreads) k_stats(Geo g, const C<T>* __restrict__ u,
+                                                        const C<T>* __restrict__ nl, const C<T>* __restrict__ ke,
+                                                        StatsAux aux, double* __restrict__ partials) {
   const size_t n = (size_t)g.sx * g.my * g.mz;
   double acc[kStatsN];
 #pragma unroll
   for (int i = 0; i < kStatsN; ++i) acc[i] = 0.0;
+  const C<T>* tau = (const C<T>*)aux.tau;
+  const C<T>* rho = (const C<T>*)aux.rho;
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
     const int cx = idx % g.sx;
     if (cx >= g.mx) continue;
@@ -801,6 +1009,38 @@ __global__ void __launch_bounds__(256) k_stats(Geo g, const C<T>* __restrict__ u
       if (ky == 0.0) acc[14 + 3 * c] += w * e3[c];
       if (kz == 0.0) acc[15 + 3 * c] += w * e3[c];
     }
+    // (i k) . tau per component (les_stats and the pressure rhs)
+    double dtau[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    if (tau) {
+      const C<T> txx = tau[idx], txy = tau[n + idx], txz = tau[2 * n + idx], tyy = tau[3 * n + idx],
+                 tyz = tau[4 * n + idx];
+      const double T6[6][2] = {{txx.x, txx.y}, {txy.x, txy.y}, {txz.x, txz.y},
+                               {tyy.x, tyy.y}, {tyz.x, tyz.y}, {-(double)txx.x - tyy.x, -(double)txx.y - tyy.y}};
+      const int row[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+      const double kv3[3] = {kx, ky, kz};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double sr = kv3[0] * T6[row[c][0]][0] + kv3[1] * T6[row[c][1]][0] + kv3[2] * T6[row[c][2]][0];
+        const double si = kv3[0] * T6[row[c][0]][1] + kv3[1] * T6[row[c][1]][1] + kv3[2] * T6[row[c][2]][1];
+        dtau[c][0] = -si;
+        dtau[c][1] = sr;
+      }
+      acc[27] += w * (proj2(dtau[0][0], dtau[0][1], ux[0], ux[1]) + proj2(dtau[1][0], dtau[1][1], uy[0], uy[1]));
+      acc[28] += w * proj2(dtau[2][0], dtau[2][1], uz[0], uz[1]);
+    }
+    double rr[2] = {0, 0};
+    if (rho) {
+      const C<T> r0 = rho[idx];
+      rr[0] = r0.x;
+      rr[1] = r0.y;
+      const double er = rr[0] * rr[0] + rr[1] * rr[1];
+      acc[22] += w * er;
+      acc[23] += w * kx * kx * er;
+      acc[24] += w * ky * ky * er;
+      acc[25] += w * kz * kz * er;
+      acc[26] += w * proj2(uz[0], uz[1], rr[0], rr[1]);
+      if (idx == 0) acc[29] += rr[0];
+    }
     if (nl) {
       const C<T> n0 = nl[idx], n1 = nl[n + idx], n2 = nl[2 * n + idx];
       const C<T> kk = ke[idx];
@@ -812,19 +1052,23 @@ __global__ void __launch_bounds__(256) k_stats(Geo g, const C<T>* __restrict__ u
       acc[5] += w * (proj2(qx[0], qx[1], ux[0], ux[1]) + proj2(qy[0], qy[1], uy[0], uy[1]));
       acc[6] += w * proj2(qz[0], qz[1], uz[0], uz[1]);
       if (idx != 0) {
-        const double p1r = -(kx * qx[0] + ky * qy[0] + kz * qz[0]) / K2;
-        const double p1i = -(kx * qx[1] + ky * qy[1] + kz * qz[1]) / K2;
+        // pressure rhs: + (i k).tau (LES) + rho g (density)
+        const double px[2] = {qx[0] + dtau[0][0], qx[1] + dtau[0][1]};
+        const double py[2] = {qy[0] + dtau[1][0], qy[1] + dtau[1][1]};
+        const double pz[2] = {qz[0] + dtau[2][0] + rr[0] * aux.gz, qz[1] + dtau[2][1] + rr[1] * aux.gz};
+        const double p1r = -(kx * px[0] + ky * py[0] + kz * pz[0]) / K2;
+        const double p1i = -(kx * px[1] + ky * py[1] + kz * pz[1]) / K2;
         acc[7] += w * (proj2(ux[0], ux[1], p1r * kx, p1i * kx) + proj2(uy[0], uy[1], p1r * ky, p1i * ky));
         acc[8] += w * proj2(uz[0], uz[1], p1r * kz, p1i * kz);
       }
     }
   }
   // block reduction in a fixed tree
-  __shared__ double red[kStatsN][256];
+  __shared__ double red[kStatsN][kStatsThreads];
 #pragma unroll
   for (int i = 0; i < kStatsN; ++i) red[i][threadIdx.x] = acc[i];
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
+  for (int off = kStatsThreads / 2; off > 0; off >>= 1) {
     if (threadIdx.x < off) {
 #pragma unroll
       for (int i = 0; i < kStatsN; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + off];
@@ -852,12 +1096,12 @@ __global__ void k_reduce_partials(const double* __restrict__ partials, int nblk,
 }
 
 template <typename T>
-int launch_stats(const Geo& g, const void* u, const void* nl, const void* ke, double* partials, double* out,
-                 hipStream_t s) {
+int launch_stats(const Geo& g, const void* u, const void* nl, const void* ke, const StatsAux& aux, double* partials,
+                 double* out, hipStream_t s) {
   const size_t n = (size_t)g.sx * g.my * g.mz;
-  int nblk = (int)((n + 255) / 256);
+  int nblk = (int)((n + kStatsThreads - 1) / kStatsThreads);
   if (nblk > kStatsBlocks) nblk = kStatsBlocks;
-  k_stats<T><<<nblk, 256, 0, s>>>(g, (const C<T>*)u, (const C<T>*)nl, (const C<T>*)ke, partials);
+  k_stats<T><<<nblk, kStatsThreads, 0, s>>>(g, (const C<T>*)u, (const C<T>*)nl, (const C<T>*)ke, aux, partials);
   k_reduce_partials<<<1, 256, 0, s>>>(partials, nblk, kStatsN, out);
   return kStatsN;
 }
@@ -1011,10 +1255,13 @@ void launch_forcing_sum(const Geo& g, const void* u, const int* band, int nband,
 }
 
 // ===========================================================================
-// Variable time step (set_dt!, time_step_functions.jl:86-129) with the AB3
-// history shift (Adams_Bashforth.jl:166-176) and coefficients (:178-218).
-// `it` is s.iteration[], `tit` the integrator's own iteration counter.
-__device__ void ab3_coefficients(DevState* st) {
+// Variable time step (set_dt!, time_step_functions.jl:86-129) with the history
+// shift of the integrator (AB3 Adams_Bashforth.jl:166-176 halves the first
+// step; ETD3rdO ETD.jl:110-114 shifts) and the coefficients of the step
+// (AB3 :178-218; ETD3 scalars of set_ABCt! ETD.jl:142-227).  `it` is
+// s.iteration[], `tit` the integrator's own iteration counter.
+__host__ __device__ inline void ab3_coefficients(DevState* st) {
+#pragma clang fp contract(off)
   const double dt = st->dt, dt2 = st->dt2, dt3 = st->dt3;
   double At = 1. + (dt * (2 * dt + 6 * dt2 + 3 * dt3)) / (6 * dt2 * (dt2 + dt3));
   At *= dt;
@@ -1027,7 +1274,32 @@ __device__ void ab3_coefficients(DevState* st) {
   st->Ct = Ct;
 }
 
-__global__ void k_set_dt_ab3(DevState* st, int it, int tit, double cfl, double dx) {
+__host__ __device__ inline void etd_scalars(DevState* st) {
+#pragma clang fp contract(off)
+  const double dt = st->dt, dt2 = st->dt2, dt3 = st->dt3;
+  double* e = st->etd;
+  e[0] = (dt * (2 * dt * dt + 6 * dt2 * (dt2 + dt3) + 3 * dt * (2 * dt2 + dt3))) / (6 * dt2 * (dt2 + dt3));
+  e[1] = (dt * dt * (dt * dt + 6 * dt2 * (dt2 + dt3) + 2 * dt * (2 * dt2 + dt3))) / (12 * dt2 * (dt2 + dt3));
+  e[2] = (dt * dt * dt * (4 * dt2 * (dt2 + dt3) + dt * (2 * dt2 + dt3))) / (24 * dt2 * (dt2 + dt3));
+  e[3] = dt * dt * dt * dt / 24;
+  e[4] = -((dt * dt * (2 * dt + 3 * (dt2 + dt3))) / (6 * (dt2 * dt3)));
+  e[5] = -(((dt * dt * dt * (dt + 2 * (dt2 + dt3)))) / (12 * (dt2 * dt3)));
+  e[6] = -((dt * dt * dt * dt * (dt2 + dt3)) / (24 * (dt2 * dt3)));
+  e[7] = (dt * dt * (2 * dt + 3 * dt2)) / (6 * dt3 * (dt2 + dt3));
+  e[8] = (dt * dt * dt * (dt + 2 * dt2)) / (12 * dt3 * (dt2 + dt3));
+  e[9] = (dt * dt * dt * dt * dt2) / (24 * dt3 * (dt2 + dt3));
+  e[10] = dt * dt + dt * (2 * dt2 + dt3);  // e1
+  e[11] = dt2 * (dt2 + dt3);               // e2
+  e[12] = 2 * dt;                          // e3
+  e[13] = 2 * dt2 + dt3;                   // e4
+  e[14] = dt * (dt + dt2 + dt3);           // e5
+  e[15] = dt2 + dt3;                       // e6
+  e[16] = dt2 * dt3;                       // e7
+  e[17] = dt * (dt + dt2);                 // e8
+  e[18] = dt3 * (dt2 + dt3);               // e9
+}
+
+__global__ void k_set_dt(DevState* st, int it, int tit, double cfl, double dx, int etd, double gnorm) {
   const double umax = __longlong_as_double((long long)st->umax);
   st->umax_d = umax;
   st->umax = 0ull;  // cleared for the next step's atomicMax
@@ -1036,12 +1308,18 @@ __global__ void k_set_dt_ab3(DevState* st, int it, int tit, double cfl, double d
   else if (it == 2) cfl *= 0.4;
   else if (it == 3) cfl *= 0.8;
   double newdt = cfl * dx / umax;
+  if (gnorm > 0) {  // density: newdt = min(newdt, cfl sqrt(dx / (rho_max g)))
+    const double rmax = __longlong_as_double((long long)st->rhomax);
+    st->rhomax_d = rmax;
+    st->rhomax = 0ull;
+    newdt = fmin(newdt, cfl * sqrt(dx / (rmax * gnorm)));
+  }
   if (it > 3) {
     const double dt = st->dt;
     if (newdt < dt) newdt = fmax(dt / 1.25, newdt);
     else newdt = fmin(1.25 * dt, newdt);
   }
-  if (tit == 0) {
+  if (!etd && tit == 0) {
     st->dt3 = newdt / 2;
     st->dt2 = newdt / 2;
     st->dt = newdt / 2;
@@ -1051,10 +1329,16 @@ __global__ void k_set_dt_ab3(DevState* st, int it, int tit, double cfl, double d
     st->dt = newdt;
   }
   ab3_coefficients(st);
+  etd_scalars(st);
 }
 
-void launch_set_dt_ab3(DevState* st, int it, int tit, double cfl, double dx, hipStream_t s) {
-  k_set_dt_ab3<<<1, 1, 0, s>>>(st, it, tit, cfl, dx);
+void launch_set_dt(DevState* st, int it, int tit, double cfl, double dx, int etd, double gnorm, hipStream_t s) {
+  k_set_dt<<<1, 1, 0, s>>>(st, it, tit, cfl, dx, etd, gnorm);
+}
+
+void host_step_coefficients(DevState* st) {
+  ab3_coefficients(st);
+  etd_scalars(st);
 }
 
 __global__ void k_end_step(DevState* st) { st->time += st->dt; }
@@ -1074,17 +1358,19 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
 
 // ---- explicit instantiations ----
 #define FFS_INST(T)                                                                                                   \
-  template void launch_inv_z<T>(const Geo&, const void*, void*, int, bool, const void*, hipStream_t);                 \
+  template void launch_inv_z<T>(const Geo&, const void*, void*, int, bool, const void*, hipStream_t, const void*);    \
   template void launch_inv_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t);                       \
-  template void launch_x_nl<T>(const Geo&, const void*, void*, bool, unsigned long long*, const void*, hipStream_t);  \
+  template void launch_x_nl<T>(const Geo&, const void*, void*, const XOps&, unsigned long long*, unsigned long long*, \
+                               const void*, hipStream_t);                                                           \
   template void launch_x_c2r<T>(const Geo&, const void*, T*, int, const void*, hipStream_t);                          \
   template void launch_x_r2c<T>(const Geo&, const T*, void*, int, const void*, hipStream_t);                          \
   template void launch_fwd_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t);                       \
-  template void launch_fwd_z<T>(const Geo&, const void*, void*, void*, int, int, double, const void*, hipStream_t);          \
-  template void launch_update_ab3<T>(const Geo&, void*, void*, const void*, const void*, const DevState*,             \
-                                     const UpdateArgs&, hipStream_t);                                                 \
+  template void launch_fwd_z<T>(const Geo&, const void*, void* const*, int, int, double, const void*, hipStream_t);   \
+  template void launch_update<T>(const Geo&, void*, void*, const void*, const void*, const DevState*,                 \
+                                 const UpdateArgs&, hipStream_t);                                                     \
   template void launch_project<T>(const Geo&, void*, hipStream_t);                                                   \
-  template int launch_stats<T>(const Geo&, const void*, const void*, const void*, double*, double*, hipStream_t);     \
+  template int launch_stats<T>(const Geo&, const void*, const void*, const void*, const StatsAux&, double*, double*,  \
+                               hipStream_t);                                                                       \
   template void launch_spectrum<T>(const Geo&, const void*, double*, double*, double*, double*, double*, double*,     \
                                    hipStream_t);                                                                      \
   template void launch_forcing_sum<T>(const Geo&, const void*, const int*, int, DevState*, double, hipStream_t);               \

@@ -102,6 +102,16 @@ struct ffs_sim {
   void* fm1 = nullptr;
   void* fm2 = nullptr;
   void* ke = nullptr;
+  void* tau = nullptr;   // Smagorinsky stress, 5 components
+  void* flux = nullptr;  // F(-rho u), 3 components
+  void* rho = nullptr;   // density
+  void* rrhs = nullptr;  // density rhs / history (rotated like the velocity ones)
+  void* rfm1 = nullptr;
+  void* rfm2 = nullptr;
+  bool les = false, dens = false, etd = false;
+  int nin = 6, nfw = 3;     // physical inputs / forward outputs of the nonlinear term (without 1/2|u|^2)
+  double les_alpha = 0.0;   // (c Delta)^2
+  double alpha = 0.0, gz = 0.0, drhodz = 0.0;
   void* w1 = nullptr;
   void* w2 = nullptr;
   void *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr;
@@ -270,8 +280,21 @@ struct ffs_sim {
     HIPCHECK(hipMemset(fm1, 0, 3 * S * csz));
     HIPCHECK(hipMemset(fm2, 0, 3 * S * csz));
     HIPCHECK(hipMemset(ke, 0, S * csz));
-    const size_t a1 = 6 * (size_t)g.sx * g.my * g.nz, b2 = 4 * (size_t)g.sx * g.ny * g.nz;
-    const size_t a2 = 6 * (size_t)g.sx * g.ny * g.nz, b1 = 4 * (size_t)g.sx * g.my * g.nz;
+    if (les) {
+      tau = dalloc(5 * S * csz);
+      HIPCHECK(hipMemset(tau, 0, 5 * S * csz));
+    }
+    if (dens) {
+      flux = dalloc(3 * S * csz);
+      rho = dalloc(S * csz);
+      rrhs = dalloc(S * csz);
+      rfm1 = dalloc(S * csz);
+      rfm2 = dalloc(S * csz);
+      for (void* b : {flux, rho, rrhs, rfm1, rfm2}) HIPCHECK(hipMemset(b, 0, (b == flux ? 3 : 1) * S * csz));
+    }
+    const size_t nmax = (size_t)std::max(nin, nfw + 1);
+    const size_t a1 = nmax * g.sx * g.my * g.nz, b2 = nmax * g.sx * g.ny * g.nz;
+    const size_t a2 = nmax * g.sx * g.ny * g.nz, b1 = nmax * g.sx * g.my * g.nz;
     w1 = dalloc(std::max(a1, b2) * csz);
     w2 = dalloc(std::max(a2, b1) * csz);
     if (fp32) {
@@ -307,35 +330,39 @@ struct ffs_sim {
     DevState h{};
     h.dt = h.dt2 = h.dt3 = p.dt;
     h.time = p.start_time;
-    set_ab3_coefficients(h);
+    host_step_coefficients(&h);
     HIPCHECK(hipMemcpy(st, &h, sizeof(h), hipMemcpyHostToDevice));
     host_time = p.start_time;
     iteration = p.start;
     tit = 0;
   }
 
-  // get_At/get_Bt/get_Ct of Adams_Bashforth.jl:178-218 (variable-step formulas;
-  // with dt = dt2 = dt3 they give 23/12, -16/12, 5/12 times dt)
-  static void set_ab3_coefficients(DevState& h) {
-    const double dt = h.dt, dt2 = h.dt2, dt3 = h.dt3;
-    double At = 1. + (dt * (2 * dt + 6 * dt2 + 3 * dt3)) / (6 * dt2 * (dt2 + dt3));
-    At *= dt;
-    double Bt = -dt * dt * (2 * dt + 3 * dt2 + 3 * dt3) / (6 * dt2 * dt3 * (dt + dt2));
-    Bt *= (dt + dt2);
-    double Ct = dt * dt * (2 * dt + 3 * dt2) / (6 * dt3 * (dt + dt2 + dt3) * (dt2 + dt3));
-    Ct *= (dt + dt2 + dt3);
-    h.At = At;
-    h.Bt = Bt;
-    h.Ct = Ct;
-  }
-
   void create(const ffs_params& in) {
     p = in;
     if (p.size != sizeof(ffs_params)) throw FfsError(FFS_ERR_ARG, "ffs_params.size mismatch (ABI version)");
-    if (p.integrator != FFS_AB3)
-      throw FfsError(FFS_ERR_ARG, "only the Adams_Bashforth3rdO integrator is available in this build");
-    if (p.les_model != FFS_LES_NONE) throw FfsError(FFS_ERR_ARG, "LES models are not available in this build");
-    if (p.density) throw FfsError(FFS_ERR_ARG, "density stratification is not available in this build");
+    if (p.integrator != FFS_AB3 && p.integrator != FFS_ETD3) throw FfsError(FFS_ERR_ARG, "unknown integrator");
+    if (p.les_model != FFS_LES_NONE && p.les_model != FFS_LES_SMAGORINSKY)
+      throw FfsError(FFS_ERR_ARG, "unsupported LES model (Smagorinsky is available)");
+    if (p.forcing != FFS_FORCING_NONE && p.forcing != FFS_FORCING_SHELL)
+      throw FfsError(FFS_ERR_ARG, "unsupported forcing");
+    if (p.precision != FFS_FP64 && p.precision != FFS_FP32) throw FfsError(FFS_ERR_ARG, "unknown precision");
+    etd = (p.integrator == FFS_ETD3);
+    les = (p.les_model == FFS_LES_SMAGORINSKY);
+    dens = (p.density != 0);
+    if (les) {
+      // Delta default (2 pi lx / nx) / cutoff (les_types.jl:366)
+      const double delta = p.filter_width > 0 ? p.filter_width : (p.x_domain * 2 * M_PI / p.nx) / p.cutoff;
+      les_alpha = p.smagorinsky_c * p.smagorinsky_c * delta * delta;
+    }
+    if (dens) {
+      if (!(p.pr > 0) || !(p.reference_density != 0))
+        throw FfsError(FFS_ERR_ARG, "density needs Pr > 0 and referenceDensity != 0");
+      alpha = p.nu / p.pr;
+      gz = p.z_acceleration / p.reference_density;
+      drhodz = p.density_gradient;
+    }
+    nin = 6 + (dens ? 1 : 0) + (les ? 5 : 0);
+    nfw = 3 + (dens ? 3 : 0) + (les ? 5 : 0);
     if (p.dt_stat <= 0 || p.dt_spec <= 0 || p.write_time <= 0)
       throw FfsError(FFS_ERR_ARG, "dtStat, dtSpec and writeTime must be positive");
     if (!p.variable_dt && !(p.dt > 0)) throw FfsError(FFS_ERR_ARG, "a fixed time step needs dt > 0");
@@ -393,7 +420,9 @@ struct ffs_sim {
     if (fp32) launch("x_r2c", [&] { launch_x_r2c<float>(g, (const float*)real, w1, nf, tw_x, stream); });
     else launch("x_r2c", [&] { launch_x_r2c<double>(g, (const double*)real, w1, nf, tw_x, stream); });
     launch("fwd_y", [&] { FFS_DISPATCH(launch_fwd_y, g, w1, w2, nf, tw_y, stream); });
-    launch("fwd_z", [&] { FFS_DISPATCH(launch_fwd_z, g, w2, spec, nullptr, nf, 0, scale, tw_z, stream); });
+    void* dst[kMaxFields] = {};
+    for (int f = 0; f < nf && f < kMaxFields; ++f) dst[f] = cptr(spec, (size_t)f * S);
+    launch("fwd_z", [&] { FFS_DISPATCH(launch_fwd_z, g, w2, dst, nf, 0, scale, tw_z, stream); });
   }
 
   // ---------------------------------------------------------------- the step
@@ -402,21 +431,42 @@ struct ffs_sim {
   bool is_output_time(int64_t it) const { return tit != 0 && it % p.write_time == 0; }
 
   // calculate_rhs! up to (not including) fourierspacep2!: rhs <- dealias(F(u x w))/N^3,
-  // ke <- F(1/2|u|^2) on stats/spectra steps, max|u_i| into st->umax
+  // flux <- F(-rho u), tau <- F(2 (c Delta)^2 |S| S), ke <- F(1/2|u|^2) on stats/spectra steps,
+  // max|u_i| / max|rho| into st (variable dt)
   void nonlinear(bool need_ke) {
-    launch("inv_z_curl", [&] { FFS_DISPATCH(launch_inv_z, g, u, w1, 6, true, tw_z, stream); });
-    launch("inv_y", [&] { FFS_DISPATCH(launch_inv_y, g, w1, w2, 6, tw_y, stream); });
+    launch("inv_z_curl", [&] { FFS_DISPATCH(launch_inv_z, g, u, w1, nin, true, tw_z, stream, rho); });
+    launch("inv_y", [&] { FFS_DISPATCH(launch_inv_y, g, w1, w2, nin, tw_y, stream); });
+    XOps ops{};
+    ops.nin = nin;
+    ops.nout = nfw + (need_ke ? 1 : 0);
+    ops.rho = dens ? 1 : 0;
+    ops.les = les ? 1 : 0;
+    ops.ke = need_ke ? 1 : 0;
+    ops.les_2alpha = 2 * les_alpha;
     unsigned long long* um = p.variable_dt ? &st->umax : nullptr;
-    launch("x_nl", [&] { FFS_DISPATCH(launch_x_nl, g, w2, w1, need_ke, um, tw_x, stream); });
-    const int nf = need_ke ? 4 : 3;
+    unsigned long long* rm = (p.variable_dt && dens) ? &st->rhomax : nullptr;
+    launch("x_nl", [&] { FFS_DISPATCH(launch_x_nl, g, w2, w1, ops, um, rm, tw_x, stream); });
+    const int nf = ops.nout;
     launch("fwd_y", [&] { FFS_DISPATCH(launch_fwd_y, g, w1, w2, nf, tw_y, stream); });
-    launch("fwd_z", [&] { FFS_DISPATCH(launch_fwd_z, g, w2, rhs, ke, nf, 3, scale, tw_z, stream); });
+    void* dst[kMaxFields] = {};
+    int f = 0;
+    for (int c = 0; c < 3; ++c) dst[f++] = cptr(rhs, c * S);
+    if (dens)
+      for (int c = 0; c < 3; ++c) dst[f++] = cptr(flux, c * S);
+    if (les)
+      for (int c = 0; c < 5; ++c) dst[f++] = cptr(tau, c * S);
+    if (need_ke) dst[f++] = ke;
+    launch("fwd_z", [&] { FFS_DISPATCH(launch_fwd_z, g, w2, dst, nf, 3, scale, tw_z, stream); });
   }
 
   void set_dt() {
     if (!p.variable_dt) return;
     const double dx = 2 * M_PI * p.y_domain / p.ny;
-    launch("set_dt", [&] { launch_set_dt_ab3(st, (int)std::min<int64_t>(iteration, 1000), (int)std::min<int64_t>(tit, 1000), p.cfl, dx, stream); });
+    const double gnorm = dens ? std::fabs(gz) : 0.0;
+    launch("set_dt", [&] {
+      launch_set_dt(st, (int)std::min<int64_t>(iteration, 1000), (int)std::min<int64_t>(tit, 1000), p.cfl,
+                    dx, etd ? 1 : 0, gnorm, stream);
+    });
   }
 
   void forcing() {
@@ -441,8 +491,9 @@ struct ffs_sim {
         }
     }
     auto& sl = slots[slot];
-    launch("stats", [&] { FFS_DISPATCH(launch_stats, g, u, rhs, ke, partials, red, stream); });
-    HIPCHECK(hipMemcpyAsync(sl.red, red, 32 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    StatsAux aux{les ? tau : nullptr, dens ? rho : nullptr, gz};
+    launch("stats", [&] { FFS_DISPATCH(launch_stats, g, u, rhs, ke, aux, partials, red, stream); });
+    HIPCHECK(hipMemcpyAsync(sl.red, red, kStatsSlots * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipMemcpyAsync(sl.st, st, sizeof(DevState), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipEventRecord(sl.ev, stream));
     sl.iteration = iteration;
@@ -460,7 +511,7 @@ struct ffs_sim {
     sl.busy = false;
   }
 
-  // stats(s) of stats.jl:64-165 from the device sums
+  // stats(s) of stats.jl:64-165 from the device sums (IEEE division semantics: x/0 = Inf as in Julia)
   Record make_record(const double* a, const DevState& sd, int64_t it) const {
     const double nu = p.nu;
     const double u1 = 0, u2 = 0, u3 = 0;  // real(u[1,1,1]): the k=0 mode is zero (dealiased mean flow)
@@ -472,6 +523,19 @@ struct ffs_sim {
     const double time = sd.time;
     r.stats = {(double)it, time, u1, u2, u3, u12, u22, u32, kh, kv, kaniso, k, eh, ev, e,
                nlh, nlv, pressh, pressv, -(nlh + pressh), (nlv + pressv)};
+    double tdiss = e;
+    std::vector<double> other;
+    if (dens) {  // stats(::AbstractDensityStratification), density_stratification_types.jl:46-87
+      const double rhodiss = alpha * (a[23] + a[24] + a[25]);
+      other = {a[29], a[22], a[23], a[24], a[25], rhodiss, rhodiss, -gz * a[26]};
+    }
+    if (les) {  // les_stats, stats.jl:679-705
+      const double prh = -a[27], prv = -a[28];
+      other.insert(other.end(), {prh, prv, (2 * prv) / prh, prh + prv});
+      tdiss += prh + prv;
+    }
+    r.stats.push_back(tdiss);
+    r.stats.insert(r.stats.end(), other.begin(), other.end());
     const double u1p2 = u12, u2p2 = u22, u3p2 = u32;
     const double uh = std::sqrt((u1p2 + u2p2) / 2);
     const double urms = std::sqrt((u1p2 + u2p2 + u3p2) / 3);
@@ -486,32 +550,65 @@ struct ffs_sim {
     const double ll = std::sqrt(2 * (u1p2 + u2p2) / (du1dxp2 + du2dyp2));
     const double lt = std::sqrt(2 * (u1p2 + u2p2) / (du1dyp2 + du2dxp2));
     const double Lhv = (L13 + L23) / 2, Lvv = L33, Lvh = (L31 + L32) / 2;
-    const double tdiss = e;
-    r.stats.push_back(tdiss);
     const double Lt = std::sqrt(k * k * k) / tdiss, Tt = k / tdiss;
     const double Lk = std::sqrt(std::sqrt(nu * nu * nu / tdiss)), Tk = std::sqrt(nu / tdiss);
     const double lg = std::sqrt(15 * nu / tdiss) * urms;
     r.scales = {(double)it, time, Lk, Lt, lg, ll, lt, Lhh, Lvv, Lhv, Lvh, l12, l21, L11, L12, L13, L21, L22, L23,
-                L31, L32, L33, Tk, Tt, Thh};
+                L31, L32, L33};
     const double Reh = uh * Lhh / nu, Reh1 = std::pow(Lhh / Lk, 4.0 / 3);
     const double Rel = urms * ll / nu, Rel1 = std::pow(ll / Lk, 4.0 / 3);
     const double Ret = urms * lt / nu, Ret1 = std::pow(lt / Lk, 4.0 / 3);
-    r.numbers = {(double)it, time, Reh, Reh1, Rel, Ret, Rel1, Ret1};
+    if (dens) {
+      const double Nb = std::sqrt(gz * drhodz);  // calculate_N
+      const double Lb = 2 * M_PI * urms / Nb, Lo = std::sqrt(tdiss / (Nb * Nb * Nb));
+      const double Lom = std::sqrt(tdiss / (Nb * Nb * Nb / (8 * M_PI * M_PI * M_PI)));
+      const double Tb = 2 * M_PI / Nb, Tbm = 1 / Nb;
+      const double Rlg = urms * lg / nu, Rlg1 = std::pow(lg / Lk, 4.0 / 3);
+      r.scales.insert(r.scales.end(), {Lb, Lo, Lom, Tk, Tt, Thh, Tb, Tbm});
+      r.numbers = {(double)it, time, Reh, Reh1, Rlg, Rel, Ret, Rlg1, Rel1, Ret1, std::pow(Lo / Lk, 4.0 / 3),
+                   std::pow(Lom / Lk, 4.0 / 3), 2 * M_PI * uh / (Nb * Lhh), Lhv / Lhh};
+    } else {
+      r.scales.insert(r.scales.end(), {Tk, Tt, Thh});
+      r.numbers = {(double)it, time, Reh, Reh1, Rel, Ret, Rel1, Ret1};
+    }
     return r;
   }
 
-  void time_step() {
+  UpdateArgs update_args(int step) const {
     UpdateArgs a{};
-    a.mode = (tit == 1) ? 1 : 0;
+    a.integrator = etd ? 1 : 0;
+    a.step = step;
     a.nu = p.nu;
     a.forcing = (p.forcing == FFS_FORCING_SHELL && nband > 0) ? 1 : 0;
     a.kf2 = kf2;
-    launch("update_ab3", [&] { FFS_DISPATCH(launch_update_ab3, g, u, rhs, fm1, fm2, st, a, stream); });
+    a.tau = les ? tau : nullptr;
+    if (dens) {
+      a.rho = rho;
+      a.rrhs = rrhs;
+      a.rfm1 = rfm1;
+      a.rfm2 = rfm2;
+      a.flux = flux;
+      a.alpha = alpha;
+      a.drhodz = drhodz;
+      a.gz = gz;
+    }
+    return a;
+  }
+
+  void time_step() {
+    const UpdateArgs a = update_args((int)std::min<int64_t>(tit + 1, 1 << 20));
+    launch("update", [&] { FFS_DISPATCH(launch_update, g, u, rhs, fm1, fm2, st, a, stream); });
     // history rotation: fm2 <- fm1, fm1 <- rhs, old fm2 becomes the next rhs buffer
     void* old2 = fm2;
     fm2 = fm1;
     fm1 = rhs;
     rhs = old2;
+    if (dens) {
+      void* r2 = rfm2;
+      rfm2 = rfm1;
+      rfm1 = rrhs;
+      rrhs = r2;
+    }
     tit += 1;
     iteration += 1;
     launch("end_step", [&] { launch_end_step(st, stream); });
@@ -535,13 +632,15 @@ struct ffs_sim {
     nonlinear(false);
     set_dt();
     launch("project", [&] { FFS_DISPATCH(launch_project, g, u, stream); });
-    UpdateArgs a{};
-    a.mode = 3;
-    a.nu = p.nu;
-    launch("update_rhs", [&] { FFS_DISPATCH(launch_update_ab3, g, u, rhs, fm1, fm2, st, a, stream); });
-    // initialize!(timestep): fm1 = fm2 = rhs; dt2 = dt3 = dt (Adams_Bashforth.jl:157-164)
+    const UpdateArgs a = update_args(0);
+    launch("update_rhs", [&] { FFS_DISPATCH(launch_update, g, u, rhs, fm1, fm2, st, a, stream); });
+    // initialize!(timestep): fm1 = fm2 = rhs; dt2 = dt3 = dt (Adams_Bashforth.jl:157-164, ETD.jl:99-108)
     HIPCHECK(hipMemcpyAsync(fm1, rhs, 3 * S * csz, hipMemcpyDeviceToDevice, stream));
     HIPCHECK(hipMemcpyAsync(fm2, rhs, 3 * S * csz, hipMemcpyDeviceToDevice, stream));
+    if (dens) {
+      HIPCHECK(hipMemcpyAsync(rfm1, rrhs, S * csz, hipMemcpyDeviceToDevice, stream));
+      HIPCHECK(hipMemcpyAsync(rfm2, rrhs, S * csz, hipMemcpyDeviceToDevice, stream));
+    }
     initialized = true;
   }
 
@@ -629,11 +728,12 @@ int ffs_create(const ffs_params* p, ffs_sim** out) {
 int ffs_set_field(ffs_sim* s, int field, int space, const double* host, size_t n) {
   if (!s || !host) return FFS_ERR_ARG;
   return guard(s, [&] {
-    if (field < FFS_FIELD_U1 || field > FFS_FIELD_U3) throw FfsError(FFS_ERR_ARG, "unknown field");
+    if (field < FFS_FIELD_U1 || field > FFS_FIELD_RHO) throw FfsError(FFS_ERR_ARG, "unknown field");
+    if (field == FFS_FIELD_RHO && !s->dens) throw FfsError(FFS_ERR_ARG, "the run has no density field");
     if (s->initialized) throw FfsError(FFS_ERR_STATE, "fields are set before ffs_initialize");
     const size_t nxh = s->g.nxh, ny = s->g.ny, nz = s->g.nz, nrx = s->g.nrx;
     const size_t padded = 2 * nxh * ny * nz, unpadded = nrx * ny * nz;
-    void* dst = s->cptr(s->u, (size_t)field * s->S);
+    void* dst = field == FFS_FIELD_RHO ? s->rho : s->cptr(s->u, (size_t)field * s->S);
     if (space == FFS_SPACE_REAL) {
       std::vector<double> buf;
       const double* src = host;
@@ -692,11 +792,12 @@ int ffs_set_field(ffs_sim* s, int field, int space, const double* host, size_t n
 int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n) {
   if (!s || !host) return FFS_ERR_ARG;
   return guard(s, [&] {
-    if (field < FFS_FIELD_U1 || field > FFS_FIELD_U3) throw FfsError(FFS_ERR_ARG, "unknown field");
+    if (field < FFS_FIELD_U1 || field > FFS_FIELD_RHO) throw FfsError(FFS_ERR_ARG, "unknown field");
+    if (field == FFS_FIELD_RHO && !s->dens) throw FfsError(FFS_ERR_ARG, "the run has no density field");
     const size_t nxh = s->g.nxh, ny = s->g.ny, nz = s->g.nz;
     const size_t padded = 2 * nxh * ny * nz;
     if (n != padded) throw FfsError(FFS_ERR_IO, "output buffer must hold 2*(nx/2+1)*ny*nz doubles");
-    void* src = s->cptr(s->u, (size_t)field * s->S);
+    void* src = field == FFS_FIELD_RHO ? s->rho : s->cptr(s->u, (size_t)field * s->S);
     if (space == FFS_SPACE_FOURIER) {
       std::vector<double> c(2 * s->S);
       if (s->fp32) {
@@ -799,14 +900,19 @@ int ffs_stats_pop(ffs_sim* s, double* stats, int cap_stats, int* n_stats, double
 int ffs_stats_header(const ffs_sim* s, int table, char* buf, size_t cap) {
   if (!s || !buf || cap == 0) return FFS_ERR_ARG;
   std::string h;
-  if (table == FFS_TABLE_STATS)
+  if (table == FFS_TABLE_STATS) {
     h = "iteration,time,u1,u2,u3,u1p2,u2p2,u3p2,kh,kv,kaniso,k,dissh,dissv,diss,nlh,nlv,pressh,pressv,tnlh,tnlv,tdiss";
-  else if (table == FFS_TABLE_SCALES)
-    h = "iteration,time,Lk,Lt,lamg,laml,lamt,Lhh,Lvv,Lhv,Lvh,lam12,lam21,L11,L12,L13,L21,L22,L23,L31,L32,L33,Tk,Tt,Thh";
-  else if (table == FFS_TABLE_NUMBERS)
+    if (s->dens) h += ",rho,rhop2,drhodxp2,drhodyp2,drhodzp2,rhodiss,trhodiss,bflux";
+    if (s->les) h += ",prh,prv,praniso,pr";
+  } else if (table == FFS_TABLE_SCALES) {
+    h = "iteration,time,Lk,Lt,lamg,laml,lamt,Lhh,Lvv,Lhv,Lvh,lam12,lam21,L11,L12,L13,L21,L22,L23,L31,L32,L33";
+    h += s->dens ? ",Lb,Lo,Lom,Tk,Tt,Thh,Tb,Tbm" : ",Tk,Tt,Thh";
+  } else if (table == FFS_TABLE_NUMBERS) {
     h = "iteration,time,Reh,Reh1,Relamg,Relaml,Relamt,Relamg1,Relaml1,Relamt1";
-  else
+    if (s->dens) h += ",Reb,Rebm,Frh,Frh1";
+  } else {
     return FFS_ERR_ARG;
+  }
   std::snprintf(buf, cap, "%s", h.c_str());
   return h.size() < cap ? FFS_OK : FFS_ERR_ARG;
 }

@@ -88,6 +88,13 @@ class Simulation:
         c = buf[0::2] + 1j * buf[1::2]
         return c.reshape((self.nx // 2 + 1, self.ny, self.nz), order="F")
 
+    def set_density(self, rho):
+        """rho.$start of the Boussinesq approximation (density_stratification_types.jl:165-166)."""
+        self.set_field(_lib.FFS_FIELD_RHO, rho)
+
+    def density_hat(self):
+        return self.get_field(_lib.FFS_FIELD_RHO, _lib.FFS_SPACE_FOURIER)
+
     def velocity_hat(self):
         return [self.get_field(f, _lib.FFS_SPACE_FOURIER) for f in range(3)]
 

@@ -95,6 +95,7 @@ class Grid:
         self.MAXDKH = self.KH[1]
         self.KRZ = rfftfreq(nz, lz)
         self.DKZ = self.KRZ[1]
+        self.cutoff = cutoff
 
     def coords(self):
         """Physical grid points x_i = 2 pi L i / n."""
@@ -150,9 +151,24 @@ class OracleSim:
     (tests/test_oracle.py)."""
 
     def __init__(self, grid, nu, dt=0.0, variable_dt=False, cfl=0.18, dt_stat=1 << 30, dt_spec=1 << 30,
-                 forcing_kf=None, forcing_power=0.1, start=0, start_time=0.0):
+                 forcing_kf=None, forcing_power=0.1, start=0, start_time=0.0, integrator="AB3", les_c=None,
+                 les_delta=None, density=None):
+        """integrator: "AB3" (Adams_Bashforth3rdO) or "ETD3" (ETD3rdO);
+        les_c: Smagorinsky constant (None: no LES), les_delta: filter width
+        (default (2 pi L/nx)/cutoff, les_types.jl:366);
+        density: dict(Pr=, gradient=, g=) for the Boussinesq approximation."""
         self.g = grid
         self.nu = nu
+        self.integrator = integrator
+        self.les_c = les_c
+        if les_c is not None:
+            delta = les_delta if les_delta else (grid.LX * 2 * np.pi / grid.NRX) / grid.cutoff
+            self.les_alpha = les_c * les_c * delta * delta
+        self.density = density
+        if density is not None:
+            self.alpha = nu / density["Pr"]
+            self.drhodz = density["gradient"]
+            self.gz = density["g"]
         self.variable_dt = variable_dt
         self.cfl = cfl
         self.dt = self.dt2 = self.dt3 = dt
@@ -173,13 +189,22 @@ class OracleSim:
         """Real (3, nx, ny, nz) field -> myfourier! of initialize! (FluidFlowSimulation.jl:69)."""
         self.uh = np.array([self.g.myfourier(np.asarray(c, dtype=np.float64)) for c in u])
 
+    def set_density(self, rho):
+        """myfourier!(rho) + dealias! of initialize! (FluidFlowSimulation.jl:77-80)."""
+        self.rhoh = self.g.myfourier(np.asarray(rho, dtype=np.float64))
+
     def initialize(self):
-        """initialize!(s) (FluidFlowSimulation.jl:67-91) for the AB3 velocity path."""
+        """initialize!(s) (FluidFlowSimulation.jl:67-91)."""
+        if self.density is not None and not hasattr(self, "rhoh"):
+            self.rhoh = np.zeros_like(self.uh[0])
         self.calculate_rhs(collect=False)
         self.fm1 = self.rhs.copy()
         self.fm2 = self.fm1.copy()
         self.dt2 = self.dt
         self.dt3 = self.dt
+        if self.density is not None:
+            self.rfm1 = self.rhorhs.copy()
+            self.rfm2 = self.rfm1.copy()
 
     # -------------------------------------------------------------- the step
     def is_stats_time(self):
@@ -192,21 +217,43 @@ class OracleSim:
         g = self.g
         u = self.uh
         kx, ky, kz = g.kx, g.ky, g.kz
-        # fourierspacep1!: rhs = (ik) x u on the retained range
+        kv = (kx, ky, kz)
+        # fourierspacep1!: rhs = (ik) x u; LES: tau = S = symouter(ik, u) (xx, xy, xz, yy, yz)
         w = np.array([1j * (ky * u[2] - kz * u[1]), 1j * (kz * u[0] - kx * u[2]), 1j * (kx * u[1] - ky * u[0])])
         w = np.where(g.RANGE, w, 0)
+        if self.les_c is not None:
+            pairs = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2))
+            S = np.array([np.where(g.RANGE, 0.5 * (1j * kv[a] * u[b] + 1j * kv[b] * u[a]), 0) for a, b in pairs])
         # realspace!
         ur = np.array([g.irfft(c) for c in u])
         wr = np.array([g.irfft(c) for c in w])
         need_k = self.is_spec_time() or self.is_stats_time()
         if need_k:
             self.kspec = g.myfourier(0.5 * (ur[0] * ur[0] + ur[1] * ur[1] + ur[2] * ur[2]))
+        if self.density is not None:
+            rhor = g.irfft(self.rhoh)
+        if self.les_c is not None:
+            Sr = np.array([g.irfft(c) for c in S])
+            # calc_les! Smagorinsky branch (les_real_calc.jl:77-79): tau = 2 (c Delta)^2 |S| S,
+            # |S| = sqrt(2 S:S) with Szz = -Sxx - Syy
+            szz = -Sr[0] - Sr[3]
+            ss = Sr[0] ** 2 + Sr[3] ** 2 + szz**2 + 2 * (Sr[1] ** 2 + Sr[2] ** 2 + Sr[4] ** 2)
+            taur = 2 * (self.les_alpha * np.sqrt(2 * ss)) * Sr
         cross = np.array([ur[1] * wr[2] - ur[2] * wr[1], ur[2] * wr[0] - ur[0] * wr[2], ur[0] * wr[1] - ur[1] * wr[0]])
+        if self.density is not None:
+            fluxr = np.array([(-rhor) * ur[c] for c in range(3)])
         if self.variable_dt:
             self.umax = float(np.max(np.abs(ur)))
+            if self.density is not None:
+                self.rhomax = float(np.max(np.abs(rhor)))
         rhs = np.array([g.myfourier(c) for c in cross])
         rhs[:, 0, 0, 0] = 0
         self.uh = g.project(np.array([g.myfourier(c) for c in ur]))
+        if self.density is not None:
+            fluxh = np.array([g.myfourier(c) for c in fluxr])
+            self.rhoh = g.myfourier(rhor)
+        if self.les_c is not None:
+            self.tauh = np.array([g.myfourier(c) for c in taur])
         if self.variable_dt:
             self.set_dt()
         if collect and self.is_spec_time():
@@ -214,10 +261,30 @@ class OracleSim:
         if collect and self.is_stats_time():
             self.nlstats = self.non_linear_stats(rhs)
             self.pressstats = self.pressure_stats(rhs)
-        # fourierspacep2_velocity!
-        rhs = rhs + (g.K2 * (-self.nu)) * self.uh
+        # fourierspacep2_velocity! (models.jl:292-341)
+        if self.integrator == "AB3":
+            rhs = rhs + (g.K2 * (-self.nu)) * self.uh
+        if self.les_c is not None:
+            rhs = rhs + self.div_tau()
+        if self.density is not None:
+            rhs = rhs.copy()
+            rhs[2] = rhs[2] + self.rhoh * self.gz
         rhs = np.where(g.RANGE, rhs, 0)
         self.rhs = g.project(rhs)
+        if self.density is not None:
+            # div!(rhorhs, flux, u_z, -drhodz) (vectorfunctions.jl:93-103); explicit diffusion is
+            # absent in the reference (models.jl:361), so AB3 density has none
+            r = 1j * (kx * fluxh[0] + (ky * fluxh[1] + kz * fluxh[2])) + (-self.drhodz) * self.uh[2]
+            self.rhorhs = np.where(g.RANGE, r, 0)
+
+    def div_tau(self):
+        """(i k) . tau with tau symmetric traceless (models.jl:330)."""
+        g = self.g
+        t = self.tauh
+        txx, txy, txz, tyy, tyz = t
+        tzz = -txx - tyy
+        return np.array([1j * (g.kx * txx + g.ky * txy + g.kz * txz), 1j * (g.kx * txy + g.ky * tyy + g.kz * tyz),
+                         1j * (g.kx * txz + g.ky * tyz + g.kz * tzz)])
 
     def set_dt(self):
         """set_dt! (time_step_functions.jl:86-129) with the intended fresh max|u_i|."""
@@ -227,13 +294,16 @@ class OracleSim:
         it = self.iteration
         cfl *= {0: 0.1, 1: 0.2, 2: 0.4, 3: 0.8}.get(it, 1.0)
         newdt = cfl * dx / self.umax
+        if self.density is not None:
+            newdt = min(newdt, cfl * math.sqrt(dx / (self.rhomax * abs(self.gz))))
         if it > 3:
             dt = self.dt
             newdt = max(dt / 1.25, newdt) if newdt < dt else min(1.25 * dt, newdt)
-        # Adams_Bashforth3rdO set_dt! (Adams_Bashforth.jl:166-176)
-        if self.tit == 0:
+        if self.integrator == "AB3" and self.tit == 0:
+            # Adams_Bashforth3rdO set_dt! (Adams_Bashforth.jl:166-176)
             self.dt3 = self.dt2 = self.dt = newdt / 2
         else:
+            # ETD3rdO set_dt! (ETD.jl:110-114) and AB3 after its first step
             self.dt3, self.dt2, self.dt = self.dt2, self.dt, newdt
 
     def ab3_coefficients(self):
@@ -257,20 +327,112 @@ class OracleSim:
         self.force_factor = self.forcing_power / e if e > 0 else 0.0
 
     def time_step(self):
-        """VectorTimeStep functor (time_step_functions.jl:72-80) with AB3."""
+        """VectorTimeStep functor (time_step_functions.jl:72-80) and the density
+        integrator (models.jl:471)."""
         g = self.g
         self.tit += 1
-        if self.tit == 2:
-            self.fm2 = -(self.dt3 + self.dt2) * (self.rhs - self.fm1) / self.dt2 + self.rhs
-        At, Bt, Ct = self.ab3_coefficients()
-        u = At * self.rhs + (Bt * self.fm1 + (Ct * self.fm2 + self.uh))
+        if self.integrator == "AB3":
+            u = self.ab3(self.uh, self.rhs, "fm1", "fm2")
+        else:
+            u = self.etd(self.uh, self.rhs, "fm1", "fm2", -self.nu)
         u = np.where(g.RANGE, u, 0)
-        self.fm2 = self.fm1
-        self.fm1 = self.rhs
         if self.forcing_kf is not None:
             band = (g.K2 <= self.forcing_kf**2) & (g.K2 > 0) & g.RANGE
             u = u + np.where(band, self.dt * self.force_factor, 0.0) * u
         self.uh = g.project(u)
+        if self.density is not None:
+            if self.integrator == "AB3":
+                r = self.ab3(self.rhoh, self.rhorhs, "rfm1", "rfm2")
+            else:
+                r = self.etd(self.rhoh, self.rhorhs, "rfm1", "rfm2", -self.alpha)
+            self.rhoh = np.where(g.RANGE, r, 0)
+
+    def ab3(self, u, rhs, n1, n2):
+        """_tAdams_Bashforth3rdO! (Adams_Bashforth.jl:231-256) with fix_fm2! at step 2."""
+        fm1, fm2 = getattr(self, n1), getattr(self, n2)
+        if self.tit == 2:
+            fm2 = -(self.dt3 + self.dt2) * (rhs - fm1) / self.dt2 + rhs
+        At, Bt, Ct = self.ab3_coefficients()
+        out = At * rhs + (Bt * fm1 + (Ct * fm2 + u))
+        setattr(self, n2, fm1)
+        setattr(self, n1, rhs)
+        return out
+
+    def etd_coefficients(self, c):
+        """set_ABCt! (ETD.jl:142-196) with get_ats/bts/cts (:198-227)."""
+        dt, dt2, dt3 = self.dt, self.dt2, self.dt3
+        a0 = (dt * (2 * dt * dt + 6 * dt2 * (dt2 + dt3) + 3 * dt * (2 * dt2 + dt3))) / (6 * dt2 * (dt2 + dt3))
+        a1 = (dt * dt * (dt**2 + 6 * dt2 * (dt2 + dt3) + 2 * dt * (2 * dt2 + dt3))) / (12 * dt2 * (dt2 + dt3))
+        a2 = (dt * dt * dt * (4 * dt2 * (dt2 + dt3) + dt * (2 * dt2 + dt3))) / (24 * dt2 * (dt2 + dt3))
+        a3 = dt * dt * dt * dt / 24
+        b0 = -((dt**2 * (2 * dt + 3 * (dt2 + dt3))) / (6 * (dt2 * dt3)))
+        b1 = -(((dt * dt * dt * (dt + 2 * (dt2 + dt3)))) / (12 * (dt2 * dt3)))
+        b2 = -((dt * dt * dt * dt * (dt2 + dt3)) / (24 * (dt2 * dt3)))
+        c0 = (dt * dt * (2 * dt + 3 * dt2)) / (6 * dt3 * (dt2 + dt3))
+        c1 = (dt * dt * dt * (dt + 2 * dt2)) / (12 * dt3 * (dt2 + dt3))
+        c2 = (dt * dt * dt * dt * dt2) / (24 * dt3 * (dt2 + dt3))
+        e1 = dt**2 + dt * (2 * dt2 + dt3)
+        e2 = dt2 * (dt2 + dt3)
+        e3 = 2 * dt
+        e4 = 2 * dt2 + dt3
+        e5 = dt * (dt + dt2 + dt3)
+        e6 = dt2 + dt3
+        e7 = dt2 * dt3
+        e8 = dt * (dt + dt2)
+        e9 = dt3 * (dt2 + dt3)
+        l1 = c
+        l2 = l1 * l1
+        l3 = l2 * l1
+        ldt = l1 * dt
+        test = -l1 * dt <= 1e-4
+        em = np.expm1(ldt)
+        with np.errstate(all="ignore"):
+            At = np.where(test, a1 * l1 + (a2 * l2 + (a3 * l3 + a0)),
+                          (2 * em - l2 * (e1 - e2 * em) - l1 * (e3 - e4 * em)) / (l3 * e2))
+            Bt = np.where(test, b1 * l1 + (b2 * l2 + b0), (e5 * l2 - 2 * em + l1 * (e3 - e6 * em)) / (l3 * e7))
+            Ct = np.where(test, c1 * l1 + (c2 * l2 + c0), (2 * em - e8 * l2 - l1 * (e3 - dt2 * em)) / (l3 * e9))
+        return At, Bt, Ct
+
+    def etd(self, u, rhs, n1, n2, mnu):
+        """ETD3rdO functor (ETD.jl:229-247) with ETD1stO!/ETD2ndO! (ETD1stand2ndO.jl)
+        for the first two steps; c = -nu k^2 (init_c!, ETD.jl:60-69)."""
+        g = self.g
+        c = np.where(g.RANGE, g.K2 * mnu, 0.0)
+        dt = self.dt
+        fm1, fm2 = getattr(self, n1), getattr(self, n2)
+        with np.errstate(all="ignore"):
+            if self.tit == 1:
+                cdt = c * dt
+                cdt2 = cdt * dt
+                c2dt3 = cdt2 * cdt
+                c3dt4 = c2dt3 * cdt
+                test = -cdt <= 1e-4
+                f = np.where(test, dt + 0.5 * cdt2 + c2dt3 / 6 + c3dt4 / 24, np.expm1(cdt) / c)
+                out = np.exp(cdt) * u + f * rhs
+            elif self.tit == 2:
+                dt2 = self.dt2
+                cp2 = c * c
+                cdt = c * dt
+                cdtp2 = cdt * dt
+                c2dtp3 = cdtp2 * cdt
+                c3dtp4 = c2dtp3 * cdt
+                em = np.expm1(cdt)
+                c2dt2 = c * c * dt2
+                dtp2 = dt * dt
+                dtp3 = dtp2 * dt
+                dtp4 = dtp3 * dt
+                test = -cdt <= 1e-4
+                At = np.where(test, dt + cdtp2 / 2 + c2dtp3 / 6 + c3dtp4 / 24 + dtp2 / (2 * dt2) + (c * dtp3) / (6 * dt2)
+                              + (cp2 * dtp4) / (24 * dt2), (em - c * (dt - dt2 * em)) / c2dt2)
+                Bt = np.where(test, (-(dtp2 / (2 * dt2)) - (c * dtp3) / (6 * dt2) - (cp2 * dtp4) / (24 * dt2)),
+                              (cdt - em) / c2dt2)
+                out = np.exp(cdt) * u + (At * rhs + Bt * fm1)
+            else:
+                At, Bt, Ct = self.etd_coefficients(c)
+                out = At * rhs + (Bt * fm1 + (Ct * fm2 + np.exp(c * dt) * u))
+        setattr(self, n2, fm1)
+        setattr(self, n1, rhs)
+        return out
 
     def advance_in_time(self, n=1):
         for _ in range(n):
@@ -293,10 +455,15 @@ class OracleSim:
         return nlh, nlv
 
     def pressure_stats(self, nl):
-        """pressure_stats (stats.jl:750-814) without LES or density."""
+        """pressure_stats (stats.jl:750-814)."""
         g = self.g
         kv = (g.kx, g.ky, g.kz)
         r = [nl[c] - (1j * self.kspec) * kv[c] for c in range(3)]
+        if self.les_c is not None:
+            dt_ = self.div_tau()
+            r = [r[c] + dt_[c] for c in range(3)]
+        if self.density is not None:
+            r[2] = r[2] + self.rhoh * self.gz
         with np.errstate(invalid="ignore", divide="ignore"):
             p1 = -(g.kx * r[0] + g.ky * r[1] + g.kz * r[2]) / g.K2
         p1[0, 0, 0] = 0
@@ -359,17 +526,53 @@ class OracleSim:
             Lhv = (L13 + L23) / 2
             Lvv = L33
             Lvh = (L31 + L32) / 2
+        other = []
+        if self.density is not None:
+            # stats(::AbstractDensityStratification) (density_stratification_types.jl:46-87)
+            r = self.rhoh
+            er = np.abs(r) ** 2
+            rho = float(r[0, 0, 0].real)
+            rho2 = float(np.sum(W * er))
+            drd = [float(np.sum(W * kk * kk * er)) for kk in (g.kx, g.ky, g.kz)]
+            rhodiss = self.alpha * (drd[0] + drd[1] + drd[2])
+            bf = -self.gz * float(np.sum(W * proj(u[2], r)))  # buoyancy_flux (:192-211)
+            other += [rho, rho2, *drd, rhodiss, rhodiss, bf]
+        if self.les_c is not None:
+            # les_stats (stats.jl:679-705)
+            d = self.div_tau()
+            prh = -float(np.sum(W * (proj(d[0], u[0]) + proj(d[1], u[1]))))
+            prv = -float(np.sum(W * proj(d[2], u[2])))
+            with np.errstate(all="ignore"):
+                other += [prh, prv, np.float64(2 * prv) / np.float64(prh), prh + prv]
+            tdiss = tdiss + (prh + prv)
+        with np.errstate(all="ignore"):
+            tdiss = np.float64(tdiss)
             Lt = np.sqrt(k**3) / tdiss
             Tt = k / tdiss
             Lk = np.sqrt(np.sqrt(nu**3 / tdiss))
             Tk = np.sqrt(nu / tdiss)
             lg = np.sqrt(15 * nu / tdiss) * urms
-        stats_row = [self.iteration, self.time, *vstats, tdiss]
+        stats_row = [self.iteration, self.time, *vstats, tdiss, *other]
         scales = [self.iteration, self.time, Lk, Lt, lg, ll, lt, Lhh, Lvv, Lhv, Lvh, l12, l21, L11, L12, L13, L21,
-                  L22, L23, L31, L32, L33, Tk, Tt, Thh]
+                  L22, L23, L31, L32, L33]
         with np.errstate(all="ignore"):
-            numbers = [self.iteration, self.time, uh * Lhh / nu, (Lhh / Lk) ** (4 / 3), urms * ll / nu,
-                       urms * lt / nu, (ll / Lk) ** (4 / 3), (lt / Lk) ** (4 / 3)]
+            Reh, Reh1 = uh * Lhh / nu, (Lhh / Lk) ** (4 / 3)
+            Rel, Rel1 = urms * ll / nu, (ll / Lk) ** (4 / 3)
+            Ret, Ret1 = urms * lt / nu, (lt / Lk) ** (4 / 3)
+            if self.density is not None:
+                Nb = np.sqrt(np.float64(self.gz * self.drhodz))  # calculate_N
+                Lb = 2 * np.pi * urms / Nb
+                Lo = np.sqrt(tdiss / (Nb * Nb * Nb))
+                Lom = np.sqrt(tdiss / (Nb * Nb * Nb / (8 * np.pi * np.pi * np.pi)))
+                Tb = 2 * np.pi / Nb
+                Tbm = 1 / Nb
+                Rlg, Rlg1 = urms * lg / nu, (lg / Lk) ** (4 / 3)
+                scales += [Lb, Lo, Lom, Tk, Tt, Thh, Tb, Tbm]
+                numbers = [self.iteration, self.time, Reh, Reh1, Rlg, Rel, Ret, Rlg1, Rel1, Ret1, (Lo / Lk) ** (4 / 3),
+                           (Lom / Lk) ** (4 / 3), 2 * np.pi * uh / (Nb * Lhh), Lhv / Lhh]
+            else:
+                scales += [Tk, Tt, Thh]
+                numbers = [self.iteration, self.time, Reh, Reh1, Rel, Ret, Rel1, Ret1]
         return np.array(stats_row), np.array(scales), np.array(numbers)
 
     def spectrum(self):

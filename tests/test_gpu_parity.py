@@ -229,3 +229,138 @@ def test_fp32_bulk_against_fp64_oracle(n):
     assert spec_err <= 1e-5, spec_err
     for (st, _, _), (ost, _, _) in zip(rows, ref.records):
         assert abs(st[11] - ost[11]) <= 1e-6 * ost[11] and abs(st[14] - ost[14]) <= 1e-5 * ost[14]
+
+
+def make_ext(n, nu, dt, u0, integrator="AB3", les_c=None, density=None, rho0=None, variable_dt=False, cfl=0.5,
+             dt_stat=1, precision="fp64"):
+    g = Grid(n, n, n)
+    ref = OracleSim(g, nu, dt=dt, variable_dt=variable_dt, cfl=cfl, dt_stat=dt_stat, integrator=integrator,
+                    les_c=les_c, density=density)
+    ref.set_velocity(u0)
+    if density is not None:
+        ref.set_density(rho0)
+    p = ffs.Params(nx=n, ny=n, nz=n, nu=nu, dt=dt, variable_dt=variable_dt, cfl=cfl, dt_stat=dt_stat,
+                   dt_spec=1 << 30, write_time=1 << 30, precision=precision,
+                   integrator="Adams_Bashforth3rdO" if integrator == "AB3" else "ETD3rdO")
+    if les_c is not None:
+        p.les_model, p.smagorinsky_c = "Smagorinsky", les_c
+    if density is not None:
+        p.density, p.pr, p.density_gradient = True, density["Pr"], density["gradient"]
+        p.z_acceleration, p.reference_density = density["g"], 1.0
+    s = ffs.Simulation(p)
+    s.set_velocity(*u0)
+    if density is not None:
+        s.set_density(rho0)
+    return ref, s
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+# ETD3's closed-form coefficients (set_ABCt!) cancel down to O((c dt)^3) near the series switch
+# |c dt| = 1e-4.  With a fixed dt both sides evaluate them from identical inputs (no FMA contraction
+# on the device) and agree to roundoff.  With a variable dt, dt = cfl dx / max|u| inherits the
+# 1-ulp differences of two FFT implementations, and the closed forms amplify a relative change of
+# dt by up to ~1/(c dt)^2 ~ 1e8: the variable-dt ETD field tolerance is therefore 1e-8.
+ETD_TOL = FIELD_TOL
+ETD_TOL_VARIABLE_DT = 1e-8
+
+
+@pytest.mark.parametrize("variable_dt", [False, True])
+def test_etd3_steps(variable_dt):
+    n = 32
+    print_case(f"ETD3 random field, variable_dt={variable_dt}")
+    ref, s = make_ext(n, 0.01, 0.01, random_spectrum(n, seed=21), integrator="ETD3", variable_dt=variable_dt,
+                      dt_stat=2)
+    with s:
+        ref.initialize()
+        s.initialize()
+        for step in range(6):
+            ref.advance_in_time(1)
+            s.advance_in_time(1)
+            err = rel_field_err(s.velocity_hat(), ref.uh)
+            print(f"etd step {step} err {err:.2e}")
+            assert err < (ETD_TOL_VARIABLE_DT if variable_dt else ETD_TOL * (step + 1)), (step, err)
+            assert abs(s.dt - ref.dt) <= 1e-13 * ref.dt
+        rows = s.pop_stats()
+    for (st, _, _), (ost, _, _) in zip(rows, ref.records):
+        assert abs(st[11] - ost[11]) <= STATS_TOL * ost[11] and abs(st[14] - ost[14]) <= STATS_TOL * ost[14]
+
+
+def test_smagorinsky_les():
+    n = 32
+    print_case("Smagorinsky LES c=0.16, AB3")
+    ref, s = make_ext(n, 1e-3, 0.005, random_spectrum(n, seed=31), les_c=0.16, dt_stat=1)
+    with s:
+        ref.initialize()
+        s.initialize()
+        for step in range(5):
+            ref.advance_in_time(1)
+            s.advance_in_time(1)
+            err = rel_field_err(s.velocity_hat(), ref.uh)
+            assert err < FIELD_TOL * (step + 1), (step, err)
+        rows = s.pop_stats()
+        hdr = s.header(_lib.FFS_TABLE_STATS).split(",")
+    assert hdr[-4:] == ["prh", "prv", "praniso", "pr"]
+    for (st, _, _), (ost, _, _) in zip(rows, ref.records):
+        assert len(st) == len(ost) == len(hdr)
+        for col in (11, 14, 21, 22, 23, 25):  # k, diss, tdiss, prh, prv, pr
+            assert abs(st[col] - ost[col]) <= STATS_TOL * abs(ost[col]) + 1e-14 * abs(ost[14]), (col, st[col], ost[col])
+        assert ost[25] > 0  # the Smagorinsky stress drains energy
+
+
+@pytest.mark.parametrize("integrator", ["AB3", "ETD3"])
+def test_boussinesq(integrator):
+    n = 32
+    print_case(f"Boussinesq N=1 Pr=1, {integrator}")
+    from ffs.initial import random_scalar
+
+    dens = {"Pr": 1.0, "gradient": 1.0, "g": 1.0}
+    ref, s = make_ext(n, 0.005, 0.01, random_spectrum(n, seed=41), integrator=integrator, density=dens,
+                      rho0=random_scalar(n, seed=42, rms=0.1), dt_stat=1)
+    tol = FIELD_TOL if integrator == "AB3" else ETD_TOL
+    with s:
+        ref.initialize()
+        s.initialize()
+        for step in range(5):
+            ref.advance_in_time(1)
+            s.advance_in_time(1)
+            err = rel_field_err(s.velocity_hat(), ref.uh)
+            rerr = rel_err(s.density_hat(), ref.rhoh)
+            assert err < tol * (step + 1) and rerr < tol * (step + 1), (step, err, rerr)
+        rows = s.pop_stats()
+        hdr = s.header(_lib.FFS_TABLE_STATS).split(",")
+        shdr = s.header(_lib.FFS_TABLE_SCALES).split(",")
+        nhdr = s.header(_lib.FFS_TABLE_NUMBERS).split(",")
+    assert hdr[22:] == ["rho", "rhop2", "drhodxp2", "drhodyp2", "drhodzp2", "rhodiss", "trhodiss", "bflux"]
+    for (st, sc, nm), (ost, osc, onm) in zip(rows, ref.records):
+        assert len(st) == len(hdr) and len(sc) == len(shdr) and len(nm) == len(nhdr)
+        for col in (11, 14, 23, 24, 25, 26, 27, 28, 29):
+            assert abs(st[col] - ost[col]) <= STATS_TOL * abs(ost[col]) + 1e-14 * abs(ost[23]), (col, st[col], ost[col])
+        for i in range(2, len(osc)):
+            if np.isfinite(osc[i]) and abs(osc[i]) > 1e-12:
+                assert abs(sc[i] - osc[i]) <= 1e-8 * abs(osc[i]), (i, sc[i], osc[i])
+
+
+def test_boussinesq_variable_dt_fp32():
+    """Stratified run in the FP32 bulk path with ETD3 and variable dt (density CFL limit) against FP64."""
+    n = 32
+    print_case("Boussinesq FP32 ETD3 variable dt")
+    from ffs.initial import random_scalar
+
+    dens = {"Pr": 1.0, "gradient": 1.0, "g": 1.0}
+    u0, r0 = random_spectrum(n, seed=51), random_scalar(n, seed=52, rms=0.01)
+    ref, s = make_ext(n, 1e-3, 0.0, u0, integrator="ETD3", density=dens, rho0=r0, variable_dt=True, dt_stat=4,
+                      precision="fp32")
+    with s:
+        ref.initialize()
+        s.initialize()
+        ref.advance_in_time(12)
+        s.advance_in_time(12)
+        err = rel_field_err(s.velocity_hat(), ref.uh)
+        rerr = rel_err(s.density_hat(), ref.rhoh)
+        assert abs(s.dt - ref.dt) <= 1e-5 * ref.dt
+        assert ref.dt < 0.5 * 2 * np.pi / n / ref.umax + 1e-12  # the CFL limit is active
+    print(f"fp32 stratified: field err {err:.2e} density err {rerr:.2e}")
+    assert err < 2e-5 and rerr < 2e-5, (err, rerr)
