@@ -1,0 +1,62 @@
+"""Taylor-Green 64^3 golden statistics (tests/golden, written by make_golden.py
+from the oracle).  GPU: kinetic energy and dissipation within 1e-10 relative at
+every stats row over 500 steps.  CPU: the fixture is what the oracle computes."""
+import os
+
+import numpy as np
+import pytest
+
+import ffs
+from conftest import REPO, print_case
+
+GOLDEN = os.path.join(REPO, "tests", "golden")
+CASES = {
+    "taylor_green_64_ab3_dt0.05": ("Adams_Bashforth3rdO", 0.05, 21),
+    "taylor_green_64_ab3_dt0.01": ("Adams_Bashforth3rdO", 0.01, 500),
+    "taylor_green_64_etd3_dt0.05": ("ETD3rdO", 0.05, 500),
+}
+
+
+def load(name):
+    return np.genfromtxt(os.path.join(GOLDEN, name + ".csv"), delimiter=",", skip_header=1)
+
+
+def test_fixture_matches_oracle_prefix():
+    import sys
+
+    sys.path.insert(0, GOLDEN)
+    from make_golden import taylor_green_64
+
+    rows = taylor_green_64("AB3", 0.05, 21)
+    np.testing.assert_array_equal(rows, load("taylor_green_64_ab3_dt0.05"))
+
+
+def test_fixture_closed_forms():
+    """Row 0 of every case: k = 1/8 and diss = 6 nu k (Taylor-Green at t = 0)."""
+    for name in CASES:
+        r0 = load(name)[0]
+        assert r0[0] == 0 and abs(r0[11] - 0.125) < 1e-15 and abs(r0[14] - 6 * 0.0333333 * 0.125) < 1e-16
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(CASES))
+def test_taylor_green_64_golden(name):
+    integ, dt, nsteps = CASES[name]
+    print_case(f"golden {name}")
+    ref = load(name)
+    p = ffs.Params(nx=64, ny=64, nz=64, nu=0.0333333, dt=dt, integrator=integ, variable_dt=False, dt_stat=20,
+                   dt_spec=1000000, write_time=1 << 30)
+    from ffs.initial import taylor_green
+
+    with ffs.Simulation(p) as s:
+        s.set_velocity(*taylor_green(64))
+        s.initialize()
+        s.advance_in_time(nsteps)
+        rows = np.array([r[0] for r in s.pop_stats()])
+    assert rows.shape == ref.shape
+    assert np.array_equal(rows[:, 0], ref[:, 0])
+    np.testing.assert_allclose(rows[:, 1], ref[:, 1], rtol=1e-13)
+    for col in (11, 14):  # k, diss
+        err = np.max(np.abs(rows[:, col] - ref[:, col]) / np.abs(ref[:, col]))
+        print(f"{name} col {col}: max rel err {err:.2e}")
+        assert err <= 1e-10

@@ -256,3 +256,108 @@ def test_spectrum_sum_rules():
     assert abs(hV.sum() - (e_i1 + 2 * g.DKZ * (eh - e_i1))) < 1e-14
     # H: h counts every mode twice (its kx bin and its ky bin)
     assert abs(hH.sum() - (e_i1 + 2 * g.MAXDKH * (eh - e_i1))) < 1e-14
+
+
+# ---------------------------------------------------------------- ETD3
+def test_etd_exact_for_linear_decay():
+    """u = (sin y, 0, 0): u x w is a gradient, so the projected rhs vanishes and
+    ETD integrates exp(-nu k^2 t) exactly for any dt (ETD1, ETD2 and ETD3 steps)."""
+    g = Grid(16, 16, 16)
+    x, y, z = g.coords()
+    nu, dt = 0.3, 0.25
+    s = OracleSim(g, nu, dt=dt, integrator="ETD3")
+    s.set_velocity(np.array([np.sin(y), 0 * x, 0 * x]))
+    s.initialize()
+    s.advance_in_time(6)
+    amp = abs(s.uh[0][0, 1, 0]) * 2
+    assert abs(amp - math.exp(-nu * 6 * dt)) < 1e-14
+
+
+def test_etd3_coefficients_constant_forcing_and_continuity():
+    """For y' = c y + f with constant f, sum(At, Bt, Ct) = (e^{c dt} - 1)/c; the
+    series and closed-form branches meet at -c dt = 1e-4 (set_ABCt!, ETD.jl:181)."""
+    s = OracleSim(Grid(8, 8, 8), 0.0, dt=0.01, integrator="ETD3")
+    s.dt, s.dt2, s.dt3 = 0.01, 0.012, 0.009
+    c = np.array([-0.5, -10.0, -200.0, -1e-4 / 0.01 * 0.999, -1e-4 / 0.01 * 1.001])
+    At, Bt, Ct = s.etd_coefficients(c)
+    exact = np.expm1(c * s.dt) / c
+    assert np.allclose(At + Bt + Ct, exact, rtol=1e-9)
+    # the two branches agree at the switch to the accuracy the closed form allows there
+    assert abs(At[3] - At[4]) / abs(At[3]) < 1e-6 and abs(Ct[3] - Ct[4]) / abs(Ct[3]) < 1e-3
+    # constant step, c -> 0: AB3 weights
+    s.dt = s.dt2 = s.dt3 = 0.01
+    A0, B0, C0 = s.etd_coefficients(np.array([0.0]))
+    assert np.allclose([A0[0], B0[0], C0[0]], [23 / 12 * 0.01, -16 / 12 * 0.01, 5 / 12 * 0.01], rtol=1e-14)
+
+
+# ---------------------------------------------------------------- Smagorinsky
+def test_smagorinsky_stress_of_shear_layer():
+    """u = (sin y, 0, 0): S_xy = cos(y)/2, |S| = sqrt(2 S:S) = |cos y|, so
+    tau_xy = 2 (c Delta)^2 |S| S_xy = (c Delta)^2 |cos y| cos y; other components vanish."""
+    g = Grid(32, 32, 32)
+    x, y, z = g.coords()
+    c = 0.16
+    s = OracleSim(g, 0.0, dt=0.01, les_c=c)
+    s.set_velocity(np.array([np.sin(y), 0 * x, 0 * x]))
+    s.initialize()
+    delta = (2 * np.pi / 32) / (2 / 3)
+    expect = g.myfourier((c * delta) ** 2 * np.abs(np.cos(y)) * np.cos(y))
+    assert np.abs(s.tauh[1] - expect).max() < 1e-15
+    for i in (0, 2, 3, 4):
+        assert np.abs(s.tauh[i]).max() < 1e-16
+
+
+def test_smagorinsky_drains_energy():
+    g = Grid(16, 16, 16)
+    from ffs.initial import random_spectrum
+
+    s = OracleSim(g, 0.0, dt=0.001, les_c=0.17, dt_stat=1)
+    s.set_velocity(random_spectrum(16, seed=5))
+    s.initialize()
+    s.advance_in_time(2)
+    st = s.records[-1][0]
+    pr = st[-1]
+    assert pr > 0 and abs(st[21] - (st[14] + pr)) < 1e-15  # tdiss = diss + pr
+
+
+# ---------------------------------------------------------------- Boussinesq
+def test_internal_gravity_wave_frequency():
+    """Linear internal wave with k = (1, 0, 1): omega = N k_h / |k| = 1/sqrt(2) for
+    N = sqrt(g dρ/dz) = 1 (inviscid, small amplitude)."""
+    g = Grid(16, 16, 16)
+    x, y, z = g.coords()
+    eps = 1e-6
+    # rho = eps cos(x + z), u = 0 initially: rho(t) = eps cos(omega t) cos(x + z)
+    dens = {"Pr": 1.0, "gradient": 1.0, "g": 1.0}
+    dt = 0.01
+    s = OracleSim(g, 0.0, dt=dt, density=dens, integrator="AB3")
+    s.set_velocity(np.zeros((3, 16, 16, 16)))
+    s.set_density(eps * np.cos(x + z))
+    s.initialize()
+    steps = 300
+    s.advance_in_time(steps)
+    omega = 1 / math.sqrt(2)
+    amp = 2 * s.rhoh[1, 0, 1].real / eps
+    assert abs(amp - math.cos(omega * steps * dt)) < 2e-3
+    # w and u_x are the only velocity components; div u = 0 keeps u_x = -w
+    assert abs(s.uh[0][1, 0, 1] + s.uh[2][1, 0, 1]) < 1e-12 * eps
+
+
+def test_boussinesq_energy_exchange():
+    """The buoyancy flux bflux = -g <w rho> (buoyancy_flux, density_stratification_types.jl:196)
+    moves kinetic into potential energy: d k/dt = -bflux - diss (unforced, without LES)."""
+    g = Grid(16, 16, 16)
+    from ffs.initial import random_scalar, random_spectrum
+
+    dens = {"Pr": 1.0, "gradient": 1.0, "g": 1.0}
+    dt = 1e-3
+    s = OracleSim(g, 0.01, dt=dt, density=dens, dt_stat=1, integrator="AB3")
+    s.set_velocity(random_spectrum(16, seed=3))
+    s.set_density(random_scalar(16, seed=4, rms=0.3))
+    s.initialize()
+    s.advance_in_time(6)
+    k = [r[0][11] for r in s.records]
+    st = s.records[3][0]
+    bflux, diss = st[29], st[14]
+    dk = (k[4] - k[2]) / (2 * dt)
+    assert abs(dk - (-bflux - diss)) < 2e-3 * max(abs(bflux), diss)
