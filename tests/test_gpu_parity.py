@@ -282,7 +282,8 @@ def test_etd3_steps(variable_dt):
             err = rel_field_err(s.velocity_hat(), ref.uh)
             print(f"etd step {step} err {err:.2e}")
             assert err < (ETD_TOL_VARIABLE_DT if variable_dt else ETD_TOL * (step + 1)), (step, err)
-            assert abs(s.dt - ref.dt) <= 1e-13 * ref.dt
+            # dt = cfl dx / max|u| follows the field difference
+            assert abs(s.dt - ref.dt) <= (1e-9 if variable_dt else 1e-15) * ref.dt
         rows = s.pop_stats()
     for (st, _, _), (ost, _, _) in zip(rows, ref.records):
         assert abs(st[11] - ost[11]) <= STATS_TOL * ost[11] and abs(st[14] - ost[14]) <= STATS_TOL * ost[14]
