@@ -1,0 +1,242 @@
+# FFSHip.jl -- Julia binding of libffs_hip.so (the C ABI of include/ffs/ffs.h).
+#
+# Drop-in for the time-step hot path of FluidFlowSimulation.jl: a simulation
+# object whose initialize!/advance_in_time!/writestats/writeoutput call the
+# MI355X library instead of the Julia/FFTW step, so run_simulation's driver
+# loop (FluidFlowSimulation.jl:31-65), the `global` file and the u1.N/u2.N/u3.N
+# padded field files stay as they are.  Every ccall below mirrors one
+# declaration of ffs.h; tests/test_julia_shim.py checks the correspondence.
+module FFSHip
+
+export FFSSimulation, ffs_params, initialize!, advance_in_time!, writestats, writeoutput, run_simulation
+
+const libffs = get(ENV, "FFS_LIB", joinpath(@__DIR__, "..", "ffs", "libffs_hip.so"))
+
+# ---- ffs_params (field order and types exactly as in ffs.h) ----
+mutable struct ffs_params
+    size::UInt32
+    nx::Int32
+    ny::Int32
+    nz::Int32
+    x_domain::Float64
+    y_domain::Float64
+    z_domain::Float64
+    nu::Float64
+    cutoff::Float64
+    integrator::Int32
+    variable_dt::Int32
+    cfl::Float64
+    dt::Float64
+    start::Int64
+    start_time::Float64
+    dt_stat::Int32
+    dt_spec::Int32
+    write_time::Int32
+    precision::Int32
+    les_model::Int32
+    smagorinsky_c::Float64
+    filter_width::Float64
+    density::Int32
+    pr::Float64
+    density_gradient::Float64
+    z_acceleration::Float64
+    reference_density::Float64
+    forcing::Int32
+    forcing_kf::Float64
+    forcing_power::Float64
+    ffs_params() = new()
+end
+
+const FFS_AB3 = Int32(0)
+const FFS_ETD3 = Int32(1)
+const FFS_SPACE_REAL = Int32(0)
+const FFS_SPACE_FOURIER = Int32(1)
+const FFS_EVENT_OUTPUT = Int32(1)
+
+function check(rc::Integer, h::Ptr{Cvoid})
+    rc < 0 || return rc
+    msg = unsafe_string(ccall((:ffs_last_error, libffs), Cstring, (Ptr{Cvoid},), h))
+    error("libffs_hip error $rc: $msg")
+end
+
+abi_version() = ccall((:ffs_abi_version, libffs), Cint, ())
+
+function default_params()
+    p = ffs_params()
+    check(ccall((:ffs_params_default, libffs), Cint, (Ref{ffs_params},), p), C_NULL)
+    return p
+end
+
+# parameters(d) of simulationtypes.jl:242-365 -> ffs_params
+function params_from_global(d::AbstractDict)
+    p = default_params()
+    num(k) = Float64(eval(Meta.parse(d[k])))
+    p.nx, p.ny, p.nz = parse(Int32, d[:nx]), parse(Int32, d[:ny]), parse(Int32, d[:nz])
+    p.x_domain, p.y_domain, p.z_domain = num(:xDomainSize), num(:yDomainSize), num(:zDomainSize)
+    p.nu = num(:kinematicViscosity)
+    p.dt_stat, p.dt_spec, p.write_time = parse(Int32, d[:dtStat]), parse(Int32, d[:dtSpec]), parse(Int32, d[:writeTime])
+    haskey(d, :start) && (p.start = parse(Int64, d[:start]))
+    haskey(d, :startTime) && (p.start_time = parse(Float64, d[:startTime]))
+    haskey(d, :cutoff) && (p.cutoff = num(:cutoff))
+    p.integrator = get(d, :velocityTimeStep, "ETD3rdO") == "Adams_Bashforth3rdO" ? FFS_AB3 : FFS_ETD3
+    haskey(d, :variableTimestepFlag) && (p.variable_dt = parse(Bool, d[:variableTimestepFlag]))
+    haskey(d, :cfl) && (p.cfl = parse(Float64, d[:cfl]))
+    haskey(d, :dt) && (p.dt = num(:dt))
+    if haskey(d, :lesModel) && d[:lesModel] == "Smagorinsky"
+        p.les_model = Int32(1)
+        haskey(d, :smagorinskyConstant) && (p.smagorinsky_c = num(:smagorinskyConstant))
+        haskey(d, :filterWidth) && (p.filter_width = num(:filterWidth))
+    end
+    if haskey(d, :densityStratification)
+        p.density = Int32(1)
+        p.pr, p.density_gradient = num(:Pr), num(:densityGradient)
+        p.z_acceleration, p.reference_density = num(:zAcceleration), num(:referenceDensity)
+    end
+    return p
+end
+
+mutable struct FFSSimulation
+    h::Ptr{Cvoid}
+    p::ffs_params
+    function FFSSimulation(p::ffs_params)
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:ffs_create, libffs), Cint, (Ref{ffs_params}, Ref{Ptr{Cvoid}}), p, out), C_NULL)
+        s = new(out[], p)
+        finalizer(x -> ccall((:ffs_destroy, libffs), Cvoid, (Ptr{Cvoid},), x.h), s)
+        return s
+    end
+end
+
+padded_length(s::FFSSimulation) = 2 * (div(s.p.nx, 2) + 1) * s.p.ny * s.p.nz
+
+function set_field!(s::FFSSimulation, field::Integer, a::Array{Float64}; space = FFS_SPACE_REAL)
+    GC.@preserve a check(ccall((:ffs_set_field, libffs), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Csize_t),
+                               s.h, field, space, a, length(a)), s.h)
+end
+
+function get_field(s::FFSSimulation, field::Integer; space = FFS_SPACE_REAL)
+    a = Vector{Float64}(undef, padded_length(s))
+    GC.@preserve a check(ccall((:ffs_get_field, libffs), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Csize_t),
+                               s.h, field, space, a, length(a)), s.h)
+    return a
+end
+
+initialize!(s::FFSSimulation) = check(ccall((:ffs_initialize, libffs), Cint, (Ptr{Cvoid},), s.h), s.h)
+
+function advance!(s::FFSSimulation, n::Integer)
+    done = Ref{Int64}(0)
+    ev = Ref{Cint}(0)
+    check(ccall((:ffs_advance, libffs), Cint, (Ptr{Cvoid}, Int64, Ref{Int64}, Ref{Cint}), s.h, n, done, ev), s.h)
+    return done[], ev[]
+end
+
+# advance_in_time!(s) (models.jl:3-17): one step; writes the output files first when due
+function advance_in_time!(s::FFSSimulation)
+    done, ev = advance!(s, 1)
+    if ev == FFS_EVENT_OUTPUT
+        writeoutput(s)
+        advance!(s, 1)
+    end
+    writestats(s)
+    return nothing
+end
+
+writestats_now!(s::FFSSimulation) = check(ccall((:ffs_writestats_now, libffs), Cint, (Ptr{Cvoid},), s.h), s.h)
+
+function stats_header(s::FFSSimulation, table::Integer)
+    buf = Vector{UInt8}(undef, 4096)
+    check(ccall((:ffs_stats_header, libffs), Cint, (Ptr{Cvoid}, Cint, Ptr{UInt8}, Csize_t), s.h, table, buf, 4096), s.h)
+    return unsafe_string(pointer(buf))
+end
+
+# writeheader / writestats (stats.jl:9-62): appends every queued record
+function writestats(s::FFSSimulation)
+    a, b, c = zeros(64), zeros(64), zeros(64)
+    na, nb, nc = Ref{Cint}(0), Ref{Cint}(0), Ref{Cint}(0)
+    while true
+        r = check(ccall((:ffs_stats_pop, libffs), Cint,
+                        (Ptr{Cvoid}, Ptr{Float64}, Cint, Ref{Cint}, Ptr{Float64}, Cint, Ref{Cint}, Ptr{Float64}, Cint,
+                         Ref{Cint}), s.h, a, 64, na, b, 64, nb, c, 64, nc), s.h)
+        r == 0 && return nothing
+        for (name, row, n) in (("Stats.txt", a, na[]), ("Scales.txt", b, nb[]), ("Numbers.txt", c, nc[]))
+            open(name, "a+") do f
+                join(f, (Int(row[1]), row[2:n]..., "\n"), ",", "")
+            end
+        end
+    end
+end
+
+function spectrum(s::FFSSimulation)
+    h, v = zeros(8192), zeros(8192)
+    nh, nv = Ref{Cint}(0), Ref{Cint}(0)
+    check(ccall((:ffs_spectrum, libffs), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint, Ref{Cint}, Ptr{Float64}, Cint, Ref{Cint}),
+                s.h, h, 8192, nh, v, 8192, nv), s.h)
+    return h[1:2nh[]], v[1:2nv[]]
+end
+
+function scalar(s::FFSSimulation, what::Integer)
+    out = Ref{Float64}(0)
+    check(ccall((:ffs_get_scalar, libffs), Cint, (Ptr{Cvoid}, Cint, Ref{Float64}), s.h, what, out), s.h)
+    return out[]
+end
+
+profile_enable!(s::FFSSimulation, on::Bool) =
+    check(ccall((:ffs_profile_enable, libffs), Cint, (Ptr{Cvoid}, Cint), s.h, on), s.h)
+profile_reset!(s::FFSSimulation) = check(ccall((:ffs_profile_reset, libffs), Cint, (Ptr{Cvoid},), s.h), s.h)
+
+function profile(s::FFSSimulation)
+    names = Vector{Cstring}(undef, 64)
+    ms = zeros(64)
+    cnt = zeros(Int64, 64)
+    n = Ref{Cint}(0)
+    check(ccall((:ffs_profile_get, libffs), Cint, (Ptr{Cvoid}, Ptr{Cstring}, Ptr{Float64}, Ptr{Int64}, Cint, Ref{Cint}),
+                s.h, names, ms, cnt, 64, n), s.h)
+    return Dict(unsafe_string(names[i]) => (ms[i], cnt[i]) for i in 1:min(n[], 64))
+end
+
+function device_bytes(s::FFSSimulation)
+    b = Ref{Csize_t}(0)
+    check(ccall((:ffs_device_bytes, libffs), Cint, (Ptr{Cvoid}, Ref{Csize_t}), s.h, b), s.h)
+    return Int(b[])
+end
+
+synchronize(s::FFSSimulation) = check(ccall((:ffs_synchronize, libffs), Cint, (Ptr{Cvoid},), s.h), s.h)
+
+# writeoutput (output.jl:1-8): u1.$it u2.$it u3.$it in the padded real layout
+function writeoutput(s::FFSSimulation)
+    it = Int(scalar(s, 0))
+    for f in 0:2
+        write("u$(f + 1).$it", get_field(s, f))
+    end
+end
+
+# run_simulation() (FluidFlowSimulation.jl:23-65) over the library
+function run_simulation(d::AbstractDict)
+    p = params_from_global(d)
+    s = FFSSimulation(p)
+    for f in 0:2
+        a = Vector{Float64}(undef, filesize("u$(f + 1).$(p.start)") ÷ 8)
+        read!("u$(f + 1).$(p.start)", a)
+        set_field!(s, f, a)
+    end
+    nt = haskey(d, :nt) ? parse(Int, d[:nt]) : typemax(Int) - p.start
+    if p.start == 0
+        for (name, t) in (("Stats.txt", 0), ("Scales.txt", 1), ("Numbers.txt", 2))
+            open(f -> write(f, stats_header(s, t), "\n"), name, "w")
+        end
+    end
+    initialize!(s)
+    left = nt
+    while left > 0
+        done, ev = advance!(s, left)
+        left -= done
+        ev == FFS_EVENT_OUTPUT && writeoutput(s)
+        writestats(s)
+    end
+    writestats_now!(s)
+    writestats(s)
+    mod(Int(scalar(s, 0)), p.write_time) == 0 || writeoutput(s)
+    return s
+end
+
+end # module

@@ -1,0 +1,57 @@
+"""The Julia binding (fluidflowsimulation.jl_amd/julia/FFSHip.jl) is source
+only -- no Julia exists on either machine -- so this checks it against the C
+header: every declared function is ccall'ed with the right argument count,
+and the ffs_params mirror lists the C fields in order with matching types."""
+import os
+import re
+
+from conftest import PKG, REPO
+from test_abi import declared_functions
+
+SHIM = os.path.join(PKG, "julia", "FFSHip.jl")
+HEADER = os.path.join(REPO, "include", "ffs", "ffs.h")
+
+C_TO_JL = {"uint32_t": "UInt32", "int32_t": "Int32", "int64_t": "Int64", "double": "Float64"}
+
+
+def header_decls():
+    src = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"^\s*(?:const\s+)?[\w\s\*]+?\b(ffs_\w+)\s*\(([^)]*)\)\s*;", src, flags=re.M):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def shim_calls():
+    src = open(SHIM).read()
+    out = {}
+    for m in re.finditer(r"ccall\(\(:(ffs_\w+), libffs\),\s*\w+(?:\{\w+\})?,\s*\(([^()]*(?:\([^()]*\))?[^()]*)\)", src):
+        types = m.group(2).strip().rstrip(",")
+        out[m.group(1)] = 0 if not types else len([t for t in types.split(",") if t.strip()])
+    return out
+
+
+def test_every_header_function_is_bound_with_matching_arity():
+    decl = header_decls()
+    calls = shim_calls()
+    assert set(decl) == set(declared_functions())
+    assert set(calls) == set(decl), set(decl) ^ set(calls)
+    for name, n in decl.items():
+        assert calls[name] == n, (name, calls[name], n)
+
+
+def test_params_struct_mirror():
+    src = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"typedef struct ffs_params \{(.*?)\} ffs_params;", src, flags=re.S).group(1)
+    cfields = []
+    for line in body.split(";"):
+        line = line.strip()
+        if not line:
+            continue
+        ctype, names = line.split(None, 1)
+        cfields += [(n.strip(), C_TO_JL[ctype]) for n in names.split(",")]
+    jl = open(SHIM).read()
+    jbody = re.search(r"mutable struct ffs_params\n(.*?)\n    ffs_params\(\) = new\(\)", jl, flags=re.S).group(1)
+    jfields = [tuple(x.strip().split("::")) for x in jbody.strip().splitlines()]
+    assert jfields == cfields
