@@ -50,7 +50,7 @@ def kernel_bytes(g, name, elem, with_ke_frac):
         "x_nl": (6 * S2 + nfw * S2) * elem,
         "fwd_y": (nfw * S2 + nfw * S1) * elem,
         "fwd_z": (nfw * S1 + nfw * S) * elem,
-        "update_ab3": (12 * S + 6 * S) * elem,
+        "update": (12 * S + 6 * S) * elem,
     }.get(name)
 
 
@@ -188,7 +188,7 @@ def bench(args):
     step_bytes = 54 * F + 15 * R
     step_eff = step_bytes / (ms_per_step * 1e-3) / 1e9
     step_actual = sum(kernel_bytes(g, k, 2 * b, ke_frac) for k in ("inv_z_curl", "inv_y", "x_nl", "fwd_y", "fwd_z",
-                                                                  "update_ab3"))
+                                                                  "update"))
     log(f"dominant kernel {dom}: {d['avg_us']:.1f} us, {achieved:.0f} GB/s; step model {step_eff:.0f} GB/s effective")
     for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["avg_us"] * kv[1]["launches"]):
         log(f"  {k:12s} {v}")

@@ -95,6 +95,8 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) 
     for (int i = 1; i < nw; ++i) m = nanmax(m, red[i]);
     atomicMax(dst, (unsigned long long)__double_as_longlong(m));
   }
+  // `red` is one array per kernel: a following call must not overwrite it before thread 0 has read it
+  __syncthreads();
 }
 
 // ===========================================================================

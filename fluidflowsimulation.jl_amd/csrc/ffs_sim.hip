@@ -11,6 +11,7 @@
 #include "ffs/ffs.h"
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <deque>
@@ -135,6 +136,14 @@ struct ffs_sim {
   std::deque<Record> ready;
   int nstats = 0;
 
+  // steady-state graphs: one per history-rotation phase
+  bool graph_on = true;
+  int phase = 0;  // history rotations mod 3
+  hipGraph_t graphs[3] = {};
+  hipGraphExec_t gexec[3] = {};
+  // dry run: sizes only (ffs_footprint)
+  bool dry = false;
+
   // profiling
   bool prof = false;
   struct ProfEv {
@@ -151,6 +160,10 @@ struct ffs_sim {
   void* dalloc(size_t bytes) {
     void* ptr = nullptr;
     if (bytes == 0) bytes = 256;
+    if (dry) {  // footprint only: distinct non-null placeholders, nothing allocated
+      dev_bytes += bytes;
+      return (void*)(uintptr_t)(dev_bytes);
+    }
     hipError_t e = hipMalloc(&ptr, bytes);
     if (e != hipSuccess)
       throw FfsError(FFS_ERR_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
@@ -162,6 +175,7 @@ struct ffs_sim {
   template <typename V>
   V* upload(const std::vector<V>& v) {
     V* d = (V*)dalloc(v.size() * sizeof(V));
+    if (dry) return d;
     HIPCHECK(hipMemcpy(d, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice));
     return d;
   }
@@ -275,14 +289,16 @@ struct ffs_sim {
     fm1 = dalloc(3 * S * csz);
     fm2 = dalloc(3 * S * csz);
     ke = dalloc(S * csz);
-    HIPCHECK(hipMemset(u, 0, 3 * S * csz));
-    HIPCHECK(hipMemset(rhs, 0, 3 * S * csz));
-    HIPCHECK(hipMemset(fm1, 0, 3 * S * csz));
-    HIPCHECK(hipMemset(fm2, 0, 3 * S * csz));
-    HIPCHECK(hipMemset(ke, 0, S * csz));
+    if (!dry) {
+      HIPCHECK(hipMemset(u, 0, 3 * S * csz));
+      HIPCHECK(hipMemset(rhs, 0, 3 * S * csz));
+      HIPCHECK(hipMemset(fm1, 0, 3 * S * csz));
+      HIPCHECK(hipMemset(fm2, 0, 3 * S * csz));
+      HIPCHECK(hipMemset(ke, 0, S * csz));
+    }
     if (les) {
       tau = dalloc(5 * S * csz);
-      HIPCHECK(hipMemset(tau, 0, 5 * S * csz));
+      if (!dry) HIPCHECK(hipMemset(tau, 0, 5 * S * csz));
     }
     if (dens) {
       flux = dalloc(3 * S * csz);
@@ -290,7 +306,8 @@ struct ffs_sim {
       rrhs = dalloc(S * csz);
       rfm1 = dalloc(S * csz);
       rfm2 = dalloc(S * csz);
-      for (void* b : {flux, rho, rrhs, rfm1, rfm2}) HIPCHECK(hipMemset(b, 0, (b == flux ? 3 : 1) * S * csz));
+      if (!dry)
+        for (void* b : {flux, rho, rrhs, rfm1, rfm2}) HIPCHECK(hipMemset(b, 0, (b == flux ? 3 : 1) * S * csz));
     }
     const size_t nmax = (size_t)std::max(nin, nfw + 1);
     const size_t a1 = nmax * g.sx * g.my * g.nz, b2 = nmax * g.sx * g.ny * g.nz;
@@ -367,6 +384,13 @@ struct ffs_sim {
       throw FfsError(FFS_ERR_ARG, "dtStat, dtSpec and writeTime must be positive");
     if (!p.variable_dt && !(p.dt > 0)) throw FfsError(FFS_ERR_ARG, "a fixed time step needs dt > 0");
     fp32 = (p.precision == FFS_FP32);
+    if (dry) {  // footprint: the same allocations, counted, plus the transient staging of set_field
+      setup_grid();
+      setup_buffers();
+      const size_t padded = 2 * (size_t)(p.nx / 2 + 1) * p.ny * p.nz;
+      dev_bytes += padded * (sizeof(double) + (fp32 ? sizeof(float) : 0));
+      return;
+    }
     HIPCHECK(hipGetDevice(&device));
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     setup_grid();
@@ -380,8 +404,22 @@ struct ffs_sim {
     }
   }
 
+  void drop_graphs() {
+    for (int i = 0; i < 3; ++i) {
+      if (gexec[i]) hipGraphExecDestroy(gexec[i]);
+      if (graphs[i]) hipGraphDestroy(graphs[i]);
+      gexec[i] = nullptr;
+      graphs[i] = nullptr;
+    }
+  }
+
   void destroy() {
+    if (dry) {
+      allocs.clear();
+      return;
+    }
     if (stream) hipStreamSynchronize(stream);
+    drop_graphs();
     for (auto& e : prof_pending) {
       hipEventDestroy(e.a);
       hipEventDestroy(e.b);
@@ -464,8 +502,9 @@ struct ffs_sim {
     const double dx = 2 * M_PI * p.y_domain / p.ny;
     const double gnorm = dens ? std::fabs(gz) : 0.0;
     launch("set_dt", [&] {
-      launch_set_dt(st, (int)std::min<int64_t>(iteration, 1000), (int)std::min<int64_t>(tit, 1000), p.cfl,
-                    dx, etd ? 1 : 0, gnorm, stream);
+      // it matters only through it <= 3 (cfl ramp) and tit through tit == 0 (AB3 halving)
+      launch_set_dt(st, (int)std::min<int64_t>(iteration, 4), (int)std::min<int64_t>(tit, 1), p.cfl, dx,
+                    etd ? 1 : 0, gnorm, stream);
     });
   }
 
@@ -595,10 +634,16 @@ struct ffs_sim {
     return a;
   }
 
-  void time_step() {
-    const UpdateArgs a = update_args((int)std::min<int64_t>(tit + 1, 1 << 20));
+  // the update kernel of one step; the integrator step enters as min(tit + 1, 3) (AB3 distinguishes
+  // step 2, ETD steps 1, 2 and >= 3)
+  void enqueue_update() {
+    const UpdateArgs a = update_args((int)std::min<int64_t>(tit + 1, 3));
     launch("update", [&] { FFS_DISPATCH(launch_update, g, u, rhs, fm1, fm2, st, a, stream); });
-    // history rotation: fm2 <- fm1, fm1 <- rhs, old fm2 becomes the next rhs buffer
+    launch("end_step", [&] { launch_end_step(st, stream); });
+  }
+
+  // host bookkeeping after a step: fm2 <- fm1, fm1 <- rhs, old fm2 becomes the next rhs buffer
+  void rotate() {
     void* old2 = fm2;
     fm2 = fm1;
     fm1 = rhs;
@@ -611,11 +656,44 @@ struct ffs_sim {
     }
     tit += 1;
     iteration += 1;
-    launch("end_step", [&] { launch_end_step(st, stream); });
+    phase = (phase + 1) % 3;
     if (!p.variable_dt) host_time += p.dt;
   }
 
+  void time_step() {
+    enqueue_update();
+    rotate();
+  }
+
+  // a step whose kernel arguments equal those of every later non-stats step of the same phase
+  bool steady() const {
+    return tit >= 2 && iteration >= 4 && !is_stats_time(iteration) && !is_spec_time(iteration) && !prof;
+  }
+
+  // kernels of one steady step (no host bookkeeping), for capture
+  void enqueue_steady_step() {
+    nonlinear(false);
+    set_dt();
+    forcing();
+    enqueue_update();
+  }
+
+  hipGraphExec_t graph_for_phase() {
+    if (!gexec[phase]) {
+      HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      enqueue_steady_step();
+      HIPCHECK(hipStreamEndCapture(stream, &graphs[phase]));
+      HIPCHECK(hipGraphInstantiate(&gexec[phase], graphs[phase], nullptr, nullptr, 0));
+    }
+    return gexec[phase];
+  }
+
   void advance_one() {
+    if (graph_on && steady()) {
+      HIPCHECK(hipGraphLaunch(graph_for_phase(), stream));
+      rotate();
+      return;
+    }
     const bool stats_t = is_stats_time(iteration);
     const bool spec_t = is_spec_time(iteration);
     nonlinear(stats_t || spec_t);
@@ -648,6 +726,134 @@ struct ffs_sim {
     nonlinear(true);
     forcing();
     queue_stats();
+  }
+};
+
+// An ensemble advanced in lockstep.  Steps in which every run is steady replay one graph whose
+// branches (forked from `lead` by an event, joined back by one event per run) are the runs' steady
+// steps; the branch structure is fixed by the runs' rotation phases, so graphs are cached per phase
+// signature.  Ordering against each run's own stream: before a graph launch `lead` waits for every
+// run's stream, and before a run's stream is used again it waits for `lead`.
+struct ffs_batch {
+  std::vector<ffs_sim*> runs;
+  hipStream_t lead = nullptr;
+  hipEvent_t fork = nullptr, join_lead = nullptr;
+  std::vector<hipEvent_t> joins;
+  struct Cached {
+    std::vector<int> sig;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<Cached> cache;
+  bool lead_behind = true;    // lead must wait for the runs' streams before the next graph
+  bool runs_behind = false;   // the runs' streams must wait for lead before their next work
+  std::string err;
+
+  void create(ffs_sim* const* sims, int n) {
+    if (n <= 0 || n > FFS_MAX_BATCH) throw FfsError(FFS_ERR_ARG, "batch size must be 1.." + std::to_string(FFS_MAX_BATCH));
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    for (int i = 0; i < n; ++i) {
+      ffs_sim* s = sims[i];
+      if (!s) throw FfsError(FFS_ERR_ARG, "null handle in batch");
+      if (!s->initialized) throw FfsError(FFS_ERR_STATE, "every run must be initialized before batching");
+      if (s->device != dev) throw FfsError(FFS_ERR_ARG, "every run of a batch must live on the current device");
+      for (int j = 0; j < i; ++j)
+        if (sims[j] == s) throw FfsError(FFS_ERR_ARG, "a handle appears twice in the batch");
+      runs.push_back(s);
+    }
+    HIPCHECK(hipStreamCreateWithFlags(&lead, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&join_lead, hipEventDisableTiming));
+    joins.resize(n);
+    for (auto& e : joins) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+
+  void destroy() {
+    if (lead) hipStreamSynchronize(lead);
+    for (auto& c : cache) {
+      if (c.exec) hipGraphExecDestroy(c.exec);
+      if (c.graph) hipGraphDestroy(c.graph);
+    }
+    cache.clear();
+    for (auto e : joins)
+      if (e) hipEventDestroy(e);
+    if (fork) hipEventDestroy(fork);
+    if (join_lead) hipEventDestroy(join_lead);
+    if (lead) hipStreamDestroy(lead);
+    lead = nullptr;
+  }
+
+  std::vector<int> signature() const {
+    std::vector<int> sig(runs.size());
+    for (size_t i = 0; i < runs.size(); ++i) sig[i] = runs[i]->phase;
+    return sig;
+  }
+
+  hipGraphExec_t graph_for(const std::vector<int>& sig) {
+    for (auto& c : cache)
+      if (c.sig == sig) return c.exec;
+    Cached c;
+    c.sig = sig;
+    HIPCHECK(hipStreamBeginCapture(lead, hipStreamCaptureModeThreadLocal));
+    HIPCHECK(hipEventRecord(fork, lead));
+    for (size_t i = 0; i < runs.size(); ++i) {
+      HIPCHECK(hipStreamWaitEvent(runs[i]->stream, fork, 0));
+      runs[i]->enqueue_steady_step();
+      HIPCHECK(hipEventRecord(joins[i], runs[i]->stream));
+      HIPCHECK(hipStreamWaitEvent(lead, joins[i], 0));
+    }
+    HIPCHECK(hipStreamEndCapture(lead, &c.graph));
+    HIPCHECK(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
+    cache.push_back(c);
+    return cache.back().exec;
+  }
+
+  // lead waits for the work already queued on every run's stream
+  void lead_waits_runs() {
+    for (size_t i = 0; i < runs.size(); ++i) {
+      HIPCHECK(hipEventRecord(joins[i], runs[i]->stream));
+      HIPCHECK(hipStreamWaitEvent(lead, joins[i], 0));
+    }
+    lead_behind = false;
+  }
+
+  // every run's stream waits for the graphs already queued on lead
+  void runs_wait_lead() {
+    HIPCHECK(hipEventRecord(join_lead, lead));
+    for (auto* r : runs) HIPCHECK(hipStreamWaitEvent(r->stream, join_lead, 0));
+    runs_behind = false;
+  }
+
+  void advance(int64_t nsteps, int64_t* done, int* event, int* who) {
+    lead_behind = true;  // the runs may have been used on their own since the last call
+    for (int64_t k = 0; k < nsteps; ++k) {
+      for (size_t i = 0; i < runs.size(); ++i) {
+        ffs_sim* r = runs[i];
+        if (r->is_output_time(r->iteration) && r->output_served != r->iteration) {
+          for (auto* q : runs)  // every run due at this step is served by the one event
+            if (q->is_output_time(q->iteration)) q->output_served = q->iteration;
+          if (runs_behind) runs_wait_lead();
+          if (event) *event = FFS_EVENT_OUTPUT;
+          if (who) *who = (int)i;
+          return;
+        }
+      }
+      bool all_steady = true;
+      for (auto* r : runs) all_steady = all_steady && r->steady();
+      if (all_steady) {
+        if (lead_behind) lead_waits_runs();
+        HIPCHECK(hipGraphLaunch(graph_for(signature()), lead));
+        for (auto* r : runs) r->rotate();
+        runs_behind = true;
+      } else {
+        if (runs_behind) runs_wait_lead();
+        for (auto* r : runs) r->advance_one();
+        lead_behind = true;
+      }
+      if (done) *done += 1;
+    }
+    if (runs_behind) runs_wait_lead();
   }
 };
 
@@ -1036,6 +1242,107 @@ int ffs_device_bytes(const ffs_sim* s, size_t* bytes) {
 int ffs_synchronize(ffs_sim* s) {
   if (!s) return FFS_ERR_ARG;
   return guard(s, [&] { HIPCHECK(hipStreamSynchronize(s->stream)); });
+}
+
+int ffs_footprint(const ffs_params* p, size_t* bytes) {
+  if (!p || !bytes) return FFS_ERR_ARG;
+  ffs_sim tmp;
+  tmp.dry = true;
+  const int rc = guard(&tmp, [&] { tmp.create(*p); });
+  if (rc != FFS_OK) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_last_create_error = tmp.err;
+    return rc;
+  }
+  *bytes = tmp.dev_bytes;
+  tmp.destroy();
+  return FFS_OK;
+}
+
+int ffs_device_memory(size_t* free_bytes, size_t* total_bytes) {
+  size_t f = 0, t = 0;
+  const hipError_t e = hipMemGetInfo(&f, &t);
+  if (e != hipSuccess) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_last_create_error = std::string("hipMemGetInfo: ") + hipGetErrorString(e);
+    return FFS_ERR_HIP;
+  }
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return FFS_OK;
+}
+
+int ffs_set_graph(ffs_sim* s, int on) {
+  if (!s) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    s->graph_on = on != 0;
+    if (!s->graph_on) {
+      HIPCHECK(hipStreamSynchronize(s->stream));
+      s->drop_graphs();
+    }
+  });
+}
+
+int ffs_batch_create(ffs_sim* const* sims, int n, ffs_batch** out) {
+  if (!sims || !out) return FFS_ERR_ARG;
+  *out = nullptr;
+  auto* b = new ffs_batch();
+  try {
+    b->create(sims, n);
+  } catch (const FfsError& e) {
+    {
+      std::lock_guard<std::mutex> lk(g_err_mu);
+      g_last_create_error = e.what();
+    }
+    b->destroy();
+    delete b;
+    return e.code;
+  }
+  *out = b;
+  return FFS_OK;
+}
+
+int ffs_batch_advance(ffs_batch* b, int64_t nsteps, int64_t* done, int* event, int* who) {
+  if (!b) return FFS_ERR_ARG;
+  if (done) *done = 0;
+  if (event) *event = FFS_EVENT_NONE;
+  if (who) *who = -1;
+  try {
+    b->advance(nsteps, done, event, who);
+    return FFS_OK;
+  } catch (const FfsError& e) {
+    b->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    b->err = e.what();
+    return FFS_ERR_ARG;
+  }
+}
+
+int ffs_batch_synchronize(ffs_batch* b) {
+  if (!b) return FFS_ERR_ARG;
+  try {
+    HIPCHECK(hipStreamSynchronize(b->lead));
+    for (auto* r : b->runs) HIPCHECK(hipStreamSynchronize(r->stream));
+    return FFS_OK;
+  } catch (const FfsError& e) {
+    b->err = e.what();
+    return e.code;
+  }
+}
+
+const char* ffs_batch_last_error(const ffs_batch* b) {
+  if (!b) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_last_create_error.c_str();
+  }
+  return b->err.c_str();
+}
+
+void ffs_batch_destroy(ffs_batch* b) {
+  if (!b) return;
+  b->destroy();
+  delete b;
 }
 
 const char* ffs_last_error(const ffs_sim* s) {

@@ -7,11 +7,14 @@ reference's driver API.
 from . import _lib
 from .config import Params, parameters, parse_expr, readglobal
 from .fields import pad, padded_shape, read_field, unpad, write_field
-from .simulation import Simulation, run_simulation
+from .ensemble import Ensemble
+from .simulation import Simulation, footprint, run_simulation
 
 __all__ = [
+    "Ensemble",
     "Params",
     "Simulation",
+    "footprint",
     "parameters",
     "parse_expr",
     "readglobal",

@@ -28,7 +28,7 @@ CFLAGS = [
     "-Wno-unused-result",
     "-Wno-unused-command-line-argument",
     f"-I{INCLUDE}",
-    # packed FP32 instructions stay off (see the build notes in DESIGN.md)
+    # packed FP32 instructions stay off: some of their forms have produced wrong results on gfx950
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 

@@ -90,6 +90,14 @@ SIGNATURES = {
     "ffs_profile_reset": (ctypes.c_int, [_P]),
     "ffs_device_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     "ffs_synchronize": (ctypes.c_int, [_P]),
+    "ffs_footprint": (ctypes.c_int, [ctypes.POINTER(FfsParams), ctypes.POINTER(ctypes.c_size_t)]),
+    "ffs_device_memory": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+    "ffs_set_graph": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ffs_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(_P)]),
+    "ffs_batch_advance": (ctypes.c_int, [_P, ctypes.c_int64, _I64, _I, _I]),
+    "ffs_batch_synchronize": (ctypes.c_int, [_P]),
+    "ffs_batch_last_error": (ctypes.c_char_p, [_P]),
+    "ffs_batch_destroy": (None, [_P]),
     "ffs_last_error": (ctypes.c_char_p, [_P]),
     "ffs_destroy": (None, [_P]),
 }

@@ -6,7 +6,10 @@ random_scalar: Gaussian random scalar with the same spectral shape, given rms.
 taylor_green: u1 = cos x sin y sin z, u2 = -sin x cos y sin z, u3 = 0.
 All return real arrays on the grid x_i = 2 pi L i / n (index order x, y, z).
 """
+import os
+
 import numpy as np
+import scipy.fft
 
 __all__ = ["taylor_green", "random_spectrum", "random_scalar"]
 
@@ -38,12 +41,15 @@ def _dealias_mask(nx, ny, nz, lx, ly, lz, cutoff):
             | (kz**2 > (cutoff * np.abs(kz).max()) ** 2))
 
 
+_WORKERS = max(1, min(16, os.cpu_count() or 1))
+
+
 def _rfft(a):
-    return np.fft.rfftn(a, axes=(2, 1, 0))
+    return scipy.fft.rfftn(a, axes=(2, 1, 0), workers=_WORKERS)
 
 
 def _irfft(c, shape):
-    return np.fft.irfftn(c, s=shape[::-1], axes=(2, 1, 0))
+    return scipy.fft.irfftn(c, s=shape[::-1], axes=(2, 1, 0), workers=_WORKERS)
 
 
 def random_spectrum(nx, ny=None, nz=None, seed=0, k0=4.0, energy=0.5, lx=1.0, ly=1.0, lz=1.0, cutoff=2 / 3):

@@ -15,7 +15,7 @@ from . import _lib
 from .config import Params, parameters, readglobal
 from .fields import padded_shape, read_field, write_field
 
-__all__ = ["Simulation", "run_simulation"]
+__all__ = ["Simulation", "footprint", "run_simulation"]
 
 
 def _dptr(a):
@@ -129,6 +129,10 @@ class Simulation:
     def synchronize(self):
         self._check(self._lib.ffs_synchronize(self._h))
 
+    def set_graph(self, on=True):
+        """Replay steady steps as HIP graphs (default on; bitwise identical to eager)."""
+        self._check(self._lib.ffs_set_graph(self._h, int(on)))
+
     # ---------------------------------------------------------------- diagnostics
     def pop_stats(self):
         """Returns the queued stats records as a list of (stats, scales, numbers) rows."""
@@ -223,6 +227,15 @@ class Simulation:
         it = self.iteration
         for f in range(3):
             write_field(os.path.join(directory, f"u{f + 1}.{it}"), self.get_field(f, _lib.FFS_SPACE_REAL))
+
+
+def footprint(params: Params):
+    """Device bytes a run with these parameters needs (no device work)."""
+    lib = _lib.load()
+    cp = params.to_c()
+    b = ctypes.c_size_t()
+    _lib.check(lib.ffs_footprint(ctypes.byref(cp), ctypes.byref(b)))
+    return b.value
 
 
 def _fmt(v, i):

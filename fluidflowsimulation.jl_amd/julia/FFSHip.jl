@@ -8,7 +8,8 @@
 # declaration of ffs.h; tests/test_julia_shim.py checks the correspondence.
 module FFSHip
 
-export FFSSimulation, ffs_params, initialize!, advance_in_time!, writestats, writeoutput, run_simulation
+export FFSSimulation, FFSBatch, ffs_params, initialize!, advance_in_time!, writestats, writeoutput, run_simulation,
+       footprint
 
 const libffs = get(ENV, "FFS_LIB", joinpath(@__DIR__, "..", "ffs", "libffs_hip.so"))
 
@@ -201,6 +202,51 @@ function device_bytes(s::FFSSimulation)
 end
 
 synchronize(s::FFSSimulation) = check(ccall((:ffs_synchronize, libffs), Cint, (Ptr{Cvoid},), s.h), s.h)
+
+# device bytes a run with these parameters needs (admission control; touches no device)
+function footprint(p::ffs_params)
+    b = Ref{Csize_t}(0)
+    check(ccall((:ffs_footprint, libffs), Cint, (Ref{ffs_params}, Ref{Csize_t}), p, b), C_NULL)
+    return Int(b[])
+end
+
+function device_memory()
+    f, t = Ref{Csize_t}(0), Ref{Csize_t}(0)
+    check(ccall((:ffs_device_memory, libffs), Cint, (Ref{Csize_t}, Ref{Csize_t}), f, t), C_NULL)
+    return Int(f[]), Int(t[])
+end
+
+set_graph!(s::FFSSimulation, on::Bool) = check(ccall((:ffs_set_graph, libffs), Cint, (Ptr{Cvoid}, Cint), s.h, on), s.h)
+
+# ---- ensembles: runs advanced in lockstep, one graph launch per steady step ----
+function batch_check(rc::Integer, b::Ptr{Cvoid})
+    rc < 0 || return rc
+    msg = unsafe_string(ccall((:ffs_batch_last_error, libffs), Cstring, (Ptr{Cvoid},), b))
+    error("libffs_hip batch error $rc: $msg")
+end
+
+mutable struct FFSBatch
+    h::Ptr{Cvoid}
+    runs::Vector{FFSSimulation}          # kept alive while the batch exists
+    function FFSBatch(runs::Vector{FFSSimulation})
+        hs = [r.h for r in runs]
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        batch_check(ccall((:ffs_batch_create, libffs), Cint, (Ptr{Ptr{Cvoid}}, Cint, Ref{Ptr{Cvoid}}),
+                          hs, length(hs), out), C_NULL)
+        b = new(out[], runs)
+        finalizer(x -> ccall((:ffs_batch_destroy, libffs), Cvoid, (Ptr{Cvoid},), x.h), b)
+        return b
+    end
+end
+
+function advance!(b::FFSBatch, n::Integer)
+    done, ev, who = Ref{Int64}(0), Ref{Cint}(0), Ref{Cint}(-1)
+    batch_check(ccall((:ffs_batch_advance, libffs), Cint, (Ptr{Cvoid}, Int64, Ref{Int64}, Ref{Cint}, Ref{Cint}),
+                      b.h, n, done, ev, who), b.h)
+    return done[], ev[], who[]
+end
+
+synchronize(b::FFSBatch) = batch_check(ccall((:ffs_batch_synchronize, libffs), Cint, (Ptr{Cvoid},), b.h), b.h)
 
 # writeoutput (output.jl:1-8): u1.$it u2.$it u3.$it in the padded real layout
 function writeoutput(s::FFSSimulation)

@@ -166,6 +166,45 @@ int ffs_device_bytes(const ffs_sim* s, size_t* bytes);
 /* Blocks until all queued work of the handle is done. */
 int ffs_synchronize(ffs_sim* s);
 
+/* Device bytes ffs_create would allocate for these parameters (state, pencil
+ * work buffers, tables, and the transient staging of ffs_set_field), computed
+ * without touching the device.  No counterpart in the reference (one Julia
+ * process per run, FluidFlowSimulation.jl:23); used for HBM admission control
+ * of many runs on one GPU. */
+int ffs_footprint(const ffs_params* p, size_t* bytes);
+
+/* Free and total bytes of the current device (hipMemGetInfo), for the same
+ * admission control.  No counterpart in the reference. */
+int ffs_device_memory(size_t* free_bytes, size_t* total_bytes);
+
+/* Replays steady steps (no stats, spectrum or output at that iteration, after
+ * the integrator's start-up steps) as HIP graphs, one per rotation of the
+ * three-level history: the same kernels with the same arguments as the eager
+ * path, so results are bitwise identical.  on = 0 turns it off.  Default on.
+ * Steps under ffs_profile_enable always run eagerly.  No counterpart in the
+ * reference (the Julia step is host code). */
+int ffs_set_graph(ffs_sim* s, int on);
+
+/* An ensemble: up to FFS_MAX_BATCH initialized handles on the current device
+ * advanced in lockstep.  A step in which every run is steady is one graph
+ * launch whose branches are the runs' steps (independent, so they overlap on
+ * the device); other steps run each handle's own path.  Each run's results
+ * are identical to running it alone.  The batch does not own the handles:
+ * destroy it before any of them.  No counterpart in the reference (one
+ * simulation per process). */
+#define FFS_MAX_BATCH 256
+typedef struct ffs_batch ffs_batch;
+int ffs_batch_create(ffs_sim* const* sims, int n, ffs_batch** out);
+/* Advances every run by up to nsteps steps.  Stops early, with *event =
+ * FFS_EVENT_OUTPUT and *who = the first such run's index, before a step at
+ * which some run has an output iteration; the next call performs that step.
+ * *done receives the lockstep steps taken. */
+int ffs_batch_advance(ffs_batch* b, int64_t nsteps, int64_t* done, int* event, int* who);
+/* Blocks until every run's queued work is done. */
+int ffs_batch_synchronize(ffs_batch* b);
+const char* ffs_batch_last_error(const ffs_batch* b);
+void ffs_batch_destroy(ffs_batch* b);
+
 const char* ffs_last_error(const ffs_sim* s);
 void ffs_destroy(ffs_sim* s);
 int ffs_abi_version(void);
