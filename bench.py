@@ -65,6 +65,19 @@ def main():
     ap.add_argument("--fp32", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--legs", default="fp32,ensemble,serve",
+                    help="extra measurements after the main line: fp32, ensemble, serve (comma separated)")
+    ap.add_argument("--fp32-batch", type=int, default=2)
+    ap.add_argument("--fp32-err-steps", type=int, default=20)
+    ap.add_argument("--ensemble-runs", type=int, default=64)
+    ap.add_argument("--ensemble-steps", type=int, default=1000)
+    ap.add_argument("--latency-samples", type=int, default=50)
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-max-runs", type=int, default=64)
+    ap.add_argument("--serve-chunk", type=int, default=25)
+    ap.add_argument("--serve-seeds", type=int, default=2)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first time a plain 1 GiB device copy (HBM calibration for the traffic counters)")
     args = ap.parse_args()
@@ -119,28 +132,58 @@ def copy_probe():
     return gbs
 
 
+def make_forced(n, nu, prec, dt_stat, seed_field, les=False):
+    import ffs
+
+    p = ffs.Params(nx=n, ny=n, nz=n, nu=nu, dt=0.0, integrator="Adams_Bashforth3rdO", variable_dt=True, cfl=0.5,
+                   dt_stat=dt_stat, dt_spec=1 << 30, write_time=1 << 30, precision=prec,
+                   forcing="shellForcing", forcing_kf=2.5, forcing_power=0.1,
+                   les_model="Smagorinsky" if les else None, smagorinsky_c=0.16)
+    s = ffs.Simulation(p)
+    s.set_velocity(*seed_field)
+    s.initialize()
+    return s
+
+
+def timed_steps(s, steps):
+    """Wall time of `steps` steps on the default (graph-replay) path, bracketed by synchronisation."""
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.advance_in_time(steps)
+    s.synchronize()
+    return time.perf_counter() - t0
+
+
+def profiled_steps(s, steps):
+    """Per-kernel device time (HIP events on the solver's stream around every launch) over `steps` steps."""
+    s.profile(True)
+    s.profile_reset()
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.advance_in_time(steps)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    prof = s.profile_get()
+    s.profile(False)
+    return prof, el
+
+
 def bench(args):
     import numpy as np
 
     copy_gbs = copy_probe() if args.copy_probe else None
 
-    import ffs
     from ffs.initial import random_spectrum
 
     n = args.n
     nu = nu_for_kmax_eta(n)
     prec = "fp32" if args.fp32 else "fp64"
     b = 4 if args.fp32 else 8
-    p = ffs.Params(nx=n, ny=n, nz=n, nu=nu, dt=0.0, integrator="Adams_Bashforth3rdO", variable_dt=True, cfl=0.5,
-                   dt_stat=args.dt_stat, dt_spec=1 << 30, write_time=1 << 30, precision=prec,
-                   forcing="shellForcing", forcing_kf=2.5, forcing_power=0.1)
     log(f"workload n={n} nu={nu:.6e} precision={prec}")
     t0 = time.time()
     u0 = random_spectrum(n, seed=args.seed)
     log(f"initial field generated in {time.time() - t0:.1f}s")
-    s = ffs.Simulation(p)
-    s.set_velocity(*u0)
-    s.initialize()
+    s = make_forced(n, nu, prec, args.dt_stat, u0)
     s.synchronize()
     log(f"initialized; device bytes {s.device_bytes() / 2**30:.2f} GiB")
     t0 = time.time()
@@ -148,21 +191,16 @@ def bench(args):
     s.synchronize()
     log(f"warmup {args.warmup} steps in {time.time() - t0:.2f}s")
     s.pop_stats()
-    # timed region: K steps bracketed by synchronisation; per-kernel HIP events on the solver's stream
-    s.profile(True)
-    s.profile_reset()
-    s.synchronize()
-    t0 = time.perf_counter()
-    s.advance_in_time(args.steps)
-    s.synchronize()
-    t1 = time.perf_counter()
-    prof = s.profile_get()
-    s.profile(False)
+    # timed region: K steps of the product path (steady steps replayed from HIP graphs)
+    elapsed = timed_steps(s, args.steps)
     rows = s.pop_stats()
-    elapsed = t1 - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = n**3 * args.steps / elapsed
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.4e} N^3*steps/s, dt={s.dt:.4e}")
+    # kernel timing: the same K steps again with HIP events around every launch (eager launches)
+    prof, el_prof = profiled_steps(s, args.steps)
+    rows += s.pop_stats()
+    log(f"profiled {args.steps} steps: {1e3 * el_prof / args.steps:.3f} ms/step")
     for r in rows[-2:]:
         log(f"stats it={int(r[0][0])} t={r[0][1]:.4f} k={r[0][11]:.6f} diss={r[0][14]:.6f}")
     finite = all(np.all(np.isfinite(r[0][2:22])) for r in rows)
@@ -174,11 +212,12 @@ def bench(args):
     for name, (ms, cnt) in prof.items():
         avg_us = 1e3 * ms / cnt
         kb = kernel_bytes(g, name, 2 * b, ke_frac)
-        kernels[name] = {"launches": cnt, "avg_us": round(avg_us, 3), "share": round(ms / (1e3 * elapsed), 4)}
+        kernels[name] = {"launches": cnt, "avg_us": round(avg_us, 3), "share": round(ms / (1e3 * el_prof), 4)}
         if kb:
             kernels[name]["alg_bytes"] = int(kb)
             kernels[name]["GBs"] = round(kb / (avg_us * 1e-6) / 1e9, 1)
-    dom = max((k for k in kernels if "alg_bytes" in kernels[k]), key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+    dom = max((k for k in kernels if "alg_bytes" in kernels[k]),
+              key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
     d = kernels[dom]
     achieved = d["GBs"]
     # survey byte model for the whole step: 54 F_b + 15 R_b
@@ -192,6 +231,8 @@ def bench(args):
     log(f"dominant kernel {dom}: {d['avg_us']:.1f} us, {achieved:.0f} GB/s; step model {step_eff:.0f} GB/s effective")
     for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["avg_us"] * kv[1]["launches"]):
         log(f"  {k:12s} {v}")
+    final_dt = s.dt
+    s.close()
 
     out = {
         "metric": METRIC,
@@ -210,7 +251,7 @@ def bench(args):
             "workload": f"forced isotropic turbulence DNS {n}^3 {prec.upper()} AB3 CFL 0.5, shell forcing |k|<=2.5 "
                         f"P=0.1, nu={nu:.4e} (k_max eta=1.5), stats every {args.dt_stat} steps",
             "n": n, "precision": prec, "integrator": "Adams_Bashforth3rdO", "cfl": 0.5, "dt_stat": args.dt_stat,
-            "final_dt": s.dt, "finite": bool(finite),
+            "final_dt": final_dt, "finite": bool(finite),
         },
         "roofline": {
             "bound": "hbm",
@@ -222,6 +263,9 @@ def bench(args):
             "traffic": None,
             "alg_bytes_per_launch": d["alg_bytes"],
             "avg_us": d["avg_us"],
+            "timing": "HIP events on the solver's stream around every launch, over a second stretch of the same "
+                      "K steps launched eagerly",
+            "profiled_ms_per_step": 1e3 * el_prof / args.steps,
         },
         "step_roofline": {
             "model_bytes_per_step": step_bytes,
@@ -233,11 +277,174 @@ def bench(args):
         "kernels": kernels,
         "copy_probe_GBs": copy_gbs,
     }
-    s.close()
+    legs = [x for x in args.legs.split(",") if x]
+    if "fp32" in legs and not args.fp32:
+        out["fp32"] = leg_fp32(args, n, nu, u0, value)
+    del u0
+    if "ensemble" in legs:
+        out["ensemble"] = leg_ensemble(args)
+    if "serve" in legs:
+        out["serving"] = leg_serve(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, nu, args)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def spectrum_rel_err(a, b):
+    """max over the four spectra of max_k |a - b| / max_k |b|."""
+    import numpy as np
+
+    err = 0.0
+    for x, y in zip(a, b):
+        for u, v in zip(x, y):
+            err = max(err, float(np.abs(u - v).max() / max(np.abs(v).max(), 1e-300)))
+    return err
+
+
+def leg_fp32(args, n, nu, u0, fp64_value):
+    """FP32 bulk path at the same workload: one run, and a lockstep batch of `fp32_batch` runs
+    (each its own seed); the spectrum error against the FP64 path after `fp32_err_steps` steps."""
+    import ffs
+    from ffs.initial import random_spectrum
+
+    out = {}
+    s = make_forced(n, nu, "fp32", 1 << 30, u0)
+    s.advance_in_time(args.warmup)
+    el = timed_steps(s, args.steps)
+    out["single"] = {"value": n**3 * args.steps / el, "ms_per_step": 1e3 * el / args.steps}
+    s.close()
+    log(f"fp32 single: {1e3 * el / args.steps:.3f} ms/step, {n**3 * args.steps / el:.4e} N^3*steps/s")
+    nb = args.fp32_batch
+    fields = [u0] + [random_spectrum(n, seed=args.seed + 1 + i) for i in range(nb - 1)]
+    sims = [make_forced(n, nu, "fp32", 1 << 30, f) for f in fields]
+    del fields
+    with ffs.Ensemble(sims) as ens:
+        ens.advance_in_time(args.warmup)
+        ens.synchronize()
+        t0 = time.perf_counter()
+        ens.advance_in_time(args.steps)
+        ens.synchronize()
+        el = time.perf_counter() - t0
+    for x in sims:
+        x.close()
+    v = nb * n**3 * args.steps / el
+    out["batched"] = {"runs": nb, "value": v, "ms_per_lockstep_step": 1e3 * el / args.steps,
+                      "vs_fp64_value": v / fp64_value}
+    log(f"fp32 batched x{nb}: {1e3 * el / args.steps:.3f} ms/lockstep step, {v:.4e} N^3*steps/s "
+        f"({v / fp64_value:.2f}x FP64)")
+    # precision: FP32 vs FP64 from the same field after K steps (each with its own variable dt)
+    k = args.fp32_err_steps
+    a = make_forced(n, nu, "fp64", 1 << 30, u0)
+    c = make_forced(n, nu, "fp32", 1 << 30, u0)
+    a.advance_in_time(k)
+    c.advance_in_time(k)
+    err = spectrum_rel_err(c.spectrum(), a.spectrum())
+    a.writestats_now()
+    c.writestats_now()
+    ra, rc = a.pop_stats()[-1][0], c.pop_stats()[-1][0]
+    out["spectrum_rel_err_vs_fp64"] = err
+    out["energy_rel_err_vs_fp64"] = float(abs(rc[11] - ra[11]) / abs(ra[11]))
+    out["dissipation_rel_err_vs_fp64"] = float(abs(rc[14] - ra[14]) / abs(ra[14]))
+    out["err_after_steps"] = k
+    log(f"fp32 vs fp64 after {k} steps: spectrum {err:.2e}, energy {out['energy_rel_err_vs_fp64']:.2e}")
+    a.close()
+    c.close()
+    return out
+
+
+def leg_ensemble(args):
+    """Batched LES ensemble: `ensemble_runs` Smagorinsky (c=0.16) runs, sizes drawn uniformly from
+    {32,48,64,96,128}^3, nu log-uniform in [1e-4, 1e-2], own seeded field, shell forcing, FP32,
+    AB3 CFL 0.5, advanced in lockstep (one graph launch per steady step)."""
+    import numpy as np
+
+    import ffs
+    from ffs.initial import random_spectrum
+
+    rng = np.random.default_rng(args.seed)
+    R = args.ensemble_runs
+    sizes = [int(x) for x in rng.choice([32, 48, 64, 96, 128], size=R)]
+    nus = np.exp(rng.uniform(np.log(1e-4), np.log(1e-2), size=R))
+    t0 = time.time()
+    sims = [make_forced(n, float(nus[i]), "fp32", 1 << 30, random_spectrum(n, seed=args.seed + 100 + i), les=True)
+            for i, n in enumerate(sizes)]
+    log(f"ensemble: {R} runs created in {time.time() - t0:.1f}s; sizes {sorted(set(sizes))}")
+    pts = float(sum(n**3 for n in sizes))
+    K = args.ensemble_steps
+    with ffs.Ensemble(sims) as ens:
+        ens.advance_in_time(args.warmup)
+        ens.synchronize()
+        t0 = time.perf_counter()
+        ens.advance_in_time(K)
+        ens.synchronize()
+        el = time.perf_counter() - t0
+        lat = []
+        for _ in range(args.latency_samples):
+            a = time.perf_counter()
+            ens.advance(1)
+            ens.synchronize()
+            lat.append(1e3 * (time.perf_counter() - a))
+    finite = sum(1 for x in sims if math.isfinite(x.dt))
+    fp = sum(ffs.footprint(x.params) for x in sims)
+    for x in sims:
+        x.close()
+    out = {"runs": R, "steps": K, "value": pts * K / el, "unit": "N^3*steps/s (sum over runs)",
+           "ms_per_lockstep_step": 1e3 * el / K, "step_ms_p50": float(np.percentile(lat, 50)),
+           "step_ms_p99": float(np.percentile(lat, 99)), "latency_samples": len(lat), "finite_runs": finite,
+           "device_bytes": fp, "size_counts": {str(n): sizes.count(n) for n in sorted(set(sizes))}}
+    log(f"ensemble: {1e3 * el / K:.3f} ms/lockstep step, {pts * K / el:.4e} N^3*steps/s, "
+        f"p50 {out['step_ms_p50']:.3f} ms p99 {out['step_ms_p99']:.3f} ms")
+    return out
+
+
+def leg_serve(args):
+    """Serving: Poisson arrivals (mean `serve_rate`/s) of `serve_requests` solves, size from
+    {64,128,192,256}^3 weighted 4:3:2:1, FP32:FP64 3:1, nu log-uniform in [1e-4, 1e-2], horizon
+    uniform in [200, 2000] steps, Smagorinsky LES with probability 1/2, admitted under an HBM budget
+    with state resident on the device; continuous batching in lockstep."""
+    import numpy as np
+
+    import ffs
+    from ffs.initial import random_spectrum
+    from ffs.serve import Job, Server
+
+    rng = np.random.default_rng(args.seed + 7)
+    R = args.serve_requests
+    sizes = [int(x) for x in rng.choice([64, 128, 192, 256], size=R, p=[0.4, 0.3, 0.2, 0.1])]
+    precs = ["fp32" if x < 0.75 else "fp64" for x in rng.uniform(size=R)]
+    nus = np.exp(rng.uniform(np.log(1e-4), np.log(1e-2), size=R))
+    horizons = [int(x) for x in rng.integers(200, 2001, size=R)]
+    les = [bool(x < 0.5) for x in rng.uniform(size=R)]
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=R))
+    # initial fields: a pool of `serve_seeds` seeded fields per size (host generation at 256^3 takes seconds)
+    t0 = time.time()
+    pool = {(n, k): random_spectrum(n, seed=args.seed + 500 + 10 * k + n) for n in sorted(set(sizes))
+            for k in range(args.serve_seeds)}
+    log(f"serving: {R} requests, field pool built in {time.time() - t0:.1f}s")
+
+    def job(i):
+        n = sizes[i]
+        p = ffs.Params(nx=n, ny=n, nz=n, nu=float(nus[i]), integrator="Adams_Bashforth3rdO", variable_dt=True,
+                       cfl=0.5, dt_stat=100, dt_spec=1 << 30, write_time=1 << 30, precision=precs[i],
+                       forcing="shellForcing", forcing_kf=2.5, forcing_power=0.1,
+                       les_model="Smagorinsky" if les[i] else None, smagorinsky_c=0.16)
+        f = pool[(n, i % args.serve_seeds)]
+        return Job(p, horizons[i], lambda s, f=f: s.set_velocity(*f), name=f"req{i}", keep_fields=False)
+
+    srv = Server(budget=int(args.serve_budget_gb * 1e9), max_runs=args.serve_max_runs, chunk=args.serve_chunk)
+    t0 = time.perf_counter()
+    done = srv.run(arrivals=[(float(arrivals[i]), job(i)) for i in range(R)])
+    el = time.perf_counter() - t0
+    pts_steps = float(sum(j.params.nx**3 * j.nsteps for j in done))
+    lat = srv.latency()
+    out = {"requests": R, "rate_per_s": args.serve_rate, "wall_s": el, "value": pts_steps / el,
+           "unit": "N^3*steps/s (sum over requests)", **lat, "max_active": srv.max_active,
+           "budget_bytes": srv.budget, "peak_bytes": srv.peak_used,
+           "finite": sum(1 for j in done if all(np.all(np.isfinite(r[0][2:12])) for r in j.stats))}
+    log(f"serving: {R} requests in {el:.1f}s, {pts_steps / el:.4e} N^3*steps/s, step p50 "
+        f"{lat.get('step_ms_p50', 0):.3f} ms p99 {lat.get('step_ms_p99', 0):.3f} ms, max active {srv.max_active}")
+    return out
 
 
 def cpu_baseline(n, nu, args):

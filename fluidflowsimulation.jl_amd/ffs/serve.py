@@ -89,7 +89,9 @@ class Server:
         self.done = []
         self.used = 0
         self.step_ms = []         # per lockstep step: wall ms of the chunk / its steps
-        self.step_weights = []
+        self.step_weights = []    # request-steps at that latency (steps x active runs)
+        self.max_active = 0
+        self.peak_used = 0
         self._ens = None
 
     def submit(self, job):
@@ -112,6 +114,8 @@ class Server:
             job.started = time.perf_counter()
             self.used += job.footprint
             self.active.append([job, sim, job.nsteps])
+        self.max_active = max(self.max_active, len(self.active))
+        self.peak_used = max(self.peak_used, self.used)
         return n
 
     def _retire(self):
@@ -137,9 +141,24 @@ class Server:
         self.active = keep
         return changed
 
-    def run(self):
-        """Serves until the queue is empty and every run has finished; returns the finished jobs."""
-        while self.queue or self.active:
+    def run(self, arrivals=None):
+        """Serves until the queue is empty and every run has finished; returns the finished jobs.
+
+        arrivals: optional list of (seconds after the start, Job); each job is submitted once its
+        time has come, between lockstep chunks (or by waiting when nothing is running)."""
+        pending = deque(sorted(arrivals or [], key=lambda a: a[0]))
+        t_start = time.perf_counter()
+
+        def arrive():
+            now = time.perf_counter() - t_start
+            while pending and pending[0][0] <= now:
+                self.submit(pending.popleft()[1])
+
+        while self.queue or self.active or pending:
+            arrive()
+            if not self.queue and not self.active:
+                time.sleep(max(0.0, pending[0][0] - (time.perf_counter() - t_start)))
+                continue
             admitted = self._admit()
             if not self.active:
                 raise RuntimeError("no job could be admitted")
@@ -154,7 +173,7 @@ class Server:
                 self._ens.synchronize()
                 dt = (time.perf_counter() - t0) * 1e3
                 self.step_ms.append(dt / done)
-                self.step_weights.append(done)
+                self.step_weights.append(done * len(self.active))
             for entry in self.active:
                 entry[2] -= done
             if ev == _lib.FFS_EVENT_OUTPUT:
@@ -173,7 +192,8 @@ class Server:
 
     # ---------------------------------------------------------------- metrics
     def latency(self):
-        """Per-step wall latency percentiles (ms) over lockstep steps, and job turnaround percentiles (s)."""
+        """Per-step wall latency percentiles (ms) over request-steps (a lockstep step's latency counts once
+        per active run), and job turnaround percentiles (s, submission to retirement)."""
         out = {}
         if self.step_ms:
             ms = np.repeat(np.array(self.step_ms), np.array(self.step_weights))

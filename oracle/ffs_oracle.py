@@ -35,7 +35,8 @@ __all__ = ["Grid", "OracleSim", "taylor_green", "workers"]
 
 
 def workers():
-    return int(os.environ.get("FFS_ORACLE_WORKERS", os.cpu_count() or 1))
+    """FFT threads: FFS_ORACLE_WORKERS, else the host's cores capped at 16."""
+    return int(os.environ.get("FFS_ORACLE_WORKERS", min(16, os.cpu_count() or 1)))
 
 
 def rfftfreq(n, s):
