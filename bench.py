@@ -65,8 +65,10 @@ def main():
     ap.add_argument("--fp32", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="fp32,ensemble,serve",
-                    help="extra measurements after the main line: fp32, ensemble, serve (comma separated)")
+    ap.add_argument("--legs", default="fp32,ensemble,serve,strat",
+                    help="extra measurements after the main line: fp32, ensemble, serve, strat (comma separated)")
+    ap.add_argument("--strat-n", type=int, default=512)
+    ap.add_argument("--strat-steps", type=int, default=500)
     ap.add_argument("--fp32-batch", type=int, default=2)
     ap.add_argument("--fp32-err-steps", type=int, default=20)
     ap.add_argument("--ensemble-runs", type=int, default=64)
@@ -77,7 +79,6 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-max-runs", type=int, default=64)
     ap.add_argument("--serve-chunk", type=int, default=25)
-    ap.add_argument("--serve-seeds", type=int, default=2)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first time a plain 1 GiB device copy (HBM calibration for the traffic counters)")
     args = ap.parse_args()
@@ -132,7 +133,8 @@ def copy_probe():
     return gbs
 
 
-def make_forced(n, nu, prec, dt_stat, seed_field, les=False):
+def make_forced(n, nu, prec, dt_stat, seed, les=False):
+    """One run of the forced workload, its initial field generated on the device from `seed`."""
     import ffs
 
     p = ffs.Params(nx=n, ny=n, nz=n, nu=nu, dt=0.0, integrator="Adams_Bashforth3rdO", variable_dt=True, cfl=0.5,
@@ -140,7 +142,7 @@ def make_forced(n, nu, prec, dt_stat, seed_field, les=False):
                    forcing="shellForcing", forcing_kf=2.5, forcing_power=0.1,
                    les_model="Smagorinsky" if les else None, smagorinsky_c=0.16)
     s = ffs.Simulation(p)
-    s.set_velocity(*seed_field)
+    s.seed_velocity(seed, k0=4.0, energy=0.5)
     s.initialize()
     return s
 
@@ -173,17 +175,12 @@ def bench(args):
 
     copy_gbs = copy_probe() if args.copy_probe else None
 
-    from ffs.initial import random_spectrum
-
     n = args.n
     nu = nu_for_kmax_eta(n)
     prec = "fp32" if args.fp32 else "fp64"
     b = 4 if args.fp32 else 8
     log(f"workload n={n} nu={nu:.6e} precision={prec}")
-    t0 = time.time()
-    u0 = random_spectrum(n, seed=args.seed)
-    log(f"initial field generated in {time.time() - t0:.1f}s")
-    s = make_forced(n, nu, prec, args.dt_stat, u0)
+    s = make_forced(n, nu, prec, args.dt_stat, args.seed)
     s.synchronize()
     log(f"initialized; device bytes {s.device_bytes() / 2**30:.2f} GiB")
     t0 = time.time()
@@ -246,7 +243,8 @@ def bench(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if args.fp32 else "f64",
-        "data": "synthetic: seeded random-phase solenoidal field, E(k)~k^4 exp(-2(k/4)^2), energy 0.5",
+        "data": "synthetic: seeded random-phase solenoidal field generated on the device (SplitMix64 + Box-Muller "
+                "per mode), E(k)~k^4 exp(-2(k/4)^2), energy 0.5",
         "config": {
             "workload": f"forced isotropic turbulence DNS {n}^3 {prec.upper()} AB3 CFL 0.5, shell forcing |k|<=2.5 "
                         f"P=0.1, nu={nu:.4e} (k_max eta=1.5), stats every {args.dt_stat} steps",
@@ -279,12 +277,13 @@ def bench(args):
     }
     legs = [x for x in args.legs.split(",") if x]
     if "fp32" in legs and not args.fp32:
-        out["fp32"] = leg_fp32(args, n, nu, u0, value)
-    del u0
+        out["fp32"] = leg_fp32(args, n, nu, value)
     if "ensemble" in legs:
         out["ensemble"] = leg_ensemble(args)
     if "serve" in legs:
         out["serving"] = leg_serve(args)
+    if "strat" in legs:
+        out["stratified"] = leg_strat(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, nu, args)
     print(json.dumps(out), flush=True)
@@ -302,23 +301,20 @@ def spectrum_rel_err(a, b):
     return err
 
 
-def leg_fp32(args, n, nu, u0, fp64_value):
+def leg_fp32(args, n, nu, fp64_value):
     """FP32 bulk path at the same workload: one run, and a lockstep batch of `fp32_batch` runs
     (each its own seed); the spectrum error against the FP64 path after `fp32_err_steps` steps."""
     import ffs
-    from ffs.initial import random_spectrum
 
     out = {}
-    s = make_forced(n, nu, "fp32", 1 << 30, u0)
+    s = make_forced(n, nu, "fp32", 1 << 30, args.seed)
     s.advance_in_time(args.warmup)
     el = timed_steps(s, args.steps)
     out["single"] = {"value": n**3 * args.steps / el, "ms_per_step": 1e3 * el / args.steps}
     s.close()
     log(f"fp32 single: {1e3 * el / args.steps:.3f} ms/step, {n**3 * args.steps / el:.4e} N^3*steps/s")
     nb = args.fp32_batch
-    fields = [u0] + [random_spectrum(n, seed=args.seed + 1 + i) for i in range(nb - 1)]
-    sims = [make_forced(n, nu, "fp32", 1 << 30, f) for f in fields]
-    del fields
+    sims = [make_forced(n, nu, "fp32", 1 << 30, args.seed + i) for i in range(nb)]
     with ffs.Ensemble(sims) as ens:
         ens.advance_in_time(args.warmup)
         ens.synchronize()
@@ -335,8 +331,8 @@ def leg_fp32(args, n, nu, u0, fp64_value):
         f"({v / fp64_value:.2f}x FP64)")
     # precision: FP32 vs FP64 from the same field after K steps (each with its own variable dt)
     k = args.fp32_err_steps
-    a = make_forced(n, nu, "fp64", 1 << 30, u0)
-    c = make_forced(n, nu, "fp32", 1 << 30, u0)
+    a = make_forced(n, nu, "fp64", 1 << 30, args.seed)
+    c = make_forced(n, nu, "fp32", 1 << 30, args.seed)
     a.advance_in_time(k)
     c.advance_in_time(k)
     err = spectrum_rel_err(c.spectrum(), a.spectrum())
@@ -360,15 +356,13 @@ def leg_ensemble(args):
     import numpy as np
 
     import ffs
-    from ffs.initial import random_spectrum
 
     rng = np.random.default_rng(args.seed)
     R = args.ensemble_runs
     sizes = [int(x) for x in rng.choice([32, 48, 64, 96, 128], size=R)]
     nus = np.exp(rng.uniform(np.log(1e-4), np.log(1e-2), size=R))
     t0 = time.time()
-    sims = [make_forced(n, float(nus[i]), "fp32", 1 << 30, random_spectrum(n, seed=args.seed + 100 + i), les=True)
-            for i, n in enumerate(sizes)]
+    sims = [make_forced(n, float(nus[i]), "fp32", 1 << 30, args.seed + 100 + i, les=True) for i, n in enumerate(sizes)]
     log(f"ensemble: {R} runs created in {time.time() - t0:.1f}s; sizes {sorted(set(sizes))}")
     pts = float(sum(n**3 for n in sizes))
     K = args.ensemble_steps
@@ -402,11 +396,11 @@ def leg_serve(args):
     """Serving: Poisson arrivals (mean `serve_rate`/s) of `serve_requests` solves, size from
     {64,128,192,256}^3 weighted 4:3:2:1, FP32:FP64 3:1, nu log-uniform in [1e-4, 1e-2], horizon
     uniform in [200, 2000] steps, Smagorinsky LES with probability 1/2, admitted under an HBM budget
-    with state resident on the device; continuous batching in lockstep."""
+    with state resident on the device; continuous batching in lockstep.  Each request's initial field
+    is generated on the device from its own seed."""
     import numpy as np
 
     import ffs
-    from ffs.initial import random_spectrum
     from ffs.serve import Job, Server
 
     rng = np.random.default_rng(args.seed + 7)
@@ -417,11 +411,7 @@ def leg_serve(args):
     horizons = [int(x) for x in rng.integers(200, 2001, size=R)]
     les = [bool(x < 0.5) for x in rng.uniform(size=R)]
     arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=R))
-    # initial fields: a pool of `serve_seeds` seeded fields per size (host generation at 256^3 takes seconds)
-    t0 = time.time()
-    pool = {(n, k): random_spectrum(n, seed=args.seed + 500 + 10 * k + n) for n in sorted(set(sizes))
-            for k in range(args.serve_seeds)}
-    log(f"serving: {R} requests, field pool built in {time.time() - t0:.1f}s")
+    log(f"serving: {R} requests")
 
     def job(i):
         n = sizes[i]
@@ -429,8 +419,8 @@ def leg_serve(args):
                        cfl=0.5, dt_stat=100, dt_spec=1 << 30, write_time=1 << 30, precision=precs[i],
                        forcing="shellForcing", forcing_kf=2.5, forcing_power=0.1,
                        les_model="Smagorinsky" if les[i] else None, smagorinsky_c=0.16)
-        f = pool[(n, i % args.serve_seeds)]
-        return Job(p, horizons[i], lambda s, f=f: s.set_velocity(*f), name=f"req{i}", keep_fields=False)
+        return Job(p, horizons[i], lambda s, k=args.seed + 500 + i: s.seed_velocity(k), name=f"req{i}",
+                   keep_fields=False)
 
     srv = Server(budget=int(args.serve_budget_gb * 1e9), max_runs=args.serve_max_runs, chunk=args.serve_chunk)
     t0 = time.perf_counter()
@@ -447,18 +437,81 @@ def leg_serve(args):
     return out
 
 
+def leg_strat(args):
+    """Stably stratified Boussinesq DNS: seeded velocity (energy 0.5, k0 4) and density (rms 0.01, k0 4),
+    N = 1 (Pr 1, densityGradient 1, zAcceleration 1, referenceDensity 1), nu = 1e-4, ETD3 with a fixed dt
+    from the initial CFL 0.18 (so both precisions share dt); the FP32 bulk path against the FP64 path
+    after `strat_steps` steps: energy, potential energy (<rho^2>) and spectrum errors, and each one's rate."""
+    import numpy as np
+
+    import ffs
+    from ffs import _lib
+
+    n = args.strat_n
+    nu = 1e-4
+
+    def params(prec, variable):
+        return ffs.Params(nx=n, ny=n, nz=n, nu=nu, integrator="ETD3rdO", variable_dt=variable, cfl=0.18,
+                          dt=dt if not variable else 0.0, dt_stat=1 << 30, dt_spec=1 << 30, write_time=1 << 30,
+                          precision=prec, density=True, pr=1.0, density_gradient=1.0, z_acceleration=1.0,
+                          reference_density=1.0)
+
+    def seeded(p):
+        s = ffs.Simulation(p)
+        s.seed_velocity(args.seed, k0=4.0, energy=0.5)
+        s.seed_density(args.seed + 1, k0=4.0, rms=0.01)
+        s.initialize()
+        return s
+
+    dt = 0.0
+    with seeded(params("fp64", True)) as probe:  # max|u_i| of the initial field
+        umax = probe.scalar(_lib.FFS_SCALAR_UMAX)
+    dt = 0.18 * (2 * math.pi / n) / umax
+    out = {"n": n, "steps": args.strat_steps, "dt": dt, "integrator": "ETD3rdO", "nu": nu}
+    res = {}
+    for prec in ("fp64", "fp32"):
+        s = seeded(params(prec, False))
+        s.advance_in_time(3)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.advance_in_time(args.strat_steps - 3)
+        s.synchronize()
+        el = time.perf_counter() - t0
+        s.writestats_now()
+        row = s.pop_stats()[-1][0]
+        spec = s.spectrum()
+        res[prec] = (row, spec)
+        out[prec] = {"value": n**3 * (args.strat_steps - 3) / el, "ms_per_step": 1e3 * el / (args.strat_steps - 3),
+                     "device_bytes": s.device_bytes(), "finite": bool(np.all(np.isfinite(row[2:30])))}
+        log(f"stratified {n}^3 {prec}: {out[prec]['ms_per_step']:.3f} ms/step, {out[prec]['value']:.4e} N^3*steps/s")
+        s.close()
+    (r64, s64), (r32, s32) = res["fp64"], res["fp32"]
+    out["energy_rel_err"] = float(abs(r32[11] - r64[11]) / abs(r64[11]))
+    out["potential_energy_rel_err"] = float(abs(r32[23] - r64[23]) / abs(r64[23]))
+    errs = []
+    for x, y in zip(s32, s64):
+        for u, v in zip(x, y):
+            sel = v >= 1e-6 * v.max()
+            errs.append(float(np.max(np.abs(u[sel] - v[sel]) / v[sel])))
+    out["spectrum_rel_err"] = max(errs)
+    out["fp32_vs_fp64_value"] = out["fp32"]["value"] / out["fp64"]["value"]
+    log(f"stratified fp32 vs fp64: energy {out['energy_rel_err']:.2e}, potential energy "
+        f"{out['potential_energy_rel_err']:.2e}, spectrum {out['spectrum_rel_err']:.2e}")
+    return out
+
+
 def cpu_baseline(n, nu, args):
     """The FP64 oracle (numpy + multi-threaded scipy.fft) on the host cores:
     initialize + `cpu_steps` steps of the same workload."""
-    from ffs.initial import random_spectrum
     from oracle.ffs_oracle import Grid, OracleSim, workers
+    from oracle.seeded import seeded_velocity_hat
 
     cores = workers()
     log(f"cpu baseline: oracle, {args.cpu_steps} steps at {n}^3 on {cores} threads")
     g = Grid(n, n, n)
     o = OracleSim(g, nu, dt=0.0, variable_dt=True, cfl=0.5, dt_stat=args.dt_stat, forcing_kf=2.5,
                   forcing_power=0.1)
-    o.set_velocity(random_spectrum(n, seed=args.seed))
+    o.set_velocity([g.irfft(x) for x in seeded_velocity_hat(g, args.seed, k0=4.0, energy=0.5)])
     o.initialize()
     t0 = time.perf_counter()
     o.advance_in_time(args.cpu_steps)

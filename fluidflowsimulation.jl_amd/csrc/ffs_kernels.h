@@ -128,4 +128,10 @@ void launch_end_step(DevState* st, hipStream_t s);
 template <typename T>
 void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s);
 
+// seeded random-phase field in the compact layout: nf = 3 solenoidal velocity scaled to energy `target`,
+// nf = 1 scalar scaled to rms `target` (partials: >= 512 * kStatsSlots doubles; red: >= 1 double)
+template <typename T>
+void launch_seed(const Geo& g, void* out, int nf, unsigned long long seed, double k0, double target, double* partials,
+                 double* red, hipStream_t s);
+
 }  // namespace fsk

@@ -1352,6 +1352,126 @@ __global__ void k_cast_copy(const S* __restrict__ a, D* __restrict__ b, size_t n
   if (i < n) b[i] = (D)a[i];
 }
 
+
+// ===========================================================================
+// Seeded random-phase initial fields, generated in place in the compact
+// spectral layout.  Mode (kx, ky, kz) (signed integer indices) of component c
+// draws one complex Gaussian xi from a counter-based generator: two SplitMix64
+// hashes of (seed, c, kx, ky, kz) -> two uniforms in (0,1) -> Box-Muller.  The
+// kx = 0 plane takes xi(0,ky,kz) = conj(xi(0,-ky,-kz)) for the non-canonical
+// half (ky < 0, or ky = 0 and kz < 0) so the field is real.  Velocity:
+// u_hat = A(|k|) (I - k k / |k|^2) xi, A = |k| exp(-(|k|/k0)^2) / sqrt(4 pi)
+// (shell energy ~ k^4 exp(-2 (k/k0)^2)); scalar: A(|k|) xi.  The k = 0 mode is
+// zero.  A second kernel scales the result to the requested energy / rms from
+// deterministic block sums of w |u_hat|^2 (w = 2 for kx > 0).
+
+__device__ __forceinline__ unsigned long long seed_mix64(unsigned long long z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ void seed_gauss(unsigned long long seed, int c, int ix, int iy, int iz, double& re,
+                                           double& im) {
+  const unsigned long long ctr = ((((unsigned long long)(c + 1) << 16 | (unsigned long long)(ix & 0xffff)) << 16 |
+                                   (unsigned long long)(iy & 0xffff))
+                                  << 16) |
+                                 (unsigned long long)(iz & 0xffff);
+  const unsigned long long s = seed_mix64(seed);
+  const unsigned long long h1 = seed_mix64(s ^ seed_mix64(2 * ctr));
+  const unsigned long long h2 = seed_mix64(s ^ seed_mix64(2 * ctr + 1));
+  const double u1 = ((double)(h1 >> 11) + 0.5) * 0x1.0p-53;
+  const double u2 = ((double)(h2 >> 11) + 0.5) * 0x1.0p-53;
+  const double r = sqrt(-2.0 * log(u1));
+  const double th = 6.283185307179586 * u2;
+  re = r * cos(th);
+  im = r * sin(th);
+}
+
+__device__ __forceinline__ int signed_index(int j, int n) { return j < (n + 1) / 2 ? j : j - n; }
+
+// nf = 3: solenoidal velocity into comps 0..2 of `out`; nf = 1: scalar (generator component 3)
+template <typename T>
+__global__ void __launch_bounds__(256) k_seed(Geo g, C<T>* __restrict__ out, int nf, unsigned long long seed,
+                                              double k0, double* __restrict__ partials) {
+  const size_t n = (size_t)g.sx * g.my * g.mz;
+  double acc = 0.0;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const int cx = idx % g.sx;
+    const int cy = (idx / g.sx) % g.my;
+    const int cz = idx / ((size_t)g.sx * g.my);
+    if (cx >= g.mx) {
+      for (int c = 0; c < nf; ++c) out[c * n + idx] = mk<T>(0, 0);
+      continue;
+    }
+    const double kx = g.kx[cx], ky = g.ky[cy], kz = g.kz[cz];
+    const double K2 = kx * kx + ky * ky + kz * kz;
+    int iy = signed_index(g.ymap[cy], g.ny), iz = signed_index(g.zmap[cz], g.nz);
+    bool conj = false;
+    if (cx == 0 && (iy < 0 || (iy == 0 && iz < 0))) {
+      iy = -iy;
+      iz = -iz;
+      conj = true;
+    }
+    double xr[3] = {0, 0, 0}, xi[3] = {0, 0, 0};
+    if (K2 > 0) {
+      for (int c = 0; c < nf; ++c) {
+        seed_gauss(seed, nf == 3 ? c : 3, cx, iy, iz, xr[c], xi[c]);
+        if (conj) xi[c] = -xi[c];
+      }
+    }
+    const double kk = sqrt(K2);
+    const double amp = K2 > 0 ? kk * exp(-(kk / k0) * (kk / k0)) / sqrt(4.0 * 3.141592653589793) : 0.0;
+    if (nf == 3 && K2 > 0) {
+      const double pr = (kx * xr[0] + ky * xr[1] + kz * xr[2]) / K2;
+      const double pi = (kx * xi[0] + ky * xi[1] + kz * xi[2]) / K2;
+      const double kv[3] = {kx, ky, kz};
+      for (int c = 0; c < 3; ++c) {
+        xr[c] -= pr * kv[c];
+        xi[c] -= pi * kv[c];
+      }
+    }
+    const double w = cx > 0 ? 2.0 : 1.0;
+    for (int c = 0; c < nf; ++c) {
+      const double a = amp * xr[c], b = amp * xi[c];
+      acc += w * (a * a + b * b);
+      out[c * n + idx] = mk<T>((T)a, (T)b);
+    }
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[(size_t)blockIdx.x * kStatsSlots] = red[0];
+}
+
+// out *= sqrt(target / (norm * sum)), sum = sum of the seed kernel's partials (read on the device)
+template <typename T>
+__global__ void k_seed_scale(C<T>* __restrict__ out, size_t n, const double* __restrict__ sum, double target,
+                             double norm) {
+  const double f = sqrt(target / (norm * sum[0]));
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const C<T> v = out[i];
+    out[i] = mk<T>((T)(f * v.x), (T)(f * v.y));
+  }
+}
+
+template <typename T>
+void launch_seed(const Geo& g, void* out, int nf, unsigned long long seed, double k0, double target, double* partials,
+                 double* red, hipStream_t s) {
+  const size_t n = (size_t)g.sx * g.my * g.mz;
+  const int nblk = (int)std::min<size_t>((n + 255) / 256, 512);
+  k_seed<T><<<nblk, 256, 0, s>>>(g, (C<T>*)out, nf, seed, k0, partials);
+  k_reduce_partials<<<1, 256, 0, s>>>(partials, nblk, 1, red);
+  // velocity: energy = 1/2 sum w |u|^2; scalar: rms^2 = sum w |rho|^2
+  k_seed_scale<T><<<1024, 256, 0, s>>>((C<T>*)out, nf * n, red, target * (nf == 3 ? 1.0 : target),
+                                       nf == 3 ? 0.5 : 1.0);
+}
+
 // copies n reals from double to T
 template <typename T>
 void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
@@ -1376,7 +1496,9 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   template void launch_spectrum<T>(const Geo&, const void*, double*, double*, double*, double*, double*, double*,     \
                                    hipStream_t);                                                                      \
   template void launch_forcing_sum<T>(const Geo&, const void*, const int*, int, DevState*, double, hipStream_t);               \
-  template void launch_cast_copy<T>(const void*, void*, size_t, hipStream_t);
+  template void launch_cast_copy<T>(const void*, void*, size_t, hipStream_t);                                       \
+  template void launch_seed<T>(const Geo&, void*, int, unsigned long long, double, double, double*, double*,           \
+                               hipStream_t);
 
 FFS_INST(double)
 FFS_INST(float)

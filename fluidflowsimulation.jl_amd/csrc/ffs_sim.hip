@@ -857,6 +857,12 @@ struct ffs_batch {
   }
 };
 
+#define FFS_DISPATCH_S(fn, ...)            \
+  do {                                     \
+    if (s->fp32) fn<float>(__VA_ARGS__);   \
+    else fn<double>(__VA_ARGS__);          \
+  } while (0)
+
 namespace {
 
 int fail(ffs_sim* s, const FfsError& e) {
@@ -992,6 +998,27 @@ int ffs_set_field(ffs_sim* s, int field, int space, const double* host, size_t n
     } else {
       throw FfsError(FFS_ERR_ARG, "unknown space");
     }
+  });
+}
+
+int ffs_seed_velocity(ffs_sim* s, uint64_t seed, double k0, double energy) {
+  if (!s) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    if (s->initialized) throw FfsError(FFS_ERR_STATE, "fields are set before ffs_initialize");
+    if (!(k0 > 0) || !(energy > 0)) throw FfsError(FFS_ERR_ARG, "k0 and energy must be positive");
+    FFS_DISPATCH_S(launch_seed, s->g, s->u, 3, (unsigned long long)seed, k0, energy, s->partials, s->red, s->stream);
+    HIPCHECK(hipStreamSynchronize(s->stream));
+  });
+}
+
+int ffs_seed_density(ffs_sim* s, uint64_t seed, double k0, double rms) {
+  if (!s) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    if (!s->dens) throw FfsError(FFS_ERR_ARG, "the run has no density field");
+    if (s->initialized) throw FfsError(FFS_ERR_STATE, "fields are set before ffs_initialize");
+    if (!(k0 > 0) || !(rms > 0)) throw FfsError(FFS_ERR_ARG, "k0 and rms must be positive");
+    FFS_DISPATCH_S(launch_seed, s->g, s->rho, 1, (unsigned long long)seed, k0, rms, s->partials, s->red, s->stream);
+    HIPCHECK(hipStreamSynchronize(s->stream));
   });
 }
 

@@ -88,6 +88,14 @@ class Simulation:
         c = buf[0::2] + 1j * buf[1::2]
         return c.reshape((self.nx // 2 + 1, self.ny, self.nz), order="F")
 
+    def seed_velocity(self, seed, k0=4.0, energy=0.5):
+        """Seeded random-phase solenoidal velocity generated on the device (see ffs_seed_velocity)."""
+        self._check(self._lib.ffs_seed_velocity(self._h, int(seed) & (2**64 - 1), float(k0), float(energy)))
+
+    def seed_density(self, seed, k0=4.0, rms=0.01):
+        """Seeded Gaussian random density field generated on the device (see ffs_seed_density)."""
+        self._check(self._lib.ffs_seed_density(self._h, int(seed) & (2**64 - 1), float(k0), float(rms)))
+
     def set_density(self, rho):
         """rho.$start of the Boussinesq approximation (density_stratification_types.jl:165-166)."""
         self.set_field(_lib.FFS_FIELD_RHO, rho)

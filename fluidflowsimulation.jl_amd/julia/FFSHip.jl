@@ -115,6 +115,11 @@ function set_field!(s::FFSSimulation, field::Integer, a::Array{Float64}; space =
                                s.h, field, space, a, length(a)), s.h)
 end
 
+seed_velocity!(s::FFSSimulation, seed::Integer; k0 = 4.0, energy = 0.5) =
+    check(ccall((:ffs_seed_velocity, libffs), Cint, (Ptr{Cvoid}, UInt64, Float64, Float64), s.h, seed, k0, energy), s.h)
+seed_density!(s::FFSSimulation, seed::Integer; k0 = 4.0, rms = 0.01) =
+    check(ccall((:ffs_seed_density, libffs), Cint, (Ptr{Cvoid}, UInt64, Float64, Float64), s.h, seed, k0, rms), s.h)
+
 function get_field(s::FFSSimulation, field::Integer; space = FFS_SPACE_REAL)
     a = Vector{Float64}(undef, padded_length(s))
     GC.@preserve a check(ccall((:ffs_get_field, libffs), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Csize_t),

@@ -119,6 +119,18 @@ int ffs_create(const ffs_params* p, ffs_sim** out);
  * half spectrum) or nx*ny*nz (unpadded real, space must be REAL). */
 int ffs_set_field(ffs_sim* s, int field, int space, const double* host, size_t n_doubles);
 
+/* Seeded random-phase initial fields generated on the device (no host field, no
+ * transfer).  The reference has no built-in generator (its runs read u1.0 ...
+ * produced elsewhere); these produce the `u?.0` / `rho.0` a case would load:
+ * velocity u_hat = A(|k|) (I - k k/|k|^2) xi with A = |k| exp(-(|k|/k0)^2)/sqrt(4 pi)
+ * (shell energy ~ k^4 exp(-2 (k/k0)^2)), scaled to 1/2 <|u|^2> = energy; density
+ * rho_hat = A(|k|) xi scaled to <rho^2> = rms^2.  xi is a complex Gaussian per
+ * (component, mode) from a counter-based generator keyed on the seed (two
+ * SplitMix64 hashes -> Box-Muller), conjugate-symmetric on the kx = 0 plane, so
+ * the field is independent of the grid's launch shape and reproducible on the host. */
+int ffs_seed_velocity(ffs_sim* s, uint64_t seed, double k0, double energy);
+int ffs_seed_density(ffs_sim* s, uint64_t seed, double k0, double rms);
+
 /* Replaces real!(s.u) + write("u1.$it", s.u.rr.x) (output.jl:6-8) and
  * complex views of the state: copies one component out in the padded layout. */
 int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n_doubles);
