@@ -21,7 +21,34 @@ struct Geo {
   const double* kx;  // wavenumbers per compact index
   const double* ky;
   const double* kz;
+  // The retained set along y (z) is two contiguous blocks of FFT indices: [0, fy) and
+  // [ny - (my - fy), ny).  Kernels map indices arithmetically, so no table load sits in
+  // front of a data load; wavenumbers come from the tables (their loads depend only on the
+  // index and overlap the data loads; a double division per element costs more).
+  int fy, fz;
+  double lx, ly, lz;  // domain factors L (the box is 2 pi L per side): k = signed index / L
 };
+
+// FFT index -> compact index, or -1 for a dealiased row
+__host__ __device__ __forceinline__ int y_compact(const Geo& g, int p) {
+  return p < g.fy ? p : (p >= g.ny - (g.my - g.fy) ? p - g.ny + g.my : -1);
+}
+__host__ __device__ __forceinline__ int z_compact(const Geo& g, int p) {
+  return p < g.fz ? p : (p >= g.nz - (g.mz - g.fz) ? p - g.nz + g.mz : -1);
+}
+// compact index -> FFT index
+__host__ __device__ __forceinline__ int y_fft(const Geo& g, int c) { return c < g.fy ? c : c + g.ny - g.my; }
+__host__ __device__ __forceinline__ int z_fft(const Geo& g, int c) { return c < g.fz ? c : c + g.nz - g.mz; }
+// wavenumbers exactly as rfftfreq / fftfreq (Globals.jl:95-104): signed index / L
+__host__ __device__ __forceinline__ double kx_of(const Geo& g, int cx) { return (double)cx / g.lx; }
+__host__ __device__ __forceinline__ double ky_of(const Geo& g, int cy) {
+  const int j = y_fft(g, cy);
+  return (double)(j < (g.ny + 1) / 2 ? j : j - g.ny) / g.ly;
+}
+__host__ __device__ __forceinline__ double kz_of(const Geo& g, int cz) {
+  const int j = z_fft(g, cz);
+  return (double)(j < (g.nz + 1) / 2 ? j : j - g.nz) / g.lz;
+}
 
 // Device-resident scalar state of a run (so variable-dt runs never stall the
 // host): mirrors the Refs of Adams_Bashforth3rdO / ETD3rdO and Simulation.

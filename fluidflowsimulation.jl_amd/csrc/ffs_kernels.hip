@@ -29,6 +29,27 @@ namespace fsk {
 // columns per tile of the y/z passes: one 128-byte line of x-consecutive values
 template <typename T> __host__ __device__ constexpr int col_tx(int) { return 128 / (2 * (int)sizeof(T)); }
 template <int N> __host__ __device__ constexpr int tper() { return N / default_ppt(N); }
+// The inverse-z pass that forms w = ik x u (and rho, S) on load issues two loads per point in its
+// first stage; 8 points per thread (radix <= 8) instead of 16 halves the registers those loads hold.
+// FFS_COL_PPT8: 0 = that variant only, 1 = every column pass, -1 = none.
+#ifndef FFS_COL_PPT8
+#define FFS_COL_PPT8 1
+#endif
+// FFS_ZMAP_ARITH: the inverse-z load maps z indices arithmetically (1) or through the table (0)
+// FFS_CURL_SPLIT: the inverse-z pass that forms w (and rho, S) runs one field per block (1) or
+// loops over the fields in each block (0)
+#ifndef FFS_CURL_SPLIT
+#define FFS_CURL_SPLIT 1
+#endif
+#ifndef FFS_ZMAP_ARITH
+#define FFS_ZMAP_ARITH 1
+#endif
+template <int N> __host__ __device__ constexpr bool ppt8_ok() { return N % 8 == 0 && plan_even(N, N / 8, 8); }
+template <int N, bool CURL> __host__ __device__ constexpr bool col_ppt8() {
+  return ppt8_ok<N>() && ((FFS_COL_PPT8 == 0 && CURL) || FFS_COL_PPT8 == 1);
+}
+template <int N, bool CURL> __host__ __device__ constexpr int col_tper() { return col_ppt8<N, CURL>() ? N / 8 : tper<N>(); }
+template <int N, bool CURL> __host__ __device__ constexpr int col_maxr() { return col_ppt8<N, CURL>() ? 8 : 16; }
 template <int N> __host__ __device__ constexpr int xrows() { return (64 / tper<N>()) > 0 ? (64 / tper<N>()) : 1; }
 // rows per block of the general x pass: at most 256 threads
 template <int N, int NSL> __host__ __device__ constexpr int xgen_rows() {
@@ -37,11 +58,17 @@ template <int N, int NSL> __host__ __device__ constexpr int xgen_rows() {
 
 // Tunables of the fused x pass: points per lane, minimum waves per SIMD the
 // register allocation must allow, and the LDS padding of the exchange.
-#ifndef FFS_XNL_PPT
-#define FFS_XNL_PPT 8
+#ifndef FFS_XNL_PPT64
+#define FFS_XNL_PPT64 8
+#endif
+#ifndef FFS_XNL_PPT32
+#define FFS_XNL_PPT32 8
 #endif
 #ifndef FFS_XNL_WAVES
 #define FFS_XNL_WAVES 1
+#endif
+#ifndef FFS_XNL_PREFETCH
+#define FFS_XNL_PREFETCH 0  // 1: request every input of a row before its first transform
 #endif
 #ifndef FFS_XNL_PAD_SHIFT
 #define FFS_XNL_PAD_SHIFT 0  // 0: no padding; s > 0: one slot per 2^s values
@@ -51,12 +78,14 @@ template <int N, int NSL> __host__ __device__ constexpr int xgen_rows() {
 #endif
 
 // points per lane of the register-resident x pass and whether it applies
-template <int N> __host__ __device__ constexpr int xreg_ppt() { return FFS_XNL_PPT; }
-template <int N> __host__ __device__ constexpr int xreg_ldx() {
+template <typename T, int N> __host__ __device__ constexpr int xreg_ppt() {
+  return sizeof(T) == 8 ? FFS_XNL_PPT64 : FFS_XNL_PPT32;
+}
+template <typename T, int N> __host__ __device__ constexpr int xreg_ldx() {
   return FFS_XNL_PAD_SHIFT > 0 ? N + (N >> FFS_XNL_PAD_SHIFT) + 1 : N + 1;
 }
-template <int N> __host__ __device__ constexpr bool xreg_ok() {
-  constexpr int P = FFS_XNL_PPT;
+template <typename T, int N> __host__ __device__ constexpr bool xreg_ok() {
+  constexpr int P = xreg_ppt<T, N>();
   return N % P == 0 && 64 % (N / P) == 0 && plan_even(N, N / P, P);
 }
 
@@ -108,16 +137,20 @@ template <typename T, int N, int KIND, bool CURL>
 __global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ out,
                                             ColAux<T> aux, const C<T>* __restrict__ tw_g, int nzero, T scale) {
   constexpr int TX = col_tx<T>(N);
-  constexpr int TP = tper<N>();
+  constexpr int TP = col_tper<N, CURL>();
+  constexpr int MAXR = col_maxr<N, CURL>();
   extern __shared__ unsigned char smem_raw[];
   C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
   const C<T>* tw = stage_twiddles<T, N>(lds + N * TX, tw_g);
   __syncthreads();
   const int c = threadIdx.x % TX;
   const int q = threadIdx.x / TX;
-  const int cx = blockIdx.x * TX + c;
+  // the split variant of the inverse-z pass puts the field in blockIdx.x (fastest), so the blocks
+  // that read the same u tile run together and share it through L2
+  constexpr bool SPLIT = CURL && FFS_CURL_SPLIT;
+  const int cx = (SPLIT ? blockIdx.y : blockIdx.x) * TX + c;
   const bool valid = cx < g.mx;
-  const int b = blockIdx.y;
+  const int b = SPLIT ? blockIdx.z : blockIdx.y;
   const size_t sx = g.sx;
   const size_t Ssp = sx * g.my * g.mz;  // compact spectral
   const size_t S1 = sx * g.my * g.nz;   // (sx, my, nz)
@@ -130,13 +163,13 @@ __global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* _
     // nonlinear term -- u (0..2), w = ik x u (3..5) (fourierspacep1!,
     // models.jl:88-95), rho (if aux.rho), S = sym(ik (x) u) (xx,xy,xz,yy,yz; if
     // aux.les) -- so the u tile is fetched from HBM once and re-read from L2
-    const int f0 = CURL ? 0 : blockIdx.z;
-    const int f1 = CURL ? nzero : blockIdx.z + 1;
+    const int f0 = SPLIT ? blockIdx.x : (CURL ? 0 : blockIdx.z);
+    const int f1 = SPLIT ? blockIdx.x + 1 : (CURL ? nzero : blockIdx.z + 1);
     const T kx = valid ? (T)g.kx[cx] : (T)0, ky = (T)g.ky[cy];
     for (int f = f0; f < f1; ++f) {
       if (f != f0) __syncthreads();
       auto ld = [&](int p, int, int) -> C<T> {
-        const int cz = g.zinv[p];
+        const int cz = FFS_ZMAP_ARITH ? z_compact(g, p) : g.zinv[p];
         if (!valid || cz < 0) return mk<T>(0, 0);
         const size_t idx = cx + sx * (cy + (size_t)g.my * cz);
         if (!CURL || f < 3) return in[f * Ssp + idx];
@@ -172,19 +205,19 @@ __global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* _
       auto st = [&](int p, int, int, C<T> v) {
         if (valid) out[f * S1 + cx + sx * (cy + (size_t)g.my * p)] = v;
       };
-      Fft<T, N, +1, TP>::run(q, tw, ld, st, ldsf);
+      Fft<T, N, +1, TP, false, false, SYNC_BLOCK, MAXR>::run(q, tw, ld, st, ldsf);
     }
   } else if constexpr (KIND == INV_Y) {
     const int z = b, f = blockIdx.z;
     auto ld = [&](int p, int, int) -> C<T> {
-      const int cy = g.yinv[p];
+      const int cy = y_compact(g, p);
       if (!valid || cy < 0) return mk<T>(0, 0);
       return in[f * S1 + cx + sx * (cy + (size_t)g.my * z)];
     };
     auto st = [&](int p, int, int, C<T> v) {
       if (valid) out[f * S2 + cx + sx * (p + (size_t)g.ny * z)] = v;
     };
-    Fft<T, N, +1, TP>::run(q, tw, ld, st, ldsf);
+    Fft<T, N, +1, TP, false, false, SYNC_BLOCK, MAXR>::run(q, tw, ld, st, ldsf);
   } else if constexpr (KIND == FWD_Y) {
     const int z = b, f = blockIdx.z;
     auto ld = [&](int p, int, int) -> C<T> {
@@ -192,10 +225,10 @@ __global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* _
       return in[f * S2 + cx + sx * (p + (size_t)g.ny * z)];
     };
     auto st = [&](int p, int, int, C<T> v) {
-      const int cy = g.yinv[p];
+      const int cy = y_compact(g, p);
       if (valid && cy >= 0) out[f * S1 + cx + sx * (cy + (size_t)g.my * z)] = v;
     };
-    Fft<T, N, -1, TP>::run(q, tw, ld, st, ldsf);
+    Fft<T, N, -1, TP, false, false, SYNC_BLOCK, MAXR>::run(q, tw, ld, st, ldsf);
   } else {  // FWD_Z
     const int cy = b, f = blockIdx.z;
     auto ld = [&](int p, int, int) -> C<T> {
@@ -203,7 +236,7 @@ __global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* _
       return in[f * S1 + cx + sx * (cy + (size_t)g.my * p)];
     };
     auto st = [&](int p, int, int, C<T> v) {
-      const int cz = g.zinv[p];
+      const int cz = z_compact(g, p);
       if (valid && cz >= 0) {
         C<T> r = cscale<T>(v, scale);
         if (f < nzero && cx == 0 && cy == 0 && cz == 0) r = mk<T>(0, 0);
@@ -211,7 +244,7 @@ __global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* _
         aux.dst[f][o] = r;  // rhs components, density flux, tau, 1/2|u|^2 ...
       }
     };
-    Fft<T, N, -1, TP>::run(q, tw, ld, st, ldsf);
+    Fft<T, N, -1, TP, false, false, SYNC_BLOCK, MAXR>::run(q, tw, ld, st, ldsf);
   }
 }
 
@@ -219,14 +252,15 @@ template <typename T, int N, int KIND, bool CURL>
 static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, const void* tw, int nzero, double scale,
                     hipStream_t s, const ColAux<T>& aux = ColAux<T>{}) {
   constexpr int TX = col_tx<T>(N);
-  constexpr int TP = tper<N>();
+  constexpr int TP = col_tper<N, CURL>();
   const size_t lds = (size_t)(N * TX + N) * sizeof(C<T>);
   static bool attr = false;
   if (!attr) {
     hipFuncSetAttribute((const void*)&k_col<T, N, KIND, CURL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr = true;
   }
-  dim3 grid((g.mx + TX - 1) / TX, nb, CURL ? 1 : nf);
+  const dim3 grid = (CURL && FFS_CURL_SPLIT) ? dim3(nzero, (g.mx + TX - 1) / TX, nb)
+                                             : dim3((g.mx + TX - 1) / TX, nb, CURL ? 1 : nf);
   k_col<T, N, KIND, CURL><<<grid, TX * TP, lds, s>>>(g, (const C<T>*)in, (C<T>*)out, aux, (const C<T>*)tw, nzero,
                                                     (T)scale);
 }
@@ -429,14 +463,14 @@ __global__ void __launch_bounds__(256) k_x_gen(Geo g, const C<T>* __restrict__ a
 template <typename T, int N, bool KE>
 __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
                                                  const C<T>* __restrict__ tw_g, unsigned long long* __restrict__ umax) {
-  constexpr int PPT = xreg_ppt<N>();
+  constexpr int PPT = xreg_ppt<T, N>();
   constexpr int TP = N / PPT;
   constexpr int GPW = 64 / TP;  // rows per wavefront
-  constexpr int LDX = xreg_ldx<N>();
+  constexpr int LDX = xreg_ldx<T, N>();
   constexpr int NSTG = plan_ns(N, PPT);
   constexpr int M0 = (N / plan_r(N, 0, PPT)) / TP;         // register stride of first-stage legs
   constexpr int ML = (N / plan_r(N, NSTG - 1, PPT)) / TP;  // register stride of last-stage legs
-  static_assert(xreg_ok<N>(), "register-resident x pass needs an even plan");
+  static_assert(xreg_ok<T, N>(), "register-resident x pass needs an even plan");
   extern __shared__ unsigned char smem_raw[];
   C<T>* twl = reinterpret_cast<C<T>*>(smem_raw);
   C<T>* lds = twl + N;
@@ -457,24 +491,37 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
   };
 
   C<T> R[3][PPT];
+  // Z = A + iB of a pair of real rows at position p, from their retained half spectra
+  auto zval = [&](const C<T>* A, const C<T>* B, int p) -> C<T> {
+    if (!row_ok) return mk<T>(0, 0);
+    if (p < mx) {
+      const C<T> a = A[p], b = B[p];
+      if (p == 0 || 2 * p == N) return mk<T>(a.x, b.x);
+      return mk<T>(a.x - b.y, a.y + b.x);
+    }
+    const int k = N - p;
+    if (k < mx && 2 * p != N) {
+      const C<T> a = A[k], b = B[k];
+      return mk<T>(a.x + b.y, b.x - a.y);
+    }
+    return mk<T>(0, 0);
+  };
+#if FFS_XNL_PREFETCH
+  // every input of the row is requested before the first transform (lane q's first-stage legs
+  // are its register positions p = q + TP m), so the loads of all three pairs are in flight together
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) R[j][m] = zval(a2 + (size_t)(2 * j) * S2 + roff, a2 + (size_t)(2 * j + 1) * S2 + roff, q + TP * m);
+#endif
   // inverse transforms of the pairs (ux,uy) (uz,wx) (wy,wz); Z built from the half spectra on load
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     const C<T>* A = a2 + (size_t)(2 * j) * S2 + roff;
     const C<T>* B = a2 + (size_t)(2 * j + 1) * S2 + roff;
-    auto ld = [&](int p, int, int) -> C<T> {
-      if (!row_ok) return mk<T>(0, 0);
-      if (p < mx) {
-        const C<T> a = A[p], b = B[p];
-        if (p == 0 || 2 * p == N) return mk<T>(a.x, b.x);
-        return mk<T>(a.x - b.y, a.y + b.x);
-      }
-      const int k = N - p;
-      if (k < mx && 2 * p != N) {
-        const C<T> a = A[k], b = B[k];
-        return mk<T>(a.x + b.y, b.x - a.y);
-      }
-      return mk<T>(0, 0);
+    auto ld = [&](int p, int t, int r) -> C<T> {
+      if constexpr (FFS_XNL_PREFETCH) return R[j][t + r * M0];
+      else return zval(A, B, p);
     };
     auto st = [&](int, int t, int r, C<T> v) { R[j][t + r * ML] = v; };
     if (j > 0) __syncwarp();
@@ -641,12 +688,12 @@ static void run_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, un
                      unsigned long long* rhomax, const void* tw, hipStream_t s) {
   const int nrows = g.ny * g.nz;
   const bool plain = !ops.rho && !ops.les;
-  if constexpr (xreg_ok<N>()) {
+  if constexpr (xreg_ok<T, N>()) {
     if (plain) {
-      constexpr int TP = N / xreg_ppt<N>();
+      constexpr int TP = N / xreg_ppt<T, N>();
       constexpr int WPB = 2;  // wavefronts per block
       constexpr int RPB = WPB * (64 / TP);
-      const size_t lds = ((size_t)RPB * xreg_ldx<N>() + N) * sizeof(C<T>);
+      const size_t lds = ((size_t)RPB * xreg_ldx<T, N>() + N) * sizeof(C<T>);
       static bool attr = false;
       if (!attr) {
         (void)hipFuncSetAttribute((const void*)&k_x_nl_reg<T, N, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1407,7 +1454,7 @@ __global__ void __launch_bounds__(256) k_seed(Geo g, C<T>* __restrict__ out, int
     }
     const double kx = g.kx[cx], ky = g.ky[cy], kz = g.kz[cz];
     const double K2 = kx * kx + ky * ky + kz * kz;
-    int iy = signed_index(g.ymap[cy], g.ny), iz = signed_index(g.zmap[cz], g.nz);
+    int iy = signed_index(y_fft(g, cy), g.ny), iz = signed_index(z_fft(g, cz), g.nz);
     bool conj = false;
     if (cx == 0 && (iy < 0 || (iy == 0 && iz < 0))) {
       iy = -iy;

@@ -264,6 +264,24 @@ struct ffs_sim {
     g.mx = mx;
     g.my = (int)ymap.size();
     g.mz = (int)zmap.size();
+    // the kernels' arithmetic maps (two contiguous retained blocks per axis, k = index / L)
+    g.fy = 0;
+    while (g.fy < g.my && ymap[g.fy] == g.fy) ++g.fy;
+    g.fz = 0;
+    while (g.fz < g.mz && zmap[g.fz] == g.fz) ++g.fz;
+    g.lx = p.x_domain;
+    g.ly = p.y_domain;
+    g.lz = p.z_domain;
+    for (int j = 0; j < ny; ++j)
+      if (y_compact(g, j) != yinv[j]) throw FfsError(FFS_ERR_ARG, "retained y modes are not two contiguous blocks");
+    for (int k = 0; k < nz; ++k)
+      if (z_compact(g, k) != zinv[k]) throw FfsError(FFS_ERR_ARG, "retained z modes are not two contiguous blocks");
+    for (int c = 0; c < mx; ++c)
+      if (kx_of(g, c) != kxc[c]) throw FfsError(FFS_ERR_ARG, "kx map mismatch");
+    for (int c = 0; c < g.my; ++c)
+      if (ky_of(g, c) != kyc[c]) throw FfsError(FFS_ERR_ARG, "ky map mismatch");
+    for (int c = 0; c < g.mz; ++c)
+      if (kz_of(g, c) != kzc[c]) throw FfsError(FFS_ERR_ARG, "kz map mismatch");
     const int line = fp32 ? 16 : 8;  // complex values per 128-byte line
     g.sx = (mx + line - 1) / line * line;
     g.ymap = upload(ymap);
