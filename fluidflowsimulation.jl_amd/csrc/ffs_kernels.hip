@@ -41,6 +41,10 @@ template <int N> __host__ __device__ constexpr int tper() { return N / default_p
 #ifndef FFS_CURL_SPLIT
 #define FFS_CURL_SPLIT 1
 #endif
+// FFS_COL_BRANCHFREE: the inverse column passes load every position and zero by select (1)
+#ifndef FFS_COL_BRANCHFREE
+#define FFS_COL_BRANCHFREE 0
+#endif
 #ifndef FFS_ZMAP_ARITH
 #define FFS_ZMAP_ARITH 1
 #endif
@@ -62,16 +66,19 @@ template <int N, int NSL> __host__ __device__ constexpr int xgen_rows() {
 #define FFS_XNL_PPT64 8
 #endif
 #ifndef FFS_XNL_PPT32
-#define FFS_XNL_PPT32 8
+#define FFS_XNL_PPT32 16
 #endif
 #ifndef FFS_XNL_WAVES
 #define FFS_XNL_WAVES 1
 #endif
+#ifndef FFS_XNL_BRANCHFREE
+#define FFS_XNL_BRANCHFREE 1  // 1: the x pass loads its inputs without branches
+#endif
 #ifndef FFS_XNL_PREFETCH
-#define FFS_XNL_PREFETCH 0  // 1: request every input of a row before its first transform
+#define FFS_XNL_PREFETCH 1  // 1: request every input of a row before its first transform
 #endif
 #ifndef FFS_XNL_PAD_SHIFT
-#define FFS_XNL_PAD_SHIFT 0  // 0: no padding; s > 0: one slot per 2^s values
+#define FFS_XNL_PAD_SHIFT 3  // 0: no padding; s > 0: one slot per 2^s values
 #endif
 #ifndef FFS_COL_WAVES
 #define FFS_COL_WAVES 1
@@ -168,9 +175,14 @@ __global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* _
     const T kx = valid ? (T)g.kx[cx] : (T)0, ky = (T)g.ky[cy];
     for (int f = f0; f < f1; ++f) {
       if (f != f0) __syncthreads();
-      auto ld = [&](int p, int, int) -> C<T> {
-        const int cz = FFS_ZMAP_ARITH ? z_compact(g, p) : g.zinv[p];
+      auto ldv = [&](int p) -> C<T> {
+        const int cz0 = FFS_ZMAP_ARITH ? z_compact(g, p) : g.zinv[p];
+#if FFS_COL_BRANCHFREE
+        const int cz = cz0 < 0 ? 0 : cz0;  // dealiased rows load row 0 and are zeroed by the caller
+#else
+        const int cz = cz0;
         if (!valid || cz < 0) return mk<T>(0, 0);
+#endif
         const size_t idx = cx + sx * (cy + (size_t)g.my * cz);
         if (!CURL || f < 3) return in[f * Ssp + idx];
         const T kz = (T)g.kz[cz];
@@ -202,6 +214,17 @@ __global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* _
         const C<T> sum = mk<T>((T)0.5 * (ka * ub.x + kb * ua.x), (T)0.5 * (ka * ub.y + kb * ua.y));
         return mk<T>(-sum.y, sum.x);
       };
+      // branch-free: every lane loads (columns past mx read the in-bounds row padding), so the
+      // first stage's loads issue back to back; the zero of dealiased / padding positions is a select
+      auto ld = [&](int p, int, int) -> C<T> {
+#if FFS_COL_BRANCHFREE
+        const C<T> v = ldv(p);
+        const bool keep = valid && (FFS_ZMAP_ARITH ? z_compact(g, p) : g.zinv[p]) >= 0;
+        return keep ? v : mk<T>(0, 0);
+#else
+        return ldv(p);
+#endif
+      };
       auto st = [&](int p, int, int, C<T> v) {
         if (valid) out[f * S1 + cx + sx * (cy + (size_t)g.my * p)] = v;
       };
@@ -211,8 +234,13 @@ __global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* _
     const int z = b, f = blockIdx.z;
     auto ld = [&](int p, int, int) -> C<T> {
       const int cy = y_compact(g, p);
+#if FFS_COL_BRANCHFREE
+      const C<T> v = in[f * S1 + cx + sx * ((cy < 0 ? 0 : cy) + (size_t)g.my * z)];
+      return (valid && cy >= 0) ? v : mk<T>(0, 0);
+#else
       if (!valid || cy < 0) return mk<T>(0, 0);
       return in[f * S1 + cx + sx * (cy + (size_t)g.my * z)];
+#endif
     };
     auto st = [&](int p, int, int, C<T> v) {
       if (valid) out[f * S2 + cx + sx * (p + (size_t)g.ny * z)] = v;
@@ -493,6 +521,18 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
   C<T> R[3][PPT];
   // Z = A + iB of a pair of real rows at position p, from their retained half spectra
   auto zval = [&](const C<T>* A, const C<T>* B, int p) -> C<T> {
+#if FFS_XNL_BRANCHFREE
+    // straight-line: every lane loads (index 0 where the value is zero), so no branch
+    // separates the loads and they issue back to back
+    const int k = N - p;
+    const bool lo = p < mx;
+    const bool hi = !lo && k < mx && 2 * p != N;
+    const int idx = lo ? p : (hi ? k : 0);
+    const C<T> a = A[idx], b = B[idx];
+    const bool real = p == 0 || 2 * p == N;
+    C<T> z = lo ? (real ? mk<T>(a.x, b.x) : mk<T>(a.x - b.y, a.y + b.x)) : mk<T>(a.x + b.y, b.x - a.y);
+    return (row_ok && (lo || hi)) ? z : mk<T>(0, 0);
+#else
     if (!row_ok) return mk<T>(0, 0);
     if (p < mx) {
       const C<T> a = A[p], b = B[p];
@@ -505,6 +545,7 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
       return mk<T>(a.x + b.y, b.x - a.y);
     }
     return mk<T>(0, 0);
+#endif
   };
 #if FFS_XNL_PREFETCH
   // every input of the row is requested before the first transform (lane q's first-stage legs
