@@ -123,6 +123,7 @@ struct UpdateArgs {
   double alpha;        // density diffusivity nu / Pr (ETD only, as in the reference)
   double drhodz;       // mean density gradient
   double gz;           // g_z = zAcceleration / referenceDensity
+  double* time;        // s.time[] (DevState::time): advanced by dt once per step, or null
 };
 template <typename T>
 void launch_update(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
@@ -148,6 +149,16 @@ void launch_forcing_sum(const Geo& g, const void* u, const int* band, int nband,
 
 // set_dt! on device; etd selects the ETD3rdO history rule; gnorm > 0 adds the density limit
 void launch_set_dt(DevState* st, int it, int tit, double cfl, double dx, int etd, double gnorm, hipStream_t s);
+// set_dt (if do_dt) and the forcing band sum (if nband > 0) in one single-block launch
+struct ScalarArgs {
+  int do_dt, it, tit, etd;
+  double cfl, dx, gnorm;
+  const int* band;
+  int nband;
+  double power;
+};
+template <typename T>
+void launch_step_scalars(const Geo& g, const void* u, DevState* st, const ScalarArgs& a, hipStream_t s);
 // AB3 coefficients and ETD3 scalars of the current dt, dt2, dt3 (fixed-step runs)
 void host_step_coefficients(DevState* st);
 void launch_end_step(DevState* st, hipStream_t s);

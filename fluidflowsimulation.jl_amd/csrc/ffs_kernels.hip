@@ -867,6 +867,8 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
                                                const DevState* __restrict__ st, UpdateArgs a) {
   const size_t n = (size_t)g.sx * g.my * g.mz;
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // s.time[] += dt, once per step (no kernel reads the time during the update)
+  if (a.time && idx == 0) *a.time += st->dt;
   if (idx >= n) return;
   const int cx = idx % g.sx;
   if (cx >= g.mx) return;  // row padding
@@ -1389,7 +1391,7 @@ __host__ __device__ inline void etd_scalars(DevState* st) {
   e[18] = dt3 * (dt2 + dt3);               // e9
 }
 
-__global__ void k_set_dt(DevState* st, int it, int tit, double cfl, double dx, int etd, double gnorm) {
+__device__ void set_dt_body(DevState* st, int it, int tit, double cfl, double dx, int etd, double gnorm) {
   const double umax = __longlong_as_double((long long)st->umax);
   st->umax_d = umax;
   st->umax = 0ull;  // cleared for the next step's atomicMax
@@ -1420,6 +1422,42 @@ __global__ void k_set_dt(DevState* st, int it, int tit, double cfl, double dx, i
   }
   ab3_coefficients(st);
   etd_scalars(st);
+}
+
+__global__ void k_set_dt(DevState* st, int it, int tit, double cfl, double dx, int etd, double gnorm) {
+  set_dt_body(st, it, tit, cfl, dx, etd, gnorm);
+}
+
+// one block of 64 threads: the forcing band sum (as k_forcing_sum) and, on thread 0, set_dt
+template <typename T>
+__global__ void k_step_scalars(Geo g, const C<T>* __restrict__ u, DevState* st, ScalarArgs a) {
+  if (a.nband > 0) {
+    const size_t n = (size_t)g.sx * g.my * g.mz;
+    double acc = 0;
+    for (int b = threadIdx.x; b < a.nband; b += blockDim.x) {
+      const int idx = a.band[b];
+      const double w = (idx % g.sx) > 0 ? 2.0 : 1.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const C<T> v = u[c * n + idx];
+        acc += w * ((double)v.x * v.x + (double)v.y * v.y);
+      }
+    }
+    __shared__ double red[64];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) st->force_factor = red[0] > 0 ? a.power / red[0] : 0.0;
+  }
+  if (a.do_dt && threadIdx.x == 0) set_dt_body(st, a.it, a.tit, a.cfl, a.dx, a.etd, a.gnorm);
+}
+
+template <typename T>
+void launch_step_scalars(const Geo& g, const void* u, DevState* st, const ScalarArgs& a, hipStream_t s) {
+  k_step_scalars<T><<<1, 64, 0, s>>>(g, (const C<T>*)u, st, a);
 }
 
 void launch_set_dt(DevState* st, int it, int tit, double cfl, double dx, int etd, double gnorm, hipStream_t s) {
@@ -1585,6 +1623,7 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
                                    hipStream_t);                                                                      \
   template void launch_forcing_sum<T>(const Geo&, const void*, const int*, int, DevState*, double, hipStream_t);               \
   template void launch_cast_copy<T>(const void*, void*, size_t, hipStream_t);                                       \
+  template void launch_step_scalars<T>(const Geo&, const void*, DevState*, const ScalarArgs&, hipStream_t);          \
   template void launch_seed<T>(const Geo&, void*, int, unsigned long long, double, double, double*, double*,           \
                                hipStream_t);
 

@@ -515,21 +515,26 @@ struct ffs_sim {
     launch("fwd_z", [&] { FFS_DISPATCH(launch_fwd_z, g, w2, dst, nf, 3, scale, tw_z, stream); });
   }
 
-  void set_dt() {
-    if (!p.variable_dt) return;
-    const double dx = 2 * M_PI * p.y_domain / p.ny;
-    const double gnorm = dens ? std::fabs(gz) : 0.0;
-    launch("set_dt", [&] {
-      // it matters only through it <= 3 (cfl ramp) and tit through tit == 0 (AB3 halving)
-      launch_set_dt(st, (int)std::min<int64_t>(iteration, 4), (int)std::min<int64_t>(tit, 1), p.cfl, dx,
-                    etd ? 1 : 0, gnorm, stream);
-    });
+  // set_dt! (variable dt) and the forcing band sum, as one single-block kernel
+  void step_scalars(bool do_dt, bool do_force) {
+    ScalarArgs a{};
+    a.do_dt = (do_dt && p.variable_dt) ? 1 : 0;
+    // it matters only through it <= 3 (cfl ramp) and tit through tit == 0 (AB3 halving)
+    a.it = (int)std::min<int64_t>(iteration, 4);
+    a.tit = (int)std::min<int64_t>(tit, 1);
+    a.etd = etd ? 1 : 0;
+    a.cfl = p.cfl;
+    a.dx = 2 * M_PI * p.y_domain / p.ny;
+    a.gnorm = dens ? std::fabs(gz) : 0.0;
+    const bool force = do_force && p.forcing == FFS_FORCING_SHELL && nband > 0;
+    a.band = band;
+    a.nband = force ? nband : 0;
+    a.power = p.forcing_power;
+    if (!a.do_dt && !a.nband) return;
+    launch("step_scalars", [&] { FFS_DISPATCH(launch_step_scalars, g, u, st, a, stream); });
   }
-
-  void forcing() {
-    if (p.forcing == FFS_FORCING_SHELL && nband > 0)
-      launch("forcing_sum", [&] { FFS_DISPATCH(launch_forcing_sum, g, u, band, nband, st, p.forcing_power, stream); });
-  }
+  void set_dt() { step_scalars(true, false); }
+  void forcing() { step_scalars(false, true); }
 
   void queue_stats() {
     // find a free slot (complete the oldest pending record if all are busy)
@@ -656,8 +661,9 @@ struct ffs_sim {
   // step 2, ETD steps 1, 2 and >= 3)
   void enqueue_update() {
     const UpdateArgs a = update_args((int)std::min<int64_t>(tit + 1, 3));
-    launch("update", [&] { FFS_DISPATCH(launch_update, g, u, rhs, fm1, fm2, st, a, stream); });
-    launch("end_step", [&] { launch_end_step(st, stream); });
+    UpdateArgs b = a;
+    b.time = &st->time;  // the update advances s.time[] by dt
+    launch("update", [&] { FFS_DISPATCH(launch_update, g, u, rhs, fm1, fm2, st, b, stream); });
   }
 
   // host bookkeeping after a step: fm2 <- fm1, fm1 <- rhs, old fm2 becomes the next rhs buffer
@@ -691,8 +697,7 @@ struct ffs_sim {
   // kernels of one steady step (no host bookkeeping), for capture
   void enqueue_steady_step() {
     nonlinear(false);
-    set_dt();
-    forcing();
+    step_scalars(true, true);
     enqueue_update();
   }
 
@@ -715,8 +720,7 @@ struct ffs_sim {
     const bool stats_t = is_stats_time(iteration);
     const bool spec_t = is_spec_time(iteration);
     nonlinear(stats_t || spec_t);
-    set_dt();
-    forcing();
+    step_scalars(true, true);
     if (stats_t) queue_stats();
     time_step();
   }
