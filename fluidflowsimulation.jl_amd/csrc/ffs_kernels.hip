@@ -74,6 +74,9 @@ template <int N, int NSL> __host__ __device__ constexpr int xgen_rows() {
 #ifndef FFS_XNL_BRANCHFREE
 #define FFS_XNL_BRANCHFREE 1  // 1: the x pass loads its inputs without branches
 #endif
+#ifndef FFS_XNL_MODES
+#define FFS_XNL_MODES 1  // 1: density flux and LES stress through the register x pass too
+#endif
 #ifndef FFS_XNL_PREFETCH
 #define FFS_XNL_PREFETCH 1  // 1: request every input of a row before its first transform
 #endif
@@ -488,9 +491,24 @@ __global__ void __launch_bounds__(256) k_x_gen(Geo g, const C<T>* __restrict__ a
 // positions p = q (mod TPER) in registers from the first load to the last
 // store, so LDS carries only the one exchange inside each transform (plus the
 // k <-> N-k pairing at the end), synchronised per wavefront.
-template <typename T, int N, bool KE>
+// What one launch of the register-resident x pass computes (MODE):
+//   0  u x w (+ 1/2|u|^2 into slot ke_slot) from u, w (slots 0..5), max|u_i|
+//   1  the Smagorinsky stress tau = c2a |S| S (c2a = 2 (c Delta)^2, |S| = sqrt(2 S:S),
+//      calc_les! les_real_calc.jl:77-79) from S xx xy xz yy yz (slots in0..in0+4) into out0..out0+4
+//   2  the density flux -rho u from u (slots 0..2) and rho (slot in0) into out0..out0+2, max|rho|
+struct XRegArgs {
+  int in0, out0, ke_slot;
+  double c2a;
+};
+template <int MODE> __host__ __device__ constexpr int xreg_npin() { return MODE == 2 ? 2 : 3; }
+template <int MODE, bool KE> __host__ __device__ constexpr int xreg_npout() {
+  return MODE == 1 ? 3 : 2;
+}
+
+template <typename T, int N, int MODE, bool KE>
 __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
-                                                 const C<T>* __restrict__ tw_g, unsigned long long* __restrict__ umax) {
+                                                 const C<T>* __restrict__ tw_g, unsigned long long* __restrict__ vmax,
+                                                 XRegArgs xa) {
   constexpr int PPT = xreg_ppt<T, N>();
   constexpr int TP = N / PPT;
   constexpr int GPW = 64 / TP;  // rows per wavefront
@@ -498,6 +516,8 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
   constexpr int NSTG = plan_ns(N, PPT);
   constexpr int M0 = (N / plan_r(N, 0, PPT)) / TP;         // register stride of first-stage legs
   constexpr int ML = (N / plan_r(N, NSTG - 1, PPT)) / TP;  // register stride of last-stage legs
+  constexpr int NPI = xreg_npin<MODE>();
+  constexpr int NPO = xreg_npout<MODE, KE>();
   static_assert(xreg_ok<T, N>(), "register-resident x pass needs an even plan");
   extern __shared__ unsigned char smem_raw[];
   C<T>* twl = reinterpret_cast<C<T>*>(smem_raw);
@@ -517,6 +537,17 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
     if constexpr (FFS_XNL_PAD_SHIFT > 0) return X[p + (p >> FFS_XNL_PAD_SHIFT)];
     else return X[p];
   };
+  // input slots of pair j (B < 0: the pair's imaginary row is zero)
+  auto slot_a = [&](int j) -> int {
+    if constexpr (MODE == 0) return 2 * j;
+    else if constexpr (MODE == 1) return xa.in0 + 2 * j;
+    else return 2 * j;
+  };
+  auto slot_b = [&](int j) -> int {
+    if constexpr (MODE == 0) return 2 * j + 1;
+    else if constexpr (MODE == 1) return j < 2 ? xa.in0 + 2 * j + 1 : -1;
+    else return j == 0 ? 1 : xa.in0;
+  };
 
   C<T> R[3][PPT];
   // Z = A + iB of a pair of real rows at position p, from their retained half spectra
@@ -528,60 +559,76 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
     const bool lo = p < mx;
     const bool hi = !lo && k < mx && 2 * p != N;
     const int idx = lo ? p : (hi ? k : 0);
-    const C<T> a = A[idx], b = B[idx];
+    const C<T> a = A[idx];
+    const C<T> b = B ? B[idx] : mk<T>(0, 0);
     const bool real = p == 0 || 2 * p == N;
     C<T> z = lo ? (real ? mk<T>(a.x, b.x) : mk<T>(a.x - b.y, a.y + b.x)) : mk<T>(a.x + b.y, b.x - a.y);
     return (row_ok && (lo || hi)) ? z : mk<T>(0, 0);
 #else
     if (!row_ok) return mk<T>(0, 0);
     if (p < mx) {
-      const C<T> a = A[p], b = B[p];
+      const C<T> a = A[p], b = B ? B[p] : mk<T>(0, 0);
       if (p == 0 || 2 * p == N) return mk<T>(a.x, b.x);
       return mk<T>(a.x - b.y, a.y + b.x);
     }
     const int k = N - p;
     if (k < mx && 2 * p != N) {
-      const C<T> a = A[k], b = B[k];
+      const C<T> a = A[k], b = B ? B[k] : mk<T>(0, 0);
       return mk<T>(a.x + b.y, b.x - a.y);
     }
     return mk<T>(0, 0);
 #endif
   };
+  auto row_a = [&](int j) { return a2 + (size_t)slot_a(j) * S2 + roff; };
+  auto row_b = [&](int j) -> const C<T>* { return slot_b(j) < 0 ? nullptr : a2 + (size_t)slot_b(j) * S2 + roff; };
 #if FFS_XNL_PREFETCH
   // every input of the row is requested before the first transform (lane q's first-stage legs
-  // are its register positions p = q + TP m), so the loads of all three pairs are in flight together
+  // are its register positions p = q + TP m), so the loads of all pairs are in flight together
 #pragma unroll
-  for (int j = 0; j < 3; ++j)
+  for (int j = 0; j < NPI; ++j)
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) R[j][m] = zval(a2 + (size_t)(2 * j) * S2 + roff, a2 + (size_t)(2 * j + 1) * S2 + roff, q + TP * m);
+    for (int m = 0; m < PPT; ++m) R[j][m] = zval(row_a(j), row_b(j), q + TP * m);
 #endif
-  // inverse transforms of the pairs (ux,uy) (uz,wx) (wy,wz); Z built from the half spectra on load
+  // inverse transforms of the input pairs; Z built from the half spectra on load
 #pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    const C<T>* A = a2 + (size_t)(2 * j) * S2 + roff;
-    const C<T>* B = a2 + (size_t)(2 * j + 1) * S2 + roff;
+  for (int j = 0; j < NPI; ++j) {
     auto ld = [&](int p, int t, int r) -> C<T> {
       if constexpr (FFS_XNL_PREFETCH) return R[j][t + r * M0];
-      else return zval(A, B, p);
+      else return zval(row_a(j), row_b(j), p);
     };
     auto st = [&](int, int t, int r, C<T> v) { R[j][t + r * ML] = v; };
     if (j > 0) __syncwarp();
     Fft<T, N, +1, TP, false, false, SYNC_WAVE, PPT>::run(q, tw, ld, st, xref);
   }
-  // u x w, 1/2|u|^2, max|u_i|
+  // pointwise products
   double lmax = 0.0;
 #pragma unroll
   for (int m = 0; m < PPT; ++m) {
-    const T ux = R[0][m].x, uy = R[0][m].y, uz = R[1][m].x, wx = R[1][m].y, wy = R[2][m].x, wz = R[2][m].y;
-    T ke = 0;
-    if constexpr (KE) ke = (T)0.5 * (ux * ux + uy * uy + uz * uz);
-    R[0][m] = mk<T>(uy * wz - uz * wy, uz * wx - ux * wz);
-    R[1][m] = mk<T>(ux * wy - uy * wx, ke);
-    if (row_ok) lmax = nanmax(lmax, (double)nanmax(fabs(ux), nanmax(fabs(uy), fabs(uz))));
+    if constexpr (MODE == 0) {  // u x w, 1/2|u|^2, max|u_i|
+      const T ux = R[0][m].x, uy = R[0][m].y, uz = R[1][m].x, wx = R[1][m].y, wy = R[2][m].x, wz = R[2][m].y;
+      T ke = 0;
+      if constexpr (KE) ke = (T)0.5 * (ux * ux + uy * uy + uz * uz);
+      R[0][m] = mk<T>(uy * wz - uz * wy, uz * wx - ux * wz);
+      R[1][m] = mk<T>(ux * wy - uy * wx, ke);
+      if (row_ok) lmax = nanmax(lmax, (double)nanmax(fabs(ux), nanmax(fabs(uy), fabs(uz))));
+    } else if constexpr (MODE == 1) {  // tau = c2a |S| S
+      const T sxx = R[0][m].x, sxy = R[0][m].y, sxz = R[1][m].x, syy = R[1][m].y, syz = R[2][m].x;
+      const T szz = -sxx - syy;
+      const T ss = sxx * sxx + syy * syy + szz * szz + (T)2 * (sxy * sxy + sxz * sxz + syz * syz);
+      const T f = (T)xa.c2a * sqrt((T)2 * ss);
+      R[0][m] = mk<T>(f * sxx, f * sxy);
+      R[1][m] = mk<T>(f * sxz, f * syy);
+      R[2][m] = mk<T>(f * syz, (T)0);
+    } else {  // -rho u, max|rho|
+      const T ux = R[0][m].x, uy = R[0][m].y, uz = R[1][m].x, rho = R[1][m].y;
+      R[0][m] = mk<T>((-rho) * ux, (-rho) * uy);
+      R[1][m] = mk<T>((-rho) * uz, (T)0);
+      if (row_ok) lmax = nanmax(lmax, (double)fabs(rho));
+    }
   }
-  // forward transforms of (px,py) and (pz,ke), split into the half spectra k < mx
+  // forward transforms of the output pairs, split into the half spectra k < mx
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
+  for (int j = 0; j < NPO; ++j) {
     auto ld = [&](int, int t, int r) { return R[j][t + r * M0]; };
     auto st = [&](int, int t, int r, C<T> v) { R[j][t + r * ML] = v; };
     __syncwarp();
@@ -590,17 +637,29 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
 #pragma unroll
     for (int m = 0; m < PPT; ++m) xref(q + TP * m) = R[j][m];
     __syncwarp();
+    // output slots of the pair's halves (-1: not written)
+    int oa, ob;
+    if constexpr (MODE == 0) {
+      oa = 2 * j;
+      ob = j == 0 ? 1 : (KE ? xa.ke_slot : -1);
+    } else if constexpr (MODE == 1) {
+      oa = xa.out0 + 2 * j;
+      ob = j < 2 ? xa.out0 + 2 * j + 1 : -1;
+    } else {
+      oa = xa.out0 + 2 * j;
+      ob = j == 0 ? xa.out0 + 1 : -1;
+    }
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
       const int k = q + TP * m;
       if (row_ok && k < mx) {
         const C<T> zk = R[j][m], zn = xref((N - k) % N);
-        b2[(size_t)(2 * j) * S2 + roff + k] = mk<T>((T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y));
-        if (j == 0 || KE) b2[(size_t)(2 * j + 1) * S2 + roff + k] = mk<T>((T)0.5 * (zk.y + zn.y), (T)0.5 * (zn.x - zk.x));
+        b2[(size_t)oa * S2 + roff + k] = mk<T>((T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y));
+        if (ob >= 0) b2[(size_t)ob * S2 + roff + k] = mk<T>((T)0.5 * (zk.y + zn.y), (T)0.5 * (zn.x - zk.x));
       }
     }
   }
-  if (umax) block_max_to(lmax, umax);
+  if (MODE != 1 && vmax) block_max_to(lmax, vmax);
 }
 
 // C2R of nf fields (rows paired 2m, 2m+1) into the padded real layout
@@ -724,31 +783,42 @@ static void run_x_gen(const Geo& g, const void* a2, void* b2, const XOps& ops, u
       g, (const C<T>*)a2, (C<T>*)b2, (const C<T>*)tw, ops, umax, rhomax);
 }
 
+template <typename T, int N, int MODE, bool KE>
+static void launch_xreg(const Geo& g, const void* a2, void* b2, const void* tw, unsigned long long* vmax,
+                        const XRegArgs& xa, hipStream_t s) {
+  constexpr int TP = N / xreg_ppt<T, N>();
+  constexpr int WPB = 2;  // wavefronts per block
+  constexpr int RPB = WPB * (64 / TP);
+  const size_t lds = ((size_t)RPB * xreg_ldx<T, N>() + N) * sizeof(C<T>);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)&k_x_nl_reg<T, N, MODE, KE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    attr = true;
+  }
+  const int nrows = g.ny * g.nz;
+  k_x_nl_reg<T, N, MODE, KE><<<(nrows + RPB - 1) / RPB, 64 * WPB, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
+                                                                          (const C<T>*)tw, vmax, xa);
+}
+
 template <typename T, int N>
 static void run_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
                      unsigned long long* rhomax, const void* tw, hipStream_t s) {
-  const int nrows = g.ny * g.nz;
   const bool plain = !ops.rho && !ops.les;
   if constexpr (xreg_ok<T, N>()) {
-    if (plain) {
-      constexpr int TP = N / xreg_ppt<T, N>();
-      constexpr int WPB = 2;  // wavefronts per block
-      constexpr int RPB = WPB * (64 / TP);
-      const size_t lds = ((size_t)RPB * xreg_ldx<T, N>() + N) * sizeof(C<T>);
-      static bool attr = false;
-      if (!attr) {
-        (void)hipFuncSetAttribute((const void*)&k_x_nl_reg<T, N, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds);
-        (void)hipFuncSetAttribute((const void*)&k_x_nl_reg<T, N, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds);
-        attr = true;
+    if (FFS_XNL_MODES || plain) {
+      // u x w (+ 1/2|u|^2, always the last output slot), then -rho u and tau as launches of their own
+      XRegArgs xa{0, 0, ops.nout - 1, 0.0};
+      if (ops.ke) launch_xreg<T, N, 0, true>(g, a2, b2, tw, umax, xa, s);
+      else launch_xreg<T, N, 0, false>(g, a2, b2, tw, umax, xa, s);
+      if (ops.rho) {
+        XRegArgs xr{6, 3, 0, 0.0};
+        launch_xreg<T, N, 2, false>(g, a2, b2, tw, rhomax, xr, s);
       }
-      if (ops.ke)
-        k_x_nl_reg<T, N, true><<<(nrows + RPB - 1) / RPB, 64 * WPB, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
-                                                                            (const C<T>*)tw, umax);
-      else
-        k_x_nl_reg<T, N, false><<<(nrows + RPB - 1) / RPB, 64 * WPB, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
-                                                                             (const C<T>*)tw, umax);
+      if (ops.les) {
+        XRegArgs xl{6 + (ops.rho ? 1 : 0), 3 + (ops.rho ? 3 : 0), 0, ops.les_2alpha};
+        launch_xreg<T, N, 1, false>(g, a2, b2, tw, nullptr, xl, s);
+      }
       return;
     }
   }
