@@ -107,6 +107,19 @@ def reference_ranges(n, cutoff=2 / 3):
     return mx, 2 * f + 2, 2 * f + 2
 
 
+def measured_traffic(kernel, prec, n):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    of this workload (profiles/step_kernels_<n>_<prec>.md), or None when none is recorded."""
+    path = os.path.join(REPO, "profiles", f"step_kernels_{n}_{prec}.md")
+    if not os.path.exists(path):
+        return None
+    key = f"traffic_bytes_per_launch {kernel} {prec} {n} ="
+    for line in open(path):
+        if line.startswith(key):
+            return int(float(line.split("=")[1]))
+    return None
+
+
 def copy_probe():
     """Plain device-to-device copy of 1 GiB (torch's copy kernel): reads 1 GiB and
     writes 1 GiB; its FETCH_SIZE / WRITE_SIZE calibrate the traffic counters and
@@ -258,7 +271,9 @@ def bench(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom, prec, n),
+            "traffic_source": f"profiles/step_kernels_{n}_{prec}.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, calibrated "
+                              "by a 1 GiB copy and fill)",
             "alg_bytes_per_launch": d["alg_bytes"],
             "avg_us": d["avg_us"],
             "timing": "HIP events on the solver's stream around every launch, over a second stretch of the same "

@@ -36,6 +36,8 @@ template <int N> __host__ __device__ constexpr int tper() { return N / default_p
 #define FFS_COL_PPT8 1
 #endif
 // FFS_ZMAP_ARITH: the inverse-z load maps z indices arithmetically (1) or through the table (0)
+// XCDs of the device (MI355X: 8); workgroups are dispatched to them round-robin
+constexpr int kXcd = 8;
 // FFS_CURL_SPLIT: the inverse-z pass that forms w (and rho, S) runs one field per block (1) or
 // loops over the fields in each block (0)
 #ifndef FFS_CURL_SPLIT
@@ -49,11 +51,18 @@ template <int N> __host__ __device__ constexpr int tper() { return N / default_p
 #define FFS_ZMAP_ARITH 1
 #endif
 template <int N> __host__ __device__ constexpr bool ppt8_ok() { return N % 8 == 0 && plan_even(N, N / 8, 8); }
-template <int N, bool CURL> __host__ __device__ constexpr bool col_ppt8() {
-  return ppt8_ok<N>() && ((FFS_COL_PPT8 == 0 && CURL) || FFS_COL_PPT8 == 1);
+// the column kernels run TX x TP threads per block, at most kColThreads (their launch bound)
+constexpr int kColThreads = 512;
+template <typename T, int N, bool CURL> __host__ __device__ constexpr bool col_ppt8() {
+  return ppt8_ok<N>() && ((FFS_COL_PPT8 == 0 && CURL) || FFS_COL_PPT8 == 1) &&
+         col_tx<T>(N) * (N / 8) <= kColThreads;
 }
-template <int N, bool CURL> __host__ __device__ constexpr int col_tper() { return col_ppt8<N, CURL>() ? N / 8 : tper<N>(); }
-template <int N, bool CURL> __host__ __device__ constexpr int col_maxr() { return col_ppt8<N, CURL>() ? 8 : 16; }
+template <typename T, int N, bool CURL> __host__ __device__ constexpr int col_tper() {
+  return col_ppt8<T, N, CURL>() ? N / 8 : tper<N>();
+}
+template <typename T, int N, bool CURL> __host__ __device__ constexpr int col_maxr() {
+  return col_ppt8<T, N, CURL>() ? 8 : 16;
+}
 template <int N> __host__ __device__ constexpr int xrows() { return (64 / tper<N>()) > 0 ? (64 / tper<N>()) : 1; }
 // rows per block of the general x pass: at most 256 threads
 template <int N, int NSL> __host__ __device__ constexpr int xgen_rows() {
@@ -144,23 +153,39 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) 
 enum ColKind { INV_Z = 0, INV_Y = 1, FWD_Y = 2, FWD_Z = 3 };
 
 template <typename T, int N, int KIND, bool CURL>
-__global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ out,
+__global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ out,
                                             ColAux<T> aux, const C<T>* __restrict__ tw_g, int nzero, T scale) {
   constexpr int TX = col_tx<T>(N);
-  constexpr int TP = col_tper<N, CURL>();
-  constexpr int MAXR = col_maxr<N, CURL>();
+  constexpr int TP = col_tper<T, N, CURL>();
+  constexpr int MAXR = col_maxr<T, N, CURL>();
+  static_assert(TX * TP <= kColThreads, "column block exceeds its launch bound");
+  // The split variant of the inverse-z pass runs one field per block.  Its 1D grid is laid out for
+  // the round-robin dispatch of workgroups over the XCDs: block L goes to XCD L mod kXcd, so the
+  // nzero field blocks of tile t are given the ids with L mod kXcd = t mod kXcd and consecutive
+  // L / kXcd; they run on one XCD, back to back, and read the u tile once from HBM into its L2.
+  constexpr bool SPLIT = CURL && FFS_CURL_SPLIT;
+  int bx, bb, bf = 0;
+  if constexpr (SPLIT) {
+    const int ntx = (g.mx + TX - 1) / TX;
+    const int L = blockIdx.x, sq = L / kXcd;
+    bf = sq % nzero;
+    const int t = (sq / nzero) * kXcd + L % kXcd;
+    if (t >= ntx * g.my) return;  // whole block: the grid is padded to a multiple of kXcd tiles
+    bx = t % ntx;
+    bb = t / ntx;
+  } else {
+    bx = blockIdx.x;
+    bb = blockIdx.y;
+  }
   extern __shared__ unsigned char smem_raw[];
   C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
   const C<T>* tw = stage_twiddles<T, N>(lds + N * TX, tw_g);
   __syncthreads();
   const int c = threadIdx.x % TX;
   const int q = threadIdx.x / TX;
-  // the split variant of the inverse-z pass puts the field in blockIdx.x (fastest), so the blocks
-  // that read the same u tile run together and share it through L2
-  constexpr bool SPLIT = CURL && FFS_CURL_SPLIT;
-  const int cx = (SPLIT ? blockIdx.y : blockIdx.x) * TX + c;
+  const int cx = bx * TX + c;
   const bool valid = cx < g.mx;
-  const int b = SPLIT ? blockIdx.z : blockIdx.y;
+  const int b = bb;
   const size_t sx = g.sx;
   const size_t Ssp = sx * g.my * g.mz;  // compact spectral
   const size_t S1 = sx * g.my * g.nz;   // (sx, my, nz)
@@ -173,8 +198,8 @@ __global__ void __launch_bounds__(512, FFS_COL_WAVES) k_col(Geo g, const C<T>* _
     // nonlinear term -- u (0..2), w = ik x u (3..5) (fourierspacep1!,
     // models.jl:88-95), rho (if aux.rho), S = sym(ik (x) u) (xx,xy,xz,yy,yz; if
     // aux.les) -- so the u tile is fetched from HBM once and re-read from L2
-    const int f0 = SPLIT ? blockIdx.x : (CURL ? 0 : blockIdx.z);
-    const int f1 = SPLIT ? blockIdx.x + 1 : (CURL ? nzero : blockIdx.z + 1);
+    const int f0 = SPLIT ? bf : (CURL ? 0 : blockIdx.z);
+    const int f1 = SPLIT ? bf + 1 : (CURL ? nzero : blockIdx.z + 1);
     const T kx = valid ? (T)g.kx[cx] : (T)0, ky = (T)g.ky[cy];
     for (int f = f0; f < f1; ++f) {
       if (f != f0) __syncthreads();
@@ -283,14 +308,16 @@ template <typename T, int N, int KIND, bool CURL>
 static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, const void* tw, int nzero, double scale,
                     hipStream_t s, const ColAux<T>& aux = ColAux<T>{}) {
   constexpr int TX = col_tx<T>(N);
-  constexpr int TP = col_tper<N, CURL>();
+  constexpr int TP = col_tper<T, N, CURL>();
+  static_assert(TX * TP <= kColThreads, "column block exceeds its launch bound");
   const size_t lds = (size_t)(N * TX + N) * sizeof(C<T>);
   static bool attr = false;
   if (!attr) {
     hipFuncSetAttribute((const void*)&k_col<T, N, KIND, CURL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr = true;
   }
-  const dim3 grid = (CURL && FFS_CURL_SPLIT) ? dim3(nzero, (g.mx + TX - 1) / TX, nb)
+  const int ntile = (g.mx + TX - 1) / TX * nb;
+  const dim3 grid = (CURL && FFS_CURL_SPLIT) ? dim3((unsigned)(nzero * ((ntile + kXcd - 1) / kXcd * kXcd)))
                                              : dim3((g.mx + TX - 1) / TX, nb, CURL ? 1 : nf);
   k_col<T, N, KIND, CURL><<<grid, TX * TP, lds, s>>>(g, (const C<T>*)in, (C<T>*)out, aux, (const C<T>*)tw, nzero,
                                                     (T)scale);

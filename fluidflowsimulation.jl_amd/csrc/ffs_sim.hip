@@ -191,15 +191,23 @@ struct ffs_sim {
     return e;
   }
 
+  // a launch that the runtime rejects (bad configuration) surfaces here, not as silently skipped work
+  static void check_launch(const char* name) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw FfsError(FFS_ERR_HIP, std::string("launch of ") + name + ": " + hipGetErrorString(e));
+  }
+
   template <class F>
   void launch(const char* name, F&& f) {
     if (!prof) {
       f();
+      check_launch(name);
       return;
     }
     hipEvent_t a = get_ev(), b = get_ev();
     HIPCHECK(hipEventRecord(a, stream));
     f();
+    check_launch(name);
     HIPCHECK(hipEventRecord(b, stream));
     prof_pending.push_back({name, a, b});
     if (prof_pending.size() > 4096) drain_prof();

@@ -1,0 +1,66 @@
+"""Parity at long transform lengths on thin grids: every FFT length of the
+x pass (nx up to 512) and of the column passes (ny, nz up to 512) against the
+oracle, for the plain, LES and Boussinesq fields, FP64 and FP32, over three
+steps.  The thin grids keep the oracle cheap while each transform length runs
+the same kernel variant as a cubic grid of that size.  The step is fixed and
+small: set_dt's CFL uses the y spacing only (time_step_functions.jl:86-129),
+so on a thin grid it would pick a dt far beyond the explicit viscous limit of
+the long axis, and the runs would amplify roundoff instead of comparing it."""
+import numpy as np
+import pytest
+
+import ffs
+from ffs.initial import random_scalar, random_spectrum
+from oracle.ffs_oracle import Grid, OracleSim
+
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+# (nx, ny, nz, x_domain): the horizontal spectrum bins ky on the x wavenumbers unless kx is the finer
+# grid (Globals.jl:148-152), so the long-y case takes a longer x box
+SHAPES = [(512, 16, 16, 1.0), (256, 16, 16, 1.0), (16, 512, 16, 2.0), (16, 16, 512, 1.0), (192, 16, 16, 1.0),
+          (96, 24, 16, 1.0)]
+MODELS = ["plain", "les", "density"]
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("model", MODELS)
+@pytest.mark.parametrize("shape", SHAPES, ids=["x".join(map(str, s[:3])) for s in SHAPES])
+def test_long_transforms_match_oracle(shape, model, prec):
+    nx, ny, nz, lx = shape
+    print_case(f"long rows {shape} {model} {prec}")
+    g = Grid(nx, ny, nz, lx, 1.0, 1.0)
+    u0 = random_spectrum(nx, ny, nz, seed=5, lx=lx)
+    les_c = 0.16 if model == "les" else None
+    dens = {"Pr": 0.7, "gradient": 1.0, "g": -1.0} if model == "density" else None
+    ref = OracleSim(g, 2e-3, dt=1e-3, variable_dt=False, dt_stat=1 << 30, les_c=les_c, density=dens)
+    ref.set_velocity(u0)
+    p = ffs.Params(nx=nx, ny=ny, nz=nz, x_domain=lx, nu=2e-3, integrator="Adams_Bashforth3rdO", variable_dt=False, dt=1e-3,
+                   dt_stat=1 << 30, dt_spec=1 << 30, write_time=1 << 30, precision=prec)
+    if les_c is not None:
+        p.les_model, p.smagorinsky_c = "Smagorinsky", les_c
+    rho0 = None
+    if dens is not None:
+        rho0 = random_scalar(nx, ny, nz, seed=6, lx=lx)
+        ref.set_density(rho0)
+        p.density, p.pr, p.density_gradient, p.z_acceleration, p.reference_density = True, 0.7, 1.0, -1.0, 1.0
+    with ffs.Simulation(p) as s:
+        s.set_velocity(*u0)
+        if rho0 is not None:
+            s.set_density(rho0)
+        ref.initialize()
+        s.initialize()
+        ref.advance_in_time(3)
+        s.advance(3)
+        tol = 1e-11 if prec == "fp64" else 2e-5
+        got = s.velocity_hat()
+        err = max(rel(got[c], ref.uh[c]) for c in range(3))
+        assert err < tol, ("velocity", err)
+        if rho0 is not None:
+            rerr = rel(s.density_hat(), ref.rhoh)
+            assert rerr < tol, ("density", rerr)
