@@ -757,6 +757,95 @@ struct ffs_sim {
     forcing();
     queue_stats();
   }
+
+  // ---------------------------------------------------------------- checkpoints
+  // Blob: header, DevState, then u, fm1, fm2 (3 components each) and, with density, rho, rfm1, rfm2,
+  // each in the compact layout of this precision.  That is the whole integrator state (the rhs
+  // buffer is rewritten before it is read), so a run restored from it continues bitwise as the
+  // original would have.
+  struct CkptHeader {
+    char magic[8];
+    uint32_t version, precision;
+    int32_t nx, ny, nz, integrator, les, dens, variable_dt, pad;
+    int64_t iteration, tit;
+    double host_time;
+    uint64_t S, csz;
+  };
+
+  std::vector<void*> ckpt_buffers() const {
+    std::vector<void*> b = {u, fm1, fm2};
+    if (dens) b.insert(b.end(), {rho, rfm1, rfm2});
+    return b;
+  }
+  size_t ckpt_buffer_bytes(const void* b) const { return (b == u || b == fm1 || b == fm2 ? 3 : 1) * S * csz; }
+
+  size_t checkpoint_bytes() const {
+    size_t n = sizeof(CkptHeader) + sizeof(DevState);
+    for (void* b : ckpt_buffers()) n += ckpt_buffer_bytes(b);
+    return n;
+  }
+
+  CkptHeader ckpt_header() const {
+    CkptHeader h{};
+    std::memcpy(h.magic, "FFSCKPT1", 8);
+    h.version = 1;
+    h.precision = fp32 ? 1 : 0;
+    h.nx = p.nx;
+    h.ny = p.ny;
+    h.nz = p.nz;
+    h.integrator = p.integrator;
+    h.les = les ? 1 : 0;
+    h.dens = dens ? 1 : 0;
+    h.variable_dt = p.variable_dt;
+    h.iteration = iteration;
+    h.tit = tit;
+    h.host_time = host_time;
+    h.S = S;
+    h.csz = csz;
+    return h;
+  }
+
+  void save_checkpoint(unsigned char* out, size_t cap) {
+    if (!initialized) throw FfsError(FFS_ERR_STATE, "ffs_initialize must be called first");
+    if (cap < checkpoint_bytes()) throw FfsError(FFS_ERR_ARG, "checkpoint buffer too small");
+    const CkptHeader h = ckpt_header();
+    std::memcpy(out, &h, sizeof h);
+    size_t off = sizeof h;
+    HIPCHECK(hipMemcpyAsync(out + off, st, sizeof(DevState), hipMemcpyDeviceToHost, stream));
+    off += sizeof(DevState);
+    for (void* b : ckpt_buffers()) {
+      HIPCHECK(hipMemcpyAsync(out + off, b, ckpt_buffer_bytes(b), hipMemcpyDeviceToHost, stream));
+      off += ckpt_buffer_bytes(b);
+    }
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+
+  void load_checkpoint(const unsigned char* in, size_t n) {
+    if (initialized) throw FfsError(FFS_ERR_STATE, "a checkpoint is loaded into a run that is not initialized");
+    if (n < sizeof(CkptHeader)) throw FfsError(FFS_ERR_IO, "checkpoint too short");
+    CkptHeader h;
+    std::memcpy(&h, in, sizeof h);
+    const CkptHeader mine = ckpt_header();
+    if (std::memcmp(h.magic, mine.magic, 8) != 0 || h.version != mine.version)
+      throw FfsError(FFS_ERR_IO, "not a checkpoint of this library version");
+    if (h.precision != mine.precision || h.nx != mine.nx || h.ny != mine.ny || h.nz != mine.nz ||
+        h.integrator != mine.integrator || h.les != mine.les || h.dens != mine.dens ||
+        h.variable_dt != mine.variable_dt || h.S != mine.S || h.csz != mine.csz)
+      throw FfsError(FFS_ERR_ARG, "checkpoint was written by a run with other parameters");
+    if (n != checkpoint_bytes()) throw FfsError(FFS_ERR_IO, "checkpoint size mismatch");
+    size_t off = sizeof h;
+    HIPCHECK(hipMemcpy(st, in + off, sizeof(DevState), hipMemcpyHostToDevice));
+    off += sizeof(DevState);
+    for (void* b : ckpt_buffers()) {
+      HIPCHECK(hipMemcpy(b, in + off, ckpt_buffer_bytes(b), hipMemcpyHostToDevice));
+      off += ckpt_buffer_bytes(b);
+    }
+    iteration = h.iteration;
+    tit = h.tit;
+    host_time = h.host_time;
+    phase = 0;  // graph cache is empty; the buffer roles were restored as saved
+    initialized = true;
+  }
 };
 
 // An ensemble advanced in lockstep.  Steps in which every run is steady replay one graph whose
@@ -1128,6 +1217,22 @@ int ffs_advance(ffs_sim* s, int64_t nsteps, int64_t* done, int* event) {
       if (done) *done += 1;
     }
   });
+}
+
+int ffs_checkpoint_size(const ffs_sim* s, size_t* bytes) {
+  if (!s || !bytes) return FFS_ERR_ARG;
+  *bytes = s->checkpoint_bytes();
+  return FFS_OK;
+}
+
+int ffs_checkpoint_save(ffs_sim* s, void* host, size_t cap) {
+  if (!s || !host) return FFS_ERR_ARG;
+  return guard(s, [&] { s->save_checkpoint((unsigned char*)host, cap); });
+}
+
+int ffs_checkpoint_load(ffs_sim* s, const void* host, size_t n) {
+  if (!s || !host) return FFS_ERR_ARG;
+  return guard(s, [&] { s->load_checkpoint((const unsigned char*)host, n); });
 }
 
 int ffs_writestats_now(ffs_sim* s) {

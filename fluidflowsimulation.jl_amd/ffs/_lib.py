@@ -83,6 +83,9 @@ SIGNATURES = {
     "ffs_initialize": (ctypes.c_int, [_P]),
     "ffs_advance": (ctypes.c_int, [_P, ctypes.c_int64, _I64, _I]),
     "ffs_writestats_now": (ctypes.c_int, [_P]),
+    "ffs_checkpoint_size": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    "ffs_checkpoint_save": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_size_t]),
+    "ffs_checkpoint_load": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_size_t]),
     "ffs_stats_pop": (ctypes.c_int, [_P, _D, ctypes.c_int, _I, _D, ctypes.c_int, _I, _D, ctypes.c_int, _I]),
     "ffs_stats_header": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
     "ffs_spectrum": (ctypes.c_int, [_P, _D, ctypes.c_int, _I, _D, ctypes.c_int, _I]),

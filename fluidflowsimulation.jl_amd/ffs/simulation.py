@@ -131,6 +131,28 @@ class Simulation:
                 on_output(self)
         return self
 
+    def checkpoint(self):
+        """The run's full integrator state as bytes (ffs_checkpoint_save): a run restored from it
+        continues bitwise as this one would."""
+        n = ctypes.c_size_t()
+        self._check(self._lib.ffs_checkpoint_size(self._h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        self._check(self._lib.ffs_checkpoint_save(self._h, buf, n.value))
+        return buf.raw
+
+    def restore(self, blob):
+        """Loads a checkpoint into this (not yet initialized) run; replaces set_field + initialize."""
+        buf = ctypes.create_string_buffer(bytes(blob), len(blob))
+        self._check(self._lib.ffs_checkpoint_load(self._h, buf, len(blob)))
+
+    def write_checkpoint(self, path):
+        with open(path, "wb") as f:
+            f.write(self.checkpoint())
+
+    def read_checkpoint(self, path):
+        with open(path, "rb") as f:
+            self.restore(f.read())
+
     def writestats_now(self):
         self._check(self._lib.ffs_writestats_now(self._h))
 

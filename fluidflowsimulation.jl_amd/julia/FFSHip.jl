@@ -147,6 +147,18 @@ function advance_in_time!(s::FFSSimulation)
     return nothing
 end
 
+# exact-resume checkpoints (the reference restarts without its integrator history)
+function checkpoint(s::FFSSimulation)
+    n = Ref{Csize_t}(0)
+    check(ccall((:ffs_checkpoint_size, libffs), Cint, (Ptr{Cvoid}, Ref{Csize_t}), s.h, n), s.h)
+    buf = Vector{UInt8}(undef, n[])
+    check(ccall((:ffs_checkpoint_save, libffs), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Csize_t), s.h, buf, n[]), s.h)
+    return buf
+end
+
+restore!(s::FFSSimulation, blob::Vector{UInt8}) =
+    check(ccall((:ffs_checkpoint_load, libffs), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Csize_t), s.h, blob, length(blob)), s.h)
+
 writestats_now!(s::FFSSimulation) = check(ccall((:ffs_writestats_now, libffs), Cint, (Ptr{Cvoid},), s.h), s.h)
 
 function stats_header(s::FFSSimulation, table::Integer)

@@ -143,6 +143,17 @@ int ffs_initialize(ffs_sim* s);
  * time; the next call performs that step.  *done receives the steps taken. */
 int ffs_advance(ffs_sim* s, int64_t nsteps, int64_t* done, int* event);
 
+/* Exact-resume checkpoints.  The reference restarts from u?.N alone and starts
+ * its integrator over (Adams_Bashforth.jl:28-35, ETD.jl:99-108: history and dt
+ * history are not restored); a run restored from a checkpoint instead continues
+ * bitwise as the original would have.  The blob holds the velocity (and density)
+ * with the integrator history, dt history, time and iteration in this build's
+ * layout; it is loaded into a run created with the same parameters and not yet
+ * initialized (ffs_checkpoint_load replaces ffs_set_field + ffs_initialize). */
+int ffs_checkpoint_size(const ffs_sim* s, size_t* bytes);
+int ffs_checkpoint_save(ffs_sim* s, void* host, size_t cap);
+int ffs_checkpoint_load(ffs_sim* s, const void* host, size_t n);
+
 /* Replaces the tail of run_simulation (FluidFlowSimulation.jl:46-49):
  * calculate_rhs!(s), forcing, writestats(s) at the current iteration. */
 int ffs_writestats_now(ffs_sim* s);
