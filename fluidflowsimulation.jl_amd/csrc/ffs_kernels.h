@@ -62,8 +62,34 @@ struct DevState {
   double force_factor;     // isotropic forcing: P / sum |u_lowk|^2
   double rhomax_d;         // last max|rho|
   double etd[20];          // ETD3 scalars a0..a3 b0..b2 c0..c2 e1..e9 of set_ABCt! (ETD.jl:142-227)
-  double pad[3];
+  double force_hp, force_vp;  // plane forcing: horizontal and vertical power (the hp, vp stats)
+  double force_vert;          // plane forcing: factor of the kx = ky = 0 modes
 };
+
+// kz = 0-plane forcing (rfForcing / QfForcing, Forcing_methods/forcing.jl:6-114, qfforcing.jl).
+// The forced elements are the retained kz = 0 modes whose 2D shell n (round(|k_h| / maxdk), 0-based)
+// has 0 < n < nsh and mean |k_h| <= kf, listed in the reference's loop order (y outer, x inner).
+struct PlaneForcing {
+  int kind;              // 2 rf, 3 qf
+  int nel;               // forced elements
+  const int* elem;       // compact index of each (kz = 0 plane)
+  const int* eslot;      // its forced-shell slot
+  const double* ew;      // its weight (1 at kx = 0, else 2)
+  int nslot;             // forced shells
+  const int* slot_shell; // shell index of each slot
+  double* R;             // per slot: oscillator state (rf) or power spectrum (qf)
+  double* Em;            // per slot: target spectrum (rf)
+  double* Ef;            // per slot: current spectrum of the plane
+  const double* Zf;      // per slot (rf)
+  double* factor;        // per shell (nsh): rf factor / qf fkn, 0 where not forced
+  int nvert;             // vertical modes kx = ky = 0 in the reference's order
+  const int* vert;       // their compact indices
+  int nsh;
+  double maxdk, tf, alpha, cfl;
+  int variable_dt;
+};
+template <typename T>
+void launch_plane_forcing(const Geo& g, const void* u, DevState* st, const PlaneForcing& pf, int it, hipStream_t s);
 
 enum : int { kStatsSlots = 32 };
 
@@ -112,7 +138,10 @@ struct UpdateArgs {
   int integrator;      // 0 = Adams_Bashforth3rdO (explicit viscosity), 1 = ETD3rdO (ETD1/ETD2 start)
   int step;            // integrator iteration after this step's increment (1, 2, ...); 0 = final rhs only
   double nu;           // kinematic viscosity
-  int forcing;         // 0 none, 1 isotropic constant-power shell forcing
+  int forcing;         // 0 none, 1 isotropic constant-power shell forcing, 2 rf / 3 qf plane forcing
+  const double* pf_factor;  // plane forcing: per-shell factor (rf) or fkn (qf)
+  double pf_maxdk;
+  int pf_nsh, pf_kz0;       // shells; smallest |kz| of the vertical modes (rf 1, qf 3; the largest is 5)
   double kf2;          // forced shell |k|^2 bound
   const void* tau;     // Smagorinsky stress, 5 compact components (or null)
   void* rho;           // density (updated in place) or null

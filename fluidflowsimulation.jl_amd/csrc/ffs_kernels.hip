@@ -981,6 +981,26 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
     U[c][0] = v.x; U[c][1] = v.y;
     R[c][0] = r.x; R[c][1] = r.y;
   }
+  // plane forcing (rf / qf): the force is built from the pre-step u1, u2 (forcing.jl:85-90)
+  double F0[2][2] = {{0, 0}, {0, 0}};
+  if (a.forcing >= 2 && a.step > 0) {
+    double f = 0.0;
+    if (cz == 0) {
+      const double kh = sqrt(fma(kv3[0], kv3[0], kv3[1] * kv3[1]));
+      const int sh = (int)rint(kh / a.pf_maxdk);
+      if (sh >= 1 && sh < a.pf_nsh) f = a.pf_factor[sh];
+      if (f != 0.0 && a.forcing == 3) f = f / sqrt(U[0][0] * U[0][0] + U[0][1] * U[0][1] + U[1][0] * U[1][0] +
+                                                   U[1][1] * U[1][1]);  // (u / |u|) fkn
+    } else if (cx == 0 && cy == 0) {
+      const int j = z_fft(g, cz), kz = j < (g.nz + 1) / 2 ? j : j - g.nz;
+      const int akz = kz < 0 ? -kz : kz;
+      if (akz >= a.pf_kz0 && akz <= 5) f = st->force_vert;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) F0[c][p] = U[c][p] * f;
+  }
   if (a.integrator == 0) {  // explicit viscosity (is_implicit false)
     const double kvis = K2 * (-a.nu);
 #pragma unroll
@@ -1091,6 +1111,11 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
 #pragma unroll
       for (int p = 0; p < 2; ++p) U[c][p] += fac * U[c][p];
   }
+  // plane forcing: u_h += force (add_Rf_forcing!, forcing_types.jl:54-64)
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int p = 0; p < 2; ++p) U[c][p] += F0[c][p];
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
     const double p1 = origin ? 0.0 : -(kv3[0] * U[0][p] + kv3[1] * U[1][p] + kv3[2] * U[2][p]) / K2;
@@ -1525,6 +1550,113 @@ __global__ void k_set_dt(DevState* st, int it, int tit, double cfl, double dx, i
   set_dt_body(st, it, tit, cfl, dx, etd, gnorm);
 }
 
+// kz = 0-plane forcing, one block (forcing.jl:6-114 for rf, qfforcing.jl for qf).  Thread s < nslot
+// sums the spectrum of its forced shell over the element list (in list order); thread 0 then runs
+// the reference's serial part: the oscillator R (rf), the per-shell factors, the horizontal power
+// hp, the vertical factor and power vp.  Runs after set_dt (it uses this step's dt).
+__device__ __forceinline__ double proj_c(double ar, double ai, double br, double bi) { return fma(ar, br, ai * bi); }
+
+template <typename T>
+__global__ void k_plane_forcing(Geo g, const C<T>* __restrict__ u, DevState* st, PlaneForcing pf, int it) {
+  const size_t n = (size_t)g.sx * g.my * g.mz;
+  const C<T>* u1 = u;
+  const C<T>* u2 = u + n;
+  for (int sl = threadIdx.x; sl < pf.nslot; sl += blockDim.x) {
+    double e = 0.0;
+    for (int i = 0; i < pf.nel; ++i) {
+      if (pf.eslot[i] != sl) continue;
+      const C<T> a = u1[pf.elem[i]], b = u2[pf.elem[i]];
+      const double m2 = (double)a.x * a.x + (double)a.y * a.y + ((double)b.x * b.x + (double)b.y * b.y);
+      // calculate_u1u2_spectrum! (spectrum.jl:1-20) / calculate_u1u2_mag_spectrum! (spectrum.jl:44-65)
+      e += (pf.kind == 2 || pf.kind == -2) ? 0.5 * pf.ew[i] * m2 : pf.ew[i] * sqrt(m2);
+    }
+    pf.Ef[sl] = e;
+    if (pf.kind < 0) pf.Em[sl] = e;  // initialize!(::RfForcing): the target is the initial spectrum
+  }
+  __syncthreads();
+  if (threadIdx.x != 0 || pf.kind < 0) return;
+  const double dt = st->dt;
+  double hp = 0.0, vp = 0.0, fvert = 0.0;
+  if (pf.kind == 2) {
+    double ts = dt / pf.cfl;
+    if (pf.variable_dt) {
+      if (it == 0) ts /= 0.1;
+      else if (it == 1) ts /= 0.2;
+      else if (it == 2) ts /= 0.4;
+      else if (it == 3) ts /= 0.8;
+    }
+    const double omega = 2 * 3.141592653589793 / (pf.tf * ts);
+    for (int sl = 0; sl < pf.nslot; ++sl) {
+      double R = pf.R[sl];
+      R += dt * (-2 * pf.alpha * omega * R - omega * omega * (pf.Ef[sl] - pf.Em[sl]));
+      R = fmax(0.0, R);
+      pf.R[sl] = R;
+      pf.factor[pf.slot_shell[sl]] = sqrt(R / fmax(pf.Ef[sl], 2.220446049250313e-16)) * pf.Zf[sl] * dt;
+    }
+    double ff = 0.0, fv = 0.0;
+    for (int i = 0; i < pf.nel; ++i) {
+      const double f = pf.factor[pf.slot_shell[pf.eslot[i]]];
+      const C<T> a = u1[pf.elem[i]], b = u2[pf.elem[i]];
+      const double s1r = a.x * f, s1i = a.y * f, s2r = b.x * f, s2i = b.y * f;
+      ff += (s1r * s1r + s1i * s1i + (s2r * s2r + s2i * s2i)) * pf.ew[i];
+      fv += (proj_c(a.x, a.y, s1r, s1i) + proj_c(b.x, b.y, s2r, s2i)) * pf.ew[i];
+    }
+    hp = ff * 0.5 / dt + fv / dt;
+    // vertical: kx = ky = 0, kz = 1..5 and -1..-5, 1 % of the power
+    double umag = 0.0;
+    for (int k = 0; k < 5; ++k) {
+      const C<T> a = u1[pf.vert[2 * k]], b = u2[pf.vert[2 * k]];
+      const C<T> c = u1[pf.vert[2 * k + 1]], d = u2[pf.vert[2 * k + 1]];
+      umag += sqrt((double)a.x * a.x + (double)a.y * a.y + ((double)b.x * b.x + (double)b.y * b.y)) +
+              sqrt((double)c.x * c.x + (double)c.y * c.y + ((double)d.x * d.x + (double)d.y * d.y));
+    }
+    fvert = dt * sqrt(2.0 * hp * (1.0 - 0.99)) / umag;
+    ff = 0.0;
+    fv = 0.0;
+    for (int k = 0; k < 5; ++k)
+      for (int h = 0; h < 2; ++h) {
+        const C<T> a = u1[pf.vert[2 * k + h]], b = u2[pf.vert[2 * k + h]];
+        const double s1r = a.x * fvert, s1i = a.y * fvert, s2r = b.x * fvert, s2i = b.y * fvert;
+        ff += s1r * s1r + s1i * s1i + (s2r * s2r + s2i * s2i);
+        fv += proj_c(a.x, a.y, s1r, s1i) + proj_c(b.x, b.y, s2r, s2i);
+      }
+    vp = ff * 0.5 / dt + fv / dt;
+  } else {
+    for (int sl = 0; sl < pf.nslot; ++sl) pf.factor[pf.slot_shell[sl]] = dt * pf.R[sl] * 0.99 / pf.Ef[sl];
+    double fv = 0.0;
+    for (int i = 0; i < pf.nel; ++i) {
+      const double fkn = pf.factor[pf.slot_shell[pf.eslot[i]]];
+      const C<T> a = u1[pf.elem[i]], b = u2[pf.elem[i]];
+      const double umag = sqrt((double)a.x * a.x + (double)a.y * a.y + ((double)b.x * b.x + (double)b.y * b.y));
+      const double s1r = a.x / umag * fkn, s1i = a.y / umag * fkn, s2r = b.x / umag * fkn, s2i = b.y / umag * fkn;
+      fv += (proj_c(a.x, a.y, s1r / dt, s1i / dt) + proj_c(b.x, b.y, s2r / dt, s2i / dt)) * pf.ew[i];
+    }
+    hp = fv;
+    // vertical: kx = ky = 0, kz = 3..5 and -3..-5
+    double umagsq = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const C<T> a = u1[pf.vert[2 * k]], b = u2[pf.vert[2 * k]];
+      const C<T> c = u1[pf.vert[2 * k + 1]], d = u2[pf.vert[2 * k + 1]];
+      umagsq += (double)a.x * a.x + (double)a.y * a.y + ((double)b.x * b.x + (double)b.y * b.y) +
+                ((double)c.x * c.x + (double)c.y * c.y) + ((double)d.x * d.x + (double)d.y * d.y);
+    }
+    fvert = dt * hp * (1 - 0.99) / (umagsq * 0.99);
+    for (int k = 0; k < 3; ++k)
+      for (int h = 0; h < 2; ++h) {
+        const C<T> a = u1[pf.vert[2 * k + h]], b = u2[pf.vert[2 * k + h]];
+        vp += proj_c(a.x, a.y, a.x * fvert / dt, a.y * fvert / dt) + proj_c(b.x, b.y, b.x * fvert / dt, b.y * fvert / dt);
+      }
+  }
+  st->force_hp = hp;
+  st->force_vp = vp;
+  st->force_vert = fvert;
+}
+
+template <typename T>
+void launch_plane_forcing(const Geo& g, const void* u, DevState* st, const PlaneForcing& pf, int it, hipStream_t s) {
+  k_plane_forcing<T><<<1, 64, 0, s>>>(g, (const C<T>*)u, st, pf, it);
+}
+
 // one block of 64 threads: the forcing band sum (as k_forcing_sum) and, on thread 0, set_dt
 template <typename T>
 __global__ void k_step_scalars(Geo g, const C<T>* __restrict__ u, DevState* st, ScalarArgs a) {
@@ -1721,6 +1853,7 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   template void launch_forcing_sum<T>(const Geo&, const void*, const int*, int, DevState*, double, hipStream_t);               \
   template void launch_cast_copy<T>(const void*, void*, size_t, hipStream_t);                                       \
   template void launch_step_scalars<T>(const Geo&, const void*, DevState*, const ScalarArgs&, hipStream_t);          \
+  template void launch_plane_forcing<T>(const Geo&, const void*, DevState*, const PlaneForcing&, int, hipStream_t);   \
   template void launch_seed<T>(const Geo&, void*, int, unsigned long long, double, double, double*, double*,           \
                                hipStream_t);
 

@@ -122,6 +122,12 @@ struct ffs_sim {
   int* band = nullptr;
   int nband = 0;
   double kf2 = 0.0;
+  // kz = 0-plane forcing (rf / qf)
+  bool plane = false;
+  PlaneForcing pfd{};
+  std::vector<double> pf_avgk;    // mean |k_h| per 2D shell
+  std::vector<int> pf_slot_shell; // shell of each forced slot
+  bool pf_em_set = false;         // Em given before initialize (targSpectrum.dat)
 
   // run state (host mirror of s.iteration[] and the integrator's iteration)
   int64_t iteration = 0;
@@ -367,6 +373,104 @@ struct ffs_sim {
       nband = (int)b.size();
       if (nband) band = upload(b);
     }
+    if (plane) setup_plane_forcing();
+  }
+
+  // compute_shells2D (utils.jl:161-182), calculate_Zf (utils.jl:184-197), the forced elements of the
+  // kz = 0 plane and the vertical modes, and for QfForcing build_Qf_power_spectrum!
+  // (forcing_types.jl:304-331)
+  void setup_plane_forcing() {
+    const int NX = g.nxh, NY = g.ny;
+    const int nsh = std::min(NX, NY / 2);
+    const double maxdk = std::max(KX[1], KY[1]);
+    std::vector<double> kh(nsh, 0.0);
+    std::vector<int> cnt(nsh, 0);
+    for (int j = 0; j < NY; ++j)
+      for (int i = 0; i < NX; ++i) {
+        const double K = std::sqrt(KX[i] * KX[i] + KY[j] * KY[j]);
+        const int ii = (int)std::nearbyint(K / maxdk);
+        if (ii >= nsh) break;
+        kh[ii] += K;
+        cnt[ii] += 1;
+      }
+    for (int i = 0; i < nsh; ++i) kh[i] = kh[i] / cnt[i];  // 0/0 = NaN for an empty shell, as the reference
+    pf_avgk = kh;
+    const double kf = p.forcing_kf;
+    // forced shells: 1-based n in 2..nsh with mean |k_h| <= kf
+    std::vector<int> slot_of(nsh, -1);
+    pf_slot_shell.clear();
+    std::vector<double> zf;
+    for (int n = 1; n < nsh; ++n)
+      if (kh[n] <= kf) {
+        slot_of[n] = (int)pf_slot_shell.size();
+        pf_slot_shell.push_back(n);
+        const double k = kh[n];
+        zf.push_back(p.forcing_zf_tanh ? std::tanh((kf - k) / (0.25 * kf)) * (((kf - k) <= 0.0) ? 0.0 : 1.0)
+                                       : (k <= kf ? 1.0 : 0.0));
+      }
+    const int nslot = (int)pf_slot_shell.size();
+    if (nslot == 0) throw FfsError(FFS_ERR_ARG, "no 2D shell has mean |k_h| <= kf");
+    // forced elements of the kz = 0 plane in the reference's order (y outer, x inner)
+    std::vector<int> elem, eslot;
+    std::vector<double> ew;
+    for (int cy = 0; cy < g.my; ++cy)
+      for (int cx = 0; cx < g.mx; ++cx) {
+        const double k = std::sqrt(std::fma(kxc[cx], kxc[cx], kyc[cy] * kyc[cy]));
+        const int n = (int)std::nearbyint(k / maxdk);
+        if (n >= 1 && n < nsh && slot_of[n] >= 0) {
+          elem.push_back(cx + g.sx * cy);
+          eslot.push_back(slot_of[n]);
+          ew.push_back(cx == 0 ? 1.0 : 2.0);
+        }
+      }
+    // vertical modes kx = ky = 0: (kz = +m, kz = -m) for m = m0..5
+    const int m0 = p.forcing == FFS_FORCING_RF ? 1 : 3;
+    std::vector<int> vert;
+    for (int m = m0; m <= 5; ++m)
+      for (int sgn : {1, -1}) {
+        const int cz = z_compact(g, sgn > 0 ? m : g.nz - m);
+        if (cz < 0) throw FfsError(FFS_ERR_ARG, "plane forcing needs the modes |kz| <= 5 (nz >= 16)");
+        vert.push_back(g.sx * g.my * cz);
+      }
+    std::vector<double> R(nslot, 0.0), Em(nslot, 0.0), Ef(nslot, 0.0), factor(nsh, 0.0);
+    if (p.forcing == FFS_FORCING_QF) {
+      double integral = 0.0;
+      for (int sl = 0; sl < nslot; ++sl) {
+        const double k = kh[pf_slot_shell[sl]];
+        const double term = std::exp(-std::pow((k - p.alvelius_kf) / p.alvelius_c, 2));
+        R[sl] = p.alvelius_p * term;
+        integral += term;
+      }
+      for (double& r : R) r /= integral;
+      if (les) {
+        const double delta = p.filter_width > 0 ? p.filter_width : (p.x_domain * 2 * M_PI / p.nx) / p.cutoff;
+        for (int sl = 0; sl < nslot; ++sl) {
+          const double k = kh[pf_slot_shell[sl]];
+          const double G = std::exp(-(delta * delta) / 24 * (k * k));  // Gaussfilter, utils.jl:220-223
+          R[sl] *= G * G;
+        }
+      }
+    }
+    pfd.kind = p.forcing;
+    pfd.nel = (int)elem.size();
+    pfd.elem = upload(elem.empty() ? std::vector<int>{0} : elem);
+    pfd.eslot = upload(eslot.empty() ? std::vector<int>{0} : eslot);
+    pfd.ew = upload(ew.empty() ? std::vector<double>{0.0} : ew);
+    pfd.nslot = nslot;
+    pfd.slot_shell = upload(pf_slot_shell);
+    pfd.R = upload(R);
+    pfd.Em = upload(Em);
+    pfd.Ef = upload(Ef);
+    pfd.Zf = upload(zf);
+    pfd.factor = upload(factor);
+    pfd.nvert = (int)vert.size();
+    pfd.vert = upload(vert);
+    pfd.nsh = nsh;
+    pfd.maxdk = maxdk;
+    pfd.tf = p.forcing_tf;
+    pfd.alpha = p.forcing_alpha;
+    pfd.cfl = p.cfl;
+    pfd.variable_dt = p.variable_dt;
   }
 
   void setup_state() {
@@ -386,8 +490,14 @@ struct ffs_sim {
     if (p.integrator != FFS_AB3 && p.integrator != FFS_ETD3) throw FfsError(FFS_ERR_ARG, "unknown integrator");
     if (p.les_model != FFS_LES_NONE && p.les_model != FFS_LES_SMAGORINSKY)
       throw FfsError(FFS_ERR_ARG, "unsupported LES model (Smagorinsky is available)");
-    if (p.forcing != FFS_FORCING_NONE && p.forcing != FFS_FORCING_SHELL)
+    if (p.forcing != FFS_FORCING_NONE && p.forcing != FFS_FORCING_SHELL && p.forcing != FFS_FORCING_RF &&
+        p.forcing != FFS_FORCING_QF)
       throw FfsError(FFS_ERR_ARG, "unsupported forcing");
+    plane = (p.forcing == FFS_FORCING_RF || p.forcing == FFS_FORCING_QF);
+    if (p.forcing == FFS_FORCING_RF && !(p.forcing_tf > 0))
+      throw FfsError(FFS_ERR_ARG, "rfForcing needs TF > 0");
+    if (p.forcing == FFS_FORCING_QF && !(p.alvelius_c != 0))
+      throw FfsError(FFS_ERR_ARG, "QfForcing needs alveliusC != 0");
     if (p.precision != FFS_FP64 && p.precision != FFS_FP32) throw FfsError(FFS_ERR_ARG, "unknown precision");
     etd = (p.integrator == FFS_ETD3);
     les = (p.les_model == FFS_LES_SMAGORINSKY);
@@ -538,8 +648,11 @@ struct ffs_sim {
     a.band = band;
     a.nband = force ? nband : 0;
     a.power = p.forcing_power;
-    if (!a.do_dt && !a.nband) return;
-    launch("step_scalars", [&] { FFS_DISPATCH(launch_step_scalars, g, u, st, a, stream); });
+    if (a.do_dt || a.nband) launch("step_scalars", [&] { FFS_DISPATCH(launch_step_scalars, g, u, st, a, stream); });
+    if (do_force && plane) {
+      const int it = (int)std::min<int64_t>(iteration, 4);  // matters only through it <= 3 (the ramp)
+      launch("plane_forcing", [&] { FFS_DISPATCH(launch_plane_forcing, g, u, st, pfd, it, stream); });
+    }
   }
   void set_dt() { step_scalars(true, false); }
   void forcing() { step_scalars(false, true); }
@@ -606,6 +719,9 @@ struct ffs_sim {
     }
     r.stats.push_back(tdiss);
     r.stats.insert(r.stats.end(), other.begin(), other.end());
+    if (plane) {  // stats(::RfForcing / ::QfForcing): hp, vp, tp (forcing_types.jl:33, 291)
+      r.stats.insert(r.stats.end(), {sd.force_hp, sd.force_vp, sd.force_hp + sd.force_vp});
+    }
     const double u1p2 = u12, u2p2 = u22, u3p2 = u32;
     const double uh = std::sqrt((u1p2 + u2p2) / 2);
     const double urms = std::sqrt((u1p2 + u2p2 + u3p2) / 3);
@@ -650,6 +766,13 @@ struct ffs_sim {
     a.step = step;
     a.nu = p.nu;
     a.forcing = (p.forcing == FFS_FORCING_SHELL && nband > 0) ? 1 : 0;
+    if (plane) {
+      a.forcing = p.forcing;
+      a.pf_factor = pfd.factor;
+      a.pf_maxdk = pfd.maxdk;
+      a.pf_nsh = pfd.nsh;
+      a.pf_kz0 = p.forcing == FFS_FORCING_RF ? 1 : 3;
+    }
     a.kf2 = kf2;
     a.tau = les ? tau : nullptr;
     if (dens) {
@@ -749,6 +872,13 @@ struct ffs_sim {
       HIPCHECK(hipMemcpyAsync(rfm1, rrhs, S * csz, hipMemcpyDeviceToDevice, stream));
       HIPCHECK(hipMemcpyAsync(rfm2, rrhs, S * csz, hipMemcpyDeviceToDevice, stream));
     }
+    // initialize!(::RfForcing) (forcing_types.jl:66-74): without a given target spectrum, the target
+    // is the initial field's
+    if (plane && p.forcing == FFS_FORCING_RF && !pf_em_set) {
+      PlaneForcing e = pfd;
+      e.kind = -FFS_FORCING_RF;
+      launch("plane_forcing_init", [&] { FFS_DISPATCH(launch_plane_forcing, g, u, st, e, 0, stream); });
+    }
     initialized = true;
   }
 
@@ -782,6 +912,7 @@ struct ffs_sim {
   size_t checkpoint_bytes() const {
     size_t n = sizeof(CkptHeader) + sizeof(DevState);
     for (void* b : ckpt_buffers()) n += ckpt_buffer_bytes(b);
+    if (plane) n += 2 * pfd.nslot * sizeof(double);  // the forcing's R and Em
     return n;
   }
 
@@ -817,6 +948,11 @@ struct ffs_sim {
       HIPCHECK(hipMemcpyAsync(out + off, b, ckpt_buffer_bytes(b), hipMemcpyDeviceToHost, stream));
       off += ckpt_buffer_bytes(b);
     }
+    if (plane)
+      for (double* b : {pfd.R, pfd.Em}) {
+        HIPCHECK(hipMemcpyAsync(out + off, b, pfd.nslot * sizeof(double), hipMemcpyDeviceToHost, stream));
+        off += pfd.nslot * sizeof(double);
+      }
     HIPCHECK(hipStreamSynchronize(stream));
   }
 
@@ -840,6 +976,11 @@ struct ffs_sim {
       HIPCHECK(hipMemcpy(b, in + off, ckpt_buffer_bytes(b), hipMemcpyHostToDevice));
       off += ckpt_buffer_bytes(b);
     }
+    if (plane)
+      for (double* b : {pfd.R, pfd.Em}) {
+        HIPCHECK(hipMemcpy(b, in + off, pfd.nslot * sizeof(double), hipMemcpyHostToDevice));
+        off += pfd.nslot * sizeof(double);
+      }
     iteration = h.iteration;
     tit = h.tit;
     host_time = h.host_time;
@@ -1026,6 +1167,12 @@ int ffs_params_default(ffs_params* p) {
   p->reference_density = 1.0;
   p->forcing_kf = 2.5;
   p->forcing_power = 0.1;
+  p->forcing_tf = 0.0;
+  p->forcing_alpha = 0.0;
+  p->forcing_zf_tanh = 0;
+  p->alvelius_p = 0.0;
+  p->alvelius_c = 1.0;
+  p->alvelius_kf = 0.0;
   return FFS_OK;
 }
 
@@ -1219,6 +1366,39 @@ int ffs_advance(ffs_sim* s, int64_t nsteps, int64_t* done, int* event) {
   });
 }
 
+int ffs_forcing_state(ffs_sim* s, int what, int set, double* data, int cap, int* n) {
+  if (!s) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    if (!s->plane) throw FfsError(FFS_ERR_ARG, "the run has no plane forcing (rfForcing / QfForcing)");
+    const int nsh = s->pfd.nsh, nslot = s->pfd.nslot;
+    if (n) *n = nsh;
+    if (!data) return;
+    if (what == FFS_FSTATE_KH) {
+      if (set) throw FfsError(FFS_ERR_ARG, "the shell wavenumbers are read only");
+      for (int i = 0; i < nsh && i < cap; ++i) data[i] = s->pf_avgk[i];
+      return;
+    }
+    double* dev = what == FFS_FSTATE_R ? s->pfd.R : what == FFS_FSTATE_EM ? s->pfd.Em
+                                       : what == FFS_FSTATE_EF ? s->pfd.Ef : nullptr;
+    if (!dev) throw FfsError(FFS_ERR_ARG, "unknown forcing state");
+    std::vector<double> slots(nslot);
+    if (set) {
+      if (what == FFS_FSTATE_EF) throw FfsError(FFS_ERR_ARG, "the current spectrum is read only");
+      if (cap < nsh) throw FfsError(FFS_ERR_ARG, "the state has one value per 2D shell");
+      for (int sl = 0; sl < nslot; ++sl) slots[sl] = data[s->pf_slot_shell[sl]];
+      HIPCHECK(hipStreamSynchronize(s->stream));
+      HIPCHECK(hipMemcpy(dev, slots.data(), nslot * sizeof(double), hipMemcpyHostToDevice));
+      if (what == FFS_FSTATE_EM) s->pf_em_set = true;
+    } else {
+      HIPCHECK(hipStreamSynchronize(s->stream));
+      HIPCHECK(hipMemcpy(slots.data(), dev, nslot * sizeof(double), hipMemcpyDeviceToHost));
+      for (int i = 0; i < nsh && i < cap; ++i) data[i] = 0.0;
+      for (int sl = 0; sl < nslot; ++sl)
+        if (s->pf_slot_shell[sl] < cap) data[s->pf_slot_shell[sl]] = slots[sl];
+    }
+  });
+}
+
 int ffs_checkpoint_size(const ffs_sim* s, size_t* bytes) {
   if (!s || !bytes) return FFS_ERR_ARG;
   *bytes = s->checkpoint_bytes();
@@ -1272,6 +1452,7 @@ int ffs_stats_header(const ffs_sim* s, int table, char* buf, size_t cap) {
     h = "iteration,time,u1,u2,u3,u1p2,u2p2,u3p2,kh,kv,kaniso,k,dissh,dissv,diss,nlh,nlv,pressh,pressv,tnlh,tnlv,tdiss";
     if (s->dens) h += ",rho,rhop2,drhodxp2,drhodyp2,drhodzp2,rhodiss,trhodiss,bflux";
     if (s->les) h += ",prh,prv,praniso,pr";
+    if (s->plane) h += ",hp,vp,tp";
   } else if (table == FFS_TABLE_SCALES) {
     h = "iteration,time,Lk,Lt,lamg,laml,lamt,Lhh,Lvv,Lhv,Lvh,lam12,lam21,L11,L12,L13,L21,L22,L23,L31,L32,L33";
     h += s->dens ? ",Lb,Lo,Lom,Tk,Tt,Thh,Tb,Tbm" : ",Tk,Tt,Thh";

@@ -22,7 +22,8 @@ ERRORS = {
 FFS_AB3, FFS_ETD3 = 0, 1
 FFS_FP64, FFS_FP32 = 0, 1
 FFS_LES_NONE, FFS_LES_SMAGORINSKY = 0, 1
-FFS_FORCING_NONE, FFS_FORCING_SHELL = 0, 1
+FFS_FORCING_NONE, FFS_FORCING_SHELL, FFS_FORCING_RF, FFS_FORCING_QF = 0, 1, 2, 3
+FFS_FSTATE_R, FFS_FSTATE_EM, FFS_FSTATE_KH, FFS_FSTATE_EF = 0, 1, 2, 3
 FFS_FIELD_U1, FFS_FIELD_U2, FFS_FIELD_U3, FFS_FIELD_RHO = 0, 1, 2, 3
 FFS_SPACE_REAL, FFS_SPACE_FOURIER = 0, 1
 FFS_SCALAR_ITERATION, FFS_SCALAR_TIME, FFS_SCALAR_DT, FFS_SCALAR_UMAX = 0, 1, 2, 3
@@ -64,6 +65,12 @@ class FfsParams(ctypes.Structure):
         ("forcing", ctypes.c_int32),
         ("forcing_kf", ctypes.c_double),
         ("forcing_power", ctypes.c_double),
+        ("forcing_tf", ctypes.c_double),
+        ("forcing_alpha", ctypes.c_double),
+        ("forcing_zf_tanh", ctypes.c_int32),
+        ("alvelius_p", ctypes.c_double),
+        ("alvelius_c", ctypes.c_double),
+        ("alvelius_kf", ctypes.c_double),
     ]
 
 
@@ -84,6 +91,7 @@ SIGNATURES = {
     "ffs_advance": (ctypes.c_int, [_P, ctypes.c_int64, _I64, _I]),
     "ffs_writestats_now": (ctypes.c_int, [_P]),
     "ffs_checkpoint_size": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    "ffs_forcing_state": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, _I]),
     "ffs_checkpoint_save": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_size_t]),
     "ffs_checkpoint_load": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_size_t]),
     "ffs_stats_pop": (ctypes.c_int, [_P, _D, ctypes.c_int, _I, _D, ctypes.c_int, _I, _D, ctypes.c_int, _I]),

@@ -158,6 +158,12 @@ class Params:
     forcing: str = None
     forcing_kf: float = 2.5
     forcing_power: float = 0.1
+    forcing_tf: float = 0.0         # TF (rfForcing)
+    forcing_alpha: float = 0.0      # alphac (rfForcing)
+    forcing_zf: str = "cutoff"      # Zf: cutoff | tanh
+    alvelius_p: float = 0.0         # alveliusP, alveliusC, alveliusKf (QfForcing)
+    alvelius_c: float = 1.0
+    alvelius_kf: float = 0.0
     extra: dict = field(default_factory=dict)
 
     def to_c(self):
@@ -182,8 +188,12 @@ class Params:
         p.density = int(bool(self.density))
         p.pr, p.density_gradient = self.pr, self.density_gradient
         p.z_acceleration, p.reference_density = self.z_acceleration, self.reference_density
-        p.forcing = {None: _lib.FFS_FORCING_NONE, "shellForcing": _lib.FFS_FORCING_SHELL}.get(self.forcing, -1)
+        p.forcing = {None: _lib.FFS_FORCING_NONE, "shellForcing": _lib.FFS_FORCING_SHELL,
+                     "rfForcing": _lib.FFS_FORCING_RF, "QfForcing": _lib.FFS_FORCING_QF}.get(self.forcing, -1)
         p.forcing_kf, p.forcing_power = self.forcing_kf, self.forcing_power
+        p.forcing_tf, p.forcing_alpha = self.forcing_tf, self.forcing_alpha
+        p.forcing_zf_tanh = int(self.forcing_zf == "tanh")
+        p.alvelius_p, p.alvelius_c, p.alvelius_kf = self.alvelius_p, self.alvelius_c, self.alvelius_kf
         return p
 
 
@@ -234,7 +244,18 @@ def parameters(d):
     if "forcing" in d:
         known.add("forcing")
         p.forcing = d["forcing"]
-        p.forcing_kf = opt("forcingKf", parse_expr, 2.5)
-        p.forcing_power = opt("forcingPower", parse_expr, 0.1)
+        if p.forcing == "rfForcing":  # forcing_model, forcing_types.jl:338-359
+            p.forcing_tf = float(req("TF"))
+            p.forcing_alpha = float(req("alphac"))
+            p.forcing_kf = float(req("kf"))
+            p.forcing_zf = opt("Zf", str, "cutoff")
+        elif p.forcing == "QfForcing":  # forcing_types.jl:424-446
+            p.alvelius_p = float(req("alveliusP"))
+            p.alvelius_c = float(req("alveliusC"))
+            p.alvelius_kf = float(req("alveliusKf"))
+            p.forcing_kf = float(req("kf"))
+        else:
+            p.forcing_kf = opt("forcingKf", parse_expr, 2.5)
+            p.forcing_power = opt("forcingPower", parse_expr, 0.1)
     p.extra = {k: v for k, v in d.items() if k not in known}
     return p

@@ -131,6 +131,20 @@ class Simulation:
                 on_output(self)
         return self
 
+    def forcing_state(self, what=_lib.FFS_FSTATE_R):
+        """Per-2D-shell state of the plane forcing (R, Em, mean |k_h|, current spectrum)."""
+        n = ctypes.c_int()
+        self._check(self._lib.ffs_forcing_state(self._h, what, 0, None, 0, ctypes.byref(n)))
+        a = np.zeros(n.value)
+        self._check(self._lib.ffs_forcing_state(self._h, what, 0, _dptr(a), a.size, ctypes.byref(n)))
+        return a
+
+    def set_forcing_state(self, what, values):
+        """Sets R (e.g. from an R.N file) or the target spectrum Em (before initialize)."""
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        n = ctypes.c_int()
+        self._check(self._lib.ffs_forcing_state(self._h, what, 1, _dptr(a), a.size, ctypes.byref(n)))
+
     def checkpoint(self):
         """The run's full integrator state as bytes (ffs_checkpoint_save): a run restored from it
         continues bitwise as this one would."""
@@ -286,6 +300,8 @@ def run_simulation(directory=".", params=None, nt=None):
     nx, ny, nz = params.nx, params.ny, params.nz
     for f in range(3):
         s.set_field(f, read_field(os.path.join(directory, f"u{f + 1}.{params.start}"), nx, ny, nz))
+    if params.forcing in ("rfForcing", "QfForcing"):
+        _load_forcing_files(s, directory, params)
     if params.start == 0:
         s.writeheader(directory)
     s.initialize()
@@ -305,4 +321,22 @@ def run_simulation(directory=".", params=None, nt=None):
     s.writestats(directory)
     if s.iteration % params.write_time != 0:
         s.writeoutput(directory)
+    if params.forcing == "rfForcing":  # writedlm("R.$it", s.forcing.R), FluidFlowSimulation.jl:59-61
+        np.savetxt(os.path.join(directory, f"R.{s.iteration}"), s.forcing_state(_lib.FFS_FSTATE_R))
     return s
+
+
+def _load_forcing_files(s, directory, params):
+    """targSpectrum.dat (calculate_Em!, forcing_types.jl:76-83: skip the header line, linear
+    interpolation at the shells' mean |k_h|; np.interp holds the end values outside the table) and
+    R.$start (initialize!(::RfForcing), forcing_types.jl:66-74)."""
+    if params.forcing != "rfForcing":
+        return
+    targ = os.path.join(directory, "targSpectrum.dat")
+    if os.path.exists(targ):
+        data = np.loadtxt(targ, skiprows=1)
+        kh = s.forcing_state(_lib.FFS_FSTATE_KH)
+        s.set_forcing_state(_lib.FFS_FSTATE_EM, np.interp(np.nan_to_num(kh), data[:, 0], data[:, 1]))
+    rfile = os.path.join(directory, f"R.{params.start}")
+    if os.path.exists(rfile):
+        s.set_forcing_state(_lib.FFS_FSTATE_R, np.loadtxt(rfile).ravel())

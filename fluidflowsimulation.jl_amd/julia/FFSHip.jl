@@ -45,6 +45,12 @@ mutable struct ffs_params
     forcing::Int32
     forcing_kf::Float64
     forcing_power::Float64
+    forcing_tf::Float64
+    forcing_alpha::Float64
+    forcing_zf_tanh::Int32
+    alvelius_p::Float64
+    alvelius_c::Float64
+    alvelius_kf::Float64
     ffs_params() = new()
 end
 
@@ -146,6 +152,21 @@ function advance_in_time!(s::FFSSimulation)
     writestats(s)
     return nothing
 end
+
+# plane forcing state per 2D shell (R, Em, mean |k_h|, current spectrum): what = 0 R, 1 Em, 2 KH, 3 Ef
+function forcing_state(s::FFSSimulation, what::Integer)
+    n = Ref{Cint}(0)
+    check(ccall((:ffs_forcing_state, libffs), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Cint, Ref{Cint}),
+                s.h, what, 0, C_NULL, 0, n), s.h)
+    a = zeros(n[])
+    check(ccall((:ffs_forcing_state, libffs), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Cint, Ref{Cint}),
+                s.h, what, 0, a, length(a), n), s.h)
+    return a
+end
+
+set_forcing_state!(s::FFSSimulation, what::Integer, a::Vector{Float64}) =
+    check(ccall((:ffs_forcing_state, libffs), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Cint, Ref{Cint}),
+                s.h, what, 1, a, length(a), Ref{Cint}(0)), s.h)
 
 # exact-resume checkpoints (the reference restarts without its integrator history)
 function checkpoint(s::FFSSimulation)

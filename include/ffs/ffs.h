@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FFS_ABI_VERSION 1
+#define FFS_ABI_VERSION 2
 
 /* error codes */
 #define FFS_OK 0
@@ -53,6 +53,12 @@ extern "C" {
  * counterpart in the reference's kz=0-plane schemes */
 #define FFS_FORCING_NONE 0
 #define FFS_FORCING_SHELL 1
+/* the reference's kz = 0-plane schemes (Forcing_methods/forcing_types.jl): the 2D shells of the
+ * horizontal velocity on the kz = 0 plane driven toward a target spectrum Em by a damped
+ * oscillator R per shell (rfForcing, forcing.jl:6-114), or with a fixed power spectrum
+ * (QfForcing, qfforcing.jl), plus 1 % of the power on kx = ky = 0, |kz| <= 5 */
+#define FFS_FORCING_RF 2
+#define FFS_FORCING_QF 3
 
 /* fields */
 #define FFS_FIELD_U1 0
@@ -101,8 +107,12 @@ typedef struct ffs_params {
   int32_t density;                          /* densityStratification present */
   double pr, density_gradient, z_acceleration, reference_density; /* Pr, densityGradient, zAcceleration, referenceDensity */
   int32_t forcing;                          /* FFS_FORCING_* */
-  double forcing_kf;                        /* forced band |k| <= kf */
-  double forcing_power;                     /* injected power P */
+  double forcing_kf;                        /* forced band |k| <= kf (`kf` of rf/Qf forcing) */
+  double forcing_power;                     /* injected power P (shell forcing) */
+  double forcing_tf;                        /* TF (rfForcing) */
+  double forcing_alpha;                     /* alphac (rfForcing) */
+  int32_t forcing_zf_tanh;                  /* Zf tanh (1) or cutoff (0, default) */
+  double alvelius_p, alvelius_c, alvelius_kf; /* alveliusP, alveliusC, alveliusKf (QfForcing) */
 } ffs_params;
 
 typedef struct ffs_sim ffs_sim;
@@ -153,6 +163,19 @@ int ffs_advance(ffs_sim* s, int64_t nsteps, int64_t* done, int* event);
 int ffs_checkpoint_size(const ffs_sim* s, size_t* bytes);
 int ffs_checkpoint_save(ffs_sim* s, void* host, size_t cap);
 int ffs_checkpoint_load(ffs_sim* s, const void* host, size_t n);
+
+/* State of the plane forcing schemes, per 2D shell (compute_shells2D, utils.jl:161-182):
+ * FFS_FSTATE_R the oscillator state R (rfForcing; the `R.N` file of FluidFlowSimulation.jl:59-61
+ * and forcing_types.jl:66-74) or the fixed power spectrum (QfForcing), FFS_FSTATE_EM the target
+ * spectrum (taken from the initial field at ffs_initialize unless set before, as from
+ * targSpectrum.dat, forcing_types.jl:76-83), FFS_FSTATE_KH the shells' mean |k_h| (read only),
+ * FFS_FSTATE_EF the last computed horizontal spectrum of the forced shells (read only).
+ * set = 1 writes data[0..n), set = 0 reads; *n receives the number of shells. */
+#define FFS_FSTATE_R 0
+#define FFS_FSTATE_EM 1
+#define FFS_FSTATE_KH 2
+#define FFS_FSTATE_EF 3
+int ffs_forcing_state(ffs_sim* s, int what, int set, double* data, int cap, int* n);
 
 /* Replaces the tail of run_simulation (FluidFlowSimulation.jl:46-49):
  * calculate_rhs!(s), forcing, writestats(s) at the current iteration. */

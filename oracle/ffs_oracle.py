@@ -153,7 +153,7 @@ class OracleSim:
 
     def __init__(self, grid, nu, dt=0.0, variable_dt=False, cfl=0.18, dt_stat=1 << 30, dt_spec=1 << 30,
                  forcing_kf=None, forcing_power=0.1, start=0, start_time=0.0, integrator="AB3", les_c=None,
-                 les_delta=None, density=None):
+                 les_delta=None, density=None, plane_forcing=None):
         """integrator: "AB3" (Adams_Bashforth3rdO) or "ETD3" (ETD3rdO);
         les_c: Smagorinsky constant (None: no LES), les_delta: filter width
         (default (2 pi L/nx)/cutoff, les_types.jl:366);
@@ -181,6 +181,7 @@ class OracleSim:
         self.forcing_kf = forcing_kf
         self.forcing_power = forcing_power
         self.force_factor = 0.0
+        self.plane = plane_forcing  # oracle.plane_forcing.PlaneForcing (rfForcing / QfForcing) or None
         self.umax = 0.0
         self.records = []
         self.spectra = []
@@ -206,6 +207,8 @@ class OracleSim:
         if self.density is not None:
             self.rfm1 = self.rhorhs.copy()
             self.rfm2 = self.rfm1.copy()
+        if self.plane is not None:
+            self.plane.initialize(self)
 
     # -------------------------------------------------------------- the step
     def is_stats_time(self):
@@ -321,6 +324,8 @@ class OracleSim:
     def forcing(self):
         """Constant-power isotropic shell forcing (no reference counterpart):
         factor = P / sum_{0<|k|<=kf} w |u_k|^2."""
+        if self.plane is not None:
+            self.plane(self)
         if self.forcing_kf is None:
             return
         band = (self.g.K2 <= self.forcing_kf**2) & (self.g.K2 > 0) & self.g.RANGE
@@ -340,6 +345,8 @@ class OracleSim:
         if self.forcing_kf is not None:
             band = (g.K2 <= self.forcing_kf**2) & (g.K2 > 0) & g.RANGE
             u = u + np.where(band, self.dt * self.force_factor, 0.0) * u
+        if self.plane is not None:
+            u = self.plane.add(np.array(u))
         self.uh = g.project(u)
         if self.density is not None:
             if self.integrator == "AB3":
@@ -553,6 +560,8 @@ class OracleSim:
             Lk = np.sqrt(np.sqrt(nu**3 / tdiss))
             Tk = np.sqrt(nu / tdiss)
             lg = np.sqrt(15 * nu / tdiss) * urms
+        if self.plane is not None:  # stats(::RfForcing / ::QfForcing): hp, vp, tp
+            other += [self.plane.hp, self.plane.vp, self.plane.hp + self.plane.vp]
         stats_row = [self.iteration, self.time, *vstats, tdiss, *other]
         scales = [self.iteration, self.time, Lk, Lt, lg, ll, lt, Lhh, Lvv, Lhv, Lvh, l12, l21, L11, L12, L13, L21,
                   L22, L23, L31, L32, L33]

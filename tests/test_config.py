@@ -98,3 +98,18 @@ def test_padded_file_size_256():
     """Padded 256^3 field is 135,266,304 bytes; 64^3 is 2,162,688 bytes."""
     assert 8 * np.prod(fields.padded_shape(256, 256, 256)) == 135266304
     assert 8 * np.prod(fields.padded_shape(64, 64, 64)) == 2162688
+
+
+def test_plane_forcing_keys():
+    base = {"nx": "32", "ny": "32", "nz": "32", "xDomainSize": "1.0", "yDomainSize": "1.0", "zDomainSize": "1.0",
+            "kinematicViscosity": "1e-3", "dtStat": "10", "dtSpec": "100", "writeTime": "100"}
+    d = dict(base, forcing="rfForcing", TF="2.0", alphac="0.5", kf="2.5", Zf="tanh")
+    p = config.parameters(d)
+    assert (p.forcing, p.forcing_tf, p.forcing_alpha, p.forcing_kf, p.forcing_zf) == ("rfForcing", 2.0, 0.5, 2.5, "tanh")
+    c = p.to_c()
+    assert c.forcing == 2 and c.forcing_zf_tanh == 1 and c.forcing_tf == 2.0
+    with pytest.raises(KeyError):  # TF is required, as in forcing_model (forcing_types.jl:339)
+        config.parameters(dict(base, forcing="rfForcing", alphac="0.5", kf="2.5"))
+    q = config.parameters(dict(base, forcing="QfForcing", alveliusP="0.1", alveliusC="1.0", alveliusKf="2", kf="3"))
+    assert (q.alvelius_p, q.alvelius_c, q.alvelius_kf, q.forcing_kf) == (0.1, 1.0, 2.0, 3.0)
+    assert q.to_c().forcing == 3
