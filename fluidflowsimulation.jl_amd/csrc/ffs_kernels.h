@@ -66,31 +66,33 @@ struct DevState {
   double force_vert;          // plane forcing: factor of the kx = ky = 0 modes
 };
 
-// kz = 0-plane forcing (rfForcing / QfForcing, Forcing_methods/forcing.jl:6-114, qfforcing.jl).
-// The forced elements are the retained kz = 0 modes whose 2D shell n (round(|k_h| / maxdk), 0-based)
-// has 0 < n < nsh and mean |k_h| <= kf, listed in the reference's loop order (y outer, x inner).
+// kz = 0-plane forcing, the six schemes of Forcing_methods/ (forcing_types.jl): kind 2 rfForcing
+// (forcing.jl:6-114), 3 QfForcing (qfforcing.jl), 4 cRfForcing (cforcing.jl), 5 nRfForcing
+// (nrforcing.jl), 6 cNRfForcing (cnrforcing.jl), 7 aForcing (aforcing.jl).  A slot is a forced 2D
+// shell (or, for aForcing, a forced mode); the forced elements are the retained kz = 0 modes of the
+// forced slots, listed in the reference's loop order (y outer, x inner).
+enum : int { PF_RF = 2, PF_QF = 3, PF_CRF = 4, PF_NRF = 5, PF_CNRF = 6, PF_A = 7 };
 struct PlaneForcing {
-  int kind;              // 2 rf, 3 qf
+  int kind;              // PF_*; negated: compute the initial target (Em = Ef, and N, Nm1) only
   int nel;               // forced elements
   const int* elem;       // compact index of each (kz = 0 plane)
-  const int* eslot;      // its forced-shell slot
+  const int* eslot;      // its slot
   const double* ew;      // its weight (1 at kx = 0, else 2)
-  int nslot;             // forced shells
-  const int* slot_shell; // shell index of each slot
-  double* R;             // per slot: oscillator state (rf) or power spectrum (qf)
-  double* Em;            // per slot: target spectrum (rf)
-  double* Ef;            // per slot: current spectrum of the plane
-  const double* Zf;      // per slot (rf)
-  double* factor;        // per shell (nsh): rf factor / qf fkn, 0 where not forced
-  int nvert;             // vertical modes kx = ky = 0 in the reference's order
-  const int* vert;       // their compact indices
-  int nsh;
-  double maxdk, tf, alpha, cfl;
+  const double* ek2;     // its |k_h|^2 (the nu k^2 of calculate_nn_spectrum!)
+  int nslot;
+  double* R;             // per slot: oscillator state, or the power spectrum (QfForcing)
+  double* Em;            // per slot: target spectrum
+  double* Ef;            // per slot: current spectrum
+  double* N;             // per slot: nonlinear transfer spectrum (nRf, cNRf) and its previous value
+  double* Nm1;
+  const double* Zf;      // per slot
+  double* slot_factor;   // per slot: factor (rf family) / fkn (QfForcing)
+  double* plane_factor;  // per element of the compact kz = 0 plane (sx x my): the force is u * factor
+  int nvert;             // vertical modes kx = ky = 0 in the reference's order (+m, -m, ...)
+  const int* vert;
+  double tf, alpha, cfl, nu;
   int variable_dt;
 };
-template <typename T>
-void launch_plane_forcing(const Geo& g, const void* u, DevState* st, const PlaneForcing& pf, int it, hipStream_t s);
-
 enum : int { kStatsSlots = 32 };
 
 struct Tables {  // per-length twiddle tables (device pointers), by precision
@@ -139,9 +141,8 @@ struct UpdateArgs {
   int step;            // integrator iteration after this step's increment (1, 2, ...); 0 = final rhs only
   double nu;           // kinematic viscosity
   int forcing;         // 0 none, 1 isotropic constant-power shell forcing, 2 rf / 3 qf plane forcing
-  const double* pf_factor;  // plane forcing: per-shell factor (rf) or fkn (qf)
-  double pf_maxdk;
-  int pf_nsh, pf_kz0;       // shells; smallest |kz| of the vertical modes (rf 1, qf 3; the largest is 5)
+  const double* pf_factor;  // plane forcing: per-element factor on the kz = 0 plane (sx x my, 0 if not forced)
+  int pf_kz0;               // smallest |kz| of the vertical modes (qf 3, the others 1; the largest is 5)
   double kf2;          // forced shell |k|^2 bound
   const void* tau;     // Smagorinsky stress, 5 compact components (or null)
   void* rho;           // density (updated in place) or null
@@ -157,6 +158,11 @@ struct UpdateArgs {
 template <typename T>
 void launch_update(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
                    const UpdateArgs& a, hipStream_t s);
+// the plane forcing of one step (rhs_final = 0: rhs holds the nonlinear term, formed per element as
+// the update kernel does; 1: rhs is final, as after initialization)
+template <typename T>
+void launch_plane_forcing(const Geo& g, const void* u, const void* rhs, DevState* st, const PlaneForcing& pf,
+                          const UpdateArgs& ua, int rhs_final, int it, hipStream_t s);
 template <typename T>
 void launch_project(const Geo& g, void* u, hipStream_t s);
 

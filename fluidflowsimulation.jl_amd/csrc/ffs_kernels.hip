@@ -958,6 +958,48 @@ __device__ __forceinline__ EtdCoef etd_coef(double c, int step, const DevState* 
   return r;
 }
 
+// fourierspacep2_velocity! (models.jl:282-341) on one mode: rhs += -nu k^2 u (AB3: explicit
+// viscosity), + (i k) . tau (LES), + rho g (Boussinesq), then the Leray projection.  RH receives rho.
+template <typename T>
+__device__ __forceinline__ void form_rhs(const UpdateArgs& a, size_t idx, size_t n, const double kv3[3], double K2,
+                                         bool origin, const double U[3][2], double R[3][2], double RH[2]) {
+  if (a.integrator == 0) {  // explicit viscosity (is_implicit false)
+    const double kvis = K2 * (-a.nu);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) R[c][p] += kvis * U[c][p];
+  }
+  if (a.tau) {  // rhs += (i k) . tau, tau symmetric traceless (xx xy xz yy yz)
+    const C<T>* t = (const C<T>*)a.tau;
+    const C<T> txx = t[idx], txy = t[n + idx], txz = t[2 * n + idx], tyy = t[3 * n + idx], tyz = t[4 * n + idx];
+    const double T6[6][2] = {{txx.x, txx.y}, {txy.x, txy.y}, {txz.x, txz.y},
+                             {tyy.x, tyy.y}, {tyz.x, tyz.y}, {-(double)txx.x - tyy.x, -(double)txx.y - tyy.y}};
+    // rows of the symmetric tensor: x (xx xy xz), y (xy yy yz), z (xz yz zz)
+    const int row[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double sr = kv3[0] * T6[row[c][0]][0] + kv3[1] * T6[row[c][1]][0] + kv3[2] * T6[row[c][2]][0];
+      const double si = kv3[0] * T6[row[c][0]][1] + kv3[1] * T6[row[c][1]][1] + kv3[2] * T6[row[c][2]][1];
+      R[c][0] += -si;
+      R[c][1] += sr;
+    }
+  }
+  if (a.rho) {  // buoyancy rhs += rho g, g = (0, 0, g_z)
+    const C<T> r0 = ((const C<T>*)a.rho)[idx];
+    RH[0] = r0.x;
+    RH[1] = r0.y;
+    R[2][0] += RH[0] * a.gz;
+    R[2][1] += RH[1] * a.gz;
+  }
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const double p1 = origin ? 0.0 : -(kv3[0] * R[0][p] + kv3[1] * R[1][p] + kv3[2] * R[2][p]) / K2;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) R[c][p] = p1 * kv3[c] + R[c][p];
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T>* __restrict__ rhs,
                                                const C<T>* __restrict__ fm1, const C<T>* __restrict__ fm2,
@@ -986,11 +1028,7 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
   if (a.forcing >= 2 && a.step > 0) {
     double f = 0.0;
     if (cz == 0) {
-      const double kh = sqrt(fma(kv3[0], kv3[0], kv3[1] * kv3[1]));
-      const int sh = (int)rint(kh / a.pf_maxdk);
-      if (sh >= 1 && sh < a.pf_nsh) f = a.pf_factor[sh];
-      if (f != 0.0 && a.forcing == 3) f = f / sqrt(U[0][0] * U[0][0] + U[0][1] * U[0][1] + U[1][0] * U[1][0] +
-                                                   U[1][1] * U[1][1]);  // (u / |u|) fkn
+      f = a.pf_factor[cx + (size_t)g.sx * cy];  // per-element factor of this step (0 where not forced)
     } else if (cx == 0 && cy == 0) {
       const int j = z_fft(g, cz), kz = j < (g.nz + 1) / 2 ? j : j - g.nz;
       const int akz = kz < 0 ? -kz : kz;
@@ -1001,42 +1039,8 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
 #pragma unroll
       for (int p = 0; p < 2; ++p) F0[c][p] = U[c][p] * f;
   }
-  if (a.integrator == 0) {  // explicit viscosity (is_implicit false)
-    const double kvis = K2 * (-a.nu);
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int p = 0; p < 2; ++p) R[c][p] += kvis * U[c][p];
-  }
-  if (a.tau) {  // rhs += (i k) . tau, tau symmetric traceless (xx xy xz yy yz)
-    const C<T>* t = (const C<T>*)a.tau;
-    const C<T> txx = t[idx], txy = t[n + idx], txz = t[2 * n + idx], tyy = t[3 * n + idx], tyz = t[4 * n + idx];
-    const double T6[6][2] = {{txx.x, txx.y}, {txy.x, txy.y}, {txz.x, txz.y},
-                             {tyy.x, tyy.y}, {tyz.x, tyz.y}, {-(double)txx.x - tyy.x, -(double)txx.y - tyy.y}};
-    // rows of the symmetric tensor: x (xx xy xz), y (xy yy yz), z (xz yz zz)
-    const int row[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double sr = kv3[0] * T6[row[c][0]][0] + kv3[1] * T6[row[c][1]][0] + kv3[2] * T6[row[c][2]][0];
-      const double si = kv3[0] * T6[row[c][0]][1] + kv3[1] * T6[row[c][1]][1] + kv3[2] * T6[row[c][2]][1];
-      R[c][0] += -si;
-      R[c][1] += sr;
-    }
-  }
   double RH[2] = {0, 0};
-  if (a.rho) {  // buoyancy rhs += rho g, g = (0, 0, g_z)
-    const C<T> r0 = ((const C<T>*)a.rho)[idx];
-    RH[0] = r0.x;
-    RH[1] = r0.y;
-    R[2][0] += RH[0] * a.gz;
-    R[2][1] += RH[1] * a.gz;
-  }
-#pragma unroll
-  for (int p = 0; p < 2; ++p) {
-    const double p1 = origin ? 0.0 : -(kv3[0] * R[0][p] + kv3[1] * R[1][p] + kv3[2] * R[2][p]) / K2;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) R[c][p] = p1 * kv3[c] + R[c][p];
-  }
+  form_rhs<T>(a, idx, n, kv3, K2, origin, U, R, RH);
 #pragma unroll
   for (int c = 0; c < 3; ++c) rhs[c * n + idx] = mk<T>((T)R[c][0], (T)R[c][1]);
   // density rhs from the pre-step vertical velocity
@@ -1550,52 +1554,94 @@ __global__ void k_set_dt(DevState* st, int it, int tit, double cfl, double dx, i
   set_dt_body(st, it, tit, cfl, dx, etd, gnorm);
 }
 
-// kz = 0-plane forcing, one block (forcing.jl:6-114 for rf, qfforcing.jl for qf).  Thread s < nslot
-// sums the spectrum of its forced shell over the element list (in list order); thread 0 then runs
-// the reference's serial part: the oscillator R (rf), the per-shell factors, the horizontal power
-// hp, the vertical factor and power vp.  Runs after set_dt (it uses this step's dt).
+// kz = 0-plane forcing, one block (Forcing_methods/forcing.jl, cforcing.jl, nrforcing.jl,
+// cnrforcing.jl, aforcing.jl, qfforcing.jl).  Thread s < nslot sums the spectra of its slot over the
+// element list (in list order): Ef (and N for nRf / cNRf, with the final rhs formed per element as
+// the update kernel forms it); thread 0 then runs the reference's serial part: the oscillator R, the
+// factors, the horizontal power hp, the vertical factor and power vp.  Runs after set_dt.
 __device__ __forceinline__ double proj_c(double ar, double ai, double br, double bi) { return fma(ar, br, ai * bi); }
 
 template <typename T>
-__global__ void k_plane_forcing(Geo g, const C<T>* __restrict__ u, DevState* st, PlaneForcing pf, int it) {
+__global__ void k_plane_forcing(Geo g, const C<T>* __restrict__ u, const C<T>* __restrict__ rhs, DevState* st,
+                                PlaneForcing pf, UpdateArgs ua, int rhs_final, int it) {
   const size_t n = (size_t)g.sx * g.my * g.mz;
   const C<T>* u1 = u;
   const C<T>* u2 = u + n;
+  const int kind = pf.kind < 0 ? -pf.kind : pf.kind;
+  const bool with_n = kind == PF_NRF || kind == PF_CNRF;
   for (int sl = threadIdx.x; sl < pf.nslot; sl += blockDim.x) {
-    double e = 0.0;
+    double e = 0.0, nn = 0.0;
     for (int i = 0; i < pf.nel; ++i) {
       if (pf.eslot[i] != sl) continue;
-      const C<T> a = u1[pf.elem[i]], b = u2[pf.elem[i]];
+      const int idx = pf.elem[i];
+      const C<T> a = u1[idx], b = u2[idx];
       const double m2 = (double)a.x * a.x + (double)a.y * a.y + ((double)b.x * b.x + (double)b.y * b.y);
-      // calculate_u1u2_spectrum! (spectrum.jl:1-20) / calculate_u1u2_mag_spectrum! (spectrum.jl:44-65)
-      e += (pf.kind == 2 || pf.kind == -2) ? 0.5 * pf.ew[i] * m2 : pf.ew[i] * sqrt(m2);
+      // calculate_u1u2_spectrum! (spectrum.jl:1-20), calculate_abs2! (:22-37), _mag_ (:44-65)
+      e += kind == PF_QF ? pf.ew[i] * sqrt(m2) : 0.5 * pf.ew[i] * m2;
+      if (with_n) {  // calculate_nn_spectrum! (cnrforcing.jl:103-126) on the kz = 0 plane
+        double U[3][2], R[3][2], RH[2] = {0, 0};
+        for (int c = 0; c < 3; ++c) {
+          const C<T> v = u[c * n + idx], r = rhs[c * n + idx];
+          U[c][0] = v.x; U[c][1] = v.y;
+          R[c][0] = r.x; R[c][1] = r.y;
+        }
+        if (!rhs_final) {
+          const int cx = idx % g.sx, cy = idx / g.sx;
+          const double kv3[3] = {g.kx[cx], g.ky[cy], g.kz[0]};
+          const double K2 = kv3[0] * kv3[0] + kv3[1] * kv3[1] + kv3[2] * kv3[2];
+          form_rhs<T>(ua, idx, n, kv3, K2, idx == 0, U, R, RH);
+        }
+        const double k2 = pf.ek2[i];
+        nn += pf.ew[i] * (proj_c(U[0][0], U[0][1], R[0][0] - pf.nu * k2 * U[0][0], R[0][1] - pf.nu * k2 * U[0][1]) +
+                          proj_c(U[1][0], U[1][1], R[1][0] - pf.nu * k2 * U[1][0], R[1][1] - pf.nu * k2 * U[1][1]));
+      }
     }
     pf.Ef[sl] = e;
-    if (pf.kind < 0) pf.Em[sl] = e;  // initialize!(::RfForcing): the target is the initial spectrum
+    if (with_n) {
+      if (pf.kind < 0) pf.Nm1[sl] = nn;
+      else pf.Nm1[sl] = pf.N[sl];
+      pf.N[sl] = nn;
+    }
+    if (pf.kind < 0) pf.Em[sl] = e;  // initialize!: the target is the initial spectrum
   }
   __syncthreads();
   if (threadIdx.x != 0 || pf.kind < 0) return;
   const double dt = st->dt;
   double hp = 0.0, vp = 0.0, fvert = 0.0;
-  if (pf.kind == 2) {
-    double ts = dt / pf.cfl;
-    if (pf.variable_dt) {
-      if (it == 0) ts /= 0.1;
-      else if (it == 1) ts /= 0.2;
-      else if (it == 2) ts /= 0.4;
-      else if (it == 3) ts /= 0.8;
+  if (kind != PF_QF) {
+    double omega;
+    if (kind == PF_CRF) {
+      omega = 2 * 3.141592653589793 / pf.tf;  // ts = Tf
+    } else if (kind == PF_CNRF) {
+      omega = 2 * 3.141592653589793 / (pf.tf * pf.tf);  // ts = Tf, omega = 2 pi / (Tf ts)
+    } else if (kind == PF_A) {
+      omega = 2 * 3.141592653589793 / (pf.tf * (10 * dt));  // ts = 10 dt
+    } else {
+      double ts = dt / pf.cfl;
+      if (pf.variable_dt) {
+        if (it == 0) ts /= 0.1;
+        else if (it == 1) ts /= 0.2;
+        else if (it == 2) ts /= 0.4;
+        else if (it == 3) ts /= 0.8;
+      }
+      omega = 2 * 3.141592653589793 / (pf.tf * ts);
     }
-    const double omega = 2 * 3.141592653589793 / (pf.tf * ts);
+    const double dtm2 = st->dt2;  // s.timestep.x.dt2[]
     for (int sl = 0; sl < pf.nslot; ++sl) {
       double R = pf.R[sl];
-      R += dt * (-2 * pf.alpha * omega * R - omega * omega * (pf.Ef[sl] - pf.Em[sl]));
+      if (with_n)
+        R += dt * (-2 * pf.alpha * omega * (R + pf.N[sl]) - (pf.N[sl] - pf.Nm1[sl]) / dtm2 -
+                   omega * omega * (pf.Ef[sl] - pf.Em[sl]));
+      else
+        R += dt * (-2 * pf.alpha * omega * R - omega * omega * (pf.Ef[sl] - pf.Em[sl]));
       R = fmax(0.0, R);
       pf.R[sl] = R;
-      pf.factor[pf.slot_shell[sl]] = sqrt(R / fmax(pf.Ef[sl], 2.220446049250313e-16)) * pf.Zf[sl] * dt;
+      pf.slot_factor[sl] = sqrt(R / fmax(pf.Ef[sl], 2.220446049250313e-16)) * pf.Zf[sl] * dt;
     }
     double ff = 0.0, fv = 0.0;
     for (int i = 0; i < pf.nel; ++i) {
-      const double f = pf.factor[pf.slot_shell[pf.eslot[i]]];
+      const double f = pf.slot_factor[pf.eslot[i]];
+      pf.plane_factor[pf.elem[i]] = f;
       const C<T> a = u1[pf.elem[i]], b = u2[pf.elem[i]];
       const double s1r = a.x * f, s1i = a.y * f, s2r = b.x * f, s2i = b.y * f;
       ff += (s1r * s1r + s1i * s1i + (s2r * s2r + s2i * s2i)) * pf.ew[i];
@@ -1622,12 +1668,13 @@ __global__ void k_plane_forcing(Geo g, const C<T>* __restrict__ u, DevState* st,
       }
     vp = ff * 0.5 / dt + fv / dt;
   } else {
-    for (int sl = 0; sl < pf.nslot; ++sl) pf.factor[pf.slot_shell[sl]] = dt * pf.R[sl] * 0.99 / pf.Ef[sl];
+    for (int sl = 0; sl < pf.nslot; ++sl) pf.slot_factor[sl] = dt * pf.R[sl] * 0.99 / pf.Ef[sl];
     double fv = 0.0;
     for (int i = 0; i < pf.nel; ++i) {
-      const double fkn = pf.factor[pf.slot_shell[pf.eslot[i]]];
+      const double fkn = pf.slot_factor[pf.eslot[i]];
       const C<T> a = u1[pf.elem[i]], b = u2[pf.elem[i]];
       const double umag = sqrt((double)a.x * a.x + (double)a.y * a.y + ((double)b.x * b.x + (double)b.y * b.y));
+      pf.plane_factor[pf.elem[i]] = fkn / umag;  // force = (u / |u|) fkn
       const double s1r = a.x / umag * fkn, s1i = a.y / umag * fkn, s2r = b.x / umag * fkn, s2i = b.y / umag * fkn;
       fv += (proj_c(a.x, a.y, s1r / dt, s1i / dt) + proj_c(b.x, b.y, s2r / dt, s2i / dt)) * pf.ew[i];
     }
@@ -1653,8 +1700,9 @@ __global__ void k_plane_forcing(Geo g, const C<T>* __restrict__ u, DevState* st,
 }
 
 template <typename T>
-void launch_plane_forcing(const Geo& g, const void* u, DevState* st, const PlaneForcing& pf, int it, hipStream_t s) {
-  k_plane_forcing<T><<<1, 64, 0, s>>>(g, (const C<T>*)u, st, pf, it);
+void launch_plane_forcing(const Geo& g, const void* u, const void* rhs, DevState* st, const PlaneForcing& pf,
+                          const UpdateArgs& ua, int rhs_final, int it, hipStream_t s) {
+  k_plane_forcing<T><<<1, 64, 0, s>>>(g, (const C<T>*)u, (const C<T>*)rhs, st, pf, ua, rhs_final, it);
 }
 
 // one block of 64 threads: the forcing band sum (as k_forcing_sum) and, on thread 0, set_dt
@@ -1853,7 +1901,8 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   template void launch_forcing_sum<T>(const Geo&, const void*, const int*, int, DevState*, double, hipStream_t);               \
   template void launch_cast_copy<T>(const void*, void*, size_t, hipStream_t);                                       \
   template void launch_step_scalars<T>(const Geo&, const void*, DevState*, const ScalarArgs&, hipStream_t);          \
-  template void launch_plane_forcing<T>(const Geo&, const void*, DevState*, const PlaneForcing&, int, hipStream_t);   \
+  template void launch_plane_forcing<T>(const Geo&, const void*, const void*, DevState*, const PlaneForcing&,          \
+                                        const UpdateArgs&, int, int, hipStream_t);                                    \
   template void launch_seed<T>(const Geo&, void*, int, unsigned long long, double, double, double*, double*,           \
                                hipStream_t);
 

@@ -126,8 +126,11 @@ struct ffs_sim {
   bool plane = false;
   PlaneForcing pfd{};
   std::vector<double> pf_avgk;    // mean |k_h| per 2D shell
+  int pf_nsh = 0;
   std::vector<int> pf_slot_shell; // shell of each forced slot
+  std::vector<int> pf_slot_mode;  // aForcing: (NX, NY) position of each forced mode
   bool pf_em_set = false;         // Em given before initialize (targSpectrum.dat)
+  std::vector<double> pf_n_pending;  // N given before initialize (the N.N file), applied after its init
 
   // run state (host mirror of s.iteration[] and the integrator's iteration)
   int64_t iteration = 0;
@@ -376,9 +379,9 @@ struct ffs_sim {
     if (plane) setup_plane_forcing();
   }
 
-  // compute_shells2D (utils.jl:161-182), calculate_Zf (utils.jl:184-197), the forced elements of the
-  // kz = 0 plane and the vertical modes, and for QfForcing build_Qf_power_spectrum!
-  // (forcing_types.jl:304-331)
+  // compute_shells2D (utils.jl:161-182), calculate_Zf (utils.jl:184-197 and 199-218 for aForcing),
+  // the forced elements of the kz = 0 plane and the vertical modes, and for QfForcing
+  // build_Qf_power_spectrum! (forcing_types.jl:304-331)
   void setup_plane_forcing() {
     const int NX = g.nxh, NY = g.ny;
     const int nsh = std::min(NX, NY / 2);
@@ -395,36 +398,58 @@ struct ffs_sim {
       }
     for (int i = 0; i < nsh; ++i) kh[i] = kh[i] / cnt[i];  // 0/0 = NaN for an empty shell, as the reference
     pf_avgk = kh;
+    pf_nsh = nsh;
     const double kf = p.forcing_kf;
-    // forced shells: 1-based n in 2..nsh with mean |k_h| <= kf
-    std::vector<int> slot_of(nsh, -1);
-    pf_slot_shell.clear();
-    std::vector<double> zf;
-    for (int n = 1; n < nsh; ++n)
-      if (kh[n] <= kf) {
-        slot_of[n] = (int)pf_slot_shell.size();
-        pf_slot_shell.push_back(n);
-        const double k = kh[n];
-        zf.push_back(p.forcing_zf_tanh ? std::tanh((kf - k) / (0.25 * kf)) * (((kf - k) <= 0.0) ? 0.0 : 1.0)
-                                       : (k <= kf ? 1.0 : 0.0));
-      }
-    const int nslot = (int)pf_slot_shell.size();
-    if (nslot == 0) throw FfsError(FFS_ERR_ARG, "no 2D shell has mean |k_h| <= kf");
-    // forced elements of the kz = 0 plane in the reference's order (y outer, x inner)
+    auto zf_of = [&](double k) {
+      return p.forcing_zf_tanh ? std::tanh((kf - k) / (0.25 * kf)) * (((kf - k) <= 0.0) ? 0.0 : 1.0)
+                               : (k <= kf ? 1.0 : 0.0);
+    };
+    const bool per_mode = p.forcing == FFS_FORCING_A;
     std::vector<int> elem, eslot;
-    std::vector<double> ew;
-    for (int cy = 0; cy < g.my; ++cy)
-      for (int cx = 0; cx < g.mx; ++cx) {
-        const double k = std::sqrt(std::fma(kxc[cx], kxc[cx], kyc[cy] * kyc[cy]));
-        const int n = (int)std::nearbyint(k / maxdk);
-        if (n >= 1 && n < nsh && slot_of[n] >= 0) {
-          elem.push_back(cx + g.sx * cy);
-          eslot.push_back(slot_of[n]);
-          ew.push_back(cx == 0 ? 1.0 : 2.0);
+    std::vector<double> ew, ek2, zf;
+    pf_slot_shell.clear();
+    pf_slot_mode.clear();
+    if (!per_mode) {
+      // forced shells: 1-based n in 2..nsh with mean |k_h| <= kf
+      std::vector<int> slot_of(nsh, -1);
+      for (int nn = 1; nn < nsh; ++nn)
+        if (kh[nn] <= kf) {
+          slot_of[nn] = (int)pf_slot_shell.size();
+          pf_slot_shell.push_back(nn);
+          zf.push_back(zf_of(kh[nn]));
         }
-      }
+      for (int cy = 0; cy < g.my; ++cy)
+        for (int cx = 0; cx < g.mx; ++cx) {
+          const double k = std::sqrt(std::fma(kxc[cx], kxc[cx], kyc[cy] * kyc[cy]));
+          const int nn = (int)std::nearbyint(k / maxdk);
+          if (nn >= 1 && nn < nsh && slot_of[nn] >= 0) {
+            elem.push_back(cx + g.sx * cy);
+            eslot.push_back(slot_of[nn]);
+            ew.push_back(cx == 0 ? 1.0 : 2.0);
+            ek2.push_back(kxc[cx] * kxc[cx] + kyc[cy] * kyc[cy]);
+          }
+        }
+    } else {
+      // one oscillator per retained kz = 0 mode with 0 < |k_h| <= kf (aforcing.jl:25-36)
+      for (int cy = 0; cy < g.my; ++cy)
+        for (int cx = 0; cx < g.mx; ++cx) {
+          const double k = std::sqrt(std::fma(kxc[cx], kxc[cx], kyc[cy] * kyc[cy]));
+          if (0 < k && k <= kf) {
+            elem.push_back(cx + g.sx * cy);
+            eslot.push_back((int)pf_slot_mode.size());
+            pf_slot_mode.push_back(cx + NX * ymap[cy]);  // position in the reference's (NX, NY) matrices
+            ew.push_back(cx == 0 ? 1.0 : 2.0);
+            ek2.push_back(kxc[cx] * kxc[cx] + kyc[cy] * kyc[cy]);
+            // calculate_Zf(d, kf, KX, KY): tanh only when asked, else the cutoff
+            zf.push_back(p.forcing_zf_tanh ? std::tanh((kf - k) / (0.25 * kf)) * (((kf - k) <= 0.0) ? 0.0 : 1.0)
+                                           : (k <= kf ? 1.0 : 0.0));
+          }
+        }
+    }
+    const int nslot = (int)zf.size();
+    if (nslot == 0) throw FfsError(FFS_ERR_ARG, "no kz = 0 mode or 2D shell is forced (kf too small)");
     // vertical modes kx = ky = 0: (kz = +m, kz = -m) for m = m0..5
-    const int m0 = p.forcing == FFS_FORCING_RF ? 1 : 3;
+    const int m0 = p.forcing == FFS_FORCING_QF ? 3 : 1;
     std::vector<int> vert;
     for (int m = m0; m <= 5; ++m)
       for (int sgn : {1, -1}) {
@@ -432,7 +457,7 @@ struct ffs_sim {
         if (cz < 0) throw FfsError(FFS_ERR_ARG, "plane forcing needs the modes |kz| <= 5 (nz >= 16)");
         vert.push_back(g.sx * g.my * cz);
       }
-    std::vector<double> R(nslot, 0.0), Em(nslot, 0.0), Ef(nslot, 0.0), factor(nsh, 0.0);
+    std::vector<double> R(nslot, 0.0), zeros(nslot, 0.0);
     if (p.forcing == FFS_FORCING_QF) {
       double integral = 0.0;
       for (int sl = 0; sl < nslot; ++sl) {
@@ -451,27 +476,34 @@ struct ffs_sim {
         }
       }
     }
+    auto one = [](auto v) { if (v.empty()) v.resize(1); return v; };
     pfd.kind = p.forcing;
     pfd.nel = (int)elem.size();
-    pfd.elem = upload(elem.empty() ? std::vector<int>{0} : elem);
-    pfd.eslot = upload(eslot.empty() ? std::vector<int>{0} : eslot);
-    pfd.ew = upload(ew.empty() ? std::vector<double>{0.0} : ew);
+    pfd.elem = upload(one(elem));
+    pfd.eslot = upload(one(eslot));
+    pfd.ew = upload(one(ew));
+    pfd.ek2 = upload(one(ek2));
     pfd.nslot = nslot;
-    pfd.slot_shell = upload(pf_slot_shell);
     pfd.R = upload(R);
-    pfd.Em = upload(Em);
-    pfd.Ef = upload(Ef);
+    pfd.Em = upload(zeros);
+    pfd.Ef = upload(zeros);
+    pfd.N = upload(zeros);
+    pfd.Nm1 = upload(zeros);
     pfd.Zf = upload(zf);
-    pfd.factor = upload(factor);
+    pfd.slot_factor = upload(zeros);
+    pfd.plane_factor = upload(std::vector<double>((size_t)g.sx * g.my, 0.0));
     pfd.nvert = (int)vert.size();
     pfd.vert = upload(vert);
-    pfd.nsh = nsh;
-    pfd.maxdk = maxdk;
     pfd.tf = p.forcing_tf;
     pfd.alpha = p.forcing_alpha;
     pfd.cfl = p.cfl;
+    pfd.nu = p.nu;
     pfd.variable_dt = p.variable_dt;
   }
+
+  // per-slot state <-> the reference's arrays (per 2D shell; aForcing: per (NX, NY) mode, x fastest)
+  int pf_state_len() const { return p.forcing == FFS_FORCING_A ? g.nxh * g.ny : pf_nsh; }
+  int pf_state_pos(int sl) const { return p.forcing == FFS_FORCING_A ? pf_slot_mode[sl] : pf_slot_shell[sl]; }
 
   void setup_state() {
     DevState h{};
@@ -490,12 +522,10 @@ struct ffs_sim {
     if (p.integrator != FFS_AB3 && p.integrator != FFS_ETD3) throw FfsError(FFS_ERR_ARG, "unknown integrator");
     if (p.les_model != FFS_LES_NONE && p.les_model != FFS_LES_SMAGORINSKY)
       throw FfsError(FFS_ERR_ARG, "unsupported LES model (Smagorinsky is available)");
-    if (p.forcing != FFS_FORCING_NONE && p.forcing != FFS_FORCING_SHELL && p.forcing != FFS_FORCING_RF &&
-        p.forcing != FFS_FORCING_QF)
-      throw FfsError(FFS_ERR_ARG, "unsupported forcing");
-    plane = (p.forcing == FFS_FORCING_RF || p.forcing == FFS_FORCING_QF);
-    if (p.forcing == FFS_FORCING_RF && !(p.forcing_tf > 0))
-      throw FfsError(FFS_ERR_ARG, "rfForcing needs TF > 0");
+    if (p.forcing < FFS_FORCING_NONE || p.forcing > FFS_FORCING_A) throw FfsError(FFS_ERR_ARG, "unsupported forcing");
+    plane = p.forcing >= FFS_FORCING_RF && p.forcing <= FFS_FORCING_A;
+    if (plane && p.forcing != FFS_FORCING_QF && !(p.forcing_tf > 0))
+      throw FfsError(FFS_ERR_ARG, "the oscillator forcing schemes need TF > 0");
     if (p.forcing == FFS_FORCING_QF && !(p.alvelius_c != 0))
       throw FfsError(FFS_ERR_ARG, "QfForcing needs alveliusC != 0");
     if (p.precision != FFS_FP64 && p.precision != FFS_FP32) throw FfsError(FFS_ERR_ARG, "unknown precision");
@@ -651,7 +681,8 @@ struct ffs_sim {
     if (a.do_dt || a.nband) launch("step_scalars", [&] { FFS_DISPATCH(launch_step_scalars, g, u, st, a, stream); });
     if (do_force && plane) {
       const int it = (int)std::min<int64_t>(iteration, 4);  // matters only through it <= 3 (the ramp)
-      launch("plane_forcing", [&] { FFS_DISPATCH(launch_plane_forcing, g, u, st, pfd, it, stream); });
+      const UpdateArgs ua = update_args(3);
+      launch("plane_forcing", [&] { FFS_DISPATCH(launch_plane_forcing, g, u, rhs, st, pfd, ua, 0, it, stream); });
     }
   }
   void set_dt() { step_scalars(true, false); }
@@ -719,8 +750,9 @@ struct ffs_sim {
     }
     r.stats.push_back(tdiss);
     r.stats.insert(r.stats.end(), other.begin(), other.end());
-    if (plane) {  // stats(::RfForcing / ::QfForcing): hp, vp, tp (forcing_types.jl:33, 291)
-      r.stats.insert(r.stats.end(), {sd.force_hp, sd.force_vp, sd.force_hp + sd.force_vp});
+    if (plane) {  // stats(forcing): hp, vp, tp (forcing_types.jl:33, 291); aForcing hp, vp (:105)
+      r.stats.insert(r.stats.end(), {sd.force_hp, sd.force_vp});
+      if (p.forcing != FFS_FORCING_A) r.stats.push_back(sd.force_hp + sd.force_vp);
     }
     const double u1p2 = u12, u2p2 = u22, u3p2 = u32;
     const double uh = std::sqrt((u1p2 + u2p2) / 2);
@@ -768,10 +800,8 @@ struct ffs_sim {
     a.forcing = (p.forcing == FFS_FORCING_SHELL && nband > 0) ? 1 : 0;
     if (plane) {
       a.forcing = p.forcing;
-      a.pf_factor = pfd.factor;
-      a.pf_maxdk = pfd.maxdk;
-      a.pf_nsh = pfd.nsh;
-      a.pf_kz0 = p.forcing == FFS_FORCING_RF ? 1 : 3;
+      a.pf_factor = pfd.plane_factor;
+      a.pf_kz0 = p.forcing == FFS_FORCING_QF ? 3 : 1;
     }
     a.kf2 = kf2;
     a.tau = les ? tau : nullptr;
@@ -872,13 +902,22 @@ struct ffs_sim {
       HIPCHECK(hipMemcpyAsync(rfm1, rrhs, S * csz, hipMemcpyDeviceToDevice, stream));
       HIPCHECK(hipMemcpyAsync(rfm2, rrhs, S * csz, hipMemcpyDeviceToDevice, stream));
     }
-    // initialize!(::RfForcing) (forcing_types.jl:66-74): without a given target spectrum, the target
-    // is the initial field's
-    if (plane && p.forcing == FFS_FORCING_RF && !pf_em_set) {
+    // initialize! of the oscillator schemes (forcing_types.jl:66-74, 112-118, 151-164, 207-216,
+    // 251-264): without a given target spectrum the target is the initial field's, and the N variants
+    // take N = Nm1 from the initial right-hand side; with a target given (targSpectrum.dat) `init` is
+    // false: Em stays as given and N = Nm1 = 0.  An N given (the N.N file) replaces N afterwards.
+    // aForcing always takes the initial field's per-mode energies (forcing_types.jl:112-118); its
+    // target can be replaced after initialize.
+    if (plane && p.forcing != FFS_FORCING_QF && (!pf_em_set || p.forcing == FFS_FORCING_A)) {
       PlaneForcing e = pfd;
-      e.kind = -FFS_FORCING_RF;
-      launch("plane_forcing_init", [&] { FFS_DISPATCH(launch_plane_forcing, g, u, st, e, 0, stream); });
+      e.kind = -p.forcing;
+      const UpdateArgs ua = update_args(0);
+      launch("plane_forcing_init", [&] { FFS_DISPATCH(launch_plane_forcing, g, u, rhs, st, e, ua, 1, 0, stream); });
     }
+    if (plane && !pf_n_pending.empty())
+      HIPCHECK(hipMemcpyAsync(pfd.N, pf_n_pending.data(), pfd.nslot * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    pf_n_pending.clear();
     initialized = true;
   }
 
@@ -912,7 +951,7 @@ struct ffs_sim {
   size_t checkpoint_bytes() const {
     size_t n = sizeof(CkptHeader) + sizeof(DevState);
     for (void* b : ckpt_buffers()) n += ckpt_buffer_bytes(b);
-    if (plane) n += 2 * pfd.nslot * sizeof(double);  // the forcing's R and Em
+    if (plane) n += 4 * pfd.nslot * sizeof(double);  // the forcing's R, Em, N, Nm1
     return n;
   }
 
@@ -949,7 +988,7 @@ struct ffs_sim {
       off += ckpt_buffer_bytes(b);
     }
     if (plane)
-      for (double* b : {pfd.R, pfd.Em}) {
+      for (double* b : {pfd.R, pfd.Em, pfd.N, pfd.Nm1}) {
         HIPCHECK(hipMemcpyAsync(out + off, b, pfd.nslot * sizeof(double), hipMemcpyDeviceToHost, stream));
         off += pfd.nslot * sizeof(double);
       }
@@ -977,7 +1016,7 @@ struct ffs_sim {
       off += ckpt_buffer_bytes(b);
     }
     if (plane)
-      for (double* b : {pfd.R, pfd.Em}) {
+      for (double* b : {pfd.R, pfd.Em, pfd.N, pfd.Nm1}) {
         HIPCHECK(hipMemcpy(b, in + off, pfd.nslot * sizeof(double), hipMemcpyHostToDevice));
         off += pfd.nslot * sizeof(double);
       }
@@ -1369,32 +1408,37 @@ int ffs_advance(ffs_sim* s, int64_t nsteps, int64_t* done, int* event) {
 int ffs_forcing_state(ffs_sim* s, int what, int set, double* data, int cap, int* n) {
   if (!s) return FFS_ERR_ARG;
   return guard(s, [&] {
-    if (!s->plane) throw FfsError(FFS_ERR_ARG, "the run has no plane forcing (rfForcing / QfForcing)");
-    const int nsh = s->pfd.nsh, nslot = s->pfd.nslot;
-    if (n) *n = nsh;
+    if (!s->plane) throw FfsError(FFS_ERR_ARG, "the run has no plane forcing (Forcing_methods schemes)");
+    const int len = s->pf_state_len(), nslot = s->pfd.nslot;
+    if (n) *n = len;
     if (!data) return;
     if (what == FFS_FSTATE_KH) {
       if (set) throw FfsError(FFS_ERR_ARG, "the shell wavenumbers are read only");
-      for (int i = 0; i < nsh && i < cap; ++i) data[i] = s->pf_avgk[i];
+      if (s->p.forcing == FFS_FORCING_A) throw FfsError(FFS_ERR_ARG, "aForcing has no 2D shells");
+      for (int i = 0; i < len && i < cap; ++i) data[i] = s->pf_avgk[i];
       return;
     }
     double* dev = what == FFS_FSTATE_R ? s->pfd.R : what == FFS_FSTATE_EM ? s->pfd.Em
-                                       : what == FFS_FSTATE_EF ? s->pfd.Ef : nullptr;
+                : what == FFS_FSTATE_EF ? s->pfd.Ef : what == FFS_FSTATE_N ? s->pfd.N : nullptr;
     if (!dev) throw FfsError(FFS_ERR_ARG, "unknown forcing state");
     std::vector<double> slots(nslot);
     if (set) {
       if (what == FFS_FSTATE_EF) throw FfsError(FFS_ERR_ARG, "the current spectrum is read only");
-      if (cap < nsh) throw FfsError(FFS_ERR_ARG, "the state has one value per 2D shell");
-      for (int sl = 0; sl < nslot; ++sl) slots[sl] = data[s->pf_slot_shell[sl]];
+      if (cap < len) throw FfsError(FFS_ERR_ARG, "the state needs one value per shell (aForcing: per mode)");
+      for (int sl = 0; sl < nslot; ++sl) slots[sl] = data[s->pf_state_pos(sl)];
+      if (what == FFS_FSTATE_N && !s->initialized) {
+        s->pf_n_pending = slots;
+        return;
+      }
       HIPCHECK(hipStreamSynchronize(s->stream));
       HIPCHECK(hipMemcpy(dev, slots.data(), nslot * sizeof(double), hipMemcpyHostToDevice));
       if (what == FFS_FSTATE_EM) s->pf_em_set = true;
     } else {
       HIPCHECK(hipStreamSynchronize(s->stream));
       HIPCHECK(hipMemcpy(slots.data(), dev, nslot * sizeof(double), hipMemcpyDeviceToHost));
-      for (int i = 0; i < nsh && i < cap; ++i) data[i] = 0.0;
+      for (int i = 0; i < len && i < cap; ++i) data[i] = 0.0;
       for (int sl = 0; sl < nslot; ++sl)
-        if (s->pf_slot_shell[sl] < cap) data[s->pf_slot_shell[sl]] = slots[sl];
+        if (s->pf_state_pos(sl) < cap) data[s->pf_state_pos(sl)] = slots[sl];
     }
   });
 }
@@ -1452,7 +1496,7 @@ int ffs_stats_header(const ffs_sim* s, int table, char* buf, size_t cap) {
     h = "iteration,time,u1,u2,u3,u1p2,u2p2,u3p2,kh,kv,kaniso,k,dissh,dissv,diss,nlh,nlv,pressh,pressv,tnlh,tnlv,tdiss";
     if (s->dens) h += ",rho,rhop2,drhodxp2,drhodyp2,drhodzp2,rhodiss,trhodiss,bflux";
     if (s->les) h += ",prh,prv,praniso,pr";
-    if (s->plane) h += ",hp,vp,tp";
+    if (s->plane) h += s->p.forcing == FFS_FORCING_A ? ",hp,vp" : ",hp,vp,tp";
   } else if (table == FFS_TABLE_SCALES) {
     h = "iteration,time,Lk,Lt,lamg,laml,lamt,Lhh,Lvv,Lhv,Lvh,lam12,lam21,L11,L12,L13,L21,L22,L23,L31,L32,L33";
     h += s->dens ? ",Lb,Lo,Lom,Tk,Tt,Thh,Tb,Tbm" : ",Tk,Tt,Thh";

@@ -23,7 +23,8 @@ FFS_AB3, FFS_ETD3 = 0, 1
 FFS_FP64, FFS_FP32 = 0, 1
 FFS_LES_NONE, FFS_LES_SMAGORINSKY = 0, 1
 FFS_FORCING_NONE, FFS_FORCING_SHELL, FFS_FORCING_RF, FFS_FORCING_QF = 0, 1, 2, 3
-FFS_FSTATE_R, FFS_FSTATE_EM, FFS_FSTATE_KH, FFS_FSTATE_EF = 0, 1, 2, 3
+FFS_FORCING_CRF, FFS_FORCING_NRF, FFS_FORCING_CNRF, FFS_FORCING_A = 4, 5, 6, 7
+FFS_FSTATE_R, FFS_FSTATE_EM, FFS_FSTATE_KH, FFS_FSTATE_EF, FFS_FSTATE_N = 0, 1, 2, 3, 4
 FFS_FIELD_U1, FFS_FIELD_U2, FFS_FIELD_U3, FFS_FIELD_RHO = 0, 1, 2, 3
 FFS_SPACE_REAL, FFS_SPACE_FOURIER = 0, 1
 FFS_SCALAR_ITERATION, FFS_SCALAR_TIME, FFS_SCALAR_DT, FFS_SCALAR_UMAX = 0, 1, 2, 3

@@ -189,12 +189,24 @@ class Params:
         p.pr, p.density_gradient = self.pr, self.density_gradient
         p.z_acceleration, p.reference_density = self.z_acceleration, self.reference_density
         p.forcing = {None: _lib.FFS_FORCING_NONE, "shellForcing": _lib.FFS_FORCING_SHELL,
-                     "rfForcing": _lib.FFS_FORCING_RF, "QfForcing": _lib.FFS_FORCING_QF}.get(self.forcing, -1)
+                     "rfForcing": _lib.FFS_FORCING_RF, "QfForcing": _lib.FFS_FORCING_QF,
+                     "cRfForcing": _lib.FFS_FORCING_CRF, "nRfForcing": _lib.FFS_FORCING_NRF,
+                     "cNRfForcing": _lib.FFS_FORCING_CNRF, "aForcing": _lib.FFS_FORCING_A}.get(self.forcing, -1)
         p.forcing_kf, p.forcing_power = self.forcing_kf, self.forcing_power
         p.forcing_tf, p.forcing_alpha = self.forcing_tf, self.forcing_alpha
-        p.forcing_zf_tanh = int(self.forcing_zf == "tanh")
+        # calculate_Zf (utils.jl:184-218): the shell form takes the cutoff only for "cutoff", aForcing's
+        # (NX, NY) form takes tanh only for "tanh"
+        if self.forcing == "aForcing":
+            p.forcing_zf_tanh = int(self.forcing_zf == "tanh")
+        else:
+            p.forcing_zf_tanh = int(self.forcing_zf != "cutoff")
         p.alvelius_p, p.alvelius_c, p.alvelius_kf = self.alvelius_p, self.alvelius_c, self.alvelius_kf
         return p
+
+
+# the oscillator schemes of forcing_model (TF, alphac, kf, Zf)
+RF_FAMILY = ("rfForcing", "cRfForcing", "nRfForcing", "cNRfForcing", "aForcing")
+PLANE_FORCINGS = RF_FAMILY + ("QfForcing",)
 
 
 def parameters(d):
@@ -244,7 +256,7 @@ def parameters(d):
     if "forcing" in d:
         known.add("forcing")
         p.forcing = d["forcing"]
-        if p.forcing == "rfForcing":  # forcing_model, forcing_types.jl:338-359
+        if p.forcing in RF_FAMILY:  # forcing_model, forcing_types.jl:338-422
             p.forcing_tf = float(req("TF"))
             p.forcing_alpha = float(req("alphac"))
             p.forcing_kf = float(req("kf"))

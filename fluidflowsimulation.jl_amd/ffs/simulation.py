@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 from . import _lib
-from .config import Params, parameters, readglobal
+from .config import RF_FAMILY, Params, parameters, readglobal
 from .fields import padded_shape, read_field, write_field
 
 __all__ = ["Simulation", "footprint", "run_simulation"]
@@ -132,7 +132,8 @@ class Simulation:
         return self
 
     def forcing_state(self, what=_lib.FFS_FSTATE_R):
-        """Per-2D-shell state of the plane forcing (R, Em, mean |k_h|, current spectrum)."""
+        """State of the plane forcing (R, Em, N, mean |k_h|, current spectrum): one value per 2D shell;
+        for aForcing one per mode of the reference's (NX, NY) matrices, x fastest."""
         n = ctypes.c_int()
         self._check(self._lib.ffs_forcing_state(self._h, what, 0, None, 0, ctypes.byref(n)))
         a = np.zeros(n.value)
@@ -140,7 +141,8 @@ class Simulation:
         return a
 
     def set_forcing_state(self, what, values):
-        """Sets R (e.g. from an R.N file) or the target spectrum Em (before initialize)."""
+        """Sets R (an R.N file), N (an N.N file) or the target spectrum Em (targSpectrum.dat; before
+        initialize, so the target is not taken from the initial field)."""
         a = np.ascontiguousarray(values, dtype=np.float64)
         n = ctypes.c_int()
         self._check(self._lib.ffs_forcing_state(self._h, what, 1, _dptr(a), a.size, ctypes.byref(n)))
@@ -300,8 +302,7 @@ def run_simulation(directory=".", params=None, nt=None):
     nx, ny, nz = params.nx, params.ny, params.nz
     for f in range(3):
         s.set_field(f, read_field(os.path.join(directory, f"u{f + 1}.{params.start}"), nx, ny, nz))
-    if params.forcing in ("rfForcing", "QfForcing"):
-        _load_forcing_files(s, directory, params)
+    _load_forcing_files(s, directory, params)
     if params.start == 0:
         s.writeheader(directory)
     s.initialize()
@@ -327,16 +328,21 @@ def run_simulation(directory=".", params=None, nt=None):
 
 
 def _load_forcing_files(s, directory, params):
-    """targSpectrum.dat (calculate_Em!, forcing_types.jl:76-83: skip the header line, linear
-    interpolation at the shells' mean |k_h|; np.interp holds the end values outside the table) and
-    R.$start (initialize!(::RfForcing), forcing_types.jl:66-74)."""
-    if params.forcing != "rfForcing":
+    """The files the plane forcings read at initialize!: targSpectrum.dat for the shell oscillators
+    (calculate_Em!, forcing_types.jl:76-83: skip the header line, linear interpolation at the shells'
+    mean |k_h|; np.interp holds the end values outside the table), R.$start (forcing_types.jl:66-74,
+    112-118, 151-164, 207-216, 251-264; aForcing's is the (NX, NY) matrix, read column-major as vec
+    does) and N.$start for nRfForcing / cNRfForcing."""
+    if params.forcing not in RF_FAMILY:
         return
     targ = os.path.join(directory, "targSpectrum.dat")
-    if os.path.exists(targ):
+    if params.forcing != "aForcing" and os.path.exists(targ):
         data = np.loadtxt(targ, skiprows=1)
         kh = s.forcing_state(_lib.FFS_FSTATE_KH)
         s.set_forcing_state(_lib.FFS_FSTATE_EM, np.interp(np.nan_to_num(kh), data[:, 0], data[:, 1]))
     rfile = os.path.join(directory, f"R.{params.start}")
     if os.path.exists(rfile):
-        s.set_forcing_state(_lib.FFS_FSTATE_R, np.loadtxt(rfile).ravel())
+        s.set_forcing_state(_lib.FFS_FSTATE_R, np.loadtxt(rfile).ravel(order="F"))
+    nfile = os.path.join(directory, f"N.{params.start}")
+    if params.forcing in ("nRfForcing", "cNRfForcing") and os.path.exists(nfile):
+        s.set_forcing_state(_lib.FFS_FSTATE_N, np.loadtxt(nfile).ravel(order="F"))

@@ -99,6 +99,22 @@ function params_from_global(d::AbstractDict)
         p.pr, p.density_gradient = num(:Pr), num(:densityGradient)
         p.z_acceleration, p.reference_density = num(:zAcceleration), num(:referenceDensity)
     end
+    # forcing_model (forcing_types.jl:338-446): the kz = 0-plane schemes
+    codes = Dict("rfForcing" => 2, "QfForcing" => 3, "cRfForcing" => 4, "nRfForcing" => 5,
+                 "cNRfForcing" => 6, "aForcing" => 7)
+    if haskey(d, :forcing) && haskey(codes, d[:forcing])
+        f = d[:forcing]
+        p.forcing = Int32(codes[f])
+        p.forcing_kf = parse(Float64, d[:kf])
+        if f == "QfForcing"
+            p.alvelius_p, p.alvelius_c = parse(Float64, d[:alveliusP]), parse(Float64, d[:alveliusC])
+            p.alvelius_kf = parse(Float64, d[:alveliusKf])
+        else
+            p.forcing_tf, p.forcing_alpha = parse(Float64, d[:TF]), parse(Float64, d[:alphac])
+            zf = get(d, :Zf, "cutoff")   # calculate_Zf, utils.jl:184-218
+            p.forcing_zf_tanh = Int32(f == "aForcing" ? zf == "tanh" : zf != "cutoff")
+        end
+    end
     return p
 end
 
@@ -153,7 +169,8 @@ function advance_in_time!(s::FFSSimulation)
     return nothing
 end
 
-# plane forcing state per 2D shell (R, Em, mean |k_h|, current spectrum): what = 0 R, 1 Em, 2 KH, 3 Ef
+# plane forcing state per 2D shell (aForcing: per mode of its (NX, NY) matrices, x fastest):
+# what = 0 R, 1 Em, 2 KH (mean |k_h|), 3 Ef (current spectrum), 4 N
 function forcing_state(s::FFSSimulation, what::Integer)
     n = Ref{Cint}(0)
     check(ccall((:ffs_forcing_state, libffs), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Cint, Ref{Cint}),

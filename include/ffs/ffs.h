@@ -57,8 +57,12 @@ extern "C" {
  * horizontal velocity on the kz = 0 plane driven toward a target spectrum Em by a damped
  * oscillator R per shell (rfForcing, forcing.jl:6-114), or with a fixed power spectrum
  * (QfForcing, qfforcing.jl), plus 1 % of the power on kx = ky = 0, |kz| <= 5 */
-#define FFS_FORCING_RF 2
-#define FFS_FORCING_QF 3
+#define FFS_FORCING_RF 2   /* rfForcing   (forcing.jl) */
+#define FFS_FORCING_QF 3   /* QfForcing   (qfforcing.jl) */
+#define FFS_FORCING_CRF 4  /* cRfForcing  (cforcing.jl: ts = TF) */
+#define FFS_FORCING_NRF 5  /* nRfForcing  (nrforcing.jl: with the nonlinear transfer spectrum N) */
+#define FFS_FORCING_CNRF 6 /* cNRfForcing (cnrforcing.jl) */
+#define FFS_FORCING_A 7    /* aForcing    (aforcing.jl: one oscillator per kz = 0 mode, |k_h| <= kf) */
 
 /* fields */
 #define FFS_FIELD_U1 0
@@ -170,11 +174,14 @@ int ffs_checkpoint_load(ffs_sim* s, const void* host, size_t n);
  * spectrum (taken from the initial field at ffs_initialize unless set before, as from
  * targSpectrum.dat, forcing_types.jl:76-83), FFS_FSTATE_KH the shells' mean |k_h| (read only),
  * FFS_FSTATE_EF the last computed horizontal spectrum of the forced shells (read only).
- * set = 1 writes data[0..n), set = 0 reads; *n receives the number of shells. */
+ * aForcing keeps one value per kz = 0 mode instead: an (nx/2+1) x ny array, x fastest (the
+ * reference's R matrix), and has no FFS_FSTATE_KH.
+ * set = 1 writes data[0..n), set = 0 reads; *n receives the number of values. */
 #define FFS_FSTATE_R 0
 #define FFS_FSTATE_EM 1
 #define FFS_FSTATE_KH 2
 #define FFS_FSTATE_EF 3
+#define FFS_FSTATE_N 4 /* nRf / cNRf: the transfer spectrum N (the `N.N` file, forcing_types.jl:142-144) */
 int ffs_forcing_state(ffs_sim* s, int what, int set, double* data, int cap, int* n);
 
 /* Replaces the tail of run_simulation (FluidFlowSimulation.jl:46-49):

@@ -181,7 +181,7 @@ class OracleSim:
         self.forcing_kf = forcing_kf
         self.forcing_power = forcing_power
         self.force_factor = 0.0
-        self.plane = plane_forcing  # oracle.plane_forcing.PlaneForcing (rfForcing / QfForcing) or None
+        self.plane = plane_forcing  # oracle.plane_forcing.PlaneForcing (one of the kz = 0 plane forcings) or None
         self.umax = 0.0
         self.records = []
         self.spectra = []
@@ -560,8 +560,10 @@ class OracleSim:
             Lk = np.sqrt(np.sqrt(nu**3 / tdiss))
             Tk = np.sqrt(nu / tdiss)
             lg = np.sqrt(15 * nu / tdiss) * urms
-        if self.plane is not None:  # stats(::RfForcing / ::QfForcing): hp, vp, tp
-            other += [self.plane.hp, self.plane.vp, self.plane.hp + self.plane.vp]
+        if self.plane is not None:  # stats(::RfForcing ...): hp, vp, tp; stats(::AForcing): hp, vp
+            other += [self.plane.hp, self.plane.vp]
+            if self.plane.kind != "a":
+                other.append(self.plane.hp + self.plane.vp)
         stats_row = [self.iteration, self.time, *vstats, tdiss, *other]
         scales = [self.iteration, self.time, Lk, Lt, lg, ll, lt, Lhh, Lvv, Lhv, Lvh, l12, l21, L11, L12, L13, L21,
                   L22, L23, L31, L32, L33]

@@ -5,6 +5,11 @@
   calculate_u1u2_spectrum!      Forcing_methods/spectrum.jl:1-20
   calculate_u1u2_mag_spectrum!  Forcing_methods/spectrum.jl:44-65
   RfForcing functor       Forcing_methods/forcing.jl:6-114
+  CRfForcing functor      Forcing_methods/cforcing.jl:1-97
+  NRfForcing functor      Forcing_methods/nrforcing.jl:1-117, calculate_nn_spectrum! cnrforcing.jl:103-126
+  CNRfForcing functor     Forcing_methods/cnrforcing.jl:1-101
+  AForcing functor        Forcing_methods/aforcing.jl:1-87, calculate_abs2! spectrum.jl:22-37,
+                          calculate_Zf(d,kf,kx,ky) utils.jl:199-218
   initialize!(::RfForcing)      Forcing_methods/forcing_types.jl:66-74
   build_Qf_power_spectrum!      Forcing_methods/forcing_types.jl:304-331
   QfForcing functor       Forcing_methods/qfforcing.jl:1-81
@@ -56,7 +61,8 @@ def round_half_even(x):
 
 
 class PlaneForcing:
-    """kind "rf" (rfForcing: TF, alphac, kf, Zf) or "qf" (QfForcing: alveliusP, alveliusC, alveliusKf, kf)."""
+    """kind "rf", "crf", "nrf", "cnrf", "a" (rfForcing, cRfForcing, nRfForcing, cNRfForcing, aForcing:
+    TF, alphac, kf, Zf) or "qf" (QfForcing: alveliusP, alveliusC, alveliusKf, kf)."""
 
     def __init__(self, grid, kind, kf, tf=0.0, alphac=0.0, zf="cutoff", P=0.0, C=1.0, Kf_alv=0.0, les_delta=None):
         self.g = grid
@@ -69,7 +75,14 @@ class PlaneForcing:
         self.Ef = np.zeros(n)
         self.Em = np.zeros(n)
         self.R = np.zeros(n)
+        self.N = np.zeros(n)
+        self.Nm1 = np.zeros(n)
         self.factor = np.zeros(n)
+        if kind == "a":  # one oscillator per (i, j) of the kz = 0 plane
+            shape = (grid.NX, grid.NY)
+            self.Em = np.zeros(shape)
+            self.R = np.zeros(shape)
+            self.factor = np.zeros(shape)
         self.init = True
         self.hp = 0.0
         self.vp = 0.0
@@ -79,6 +92,15 @@ class PlaneForcing:
                 self.Zf[i] = float(k <= kf)
             else:
                 self.Zf[i] = math.tanh((kf - k) / (0.25 * kf)) * (0.0 if (kf - k) <= 0.0 else 1.0)
+        if kind == "a":
+            self.Zfa = np.zeros((grid.NX, grid.NY))
+            for j in range(grid.NY):
+                for i in range(grid.NX):
+                    k = math.sqrt(grid.KX[i] ** 2 + grid.KY[j] ** 2)
+                    if zf == "tanh":
+                        self.Zfa[i, j] = math.tanh((kf - k) / (0.25 * kf)) * (0.0 if (kf - k) <= 0.0 else 1.0)
+                    else:
+                        self.Zfa[i, j] = float(k <= kf)
         if kind == "qf":
             self.build_qf(P, C, Kf_alv, les_delta)
         self.forcex = np.zeros((grid.NX, grid.NY, grid.NZ), dtype=complex)
@@ -131,16 +153,53 @@ class PlaneForcing:
                 cf = 2.0
         return Ef
 
+    def nn_spectrum(self, u, rhs, nu):
+        """calculate_nn_spectrum! (cnrforcing.jl:103-126) on the kz = 0 plane."""
+        g = self.g
+        N = np.zeros(self.nshells)
+        maxdk2d = max(g.KX[1], g.KY[1])
+        for j in g.YRANGE:
+            cf = 1.0
+            for i in g.XRANGE:
+                kxy2 = g.KX[i] ** 2 + g.KY[j] ** 2
+                k2 = kxy2 + g.KZ[0] ** 2
+                k = math.sqrt(kxy2)
+                n = int(round_half_even(k / maxdk2d))
+                if n < self.nshells:
+                    magsq = (proj(u[0][i, j, 0], rhs[0][i, j, 0] - nu * k2 * u[0][i, j, 0])
+                             + proj(u[1][i, j, 0], rhs[1][i, j, 0] - nu * k2 * u[1][i, j, 0]))
+                    N[n] += cf * magsq
+                cf = 2.0
+        return N
+
+    def abs2_plane(self, u):
+        """calculate_abs2! (spectrum.jl:22-37)."""
+        g = self.g
+        E = np.zeros((g.NX, g.NY))
+        for j in g.YRANGE:
+            cf = 1.0
+            for i in g.XRANGE:
+                E[i, j] = 0.5 * cf * (abs(u[0][i, j, 0]) ** 2 + abs(u[1][i, j, 0]) ** 2)
+                cf = 2.0
+        return E
+
     def initialize(self, sim):
-        if self.kind == "rf" and self.init:
+        if self.kind == "a":
+            self.Em = self.abs2_plane(sim.uh)
+        elif self.kind != "qf" and self.init:
             self.Em = self.u1u2_spectrum(sim.uh)
+            if self.kind in ("nrf", "cnrf"):
+                self.N = self.nn_spectrum(sim.uh, sim.rhs, sim.nu)
+                self.Nm1 = self.N.copy()
 
     # ---------------------------------------------------------------- the functors
     def __call__(self, sim):
-        if self.kind == "rf":
-            self.rf(sim)
-        else:
+        if self.kind == "qf":
             self.qf(sim)
+        elif self.kind == "a":
+            self.a(sim)
+        else:
+            self.rf(sim)
 
     def _elements(self):
         """(i, j, n0) of the forced kz = 0 elements in the reference's loop order."""
@@ -157,20 +216,24 @@ class PlaneForcing:
                 first = False
 
     def rf(self, sim):
+        """rfForcing, cRfForcing, nRfForcing, cNRfForcing (they differ in ts and the N terms)."""
         g = self.g
         dt = sim.dt
-        ts = dt / sim.cfl
-        it = sim.iteration
-        if sim.variable_dt:
-            if it == 0:
-                ts /= 0.1
-            elif it == 1:
-                ts /= 0.2
-            elif it == 2:
-                ts /= 0.4
-            elif it == 3:
-                ts /= 0.8
-        omega = 2 * math.pi / (self.tf * ts)
+        if self.kind in ("crf", "cnrf"):
+            ts = self.tf
+        else:
+            ts = dt / sim.cfl
+            it = sim.iteration
+            if sim.variable_dt:
+                if it == 0:
+                    ts /= 0.1
+                elif it == 1:
+                    ts /= 0.2
+                elif it == 2:
+                    ts /= 0.4
+                elif it == 3:
+                    ts /= 0.8
+        omega = 2 * math.pi / ts if self.kind == "crf" else 2 * math.pi / (self.tf * ts)
         eps = np.finfo(np.float64).eps
         s1 = self.forcex
         s2 = self.forcey
@@ -178,9 +241,19 @@ class PlaneForcing:
         s2[:] = 0
         u1, u2 = sim.uh[0], sim.uh[1]
         self.Ef = self.u1u2_spectrum(sim.uh)
+        with_n = self.kind in ("nrf", "cnrf")
+        if with_n:
+            self.Nm1 = self.N.copy()
+            self.N = self.nn_spectrum(sim.uh, sim.rhs, sim.nu)
+            dtm2 = sim.dt2
         for i in range(1, self.nshells):
             if self.kh[i] <= self.kf:
-                self.R[i] += dt * (-2 * self.alpha * omega * self.R[i] - omega * omega * (self.Ef[i] - self.Em[i]))
+                if with_n:
+                    self.R[i] += dt * (-2 * self.alpha * omega * (self.R[i] + self.N[i])
+                                       - (self.N[i] - self.Nm1[i]) / dtm2 - omega * omega * (self.Ef[i] - self.Em[i]))
+                else:
+                    self.R[i] += dt * (-2 * self.alpha * omega * self.R[i]
+                                       - omega * omega * (self.Ef[i] - self.Em[i]))
                 self.R[i] = max(0.0, self.R[i])
                 self.factor[i] = math.sqrt(self.R[i] / max(self.Ef[i], eps)) * self.Zf[i] * dt
         ff = 0.0
@@ -196,6 +269,49 @@ class PlaneForcing:
         fv = fv / dt
         hp = ff + fv
         self.hp = hp
+        self._vertical_rf(sim, hp)
+
+    def a(self, sim):
+        """aForcing: one oscillator per kz = 0 mode with 0 < |k_h| <= kf, ts = 10 dt."""
+        g = self.g
+        dt = sim.dt
+        omega = 2 * math.pi / (self.tf * (10 * dt))
+        eps = np.finfo(np.float64).eps
+        s1 = self.forcex
+        s2 = self.forcey
+        s1[:] = 0
+        s2[:] = 0
+        u1, u2 = sim.uh[0], sim.uh[1]
+        for j in g.YRANGE:
+            for i in g.XRANGE:
+                k = math.sqrt(fma(g.KX[i], g.KX[i], g.KY[j] ** 2))
+                if 0 < k <= self.kf:
+                    Ef = 0.5 * (1 + (i > 0)) * (abs(u1[i, j, 0]) ** 2 + abs(u2[i, j, 0]) ** 2)
+                    self.R[i, j] += dt * (-2 * self.alpha * omega * self.R[i, j] - omega * omega * (Ef - self.Em[i, j]))
+                    self.R[i, j] = max(0.0, self.R[i, j])
+                    self.factor[i, j] = math.sqrt(self.R[i, j] / max(Ef, eps)) * self.Zfa[i, j] * dt
+        ff = 0.0
+        fv = 0.0
+        for j in g.YRANGE:
+            cf = 1.0
+            for i in g.XRANGE:
+                k = math.sqrt(fma(g.KX[i], g.KX[i], g.KY[j] ** 2))
+                if 0 < k <= self.kf:
+                    s1[i, j, 0] = u1[i, j, 0] * self.factor[i, j]
+                    s2[i, j, 0] = u2[i, j, 0] * self.factor[i, j]
+                    ff += (abs(s1[i, j, 0]) ** 2 + abs(s2[i, j, 0]) ** 2) * cf
+                    fv += (proj(u1[i, j, 0], s1[i, j, 0]) + proj(u2[i, j, 0], s2[i, j, 0])) * cf
+                cf = 2.0
+        hp = ff * 0.5 / dt + fv / dt
+        self.hp = hp
+        self._vertical_rf(sim, hp)
+
+    def _vertical_rf(self, sim, hp):
+        g = self.g
+        dt = sim.dt
+        s1 = self.forcex
+        s2 = self.forcey
+        u1, u2 = sim.uh[0], sim.uh[1]
         NZ = g.NZ
         umag = 0.0
         for k in range(1, 6):  # 1-based 2:6 -> kz = 1..5 and -1..-5

@@ -113,3 +113,11 @@ def test_plane_forcing_keys():
     q = config.parameters(dict(base, forcing="QfForcing", alveliusP="0.1", alveliusC="1.0", alveliusKf="2", kf="3"))
     assert (q.alvelius_p, q.alvelius_c, q.alvelius_kf, q.forcing_kf) == (0.1, 1.0, 2.0, 3.0)
     assert q.to_c().forcing == 3
+    codes = {"cRfForcing": 4, "nRfForcing": 5, "cNRfForcing": 6, "aForcing": 7}
+    for name, code in codes.items():
+        r = config.parameters(dict(base, forcing=name, TF="0.5", alphac="0.25", kf="3", Zf="other"))
+        c = r.to_c()
+        assert (c.forcing, c.forcing_tf, c.forcing_alpha, c.forcing_kf) == (code, 0.5, 0.25, 3.0)
+        # calculate_Zf: an unknown Zf is tanh for the shell form, the cutoff for aForcing's matrix form
+        assert c.forcing_zf_tanh == (0 if name == "aForcing" else 1)
+        assert config.parameters(dict(base, forcing=name, TF="1", alphac="1", kf="3")).to_c().forcing_zf_tanh == 0
