@@ -178,6 +178,30 @@ int launch_stats(const Geo& g, const void* u, const void* nl, const void* ke, co
 template <typename T>
 void launch_spectrum(const Geo& g, const void* u, double* out_x, double* out_y0, double* out_y, double* out_z0,
                      double* out_z, double* partials, hipStream_t s);
+// the spectra of write_spec and write_spec_forcing (spec.jl:1-168) at once: per mode, the kSpecQ
+// quantities below, reduced to the marginals calculate_specHV bins (spec.jl:424-450)
+enum : int {
+  SQ_U_H, SQ_U_V, SQ_VIS_H, SQ_VIS_V, SQ_NL_H, SQ_NL_V, SQ_PRESS_H, SQ_PRESS_V, SQ_LES_H, SQ_LES_V,
+  SQ_BUOY, SQ_RHO, SQ_RHODISS, SQ_RHONL, SQ_FORCING, kSpecQ
+};
+struct SpecArgs {
+  const void* nl;     // the nonlinear term F(u x w) (rhs before fourierspacep2!)
+  const void* ke;     // F(|u|^2 / 2)
+  const void* tau;    // Smagorinsky stress (5 components) or null
+  const void* rho;    // density or null
+  const void* flux;   // F(-rho u) (3 components) or null
+  const double* pf_factor;  // plane forcing: per kz = 0 element factor, or null (no forcing_h)
+  int pf_kz0;               // plane forcing: the vertical modes are pf_kz0 <= |kz| <= 5
+  double nu, alpha, gz;
+  const DevState* st;
+};
+// per quantity q: X[mx] | Y0[my] | Y[my] | Z0[mz] | Z[mz]  (spectra_out_len doubles each)
+__host__ __device__ inline size_t spectra_out_len(const Geo& g) { return (size_t)g.mx + 2 * (size_t)g.my + 2 * (size_t)g.mz; }
+inline size_t spectra_partials_len(const Geo& g) {
+  return (size_t)kSpecQ * g.mz * ((size_t)g.mx + 2 * (size_t)g.my + 2);
+}
+template <typename T>
+void launch_spectra(const Geo& g, const void* u, const SpecArgs& a, double* partials, double* out, hipStream_t s);
 template <typename T>
 void launch_forcing_sum(const Geo& g, const void* u, const int* band, int nband, DevState* st, double power,
                         hipStream_t s);

@@ -1440,6 +1440,225 @@ void launch_spectrum(const Geo& g, const void* u, double* out_x, double* out_y0,
 }
 
 // ===========================================================================
+// The spectra of write_spec / write_spec_forcing (spec.jl:1-168), all kSpecQ
+// per-mode quantities of one mode at once (proj(a, b) = muladd(a.re, b.re,
+// a.im b.im), forcing.jl:4):
+//   u      spectrum_u (spec.jl:316-327): h = (|ux|^2+|uy|^2)/2, v = |uz|^2/2
+//   vis    spectrum_viscosity (:340-354): 2 nu k^2 times the u quantities
+//   nl     spectrum_non_linear (:210-222): proj(nl - i k ke, u), h = x + y, v = z
+//   press  spectrum_pressure (:370-422): proj(u, p1 k), p1 = -(k . r)/k^2 with
+//          r = nl - i ke k (+ (i k).tau) (+ rho g), 0 at k = 0
+//   les    spectrum_les (:224-235): proj((i k).tau, u)
+//   buoy   spectrum_buoyancy (:249-257): proj(uz, g_z rho)
+//   rho    spectrum_rho (:329-338): |rho|^2;  rhodiss (:356-368): alpha k^2 |rho|^2
+//   rhonl  spectrum_rhonl (:237-247): proj((i k).flux, rho)
+//   forcing spectrum_forcing (:259-268): (proj(ux, fx) + proj(uy, fy)) / dt with
+//          f = factor u on the forced modes
+// The nonlinear term is zero at k = 0 (rhs.c[1] = 0, models.jl:174).
+__device__ __forceinline__ double sproj(double ar, double ai, double br, double bi) { return fma(ar, br, ai * bi); }
+
+template <typename T>
+__device__ __forceinline__ void spec_values(const Geo& g, const C<T>* __restrict__ u, const SpecArgs& a, int cx,
+                                            int cy, int cz, double v[kSpecQ]) {
+  const size_t n = (size_t)g.sx * g.my * g.mz;
+  const size_t idx = cx + (size_t)g.sx * (cy + (size_t)g.my * cz);
+  const double K[3] = {g.kx[cx], g.ky[cy], g.kz[cz]};
+  const bool origin = idx == 0;
+  double U[3][2], N[3][2];
+  const C<T>* nl = (const C<T>*)a.nl;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const C<T> x = u[c * n + idx];
+    U[c][0] = x.x;
+    U[c][1] = x.y;
+    const C<T> y = nl[c * n + idx];
+    N[c][0] = origin ? 0.0 : (double)y.x;
+    N[c][1] = origin ? 0.0 : (double)y.y;
+  }
+  const C<T> ke = ((const C<T>*)a.ke)[idx];
+  const double k2 = fma(K[0], K[0], fma(K[1], K[1], K[2] * K[2]));
+  const double K2 = K[0] * K[0] + K[1] * K[1] + K[2] * K[2];
+  const double uh = (sproj(U[0][0], U[0][1], U[0][0], U[0][1]) + sproj(U[1][0], U[1][1], U[1][0], U[1][1])) / 2;
+  const double uv = sproj(U[2][0], U[2][1], U[2][0], U[2][1]) / 2;
+  v[SQ_U_H] = uh;
+  v[SQ_U_V] = uv;
+  v[SQ_VIS_H] = 2 * a.nu * k2 * uh;
+  v[SQ_VIS_V] = 2 * a.nu * k2 * uv;
+  // r = nl - (i ke) k
+  double R[3][2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    R[c][0] = N[c][0] + (double)ke.y * K[c];
+    R[c][1] = N[c][1] - (double)ke.x * K[c];
+  }
+  v[SQ_NL_H] = sproj(R[0][0], R[0][1], U[0][0], U[0][1]) + sproj(R[1][0], R[1][1], U[1][0], U[1][1]);
+  v[SQ_NL_V] = sproj(R[2][0], R[2][1], U[2][0], U[2][1]);
+  double D[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  if (a.tau) {  // (i k) . tau, tau symmetric traceless (xx xy xz yy yz)
+    const C<T>* t = (const C<T>*)a.tau;
+    const C<T> txx = t[idx], txy = t[n + idx], txz = t[2 * n + idx], tyy = t[3 * n + idx], tyz = t[4 * n + idx];
+    const double T6[6][2] = {{txx.x, txx.y}, {txy.x, txy.y}, {txz.x, txz.y},
+                             {tyy.x, tyy.y}, {tyz.x, tyz.y}, {-(double)txx.x - tyy.x, -(double)txx.y - tyy.y}};
+    const int row[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double sr = K[0] * T6[row[c][0]][0] + K[1] * T6[row[c][1]][0] + K[2] * T6[row[c][2]][0];
+      const double si = K[0] * T6[row[c][0]][1] + K[1] * T6[row[c][1]][1] + K[2] * T6[row[c][2]][1];
+      D[c][0] = -si;
+      D[c][1] = sr;
+      R[c][0] += D[c][0];
+      R[c][1] += D[c][1];
+    }
+  }
+  v[SQ_LES_H] = sproj(D[0][0], D[0][1], U[0][0], U[0][1]) + sproj(D[1][0], D[1][1], U[1][0], U[1][1]);
+  v[SQ_LES_V] = sproj(D[2][0], D[2][1], U[2][0], U[2][1]);
+  double rr = 0, ri = 0;
+  if (a.rho) {
+    const C<T> r0 = ((const C<T>*)a.rho)[idx];
+    rr = r0.x;
+    ri = r0.y;
+    R[2][0] += rr * a.gz;
+    R[2][1] += ri * a.gz;
+  }
+  double P[2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) P[p] = origin ? 0.0 : -(K[0] * R[0][p] + K[1] * R[1][p] + K[2] * R[2][p]) / K2;
+  // pressure = p1 k: proj(u_c, p1 k_c)
+  v[SQ_PRESS_H] = sproj(U[0][0], U[0][1], P[0] * K[0], P[1] * K[0]) + sproj(U[1][0], U[1][1], P[0] * K[1], P[1] * K[1]);
+  v[SQ_PRESS_V] = sproj(U[2][0], U[2][1], P[0] * K[2], P[1] * K[2]);
+  v[SQ_BUOY] = sproj(U[2][0], U[2][1], a.gz * rr, a.gz * ri);
+  const double rho2 = sproj(rr, ri, rr, ri);
+  v[SQ_RHO] = rho2;
+  v[SQ_RHODISS] = a.alpha * k2 * rho2;
+  double nr = 0, ni = 0;
+  if (a.flux) {  // (i k) . flux
+    const C<T>* f = (const C<T>*)a.flux;
+    const C<T> fx = f[idx], fy = f[n + idx], fz = f[2 * n + idx];
+    const double sr = K[0] * fx.x + K[1] * fy.x + K[2] * fz.x;
+    const double si = K[0] * fx.y + K[1] * fy.y + K[2] * fz.y;
+    nr = -si;
+    ni = sr;
+  }
+  v[SQ_RHONL] = sproj(nr, ni, rr, ri);
+  double f = 0.0;
+  if (a.pf_factor) {
+    if (cz == 0) {
+      f = a.pf_factor[cx + (size_t)g.sx * cy];
+    } else if (cx == 0 && cy == 0) {
+      const int j = z_fft(g, cz), kz = j < (g.nz + 1) / 2 ? j : j - g.nz;
+      const int akz = kz < 0 ? -kz : kz;
+      if (akz >= a.pf_kz0 && akz <= 5) f = a.st->force_vert;
+    }
+  }
+  v[SQ_FORCING] = f == 0.0 ? 0.0
+                           : (sproj(U[0][0], U[0][1], U[0][0] * f, U[0][1] * f) +
+                              sproj(U[1][0], U[1][1], U[1][0] * f, U[1][1] * f)) / a.st->dt;
+}
+
+// one block per z plane: px[q][cz][cx] = sum_cy, py[q][cz][cy] = sum_{cx>0}, py0[q][cz][cy] = value at cx = 0,
+// pz[q][cz][0|1] = sums over the plane at cx = 0 | cx > 0; every sum in a fixed order
+template <typename T>
+__global__ void __launch_bounds__(256) k_spectra_planes(Geo g, const C<T>* __restrict__ u, SpecArgs a,
+                                                       double* __restrict__ part) {
+  const int cz = blockIdx.x;
+  const size_t per = (size_t)g.mz;
+  double* px = part;
+  double* py = px + kSpecQ * per * g.mx;
+  double* py0 = py + kSpecQ * per * g.my;
+  double* pz = py0 + kSpecQ * per * g.my;
+  double v[kSpecQ];
+  for (int cx = threadIdx.x; cx < g.mx; cx += blockDim.x) {
+    double acc[kSpecQ];
+#pragma unroll
+    for (int q = 0; q < kSpecQ; ++q) acc[q] = 0;
+    for (int cy = 0; cy < g.my; ++cy) {
+      spec_values<T>(g, u, a, cx, cy, cz, v);
+#pragma unroll
+      for (int q = 0; q < kSpecQ; ++q) acc[q] += v[q];
+    }
+#pragma unroll
+    for (int q = 0; q < kSpecQ; ++q) px[((size_t)q * per + cz) * g.mx + cx] = acc[q];
+  }
+  double z0[kSpecQ], z1[kSpecQ];
+#pragma unroll
+  for (int q = 0; q < kSpecQ; ++q) z0[q] = z1[q] = 0;
+  for (int cy = threadIdx.x; cy < g.my; cy += blockDim.x) {
+    double acc[kSpecQ];
+#pragma unroll
+    for (int q = 0; q < kSpecQ; ++q) acc[q] = 0;
+    for (int cx = 0; cx < g.mx; ++cx) {
+      spec_values<T>(g, u, a, cx, cy, cz, v);
+      if (cx == 0) {
+#pragma unroll
+        for (int q = 0; q < kSpecQ; ++q) {
+          py0[((size_t)q * per + cz) * g.my + cy] = v[q];
+          z0[q] += v[q];
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < kSpecQ; ++q) acc[q] += v[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kSpecQ; ++q) {
+      py[((size_t)q * per + cz) * g.my + cy] = acc[q];
+      z1[q] += acc[q];
+    }
+  }
+  __shared__ double sh[256];
+  for (int q = 0; q < 2 * kSpecQ; ++q) {
+    double mine = 0;
+#pragma unroll
+    for (int r = 0; r < kSpecQ; ++r)
+      if (r == q % kSpecQ) mine = q < kSpecQ ? z0[r] : z1[r];
+    sh[threadIdx.x] = mine;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) pz[((size_t)(q % kSpecQ) * per + cz) * 2 + (q < kSpecQ ? 0 : 1)] = sh[0];
+    __syncthreads();
+  }
+}
+
+// out[q] = X[mx] | Y0[my] | Y[my] | Z0[mz] | Z[mz], the plane partials summed over cz in order
+__global__ void k_spectra_finish(Geo g, const double* __restrict__ part, double* __restrict__ out) {
+  const size_t per = (size_t)g.mz;
+  const double* px = part;
+  const double* py = px + kSpecQ * per * g.mx;
+  const double* py0 = py + kSpecQ * per * g.my;
+  const double* pz = py0 + kSpecQ * per * g.my;
+  const size_t L = spectra_out_len(g);
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= kSpecQ * L) return;
+  const int q = (int)(t / L);
+  const size_t r = t % L;
+  double acc = 0;
+  if (r < (size_t)g.mx) {
+    for (int cz = 0; cz < g.mz; ++cz) acc += px[((size_t)q * per + cz) * g.mx + r];
+  } else if (r < (size_t)g.mx + g.my) {
+    const size_t cy = r - g.mx;
+    for (int cz = 0; cz < g.mz; ++cz) acc += py0[((size_t)q * per + cz) * g.my + cy];
+  } else if (r < (size_t)g.mx + 2 * g.my) {
+    const size_t cy = r - g.mx - g.my;
+    for (int cz = 0; cz < g.mz; ++cz) acc += py[((size_t)q * per + cz) * g.my + cy];
+  } else if (r < (size_t)g.mx + 2 * g.my + g.mz) {
+    acc = pz[((size_t)q * per + (r - g.mx - 2 * g.my)) * 2 + 0];
+  } else {
+    acc = pz[((size_t)q * per + (r - g.mx - 2 * g.my - g.mz)) * 2 + 1];
+  }
+  out[t] = acc;
+}
+
+template <typename T>
+void launch_spectra(const Geo& g, const void* u, const SpecArgs& a, double* partials, double* out, hipStream_t s) {
+  k_spectra_planes<T><<<g.mz, 256, 0, s>>>(g, (const C<T>*)u, a, partials);
+  const size_t nt = kSpecQ * spectra_out_len(g);
+  k_spectra_finish<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(g, partials, out);
+}
+
+// ===========================================================================
 // isotropic constant-power forcing: factor = P / sum_{0<|k|^2<=kf2} w |u|^2 over
 // the (few) band modes listed by the host, summed in list order by one block.
 template <typename T>
@@ -1898,6 +2117,7 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
                                hipStream_t);                                                                       \
   template void launch_spectrum<T>(const Geo&, const void*, double*, double*, double*, double*, double*, double*,     \
                                    hipStream_t);                                                                      \
+  template void launch_spectra<T>(const Geo&, const void*, const SpecArgs&, double*, double*, hipStream_t);       \
   template void launch_forcing_sum<T>(const Geo&, const void*, const int*, int, DevState*, double, hipStream_t);               \
   template void launch_cast_copy<T>(const void*, void*, size_t, hipStream_t);                                       \
   template void launch_step_scalars<T>(const Geo&, const void*, DevState*, const ScalarArgs&, hipStream_t);          \

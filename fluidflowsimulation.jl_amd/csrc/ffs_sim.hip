@@ -145,6 +145,20 @@ struct ffs_sim {
   std::deque<Record> ready;
   int nstats = 0;
 
+  // spectra (write_spec / write_spec_forcing at the spectra iterations)
+  struct SpecSlot {
+    double* host = nullptr;  // pinned: kSpecQ marginal sets
+    hipEvent_t ev = nullptr;
+    int64_t iteration = 0;
+    bool busy = false;
+  };
+  double* spec_part = nullptr;  // device: plane partials (the transform scratch when it is large enough)
+  double* spec_out = nullptr;   // device: kSpecQ x spectra_out_len marginals
+  std::vector<SpecSlot> spec_slots;
+  std::deque<int> spec_pending;
+  std::deque<std::pair<int64_t, std::vector<double>>> spec_ready;  // iteration, kSpecQ x (H[nh], V[nv])
+  std::vector<double> spec_kh, spec_krz;
+
   // steady-state graphs: one per history-rotation phase
   bool graph_on = true;
   int phase = 0;  // history rotations mod 3
@@ -349,6 +363,10 @@ struct ffs_sim {
     const size_t a2 = nmax * g.sx * g.ny * g.nz, b1 = nmax * g.sx * g.my * g.nz;
     w1 = dalloc(std::max(a1, b2) * csz);
     w2 = dalloc(std::max(a2, b1) * csz);
+    // the spectra partials live in w1 (free between the nonlinear term and the next step) when it fits
+    const size_t spec_bytes = spectra_partials_len(g) * sizeof(double);
+    spec_part = std::max(a1, b2) * csz >= spec_bytes ? (double*)w1 : (double*)dalloc(spec_bytes);
+    spec_out = (double*)dalloc((size_t)kSpecQ * spectra_out_len(g) * sizeof(double));
     if (fp32) {
       tw_x = tw_table<float>(g.nrx);
       tw_y = tw_table<float>(g.ny);
@@ -596,6 +614,10 @@ struct ffs_sim {
       if (sl.st) hipHostFree(sl.st);
       if (sl.ev) hipEventDestroy(sl.ev);
     }
+    for (auto& sl : spec_slots) {
+      if (sl.host) hipHostFree(sl.host);
+      if (sl.ev) hipEventDestroy(sl.ev);
+    }
     for (void* a : allocs) hipFree(a);
     allocs.clear();
     if (stream) hipStreamDestroy(stream);
@@ -687,6 +709,97 @@ struct ffs_sim {
   }
   void set_dt() { step_scalars(true, false); }
   void forcing() { step_scalars(false, true); }
+
+  // KH, KRZ of Globals.jl:148-152: the horizontal bins follow the finer of the x and y wavenumber steps
+  void spec_axes() {
+    if (!spec_kh.empty()) return;
+    spec_kh = (KX.size() > 1 && KY.size() > 1 && KX[1] >= KY[1]) ? KX : rfftfreq(g.ny, p.y_domain);
+    spec_krz = rfftfreq(g.nz, p.z_domain);
+  }
+
+  // calculate_specHV (spec.jl:424-450) from the marginals of one per-mode quantity:
+  // h[1] += 1/2 sum e(i=1); h[n(kx)] += MAXDKH X[cx] (cx > 0); h[n(ky)] += 1/2 Y0 + MAXDKH Y;
+  // v[n(kz)] += Z0 + 2 DKZ Z.  Bins past the end of KH / KRZ are dropped.
+  void bin_spectrum(const double* X, const double* Y0, const double* Y, const double* Z0, const double* Z,
+                    double* H, double* V) const {
+    const int nh = (int)spec_kh.size(), nv = (int)spec_krz.size();
+    const double dkh = nh > 1 ? spec_kh[1] : 1.0, dkz = nv > 1 ? spec_krz[1] : 1.0;
+    auto bin = [](double k, double dk) { return (int)std::nearbyint(std::fabs(k) / dk); };
+    for (int i = 0; i < nh; ++i) H[i] = 0.0;
+    for (int i = 0; i < nv; ++i) V[i] = 0.0;
+    H[0] += 0.5 * X[0];
+    for (int cx = 1; cx < g.mx; ++cx) {
+      const int b = bin(kxc[cx], dkh);
+      if (b < nh) H[b] += dkh * X[cx];
+    }
+    for (int cy = 0; cy < g.my; ++cy) {
+      const int b = bin(kyc[cy], dkh);
+      if (b < nh) H[b] += 0.5 * Y0[cy] + dkh * Y[cy];
+    }
+    for (int cz = 0; cz < g.mz; ++cz) {
+      const int b = bin(kzc[cz], dkz);
+      if (b < nv) V[b] += Z0[cz] + 2 * dkz * Z[cz];
+    }
+  }
+
+  void queue_spectra() {
+    spec_axes();
+    if (spec_slots.empty()) {
+      spec_slots.resize(4);
+      for (auto& sl : spec_slots) {
+        HIPCHECK(hipHostMalloc((void**)&sl.host, (size_t)kSpecQ * spectra_out_len(g) * sizeof(double),
+                               hipHostMallocDefault));
+        HIPCHECK(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+      }
+    }
+    int slot = -1;
+    for (int pass = 0; pass < 2 && slot < 0; ++pass) {
+      if (pass == 1) finish_oldest_spectra();
+      for (size_t i = 0; i < spec_slots.size(); ++i)
+        if (!spec_slots[i].busy) {
+          slot = (int)i;
+          break;
+        }
+    }
+    SpecArgs a{};
+    a.nl = rhs;
+    a.ke = ke;
+    a.tau = les ? tau : nullptr;
+    a.rho = dens ? rho : nullptr;
+    a.flux = dens ? flux : nullptr;
+    a.pf_factor = plane ? pfd.plane_factor : nullptr;
+    a.pf_kz0 = p.forcing == FFS_FORCING_QF ? 3 : 1;
+    a.nu = p.nu;
+    a.alpha = dens ? alpha : 0.0;
+    a.gz = dens ? gz : 0.0;
+    a.st = st;
+    launch("spectra", [&] { FFS_DISPATCH(launch_spectra, g, u, a, spec_part, spec_out, stream); });
+    auto& sl = spec_slots[slot];
+    HIPCHECK(hipMemcpyAsync(sl.host, spec_out, (size_t)kSpecQ * spectra_out_len(g) * sizeof(double),
+                            hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipEventRecord(sl.ev, stream));
+    sl.iteration = iteration;
+    sl.busy = true;
+    spec_pending.push_back(slot);
+  }
+
+  void finish_oldest_spectra() {
+    if (spec_pending.empty()) return;
+    const int slot = spec_pending.front();
+    spec_pending.pop_front();
+    auto& sl = spec_slots[slot];
+    HIPCHECK(hipEventSynchronize(sl.ev));
+    const size_t L = spectra_out_len(g);
+    const int nh = (int)spec_kh.size(), nv = (int)spec_krz.size();
+    std::vector<double> rec((size_t)kSpecQ * (nh + nv));
+    for (int q = 0; q < kSpecQ; ++q) {
+      const double* m = sl.host + (size_t)q * L;
+      double* H = rec.data() + (size_t)q * (nh + nv);
+      bin_spectrum(m, m + g.mx, m + g.mx + g.my, m + g.mx + 2 * g.my, m + g.mx + 2 * g.my + g.mz, H, H + nh);
+    }
+    spec_ready.emplace_back(sl.iteration, std::move(rec));
+    sl.busy = false;
+  }
 
   void queue_stats() {
     // find a free slot (complete the oldest pending record if all are busy)
@@ -882,6 +995,7 @@ struct ffs_sim {
     const bool spec_t = is_spec_time(iteration);
     nonlinear(stats_t || spec_t);
     step_scalars(true, true);
+    if (spec_t) queue_spectra();
     if (stats_t) queue_stats();
     time_step();
   }
@@ -1528,36 +1642,17 @@ int ffs_spectrum(ffs_sim* s, double* h, int cap_h, int* n_h, double* v, int cap_
     const double* pz = s->partials + (size_t)g.mz * 2 * g.mx + (size_t)g.mz * 2 * g.my;
     HIPCHECK(hipMemcpyAsync(PZ.data(), pz, PZ.size() * 8, hipMemcpyDeviceToHost, s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
-    // KH / MAXDKH / KRZ / DKZ of Globals.jl:148-152
-    std::vector<double> KH = (s->KX.size() > 1 && s->KY.size() > 1 && s->KX[1] >= s->KY[1]) ? s->KX
-                                                                                           : rfftfreq(g.ny, s->p.y_domain);
-    const double dkh = KH.size() > 1 ? KH[1] : 1.0;
-    std::vector<double> KRZ = rfftfreq(g.nz, s->p.z_domain);
-    const double dkz = KRZ.size() > 1 ? KRZ[1] : 1.0;
-    const int nh = (int)KH.size(), nv = (int)KRZ.size();
-    // hv = 0: horizontal-energy quantity (|ux|^2+|uy|^2)/2, hv = 1: vertical |uz|^2/2
-    std::vector<double> H(2 * (size_t)nh, 0.0), V(2 * (size_t)nv, 0.0);
-    auto bin = [](double k, double dk) { return (int)std::nearbyint(std::fabs(k) / dk); };
+    s->spec_axes();
+    const int nh = (int)s->spec_kh.size(), nv = (int)s->spec_krz.size();
+    // hv = 0: (|ux|^2+|uy|^2)/2, hv = 1: |uz|^2/2
+    std::vector<double> H(2 * (size_t)nh), V(2 * (size_t)nv), Z0(g.mz), Z(g.mz);
     for (int hv = 0; hv < 2; ++hv) {
-      double* Hh = H.data() + (size_t)hv * nh;
-      double* Vh = V.data() + (size_t)hv * nv;
-      const double* Xh = X.data() + (size_t)hv * g.mx;
-      const double* Y0h = Y0.data() + (size_t)hv * g.my;
-      const double* Yh = Y.data() + (size_t)hv * g.my;
-      // h[1] += 1/2 sum e(i=1); h[nx] += dkh X[cx] (cx>0); h[nj] += 1/2 Y0 + dkh Y; v[nk] += Z0 + 2 dkz Z
-      Hh[0] += 0.5 * Xh[0];
-      for (int cx = 1; cx < g.mx; ++cx) {
-        const int b = bin(s->kxc[cx], dkh);
-        if (b < nh) Hh[b] += dkh * Xh[cx];
-      }
-      for (int cy = 0; cy < g.my; ++cy) {
-        const int b = bin(s->kyc[cy], dkh);
-        if (b < nh) Hh[b] += 0.5 * Y0h[cy] + dkh * Yh[cy];
-      }
       for (int cz = 0; cz < g.mz; ++cz) {
-        const int b = bin(s->kzc[cz], dkz);
-        if (b < nv) Vh[b] += PZ[4 * cz + hv] + 2 * dkz * PZ[4 * cz + 2 + hv];
+        Z0[cz] = PZ[4 * cz + hv];
+        Z[cz] = PZ[4 * cz + 2 + hv];
       }
+      s->bin_spectrum(X.data() + (size_t)hv * g.mx, Y0.data() + (size_t)hv * g.my, Y.data() + (size_t)hv * g.my,
+                      Z0.data(), Z.data(), H.data() + (size_t)hv * nh, V.data() + (size_t)hv * nv);
     }
     if (n_h) *n_h = nh;
     if (n_v) *n_v = nv;
@@ -1566,6 +1661,41 @@ int ffs_spectrum(ffs_sim* s, double* h, int cap_h, int* n_h, double* v, int cap_
     if (v)
       for (int i = 0; i < 2 * nv && i < cap_v; ++i) v[i] = V[i];
   });
+}
+
+static_assert(kSpecQ == FFS_SPEC_Q && SQ_U_H == FFS_SPEC_U_H && SQ_PRESS_V == FFS_SPEC_PRESS_V &&
+                  SQ_LES_H == FFS_SPEC_LES_H && SQ_BUOY == FFS_SPEC_BUOYANCY && SQ_RHONL == FFS_SPEC_RHONL &&
+                  SQ_FORCING == FFS_SPEC_FORCING,
+              "the device's spectra quantities follow the FFS_SPEC_* order");
+
+int ffs_spectra_axes(ffs_sim* s, double* kh, int cap_h, int* n_h, double* krz, int cap_v, int* n_v) {
+  if (!s) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    s->spec_axes();
+    if (n_h) *n_h = (int)s->spec_kh.size();
+    if (n_v) *n_v = (int)s->spec_krz.size();
+    if (kh)
+      for (int i = 0; i < (int)s->spec_kh.size() && i < cap_h; ++i) kh[i] = s->spec_kh[i];
+    if (krz)
+      for (int i = 0; i < (int)s->spec_krz.size() && i < cap_v; ++i) krz[i] = s->spec_krz[i];
+  });
+}
+
+int ffs_spectra_pop(ffs_sim* s, int64_t* iteration, double* data, size_t cap, size_t* n) {
+  if (!s) return FFS_ERR_ARG;
+  int got = 0;
+  int rc = guard(s, [&] {
+    if (s->spec_ready.empty()) s->finish_oldest_spectra();
+    if (s->spec_ready.empty()) return;
+    auto r = std::move(s->spec_ready.front());
+    s->spec_ready.pop_front();
+    if (iteration) *iteration = r.first;
+    if (n) *n = r.second.size();
+    if (data)
+      for (size_t i = 0; i < r.second.size() && i < cap; ++i) data[i] = r.second[i];
+    got = 1;
+  });
+  return rc < 0 ? rc : got;
 }
 
 int ffs_get_scalar(const ffs_sim* cs, int what, double* out) {

@@ -30,6 +30,10 @@ FFS_SPACE_REAL, FFS_SPACE_FOURIER = 0, 1
 FFS_SCALAR_ITERATION, FFS_SCALAR_TIME, FFS_SCALAR_DT, FFS_SCALAR_UMAX = 0, 1, 2, 3
 FFS_TABLE_STATS, FFS_TABLE_SCALES, FFS_TABLE_NUMBERS = 0, 1, 2
 FFS_EVENT_NONE, FFS_EVENT_OUTPUT = 0, 1
+# the spectra quantities of a spectra record, in record order
+SPEC_QUANTITIES = ("u_h", "u_v", "vis_h", "vis_v", "nl_h", "nl_v", "press_h", "press_v", "les_h", "les_v",
+                   "buoyancy_v", "rho", "rhodiss", "rhonl", "forcing_h")
+FFS_SPEC_Q = len(SPEC_QUANTITIES)
 
 
 class FfsParams(ctypes.Structure):
@@ -98,6 +102,8 @@ SIGNATURES = {
     "ffs_stats_pop": (ctypes.c_int, [_P, _D, ctypes.c_int, _I, _D, ctypes.c_int, _I, _D, ctypes.c_int, _I]),
     "ffs_stats_header": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
     "ffs_spectrum": (ctypes.c_int, [_P, _D, ctypes.c_int, _I, _D, ctypes.c_int, _I]),
+    "ffs_spectra_axes": (ctypes.c_int, [_P, _D, ctypes.c_int, _I, _D, ctypes.c_int, _I]),
+    "ffs_spectra_pop": (ctypes.c_int, [_P, _I64, _D, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "ffs_get_scalar": (ctypes.c_int, [_P, ctypes.c_int, _D]),
     "ffs_profile_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "ffs_profile_get": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_char_p), _D, _I64, ctypes.c_int, _I]),

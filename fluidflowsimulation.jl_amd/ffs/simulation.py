@@ -8,11 +8,12 @@ libffs_hip.so; this module only moves files and rows.
 import ctypes
 import math
 import os
+from decimal import Decimal
 
 import numpy as np
 
 from . import _lib
-from .config import RF_FAMILY, Params, parameters, readglobal
+from .config import PLANE_FORCINGS, RF_FAMILY, Params, parameters, readglobal
 from .fields import padded_shape, read_field, write_field
 
 __all__ = ["Simulation", "footprint", "run_simulation"]
@@ -210,6 +211,52 @@ class Simulation:
         a, b = nh.value, nv.value
         return (h[:a].copy(), v[:b].copy()), (h[a:2 * a].copy(), v[b:2 * b].copy())
 
+    def spectra_axes(self):
+        """(KH, KRZ) of Globals.jl:148-152: the bins of the specH and specV files."""
+        kh = np.zeros(8192)
+        kz = np.zeros(8192)
+        nh, nv = ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.ffs_spectra_axes(self._h, _dptr(kh), kh.size, ctypes.byref(nh), _dptr(kz), kz.size,
+                                               ctypes.byref(nv)))
+        return kh[: nh.value].copy(), kz[: nv.value].copy()
+
+    def pop_spectra(self):
+        """The queued spectra records (write_spec / write_spec_forcing, spec.jl:1-168, one per spectra
+        iteration) as a list of (iteration, {quantity: (specH, specV)}), quantities named as in
+        _lib.SPEC_QUANTITIES; the ones this run does not have are left out."""
+        kh, kz = self.spectra_axes()
+        nh, nv = kh.size, kz.size
+        buf = np.zeros(_lib.FFS_SPEC_Q * (nh + nv))
+        it = ctypes.c_int64()
+        n = ctypes.c_size_t()
+        have = set(_lib.SPEC_QUANTITIES[:8])
+        if self.params.les_model not in (None, "", "noLesModel"):
+            have |= {"les_h", "les_v"}
+        if self.params.density:
+            have |= {"buoyancy_v", "rho", "rhodiss", "rhonl"}
+        if self.params.forcing in PLANE_FORCINGS:
+            have.add("forcing_h")
+        out = []
+        while True:
+            r = self._check(self._lib.ffs_spectra_pop(self._h, ctypes.byref(it), _dptr(buf), buf.size,
+                                                      ctypes.byref(n)))
+            if r == 0:
+                return out
+            rec = buf.reshape(_lib.FFS_SPEC_Q, nh + nv)
+            out.append((int(it.value), {q: (rec[i, :nh].copy(), rec[i, nh:].copy())
+                                        for i, q in enumerate(_lib.SPEC_QUANTITIES) if q in have}))
+
+    def writespectra(self, directory="."):
+        """Writes every queued spectra record as write_spec / write_spec_forcing do (writespectrum,
+        spec.jl:171-194; writedlm of (KH, specH) and (KRZ, specV) rows)."""
+        kh, kz = self.spectra_axes()
+        recs = self.pop_spectra()
+        for it, q in recs:
+            for name, (h, v) in spectra_files(q, self.params).items():
+                axis = kh if name.endswith("specH") else kz
+                _writedlm(os.path.join(directory, f"{name}.{it}"), axis, h if v is None else v)
+        return recs
+
     def scalar(self, what):
         out = ctypes.c_double()
         self._check(self._lib.ffs_get_scalar(self._h, what, ctypes.byref(out)))
@@ -284,10 +331,71 @@ def footprint(params: Params):
     return b.value
 
 
+def spectra_files(q, params):
+    """{file stem: (values, None)} of one spectra record: writespectrum's five files for u, vis, nl,
+    press (and les), the specH/specV pair of the single spectra, rhosource from buoyancy_v
+    (spec.jl:1-168)."""
+    out = {}
+    for n in ("u", "vis", "nl", "press", "les"):
+        if n + "_h" not in q:
+            continue
+        (hH, hV), (vH, vV) = q[n + "_h"], q[n + "_v"]
+        out[f"{n}_h.specH"] = (hH, None)
+        out[f"{n}_h.specV"] = (hV, None)
+        out[f"{n}.specH"] = (hH + vH, None)
+        out[f"{n}_v.specH"] = (vH, None)
+        out[f"{n}_v.specV"] = (vV, None)
+    if "buoyancy_v" in q:
+        bH, bV = q["buoyancy_v"]
+        out["buoyancy_v.specH"] = (bH, None)
+        out["buoyancy_v.specV"] = (bV, None)
+        gz = params.z_acceleration / params.reference_density
+        with np.errstate(divide="ignore", invalid="ignore"):
+            out["rhosource.specH"] = (-(bH / gz) * params.density_gradient, None)
+            out["rhosource.specV"] = (-(bV / gz) * params.density_gradient, None)
+    for n in ("rho", "rhodiss", "rhonl", "forcing_h"):
+        if n in q:
+            out[f"{n}.specH"] = (q[n][0], None)
+            out[f"{n}.specV"] = (q[n][1], None)
+    return out
+
+
+def _writedlm(path, k, v):
+    with open(path, "w") as f:
+        for a, b in zip(k, v):
+            f.write(f"{jl_float(a)}\t{jl_float(b)}\n")
+
+
+def jl_float(x):
+    """A Float64 as Julia prints it (shortest round-trip digits; scientific notation below 1e-4 and
+    from 1e6 on, as 1.0e-5 / 1.048576e6)."""
+    x = float(x)
+    if x != x:
+        return "NaN"
+    if math.isinf(x):
+        return "Inf" if x > 0 else "-Inf"
+    if x == 0.0:
+        return "-0.0" if math.copysign(1.0, x) < 0 else "0.0"
+    sign, digits, exp = Decimal(repr(x)).as_tuple()
+    d = "".join(map(str, digits))
+    stripped = d.rstrip("0")
+    exp += len(d) - len(stripped)
+    d = stripped or "0"
+    e10 = len(d) - 1 + exp
+    if e10 < -4 or e10 >= 6:
+        body = f"{d[0]}.{d[1:] or '0'}e{e10}"
+    elif exp >= 0:
+        body = d + "0" * exp + ".0"
+    else:
+        point = len(d) + exp
+        body = (d[:point] + "." + d[point:]) if point > 0 else ("0." + "0" * (-point) + d)
+    return ("-" if sign else "") + body
+
+
 def _fmt(v, i):
     if i == 0:
         return str(int(v))
-    return repr(float(v))
+    return jl_float(v)
 
 
 def run_simulation(directory=".", params=None, nt=None):
@@ -318,6 +426,7 @@ def run_simulation(directory=".", params=None, nt=None):
         if ev == _lib.FFS_EVENT_OUTPUT:
             s.writeoutput(directory)
         s.writestats(directory)
+        s.writespectra(directory)
     s.writestats_now()
     s.writestats(directory)
     if s.iteration % params.write_time != 0:

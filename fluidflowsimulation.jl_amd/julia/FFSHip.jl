@@ -8,8 +8,10 @@
 # declaration of ffs.h; tests/test_julia_shim.py checks the correspondence.
 module FFSHip
 
-export FFSSimulation, FFSBatch, ffs_params, initialize!, advance_in_time!, writestats, writeoutput, run_simulation,
-       footprint
+using DelimitedFiles: writedlm
+
+export FFSSimulation, FFSBatch, ffs_params, initialize!, advance_in_time!, writestats, writespectra, writeoutput,
+       run_simulation, footprint
 
 const libffs = get(ENV, "FFS_LIB", joinpath(@__DIR__, "..", "ffs", "libffs_hip.so"))
 
@@ -166,6 +168,7 @@ function advance_in_time!(s::FFSSimulation)
         advance!(s, 1)
     end
     writestats(s)
+    writespectra(s)
     return nothing
 end
 
@@ -228,6 +231,55 @@ function spectrum(s::FFSSimulation)
     check(ccall((:ffs_spectrum, libffs), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint, Ref{Cint}, Ptr{Float64}, Cint, Ref{Cint}),
                 s.h, h, 8192, nh, v, 8192, nv), s.h)
     return h[1:2nh[]], v[1:2nv[]]
+end
+
+# KH and KRZ (Globals.jl:148-152): the first column of the specH / specV files
+function spectra_axes(s::FFSSimulation)
+    kh, krz = zeros(8192), zeros(8192)
+    nh, nv = Ref{Cint}(0), Ref{Cint}(0)
+    check(ccall((:ffs_spectra_axes, libffs), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Cint, Ref{Cint}, Ptr{Float64}, Cint, Ref{Cint}),
+                s.h, kh, 8192, nh, krz, 8192, nv), s.h)
+    return kh[1:nh[]], krz[1:nv[]]
+end
+
+const SPEC_QUANTITIES = ("u_h", "u_v", "vis_h", "vis_v", "nl_h", "nl_v", "press_h", "press_v", "les_h", "les_v",
+                         "buoyancy_v", "rho", "rhodiss", "rhonl", "forcing_h")
+
+# write_spec / write_spec_forcing (spec.jl:1-168): the queued spectra records as the reference's files
+function writespectra(s::FFSSimulation)
+    kh, krz = spectra_axes(s)
+    nh, nv = length(kh), length(krz)
+    buf = zeros(length(SPEC_QUANTITIES) * (nh + nv))
+    it, n = Ref{Int64}(0), Ref{Csize_t}(0)
+    p = s.p
+    while true
+        r = check(ccall((:ffs_spectra_pop, libffs), Cint, (Ptr{Cvoid}, Ref{Int64}, Ptr{Float64}, Csize_t, Ref{Csize_t}),
+                        s.h, it, buf, length(buf), n), s.h)
+        r == 0 && return nothing
+        i = it[]
+        q = Dict(name => (buf[(k - 1) * (nh + nv) + 1:(k - 1) * (nh + nv) + nh],
+                          buf[(k - 1) * (nh + nv) + nh + 1:k * (nh + nv)]) for (k, name) in enumerate(SPEC_QUANTITIES))
+        for n_ in ("u", "vis", "nl", "press", "les")
+            (n_ == "les" && p.les_model == 0) && continue
+            (hH, hV), (vH, vV) = q[n_ * "_h"], q[n_ * "_v"]
+            writedlm("$(n_)_h.specH.$i", zip(kh, hH)); writedlm("$(n_)_h.specV.$i", zip(krz, hV))
+            writedlm("$(n_).specH.$i", zip(kh, hH .+ vH))
+            writedlm("$(n_)_v.specH.$i", zip(kh, vH)); writedlm("$(n_)_v.specV.$i", zip(krz, vV))
+        end
+        singles = String[]
+        if p.density != 0
+            bH, bV = q["buoyancy_v"]
+            gz = p.z_acceleration / p.reference_density
+            writedlm("rhosource.specH.$i", zip(kh, (-).(bH ./ gz) .* p.density_gradient))
+            writedlm("rhosource.specV.$i", zip(krz, (-).(bV ./ gz) .* p.density_gradient))
+            append!(singles, ["buoyancy_v", "rho", "rhodiss", "rhonl"])
+        end
+        p.forcing >= 2 && push!(singles, "forcing_h")
+        for n_ in singles
+            writedlm("$(n_).specH.$i", zip(kh, q[n_][1])); writedlm("$(n_).specV.$i", zip(krz, q[n_][2]))
+        end
+    end
 end
 
 function scalar(s::FFSSimulation, what::Integer)
@@ -333,6 +385,7 @@ function run_simulation(d::AbstractDict)
         left -= done
         ev == FFS_EVENT_OUTPUT && writeoutput(s)
         writestats(s)
+        writespectra(s)
     end
     writestats_now!(s)
     writestats(s)

@@ -204,6 +204,41 @@ int ffs_stats_header(const ffs_sim* s, int table, char* buf, size_t cap);
  * u_v.specH, u_h.specV and u_v.specV files of writespectrum (spec.jl:173-194). */
 int ffs_spectrum(ffs_sim* s, double* h, int cap_h, int* n_h, double* v, int cap_v, int* n_v);
 
+/* The spectra files of write_spec and write_spec_forcing (spec.jl:1-168),
+ * computed on the device at every spectra iteration (is_spec_time,
+ * simulationtypes.jl:102-104: iteration % dtSpec == 0) at the point the
+ * reference computes them (models.jl:37, models.jl:8) and queued like the
+ * stats records.  A record holds FFS_SPEC_Q per-mode quantities, each binned by
+ * calculate_specHV (spec.jl:424-450) into specH (n_h values over KH) then specV
+ * (n_v values over KRZ), quantity after quantity in the order of FFS_SPEC_*.
+ * Quantities a run does not have (LES, density, plane forcing) read zero.
+ * The files: for u, vis, nl, press, les: n_h.specH/V (the _H quantity),
+ * n_v.specH/V (the _V one) and n.specH (their specH sum); buoyancy_v,
+ * rhosource (= -specH,V(buoyancy_v) / g_z * densityGradient), rho, rhodiss,
+ * rhonl and forcing_h: .specH and .specV. */
+#define FFS_SPEC_U_H 0
+#define FFS_SPEC_U_V 1
+#define FFS_SPEC_VIS_H 2
+#define FFS_SPEC_VIS_V 3
+#define FFS_SPEC_NL_H 4
+#define FFS_SPEC_NL_V 5
+#define FFS_SPEC_PRESS_H 6
+#define FFS_SPEC_PRESS_V 7
+#define FFS_SPEC_LES_H 8
+#define FFS_SPEC_LES_V 9
+#define FFS_SPEC_BUOYANCY 10
+#define FFS_SPEC_RHO 11
+#define FFS_SPEC_RHODISS 12
+#define FFS_SPEC_RHONL 13
+#define FFS_SPEC_FORCING 14
+#define FFS_SPEC_Q 15
+/* KH (n_h values) and KRZ (n_v values) of Globals.jl:148-152. */
+int ffs_spectra_axes(ffs_sim* s, double* kh, int cap_h, int* n_h, double* krz, int cap_v, int* n_v);
+/* Pops the oldest spectra record (waiting for it if it is still in flight):
+ * returns 1 and fills *iteration and FFS_SPEC_Q * (n_h + n_v) values (*n), 0 if
+ * no record is pending, <0 on error. */
+int ffs_spectra_pop(ffs_sim* s, int64_t* iteration, double* data, size_t cap, size_t* n);
+
 /* Scalars: s.iteration[], s.time[], get_dt(s), last max|u_i|. */
 int ffs_get_scalar(const ffs_sim* s, int what, double* out);
 

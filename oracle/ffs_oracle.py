@@ -260,8 +260,11 @@ class OracleSim:
             self.tauh = np.array([g.myfourier(c) for c in taur])
         if self.variable_dt:
             self.set_dt()
+        if self.density is not None:
+            self.fluxh = fluxh
         if collect and self.is_spec_time():
-            self.spectra.append((self.iteration, *self.spectrum()))
+            # write_spec (models.jl:37): after set_dt!, before fourierspacep2!
+            self.spectra.append((self.iteration, self.all_spectra(rhs)))
         if collect and self.is_stats_time():
             self.nlstats = self.non_linear_stats(rhs)
             self.pressstats = self.pressure_stats(rhs)
@@ -446,6 +449,8 @@ class OracleSim:
         for _ in range(n):
             self.calculate_rhs()
             self.forcing()
+            if self.plane is not None and self.is_spec_time():  # write_spec_forcing (models.jl:8)
+                self.spectra[-1][1]["forcing_h"] = self.forcing_spectrum()
             if self.is_stats_time():
                 self.records.append(self.stats())
             self.time_step()
@@ -587,28 +592,90 @@ class OracleSim:
                 numbers = [self.iteration, self.time, Reh, Reh1, Rel, Ret, Rel1, Ret1]
         return np.array(stats_row), np.array(scales), np.array(numbers)
 
+    def spec_hv(self, arr):
+        """calculate_specHV (spec.jl:424-450), loop for loop: (specH over KH, specV over KRZ)."""
+        g = self.g
+        h = np.zeros(len(g.KH))
+        v = np.zeros(len(g.KRZ))
+        nx = np.rint(np.abs(g.KX[1:]) / g.MAXDKH).astype(int)
+        for l in g.ZRANGE:
+            nk = int(round(abs(g.KZ[l]) / g.DKZ))
+            for j in g.YRANGE:
+                nj = int(round(abs(g.KY[j]) / g.MAXDKH))
+                ee = arr[0, j, l]
+                h[0] += 0.5 * ee
+                h[nj] += 0.5 * ee
+                v[nk] += ee
+                ee2 = 2 * arr[1:, j, l]
+                np.add.at(h, nx, 0.5 * ee2 * g.MAXDKH)
+                h[nj] += float(np.sum(0.5 * ee2 * g.MAXDKH))
+                v[nk] += float(np.sum(ee2 * g.DKZ))
+        return h, v
+
     def spectrum(self):
-        """spectrum_u + calculate_specHV (spec.jl:316-327,424-450), loop for loop."""
+        """spectrum_u + calculate_specHV (spec.jl:316-327,424-450)."""
         g = self.g
         u = self.uh
         hs = np.where(g.RANGE, (proj(u[0], u[0]) + proj(u[1], u[1])) / 2, 0.0)
         vs = np.where(g.RANGE, proj(u[2], u[2]) / 2, 0.0)
-        out = []
-        for arr in (hs, vs):
-            h = np.zeros(len(g.KH))
-            v = np.zeros(len(g.KRZ))
-            for l in g.ZRANGE:
-                nk = int(round(abs(g.KZ[l]) / g.DKZ))
-                for j in g.YRANGE:
-                    nj = int(round(abs(g.KY[j]) / g.MAXDKH))
-                    ee = arr[0, j, l]
-                    h[0] += 0.5 * ee
-                    h[nj] += 0.5 * ee
-                    v[nk] += ee
-                    ee2 = 2 * arr[1:, j, l]
-                    nx = np.rint(np.abs(g.KX[1:]) / g.MAXDKH).astype(int)
-                    np.add.at(h, nx, 0.5 * ee2 * g.MAXDKH)
-                    h[nj] += float(np.sum(0.5 * ee2 * g.MAXDKH))
-                    v[nk] += float(np.sum(ee2 * g.DKZ))
-            out.append((h, v))
+        return [self.spec_hv(hs), self.spec_hv(vs)]
+
+    def all_spectra(self, nl):
+        """The per-mode arrays of write_spec (spec.jl:1-168) binned by calculate_specHV:
+        {quantity: (specH, specV)} for u, vis, nl, press (h and v parts), les, buoyancy_v, rho,
+        rhodiss, rhonl.  nl is the nonlinear term F(u x w) with its k = 0 mode zero."""
+        g = self.g
+        u = self.uh
+        kv = (g.kx, g.ky, g.kz)
+        R = g.RANGE
+        out = {}
+
+        def put(name, arr):
+            out[name] = self.spec_hv(np.where(R, arr, 0.0))
+
+        # spectrum_u (:316-327)
+        hu = (proj(u[0], u[0]) + proj(u[1], u[1])) / 2
+        vu = proj(u[2], u[2]) / 2
+        put("u_h", hu)
+        put("u_v", vu)
+        # spectrum_viscosity (:340-354): 2 nu k^2 times the u spectrum, k^2 = muladd(kx, kx, muladd(ky, ky, kz^2))
+        k2 = g.kx * g.kx + (g.ky * g.ky + g.kz * g.kz)
+        put("vis_h", 2 * self.nu * k2 * hu)
+        put("vis_v", 2 * self.nu * k2 * vu)
+        # spectrum_non_linear (:210-222): vecouterproj(nl - (i k) ke, u)
+        o = [proj(nl[c] - 1j * kv[c] * self.kspec, u[c]) for c in range(3)]
+        put("nl_h", o[0] + o[1])
+        put("nl_v", o[2])
+        # spectrum_pressure (:370-422)
+        r = [nl[c] - (1j * self.kspec) * kv[c] for c in range(3)]
+        if self.les_c is not None:
+            d = self.div_tau()
+            r = [r[c] + d[c] for c in range(3)]
+        if self.density is not None:
+            r[2] = r[2] + self.rhoh * self.gz
+        with np.errstate(invalid="ignore", divide="ignore"):
+            p1 = -(g.kx * r[0] + g.ky * r[1] + g.kz * r[2]) / g.K2
+        p1[0, 0, 0] = 0
+        o = [proj(u[c], p1 * kv[c]) for c in range(3)]
+        put("press_h", o[0] + o[1])
+        put("press_v", o[2])
+        if self.les_c is not None:  # spectrum_les (:224-235): vecouterproj((i k).tau, u)
+            d = self.div_tau()
+            o = [proj(d[c], u[c]) for c in range(3)]
+            put("les_h", o[0] + o[1])
+            put("les_v", o[2])
+        if self.density is not None:
+            rho = self.rhoh
+            put("buoyancy_v", proj(u[2], self.gz * rho))                     # spectrum_buoyancy (:249-257)
+            put("rho", proj(rho, rho))                                        # spectrum_rho (:329-338)
+            put("rhodiss", self.alpha * k2 * proj(rho, rho))                  # spectrum_rhodiss (:356-368)
+            div = 1j * (g.kx * self.fluxh[0] + g.ky * self.fluxh[1] + g.kz * self.fluxh[2])
+            put("rhonl", proj(div, rho))                                      # spectrum_rhonl (:237-247)
         return out
+
+    def forcing_spectrum(self):
+        """spectrum_forcing (spec.jl:259-268) of the plane forcing: (proj(ux, fx) + proj(uy, fy)) / dt."""
+        u = self.uh
+        f = self.plane
+        arr = (proj(u[0], f.forcex) + proj(u[1], f.forcey)) / self.dt
+        return self.spec_hv(np.where(self.g.RANGE, arr, 0.0))
