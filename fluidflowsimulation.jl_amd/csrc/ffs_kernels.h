@@ -136,6 +136,15 @@ template <typename T>
 void launch_fwd_z(const Geo& g, const void* b1, void* const* dst, int nf, int nzero_origin, double scale,
                   const void* tw, hipStream_t s);
 
+// hyperviscosity (Hyperviscosity/hyperviscosity_types.jl): kind 0 none, 1 HyperViscosity{nu_h, M}
+// (-nu_h k^2M in the AB3 rhs and the ETD c), 2 SpectralBarrier (ETD c += log(esp(|k|, initk, endk, 1/4)))
+struct HvArgs {
+  int kind;
+  int m;
+  double nuh;
+  double sb_i, sb_p;  // initk, endk
+};
+
 struct UpdateArgs {
   int integrator;      // 0 = Adams_Bashforth3rdO (explicit viscosity), 1 = ETD3rdO (ETD1/ETD2 start)
   int step;            // integrator iteration after this step's increment (1, 2, ...); 0 = final rhs only
@@ -154,6 +163,7 @@ struct UpdateArgs {
   double drhodz;       // mean density gradient
   double gz;           // g_z = zAcceleration / referenceDensity
   double* time;        // s.time[] (DevState::time): advanced by dt once per step, or null
+  HvArgs hv;
 };
 template <typename T>
 void launch_update(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
@@ -171,6 +181,7 @@ struct StatsAux {
   const void* tau;  // Smagorinsky stress (5 compact components) or null
   const void* rho;  // density or null
   double gz;        // g_z for the buoyancy term of the pressure statistics
+  HvArgs hv;        // hyperviscous dissipation sums (slots 30, 31)
 };
 template <typename T>
 int launch_stats(const Geo& g, const void* u, const void* nl, const void* ke, const StatsAux& aux, double* partials,
@@ -182,7 +193,7 @@ void launch_spectrum(const Geo& g, const void* u, double* out_x, double* out_y0,
 // quantities below, reduced to the marginals calculate_specHV bins (spec.jl:424-450)
 enum : int {
   SQ_U_H, SQ_U_V, SQ_VIS_H, SQ_VIS_V, SQ_NL_H, SQ_NL_V, SQ_PRESS_H, SQ_PRESS_V, SQ_LES_H, SQ_LES_V,
-  SQ_BUOY, SQ_RHO, SQ_RHODISS, SQ_RHONL, SQ_FORCING, kSpecQ
+  SQ_BUOY, SQ_RHO, SQ_RHODISS, SQ_RHONL, SQ_FORCING, SQ_HVIS_H, SQ_HVIS_V, kSpecQ
 };
 struct SpecArgs {
   const void* nl;     // the nonlinear term F(u x w) (rhs before fourierspacep2!)
@@ -194,6 +205,7 @@ struct SpecArgs {
   int pf_kz0;               // plane forcing: the vertical modes are pf_kz0 <= |kz| <= 5
   double nu, alpha, gz;
   const DevState* st;
+  HvArgs hv;  // hvis: spectrum_hyperviscosity / spectrum_spectral_barrier
 };
 // per quantity q: X[mx] | Y0[my] | Y[my] | Z0[mz] | Z[mz]  (spectra_out_len doubles each)
 __host__ __device__ inline size_t spectra_out_len(const Geo& g) { return (size_t)g.mx + 2 * (size_t)g.my + 2 * (size_t)g.mz; }

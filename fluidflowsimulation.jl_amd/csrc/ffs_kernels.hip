@@ -911,8 +911,46 @@ void launch_x_r2c(const Geo& g, const T* real, void* b2, int nf, const void* tw,
 // (div!, vectorfunctions.jl:93-103) from the pre-step w.  rhs / rho rhs are
 // written back: they are the next step's fm1 (history rotated by pointer).
 
+// ---- hyperviscosity (Hyperviscosity/hyperviscosity_types.jl, ETD.jl:56-95) ----
+// k^2M as repeated products (M >= 1)
+__device__ __forceinline__ double ipow_m(double x, int m) {
+  double r = x;
+  for (int i = 1; i < m; ++i) r *= x;
+  return r;
+}
+// SpectralBarrier's f(k) = log(esp(k, initk, endk, 1/4)): 0 up to initk, -Inf from endk on
+__device__ __forceinline__ double barrier_f(double k, double ki, double kp) {
+  const double ak = fabs(k);
+  const double e = ak <= ki ? 1.0 : (ak < kp ? exp(-((ak - ki) * (ak - ki)) / (pow(kp, 0.25) - pow(ak, 0.25))) : 0.0);
+  return log(e);
+}
+// muladd(KX, KX, muladd(KY, KY, KZ^2)) of init_c! and the hyperviscosity loops.  Evaluated unfused: the
+// ETD3 closed forms cancel near their series switch, so c must round exactly as the host restatement's
+// (a reference that fuses muladd differs from both by an ulp of c)
+__device__ __forceinline__ double modk2(double kx, double ky, double kz) {
+#pragma clang fp contract(off)
+  return kx * kx + (ky * ky + kz * kz);
+}
+// init_c! (ETD.jl:56-95): the linear operator c of one mode for diffusivity -mnu
+__device__ __forceinline__ double etd_c(const HvArgs& hv, double K2, const double kv3[3], double mnu) {
+#pragma clang fp contract(off)
+  if (hv.kind == 0) return K2 * mnu;
+  const double mk = modk2(kv3[0], kv3[1], kv3[2]);
+  if (hv.kind == 1) return mk * mnu + ipow_m(mk, hv.m) * (-hv.nuh);
+  return mk * mnu + barrier_f(sqrt(mk), hv.sb_i, hv.sb_p);
+}
+// per-mode weight of the hyperviscous dissipation sums: k^2M (hyperviscosity_stats, stats.jl:532-561; nu_h
+// multiplies the sum) or f(|k|) with -Inf read as 0 (spectral_barrier_stats, hyperviscosity_types.jl:50-79)
+__device__ __forceinline__ double hv_weight(const HvArgs& hv, const double kv3[3]) {
+  const double mk = modk2(kv3[0], kv3[1], kv3[2]);
+  if (hv.kind == 1) return ipow_m(mk, hv.m);
+  const double f = barrier_f(sqrt(mk), hv.sb_i, hv.sb_p);
+  return f == -INFINITY ? 0.0 : f;
+}
+
 // ETD coefficients of one mode (FP64): the closed forms with expm1 and the
-// series below |c dt| = 1e-4 exactly as set_ABCt! / ETD1stO! / ETD2ndO!.
+// series below |c dt| = 1e-4 exactly as set_ABCt! / ETD1stO! / ETD2ndO!; c = -Inf
+// (beyond a spectral barrier) gives zero multipliers (their test2 branches).
 struct EtdCoef {
   double E, A, B, Cc;  // exp(c dt) and the multipliers of rhs, fm1, fm2
 };
@@ -924,6 +962,7 @@ __device__ __forceinline__ EtdCoef etd_coef(double c, int step, const DevState* 
   EtdCoef r;
   r.E = exp(c * dt);
   r.A = r.B = r.Cc = 0.0;
+  if (c == -INFINITY) return r;
   if (step == 1) {
     const double cdt = c * dt, cdt2 = cdt * dt, c2dt3 = cdt2 * cdt, c3dt4 = c2dt3 * cdt;
     r.A = (-cdt <= 1e-4) ? dt + 0.5 * cdt2 + c2dt3 / 6 + c3dt4 / 24 : expm1(cdt) / c;
@@ -963,8 +1002,12 @@ __device__ __forceinline__ EtdCoef etd_coef(double c, int step, const DevState* 
 template <typename T>
 __device__ __forceinline__ void form_rhs(const UpdateArgs& a, size_t idx, size_t n, const double kv3[3], double K2,
                                          bool origin, const double U[3][2], double R[3][2], double RH[2]) {
-  if (a.integrator == 0) {  // explicit viscosity (is_implicit false)
-    const double kvis = K2 * (-a.nu);
+  if (a.integrator == 0) {  // explicit viscosity (is_implicit false): (K2 mnu + K2^M mnuh) v
+    double kvis;
+    {
+#pragma clang fp contract(off)
+      kvis = a.hv.kind == 1 ? K2 * (-a.nu) + ipow_m(K2, a.hv.m) * (-a.hv.nuh) : K2 * (-a.nu);
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -1087,7 +1130,7 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
 #pragma unroll
     for (int p = 0; p < 2; ++p) RH[p] = fma(At, RR[p], fma(Bt, HR1[p], fma(Ct, HR2[p], RH[p])));
   } else {
-    const EtdCoef e = etd_coef(K2 * (-a.nu), a.step, st);
+    const EtdCoef e = etd_coef(etd_c(a.hv, K2, kv3, -a.nu), a.step, st);
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -1097,7 +1140,7 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
         else U[c][p] = fma(e.A, R[c][p], fma(e.B, M1[c][p], fma(e.Cc, M2[c][p], e.E * U[c][p])));
       }
     if (a.rho) {
-      const EtdCoef d = etd_coef(K2 * (-a.alpha), a.step, st);
+      const EtdCoef d = etd_coef(etd_c(a.hv, K2, kv3, -a.alpha), a.step, st);
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
         if (a.step == 1) RH[p] = fma(d.E, RH[p], d.A * RR[p]);
@@ -1184,7 +1227,7 @@ void launch_project(const Geo& g, void* u, hipStream_t s) {
 //  density (scalar_stats :221-248, buoyancy_flux density_stratification_types.jl:192-211):
 //  22 sum w|rho|^2  23..25 sum w k_i^2 |rho|^2  26 sum w proj(uz, rho)  29 Re rho(k=0)
 //  LES (les_stats :679-705): 27 sum w proj((ik.tau)_{x,y}, u_{x,y})  28 the same for z
-enum { kStatsN = 30, kStatsBlocks = 512, kStatsThreads = 128 };
+enum { kStatsN = 32, kStatsBlocks = 512, kStatsThreads = 128 };
 
 __device__ __forceinline__ double proj2(double ax, double ay, double bx, double by) { return ax * bx + ay * by; }
 
@@ -1216,6 +1259,12 @@ __global__ void __launch_bounds__(kStatsThreads) k_stats(Geo g, const C<T>* __re
     acc[2] += w * ez;
     acc[3] += w * K2 * (ex + ey);
     acc[4] += w * K2 * ez;
+    if (aux.hv.kind) {
+      const double kv[3] = {kx, ky, kz};
+      const double f = hv_weight(aux.hv, kv);
+      acc[30] += w * f * (ex + ey);
+      acc[31] += w * f * ez;
+    }
     acc[9] += w * kx * kx * ex;
     acc[10] += w * ky * ky * ex;
     acc[11] += w * kx * kx * ey;
@@ -1454,6 +1503,8 @@ void launch_spectrum(const Geo& g, const void* u, double* out_x, double* out_y0,
 //   rhonl  spectrum_rhonl (:237-247): proj((i k).flux, rho)
 //   forcing spectrum_forcing (:259-268): (proj(ux, fx) + proj(uy, fy)) / dt with
 //          f = factor u on the forced modes
+//   hvis   spectrum_hyperviscosity (:297-314): nu_h k^2M proj(u, u) (h = x + y, v = z), or
+//          spectrum_spectral_barrier (:270-295) with f(|k|) in place of nu_h k^2M
 // The nonlinear term is zero at k = 0 (rhs.c[1] = 0, models.jl:174).
 __device__ __forceinline__ double sproj(double ar, double ai, double br, double bi) { return fma(ar, br, ai * bi); }
 
@@ -1550,6 +1601,11 @@ __device__ __forceinline__ void spec_values(const Geo& g, const C<T>* __restrict
       if (akz >= a.pf_kz0 && akz <= 5) f = a.st->force_vert;
     }
   }
+  double fk = 0.0;
+  if (a.hv.kind == 1) fk = a.hv.nuh * hv_weight(a.hv, K);
+  else if (a.hv.kind == 2) fk = hv_weight(a.hv, K);
+  v[SQ_HVIS_H] = fk * (sproj(U[0][0], U[0][1], U[0][0], U[0][1]) + sproj(U[1][0], U[1][1], U[1][0], U[1][1]));
+  v[SQ_HVIS_V] = fk * sproj(U[2][0], U[2][1], U[2][0], U[2][1]);
   v[SQ_FORCING] = f == 0.0 ? 0.0
                            : (sproj(U[0][0], U[0][1], U[0][0] * f, U[0][1] * f) +
                               sproj(U[1][0], U[1][1], U[1][0] * f, U[1][1] * f)) / a.st->dt;

@@ -547,6 +547,13 @@ struct ffs_sim {
     if (p.forcing == FFS_FORCING_QF && !(p.alvelius_c != 0))
       throw FfsError(FFS_ERR_ARG, "QfForcing needs alveliusC != 0");
     if (p.precision != FFS_FP64 && p.precision != FFS_FP32) throw FfsError(FFS_ERR_ARG, "unknown precision");
+    if (p.hyperviscosity < FFS_HV_NONE || p.hyperviscosity > FFS_HV_BARRIER)
+      throw FfsError(FFS_ERR_ARG, "unknown hyperviscosity");
+    if (p.hyperviscosity == FFS_HV_HYPER && p.hv_m < 1) throw FfsError(FFS_ERR_ARG, "hyperViscosityM must be >= 1");
+    if (p.hyperviscosity == FFS_HV_BARRIER && p.integrator != FFS_ETD3)
+      // the AB3 right-hand side asks nuh() of the hyperviscosity type, which SpectralBarrier lacks
+      // (models.jl:298-300, hyperviscosity_types.jl:13-14)
+      throw FfsError(FFS_ERR_ARG, "spectralBarrier needs velocityTimeStep ETD3rdO");
     etd = (p.integrator == FFS_ETD3);
     les = (p.les_model == FFS_LES_SMAGORINSKY);
     dens = (p.density != 0);
@@ -773,6 +780,7 @@ struct ffs_sim {
     a.alpha = dens ? alpha : 0.0;
     a.gz = dens ? gz : 0.0;
     a.st = st;
+    a.hv = hv_args();
     launch("spectra", [&] { FFS_DISPATCH(launch_spectra, g, u, a, spec_part, spec_out, stream); });
     auto& sl = spec_slots[slot];
     HIPCHECK(hipMemcpyAsync(sl.host, spec_out, (size_t)kSpecQ * spectra_out_len(g) * sizeof(double),
@@ -818,7 +826,7 @@ struct ffs_sim {
         }
     }
     auto& sl = slots[slot];
-    StatsAux aux{les ? tau : nullptr, dens ? rho : nullptr, gz};
+    StatsAux aux{les ? tau : nullptr, dens ? rho : nullptr, gz, hv_args()};
     launch("stats", [&] { FFS_DISPATCH(launch_stats, g, u, rhs, ke, aux, partials, red, stream); });
     HIPCHECK(hipMemcpyAsync(sl.red, red, kStatsSlots * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipMemcpyAsync(sl.st, st, sizeof(DevState), hipMemcpyDeviceToHost, stream));
@@ -852,6 +860,14 @@ struct ffs_sim {
                nlh, nlv, pressh, pressv, -(nlh + pressh), (nlv + pressv)};
     double tdiss = e;
     std::vector<double> other;
+    double hvs[3] = {0, 0, 0};
+    if (p.hyperviscosity) {  // hyperviscosity_stats (stats.jl:532-561), spectral_barrier_stats: hdissh, hdissv, hdiss
+      const double f = p.hyperviscosity == FFS_HV_HYPER ? p.hv_nu : -1.0;
+      hvs[0] = f * a[30];
+      hvs[1] = f * a[31];
+      hvs[2] = hvs[0] + hvs[1];
+      tdiss += hvs[2];  // before the LES part (stats.jl:118-121)
+    }
     if (dens) {  // stats(::AbstractDensityStratification), density_stratification_types.jl:46-87
       const double rhodiss = alpha * (a[23] + a[24] + a[25]);
       other = {a[29], a[22], a[23], a[24], a[25], rhodiss, rhodiss, -gz * a[26]};
@@ -867,6 +883,7 @@ struct ffs_sim {
       r.stats.insert(r.stats.end(), {sd.force_hp, sd.force_vp});
       if (p.forcing != FFS_FORCING_A) r.stats.push_back(sd.force_hp + sd.force_vp);
     }
+    if (p.hyperviscosity) r.stats.insert(r.stats.end(), {hvs[0], hvs[1], hvs[2]});
     const double u1p2 = u12, u2p2 = u22, u3p2 = u32;
     const double uh = std::sqrt((u1p2 + u2p2) / 2);
     const double urms = std::sqrt((u1p2 + u2p2 + u3p2) / 3);
@@ -905,8 +922,21 @@ struct ffs_sim {
     return r;
   }
 
+  // SpectralBarrier defaults (simulationtypes.jl:294-296): initk = cutoff 2 KX[end] / 3, endk = sqrt(3 cutoff) KX[end]
+  HvArgs hv_args() const {
+    HvArgs h{};
+    h.kind = p.hyperviscosity;
+    h.m = p.hv_m;
+    h.nuh = p.hv_nu;
+    const double kxe = KX.empty() ? 0.0 : KX.back();
+    h.sb_i = p.sb_initk > 0 ? p.sb_initk : p.cutoff * 2 * kxe / 3;
+    h.sb_p = p.sb_endk > 0 ? p.sb_endk : std::sqrt(p.cutoff * 3) * kxe;
+    return h;
+  }
+
   UpdateArgs update_args(int step) const {
     UpdateArgs a{};
+    a.hv = hv_args();
     a.integrator = etd ? 1 : 0;
     a.step = step;
     a.nu = p.nu;
@@ -1326,6 +1356,10 @@ int ffs_params_default(ffs_params* p) {
   p->alvelius_p = 0.0;
   p->alvelius_c = 1.0;
   p->alvelius_kf = 0.0;
+  p->hyperviscosity = FFS_HV_NONE;
+  p->hv_nu = 0.0;
+  p->hv_m = 2;
+  p->sb_initk = p->sb_endk = 0.0;
   return FFS_OK;
 }
 
@@ -1611,6 +1645,7 @@ int ffs_stats_header(const ffs_sim* s, int table, char* buf, size_t cap) {
     if (s->dens) h += ",rho,rhop2,drhodxp2,drhodyp2,drhodzp2,rhodiss,trhodiss,bflux";
     if (s->les) h += ",prh,prv,praniso,pr";
     if (s->plane) h += s->p.forcing == FFS_FORCING_A ? ",hp,vp" : ",hp,vp,tp";
+    if (s->p.hyperviscosity) h += ",hdissh,hdissv,hdiss";
   } else if (table == FFS_TABLE_SCALES) {
     h = "iteration,time,Lk,Lt,lamg,laml,lamt,Lhh,Lvv,Lhv,Lvh,lam12,lam21,L11,L12,L13,L21,L22,L23,L31,L32,L33";
     h += s->dens ? ",Lb,Lo,Lom,Tk,Tt,Thh,Tb,Tbm" : ",Tk,Tt,Thh";
@@ -1665,7 +1700,7 @@ int ffs_spectrum(ffs_sim* s, double* h, int cap_h, int* n_h, double* v, int cap_
 
 static_assert(kSpecQ == FFS_SPEC_Q && SQ_U_H == FFS_SPEC_U_H && SQ_PRESS_V == FFS_SPEC_PRESS_V &&
                   SQ_LES_H == FFS_SPEC_LES_H && SQ_BUOY == FFS_SPEC_BUOYANCY && SQ_RHONL == FFS_SPEC_RHONL &&
-                  SQ_FORCING == FFS_SPEC_FORCING,
+                  SQ_FORCING == FFS_SPEC_FORCING && SQ_HVIS_H == FFS_SPEC_HVIS_H && SQ_HVIS_V == FFS_SPEC_HVIS_V,
               "the device's spectra quantities follow the FFS_SPEC_* order");
 
 int ffs_spectra_axes(ffs_sim* s, double* kh, int cap_h, int* n_h, double* krz, int cap_v, int* n_v) {

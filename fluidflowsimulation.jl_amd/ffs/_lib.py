@@ -24,6 +24,7 @@ FFS_FP64, FFS_FP32 = 0, 1
 FFS_LES_NONE, FFS_LES_SMAGORINSKY = 0, 1
 FFS_FORCING_NONE, FFS_FORCING_SHELL, FFS_FORCING_RF, FFS_FORCING_QF = 0, 1, 2, 3
 FFS_FORCING_CRF, FFS_FORCING_NRF, FFS_FORCING_CNRF, FFS_FORCING_A = 4, 5, 6, 7
+FFS_HV_NONE, FFS_HV_HYPER, FFS_HV_BARRIER = 0, 1, 2
 FFS_FSTATE_R, FFS_FSTATE_EM, FFS_FSTATE_KH, FFS_FSTATE_EF, FFS_FSTATE_N = 0, 1, 2, 3, 4
 FFS_FIELD_U1, FFS_FIELD_U2, FFS_FIELD_U3, FFS_FIELD_RHO = 0, 1, 2, 3
 FFS_SPACE_REAL, FFS_SPACE_FOURIER = 0, 1
@@ -32,7 +33,7 @@ FFS_TABLE_STATS, FFS_TABLE_SCALES, FFS_TABLE_NUMBERS = 0, 1, 2
 FFS_EVENT_NONE, FFS_EVENT_OUTPUT = 0, 1
 # the spectra quantities of a spectra record, in record order
 SPEC_QUANTITIES = ("u_h", "u_v", "vis_h", "vis_v", "nl_h", "nl_v", "press_h", "press_v", "les_h", "les_v",
-                   "buoyancy_v", "rho", "rhodiss", "rhonl", "forcing_h")
+                   "buoyancy_v", "rho", "rhodiss", "rhonl", "forcing_h", "hvis_h", "hvis_v")
 FFS_SPEC_Q = len(SPEC_QUANTITIES)
 
 
@@ -76,6 +77,11 @@ class FfsParams(ctypes.Structure):
         ("alvelius_p", ctypes.c_double),
         ("alvelius_c", ctypes.c_double),
         ("alvelius_kf", ctypes.c_double),
+        ("hyperviscosity", ctypes.c_int32),
+        ("hv_nu", ctypes.c_double),
+        ("hv_m", ctypes.c_int32),
+        ("sb_initk", ctypes.c_double),
+        ("sb_endk", ctypes.c_double),
     ]
 
 

@@ -164,6 +164,11 @@ class Params:
     alvelius_p: float = 0.0         # alveliusP, alveliusC, alveliusKf (QfForcing)
     alvelius_c: float = 1.0
     alvelius_kf: float = 0.0
+    hyperviscosity: str = None      # hyperViscosity: "spectralBarrier", or any other value: HyperViscosity
+    hv_nu: float = 0.0              # hyperViscosityCoefficient
+    hv_m: int = 2                   # hyperViscosityM
+    sb_initk: float = 0.0           # initk / endk of the spectral barrier (0: the reference's defaults)
+    sb_endk: float = 0.0
     extra: dict = field(default_factory=dict)
 
     def to_c(self):
@@ -201,6 +206,14 @@ class Params:
         else:
             p.forcing_zf_tanh = int(self.forcing_zf != "cutoff")
         p.alvelius_p, p.alvelius_c, p.alvelius_kf = self.alvelius_p, self.alvelius_c, self.alvelius_kf
+        if self.hyperviscosity is None:
+            p.hyperviscosity = _lib.FFS_HV_NONE
+        elif self.hyperviscosity == "spectralBarrier":
+            p.hyperviscosity = _lib.FFS_HV_BARRIER
+        else:
+            p.hyperviscosity = _lib.FFS_HV_HYPER
+        p.hv_nu, p.hv_m = self.hv_nu, self.hv_m
+        p.sb_initk, p.sb_endk = self.sb_initk, self.sb_endk
         return p
 
 
@@ -253,6 +266,15 @@ def parameters(d):
         p.density_gradient = parse_expr(req("densityGradient"))
         p.z_acceleration = parse_expr(req("zAcceleration"))
         p.reference_density = parse_expr(req("referenceDensity"))
+    if "hyperViscosity" in d:  # simulationtypes.jl:293-305
+        known.add("hyperViscosity")
+        p.hyperviscosity = d["hyperViscosity"]
+        if p.hyperviscosity == "spectralBarrier":
+            p.sb_initk = opt("initk", float, 0.0)
+            p.sb_endk = opt("endk", float, 0.0)
+        else:
+            p.hv_nu = float(req("hyperViscosityCoefficient"))
+            p.hv_m = opt("hyperViscosityM", int, 2)
     if "forcing" in d:
         known.add("forcing")
         p.forcing = d["forcing"]

@@ -236,6 +236,8 @@ class Simulation:
             have |= {"buoyancy_v", "rho", "rhodiss", "rhonl"}
         if self.params.forcing in PLANE_FORCINGS:
             have.add("forcing_h")
+        if self.params.hyperviscosity is not None:
+            have |= {"hvis_h", "hvis_v"}
         out = []
         while True:
             r = self._check(self._lib.ffs_spectra_pop(self._h, ctypes.byref(it), _dptr(buf), buf.size,
@@ -333,10 +335,10 @@ def footprint(params: Params):
 
 def spectra_files(q, params):
     """{file stem: (values, None)} of one spectra record: writespectrum's five files for u, vis, nl,
-    press (and les), the specH/specV pair of the single spectra, rhosource from buoyancy_v
+    press (and les, hvis), the specH/specV pair of the single spectra, rhosource from buoyancy_v
     (spec.jl:1-168)."""
     out = {}
-    for n in ("u", "vis", "nl", "press", "les"):
+    for n in ("u", "vis", "nl", "press", "les", "hvis"):
         if n + "_h" not in q:
             continue
         (hH, hV), (vH, vV) = q[n + "_h"], q[n + "_v"]

@@ -53,6 +53,11 @@ mutable struct ffs_params
     alvelius_p::Float64
     alvelius_c::Float64
     alvelius_kf::Float64
+    hyperviscosity::Int32
+    hv_nu::Float64
+    hv_m::Int32
+    sb_initk::Float64
+    sb_endk::Float64
     ffs_params() = new()
 end
 
@@ -100,6 +105,18 @@ function params_from_global(d::AbstractDict)
         p.density = Int32(1)
         p.pr, p.density_gradient = num(:Pr), num(:densityGradient)
         p.z_acceleration, p.reference_density = num(:zAcceleration), num(:referenceDensity)
+    end
+    # hyperviscosity (simulationtypes.jl:293-305)
+    if haskey(d, :hyperViscosity)
+        if d[:hyperViscosity] == "spectralBarrier"
+            p.hyperviscosity = Int32(2)
+            haskey(d, :initk) && (p.sb_initk = parse(Float64, d[:initk]))
+            haskey(d, :endk) && (p.sb_endk = parse(Float64, d[:endk]))
+        else
+            p.hyperviscosity = Int32(1)
+            p.hv_nu = parse(Float64, d[:hyperViscosityCoefficient])
+            haskey(d, :hyperViscosityM) && (p.hv_m = parse(Int32, d[:hyperViscosityM]))
+        end
     end
     # forcing_model (forcing_types.jl:338-446): the kz = 0-plane schemes
     codes = Dict("rfForcing" => 2, "QfForcing" => 3, "cRfForcing" => 4, "nRfForcing" => 5,
@@ -244,7 +261,7 @@ function spectra_axes(s::FFSSimulation)
 end
 
 const SPEC_QUANTITIES = ("u_h", "u_v", "vis_h", "vis_v", "nl_h", "nl_v", "press_h", "press_v", "les_h", "les_v",
-                         "buoyancy_v", "rho", "rhodiss", "rhonl", "forcing_h")
+                         "buoyancy_v", "rho", "rhodiss", "rhonl", "forcing_h", "hvis_h", "hvis_v")
 
 # write_spec / write_spec_forcing (spec.jl:1-168): the queued spectra records as the reference's files
 function writespectra(s::FFSSimulation)
@@ -260,8 +277,9 @@ function writespectra(s::FFSSimulation)
         i = it[]
         q = Dict(name => (buf[(k - 1) * (nh + nv) + 1:(k - 1) * (nh + nv) + nh],
                           buf[(k - 1) * (nh + nv) + nh + 1:k * (nh + nv)]) for (k, name) in enumerate(SPEC_QUANTITIES))
-        for n_ in ("u", "vis", "nl", "press", "les")
+        for n_ in ("u", "vis", "nl", "press", "les", "hvis")
             (n_ == "les" && p.les_model == 0) && continue
+            (n_ == "hvis" && p.hyperviscosity == 0) && continue
             (hH, hV), (vH, vV) = q[n_ * "_h"], q[n_ * "_v"]
             writedlm("$(n_)_h.specH.$i", zip(kh, hH)); writedlm("$(n_)_h.specV.$i", zip(krz, hV))
             writedlm("$(n_).specH.$i", zip(kh, hH .+ vH))

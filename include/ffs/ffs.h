@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FFS_ABI_VERSION 2
+#define FFS_ABI_VERSION 3
 
 /* error codes */
 #define FFS_OK 0
@@ -63,6 +63,11 @@ extern "C" {
 #define FFS_FORCING_NRF 5  /* nRfForcing  (nrforcing.jl: with the nonlinear transfer spectrum N) */
 #define FFS_FORCING_CNRF 6 /* cNRfForcing (cnrforcing.jl) */
 #define FFS_FORCING_A 7    /* aForcing    (aforcing.jl: one oscillator per kz = 0 mode, |k_h| <= kf) */
+
+/* hyperviscosity (Hyperviscosity/hyperviscosity_types.jl) */
+#define FFS_HV_NONE 0
+#define FFS_HV_HYPER 1   /* HyperViscosity{nu_h, M}: -nu_h k^2M beside -nu k^2 (AB3 rhs, ETD c) */
+#define FFS_HV_BARRIER 2 /* SpectralBarrier: ETD c += log(esp(|k|, initk, endk, 1/4)); ETD3rdO only */
 
 /* fields */
 #define FFS_FIELD_U1 0
@@ -117,6 +122,11 @@ typedef struct ffs_params {
   double forcing_alpha;                     /* alphac (rfForcing) */
   int32_t forcing_zf_tanh;                  /* Zf tanh (1) or cutoff (0, default) */
   double alvelius_p, alvelius_c, alvelius_kf; /* alveliusP, alveliusC, alveliusKf (QfForcing) */
+  int32_t hyperviscosity;                   /* FFS_HV_*: hyperViscosity (simulationtypes.jl:293-305) */
+  double hv_nu;                             /* hyperViscosityCoefficient nu_h */
+  int32_t hv_m;                             /* hyperViscosityM (2) */
+  double sb_initk, sb_endk;                 /* initk, endk of spectralBarrier; <= 0 selects the defaults
+                                               cutoff 2 KX[end] / 3 and sqrt(3 cutoff) KX[end] */
 } ffs_params;
 
 typedef struct ffs_sim ffs_sim;
@@ -212,7 +222,7 @@ int ffs_spectrum(ffs_sim* s, double* h, int cap_h, int* n_h, double* v, int cap_
  * calculate_specHV (spec.jl:424-450) into specH (n_h values over KH) then specV
  * (n_v values over KRZ), quantity after quantity in the order of FFS_SPEC_*.
  * Quantities a run does not have (LES, density, plane forcing) read zero.
- * The files: for u, vis, nl, press, les: n_h.specH/V (the _H quantity),
+ * The files: for u, vis, nl, press, les, hvis: n_h.specH/V (the _H quantity),
  * n_v.specH/V (the _V one) and n.specH (their specH sum); buoyancy_v,
  * rhosource (= -specH,V(buoyancy_v) / g_z * densityGradient), rho, rhodiss,
  * rhonl and forcing_h: .specH and .specV. */
@@ -231,7 +241,9 @@ int ffs_spectrum(ffs_sim* s, double* h, int cap_h, int* n_h, double* v, int cap_
 #define FFS_SPEC_RHODISS 12
 #define FFS_SPEC_RHONL 13
 #define FFS_SPEC_FORCING 14
-#define FFS_SPEC_Q 15
+#define FFS_SPEC_HVIS_H 15
+#define FFS_SPEC_HVIS_V 16
+#define FFS_SPEC_Q 17
 /* KH (n_h values) and KRZ (n_v values) of Globals.jl:148-152. */
 int ffs_spectra_axes(ffs_sim* s, double* kh, int cap_h, int* n_h, double* krz, int cap_v, int* n_v);
 /* Pops the oldest spectra record (waiting for it if it is still in flight):

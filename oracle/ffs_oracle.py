@@ -153,11 +153,13 @@ class OracleSim:
 
     def __init__(self, grid, nu, dt=0.0, variable_dt=False, cfl=0.18, dt_stat=1 << 30, dt_spec=1 << 30,
                  forcing_kf=None, forcing_power=0.1, start=0, start_time=0.0, integrator="AB3", les_c=None,
-                 les_delta=None, density=None, plane_forcing=None):
+                 les_delta=None, density=None, plane_forcing=None, hv=None):
         """integrator: "AB3" (Adams_Bashforth3rdO) or "ETD3" (ETD3rdO);
         les_c: Smagorinsky constant (None: no LES), les_delta: filter width
         (default (2 pi L/nx)/cutoff, les_types.jl:366);
-        density: dict(Pr=, gradient=, g=) for the Boussinesq approximation."""
+        density: dict(Pr=, gradient=, g=) for the Boussinesq approximation;
+        hv: dict(kind="hyper", nu=, m=) (HyperViscosity{nu_h, M}) or dict(kind="barrier",
+        initk=, endk=) (SpectralBarrier), Hyperviscosity/hyperviscosity_types.jl."""
         self.g = grid
         self.nu = nu
         self.integrator = integrator
@@ -182,6 +184,7 @@ class OracleSim:
         self.forcing_power = forcing_power
         self.force_factor = 0.0
         self.plane = plane_forcing  # oracle.plane_forcing.PlaneForcing (one of the kz = 0 plane forcings) or None
+        self.hv = hv
         self.umax = 0.0
         self.records = []
         self.spectra = []
@@ -270,7 +273,10 @@ class OracleSim:
             self.pressstats = self.pressure_stats(rhs)
         # fourierspacep2_velocity! (models.jl:292-341)
         if self.integrator == "AB3":
-            rhs = rhs + (g.K2 * (-self.nu)) * self.uh
+            if self.hv is not None:  # (K2 mnu + K2^M mnuh) v, models.jl:320-322
+                rhs = rhs + (g.K2 * (-self.nu) + self.ipow(g.K2, self.hv["m"]) * (-self.hv["nu"])) * self.uh
+            else:
+                rhs = rhs + (g.K2 * (-self.nu)) * self.uh
         if self.les_c is not None:
             rhs = rhs + self.div_tau()
         if self.density is not None:
@@ -402,13 +408,62 @@ class OracleSim:
                           (2 * em - l2 * (e1 - e2 * em) - l1 * (e3 - e4 * em)) / (l3 * e2))
             Bt = np.where(test, b1 * l1 + (b2 * l2 + b0), (e5 * l2 - 2 * em + l1 * (e3 - e6 * em)) / (l3 * e7))
             Ct = np.where(test, c1 * l1 + (c2 * l2 + c0), (2 * em - e8 * l2 - l1 * (e3 - dt2 * em)) / (l3 * e9))
-        return At, Bt, Ct
+        inf = l1 == -np.inf
+        return np.where(inf, 0.0, At), np.where(inf, 0.0, Bt), np.where(inf, 0.0, Ct)
+
+    def modk(self):
+        """muladd(KX, KX, muladd(KY, KY, KZ^2)) of init_c! and the hyperviscosity loops (unfused here
+        and on the device, so both round alike)."""
+        g = self.g
+        return g.kx * g.kx + (g.ky * g.ky + g.kz * g.kz)
+
+    @staticmethod
+    def ipow(x, m):
+        """x^m as m - 1 products (as the device forms k^2M)."""
+        r = x
+        for _ in range(1, m):
+            r = r * x
+        return r
+
+    def barrier(self, k):
+        """SpectralBarrier's f(k) = log(esp(k, initk, endk, 1/4)) (hyperviscosity_types.jl:44-48)."""
+        i, p = self.hv["initk"], self.hv["endk"]
+        ak = np.abs(k)
+        with np.errstate(all="ignore"):
+            e = np.where(ak <= i, 1.0, np.where(ak < p, np.exp(-((ak - i) ** 2) / (p ** 0.25 - ak ** 0.25)), 0.0))
+            return np.log(e)
+
+    def etd_c(self, mnu):
+        """init_c! (ETD.jl:56-95): c = -nu k^2, plus -nu_h k^2M (HyperViscosity) or f(|k|)
+        (SpectralBarrier, -Inf beyond endk)."""
+        g = self.g
+        if self.hv is None:
+            c = g.K2 * mnu
+        elif self.hv["kind"] == "hyper":
+            mk = self.modk()
+            c = mk * mnu + self.ipow(mk, self.hv["m"]) * (-self.hv["nu"])
+        else:
+            mk = self.modk()
+            c = mk * mnu + self.barrier(np.sqrt(mk))
+        return np.where(g.RANGE, c, 0.0)
+
+    def hv_weight(self):
+        """Per-mode factor of the hyperviscous dissipation: nu_h k^2M (hyperviscosity_stats,
+        stats.jl:532-561) or -f(|k|) with f = -Inf read as 0 (spectral_barrier_stats,
+        hyperviscosity_types.jl:50-79)."""
+        mk = self.modk()
+        if self.hv["kind"] == "hyper":
+            return self.hv["nu"] * self.ipow(mk, self.hv["m"])
+        f = self.barrier(np.sqrt(mk))
+        return -np.where(f == -np.inf, 0.0, f)
 
     def etd(self, u, rhs, n1, n2, mnu):
         """ETD3rdO functor (ETD.jl:229-247) with ETD1stO!/ETD2ndO! (ETD1stand2ndO.jl)
-        for the first two steps; c = -nu k^2 (init_c!, ETD.jl:60-69)."""
+        for the first two steps; c from init_c! (ETD.jl:56-95); c = -Inf gives zero
+        coefficients (the test2 branches)."""
         g = self.g
-        c = np.where(g.RANGE, g.K2 * mnu, 0.0)
+        c = self.etd_c(mnu)
+        inf = c == -np.inf
         dt = self.dt
         fm1, fm2 = getattr(self, n1), getattr(self, n2)
         with np.errstate(all="ignore"):
@@ -418,7 +473,7 @@ class OracleSim:
                 c2dt3 = cdt2 * cdt
                 c3dt4 = c2dt3 * cdt
                 test = -cdt <= 1e-4
-                f = np.where(test, dt + 0.5 * cdt2 + c2dt3 / 6 + c3dt4 / 24, np.expm1(cdt) / c)
+                f = np.where(inf, 0.0, np.where(test, dt + 0.5 * cdt2 + c2dt3 / 6 + c3dt4 / 24, np.expm1(cdt) / c))
                 out = np.exp(cdt) * u + f * rhs
             elif self.tit == 2:
                 dt2 = self.dt2
@@ -437,6 +492,8 @@ class OracleSim:
                               + (cp2 * dtp4) / (24 * dt2), (em - c * (dt - dt2 * em)) / c2dt2)
                 Bt = np.where(test, (-(dtp2 / (2 * dt2)) - (c * dtp3) / (6 * dt2) - (cp2 * dtp4) / (24 * dt2)),
                               (cdt - em) / c2dt2)
+                At = np.where(inf, 0.0, At)
+                Bt = np.where(inf, 0.0, Bt)
                 out = np.exp(cdt) * u + (At * rhs + Bt * fm1)
             else:
                 At, Bt, Ct = self.etd_coefficients(c)
@@ -540,6 +597,14 @@ class OracleSim:
             Lvv = L33
             Lvh = (L31 + L32) / 2
         other = []
+        hvstats = []
+        if self.hv is not None:  # hyperviscosity_stats / spectral_barrier_stats: hdissh, hdissv, hdiss
+            # stats.jl:118: tdiss takes the hyperviscous part before the LES part
+            wgt = np.where(g.RANGE, g.W * self.hv_weight(), 0.0)
+            hh = float(np.sum(wgt * (e[0] + e[1])))
+            hv_ = float(np.sum(wgt * e[2]))
+            hvstats = [hh, hv_, hh + hv_]
+            tdiss = tdiss + (hh + hv_)
         if self.density is not None:
             # stats(::AbstractDensityStratification) (density_stratification_types.jl:46-87)
             r = self.rhoh
@@ -569,6 +634,7 @@ class OracleSim:
             other += [self.plane.hp, self.plane.vp]
             if self.plane.kind != "a":
                 other.append(self.plane.hp + self.plane.vp)
+        other += hvstats
         stats_row = [self.iteration, self.time, *vstats, tdiss, *other]
         scales = [self.iteration, self.time, Lk, Lt, lg, ll, lt, Lhh, Lvv, Lhv, Lvh, l12, l21, L11, L12, L13, L21,
                   L22, L23, L31, L32, L33]
@@ -664,6 +730,15 @@ class OracleSim:
             o = [proj(d[c], u[c]) for c in range(3)]
             put("les_h", o[0] + o[1])
             put("les_v", o[2])
+        if self.hv is not None:
+            # spectrum_hyperviscosity (:297-314): nu_h k^2M vecouterproj(u, u); spectrum_spectral_barrier
+            # (:270-295): f(|k|) vecouterproj(u, u), f = -Inf read as 0
+            if self.hv["kind"] == "hyper":
+                fk = self.hv["nu"] * self.ipow(self.modk(), self.hv["m"])
+            else:
+                fk = -self.hv_weight()
+            put("hvis_h", fk * (proj(u[0], u[0]) + proj(u[1], u[1])))
+            put("hvis_v", fk * proj(u[2], u[2]))
         if self.density is not None:
             rho = self.rhoh
             put("buoyancy_v", proj(u[2], self.gz * rho))                     # spectrum_buoyancy (:249-257)
