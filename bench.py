@@ -65,8 +65,10 @@ def main():
     ap.add_argument("--fp32", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="fp32,ensemble,serve,strat",
-                    help="extra measurements after the main line: fp32, ensemble, serve, strat (comma separated)")
+    ap.add_argument("--legs", default="fp32,ensemble,serve,strat,features",
+                    help="extra measurements after the main line: fp32, ensemble, serve, strat, features "
+                         "(comma separated)")
+    ap.add_argument("--feature-steps", type=int, default=100)
     ap.add_argument("--strat-n", type=int, default=512)
     ap.add_argument("--strat-steps", type=int, default=500)
     ap.add_argument("--fp32-batch", type=int, default=2)
@@ -299,6 +301,8 @@ def bench(args):
         out["serving"] = leg_serve(args)
     if "strat" in legs:
         out["stratified"] = leg_strat(args)
+    if "features" in legs:
+        out["features"] = leg_features(args, n, nu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, nu, args)
     print(json.dumps(out), flush=True)
@@ -512,6 +516,57 @@ def leg_strat(args):
     out["fp32_vs_fp64_value"] = out["fp32"]["value"] / out["fp64"]["value"]
     log(f"stratified fp32 vs fp64: energy {out['energy_rel_err']:.2e}, potential energy "
         f"{out['potential_energy_rel_err']:.2e}, spectrum {out['spectrum_rel_err']:.2e}")
+    return out
+
+
+def leg_features(args, n, nu):
+    """Step cost of the options beyond the plain AB3 step, at the main workload's grid (FP64, seeded
+    field, variable dt, graph replay): ETD3, hyperviscosity, the spectral barrier, three plane forcing
+    schemes, Smagorinsky; and the device time of one spectra record (the 17 write_spec quantities and
+    their marginals) at a spectra iteration, from HIP events around its launches."""
+    import ffs
+
+    base = dict(nx=n, ny=n, nz=n, nu=nu, dt=0.0, integrator="Adams_Bashforth3rdO", variable_dt=True, cfl=0.5,
+                dt_stat=1 << 30, dt_spec=1 << 30, write_time=1 << 30)
+    cases = {
+        "ab3": {},
+        "etd3": dict(integrator="ETD3rdO", cfl=0.4),
+        "etd3 hyperviscosity m=2": dict(integrator="ETD3rdO", cfl=0.4, hyperviscosity="hyperViscosity",
+                                        hv_nu=1e-9, hv_m=2),
+        "etd3 spectral barrier": dict(integrator="ETD3rdO", cfl=0.4, hyperviscosity="spectralBarrier"),
+        "ab3 rfForcing": dict(forcing="rfForcing", forcing_kf=2.5, forcing_tf=1.0, forcing_alpha=0.5),
+        "ab3 nRfForcing": dict(forcing="nRfForcing", forcing_kf=2.5, forcing_tf=1.0, forcing_alpha=0.5),
+        "ab3 aForcing": dict(forcing="aForcing", forcing_kf=3.0, forcing_tf=1.0, forcing_alpha=0.5),
+        "ab3 smagorinsky": dict(les_model="Smagorinsky", smagorinsky_c=0.16),
+    }
+    out = {"n": n, "steps": args.feature_steps, "ms_per_step": {}}
+    for name, kw in cases.items():
+        p = ffs.Params(**dict(base, **kw))
+        with ffs.Simulation(p) as s:
+            s.seed_velocity(args.seed, k0=4.0, energy=0.5)
+            s.initialize()
+            s.advance_in_time(10)
+            el = timed_steps(s, args.feature_steps)
+            out["ms_per_step"][name] = 1e3 * el / args.feature_steps
+            log(f"features {n}^3 {name}: {out['ms_per_step'][name]:.3f} ms/step")
+    p = ffs.Params(**dict(base, dt_spec=5, les_model="Smagorinsky", smagorinsky_c=0.16,
+                          forcing="rfForcing", forcing_kf=2.5, forcing_tf=1.0, forcing_alpha=0.5))
+    with ffs.Simulation(p) as s:
+        s.seed_velocity(args.seed, k0=4.0, energy=0.5)
+        s.initialize()
+        s.advance_in_time(10)
+        s.pop_spectra()
+        s.profile(True)
+        s.profile_reset()
+        s.advance_in_time(20)
+        s.synchronize()
+        prof = s.profile_get()
+        s.profile(False)
+        recs = s.pop_spectra()
+        ms, cnt = prof.get("spectra", (0.0, 0))
+        out["spectra_record_ms"] = ms / max(cnt, 1)
+        out["spectra_records"] = len(recs)
+        log(f"features {n}^3: one spectra record {out['spectra_record_ms']:.3f} ms device time ({cnt} records)")
     return out
 
 

@@ -79,6 +79,13 @@ struct PlaneForcing {
   const int* eslot;      // its slot
   const double* ew;      // its weight (1 at kx = 0, else 2)
   const double* ek2;     // its |k_h|^2 (the nu k^2 of calculate_nn_spectrum!)
+  const int* slot_ptr;   // elements of slot s: slot_el[slot_ptr[s] .. slot_ptr[s + 1]) (list order)
+  const int* slot_el;
+  // every 2D shell (the Ef of the reference, written as Eh.N): its retained kz = 0 modes
+  int nshell;            // 0 for aForcing
+  const int* shell_ptr;  // modes of shell n: shell_el[shell_ptr[n] .. shell_ptr[n + 1]) (y outer, x inner)
+  const int* shell_el;   // compact index
+  double* Ef_all;        // per shell
   int nslot;
   double* R;             // per slot: oscillator state, or the power spectrum (QfForcing)
   double* Em;            // per slot: target spectrum

@@ -1830,8 +1830,8 @@ __global__ void k_set_dt(DevState* st, int it, int tit, double cfl, double dx, i
 }
 
 // kz = 0-plane forcing, one block (Forcing_methods/forcing.jl, cforcing.jl, nrforcing.jl,
-// cnrforcing.jl, aforcing.jl, qfforcing.jl).  Thread s < nslot sums the spectra of its slot over the
-// element list (in list order): Ef (and N for nRf / cNRf, with the final rhs formed per element as
+// cnrforcing.jl, aforcing.jl, qfforcing.jl).  Thread s < nslot sums the spectra of its slot over its
+// elements (in list order, from the per-slot index): Ef (and N for nRf / cNRf, with the final rhs formed per element as
 // the update kernel forms it); thread 0 then runs the reference's serial part: the oscillator R, the
 // factors, the horizontal power hp, the vertical factor and power vp.  Runs after set_dt.
 __device__ __forceinline__ double proj_c(double ar, double ai, double br, double bi) { return fma(ar, br, ai * bi); }
@@ -1846,8 +1846,8 @@ __global__ void k_plane_forcing(Geo g, const C<T>* __restrict__ u, const C<T>* _
   const bool with_n = kind == PF_NRF || kind == PF_CNRF;
   for (int sl = threadIdx.x; sl < pf.nslot; sl += blockDim.x) {
     double e = 0.0, nn = 0.0;
-    for (int i = 0; i < pf.nel; ++i) {
-      if (pf.eslot[i] != sl) continue;
+    for (int jj = pf.slot_ptr[sl]; jj < pf.slot_ptr[sl + 1]; ++jj) {
+      const int i = pf.slot_el[jj];
       const int idx = pf.elem[i];
       const C<T> a = u1[idx], b = u2[idx];
       const double m2 = (double)a.x * a.x + (double)a.y * a.y + ((double)b.x * b.x + (double)b.y * b.y);
@@ -1878,6 +1878,19 @@ __global__ void k_plane_forcing(Geo g, const C<T>* __restrict__ u, const C<T>* _
       pf.N[sl] = nn;
     }
     if (pf.kind < 0) pf.Em[sl] = e;  // initialize!: the target is the initial spectrum
+  }
+  // calculate_u1u2_spectrum! (u1u2_mag for QfForcing) over every shell: the Ef the reference keeps
+  for (int n = threadIdx.x; n < pf.nshell; n += blockDim.x) {
+    double e = 0.0;
+    for (int jj = pf.shell_ptr[n]; jj < pf.shell_ptr[n + 1]; ++jj) {
+      const int idx = pf.shell_el[jj];
+      const int cx = idx % g.sx;
+      const double w = cx == 0 ? 1.0 : 2.0;
+      const C<T> a = u1[idx], b = u2[idx];
+      const double m2 = (double)a.x * a.x + (double)a.y * a.y + ((double)b.x * b.x + (double)b.y * b.y);
+      e += kind == PF_QF ? w * sqrt(m2) : 0.5 * w * m2;
+    }
+    pf.Ef_all[n] = e;
   }
   __syncthreads();
   if (threadIdx.x != 0 || pf.kind < 0) return;

@@ -501,6 +501,37 @@ struct ffs_sim {
     pfd.eslot = upload(one(eslot));
     pfd.ew = upload(one(ew));
     pfd.ek2 = upload(one(ek2));
+    // per-slot element index (counting sort keeps the list order within a slot)
+    std::vector<int> sptr(nslot + 1, 0), sel(elem.size());
+    for (int sl : eslot) sptr[sl + 1] += 1;
+    for (int i = 0; i < nslot; ++i) sptr[i + 1] += sptr[i];
+    {
+      std::vector<int> fill(sptr.begin(), sptr.end() - 1);
+      for (int i = 0; i < (int)eslot.size(); ++i) sel[fill[eslot[i]]++] = i;
+    }
+    pfd.slot_ptr = upload(sptr);
+    pfd.slot_el = upload(one(sel));
+    // every shell's retained kz = 0 modes, in the reference's loop order (y outer, x inner)
+    if (!per_mode) {
+      std::vector<std::vector<int>> by(nsh);
+      for (int cy = 0; cy < g.my; ++cy)
+        for (int cx = 0; cx < g.mx; ++cx) {
+          const double k = std::sqrt(kxc[cx] * kxc[cx] + kyc[cy] * kyc[cy]);
+          const int nn = (int)std::nearbyint(k / maxdk);
+          if (nn < nsh) by[nn].push_back(cx + g.sx * cy);
+        }
+      std::vector<int> hptr(nsh + 1, 0), hel;
+      for (int nn = 0; nn < nsh; ++nn) {
+        hel.insert(hel.end(), by[nn].begin(), by[nn].end());
+        hptr[nn + 1] = (int)hel.size();
+      }
+      pfd.nshell = nsh;
+      pfd.shell_ptr = upload(hptr);
+      pfd.shell_el = upload(one(hel));
+      pfd.Ef_all = upload(std::vector<double>(nsh, 0.0));
+    } else {
+      pfd.nshell = 0;
+    }
     pfd.nslot = nslot;
     pfd.R = upload(R);
     pfd.Em = upload(zeros);
@@ -1566,12 +1597,20 @@ int ffs_forcing_state(ffs_sim* s, int what, int set, double* data, int cap, int*
       for (int i = 0; i < len && i < cap; ++i) data[i] = s->pf_avgk[i];
       return;
     }
+    if (what == FFS_FSTATE_EF) {  // the last forcing call's spectrum over every shell
+      if (set) throw FfsError(FFS_ERR_ARG, "the current spectrum is read only");
+      if (s->p.forcing == FFS_FORCING_A) throw FfsError(FFS_ERR_ARG, "aForcing keeps no shell spectrum");
+      std::vector<double> e(len);
+      HIPCHECK(hipStreamSynchronize(s->stream));
+      HIPCHECK(hipMemcpy(e.data(), s->pfd.Ef_all, len * sizeof(double), hipMemcpyDeviceToHost));
+      for (int i = 0; i < len && i < cap; ++i) data[i] = e[i];
+      return;
+    }
     double* dev = what == FFS_FSTATE_R ? s->pfd.R : what == FFS_FSTATE_EM ? s->pfd.Em
-                : what == FFS_FSTATE_EF ? s->pfd.Ef : what == FFS_FSTATE_N ? s->pfd.N : nullptr;
+                : what == FFS_FSTATE_N ? s->pfd.N : nullptr;
     if (!dev) throw FfsError(FFS_ERR_ARG, "unknown forcing state");
     std::vector<double> slots(nslot);
     if (set) {
-      if (what == FFS_FSTATE_EF) throw FfsError(FFS_ERR_ARG, "the current spectrum is read only");
       if (cap < len) throw FfsError(FFS_ERR_ARG, "the state needs one value per shell (aForcing: per mode)");
       for (int sl = 0; sl < nslot; ++sl) slots[sl] = data[s->pf_state_pos(sl)];
       if (what == FFS_FSTATE_N && !s->initialized) {

@@ -318,10 +318,25 @@ class Simulation:
         return recs
 
     def writeoutput(self, directory="."):
-        """writeoutput (output.jl:1-8): u1.$it u2.$it u3.$it, padded real."""
+        """writeoutput (output.jl:1-67): u1.$it u2.$it u3.$it (padded real), rho.$it with density, and
+        the forcing state: R.$it (the oscillators), N.$it (nRf, cNRf), Eh.$it = (mean |k_h|, Ef)
+        rows (rf, cRf, nRf, cNRf)."""
         it = self.iteration
         for f in range(3):
             write_field(os.path.join(directory, f"u{f + 1}.{it}"), self.get_field(f, _lib.FFS_SPACE_REAL))
+        if self.params.density:
+            write_field(os.path.join(directory, f"rho.{it}"), self.get_field(_lib.FFS_FIELD_RHO, _lib.FFS_SPACE_REAL))
+        forcing = self.params.forcing
+        if forcing in RF_FAMILY:
+            R = self.forcing_state(_lib.FFS_FSTATE_R)
+            if forcing == "aForcing":  # the (NX, NY) matrix, one row per kx
+                R = R.reshape(self.params.ny, -1).T
+            _writedlm_values(os.path.join(directory, f"R.{it}"), R)
+        if forcing in ("nRfForcing", "cNRfForcing"):
+            _writedlm_values(os.path.join(directory, f"N.{it}"), self.forcing_state(_lib.FFS_FSTATE_N))
+        if forcing in ("rfForcing", "cRfForcing", "nRfForcing", "cNRfForcing"):
+            _writedlm(os.path.join(directory, f"Eh.{it}"), self.forcing_state(_lib.FFS_FSTATE_KH),
+                      self.forcing_state(_lib.FFS_FSTATE_EF))
 
 
 def footprint(params: Params):
@@ -360,6 +375,14 @@ def spectra_files(q, params):
             out[f"{n}.specH"] = (q[n][0], None)
             out[f"{n}.specV"] = (q[n][1], None)
     return out
+
+
+def _writedlm_values(path, a):
+    """writedlm of a vector (one value per line) or a matrix (one row per line, tab separated)."""
+    a = np.asarray(a, dtype=np.float64)
+    with open(path, "w") as f:
+        for row in (a.reshape(-1, 1) if a.ndim == 1 else a):
+            f.write("\t".join(jl_float(x) for x in row) + "\n")
 
 
 def _writedlm(path, k, v):
@@ -412,6 +435,9 @@ def run_simulation(directory=".", params=None, nt=None):
     nx, ny, nz = params.nx, params.ny, params.nz
     for f in range(3):
         s.set_field(f, read_field(os.path.join(directory, f"u{f + 1}.{params.start}"), nx, ny, nz))
+    rho_file = os.path.join(directory, f"rho.{params.start}")
+    if params.density and os.path.exists(rho_file):  # zero density without one (density_stratification_types.jl:166)
+        s.set_field(_lib.FFS_FIELD_RHO, read_field(rho_file, nx, ny, nz))
     _load_forcing_files(s, directory, params)
     if params.start == 0:
         s.writeheader(directory)
@@ -434,7 +460,7 @@ def run_simulation(directory=".", params=None, nt=None):
     if s.iteration % params.write_time != 0:
         s.writeoutput(directory)
     if params.forcing == "rfForcing":  # writedlm("R.$it", s.forcing.R), FluidFlowSimulation.jl:59-61
-        np.savetxt(os.path.join(directory, f"R.{s.iteration}"), s.forcing_state(_lib.FFS_FSTATE_R))
+        _writedlm_values(os.path.join(directory, f"R.{s.iteration}"), s.forcing_state(_lib.FFS_FSTATE_R))
     return s
 
 

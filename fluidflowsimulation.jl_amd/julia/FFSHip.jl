@@ -379,6 +379,16 @@ function writeoutput(s::FFSSimulation)
     for f in 0:2
         write("u$(f + 1).$it", get_field(s, f))
     end
+    s.p.density != 0 && write("rho.$it", get_field(s, 3))
+    # the forcing state files (output.jl:44-66): R, N (nRf, cNRf), Eh = (mean |k_h|, Ef) (rf, cRf, nRf, cNRf)
+    f = s.p.forcing
+    if f == 2 || f >= 4
+        R = forcing_state(s, 0)
+        f == 7 && (R = reshape(R, :, Int(s.p.ny)))   # aForcing: the (NX, NY) matrix
+        writedlm("R.$it", R)
+    end
+    (f == 5 || f == 6) && writedlm("N.$it", forcing_state(s, 4))
+    (f == 2 || f == 4 || f == 5 || f == 6) && writedlm("Eh.$it", zip(forcing_state(s, 2), forcing_state(s, 3)))
 end
 
 # run_simulation() (FluidFlowSimulation.jl:23-65) over the library
@@ -389,6 +399,11 @@ function run_simulation(d::AbstractDict)
         a = Vector{Float64}(undef, filesize("u$(f + 1).$(p.start)") ÷ 8)
         read!("u$(f + 1).$(p.start)", a)
         set_field!(s, f, a)
+    end
+    if p.density != 0 && isfile("rho.$(p.start)")   # zero density without one (density_stratification_types.jl:166)
+        a = Vector{Float64}(undef, filesize("rho.$(p.start)") ÷ 8)
+        read!("rho.$(p.start)", a)
+        set_field!(s, 3, a)
     end
     nt = haskey(d, :nt) ? parse(Int, d[:nt]) : typemax(Int) - p.start
     if p.start == 0

@@ -183,7 +183,8 @@ int ffs_checkpoint_load(ffs_sim* s, const void* host, size_t n);
  * and forcing_types.jl:66-74) or the fixed power spectrum (QfForcing), FFS_FSTATE_EM the target
  * spectrum (taken from the initial field at ffs_initialize unless set before, as from
  * targSpectrum.dat, forcing_types.jl:76-83), FFS_FSTATE_KH the shells' mean |k_h| (read only),
- * FFS_FSTATE_EF the last computed horizontal spectrum of the forced shells (read only).
+ * FFS_FSTATE_EF the spectrum of the last forcing call over every shell (calculate_u1u2_spectrum!,
+ * u1u2_mag for QfForcing; the values of the `Eh.N` file, output.jl:44-66; read only).
  * aForcing keeps one value per kz = 0 mode instead: an (nx/2+1) x ny array, x fastest (the
  * reference's R matrix), and has no FFS_FSTATE_KH.
  * set = 1 writes data[0..n), set = 0 reads; *n receives the number of values. */

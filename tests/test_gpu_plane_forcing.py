@@ -156,6 +156,10 @@ def test_plane_forcing_matches_oracle(kind, les, target, dens):
         assert err < tol, (err, tol)
         if dens:
             assert rel(s.density_hat(), ref.rhoh) < tol
+        if kind != "a":  # Ef over every shell (the Eh.N values) from the last forcing call
+            ef = s.forcing_state(_lib.FFS_FSTATE_EF)
+            assert ef.size == pf.nshells
+            assert np.allclose(ef, pf.Ef, rtol=max(1e-12, 1e2 * tol), atol=1e-14 * np.abs(pf.Ef).max())
         R = s.forcing_state(_lib.FFS_FSTATE_R)
         rtol = 1e-11 if tol == 1e-12 else 1e3 * tol
         assert np.allclose(R, flat(pf.R), rtol=rtol, atol=rtol * max(np.abs(pf.R).max(), 1e-300)), (R, pf.R)
@@ -199,3 +203,60 @@ def test_plane_forcing_graph_replay_and_resume_bitwise(kind):
             for what in (_lib.FFS_FSTATE_R, _lib.FFS_FSTATE_EM) + ((_lib.FFS_FSTATE_N,) if kind == "cnrf" else ()):
                 assert np.array_equal(s.forcing_state(what), a.forcing_state(what))
         d.close()
+
+
+def test_run_simulation_writes_the_forcing_files(tmp_path):
+    """writeoutput's forcing and density files (output.jl:1-67) from run_simulation in a directory:
+    R.N and N.N hold the state of the forcing call before iteration N, Eh.N its spectrum over every
+    shell next to the shells' mean |k_h|, rho.N the density."""
+    import os
+    from ffs import fields
+    from ffs.simulation import jl_float
+
+    print_case("run_simulation with nRfForcing and density: forcing state files")
+    d = str(tmp_path)
+    n = 32
+    entries = {"nx": n, "ny": n, "nz": n, "xDomainSize": "1.0", "yDomainSize": "1.0", "zDomainSize": "1.0",
+               "kinematicViscosity": 2e-3, "velocityTimeStep": "Adams_Bashforth3rdO", "variableTimestepFlag": "true",
+               "cfl": 0.4, "dtStat": 2, "dtSpec": 1000, "writeTime": 3, "nt": 7, "forcing": "nRfForcing",
+               "TF": 1.0, "alphac": 0.5, "kf": 2.5, "densityStratification": "true", "Pr": 0.7,
+               "densityGradient": 1.0, "zAcceleration": -1.0, "referenceDensity": 1.0}
+    with open(os.path.join(d, "global"), "w") as f:
+        for k, v in entries.items():
+            f.write(f"{k} {v}\n")
+    u0 = random_spectrum(n, seed=12, k0=3.0)
+    padded = 2 * (n // 2 + 1)
+    for c in range(3):
+        a = np.zeros((padded, n, n))
+        a[:n] = u0[c]
+        fields.write_field(os.path.join(d, f"u{c + 1}.0"), a)
+    r = np.zeros((padded, n, n))
+    r[:n] = random_scalar(n, n, n, seed=13)
+    fields.write_field(os.path.join(d, "rho.0"), r)
+    with ffs.run_simulation(d) as s:
+        kh = s.forcing_state(_lib.FFS_FSTATE_KH)
+        for it in (3, 6):
+            for name in ("R", "N", "Eh", "rho", "u1"):
+                assert os.path.exists(os.path.join(d, f"{name}.{it}")), (name, it)
+            R = np.loadtxt(os.path.join(d, f"R.{it}"))
+            N = np.loadtxt(os.path.join(d, f"N.{it}"))
+            assert R.shape == N.shape == kh.shape
+            eh = [line.split("\t") for line in open(os.path.join(d, f"Eh.{it}")).read().splitlines()]
+            assert [a for a, _ in eh] == [jl_float(x) for x in kh]   # NaN for an empty shell, as Julia prints it
+            assert all(float(b) >= 0 for _, b in eh)
+        # the closing writeoutput at iteration 7 (7 % writeTime != 0) writes the state files too
+        assert s.iteration == 7 and os.path.exists(os.path.join(d, "N.7"))
+    # the N.6 file is the state after the forcing call at iteration 5: rerun to 6 and compare
+    with ffs.Simulation(ffs.config.parameters(ffs.config.readglobal(os.path.join(d, "global")))) as t:
+        for c in range(3):
+            t.set_field(c, fields.read_field(os.path.join(d, f"u{c + 1}.0"), n, n, n))
+        t.set_field(_lib.FFS_FIELD_RHO, fields.read_field(os.path.join(d, "rho.0"), n, n, n))
+        t.initialize()
+        done = 0
+        while done < 6:
+            k, _ = t.advance(6 - done)
+            done += k
+        assert t.iteration == 6
+        assert np.array_equal(np.loadtxt(os.path.join(d, "N.6")), t.forcing_state(_lib.FFS_FSTATE_N))
+        rho6 = t.get_field(_lib.FFS_FIELD_RHO, _lib.FFS_SPACE_REAL)
+        assert np.array_equal(fields.read_field(os.path.join(d, "rho.6"), n, n, n), rho6)
