@@ -100,7 +100,7 @@ struct PlaneForcing {
   double tf, alpha, cfl, nu;
   int variable_dt;
 };
-enum : int { kStatsSlots = 32 };
+enum : int { kStatsSlots = 40 };
 
 struct Tables {  // per-length twiddle tables (device pointers), by precision
   const void* tw;
@@ -111,6 +111,7 @@ enum : int { kMaxFields = 16 };
 // extra operands of the column kernels
 template <typename T> struct ColAux {
   const typename std::conditional<sizeof(T) == 8, double2, float2>::type* rho = nullptr;  // inverse z: density
+  const typename std::conditional<sizeof(T) == 8, double2, float2>::type* phi = nullptr;  // and passive scalar
   typename std::conditional<sizeof(T) == 8, double2, float2>::type* dst[kMaxFields] = {};  // forward z outputs
 };
 
@@ -119,13 +120,14 @@ template <typename T> struct ColAux {
 // (u, w, [rho], [S]) are formed on the fly from u (and rho)
 template <typename T>
 void launch_inv_z(const Geo& g, const void* u, void* a1, int nf, bool curl, const void* tw, hipStream_t s,
-                  const void* rho = nullptr);
+                  const void* rho = nullptr, const void* phi = nullptr);
 template <typename T>
 void launch_inv_y(const Geo& g, const void* a1, void* a2, int nf, const void* tw, hipStream_t s);
 // field set of the fused x pass
 struct XOps {
-  int nin, nout;      // physical inputs (6 [+1 rho] [+5 S]) and forward outputs (3 [+3] [+5] [+1 ke])
-  int rho, les, ke;   // flags, in that order after u x w
+  int nin, nout;      // physical inputs (6 [+1 rho] [+1 phi] [+5 S]) and forward outputs (3 [+3] [+3] [+5] [+1 ke])
+  int rho, les, ke;   // flags: rho, phi and les in that order after u x w, ke last
+  int phi;
   double les_2alpha;  // 2 (c Delta)^2 of the Smagorinsky stress
 };
 template <typename T>
@@ -171,6 +173,15 @@ struct UpdateArgs {
   double gz;           // g_z = zAcceleration / referenceDensity
   double* time;        // s.time[] (DevState::time): advanced by dt once per step, or null
   HvArgs hv;
+  // passive scalar (PassiveScalar, simulationtypes.jl:198-224; models.jl:353-357, 470)
+  void* phi;           // updated in place, or null
+  void* prhs;          // its rhs out
+  const void* pfm1;    // its history
+  const void* pfm2;
+  const void* pflux;   // F(-phi u), 3 compact components
+  double palpha;       // diffusivity nu / scalarPr (ETD only)
+  double pgrad;        // scalarGradient
+  int pdir;            // scalarDirection: 0 x, 1 y, 2 z
 };
 template <typename T>
 void launch_update(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
@@ -189,6 +200,7 @@ struct StatsAux {
   const void* rho;  // density or null
   double gz;        // g_z for the buoyancy term of the pressure statistics
   HvArgs hv;        // hyperviscous dissipation sums (slots 30, 31)
+  const void* phi;  // passive scalar or null (slots 32-36)
 };
 template <typename T>
 int launch_stats(const Geo& g, const void* u, const void* nl, const void* ke, const StatsAux& aux, double* partials,

@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
     const int cy = b;
     // SRC variant: the block loops over the nin physical-space inputs of the
     // nonlinear term -- u (0..2), w = ik x u (3..5) (fourierspacep1!,
-    // models.jl:88-95), rho (if aux.rho), S = sym(ik (x) u) (xx,xy,xz,yy,yz; if
+    // models.jl:88-95), rho (if aux.rho), phi (if aux.phi), S = sym(ik (x) u) (xx,xy,xz,yy,yz; if
     // aux.les) -- so the u tile is fetched from HBM once and re-read from L2
     const int f0 = SPLIT ? bf : (CURL ? 0 : blockIdx.z);
     const int f1 = SPLIT ? bf + 1 : (CURL ? nzero : blockIdx.z + 1);
@@ -231,6 +231,10 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
         int e = f - 6;
         if (aux.rho) {
           if (e == 0) return aux.rho[idx];
+          --e;
+        }
+        if (aux.phi) {
+          if (e == 0) return aux.phi[idx];
           --e;
         }
         // S_ab = (i k_a u_b + i k_b u_a) / 2 for ab = xx, xy, xz, yy, yz (symouter, models.jl:98)
@@ -331,9 +335,10 @@ static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, con
 
 template <typename T>
 void launch_inv_z(const Geo& g, const void* u, void* a1, int nf, bool curl, const void* tw, hipStream_t s,
-                  const void* rho) {
+                  const void* rho, const void* phi) {
   ColAux<T> aux{};
   aux.rho = (const C<T>*)rho;
+  aux.phi = (const C<T>*)phi;
   switch (g.nz) {
 #define X(n)                                                                                    \
   case n:                                                                                       \
@@ -475,6 +480,12 @@ __global__ void __launch_bounds__(256) k_x_gen(Geo g, const C<T>* __restrict__ a
       out[o++] = (-rho) * uy;
       out[o++] = (-rho) * uz;
       if (row0 + r < nrows) rmax = nanmax(rmax, (double)fabs(rho));
+    }
+    if (ops.phi) {  // -phi u (the passive scalar enters no time-step bound)
+      const T phi = in[i++];
+      out[o++] = (-phi) * ux;
+      out[o++] = (-phi) * uy;
+      out[o++] = (-phi) * uz;
     }
     if (ops.les) {
       const T sxx = in[i], sxy = in[i + 1], sxz = in[i + 2], syy = in[i + 3], syz = in[i + 4];
@@ -831,19 +842,24 @@ static void launch_xreg(const Geo& g, const void* a2, void* b2, const void* tw, 
 template <typename T, int N>
 static void run_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
                      unsigned long long* rhomax, const void* tw, hipStream_t s) {
-  const bool plain = !ops.rho && !ops.les;
+  const bool plain = !ops.rho && !ops.les && !ops.phi;
   if constexpr (xreg_ok<T, N>()) {
     if (FFS_XNL_MODES || plain) {
       // u x w (+ 1/2|u|^2, always the last output slot), then -rho u and tau as launches of their own
       XRegArgs xa{0, 0, ops.nout - 1, 0.0};
       if (ops.ke) launch_xreg<T, N, 0, true>(g, a2, b2, tw, umax, xa, s);
       else launch_xreg<T, N, 0, false>(g, a2, b2, tw, umax, xa, s);
+      const int ns = (ops.rho ? 1 : 0) + (ops.phi ? 1 : 0);
       if (ops.rho) {
         XRegArgs xr{6, 3, 0, 0.0};
         launch_xreg<T, N, 2, false>(g, a2, b2, tw, rhomax, xr, s);
       }
+      if (ops.phi) {  // -phi u through the same kernel, without a maximum
+        XRegArgs xp{6 + (ops.rho ? 1 : 0), 3 + (ops.rho ? 3 : 0), 0, 0.0};
+        launch_xreg<T, N, 2, false>(g, a2, b2, tw, nullptr, xp, s);
+      }
       if (ops.les) {
-        XRegArgs xl{6 + (ops.rho ? 1 : 0), 3 + (ops.rho ? 3 : 0), 0, ops.les_2alpha};
+        XRegArgs xl{6 + ns, 3 + 3 * ns, 0, ops.les_2alpha};
         launch_xreg<T, N, 1, false>(g, a2, b2, tw, nullptr, xl, s);
       }
       return;
@@ -1097,6 +1113,22 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
     RR[1] = kf_r + (-a.drhodz) * U[2][1];
     ((C<T>*)a.rrhs)[idx] = mk<T>((T)RR[0], (T)RR[1]);
   }
+  // passive scalar rhs: div!(rhs, flux, u_dir, -scalarGradient) (models.jl:353-355) from the pre-step u
+  double PR[2] = {0, 0}, PH[2] = {0, 0};
+  if (a.phi) {
+    const C<T> p0 = ((const C<T>*)a.phi)[idx];
+    PH[0] = p0.x;
+    PH[1] = p0.y;
+    const C<T>* f = (const C<T>*)a.pflux;
+    const C<T> fx = f[idx], fy = f[n + idx], fz = f[2 * n + idx];
+    const double kf_r = fma(kv3[0], (double)fx.x, fma(kv3[1], (double)fy.x, kv3[2] * fz.x));
+    const double kf_i = fma(kv3[0], (double)fx.y, fma(kv3[1], (double)fy.y, kv3[2] * fz.y));
+    const double wr = a.pdir == 0 ? U[0][0] : (a.pdir == 1 ? U[1][0] : U[2][0]);
+    const double wi = a.pdir == 0 ? U[0][1] : (a.pdir == 1 ? U[1][1] : U[2][1]);
+    PR[0] = -kf_i + (-a.pgrad) * wr;
+    PR[1] = kf_r + (-a.pgrad) * wi;
+    ((C<T>*)a.prhs)[idx] = mk<T>((T)PR[0], (T)PR[1]);
+  }
   if (a.step == 0) return;  // initialisation: final rhs only (initialize!, FluidFlowSimulation.jl:82)
 
   double M1[3][2], M2[3][2];
@@ -1112,6 +1144,12 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
     HR1[0] = h1.x; HR1[1] = h1.y;
     HR2[0] = h2.x; HR2[1] = h2.y;
   }
+  double HP1[2] = {0, 0}, HP2[2] = {0, 0};
+  if (a.phi) {
+    const C<T> h1 = ((const C<T>*)a.pfm1)[idx], h2 = ((const C<T>*)a.pfm2)[idx];
+    HP1[0] = h1.x; HP1[1] = h1.y;
+    HP2[0] = h2.x; HP2[1] = h2.y;
+  }
   if (a.integrator == 0) {
     const double At = st->At, Bt = st->Bt, Ct = st->Ct;
     if (a.step == 2) {  // fix_fm2!: fm2 = -(dt3+dt2)(rhs - fm1)/dt2 + rhs
@@ -1122,6 +1160,8 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
         for (int p = 0; p < 2; ++p) M2[c][p] = -(dt3 + dt2) * (R[c][p] - M1[c][p]) / dt2 + R[c][p];
 #pragma unroll
       for (int p = 0; p < 2; ++p) HR2[p] = -(dt3 + dt2) * (RR[p] - HR1[p]) / dt2 + RR[p];
+#pragma unroll
+      for (int p = 0; p < 2; ++p) HP2[p] = -(dt3 + dt2) * (PR[p] - HP1[p]) / dt2 + PR[p];
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -1129,6 +1169,8 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
       for (int p = 0; p < 2; ++p) U[c][p] = fma(At, R[c][p], fma(Bt, M1[c][p], fma(Ct, M2[c][p], U[c][p])));
 #pragma unroll
     for (int p = 0; p < 2; ++p) RH[p] = fma(At, RR[p], fma(Bt, HR1[p], fma(Ct, HR2[p], RH[p])));
+#pragma unroll
+    for (int p = 0; p < 2; ++p) PH[p] = fma(At, PR[p], fma(Bt, HP1[p], fma(Ct, HP2[p], PH[p])));
   } else {
     const EtdCoef e = etd_coef(etd_c(a.hv, K2, kv3, -a.nu), a.step, st);
 #pragma unroll
@@ -1146,6 +1188,15 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
         if (a.step == 1) RH[p] = fma(d.E, RH[p], d.A * RR[p]);
         else if (a.step == 2) RH[p] = fma(d.E, RH[p], fma(d.A, RR[p], d.B * HR1[p]));
         else RH[p] = fma(d.A, RR[p], fma(d.B, HR1[p], fma(d.Cc, HR2[p], d.E * RH[p])));
+      }
+    }
+    if (a.phi) {
+      const EtdCoef d = etd_coef(etd_c(a.hv, K2, kv3, -a.palpha), a.step, st);
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        if (a.step == 1) PH[p] = fma(d.E, PH[p], d.A * PR[p]);
+        else if (a.step == 2) PH[p] = fma(d.E, PH[p], fma(d.A, PR[p], d.B * HP1[p]));
+        else PH[p] = fma(d.A, PR[p], fma(d.B, HP1[p], fma(d.Cc, HP2[p], d.E * PH[p])));
       }
     }
   }
@@ -1172,6 +1223,7 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
 #pragma unroll
   for (int c = 0; c < 3; ++c) u[c * n + idx] = mk<T>((T)U[c][0], (T)U[c][1]);
   if (a.rho) ((C<T>*)a.rho)[idx] = mk<T>((T)RH[0], (T)RH[1]);
+  if (a.phi) ((C<T>*)a.phi)[idx] = mk<T>((T)PH[0], (T)PH[1]);
 }
 
 template <typename T>
@@ -1227,7 +1279,8 @@ void launch_project(const Geo& g, void* u, hipStream_t s) {
 //  density (scalar_stats :221-248, buoyancy_flux density_stratification_types.jl:192-211):
 //  22 sum w|rho|^2  23..25 sum w k_i^2 |rho|^2  26 sum w proj(uz, rho)  29 Re rho(k=0)
 //  LES (les_stats :679-705): 27 sum w proj((ik.tau)_{x,y}, u_{x,y})  28 the same for z
-enum { kStatsN = 32, kStatsBlocks = 512, kStatsThreads = 128 };
+enum { kStatsN = 37, kStatsBlocks = 512, kStatsThreads = 128 };
+static_assert(kStatsN <= kStatsSlots, "a stats row of partial sums fits its slot");
 
 __device__ __forceinline__ double proj2(double ax, double ay, double bx, double by) { return ax * bx + ay * by; }
 
@@ -1259,6 +1312,15 @@ __global__ void __launch_bounds__(kStatsThreads) k_stats(Geo g, const C<T>* __re
     acc[2] += w * ez;
     acc[3] += w * K2 * (ex + ey);
     acc[4] += w * K2 * ez;
+    if (aux.phi) {  // scalar_stats (stats.jl:221-248)
+      const C<T> p0 = ((const C<T>*)aux.phi)[idx];
+      const double ep = (double)p0.x * p0.x + (double)p0.y * p0.y;
+      acc[32] += w * ep;
+      acc[33] += w * kx * kx * ep;
+      acc[34] += w * ky * ky * ep;
+      acc[35] += w * kz * kz * ep;
+      if (idx == 0) acc[36] += p0.x;
+    }
     if (aux.hv.kind) {
       const double kv[3] = {kx, ky, kz};
       const double f = hv_weight(aux.hv, kv);
@@ -2171,7 +2233,8 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
 
 // ---- explicit instantiations ----
 #define FFS_INST(T)                                                                                                   \
-  template void launch_inv_z<T>(const Geo&, const void*, void*, int, bool, const void*, hipStream_t, const void*);    \
+  template void launch_inv_z<T>(const Geo&, const void*, void*, int, bool, const void*, hipStream_t, const void*,      \
+                                const void*);                                                                      \
   template void launch_inv_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t);                       \
   template void launch_x_nl<T>(const Geo&, const void*, void*, const XOps&, unsigned long long*, unsigned long long*, \
                                const void*, hipStream_t);                                                           \

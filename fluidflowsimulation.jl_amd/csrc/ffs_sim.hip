@@ -109,10 +109,16 @@ struct ffs_sim {
   void* rrhs = nullptr;  // density rhs / history (rotated like the velocity ones)
   void* rfm1 = nullptr;
   void* rfm2 = nullptr;
-  bool les = false, dens = false, etd = false;
+  void* phi = nullptr;    // passive scalar, its rhs / history and F(-phi u)
+  void* prhs = nullptr;
+  void* pfm1 = nullptr;
+  void* pfm2 = nullptr;
+  void* pflux = nullptr;
+  bool les = false, dens = false, etd = false, pscal = false;
   int nin = 6, nfw = 3;     // physical inputs / forward outputs of the nonlinear term (without 1/2|u|^2)
   double les_alpha = 0.0;   // (c Delta)^2
   double alpha = 0.0, gz = 0.0, drhodz = 0.0;
+  double palpha = 0.0;  // passive scalar diffusivity nu / scalarPr
   void* w1 = nullptr;
   void* w2 = nullptr;
   void *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr;
@@ -358,6 +364,15 @@ struct ffs_sim {
       if (!dry)
         for (void* b : {flux, rho, rrhs, rfm1, rfm2}) HIPCHECK(hipMemset(b, 0, (b == flux ? 3 : 1) * S * csz));
     }
+    if (pscal) {
+      pflux = dalloc(3 * S * csz);
+      phi = dalloc(S * csz);
+      prhs = dalloc(S * csz);
+      pfm1 = dalloc(S * csz);
+      pfm2 = dalloc(S * csz);
+      if (!dry)
+        for (void* b : {pflux, phi, prhs, pfm1, pfm2}) HIPCHECK(hipMemset(b, 0, (b == pflux ? 3 : 1) * S * csz));
+    }
     const size_t nmax = (size_t)std::max(nin, nfw + 1);
     const size_t a1 = nmax * g.sx * g.my * g.nz, b2 = nmax * g.sx * g.ny * g.nz;
     const size_t a2 = nmax * g.sx * g.ny * g.nz, b1 = nmax * g.sx * g.my * g.nz;
@@ -588,6 +603,14 @@ struct ffs_sim {
     etd = (p.integrator == FFS_ETD3);
     les = (p.les_model == FFS_LES_SMAGORINSKY);
     dens = (p.density != 0);
+    pscal = (p.passive_scalar != 0);
+    if (pscal) {
+      // with an LES model the reference gives the scalar an eddy-diffusion flux (simulationtypes.jl:335-348)
+      if (les) throw FfsError(FFS_ERR_ARG, "the passive scalar's LES models (EddyDiffusion, VorticityDiffusion) are not built");
+      if (!(p.scalar_pr > 0)) throw FfsError(FFS_ERR_ARG, "scalarPr must be > 0 (Inf: no diffusion)");
+      if (p.scalar_direction < 0 || p.scalar_direction > 2) throw FfsError(FFS_ERR_ARG, "scalarDirection is x, y or z");
+      palpha = p.nu / p.scalar_pr;  // nu / Inf = 0
+    }
     if (les) {
       // Delta default (2 pi lx / nx) / cutoff (les_types.jl:366)
       const double delta = p.filter_width > 0 ? p.filter_width : (p.x_domain * 2 * M_PI / p.nx) / p.cutoff;
@@ -600,8 +623,8 @@ struct ffs_sim {
       gz = p.z_acceleration / p.reference_density;
       drhodz = p.density_gradient;
     }
-    nin = 6 + (dens ? 1 : 0) + (les ? 5 : 0);
-    nfw = 3 + (dens ? 3 : 0) + (les ? 5 : 0);
+    nin = 6 + (dens ? 1 : 0) + (pscal ? 1 : 0) + (les ? 5 : 0);
+    nfw = 3 + (dens ? 3 : 0) + (pscal ? 3 : 0) + (les ? 5 : 0);
     if (p.dt_stat <= 0 || p.dt_spec <= 0 || p.write_time <= 0)
       throw FfsError(FFS_ERR_ARG, "dtStat, dtSpec and writeTime must be positive");
     if (!p.variable_dt && !(p.dt > 0)) throw FfsError(FFS_ERR_ARG, "a fixed time step needs dt > 0");
@@ -698,12 +721,13 @@ struct ffs_sim {
   // flux <- F(-rho u), tau <- F(2 (c Delta)^2 |S| S), ke <- F(1/2|u|^2) on stats/spectra steps,
   // max|u_i| / max|rho| into st (variable dt)
   void nonlinear(bool need_ke) {
-    launch("inv_z_curl", [&] { FFS_DISPATCH(launch_inv_z, g, u, w1, nin, true, tw_z, stream, rho); });
+    launch("inv_z_curl", [&] { FFS_DISPATCH(launch_inv_z, g, u, w1, nin, true, tw_z, stream, rho, phi); });
     launch("inv_y", [&] { FFS_DISPATCH(launch_inv_y, g, w1, w2, nin, tw_y, stream); });
     XOps ops{};
     ops.nin = nin;
     ops.nout = nfw + (need_ke ? 1 : 0);
     ops.rho = dens ? 1 : 0;
+    ops.phi = pscal ? 1 : 0;
     ops.les = les ? 1 : 0;
     ops.ke = need_ke ? 1 : 0;
     ops.les_2alpha = 2 * les_alpha;
@@ -717,6 +741,8 @@ struct ffs_sim {
     for (int c = 0; c < 3; ++c) dst[f++] = cptr(rhs, c * S);
     if (dens)
       for (int c = 0; c < 3; ++c) dst[f++] = cptr(flux, c * S);
+    if (pscal)
+      for (int c = 0; c < 3; ++c) dst[f++] = cptr(pflux, c * S);
     if (les)
       for (int c = 0; c < 5; ++c) dst[f++] = cptr(tau, c * S);
     if (need_ke) dst[f++] = ke;
@@ -857,7 +883,7 @@ struct ffs_sim {
         }
     }
     auto& sl = slots[slot];
-    StatsAux aux{les ? tau : nullptr, dens ? rho : nullptr, gz, hv_args()};
+    StatsAux aux{les ? tau : nullptr, dens ? rho : nullptr, gz, hv_args(), pscal ? phi : nullptr};
     launch("stats", [&] { FFS_DISPATCH(launch_stats, g, u, rhs, ke, aux, partials, red, stream); });
     HIPCHECK(hipMemcpyAsync(sl.red, red, kStatsSlots * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipMemcpyAsync(sl.st, st, sizeof(DevState), hipMemcpyDeviceToHost, stream));
@@ -899,9 +925,13 @@ struct ffs_sim {
       hvs[2] = hvs[0] + hvs[1];
       tdiss += hvs[2];  // before the LES part (stats.jl:118-121)
     }
+    if (pscal) {  // scalar_stats (stats.jl:221-248): the passive scalar comes first (sim_fields, macros.jl:48)
+      const double sdiss = palpha * (a[33] + a[34] + a[35]);
+      other = {a[36], a[32], a[33], a[34], a[35], sdiss};
+    }
     if (dens) {  // stats(::AbstractDensityStratification), density_stratification_types.jl:46-87
       const double rhodiss = alpha * (a[23] + a[24] + a[25]);
-      other = {a[29], a[22], a[23], a[24], a[25], rhodiss, rhodiss, -gz * a[26]};
+      other.insert(other.end(), {a[29], a[22], a[23], a[24], a[25], rhodiss, rhodiss, -gz * a[26]});
     }
     if (les) {  // les_stats, stats.jl:679-705
       const double prh = -a[27], prv = -a[28];
@@ -989,6 +1019,16 @@ struct ffs_sim {
       a.drhodz = drhodz;
       a.gz = gz;
     }
+    if (pscal) {
+      a.phi = phi;
+      a.prhs = prhs;
+      a.pfm1 = pfm1;
+      a.pfm2 = pfm2;
+      a.pflux = pflux;
+      a.palpha = palpha;
+      a.pgrad = p.scalar_gradient;
+      a.pdir = p.scalar_direction;
+    }
     return a;
   }
 
@@ -1012,6 +1052,12 @@ struct ffs_sim {
       rfm2 = rfm1;
       rfm1 = rrhs;
       rrhs = r2;
+    }
+    if (pscal) {
+      void* q2 = pfm2;
+      pfm2 = pfm1;
+      pfm1 = prhs;
+      prhs = q2;
     }
     tit += 1;
     iteration += 1;
@@ -1077,6 +1123,10 @@ struct ffs_sim {
       HIPCHECK(hipMemcpyAsync(rfm1, rrhs, S * csz, hipMemcpyDeviceToDevice, stream));
       HIPCHECK(hipMemcpyAsync(rfm2, rrhs, S * csz, hipMemcpyDeviceToDevice, stream));
     }
+    if (pscal) {
+      HIPCHECK(hipMemcpyAsync(pfm1, prhs, S * csz, hipMemcpyDeviceToDevice, stream));
+      HIPCHECK(hipMemcpyAsync(pfm2, prhs, S * csz, hipMemcpyDeviceToDevice, stream));
+    }
     // initialize! of the oscillator schemes (forcing_types.jl:66-74, 112-118, 151-164, 207-216,
     // 251-264): without a given target spectrum the target is the initial field's, and the N variants
     // take N = Nm1 from the initial right-hand side; with a target given (targSpectrum.dat) `init` is
@@ -1096,9 +1146,12 @@ struct ffs_sim {
     initialized = true;
   }
 
+  // the closing calculate_rhs! (with set_dt! and write_spec when due), forcing and writestats of
+  // run_simulation (FluidFlowSimulation.jl:46-49); the step's dt becomes the one set_dt! computes
   void writestats_now() {
     nonlinear(true);
-    forcing();
+    step_scalars(true, true);
+    if (is_spec_time(iteration)) queue_spectra();
     queue_stats();
   }
 
@@ -1110,7 +1163,7 @@ struct ffs_sim {
   struct CkptHeader {
     char magic[8];
     uint32_t version, precision;
-    int32_t nx, ny, nz, integrator, les, dens, variable_dt, pad;
+    int32_t nx, ny, nz, integrator, les, dens, variable_dt, pscal;
     int64_t iteration, tit;
     double host_time;
     uint64_t S, csz;
@@ -1119,6 +1172,7 @@ struct ffs_sim {
   std::vector<void*> ckpt_buffers() const {
     std::vector<void*> b = {u, fm1, fm2};
     if (dens) b.insert(b.end(), {rho, rfm1, rfm2});
+    if (pscal) b.insert(b.end(), {phi, pfm1, pfm2});
     return b;
   }
   size_t ckpt_buffer_bytes(const void* b) const { return (b == u || b == fm1 || b == fm2 ? 3 : 1) * S * csz; }
@@ -1141,6 +1195,7 @@ struct ffs_sim {
     h.integrator = p.integrator;
     h.les = les ? 1 : 0;
     h.dens = dens ? 1 : 0;
+    h.pscal = pscal ? 1 : 0;
     h.variable_dt = p.variable_dt;
     h.iteration = iteration;
     h.tit = tit;
@@ -1179,7 +1234,7 @@ struct ffs_sim {
     if (std::memcmp(h.magic, mine.magic, 8) != 0 || h.version != mine.version)
       throw FfsError(FFS_ERR_IO, "not a checkpoint of this library version");
     if (h.precision != mine.precision || h.nx != mine.nx || h.ny != mine.ny || h.nz != mine.nz ||
-        h.integrator != mine.integrator || h.les != mine.les || h.dens != mine.dens ||
+        h.integrator != mine.integrator || h.les != mine.les || h.dens != mine.dens || h.pscal != mine.pscal ||
         h.variable_dt != mine.variable_dt || h.S != mine.S || h.csz != mine.csz)
       throw FfsError(FFS_ERR_ARG, "checkpoint was written by a run with other parameters");
     if (n != checkpoint_bytes()) throw FfsError(FFS_ERR_IO, "checkpoint size mismatch");
@@ -1391,6 +1446,10 @@ int ffs_params_default(ffs_params* p) {
   p->hv_nu = 0.0;
   p->hv_m = 2;
   p->sb_initk = p->sb_endk = 0.0;
+  p->passive_scalar = 0;
+  p->scalar_pr = 1.0;
+  p->scalar_gradient = 0.0;
+  p->scalar_direction = 2;
   return FFS_OK;
 }
 
@@ -1424,12 +1483,13 @@ int ffs_create(const ffs_params* p, ffs_sim** out) {
 int ffs_set_field(ffs_sim* s, int field, int space, const double* host, size_t n) {
   if (!s || !host) return FFS_ERR_ARG;
   return guard(s, [&] {
-    if (field < FFS_FIELD_U1 || field > FFS_FIELD_RHO) throw FfsError(FFS_ERR_ARG, "unknown field");
+    if (field < FFS_FIELD_U1 || field > FFS_FIELD_SCALAR) throw FfsError(FFS_ERR_ARG, "unknown field");
     if (field == FFS_FIELD_RHO && !s->dens) throw FfsError(FFS_ERR_ARG, "the run has no density field");
+    if (field == FFS_FIELD_SCALAR && !s->pscal) throw FfsError(FFS_ERR_ARG, "the run has no passive scalar");
     if (s->initialized) throw FfsError(FFS_ERR_STATE, "fields are set before ffs_initialize");
     const size_t nxh = s->g.nxh, ny = s->g.ny, nz = s->g.nz, nrx = s->g.nrx;
     const size_t padded = 2 * nxh * ny * nz, unpadded = nrx * ny * nz;
-    void* dst = field == FFS_FIELD_RHO ? s->rho : s->cptr(s->u, (size_t)field * s->S);
+    void* dst = field == FFS_FIELD_RHO ? s->rho : field == FFS_FIELD_SCALAR ? s->phi : s->cptr(s->u, (size_t)field * s->S);
     if (space == FFS_SPACE_REAL) {
       std::vector<double> buf;
       const double* src = host;
@@ -1509,12 +1569,13 @@ int ffs_seed_density(ffs_sim* s, uint64_t seed, double k0, double rms) {
 int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n) {
   if (!s || !host) return FFS_ERR_ARG;
   return guard(s, [&] {
-    if (field < FFS_FIELD_U1 || field > FFS_FIELD_RHO) throw FfsError(FFS_ERR_ARG, "unknown field");
+    if (field < FFS_FIELD_U1 || field > FFS_FIELD_SCALAR) throw FfsError(FFS_ERR_ARG, "unknown field");
     if (field == FFS_FIELD_RHO && !s->dens) throw FfsError(FFS_ERR_ARG, "the run has no density field");
+    if (field == FFS_FIELD_SCALAR && !s->pscal) throw FfsError(FFS_ERR_ARG, "the run has no passive scalar");
     const size_t nxh = s->g.nxh, ny = s->g.ny, nz = s->g.nz;
     const size_t padded = 2 * nxh * ny * nz;
     if (n != padded) throw FfsError(FFS_ERR_IO, "output buffer must hold 2*(nx/2+1)*ny*nz doubles");
-    void* src = field == FFS_FIELD_RHO ? s->rho : s->cptr(s->u, (size_t)field * s->S);
+    void* src = field == FFS_FIELD_RHO ? s->rho : field == FFS_FIELD_SCALAR ? s->phi : s->cptr(s->u, (size_t)field * s->S);
     if (space == FFS_SPACE_FOURIER) {
       std::vector<double> c(2 * s->S);
       if (s->fp32) {
@@ -1681,6 +1742,7 @@ int ffs_stats_header(const ffs_sim* s, int table, char* buf, size_t cap) {
   std::string h;
   if (table == FFS_TABLE_STATS) {
     h = "iteration,time,u1,u2,u3,u1p2,u2p2,u3p2,kh,kv,kaniso,k,dissh,dissv,diss,nlh,nlv,pressh,pressv,tnlh,tnlv,tdiss";
+    if (s->pscal) h += ",scalar,scalarp2,dscalardxp2,dscalardyp2,dscalardzp2,scalardiss";
     if (s->dens) h += ",rho,rhop2,drhodxp2,drhodyp2,drhodzp2,rhodiss,trhodiss,bflux";
     if (s->les) h += ",prh,prv,praniso,pr";
     if (s->plane) h += s->p.forcing == FFS_FORCING_A ? ",hp,vp" : ",hp,vp,tp";

@@ -26,7 +26,7 @@ FFS_FORCING_NONE, FFS_FORCING_SHELL, FFS_FORCING_RF, FFS_FORCING_QF = 0, 1, 2, 3
 FFS_FORCING_CRF, FFS_FORCING_NRF, FFS_FORCING_CNRF, FFS_FORCING_A = 4, 5, 6, 7
 FFS_HV_NONE, FFS_HV_HYPER, FFS_HV_BARRIER = 0, 1, 2
 FFS_FSTATE_R, FFS_FSTATE_EM, FFS_FSTATE_KH, FFS_FSTATE_EF, FFS_FSTATE_N = 0, 1, 2, 3, 4
-FFS_FIELD_U1, FFS_FIELD_U2, FFS_FIELD_U3, FFS_FIELD_RHO = 0, 1, 2, 3
+FFS_FIELD_U1, FFS_FIELD_U2, FFS_FIELD_U3, FFS_FIELD_RHO, FFS_FIELD_SCALAR = 0, 1, 2, 3, 4
 FFS_SPACE_REAL, FFS_SPACE_FOURIER = 0, 1
 FFS_SCALAR_ITERATION, FFS_SCALAR_TIME, FFS_SCALAR_DT, FFS_SCALAR_UMAX = 0, 1, 2, 3
 FFS_TABLE_STATS, FFS_TABLE_SCALES, FFS_TABLE_NUMBERS = 0, 1, 2
@@ -82,6 +82,10 @@ class FfsParams(ctypes.Structure):
         ("hv_m", ctypes.c_int32),
         ("sb_initk", ctypes.c_double),
         ("sb_endk", ctypes.c_double),
+        ("passive_scalar", ctypes.c_int32),
+        ("scalar_pr", ctypes.c_double),
+        ("scalar_gradient", ctypes.c_double),
+        ("scalar_direction", ctypes.c_int32),
     ]
 
 

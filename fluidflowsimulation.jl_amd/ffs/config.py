@@ -169,6 +169,10 @@ class Params:
     hv_m: int = 2                   # hyperViscosityM
     sb_initk: float = 0.0           # initk / endk of the spectral barrier (0: the reference's defaults)
     sb_endk: float = 0.0
+    passive_scalar: bool = False    # passiveScalar (presence)
+    scalar_pr: float = 1.0          # scalarPr (Inf: no diffusion)
+    scalar_gradient: float = 0.0    # scalarGradient
+    scalar_direction: str = "z"     # scalarDirection
     extra: dict = field(default_factory=dict)
 
     def to_c(self):
@@ -214,6 +218,9 @@ class Params:
             p.hyperviscosity = _lib.FFS_HV_HYPER
         p.hv_nu, p.hv_m = self.hv_nu, self.hv_m
         p.sb_initk, p.sb_endk = self.sb_initk, self.sb_endk
+        p.passive_scalar = int(bool(self.passive_scalar))
+        p.scalar_pr, p.scalar_gradient = self.scalar_pr, self.scalar_gradient
+        p.scalar_direction = {"x": 0, "y": 1, "z": 2}[self.scalar_direction]
         return p
 
 
@@ -266,6 +273,14 @@ def parameters(d):
         p.density_gradient = parse_expr(req("densityGradient"))
         p.z_acceleration = parse_expr(req("zAcceleration"))
         p.reference_density = parse_expr(req("referenceDensity"))
+    if "passiveScalar" in d:  # simulationtypes.jl:320-352
+        known.add("passiveScalar")
+        p.passive_scalar = True
+        p.scalar_pr = parse_expr(req("scalarPr"))
+        p.scalar_gradient = parse_expr(req("scalarGradient"))
+        p.scalar_direction = opt("scalarDirection", str, "z")
+        if p.scalar_direction not in ("x", "y", "z"):
+            raise ValueError(f"scalarDirection {p.scalar_direction!r}: x, y or z")
     if "hyperViscosity" in d:  # simulationtypes.jl:293-305
         known.add("hyperViscosity")
         p.hyperviscosity = d["hyperViscosity"]

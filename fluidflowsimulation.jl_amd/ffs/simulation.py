@@ -318,12 +318,16 @@ class Simulation:
         return recs
 
     def writeoutput(self, directory="."):
-        """writeoutput (output.jl:1-67): u1.$it u2.$it u3.$it (padded real), rho.$it with density, and
+        """writeoutput (output.jl:1-67): u1.$it u2.$it u3.$it (padded real), scalar.$it with the passive
+        scalar, rho.$it with density, and
         the forcing state: R.$it (the oscillators), N.$it (nRf, cNRf), Eh.$it = (mean |k_h|, Ef)
         rows (rf, cRf, nRf, cNRf)."""
         it = self.iteration
         for f in range(3):
             write_field(os.path.join(directory, f"u{f + 1}.{it}"), self.get_field(f, _lib.FFS_SPACE_REAL))
+        if self.params.passive_scalar:
+            write_field(os.path.join(directory, f"scalar.{it}"),
+                        self.get_field(_lib.FFS_FIELD_SCALAR, _lib.FFS_SPACE_REAL))
         if self.params.density:
             write_field(os.path.join(directory, f"rho.{it}"), self.get_field(_lib.FFS_FIELD_RHO, _lib.FFS_SPACE_REAL))
         forcing = self.params.forcing
@@ -435,6 +439,9 @@ def run_simulation(directory=".", params=None, nt=None):
     nx, ny, nz = params.nx, params.ny, params.nz
     for f in range(3):
         s.set_field(f, read_field(os.path.join(directory, f"u{f + 1}.{params.start}"), nx, ny, nz))
+    scalar_file = os.path.join(directory, f"scalar.{params.start}")
+    if params.passive_scalar and os.path.exists(scalar_file):  # zero scalar without one (simulationtypes.jl:324)
+        s.set_field(_lib.FFS_FIELD_SCALAR, read_field(scalar_file, nx, ny, nz))
     rho_file = os.path.join(directory, f"rho.{params.start}")
     if params.density and os.path.exists(rho_file):  # zero density without one (density_stratification_types.jl:166)
         s.set_field(_lib.FFS_FIELD_RHO, read_field(rho_file, nx, ny, nz))
@@ -455,9 +462,16 @@ def run_simulation(directory=".", params=None, nt=None):
             s.writeoutput(directory)
         s.writestats(directory)
         s.writespectra(directory)
+    # the closing calculate_rhs! writes the output files when the last iteration is an output
+    # iteration (realspace!, models.jl:157), else run_simulation writes them after the forcing
+    # (FluidFlowSimulation.jl:53-61): the forcing state files differ by that forcing call
+    due = s.iteration % params.write_time == 0
+    if due:
+        s.writeoutput(directory)
     s.writestats_now()
     s.writestats(directory)
-    if s.iteration % params.write_time != 0:
+    s.writespectra(directory)
+    if not due:
         s.writeoutput(directory)
     if params.forcing == "rfForcing":  # writedlm("R.$it", s.forcing.R), FluidFlowSimulation.jl:59-61
         _writedlm_values(os.path.join(directory, f"R.{s.iteration}"), s.forcing_state(_lib.FFS_FSTATE_R))

@@ -58,6 +58,10 @@ mutable struct ffs_params
     hv_m::Int32
     sb_initk::Float64
     sb_endk::Float64
+    passive_scalar::Int32
+    scalar_pr::Float64
+    scalar_gradient::Float64
+    scalar_direction::Int32
     ffs_params() = new()
 end
 
@@ -105,6 +109,12 @@ function params_from_global(d::AbstractDict)
         p.density = Int32(1)
         p.pr, p.density_gradient = num(:Pr), num(:densityGradient)
         p.z_acceleration, p.reference_density = num(:zAcceleration), num(:referenceDensity)
+    end
+    # passive scalar (simulationtypes.jl:320-352)
+    if haskey(d, :passiveScalar)
+        p.passive_scalar = Int32(1)
+        p.scalar_pr, p.scalar_gradient = num(:scalarPr), num(:scalarGradient)
+        p.scalar_direction = Int32(Dict("x" => 0, "y" => 1, "z" => 2)[get(d, :scalarDirection, "z")])
     end
     # hyperviscosity (simulationtypes.jl:293-305)
     if haskey(d, :hyperViscosity)
@@ -379,6 +389,7 @@ function writeoutput(s::FFSSimulation)
     for f in 0:2
         write("u$(f + 1).$it", get_field(s, f))
     end
+    s.p.passive_scalar != 0 && write("scalar.$it", get_field(s, 4))
     s.p.density != 0 && write("rho.$it", get_field(s, 3))
     # the forcing state files (output.jl:44-66): R, N (nRf, cNRf), Eh = (mean |k_h|, Ef) (rf, cRf, nRf, cNRf)
     f = s.p.forcing
@@ -400,6 +411,11 @@ function run_simulation(d::AbstractDict)
         read!("u$(f + 1).$(p.start)", a)
         set_field!(s, f, a)
     end
+    if p.passive_scalar != 0 && isfile("scalar.$(p.start)")   # zero scalar without one (simulationtypes.jl:324)
+        a = Vector{Float64}(undef, filesize("scalar.$(p.start)") ÷ 8)
+        read!("scalar.$(p.start)", a)
+        set_field!(s, 4, a)
+    end
     if p.density != 0 && isfile("rho.$(p.start)")   # zero density without one (density_stratification_types.jl:166)
         a = Vector{Float64}(undef, filesize("rho.$(p.start)") ÷ 8)
         read!("rho.$(p.start)", a)
@@ -420,9 +436,15 @@ function run_simulation(d::AbstractDict)
         writestats(s)
         writespectra(s)
     end
+    # the closing calculate_rhs! writes the output when the last iteration is an output iteration,
+    # else run_simulation writes it after the forcing (FluidFlowSimulation.jl:46-61)
+    due = mod(Int(scalar(s, 0)), p.write_time) == 0
+    due && writeoutput(s)
     writestats_now!(s)
     writestats(s)
-    mod(Int(scalar(s, 0)), p.write_time) == 0 || writeoutput(s)
+    writespectra(s)
+    due || writeoutput(s)
+    p.forcing == 2 && writedlm("R.$(Int(scalar(s, 0)))", forcing_state(s, 0))   # FluidFlowSimulation.jl:59-61
     return s
 end
 

@@ -74,6 +74,7 @@ extern "C" {
 #define FFS_FIELD_U2 1
 #define FFS_FIELD_U3 2
 #define FFS_FIELD_RHO 3
+#define FFS_FIELD_SCALAR 4 /* the passive scalar (scalar.N files, output.jl:10-15) */
 
 /* spaces for get/set_field */
 #define FFS_SPACE_REAL 0    /* padded real (2*(nx/2+1), ny, nz), column major; or unpadded (nx,ny,nz) on input */
@@ -127,6 +128,10 @@ typedef struct ffs_params {
   int32_t hv_m;                             /* hyperViscosityM (2) */
   double sb_initk, sb_endk;                 /* initk, endk of spectralBarrier; <= 0 selects the defaults
                                                cutoff 2 KX[end] / 3 and sqrt(3 cutoff) KX[end] */
+  int32_t passive_scalar;                   /* passiveScalar present (simulationtypes.jl:320-352) */
+  double scalar_pr;                         /* scalarPr: diffusivity nu / scalarPr (Inf: none) */
+  double scalar_gradient;                   /* scalarGradient: the mean gradient's source -G u_dir */
+  int32_t scalar_direction;                 /* scalarDirection: 0 x, 1 y, 2 z (default z) */
 } ffs_params;
 
 typedef struct ffs_sim ffs_sim;
@@ -196,7 +201,9 @@ int ffs_checkpoint_load(ffs_sim* s, const void* host, size_t n);
 int ffs_forcing_state(ffs_sim* s, int what, int set, double* data, int cap, int* n);
 
 /* Replaces the tail of run_simulation (FluidFlowSimulation.jl:46-49):
- * calculate_rhs!(s), forcing, writestats(s) at the current iteration. */
+ * calculate_rhs!(s) (with set_dt! on variable-dt runs, and the spectra record
+ * when the iteration is a spectra iteration), the forcing, writestats(s) at the
+ * current iteration. */
 int ffs_writestats_now(ffs_sim* s);
 
 /* Pops the oldest queued statistics record (one per stats step, written by

@@ -153,13 +153,15 @@ class OracleSim:
 
     def __init__(self, grid, nu, dt=0.0, variable_dt=False, cfl=0.18, dt_stat=1 << 30, dt_spec=1 << 30,
                  forcing_kf=None, forcing_power=0.1, start=0, start_time=0.0, integrator="AB3", les_c=None,
-                 les_delta=None, density=None, plane_forcing=None, hv=None):
+                 les_delta=None, density=None, plane_forcing=None, hv=None, scalar=None):
         """integrator: "AB3" (Adams_Bashforth3rdO) or "ETD3" (ETD3rdO);
         les_c: Smagorinsky constant (None: no LES), les_delta: filter width
         (default (2 pi L/nx)/cutoff, les_types.jl:366);
         density: dict(Pr=, gradient=, g=) for the Boussinesq approximation;
         hv: dict(kind="hyper", nu=, m=) (HyperViscosity{nu_h, M}) or dict(kind="barrier",
-        initk=, endk=) (SpectralBarrier), Hyperviscosity/hyperviscosity_types.jl."""
+        initk=, endk=) (SpectralBarrier), Hyperviscosity/hyperviscosity_types.jl;
+        scalar: dict(Pr=, gradient=, dir=0|1|2) for the passive scalar (PassiveScalar,
+        simulationtypes.jl:198-224, 320-352)."""
         self.g = grid
         self.nu = nu
         self.integrator = integrator
@@ -185,6 +187,9 @@ class OracleSim:
         self.force_factor = 0.0
         self.plane = plane_forcing  # oracle.plane_forcing.PlaneForcing (one of the kz = 0 plane forcings) or None
         self.hv = hv
+        self.scalar = scalar
+        if scalar is not None:
+            self.palpha = nu / scalar["Pr"]
         self.umax = 0.0
         self.records = []
         self.spectra = []
@@ -194,6 +199,10 @@ class OracleSim:
         """Real (3, nx, ny, nz) field -> myfourier! of initialize! (FluidFlowSimulation.jl:69)."""
         self.uh = np.array([self.g.myfourier(np.asarray(c, dtype=np.float64)) for c in u])
 
+    def set_scalar(self, phi):
+        """myfourier!(phi) + dealias! of initialize! (FluidFlowSimulation.jl:72-75)."""
+        self.phih = self.g.myfourier(np.asarray(phi, dtype=np.float64))
+
     def set_density(self, rho):
         """myfourier!(rho) + dealias! of initialize! (FluidFlowSimulation.jl:77-80)."""
         self.rhoh = self.g.myfourier(np.asarray(rho, dtype=np.float64))
@@ -202,6 +211,8 @@ class OracleSim:
         """initialize!(s) (FluidFlowSimulation.jl:67-91)."""
         if self.density is not None and not hasattr(self, "rhoh"):
             self.rhoh = np.zeros_like(self.uh[0])
+        if self.scalar is not None and not hasattr(self, "phih"):
+            self.phih = np.zeros_like(self.uh[0])
         self.calculate_rhs(collect=False)
         self.fm1 = self.rhs.copy()
         self.fm2 = self.fm1.copy()
@@ -210,6 +221,9 @@ class OracleSim:
         if self.density is not None:
             self.rfm1 = self.rhorhs.copy()
             self.rfm2 = self.rfm1.copy()
+        if self.scalar is not None:
+            self.pfm1 = self.phirhs.copy()
+            self.pfm2 = self.pfm1.copy()
         if self.plane is not None:
             self.plane.initialize(self)
 
@@ -239,6 +253,8 @@ class OracleSim:
             self.kspec = g.myfourier(0.5 * (ur[0] * ur[0] + ur[1] * ur[1] + ur[2] * ur[2]))
         if self.density is not None:
             rhor = g.irfft(self.rhoh)
+        if self.scalar is not None:
+            phir = g.irfft(self.phih)
         if self.les_c is not None:
             Sr = np.array([g.irfft(c) for c in S])
             # calc_les! Smagorinsky branch (les_real_calc.jl:77-79): tau = 2 (c Delta)^2 |S| S,
@@ -249,6 +265,8 @@ class OracleSim:
         cross = np.array([ur[1] * wr[2] - ur[2] * wr[1], ur[2] * wr[0] - ur[0] * wr[2], ur[0] * wr[1] - ur[1] * wr[0]])
         if self.density is not None:
             fluxr = np.array([(-rhor) * ur[c] for c in range(3)])
+        if self.scalar is not None:  # realspacecalculation!: flux = -phi u (models.jl:245-273)
+            pfluxr = np.array([(-phir) * ur[c] for c in range(3)])
         if self.variable_dt:
             self.umax = float(np.max(np.abs(ur)))
             if self.density is not None:
@@ -259,6 +277,9 @@ class OracleSim:
         if self.density is not None:
             fluxh = np.array([g.myfourier(c) for c in fluxr])
             self.rhoh = g.myfourier(rhor)
+        if self.scalar is not None:
+            pfluxh = np.array([g.myfourier(c) for c in pfluxr])
+            self.phih = g.myfourier(phir)
         if self.les_c is not None:
             self.tauh = np.array([g.myfourier(c) for c in taur])
         if self.variable_dt:
@@ -284,6 +305,11 @@ class OracleSim:
             rhs[2] = rhs[2] + self.rhoh * self.gz
         rhs = np.where(g.RANGE, rhs, 0)
         self.rhs = g.project(rhs)
+        if self.scalar is not None:
+            # div!(rhs, flux, u_dir, -scalarGradient) (models.jl:353-355); no explicit diffusion (AB3)
+            r = 1j * (kx * pfluxh[0] + (ky * pfluxh[1] + kz * pfluxh[2])) + \
+                (-self.scalar["gradient"]) * self.uh[self.scalar["dir"]]
+            self.phirhs = np.where(g.RANGE, r, 0)
         if self.density is not None:
             # div!(rhorhs, flux, u_z, -drhodz) (vectorfunctions.jl:93-103); explicit diffusion is
             # absent in the reference (models.jl:361), so AB3 density has none
@@ -357,6 +383,12 @@ class OracleSim:
         if self.plane is not None:
             u = self.plane.add(np.array(u))
         self.uh = g.project(u)
+        if self.scalar is not None:  # the passive scalar's integrator (models.jl:470)
+            if self.integrator == "AB3":
+                r = self.ab3(self.phih, self.phirhs, "pfm1", "pfm2")
+            else:
+                r = self.etd(self.phih, self.phirhs, "pfm1", "pfm2", -self.palpha)
+            self.phih = np.where(g.RANGE, r, 0)
         if self.density is not None:
             if self.integrator == "AB3":
                 r = self.ab3(self.rhoh, self.rhorhs, "rfm1", "rfm2")
@@ -597,6 +629,12 @@ class OracleSim:
             Lvv = L33
             Lvh = (L31 + L32) / 2
         other = []
+        if self.scalar is not None:
+            # scalar_stats (stats.jl:221-248), before the density's (sim_fields, macros.jl:48)
+            ep = np.abs(self.phih) ** 2
+            sd = [float(np.sum(W * kk * kk * ep)) for kk in (g.kx, g.ky, g.kz)]
+            other += [float(self.phih[0, 0, 0].real), float(np.sum(W * ep)), *sd,
+                      self.palpha * (sd[0] + sd[1] + sd[2])]
         hvstats = []
         if self.hv is not None:  # hyperviscosity_stats / spectral_barrier_stats: hdissh, hdissv, hdiss
             # stats.jl:118: tdiss takes the hyperviscous part before the LES part

@@ -62,7 +62,8 @@ def test_parameters_defaults_and_values(tmp_path):
     assert p.nu == 0.03333333 and p.dt == 0.002 and p.dt_stat == 125 and p.write_time == 125 and p.nt == 500
     assert p.integrator == "Adams_Bashforth3rdO" and p.variable_dt is False
     assert p.cfl == 0.18 and abs(p.cutoff - 2 / 3) < 1e-16  # defaults
-    assert "passiveScalar" in p.extra and "dealias" in p.extra  # read nowhere by the velocity path
+    assert "dealias" in p.extra  # read nowhere (the box mask follows `cutoff`)
+    assert p.passive_scalar and p.scalar_pr == math.inf and p.scalar_gradient == 0.0
 
 
 @pytest.mark.parametrize("s,v", [("2π", 2 * math.pi), ("1/3", 1 / 3), ("2^-4", 2**-4), ("0.5e-3", 5e-4),
