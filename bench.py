@@ -522,7 +522,7 @@ def leg_strat(args):
 def leg_features(args, n, nu):
     """Step cost of the options beyond the plain AB3 step, at the main workload's grid (FP64, seeded
     field, variable dt, graph replay): ETD3, hyperviscosity, the spectral barrier, three plane forcing
-    schemes, Smagorinsky; and the device time of one spectra record (the 17 write_spec quantities and
+    schemes, Smagorinsky, the conservation form, a passive scalar; and the device time of one spectra record (the 17 write_spec quantities and
     their marginals) at a spectra iteration, from HIP events around its launches."""
     import ffs
 
@@ -538,6 +538,8 @@ def leg_features(args, n, nu):
         "ab3 nRfForcing": dict(forcing="nRfForcing", forcing_kf=2.5, forcing_tf=1.0, forcing_alpha=0.5),
         "ab3 aForcing": dict(forcing="aForcing", forcing_kf=3.0, forcing_tf=1.0, forcing_alpha=0.5),
         "ab3 smagorinsky": dict(les_model="Smagorinsky", smagorinsky_c=0.16),
+        "ab3 conservation form": dict(equation="NS"),
+        "ab3 passive scalar": dict(passive_scalar=True, scalar_pr=1.0, scalar_gradient=1.0),
     }
     out = {"n": n, "steps": args.feature_steps, "ms_per_step": {}}
     for name, kw in cases.items():

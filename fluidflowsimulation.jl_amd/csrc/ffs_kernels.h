@@ -112,6 +112,7 @@ enum : int { kMaxFields = 16 };
 template <typename T> struct ColAux {
   const typename std::conditional<sizeof(T) == 8, double2, float2>::type* rho = nullptr;  // inverse z: density
   const typename std::conditional<sizeof(T) == 8, double2, float2>::type* phi = nullptr;  // and passive scalar
+  int nsform = 0;  // conservation form: the inputs have no vorticity (u, then rho, phi, S)
   typename std::conditional<sizeof(T) == 8, double2, float2>::type* dst[kMaxFields] = {};  // forward z outputs
 };
 
@@ -120,7 +121,7 @@ template <typename T> struct ColAux {
 // (u, w, [rho], [S]) are formed on the fly from u (and rho)
 template <typename T>
 void launch_inv_z(const Geo& g, const void* u, void* a1, int nf, bool curl, const void* tw, hipStream_t s,
-                  const void* rho = nullptr, const void* phi = nullptr);
+                  const void* rho = nullptr, const void* phi = nullptr, int nsform = 0);
 template <typename T>
 void launch_inv_y(const Geo& g, const void* a1, void* a2, int nf, const void* tw, hipStream_t s);
 // field set of the fused x pass
@@ -128,6 +129,7 @@ struct XOps {
   int nin, nout;      // physical inputs (6 [+1 rho] [+1 phi] [+5 S]) and forward outputs (3 [+3] [+3] [+5] [+1 ke])
   int rho, les, ke;   // flags: rho, phi and les in that order after u x w, ke last
   int phi;
+  int ns;             // conservation form: inputs u (no w), outputs uu = traceless(-u u) (5) in place of u x w
   double les_2alpha;  // 2 (c Delta)^2 of the Smagorinsky stress
 };
 template <typename T>
@@ -186,6 +188,9 @@ struct UpdateArgs {
 template <typename T>
 void launch_update(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
                    const UpdateArgs& a, hipStream_t s);
+// conservation form: rhs = (i k) . uu (div!, vectorfunctions.jl:107-117 on the SymTrTen field)
+template <typename T>
+void launch_div_uu(const Geo& g, const void* uu, void* rhs, hipStream_t s);
 // the plane forcing of one step (rhs_final = 0: rhs holds the nonlinear term, formed per element as
 // the update kernel does; 1: rhs is final, as after initialization)
 template <typename T>

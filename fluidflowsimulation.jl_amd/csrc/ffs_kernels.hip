@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
         const size_t idx = cx + sx * (cy + (size_t)g.my * cz);
         if (!CURL || f < 3) return in[f * Ssp + idx];
         const T kz = (T)g.kz[cz];
-        if (f < 6) {
+        if (!aux.nsform && f < 6) {
           C<T> a;  // w = i k x u
           if (f == 3) {
             const C<T> uy = in[Ssp + idx], uz = in[2 * Ssp + idx];
@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
           }
           return mk<T>(-a.y, a.x);
         }
-        int e = f - 6;
+        int e = f - (aux.nsform ? 3 : 6);
         if (aux.rho) {
           if (e == 0) return aux.rho[idx];
           --e;
@@ -335,10 +335,11 @@ static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, con
 
 template <typename T>
 void launch_inv_z(const Geo& g, const void* u, void* a1, int nf, bool curl, const void* tw, hipStream_t s,
-                  const void* rho, const void* phi) {
+                  const void* rho, const void* phi, int nsform) {
   ColAux<T> aux{};
   aux.rho = (const C<T>*)rho;
   aux.phi = (const C<T>*)phi;
+  aux.nsform = nsform;
   switch (g.nz) {
 #define X(n)                                                                                    \
   case n:                                                                                       \
@@ -469,11 +470,26 @@ __global__ void __launch_bounds__(256) k_x_gen(Geo g, const C<T>* __restrict__ a
       out[2 * j] = 0;
       out[2 * j + 1] = 0;
     }
-    const T ux = in[0], uy = in[1], uz = in[2], wx = in[3], wy = in[4], wz = in[5];
-    out[0] = uy * wz - uz * wy;
-    out[1] = uz * wx - ux * wz;
-    out[2] = ux * wy - uy * wx;
-    int o = 3, i = 6;
+    const T ux = in[0], uy = in[1], uz = in[2];
+    int o, i;
+    if (ops.ns) {  // traceless(symouter(u, -u)) (models.jl:262): xx xy xz yy yz
+      const T xx = -(ux * ux), yy = -(uy * uy), zz = -(uz * uz);
+      const T t = (xx + yy + zz) / (T)3;
+      out[0] = xx - t;
+      out[1] = -(ux * uy);
+      out[2] = -(ux * uz);
+      out[3] = yy - t;
+      out[4] = -(uy * uz);
+      o = 5;
+      i = 3;
+    } else {
+      const T wx = in[3], wy = in[4], wz = in[5];
+      out[0] = uy * wz - uz * wy;
+      out[1] = uz * wx - ux * wz;
+      out[2] = ux * wy - uy * wx;
+      o = 3;
+      i = 6;
+    }
     if (ops.rho) {
       const T rho = in[i++];
       out[o++] = (-rho) * ux;
@@ -538,9 +554,11 @@ struct XRegArgs {
   int in0, out0, ke_slot;
   double c2a;
 };
-template <int MODE> __host__ __device__ constexpr int xreg_npin() { return MODE == 2 ? 2 : 3; }
+// MODE 0: u x w (+ 1/2|u|^2); 1: the Smagorinsky stress; 2: -s u of a scalar s; 3: the conservation
+// form's traceless(-u u) (+ 1/2|u|^2)
+template <int MODE> __host__ __device__ constexpr int xreg_npin() { return MODE == 2 || MODE == 3 ? 2 : 3; }
 template <int MODE, bool KE> __host__ __device__ constexpr int xreg_npout() {
-  return MODE == 1 ? 3 : 2;
+  return MODE == 1 || MODE == 3 ? 3 : 2;
 }
 
 template <typename T, int N, int MODE, bool KE>
@@ -584,6 +602,7 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
   auto slot_b = [&](int j) -> int {
     if constexpr (MODE == 0) return 2 * j + 1;
     else if constexpr (MODE == 1) return j < 2 ? xa.in0 + 2 * j + 1 : -1;
+    else if constexpr (MODE == 3) return j == 0 ? 1 : -1;
     else return j == 0 ? 1 : xa.in0;
   };
 
@@ -657,6 +676,16 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
       R[0][m] = mk<T>(f * sxx, f * sxy);
       R[1][m] = mk<T>(f * sxz, f * syy);
       R[2][m] = mk<T>(f * syz, (T)0);
+    } else if constexpr (MODE == 3) {  // traceless(symouter(u, -u)) (models.jl:262), 1/2|u|^2, max|u_i|
+      const T ux = R[0][m].x, uy = R[0][m].y, uz = R[1][m].x;
+      const T xx = -(ux * ux), yy = -(uy * uy), zz = -(uz * uz);
+      const T t = (xx + yy + zz) / (T)3;
+      T ke = 0;
+      if constexpr (KE) ke = (T)0.5 * (ux * ux + uy * uy + uz * uz);
+      R[0][m] = mk<T>(xx - t, -(ux * uy));
+      R[1][m] = mk<T>(-(ux * uz), yy - t);
+      R[2][m] = mk<T>(-(uy * uz), ke);
+      if (row_ok) lmax = nanmax(lmax, (double)nanmax(fabs(ux), nanmax(fabs(uy), fabs(uz))));
     } else {  // -rho u, max|rho|
       const T ux = R[0][m].x, uy = R[0][m].y, uz = R[1][m].x, rho = R[1][m].y;
       R[0][m] = mk<T>((-rho) * ux, (-rho) * uy);
@@ -683,6 +712,9 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
     } else if constexpr (MODE == 1) {
       oa = xa.out0 + 2 * j;
       ob = j < 2 ? xa.out0 + 2 * j + 1 : -1;
+    } else if constexpr (MODE == 3) {
+      oa = 2 * j;
+      ob = j < 2 ? 2 * j + 1 : (KE ? xa.ke_slot : -1);
     } else {
       oa = xa.out0 + 2 * j;
       ob = j == 0 ? xa.out0 + 1 : -1;
@@ -845,21 +877,28 @@ static void run_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, un
   const bool plain = !ops.rho && !ops.les && !ops.phi;
   if constexpr (xreg_ok<T, N>()) {
     if (FFS_XNL_MODES || plain) {
-      // u x w (+ 1/2|u|^2, always the last output slot), then -rho u and tau as launches of their own
+      // u x w or traceless(-u u) (+ 1/2|u|^2, always the last output slot), then -rho u, -phi u and
+      // tau as launches of their own
       XRegArgs xa{0, 0, ops.nout - 1, 0.0};
-      if (ops.ke) launch_xreg<T, N, 0, true>(g, a2, b2, tw, umax, xa, s);
-      else launch_xreg<T, N, 0, false>(g, a2, b2, tw, umax, xa, s);
+      if (ops.ns) {
+        if (ops.ke) launch_xreg<T, N, 3, true>(g, a2, b2, tw, umax, xa, s);
+        else launch_xreg<T, N, 3, false>(g, a2, b2, tw, umax, xa, s);
+      } else {
+        if (ops.ke) launch_xreg<T, N, 0, true>(g, a2, b2, tw, umax, xa, s);
+        else launch_xreg<T, N, 0, false>(g, a2, b2, tw, umax, xa, s);
+      }
       const int ns = (ops.rho ? 1 : 0) + (ops.phi ? 1 : 0);
+      const int i0 = ops.ns ? 3 : 6, o0 = ops.ns ? 5 : 3;  // first scalar input / output
       if (ops.rho) {
-        XRegArgs xr{6, 3, 0, 0.0};
+        XRegArgs xr{i0, o0, 0, 0.0};
         launch_xreg<T, N, 2, false>(g, a2, b2, tw, rhomax, xr, s);
       }
       if (ops.phi) {  // -phi u through the same kernel, without a maximum
-        XRegArgs xp{6 + (ops.rho ? 1 : 0), 3 + (ops.rho ? 3 : 0), 0, 0.0};
+        XRegArgs xp{i0 + (ops.rho ? 1 : 0), o0 + (ops.rho ? 3 : 0), 0, 0.0};
         launch_xreg<T, N, 2, false>(g, a2, b2, tw, nullptr, xp, s);
       }
       if (ops.les) {
-        XRegArgs xl{6 + ns, 3 + 3 * ns, 0, ops.les_2alpha};
+        XRegArgs xl{i0 + ns, o0 + 3 * ns, 0, ops.les_2alpha};
         launch_xreg<T, N, 1, false>(g, a2, b2, tw, nullptr, xl, s);
       }
       return;
@@ -1224,6 +1263,36 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
   for (int c = 0; c < 3; ++c) u[c * n + idx] = mk<T>((T)U[c][0], (T)U[c][1]);
   if (a.rho) ((C<T>*)a.rho)[idx] = mk<T>((T)RH[0], (T)RH[1]);
   if (a.phi) ((C<T>*)a.phi)[idx] = mk<T>((T)PH[0], (T)PH[1]);
+}
+
+// div!(rhs, uu) of the conservation form (models.jl:176-180): rhs_c = (i k) . uu, uu symmetric
+// traceless (xx xy xz yy yz), as the LES (i k) . tau
+template <typename T>
+__global__ void __launch_bounds__(256) k_div_uu(Geo g, const C<T>* __restrict__ uu, C<T>* __restrict__ rhs) {
+  const size_t n = (size_t)g.sx * g.my * g.mz;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int cx = idx % g.sx;
+  if (cx >= g.mx) return;
+  const int cy = (idx / g.sx) % g.my;
+  const int cz = idx / ((size_t)g.sx * g.my);
+  const double kv3[3] = {g.kx[cx], g.ky[cy], g.kz[cz]};
+  const C<T> txx = uu[idx], txy = uu[n + idx], txz = uu[2 * n + idx], tyy = uu[3 * n + idx], tyz = uu[4 * n + idx];
+  const double T6[6][2] = {{txx.x, txx.y}, {txy.x, txy.y}, {txz.x, txz.y},
+                           {tyy.x, tyy.y}, {tyz.x, tyz.y}, {-(double)txx.x - tyy.x, -(double)txx.y - tyy.y}};
+  const int row[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double sr = kv3[0] * T6[row[c][0]][0] + kv3[1] * T6[row[c][1]][0] + kv3[2] * T6[row[c][2]][0];
+    const double si = kv3[0] * T6[row[c][0]][1] + kv3[1] * T6[row[c][1]][1] + kv3[2] * T6[row[c][2]][1];
+    rhs[c * n + idx] = mk<T>((T)(-si), (T)sr);
+  }
+}
+
+template <typename T>
+void launch_div_uu(const Geo& g, const void* uu, void* rhs, hipStream_t s) {
+  const size_t n = (size_t)g.sx * g.my * g.mz;
+  k_div_uu<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, (const C<T>*)uu, (C<T>*)rhs);
 }
 
 template <typename T>
@@ -2234,7 +2303,7 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
 // ---- explicit instantiations ----
 #define FFS_INST(T)                                                                                                   \
   template void launch_inv_z<T>(const Geo&, const void*, void*, int, bool, const void*, hipStream_t, const void*,      \
-                                const void*);                                                                      \
+                                const void*, int);                                                                 \
   template void launch_inv_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t);                       \
   template void launch_x_nl<T>(const Geo&, const void*, void*, const XOps&, unsigned long long*, unsigned long long*, \
                                const void*, hipStream_t);                                                           \
@@ -2245,6 +2314,7 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   template void launch_update<T>(const Geo&, void*, void*, const void*, const void*, const DevState*,                 \
                                  const UpdateArgs&, hipStream_t);                                                     \
   template void launch_project<T>(const Geo&, void*, hipStream_t);                                                   \
+  template void launch_div_uu<T>(const Geo&, const void*, void*, hipStream_t);                                     \
   template int launch_stats<T>(const Geo&, const void*, const void*, const void*, const StatsAux&, double*, double*,  \
                                hipStream_t);                                                                       \
   template void launch_spectrum<T>(const Geo&, const void*, double*, double*, double*, double*, double*, double*,     \

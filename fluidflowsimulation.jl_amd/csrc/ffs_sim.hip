@@ -114,7 +114,8 @@ struct ffs_sim {
   void* pfm1 = nullptr;
   void* pfm2 = nullptr;
   void* pflux = nullptr;
-  bool les = false, dens = false, etd = false, pscal = false;
+  void* uu = nullptr;     // conservation form: F(traceless(-u u)), 5 components
+  bool les = false, dens = false, etd = false, pscal = false, nsform = false;
   int nin = 6, nfw = 3;     // physical inputs / forward outputs of the nonlinear term (without 1/2|u|^2)
   double les_alpha = 0.0;   // (c Delta)^2
   double alpha = 0.0, gz = 0.0, drhodz = 0.0;
@@ -364,6 +365,10 @@ struct ffs_sim {
       if (!dry)
         for (void* b : {flux, rho, rrhs, rfm1, rfm2}) HIPCHECK(hipMemset(b, 0, (b == flux ? 3 : 1) * S * csz));
     }
+    if (nsform) {
+      uu = dalloc(5 * S * csz);
+      if (!dry) HIPCHECK(hipMemset(uu, 0, 5 * S * csz));
+    }
     if (pscal) {
       pflux = dalloc(3 * S * csz);
       phi = dalloc(S * csz);
@@ -604,6 +609,8 @@ struct ffs_sim {
     les = (p.les_model == FFS_LES_SMAGORINSKY);
     dens = (p.density != 0);
     pscal = (p.passive_scalar != 0);
+    if (p.equation != FFS_EQ_ROTATIONAL && p.equation != FFS_EQ_NS) throw FfsError(FFS_ERR_ARG, "unknown equation");
+    nsform = (p.equation == FFS_EQ_NS);
     if (pscal) {
       // with an LES model the reference gives the scalar an eddy-diffusion flux (simulationtypes.jl:335-348)
       if (les) throw FfsError(FFS_ERR_ARG, "the passive scalar's LES models (EddyDiffusion, VorticityDiffusion) are not built");
@@ -623,8 +630,8 @@ struct ffs_sim {
       gz = p.z_acceleration / p.reference_density;
       drhodz = p.density_gradient;
     }
-    nin = 6 + (dens ? 1 : 0) + (pscal ? 1 : 0) + (les ? 5 : 0);
-    nfw = 3 + (dens ? 3 : 0) + (pscal ? 3 : 0) + (les ? 5 : 0);
+    nin = (nsform ? 3 : 6) + (dens ? 1 : 0) + (pscal ? 1 : 0) + (les ? 5 : 0);
+    nfw = (nsform ? 5 : 3) + (dens ? 3 : 0) + (pscal ? 3 : 0) + (les ? 5 : 0);
     if (p.dt_stat <= 0 || p.dt_spec <= 0 || p.write_time <= 0)
       throw FfsError(FFS_ERR_ARG, "dtStat, dtSpec and writeTime must be positive");
     if (!p.variable_dt && !(p.dt > 0)) throw FfsError(FFS_ERR_ARG, "a fixed time step needs dt > 0");
@@ -721,13 +728,16 @@ struct ffs_sim {
   // flux <- F(-rho u), tau <- F(2 (c Delta)^2 |S| S), ke <- F(1/2|u|^2) on stats/spectra steps,
   // max|u_i| / max|rho| into st (variable dt)
   void nonlinear(bool need_ke) {
-    launch("inv_z_curl", [&] { FFS_DISPATCH(launch_inv_z, g, u, w1, nin, true, tw_z, stream, rho, phi); });
+    launch("inv_z_curl", [&] {
+      FFS_DISPATCH(launch_inv_z, g, u, w1, nin, true, tw_z, stream, rho, phi, nsform ? 1 : 0);
+    });
     launch("inv_y", [&] { FFS_DISPATCH(launch_inv_y, g, w1, w2, nin, tw_y, stream); });
     XOps ops{};
     ops.nin = nin;
     ops.nout = nfw + (need_ke ? 1 : 0);
     ops.rho = dens ? 1 : 0;
     ops.phi = pscal ? 1 : 0;
+    ops.ns = nsform ? 1 : 0;
     ops.les = les ? 1 : 0;
     ops.ke = need_ke ? 1 : 0;
     ops.les_2alpha = 2 * les_alpha;
@@ -738,7 +748,10 @@ struct ffs_sim {
     launch("fwd_y", [&] { FFS_DISPATCH(launch_fwd_y, g, w1, w2, nf, tw_y, stream); });
     void* dst[kMaxFields] = {};
     int f = 0;
-    for (int c = 0; c < 3; ++c) dst[f++] = cptr(rhs, c * S);
+    if (nsform)
+      for (int c = 0; c < 5; ++c) dst[f++] = cptr(uu, c * S);
+    else
+      for (int c = 0; c < 3; ++c) dst[f++] = cptr(rhs, c * S);
     if (dens)
       for (int c = 0; c < 3; ++c) dst[f++] = cptr(flux, c * S);
     if (pscal)
@@ -746,7 +759,9 @@ struct ffs_sim {
     if (les)
       for (int c = 0; c < 5; ++c) dst[f++] = cptr(tau, c * S);
     if (need_ke) dst[f++] = ke;
-    launch("fwd_z", [&] { FFS_DISPATCH(launch_fwd_z, g, w2, dst, nf, 3, scale, tw_z, stream); });
+    launch("fwd_z", [&] { FFS_DISPATCH(launch_fwd_z, g, w2, dst, nf, nsform ? 0 : 3, scale, tw_z, stream); });
+    // conservation form: rhs = div!(uu) (models.jl:176-180)
+    if (nsform) launch("div_uu", [&] { FFS_DISPATCH(launch_div_uu, g, uu, rhs, stream); });
   }
 
   // set_dt! (variable dt) and the forcing band sum, as one single-block kernel
@@ -1450,6 +1465,7 @@ int ffs_params_default(ffs_params* p) {
   p->scalar_pr = 1.0;
   p->scalar_gradient = 0.0;
   p->scalar_direction = 2;
+  p->equation = FFS_EQ_ROTATIONAL;
   return FFS_OK;
 }
 

@@ -25,6 +25,7 @@ FFS_LES_NONE, FFS_LES_SMAGORINSKY = 0, 1
 FFS_FORCING_NONE, FFS_FORCING_SHELL, FFS_FORCING_RF, FFS_FORCING_QF = 0, 1, 2, 3
 FFS_FORCING_CRF, FFS_FORCING_NRF, FFS_FORCING_CNRF, FFS_FORCING_A = 4, 5, 6, 7
 FFS_HV_NONE, FFS_HV_HYPER, FFS_HV_BARRIER = 0, 1, 2
+FFS_EQ_ROTATIONAL, FFS_EQ_NS = 0, 1
 FFS_FSTATE_R, FFS_FSTATE_EM, FFS_FSTATE_KH, FFS_FSTATE_EF, FFS_FSTATE_N = 0, 1, 2, 3, 4
 FFS_FIELD_U1, FFS_FIELD_U2, FFS_FIELD_U3, FFS_FIELD_RHO, FFS_FIELD_SCALAR = 0, 1, 2, 3, 4
 FFS_SPACE_REAL, FFS_SPACE_FOURIER = 0, 1
@@ -86,6 +87,7 @@ class FfsParams(ctypes.Structure):
         ("scalar_pr", ctypes.c_double),
         ("scalar_gradient", ctypes.c_double),
         ("scalar_direction", ctypes.c_int32),
+        ("equation", ctypes.c_int32),
     ]
 
 

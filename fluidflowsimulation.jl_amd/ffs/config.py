@@ -173,6 +173,7 @@ class Params:
     scalar_pr: float = 1.0          # scalarPr (Inf: no diffusion)
     scalar_gradient: float = 0.0    # scalarGradient
     scalar_direction: str = "z"     # scalarDirection
+    equation: str = None            # equation: "NS" is the conservation form, anything else the rotational
     extra: dict = field(default_factory=dict)
 
     def to_c(self):
@@ -221,6 +222,7 @@ class Params:
         p.passive_scalar = int(bool(self.passive_scalar))
         p.scalar_pr, p.scalar_gradient = self.scalar_pr, self.scalar_gradient
         p.scalar_direction = {"x": 0, "y": 1, "z": 2}[self.scalar_direction]
+        p.equation = _lib.FFS_EQ_NS if self.equation == "NS" else _lib.FFS_EQ_ROTATIONAL
         return p
 
 
@@ -273,6 +275,7 @@ def parameters(d):
         p.density_gradient = parse_expr(req("densityGradient"))
         p.z_acceleration = parse_expr(req("zAcceleration"))
         p.reference_density = parse_expr(req("referenceDensity"))
+    p.equation = opt("equation", str, None)  # simulationtypes.jl:265-273
     if "passiveScalar" in d:  # simulationtypes.jl:320-352
         known.add("passiveScalar")
         p.passive_scalar = True

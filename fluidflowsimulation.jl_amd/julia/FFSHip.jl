@@ -62,6 +62,7 @@ mutable struct ffs_params
     scalar_pr::Float64
     scalar_gradient::Float64
     scalar_direction::Int32
+    equation::Int32
     ffs_params() = new()
 end
 
@@ -110,6 +111,8 @@ function params_from_global(d::AbstractDict)
         p.pr, p.density_gradient = num(:Pr), num(:densityGradient)
         p.z_acceleration, p.reference_density = num(:zAcceleration), num(:referenceDensity)
     end
+    # equation (simulationtypes.jl:265-273): "NS" selects the conservation form
+    p.equation = Int32(get(d, :equation, "") == "NS" ? 1 : 0)
     # passive scalar (simulationtypes.jl:320-352)
     if haskey(d, :passiveScalar)
         p.passive_scalar = Int32(1)

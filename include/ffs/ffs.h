@@ -64,6 +64,10 @@ extern "C" {
 #define FFS_FORCING_CNRF 6 /* cNRfForcing (cnrforcing.jl) */
 #define FFS_FORCING_A 7    /* aForcing    (aforcing.jl: one oscillator per kz = 0 mode, |k_h| <= kf) */
 
+/* equation: the nonlinear term's form (simulationtypes.jl:108-116, 265-273) */
+#define FFS_EQ_ROTATIONAL 0 /* VorticityEquation: u x w (the default) */
+#define FFS_EQ_NS 1         /* NavierStokesEquation ("NS"): -div(u u), u u traceless (models.jl:176-180, 262) */
+
 /* hyperviscosity (Hyperviscosity/hyperviscosity_types.jl) */
 #define FFS_HV_NONE 0
 #define FFS_HV_HYPER 1   /* HyperViscosity{nu_h, M}: -nu_h k^2M beside -nu k^2 (AB3 rhs, ETD c) */
@@ -132,6 +136,7 @@ typedef struct ffs_params {
   double scalar_pr;                         /* scalarPr: diffusivity nu / scalarPr (Inf: none) */
   double scalar_gradient;                   /* scalarGradient: the mean gradient's source -G u_dir */
   int32_t scalar_direction;                 /* scalarDirection: 0 x, 1 y, 2 z (default z) */
+  int32_t equation;                         /* FFS_EQ_*: equation (simulationtypes.jl:265-273) */
 } ffs_params;
 
 typedef struct ffs_sim ffs_sim;

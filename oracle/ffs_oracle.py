@@ -153,7 +153,7 @@ class OracleSim:
 
     def __init__(self, grid, nu, dt=0.0, variable_dt=False, cfl=0.18, dt_stat=1 << 30, dt_spec=1 << 30,
                  forcing_kf=None, forcing_power=0.1, start=0, start_time=0.0, integrator="AB3", les_c=None,
-                 les_delta=None, density=None, plane_forcing=None, hv=None, scalar=None):
+                 les_delta=None, density=None, plane_forcing=None, hv=None, scalar=None, equation="rotational"):
         """integrator: "AB3" (Adams_Bashforth3rdO) or "ETD3" (ETD3rdO);
         les_c: Smagorinsky constant (None: no LES), les_delta: filter width
         (default (2 pi L/nx)/cutoff, les_types.jl:366);
@@ -161,7 +161,8 @@ class OracleSim:
         hv: dict(kind="hyper", nu=, m=) (HyperViscosity{nu_h, M}) or dict(kind="barrier",
         initk=, endk=) (SpectralBarrier), Hyperviscosity/hyperviscosity_types.jl;
         scalar: dict(Pr=, gradient=, dir=0|1|2) for the passive scalar (PassiveScalar,
-        simulationtypes.jl:198-224, 320-352)."""
+        simulationtypes.jl:198-224, 320-352); equation: "rotational" (u x w) or "NS"
+        (the conservation form, -div(uu) with uu = traceless(symouter(u, -u)), models.jl:176-180, 262)."""
         self.g = grid
         self.nu = nu
         self.integrator = integrator
@@ -187,6 +188,7 @@ class OracleSim:
         self.force_factor = 0.0
         self.plane = plane_forcing  # oracle.plane_forcing.PlaneForcing (one of the kz = 0 plane forcings) or None
         self.hv = hv
+        self.equation = equation
         self.scalar = scalar
         if scalar is not None:
             self.palpha = nu / scalar["Pr"]
@@ -262,7 +264,14 @@ class OracleSim:
             szz = -Sr[0] - Sr[3]
             ss = Sr[0] ** 2 + Sr[3] ** 2 + szz**2 + 2 * (Sr[1] ** 2 + Sr[2] ** 2 + Sr[4] ** 2)
             taur = 2 * (self.les_alpha * np.sqrt(2 * ss)) * Sr
-        cross = np.array([ur[1] * wr[2] - ur[2] * wr[1], ur[2] * wr[0] - ur[0] * wr[2], ur[0] * wr[1] - ur[1] * wr[0]])
+        if self.equation == "NS":
+            # traceless(symouter(u, -u)): xx xy xz yy yz, then div!(rhs, uu) = (i k) . uu
+            xx, yy, zz = -(ur[0] * ur[0]), -(ur[1] * ur[1]), -(ur[2] * ur[2])
+            tr = (xx + yy + zz) / 3
+            uur = np.array([xx - tr, -(ur[0] * ur[1]), -(ur[0] * ur[2]), yy - tr, -(ur[1] * ur[2])])
+        else:
+            cross = np.array([ur[1] * wr[2] - ur[2] * wr[1], ur[2] * wr[0] - ur[0] * wr[2],
+                              ur[0] * wr[1] - ur[1] * wr[0]])
         if self.density is not None:
             fluxr = np.array([(-rhor) * ur[c] for c in range(3)])
         if self.scalar is not None:  # realspacecalculation!: flux = -phi u (models.jl:245-273)
@@ -271,8 +280,15 @@ class OracleSim:
             self.umax = float(np.max(np.abs(ur)))
             if self.density is not None:
                 self.rhomax = float(np.max(np.abs(rhor)))
-        rhs = np.array([g.myfourier(c) for c in cross])
-        rhs[:, 0, 0, 0] = 0
+        if self.equation == "NS":
+            t = [g.myfourier(c) for c in uur]
+            txx, txy, txz, tyy, tyz = t
+            tzz = -txx - tyy
+            rhs = np.array([1j * (kx * txx + ky * txy + kz * txz), 1j * (kx * txy + ky * tyy + kz * tyz),
+                            1j * (kx * txz + ky * tyz + kz * tzz)])
+        else:
+            rhs = np.array([g.myfourier(c) for c in cross])
+            rhs[:, 0, 0, 0] = 0
         self.uh = g.project(np.array([g.myfourier(c) for c in ur]))
         if self.density is not None:
             fluxh = np.array([g.myfourier(c) for c in fluxr])
