@@ -188,6 +188,9 @@ struct UpdateArgs {
 template <typename T>
 void launch_update(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
                    const UpdateArgs& a, hipStream_t s);
+// the Smagorinsky stress in real space: s[5] padded real S fields (xx xy xz yy yz) -> c2a |S| S in place
+template <typename T>
+void launch_les_tau_real(T* s5, size_t n_padded, double c2a, hipStream_t s);
 // conservation form: rhs = (i k) . uu (div!, vectorfunctions.jl:107-117 on the SymTrTen field)
 template <typename T>
 void launch_div_uu(const Geo& g, const void* uu, void* rhs, hipStream_t s);

@@ -1265,6 +1265,28 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
   if (a.phi) ((C<T>*)a.phi)[idx] = mk<T>((T)PH[0], (T)PH[1]);
 }
 
+// calc_les! Smagorinsky branch (les_real_calc.jl:77-79) on padded real S fields: tau = 2 (c Delta)^2
+// sqrt(2 S:S) S with Szz = -Sxx - Syy, as the x pass forms it
+template <typename T>
+__global__ void __launch_bounds__(256) k_les_tau_real(T* __restrict__ s5, size_t n, double c2a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T sxx = s5[i], sxy = s5[n + i], sxz = s5[2 * n + i], syy = s5[3 * n + i], syz = s5[4 * n + i];
+  const T szz = -sxx - syy;
+  const T ss = sxx * sxx + syy * syy + szz * szz + (T)2 * (sxy * sxy + sxz * sxz + syz * syz);
+  const T f = (T)c2a * sqrt((T)2 * ss);
+  s5[i] = f * sxx;
+  s5[n + i] = f * sxy;
+  s5[2 * n + i] = f * sxz;
+  s5[3 * n + i] = f * syy;
+  s5[4 * n + i] = f * syz;
+}
+
+template <typename T>
+void launch_les_tau_real(T* s5, size_t n_padded, double c2a, hipStream_t s) {
+  k_les_tau_real<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(s5, n_padded, c2a);
+}
+
 // div!(rhs, uu) of the conservation form (models.jl:176-180): rhs_c = (i k) . uu, uu symmetric
 // traceless (xx xy xz yy yz), as the LES (i k) . tau
 template <typename T>
@@ -2315,6 +2337,7 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
                                  const UpdateArgs&, hipStream_t);                                                     \
   template void launch_project<T>(const Geo&, void*, hipStream_t);                                                   \
   template void launch_div_uu<T>(const Geo&, const void*, void*, hipStream_t);                                     \
+  template void launch_les_tau_real<T>(T*, size_t, double, hipStream_t);                                          \
   template int launch_stats<T>(const Geo&, const void*, const void*, const void*, const StatsAux&, double*, double*,  \
                                hipStream_t);                                                                       \
   template void launch_spectrum<T>(const Geo&, const void*, double*, double*, double*, double*, double*, double*,     \

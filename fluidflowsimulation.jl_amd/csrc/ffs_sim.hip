@@ -1585,13 +1585,42 @@ int ffs_seed_density(ffs_sim* s, uint64_t seed, double k0, double rms) {
 int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n) {
   if (!s || !host) return FFS_ERR_ARG;
   return guard(s, [&] {
-    if (field < FFS_FIELD_U1 || field > FFS_FIELD_SCALAR) throw FfsError(FFS_ERR_ARG, "unknown field");
+    if (field < FFS_FIELD_U1 || field > FFS_FIELD_T23) throw FfsError(FFS_ERR_ARG, "unknown field");
     if (field == FFS_FIELD_RHO && !s->dens) throw FfsError(FFS_ERR_ARG, "the run has no density field");
     if (field == FFS_FIELD_SCALAR && !s->pscal) throw FfsError(FFS_ERR_ARG, "the run has no passive scalar");
+    const bool stress = field >= FFS_FIELD_T11;
+    if (stress && !s->les) throw FfsError(FFS_ERR_ARG, "the run has no LES stress");
+    if (space != FFS_SPACE_REAL && space != FFS_SPACE_FOURIER && space != FFS_SPACE_FOURIER_TO_REAL)
+      throw FfsError(FFS_ERR_ARG, "unknown space");
     const size_t nxh = s->g.nxh, ny = s->g.ny, nz = s->g.nz;
     const size_t padded = 2 * nxh * ny * nz;
     if (n != padded) throw FfsError(FFS_ERR_IO, "output buffer must hold 2*(nx/2+1)*ny*nz doubles");
-    void* src = field == FFS_FIELD_RHO ? s->rho : field == FFS_FIELD_SCALAR ? s->phi : s->cptr(s->u, (size_t)field * s->S);
+    void* src = field == FFS_FIELD_RHO      ? s->rho
+                : field == FFS_FIELD_SCALAR ? s->phi
+                : stress                    ? s->cptr(s->tau, (size_t)(field - FFS_FIELD_T11) * s->S)
+                                            : s->cptr(s->u, (size_t)field * s->S);
+    const size_t tb = s->fp32 ? sizeof(float) : sizeof(double);
+    // copies one padded real field of the run's precision from the device to host doubles
+    auto fetch_real = [&](const void* real) {
+      if (s->fp32) {
+        std::vector<float> f(padded);
+        HIPCHECK(hipMemcpyAsync(f.data(), real, padded * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+        for (size_t i = 0; i < padded; ++i) host[i] = f[i];
+      } else {
+        HIPCHECK(hipMemcpyAsync(host, real, padded * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+      }
+    };
+    auto release = [&](void* real, size_t bytes) {
+      for (size_t i = 0; i < s->allocs.size(); ++i)
+        if (s->allocs[i] == real) {
+          hipFree(real);
+          s->allocs.erase(s->allocs.begin() + i);
+          break;
+        }
+      s->dev_bytes -= bytes;
+    };
     if (space == FFS_SPACE_FOURIER) {
       std::vector<double> c(2 * s->S);
       if (s->fp32) {
@@ -1612,28 +1641,32 @@ int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n) {
             host[2 * dsti] = c[2 * d];
             host[2 * dsti + 1] = c[2 * d + 1];
           }
-    } else if (space == FFS_SPACE_REAL) {
-      const size_t tb = s->fp32 ? sizeof(float) : sizeof(double);
+    } else if (stress && space == FFS_SPACE_REAL) {
+      // realspace!: the curl pass forms S from the current velocity, the x pass's inverse transform
+      // takes the five S fields to real space, then the pointwise product (les_real_calc.jl:77-79)
+      void* real = s->dalloc(5 * padded * tb);
+      s->launch("inv_z_curl", [&] {
+        FFS_DISPATCH_S(launch_inv_z, s->g, s->u, s->w1, s->nin, true, s->tw_z, s->stream, s->rho, s->phi,
+                       s->nsform ? 1 : 0);
+      });
+      s->launch("inv_y", [&] { FFS_DISPATCH_S(launch_inv_y, s->g, s->w1, s->w2, s->nin, s->tw_y, s->stream); });
+      const int s0 = (s->nsform ? 3 : 6) + (s->dens ? 1 : 0) + (s->pscal ? 1 : 0);
+      const size_t S2 = (size_t)s->g.sx * s->g.ny * s->g.nz;
+      void* a2 = s->cptr(s->w2, (size_t)s0 * S2);
+      if (s->fp32) {
+        s->launch("x_c2r", [&] { launch_x_c2r<float>(s->g, a2, (float*)real, 5, s->tw_x, s->stream); });
+        s->launch("les_tau_real", [&] { launch_les_tau_real<float>((float*)real, padded, 2 * s->les_alpha, s->stream); });
+      } else {
+        s->launch("x_c2r", [&] { launch_x_c2r<double>(s->g, a2, (double*)real, 5, s->tw_x, s->stream); });
+        s->launch("les_tau_real", [&] { launch_les_tau_real<double>((double*)real, padded, 2 * s->les_alpha, s->stream); });
+      }
+      fetch_real((const char*)real + (size_t)(field - FFS_FIELD_T11) * padded * tb);
+      release(real, 5 * padded * tb);
+    } else {  // REAL (a state field) or FOURIER_TO_REAL: the stored transform back in real space
       void* real = s->dalloc(padded * tb);
       s->inverse_to_real(src, 1, real);
-      if (s->fp32) {
-        std::vector<float> f(padded);
-        HIPCHECK(hipMemcpyAsync(f.data(), real, padded * sizeof(float), hipMemcpyDeviceToHost, s->stream));
-        HIPCHECK(hipStreamSynchronize(s->stream));
-        for (size_t i = 0; i < padded; ++i) host[i] = f[i];
-      } else {
-        HIPCHECK(hipMemcpyAsync(host, real, padded * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-        HIPCHECK(hipStreamSynchronize(s->stream));
-      }
-      for (size_t i = 0; i < s->allocs.size(); ++i)
-        if (s->allocs[i] == real) {
-          hipFree(real);
-          s->allocs.erase(s->allocs.begin() + i);
-          break;
-        }
-      s->dev_bytes -= padded * tb;
-    } else {
-      throw FfsError(FFS_ERR_ARG, "unknown space");
+      fetch_real(real);
+      release(real, padded * tb);
     }
   });
 }

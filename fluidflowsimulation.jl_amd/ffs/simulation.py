@@ -80,11 +80,11 @@ class Simulation:
             self.set_field(f, a)
 
     def get_field(self, field, space=_lib.FFS_SPACE_REAL):
-        """Real space: padded Fortran array (2*(nx//2+1), ny, nz).
+        """Real space (FFS_SPACE_REAL, FFS_SPACE_FOURIER_TO_REAL): padded Fortran array (2*(nx//2+1), ny, nz).
         Fourier space: complex (nx//2+1, ny, nz) (the retained modes; zeros elsewhere)."""
         buf = np.empty(self.padded_size, dtype=np.float64)
         self._check(self._lib.ffs_get_field(self._h, field, space, _dptr(buf), buf.size))
-        if space == _lib.FFS_SPACE_REAL:
+        if space in (_lib.FFS_SPACE_REAL, _lib.FFS_SPACE_FOURIER_TO_REAL):
             return buf.reshape(padded_shape(self.nx, self.ny, self.nz), order="F")
         c = buf[0::2] + 1j * buf[1::2]
         return c.reshape((self.nx // 2 + 1, self.ny, self.nz), order="F")
@@ -317,11 +317,13 @@ class Simulation:
         self.stats_rows.extend(recs)
         return recs
 
-    def writeoutput(self, directory="."):
+    def writeoutput(self, directory=".", closing=False):
         """writeoutput (output.jl:1-67): u1.$it u2.$it u3.$it (padded real), scalar.$it with the passive
         scalar, rho.$it with density, and
         the forcing state: R.$it (the oscillators), N.$it (nRf, cNRf), Eh.$it = (mean |k_h|, Ef)
-        rows (rf, cRf, nRf, cNRf)."""
+        rows (rf, cRf, nRf, cNRf); with Smagorinsky the stress T11 T12 T13 T22 T23: the real-space
+        product of the current velocity, or with `closing` the stored transform back in real space
+        (the real!(tau) of run_simulation's closing output, FluidFlowSimulation.jl:53-57)."""
         it = self.iteration
         for f in range(3):
             write_field(os.path.join(directory, f"u{f + 1}.{it}"), self.get_field(f, _lib.FFS_SPACE_REAL))
@@ -330,6 +332,11 @@ class Simulation:
                         self.get_field(_lib.FFS_FIELD_SCALAR, _lib.FFS_SPACE_REAL))
         if self.params.density:
             write_field(os.path.join(directory, f"rho.{it}"), self.get_field(_lib.FFS_FIELD_RHO, _lib.FFS_SPACE_REAL))
+        if self.params.les_model not in (None, "", "noLesModel"):
+            space = _lib.FFS_SPACE_FOURIER_TO_REAL if closing else _lib.FFS_SPACE_REAL
+            for name, f in (("T11", _lib.FFS_FIELD_T11), ("T12", _lib.FFS_FIELD_T12), ("T13", _lib.FFS_FIELD_T13),
+                            ("T22", _lib.FFS_FIELD_T22), ("T23", _lib.FFS_FIELD_T23)):
+                write_field(os.path.join(directory, f"{name}.{it}"), self.get_field(f, space))
         forcing = self.params.forcing
         if forcing in RF_FAMILY:
             R = self.forcing_state(_lib.FFS_FSTATE_R)
@@ -472,7 +479,7 @@ def run_simulation(directory=".", params=None, nt=None):
     s.writestats(directory)
     s.writespectra(directory)
     if not due:
-        s.writeoutput(directory)
+        s.writeoutput(directory, closing=True)
     if params.forcing == "rfForcing":  # writedlm("R.$it", s.forcing.R), FluidFlowSimulation.jl:59-61
         _writedlm_values(os.path.join(directory, f"R.{s.iteration}"), s.forcing_state(_lib.FFS_FSTATE_R))
     return s

@@ -387,13 +387,18 @@ end
 synchronize(b::FFSBatch) = batch_check(ccall((:ffs_batch_synchronize, libffs), Cint, (Ptr{Cvoid},), b.h), b.h)
 
 # writeoutput (output.jl:1-8): u1.$it u2.$it u3.$it in the padded real layout
-function writeoutput(s::FFSSimulation)
+function writeoutput(s::FFSSimulation; closing::Bool = false)
     it = Int(scalar(s, 0))
     for f in 0:2
         write("u$(f + 1).$it", get_field(s, f))
     end
     s.p.passive_scalar != 0 && write("scalar.$it", get_field(s, 4))
     s.p.density != 0 && write("rho.$it", get_field(s, 3))
+    if s.p.les_model != 0   # the stress: real-space product, or real!(tau) at the closing output
+        for (name, f) in (("T11", 5), ("T12", 6), ("T13", 7), ("T22", 8), ("T23", 9))
+            write("$name.$it", get_field(s, f; space = closing ? 2 : 0))
+        end
+    end
     # the forcing state files (output.jl:44-66): R, N (nRf, cNRf), Eh = (mean |k_h|, Ef) (rf, cRf, nRf, cNRf)
     f = s.p.forcing
     if f == 2 || f >= 4
@@ -446,7 +451,7 @@ function run_simulation(d::AbstractDict)
     writestats_now!(s)
     writestats(s)
     writespectra(s)
-    due || writeoutput(s)
+    due || writeoutput(s; closing = true)
     p.forcing == 2 && writedlm("R.$(Int(scalar(s, 0)))", forcing_state(s, 0))   # FluidFlowSimulation.jl:59-61
     return s
 end

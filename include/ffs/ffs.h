@@ -79,10 +79,20 @@ extern "C" {
 #define FFS_FIELD_U3 2
 #define FFS_FIELD_RHO 3
 #define FFS_FIELD_SCALAR 4 /* the passive scalar (scalar.N files, output.jl:10-15) */
+/* the Smagorinsky stress xx xy xz yy yz (the T11 T12 T13 T22 T23 files, output.jl:28-35; get only):
+ * FFS_SPACE_REAL gives the real-space product 2 (c Delta)^2 |S| S of the current velocity as
+ * realspace! forms it, FFS_SPACE_FOURIER the stored transform of the last nonlinear pass,
+ * FFS_SPACE_FOURIER_TO_REAL that transform back in real space (real!(tau), FluidFlowSimulation.jl:53-57) */
+#define FFS_FIELD_T11 5
+#define FFS_FIELD_T12 6
+#define FFS_FIELD_T13 7
+#define FFS_FIELD_T22 8
+#define FFS_FIELD_T23 9
 
 /* spaces for get/set_field */
 #define FFS_SPACE_REAL 0    /* padded real (2*(nx/2+1), ny, nz), column major; or unpadded (nx,ny,nz) on input */
 #define FFS_SPACE_FOURIER 1 /* half spectrum (nx/2+1, ny, nz) complex, interleaved re/im */
+#define FFS_SPACE_FOURIER_TO_REAL 2 /* get only: the stored transform back in real space (= REAL but for the stress) */
 
 /* scalars for ffs_get_scalar */
 #define FFS_SCALAR_ITERATION 0

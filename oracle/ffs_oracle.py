@@ -264,6 +264,7 @@ class OracleSim:
             szz = -Sr[0] - Sr[3]
             ss = Sr[0] ** 2 + Sr[3] ** 2 + szz**2 + 2 * (Sr[1] ** 2 + Sr[2] ** 2 + Sr[4] ** 2)
             taur = 2 * (self.les_alpha * np.sqrt(2 * ss)) * Sr
+            self.taur = taur  # the real-space stress writeoutput writes (T11 ... T23, output.jl:28-35)
         if self.equation == "NS":
             # traceless(symouter(u, -u)): xx xy xz yy yz, then div!(rhs, uu) = (i k) . uu
             xx, yy, zz = -(ur[0] * ur[0]), -(ur[1] * ur[1]), -(ur[2] * ur[2])

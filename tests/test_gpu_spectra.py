@@ -139,3 +139,36 @@ def test_run_simulation_writes_the_reference_files(tmp_path):
         for n_ in ("u", "vis", "nl", "press"):
             tab = np.loadtxt(os.path.join(d, f"{n_}.specH.{it}"), delimiter="\t")
             assert close(tab[:, 1], q[n_ + "_h"][0] + q[n_ + "_v"][0], 1e-11, floor)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+def test_les_stress_fields(prec, tol):
+    """T11 ... T23: the real-space Smagorinsky stress of the current velocity, and the stored
+    transform back in real space, against the oracle's."""
+    print_case(f"LES stress fields {prec}")
+    g = Grid(N, N, N)
+    u0 = random_spectrum(N, seed=61, k0=3.0)
+    ref = OracleSim(g, 2e-3, dt=1e-3, les_c=0.16)
+    ref.set_velocity(u0)
+    p = ffs.Params(nx=N, ny=N, nz=N, nu=2e-3, integrator="Adams_Bashforth3rdO", variable_dt=False, dt=1e-3,
+                   dt_stat=1 << 30, dt_spec=1 << 30, write_time=1 << 30, les_model="Smagorinsky",
+                   smagorinsky_c=0.16, precision=prec)
+    with ffs.Simulation(p) as s:
+        s.set_velocity(*u0)
+        ref.initialize()
+        s.initialize()
+        ref.advance_in_time(4)
+        s.advance(4)
+        ref.calculate_rhs(collect=False)   # the stress of the current velocity (realspace!)
+        names = (_lib.FFS_FIELD_T11, _lib.FFS_FIELD_T12, _lib.FFS_FIELD_T13, _lib.FFS_FIELD_T22,
+                 _lib.FFS_FIELD_T23)
+        for c, f in enumerate(names):
+            got = s.get_field(f, _lib.FFS_SPACE_REAL)[:N]
+            want = ref.taur[c]
+            assert np.abs(got - want).max() <= tol * np.abs(want).max(), (c, np.abs(got - want).max())
+        # the stored transform is the last nonlinear pass's (before the 4th step): compare after a pass
+        s.writestats_now()
+        for c, f in enumerate(names):
+            got = s.get_field(f, _lib.FFS_SPACE_FOURIER_TO_REAL)[:N]
+            want = g.irfft(ref.tauh[c])
+            assert np.abs(got - want).max() <= max(tol, 1e-11) * np.abs(want).max(), c
