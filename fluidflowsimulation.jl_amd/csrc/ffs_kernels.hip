@@ -1054,21 +1054,23 @@ __device__ __forceinline__ EtdCoef etd_coef(double c, int step, const DevState* 
 
 // fourierspacep2_velocity! (models.jl:282-341) on one mode: rhs += -nu k^2 u (AB3: explicit
 // viscosity), + (i k) . tau (LES), + rho g (Boussinesq), then the Leray projection.  RH receives rho.
-template <typename T>
+// FULL = false: the plain step (no LES, density, passive scalar, plane forcing or hyperviscosity), whose
+// optional paths are compiled out so the update kernel holds fewer registers
+template <typename T, bool FULL = true>
 __device__ __forceinline__ void form_rhs(const UpdateArgs& a, size_t idx, size_t n, const double kv3[3], double K2,
                                          bool origin, const double U[3][2], double R[3][2], double RH[2]) {
   if (a.integrator == 0) {  // explicit viscosity (is_implicit false): (K2 mnu + K2^M mnuh) v
     double kvis;
     {
 #pragma clang fp contract(off)
-      kvis = a.hv.kind == 1 ? K2 * (-a.nu) + ipow_m(K2, a.hv.m) * (-a.hv.nuh) : K2 * (-a.nu);
+      kvis = (FULL && a.hv.kind == 1) ? K2 * (-a.nu) + ipow_m(K2, a.hv.m) * (-a.hv.nuh) : K2 * (-a.nu);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int p = 0; p < 2; ++p) R[c][p] += kvis * U[c][p];
   }
-  if (a.tau) {  // rhs += (i k) . tau, tau symmetric traceless (xx xy xz yy yz)
+  if (FULL && a.tau) {  // rhs += (i k) . tau, tau symmetric traceless (xx xy xz yy yz)
     const C<T>* t = (const C<T>*)a.tau;
     const C<T> txx = t[idx], txy = t[n + idx], txz = t[2 * n + idx], tyy = t[3 * n + idx], tyz = t[4 * n + idx];
     const double T6[6][2] = {{txx.x, txx.y}, {txy.x, txy.y}, {txz.x, txz.y},
@@ -1083,7 +1085,7 @@ __device__ __forceinline__ void form_rhs(const UpdateArgs& a, size_t idx, size_t
       R[c][1] += sr;
     }
   }
-  if (a.rho) {  // buoyancy rhs += rho g, g = (0, 0, g_z)
+  if (FULL && a.rho) {  // buoyancy rhs += rho g, g = (0, 0, g_z)
     const C<T> r0 = ((const C<T>*)a.rho)[idx];
     RH[0] = r0.x;
     RH[1] = r0.y;
@@ -1098,7 +1100,7 @@ __device__ __forceinline__ void form_rhs(const UpdateArgs& a, size_t idx, size_t
   }
 }
 
-template <typename T>
+template <typename T, bool FULL>
 __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T>* __restrict__ rhs,
                                                const C<T>* __restrict__ fm1, const C<T>* __restrict__ fm2,
                                                const DevState* __restrict__ st, UpdateArgs a) {
@@ -1123,7 +1125,7 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
   }
   // plane forcing (rf / qf): the force is built from the pre-step u1, u2 (forcing.jl:85-90)
   double F0[2][2] = {{0, 0}, {0, 0}};
-  if (a.forcing >= 2 && a.step > 0) {
+  if (FULL && a.forcing >= 2 && a.step > 0) {
     double f = 0.0;
     if (cz == 0) {
       f = a.pf_factor[cx + (size_t)g.sx * cy];  // per-element factor of this step (0 where not forced)
@@ -1138,12 +1140,12 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
       for (int p = 0; p < 2; ++p) F0[c][p] = U[c][p] * f;
   }
   double RH[2] = {0, 0};
-  form_rhs<T>(a, idx, n, kv3, K2, origin, U, R, RH);
+  form_rhs<T, FULL>(a, idx, n, kv3, K2, origin, U, R, RH);
 #pragma unroll
   for (int c = 0; c < 3; ++c) rhs[c * n + idx] = mk<T>((T)R[c][0], (T)R[c][1]);
   // density rhs from the pre-step vertical velocity
   double RR[2] = {0, 0};
-  if (a.rho) {
+  if (FULL && a.rho) {
     const C<T>* f = (const C<T>*)a.flux;
     const C<T> fx = f[idx], fy = f[n + idx], fz = f[2 * n + idx];
     const double kf_r = fma(kv3[0], (double)fx.x, fma(kv3[1], (double)fy.x, kv3[2] * fz.x));
@@ -1154,7 +1156,7 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
   }
   // passive scalar rhs: div!(rhs, flux, u_dir, -scalarGradient) (models.jl:353-355) from the pre-step u
   double PR[2] = {0, 0}, PH[2] = {0, 0};
-  if (a.phi) {
+  if (FULL && a.phi) {
     const C<T> p0 = ((const C<T>*)a.phi)[idx];
     PH[0] = p0.x;
     PH[1] = p0.y;
@@ -1178,13 +1180,13 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
     M2[c][0] = h2.x; M2[c][1] = h2.y;
   }
   double HR1[2] = {0, 0}, HR2[2] = {0, 0};
-  if (a.rho) {
+  if (FULL && a.rho) {
     const C<T> h1 = ((const C<T>*)a.rfm1)[idx], h2 = ((const C<T>*)a.rfm2)[idx];
     HR1[0] = h1.x; HR1[1] = h1.y;
     HR2[0] = h2.x; HR2[1] = h2.y;
   }
   double HP1[2] = {0, 0}, HP2[2] = {0, 0};
-  if (a.phi) {
+  if (FULL && a.phi) {
     const C<T> h1 = ((const C<T>*)a.pfm1)[idx], h2 = ((const C<T>*)a.pfm2)[idx];
     HP1[0] = h1.x; HP1[1] = h1.y;
     HP2[0] = h2.x; HP2[1] = h2.y;
@@ -1197,21 +1199,25 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int p = 0; p < 2; ++p) M2[c][p] = -(dt3 + dt2) * (R[c][p] - M1[c][p]) / dt2 + R[c][p];
+      if constexpr (FULL) {
 #pragma unroll
-      for (int p = 0; p < 2; ++p) HR2[p] = -(dt3 + dt2) * (RR[p] - HR1[p]) / dt2 + RR[p];
+        for (int p = 0; p < 2; ++p) HR2[p] = -(dt3 + dt2) * (RR[p] - HR1[p]) / dt2 + RR[p];
 #pragma unroll
-      for (int p = 0; p < 2; ++p) HP2[p] = -(dt3 + dt2) * (PR[p] - HP1[p]) / dt2 + PR[p];
+        for (int p = 0; p < 2; ++p) HP2[p] = -(dt3 + dt2) * (PR[p] - HP1[p]) / dt2 + PR[p];
+      }
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int p = 0; p < 2; ++p) U[c][p] = fma(At, R[c][p], fma(Bt, M1[c][p], fma(Ct, M2[c][p], U[c][p])));
+    if constexpr (FULL) {
 #pragma unroll
-    for (int p = 0; p < 2; ++p) RH[p] = fma(At, RR[p], fma(Bt, HR1[p], fma(Ct, HR2[p], RH[p])));
+      for (int p = 0; p < 2; ++p) RH[p] = fma(At, RR[p], fma(Bt, HR1[p], fma(Ct, HR2[p], RH[p])));
 #pragma unroll
-    for (int p = 0; p < 2; ++p) PH[p] = fma(At, PR[p], fma(Bt, HP1[p], fma(Ct, HP2[p], PH[p])));
+      for (int p = 0; p < 2; ++p) PH[p] = fma(At, PR[p], fma(Bt, HP1[p], fma(Ct, HP2[p], PH[p])));
+    }
   } else {
-    const EtdCoef e = etd_coef(etd_c(a.hv, K2, kv3, -a.nu), a.step, st);
+    const EtdCoef e = etd_coef(FULL ? etd_c(a.hv, K2, kv3, -a.nu) : K2 * (-a.nu), a.step, st);
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -1220,7 +1226,7 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
         else if (a.step == 2) U[c][p] = fma(e.E, U[c][p], fma(e.A, R[c][p], e.B * M1[c][p]));
         else U[c][p] = fma(e.A, R[c][p], fma(e.B, M1[c][p], fma(e.Cc, M2[c][p], e.E * U[c][p])));
       }
-    if (a.rho) {
+    if (FULL && a.rho) {
       const EtdCoef d = etd_coef(etd_c(a.hv, K2, kv3, -a.alpha), a.step, st);
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
@@ -1229,7 +1235,7 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
         else RH[p] = fma(d.A, RR[p], fma(d.B, HR1[p], fma(d.Cc, HR2[p], d.E * RH[p])));
       }
     }
-    if (a.phi) {
+    if (FULL && a.phi) {
       const EtdCoef d = etd_coef(etd_c(a.hv, K2, kv3, -a.palpha), a.step, st);
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
@@ -1249,10 +1255,12 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
       for (int p = 0; p < 2; ++p) U[c][p] += fac * U[c][p];
   }
   // plane forcing: u_h += force (add_Rf_forcing!, forcing_types.jl:54-64)
+  if constexpr (FULL) {
 #pragma unroll
-  for (int c = 0; c < 2; ++c)
+    for (int c = 0; c < 2; ++c)
 #pragma unroll
-    for (int p = 0; p < 2; ++p) U[c][p] += F0[c][p];
+      for (int p = 0; p < 2; ++p) U[c][p] += F0[c][p];
+  }
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
     const double p1 = origin ? 0.0 : -(kv3[0] * U[0][p] + kv3[1] * U[1][p] + kv3[2] * U[2][p]) / K2;
@@ -1261,8 +1269,8 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) u[c * n + idx] = mk<T>((T)U[c][0], (T)U[c][1]);
-  if (a.rho) ((C<T>*)a.rho)[idx] = mk<T>((T)RH[0], (T)RH[1]);
-  if (a.phi) ((C<T>*)a.phi)[idx] = mk<T>((T)PH[0], (T)PH[1]);
+  if (FULL && a.rho) ((C<T>*)a.rho)[idx] = mk<T>((T)RH[0], (T)RH[1]);
+  if (FULL && a.phi) ((C<T>*)a.phi)[idx] = mk<T>((T)PH[0], (T)PH[1]);
 }
 
 // calc_les! Smagorinsky branch (les_real_calc.jl:77-79) on padded real S fields: tau = 2 (c Delta)^2
@@ -1321,7 +1329,11 @@ template <typename T>
 void launch_update(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
                    const UpdateArgs& a, hipStream_t s) {
   const size_t n = (size_t)g.sx * g.my * g.mz;
-  k_update<T><<<(n + 255) / 256, 256, 0, s>>>(g, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1, (const C<T>*)fm2, st, a);
+  const bool full = a.rho || a.phi || a.tau || a.forcing >= 2 || a.hv.kind != 0;
+  if (full)
+    k_update<T, true><<<(n + 255) / 256, 256, 0, s>>>(g, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1, (const C<T>*)fm2, st, a);
+  else
+    k_update<T, false><<<(n + 255) / 256, 256, 0, s>>>(g, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1, (const C<T>*)fm2, st, a);
 }
 
 // Leray projection of a spectral vector field (pressure_projection!, models.jl:443-456)
@@ -2032,19 +2044,6 @@ __global__ void k_plane_forcing(Geo g, const C<T>* __restrict__ u, const C<T>* _
     }
     if (pf.kind < 0) pf.Em[sl] = e;  // initialize!: the target is the initial spectrum
   }
-  // calculate_u1u2_spectrum! (u1u2_mag for QfForcing) over every shell: the Ef the reference keeps
-  for (int n = threadIdx.x; n < pf.nshell; n += blockDim.x) {
-    double e = 0.0;
-    for (int jj = pf.shell_ptr[n]; jj < pf.shell_ptr[n + 1]; ++jj) {
-      const int idx = pf.shell_el[jj];
-      const int cx = idx % g.sx;
-      const double w = cx == 0 ? 1.0 : 2.0;
-      const C<T> a = u1[idx], b = u2[idx];
-      const double m2 = (double)a.x * a.x + (double)a.y * a.y + ((double)b.x * b.x + (double)b.y * b.y);
-      e += kind == PF_QF ? w * sqrt(m2) : 0.5 * w * m2;
-    }
-    pf.Ef_all[n] = e;
-  }
   __syncthreads();
   if (threadIdx.x != 0 || pf.kind < 0) return;
   const double dt = st->dt;
@@ -2140,10 +2139,36 @@ __global__ void k_plane_forcing(Geo g, const C<T>* __restrict__ u, const C<T>* _
   st->force_vert = fvert;
 }
 
+// calculate_u1u2_spectrum! (u1u2_mag for QfForcing) over every shell of the kz = 0 plane, the Ef the
+// reference keeps (spectrum.jl:1-20, :44-65): one block per shell, a tree sum over its elements
+template <typename T>
+__global__ void __launch_bounds__(256) k_plane_shells(Geo g, const C<T>* __restrict__ u, PlaneForcing pf) {
+  __shared__ double part[256];
+  const size_t n = (size_t)g.sx * g.my * g.mz;
+  const int kind = pf.kind < 0 ? -pf.kind : pf.kind;
+  const int sh = blockIdx.x;
+  double e = 0.0;
+  for (int jj = pf.shell_ptr[sh] + threadIdx.x; jj < pf.shell_ptr[sh + 1]; jj += blockDim.x) {
+    const int idx = pf.shell_el[jj];
+    const double w = idx % g.sx == 0 ? 1.0 : 2.0;
+    const C<T> a = u[idx], b = u[n + idx];
+    const double m2 = (double)a.x * a.x + (double)a.y * a.y + ((double)b.x * b.x + (double)b.y * b.y);
+    e += kind == PF_QF ? w * sqrt(m2) : 0.5 * w * m2;
+  }
+  part[threadIdx.x] = e;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) pf.Ef_all[sh] = part[0];
+}
+
 template <typename T>
 void launch_plane_forcing(const Geo& g, const void* u, const void* rhs, DevState* st, const PlaneForcing& pf,
                           const UpdateArgs& ua, int rhs_final, int it, hipStream_t s) {
   k_plane_forcing<T><<<1, 64, 0, s>>>(g, (const C<T>*)u, (const C<T>*)rhs, st, pf, ua, rhs_final, it);
+  if (pf.nshell > 0) k_plane_shells<T><<<pf.nshell, 256, 0, s>>>(g, (const C<T>*)u, pf);
 }
 
 // one block of 64 threads: the forcing band sum (as k_forcing_sum) and, on thread 0, set_dt
