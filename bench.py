@@ -51,6 +51,8 @@ def kernel_bytes(g, name, elem, with_ke_frac):
         "fwd_y": (nfw * S2 + nfw * S1) * elem,
         "fwd_z": (nfw * S1 + nfw * S) * elem,
         "update": (12 * S + 6 * S) * elem,
+        # the forward z pass of u x w fused with the update: no rhs round trip
+        "fwd_z_update": (3 * S1 + 9 * S + 6 * S) * elem,
     }.get(name)
 
 
@@ -81,6 +83,8 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-max-runs", type=int, default=64)
     ap.add_argument("--serve-chunk", type=int, default=25)
+    ap.add_argument("--no-fusion", action="store_true",
+                    help="run the forward z pass and the update of plain steps as two kernels")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first time a plain 1 GiB device copy (HBM calibration for the traffic counters)")
     args = ap.parse_args()
@@ -196,6 +200,8 @@ def bench(args):
     b = 4 if args.fp32 else 8
     log(f"workload n={n} nu={nu:.6e} precision={prec}")
     s = make_forced(n, nu, prec, args.dt_stat, args.seed)
+    if args.no_fusion:
+        s.set_fusion(False)
     s.synchronize()
     log(f"initialized; device bytes {s.device_bytes() / 2**30:.2f} GiB")
     t0 = time.time()
@@ -219,10 +225,13 @@ def bench(args):
 
     mx, my, mz = retained(n)
     g = (n, n, n, mx, my, mz)
-    ke_frac = (args.steps // args.dt_stat) / args.steps if args.dt_stat > 0 else 0.0
+    n_ke = args.steps // args.dt_stat if args.dt_stat > 0 else 0  # steps that also transform 1/2|u|^2
     kernels = {}
     for name, (ms, cnt) in prof.items():
         avg_us = 1e3 * ms / cnt
+        # the share of this kernel's launches that carry 1/2|u|^2 (the separate forward z pass runs on
+        # those steps only when the plain steps are fused)
+        ke_frac = min(1.0, n_ke / cnt) if name == "fwd_z" else n_ke / args.steps
         kb = kernel_bytes(g, name, 2 * b, ke_frac)
         kernels[name] = {"launches": cnt, "avg_us": round(avg_us, 3), "share": round(ms / (1e3 * el_prof), 4)}
         if kb:
@@ -238,8 +247,8 @@ def bench(args):
     R = 2 * b * xr * yr * zr
     step_bytes = 54 * F + 15 * R
     step_eff = step_bytes / (ms_per_step * 1e-3) / 1e9
-    step_actual = sum(kernel_bytes(g, k, 2 * b, ke_frac) for k in ("inv_z_curl", "inv_y", "x_nl", "fwd_y", "fwd_z",
-                                                                  "update"))
+    # the pruned-layout bytes of the step as launched (fused or not), per step
+    step_actual = sum(v["alg_bytes"] * v["launches"] for v in kernels.values() if "alg_bytes" in v) / args.steps
     log(f"dominant kernel {dom}: {d['avg_us']:.1f} us, {achieved:.0f} GB/s; step model {step_eff:.0f} GB/s effective")
     for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["avg_us"] * kv[1]["launches"]):
         log(f"  {k:12s} {v}")

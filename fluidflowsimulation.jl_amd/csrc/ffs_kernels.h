@@ -188,6 +188,15 @@ struct UpdateArgs {
 template <typename T>
 void launch_update(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
                    const UpdateArgs& a, hipStream_t s);
+// whether an update needs the optional paths (LES, density, passive scalar, plane forcing, hyperviscosity)
+inline bool update_full(const UpdateArgs& a) {
+  return a.rho || a.phi || a.tau || a.forcing >= 2 || a.hv.kind != 0;
+}
+// the forward z pass of the 3 nonlinear components (b1, as launch_fwd_z with nzero 3) fused with the update
+// of a step for which update_full is false; rhs receives the final rhs, as launch_fwd_z + launch_update
+template <typename T>
+void launch_fwd_z_update(const Geo& g, const void* b1, void* u, void* rhs, const void* fm1, const void* fm2,
+                         const DevState* st, const UpdateArgs& a, double scale, const void* tw, hipStream_t s);
 // the Smagorinsky stress in real space: s[5] padded real S fields (xx xy xz yy yz) -> c2a |S| S in place
 template <typename T>
 void launch_les_tau_real(T* s5, size_t n_padded, double c2a, hipStream_t s);

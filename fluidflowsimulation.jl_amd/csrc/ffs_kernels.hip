@@ -1100,28 +1100,21 @@ __device__ __forceinline__ void form_rhs(const UpdateArgs& a, size_t idx, size_t
   }
 }
 
+// The update of one compact mode idx = (cx, cy, cz) from its nonlinear term R (the dealiased, scaled
+// F(u x w) as the forward z pass leaves it in rhs): the final rhs is stored to rhs, the new u to u.
 template <typename T, bool FULL>
-__global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T>* __restrict__ rhs,
-                                               const C<T>* __restrict__ fm1, const C<T>* __restrict__ fm2,
-                                               const DevState* __restrict__ st, UpdateArgs a) {
-  const size_t n = (size_t)g.sx * g.my * g.mz;
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  // s.time[] += dt, once per step (no kernel reads the time during the update)
-  if (a.time && idx == 0) *a.time += st->dt;
-  if (idx >= n) return;
-  const int cx = idx % g.sx;
-  if (cx >= g.mx) return;  // row padding
-  const int cy = (idx / g.sx) % g.my;
-  const int cz = idx / ((size_t)g.sx * g.my);
+__device__ __forceinline__ void update_point(const Geo& g, C<T>* __restrict__ u, C<T>* __restrict__ rhs,
+                                             const C<T>* __restrict__ fm1, const C<T>* __restrict__ fm2,
+                                             const DevState* __restrict__ st, const UpdateArgs& a, size_t idx,
+                                             size_t n, int cx, int cy, int cz, double (&R)[3][2]) {
   const double kv3[3] = {g.kx[cx], g.ky[cy], g.kz[cz]};
   const double K2 = kv3[0] * kv3[0] + kv3[1] * kv3[1] + kv3[2] * kv3[2];
   const bool origin = (idx == 0);
-  double U[3][2], R[3][2];
+  double U[3][2];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const C<T> v = u[c * n + idx], r = rhs[c * n + idx];
+    const C<T> v = u[c * n + idx];
     U[c][0] = v.x; U[c][1] = v.y;
-    R[c][0] = r.x; R[c][1] = r.y;
   }
   // plane forcing (rf / qf): the force is built from the pre-step u1, u2 (forcing.jl:85-90)
   double F0[2][2] = {{0, 0}, {0, 0}};
@@ -1273,6 +1266,92 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
   if (FULL && a.phi) ((C<T>*)a.phi)[idx] = mk<T>((T)PH[0], (T)PH[1]);
 }
 
+template <typename T, bool FULL>
+__global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T>* __restrict__ rhs,
+                                               const C<T>* __restrict__ fm1, const C<T>* __restrict__ fm2,
+                                               const DevState* __restrict__ st, UpdateArgs a) {
+  const size_t n = (size_t)g.sx * g.my * g.mz;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // s.time[] += dt, once per step (no kernel reads the time during the update)
+  if (a.time && idx == 0) *a.time += st->dt;
+  if (idx >= n) return;
+  const int cx = idx % g.sx;
+  if (cx >= g.mx) return;  // row padding
+  const int cy = (idx / g.sx) % g.my;
+  const int cz = idx / ((size_t)g.sx * g.my);
+  double R[3][2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const C<T> r = rhs[c * n + idx];
+    R[c][0] = r.x; R[c][1] = r.y;
+  }
+  update_point<T, FULL>(g, u, rhs, fm1, fm2, st, a, idx, n, cx, cy, cz, R);
+}
+
+// The forward z pass of the three nonlinear components fused with the update of the plain step (no
+// LES, density, passive scalar, plane forcing or hyperviscosity): a block transforms the columns of
+// its TX x-columns at one cy for all three components, keeps the results in registers and updates
+// those modes, so the nonlinear term never goes through rhs in HBM.  The values are those of the
+// forward z pass followed by the update kernel, bit for bit.
+template <typename T, int N>
+__global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES)
+    k_fwd_z_update(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ u, C<T>* __restrict__ rhs,
+                   const C<T>* __restrict__ fm1, const C<T>* __restrict__ fm2, const DevState* __restrict__ st,
+                   UpdateArgs a, const C<T>* __restrict__ tw_g, T scale) {
+  constexpr int TX = col_tx<T>(N);
+  constexpr int TP = col_tper<T, N, false>();
+  constexpr int MAXR = col_maxr<T, N, false>();
+  using Last = Stage<T, N, -1, plan_ns(N, MAXR) - 1, TP, MAXR>;
+  constexpr int NSL = Last::NB * Last::R;  // outputs per thread
+  extern __shared__ unsigned char smem_raw[];
+  C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
+  const C<T>* tw = stage_twiddles<T, N>(lds + N * TX, tw_g);
+  if (a.time && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *a.time += st->dt;
+  __syncthreads();
+  const int c = threadIdx.x % TX;
+  const int q = threadIdx.x / TX;
+  const int cx = blockIdx.x * TX + c;
+  const int cy = blockIdx.y;
+  const bool valid = cx < g.mx;
+  const size_t sx = g.sx;
+  const size_t n = sx * g.my * g.mz;
+  const size_t S1 = sx * g.my * g.nz;
+  auto ldsf = [&](int p) -> C<T>& { return lds[p * TX + c]; };
+  C<T> V[3][NSL];
+  int P[NSL];
+#pragma unroll
+  for (int m = 0; m < NSL; ++m) P[m] = -1;
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    if (f > 0) __syncthreads();
+    auto ld = [&](int p, int, int) -> C<T> {
+      if (!valid) return mk<T>(0, 0);
+      return in[f * S1 + cx + sx * (cy + (size_t)g.my * p)];
+    };
+    auto stv = [&](int p, int t, int r, C<T> v) {
+      V[f][t * Last::R + r] = v;
+      P[t * Last::R + r] = p;
+    };
+    Fft<T, N, -1, TP, false, false, SYNC_BLOCK, MAXR>::run(q, tw, ld, stv, ldsf);
+  }
+  if (!valid) return;
+#pragma unroll
+  for (int m = 0; m < NSL; ++m) {
+    const int cz = P[m] < 0 ? -1 : z_compact(g, P[m]);
+    if (cz < 0) continue;
+    const size_t idx = cx + sx * (cy + (size_t)g.my * cz);
+    const bool zero = cx == 0 && cy == 0 && cz == 0;  // the mean of the nonlinear term
+    double R[3][2];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const C<T> r = zero ? mk<T>(0, 0) : cscale<T>(V[f][m], scale);
+      R[f][0] = r.x;
+      R[f][1] = r.y;
+    }
+    update_point<T, false>(g, u, rhs, fm1, fm2, st, a, idx, n, cx, cy, cz, R);
+  }
+}
+
 // calc_les! Smagorinsky branch (les_real_calc.jl:77-79) on padded real S fields: tau = 2 (c Delta)^2
 // sqrt(2 S:S) S with Szz = -Sxx - Syy, as the x pass forms it
 template <typename T>
@@ -1329,11 +1408,40 @@ template <typename T>
 void launch_update(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
                    const UpdateArgs& a, hipStream_t s) {
   const size_t n = (size_t)g.sx * g.my * g.mz;
-  const bool full = a.rho || a.phi || a.tau || a.forcing >= 2 || a.hv.kind != 0;
+  const bool full = update_full(a);
   if (full)
     k_update<T, true><<<(n + 255) / 256, 256, 0, s>>>(g, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1, (const C<T>*)fm2, st, a);
   else
     k_update<T, false><<<(n + 255) / 256, 256, 0, s>>>(g, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1, (const C<T>*)fm2, st, a);
+}
+
+template <typename T, int N>
+static void run_fwd_z_update(const Geo& g, const void* b1, void* u, void* rhs, const void* fm1, const void* fm2,
+                             const DevState* st, const UpdateArgs& a, double scale, const void* tw, hipStream_t s) {
+  constexpr int TX = col_tx<T>(N);
+  constexpr int TP = col_tper<T, N, false>();
+  const size_t lds = (size_t)(N * TX + N) * sizeof(C<T>);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)&k_fwd_z_update<T, N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
+  }
+  const dim3 grid((g.mx + TX - 1) / TX, g.my);
+  k_fwd_z_update<T, N><<<grid, TX * TP, lds, s>>>(g, (const C<T>*)b1, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1,
+                                                  (const C<T>*)fm2, st, a, (const C<T>*)tw, (T)scale);
+}
+
+template <typename T>
+void launch_fwd_z_update(const Geo& g, const void* b1, void* u, void* rhs, const void* fm1, const void* fm2,
+                         const DevState* st, const UpdateArgs& a, double scale, const void* tw, hipStream_t s) {
+  if (update_full(a) || a.step < 1) throw std::runtime_error("fused forward z + update: not a plain step");
+  switch (g.nz) {
+#define X(n)                                                                                 \
+  case n: run_fwd_z_update<T, n>(g, b1, u, rhs, fm1, fm2, st, a, scale, tw, s); break;
+    FFS_SIZES(X)
+#undef X
+    default: bad_size("forward z pass", g.nz);
+  }
 }
 
 // Leray projection of a spectral vector field (pressure_projection!, models.jl:443-456)
@@ -2360,6 +2468,8 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   template void launch_fwd_z<T>(const Geo&, const void*, void* const*, int, int, double, const void*, hipStream_t);   \
   template void launch_update<T>(const Geo&, void*, void*, const void*, const void*, const DevState*,                 \
                                  const UpdateArgs&, hipStream_t);                                                     \
+  template void launch_fwd_z_update<T>(const Geo&, const void*, void*, void*, const void*, const void*,              \
+                                       const DevState*, const UpdateArgs&, double, const void*, hipStream_t);       \
   template void launch_project<T>(const Geo&, void*, hipStream_t);                                                   \
   template void launch_div_uu<T>(const Geo&, const void*, void*, hipStream_t);                                     \
   template void launch_les_tau_real<T>(T*, size_t, double, hipStream_t);                                          \

@@ -166,6 +166,8 @@ struct ffs_sim {
   std::deque<std::pair<int64_t, std::vector<double>>> spec_ready;  // iteration, kSpecQ x (H[nh], V[nv])
   std::vector<double> spec_kh, spec_krz;
 
+  // plain steps run the forward z pass and the update as one kernel (ffs_set_fusion)
+  bool fuse_ok = true;
   // steady-state graphs: one per history-rotation phase
   bool graph_on = true;
   int phase = 0;  // history rotations mod 3
@@ -727,7 +729,9 @@ struct ffs_sim {
   // calculate_rhs! up to (not including) fourierspacep2!: rhs <- dealias(F(u x w))/N^3,
   // flux <- F(-rho u), tau <- F(2 (c Delta)^2 |S| S), ke <- F(1/2|u|^2) on stats/spectra steps,
   // max|u_i| / max|rho| into st (variable dt)
-  void nonlinear(bool need_ke) {
+  // fused = true (a plain step, fuse_update()): set_dt and the forcing sum run before the forward y pass,
+  // and the forward z pass ends in the update, so the step is complete when this returns
+  void nonlinear(bool need_ke, bool fused = false) {
     launch("inv_z_curl", [&] {
       FFS_DISPATCH(launch_inv_z, g, u, w1, nin, true, tw_z, stream, rho, phi, nsform ? 1 : 0);
     });
@@ -745,6 +749,16 @@ struct ffs_sim {
     unsigned long long* rm = (p.variable_dt && dens) ? &st->rhomax : nullptr;
     launch("x_nl", [&] { FFS_DISPATCH(launch_x_nl, g, w2, w1, ops, um, rm, tw_x, stream); });
     const int nf = ops.nout;
+    if (fused) {
+      step_scalars(true, true);  // reads u and the maximum the x pass left, as after the forward passes
+      launch("fwd_y", [&] { FFS_DISPATCH(launch_fwd_y, g, w1, w2, nf, tw_y, stream); });
+      UpdateArgs a = update_args((int)std::min<int64_t>(tit + 1, 3));
+      a.time = &st->time;
+      launch("fwd_z_update", [&] {
+        FFS_DISPATCH(launch_fwd_z_update, g, w2, u, rhs, fm1, fm2, st, a, scale, tw_z, stream);
+      });
+      return;
+    }
     launch("fwd_y", [&] { FFS_DISPATCH(launch_fwd_y, g, w1, w2, nf, tw_y, stream); });
     void* dst[kMaxFields] = {};
     int f = 0;
@@ -1090,8 +1104,18 @@ struct ffs_sim {
     return tit >= 2 && iteration >= 4 && !is_stats_time(iteration) && !is_spec_time(iteration) && !prof;
   }
 
+  // the plain step: the rotational form without LES, density, passive scalar, plane forcing or
+  // hyperviscosity, whose update the forward z pass carries out itself
+  bool fuse_update() const {
+    return fuse_ok && !nsform && !dens && !pscal && !les && !plane && p.hyperviscosity == FFS_HV_NONE;
+  }
+
   // kernels of one steady step (no host bookkeeping), for capture
   void enqueue_steady_step() {
+    if (fuse_update()) {
+      nonlinear(false, true);
+      return;
+    }
     nonlinear(false);
     step_scalars(true, true);
     enqueue_update();
@@ -1115,6 +1139,11 @@ struct ffs_sim {
     }
     const bool stats_t = is_stats_time(iteration);
     const bool spec_t = is_spec_time(iteration);
+    if (!stats_t && !spec_t && fuse_update()) {
+      nonlinear(false, true);
+      rotate();
+      return;
+    }
     nonlinear(stats_t || spec_t);
     step_scalars(true, true);
     if (spec_t) queue_spectra();
@@ -1982,6 +2011,16 @@ int ffs_set_graph(ffs_sim* s, int on) {
       HIPCHECK(hipStreamSynchronize(s->stream));
       s->drop_graphs();
     }
+  });
+}
+
+int ffs_set_fusion(ffs_sim* s, int on) {
+  if (!s) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    if (s->fuse_ok == (on != 0)) return;
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    s->drop_graphs();  // the captured steps hold the other kernel sequence
+    s->fuse_ok = on != 0;
   });
 }
 

@@ -126,6 +126,7 @@ SIGNATURES = {
     "ffs_footprint": (ctypes.c_int, [ctypes.POINTER(FfsParams), ctypes.POINTER(ctypes.c_size_t)]),
     "ffs_device_memory": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "ffs_set_graph": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ffs_set_fusion": (ctypes.c_int, [_P, ctypes.c_int]),
     "ffs_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(_P)]),
     "ffs_batch_advance": (ctypes.c_int, [_P, ctypes.c_int64, _I64, _I, _I]),
     "ffs_batch_synchronize": (ctypes.c_int, [_P]),

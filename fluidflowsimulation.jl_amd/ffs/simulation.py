@@ -180,6 +180,11 @@ class Simulation:
         """Replay steady steps as HIP graphs (default on; bitwise identical to eager)."""
         self._check(self._lib.ffs_set_graph(self._h, int(on)))
 
+    def set_fusion(self, on=True):
+        """Run the forward z pass and the update of plain steps as one kernel (default on; bitwise
+        identical to the two kernels)."""
+        self._check(self._lib.ffs_set_fusion(self._h, int(on)))
+
     # ---------------------------------------------------------------- diagnostics
     def pop_stats(self):
         """Returns the queued stats records as a list of (stats, scales, numbers) rows."""

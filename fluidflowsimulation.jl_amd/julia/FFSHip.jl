@@ -355,6 +355,7 @@ function device_memory()
 end
 
 set_graph!(s::FFSSimulation, on::Bool) = check(ccall((:ffs_set_graph, libffs), Cint, (Ptr{Cvoid}, Cint), s.h, on), s.h)
+set_fusion!(s::FFSSimulation, on::Bool) = check(ccall((:ffs_set_fusion, libffs), Cint, (Ptr{Cvoid}, Cint), s.h, on), s.h)
 
 # ---- ensembles: runs advanced in lockstep, one graph launch per steady step ----
 function batch_check(rc::Integer, b::Ptr{Cvoid})

@@ -308,6 +308,14 @@ int ffs_device_memory(size_t* free_bytes, size_t* total_bytes);
  * reference (the Julia step is host code). */
 int ffs_set_graph(ffs_sim* s, int on);
 
+/* Plain steps (the rotational form without LES, density, passive scalar,
+ * plane forcing or hyperviscosity, and no stats or spectra at that iteration)
+ * run the forward z transform of the nonlinear term and the time-step update
+ * as one kernel, so the nonlinear term is not written to and read back from
+ * device memory; the results are bitwise those of the two kernels.  on = 0
+ * runs them as two kernels.  Default on.  No counterpart in the reference. */
+int ffs_set_fusion(ffs_sim* s, int on);
+
 /* An ensemble: up to FFS_MAX_BATCH initialized handles on the current device
  * advanced in lockstep.  A step in which every run is steady is one graph
  * launch whose branches are the runs' steps (independent, so they overlap on
