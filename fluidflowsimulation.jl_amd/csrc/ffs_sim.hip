@@ -166,7 +166,7 @@ struct ffs_sim {
   std::deque<std::pair<int64_t, std::vector<double>>> spec_ready;  // iteration, kSpecQ x (H[nh], V[nv])
   std::vector<double> spec_kh, spec_krz;
 
-  // plain steps run the forward z pass and the update as one kernel (ffs_set_fusion)
+  // plain steps run the forward z pass and the update as one kernel (ffs_set_fusion; default FP64 only)
   bool fuse_ok = true;
   // steady-state graphs: one per history-rotation phase
   bool graph_on = true;
@@ -638,6 +638,8 @@ struct ffs_sim {
       throw FfsError(FFS_ERR_ARG, "dtStat, dtSpec and writeTime must be positive");
     if (!p.variable_dt && !(p.dt > 0)) throw FfsError(FFS_ERR_ARG, "a fixed time step needs dt > 0");
     fp32 = (p.precision == FFS_FP32);
+    // the FP32 tile (16 columns x 32 threads) holds too many registers to fuse profitably
+    fuse_ok = !fp32;
     if (dry) {  // footprint: the same allocations, counted, plus the transient staging of set_field
       setup_grid();
       setup_buffers();

@@ -181,8 +181,8 @@ class Simulation:
         self._check(self._lib.ffs_set_graph(self._h, int(on)))
 
     def set_fusion(self, on=True):
-        """Run the forward z pass and the update of plain steps as one kernel (default on; bitwise
-        identical to the two kernels)."""
+        """Run the forward z pass and the update of plain steps as one kernel (default on for FP64, off
+        for FP32; bitwise identical to the two kernels)."""
         self._check(self._lib.ffs_set_fusion(self._h, int(on)))
 
     # ---------------------------------------------------------------- diagnostics

@@ -313,7 +313,9 @@ int ffs_set_graph(ffs_sim* s, int on);
  * run the forward z transform of the nonlinear term and the time-step update
  * as one kernel, so the nonlinear term is not written to and read back from
  * device memory; the results are bitwise those of the two kernels.  on = 0
- * runs them as two kernels.  Default on.  No counterpart in the reference. */
+ * runs them as two kernels.  Default on for FP64, off for FP32 (whose fused
+ * tile holds too many registers to be faster).  No counterpart in the
+ * reference. */
 int ffs_set_fusion(ffs_sim* s, int on);
 
 /* An ensemble: up to FFS_MAX_BATCH initialized handles on the current device

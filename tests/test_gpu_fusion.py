@@ -5,7 +5,8 @@ or hyperviscosity, no stats or spectra at that iteration) the forward z
 transform of the nonlinear term and the update run as one kernel.  It computes
 the same operations on the same values as the forward z kernel followed by the
 update kernel, so every field, the time step and every stats row must be
-bitwise identical to the unfused run, with and without graph replay."""
+bitwise identical to the unfused run, with and without graph replay.  (FP64
+fuses by default; the FP32 cases ask for it.)"""
 import numpy as np
 import pytest
 
@@ -56,6 +57,7 @@ def test_fused_update_matches_two_kernels(name, shape, kw, graph):
     p = params(shape, **kw)
     with start(p, 3) as fused, start(p, 3) as split:
         split.set_fusion(False)
+        fused.set_fusion(True)  # FP32 runs the two kernels unless asked
         fused.set_graph(graph)
         split.set_graph(graph)
         for step in range(1, 13):
@@ -81,6 +83,7 @@ def test_fusion_toggle_mid_run():
     p = params((32, 32, 32), forcing="shellForcing", forcing_kf=2.5)
     with start(p, 9) as a, start(p, 9) as b:
         a.set_fusion(False)
+        b.set_fusion(True)
         b.advance(10)
         b.set_fusion(False)
         b.advance(7)
