@@ -125,12 +125,26 @@ void launch_inv_z(const Geo& g, const void* u, void* a1, int nf, bool curl, cons
 template <typename T>
 void launch_inv_y(const Geo& g, const void* a1, void* a2, int nf, const void* tw, hipStream_t s);
 // field set of the fused x pass
+// pointwise LES models of calc_les! (les_real_calc.jl:74-114); codes as FFS_LES_* of the C-ABI
+enum : int { LES_SMAGORINSKY = 1, LES_VREMAN = 2, LES_SILVIS = 3, LES_PRODUCTION = 4, LES_STABLE_NL = 5,
+             LES_FAKE_SMAGORINSKY = 6, LES_SANDP = 7 };
+struct LesArgs {
+  int model;   // LES_*
+  int smodel;  // SandP's S model
+  double c;    // the model constant (SandP: its S model's)
+  double cp;   // Silvis P constant or SandP's cb
+  double d2;   // Delta^2
+};
+// whether the model reads the vorticity (every one but Smagorinsky)
+inline bool les_needs_w(const LesArgs& l) { return l.model != LES_SMAGORINSKY; }
+
 struct XOps {
   int nin, nout;      // physical inputs (6 [+1 rho] [+1 phi] [+5 S]) and forward outputs (3 [+3] [+3] [+5] [+1 ke])
   int rho, les, ke;   // flags: rho, phi and les in that order after u x w, ke last
   int phi;
   int ns;             // conservation form: inputs u (no w), outputs uu = traceless(-u u) (5) in place of u x w
   double les_2alpha;  // 2 (c Delta)^2 of the Smagorinsky stress
+  LesArgs lm;         // the LES model (lm.model != LES_SMAGORINSKY: the general x pass, with w at slots 3..5)
 };
 template <typename T>
 void launch_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
@@ -200,6 +214,9 @@ void launch_fwd_z_update(const Geo& g, const void* b1, void* u, void* rhs, const
 // the Smagorinsky stress in real space: s[5] padded real S fields (xx xy xz yy yz) -> c2a |S| S in place
 template <typename T>
 void launch_les_tau_real(T* s5, size_t n_padded, double c2a, hipStream_t s);
+// the same for a model that reads the vorticity: w3[3] padded real w fields
+template <typename T>
+void launch_les_model_real(T* s5, const T* w3, size_t n_padded, const LesArgs& l, hipStream_t s);
 // conservation form: rhs = (i k) . uu (div!, vectorfunctions.jl:107-117 on the SymTrTen field)
 template <typename T>
 void launch_div_uu(const Geo& g, const void* uu, void* rhs, hipStream_t s);

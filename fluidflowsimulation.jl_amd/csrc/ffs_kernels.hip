@@ -412,6 +412,136 @@ __device__ __forceinline__ void split_pair(const C<T>* Z, int k, C<T>& A, C<T>& 
   B = mk<T>((T)0.5 * (zk.y + zn.y), (T)0.5 * (zn.x - zk.x));
 }
 
+// calc_les! for the models that read the vorticity w (les_real_calc.jl:80-114): S (xx xy xz yy yz, zz =
+// -xx - yy) and w -> the five stored components of t.  AntiSymTen(v) is the cross-product matrix of v
+// (AntiSymTen(v) x = v x x), so A = AntiSymTen(-w/2) has A_xy = w_z/2, A_xz = -w_y/2, A_yz = w_x/2 and
+// S + A is the transposed velocity gradient; Lie(a, b) = ab - ba; a:b sums every product a_ij b_ij.
+template <typename T>
+__device__ __forceinline__ void les_stress(const LesArgs& l, const T s[5], const T w[3], T t[5]) {
+  const T S[3][3] = {{s[0], s[1], s[2]}, {s[1], s[3], s[4]}, {s[2], s[4], -s[0] - s[3]}};
+  const T hx = (T)0.5 * w[0], hy = (T)0.5 * w[1], hz = (T)0.5 * w[2];
+  const T A[3][3] = {{0, hz, -hy}, {-hz, 0, hx}, {hy, -hx, 0}};  // AntiSymTen(-w/2)
+  const T d2 = (T)l.d2;
+  T ss = 0;  // S:S
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ss += S[i][j] * S[i][j];
+  // P = Lie(S, AntiSymTen(w/2)) = A S - S A (Silvis, SandP)
+  auto lie_p = [&](T P[3][3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        T v = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v += A[i][k] * S[k][j] - S[i][k] * A[k][j];
+        P[i][j] = v;
+      }
+  };
+  // c Delta^2 (square(W):S - square(S):S) / (2 S:S), W = A (production_eddy_viscosity, stable_nl_eddy_viscosity)
+  auto prod_x = [&](T c) {
+    T ws = 0, s3 = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        T w2 = 0, q2 = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          w2 += A[i][k] * A[k][j];
+          q2 += S[i][k] * S[k][j];
+        }
+        ws += w2 * S[i][j];
+        s3 += q2 * S[i][j];
+      }
+    return c * d2 * (ws - s3) / ((T)2 * ss);
+  };
+  auto eddy_nu = [&](int model, T c) -> T {  // nu_t of an eddy-viscosity model, t = 2 nu_t S
+    if (model == LES_SMAGORINSKY) return c * c * d2 * sqrt((T)2 * ss);
+    if (model == LES_VREMAN) {  // Vreman_eddy_viscosity: alpha = S + A, beta = Delta^2 alpha' alpha
+      T a[3][3], aa = 0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          a[i][j] = S[i][j] + A[i][j];
+          aa += a[i][j] * a[i][j];
+        }
+      auto beta = [&](int i, int j) {
+        T v = 0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) v += a[m][i] * a[m][j];
+        return d2 * v;
+      };
+      const T b11 = beta(0, 0), b22 = beta(1, 1), b33 = beta(2, 2), b12 = beta(0, 1), b13 = beta(0, 2),
+              b23 = beta(1, 2);
+      T B = b11 * (b22 + b33) + b22 * b33 - (b12 * b12 + b13 * b13 + b23 * b23);
+      B = B < (T)0 ? (T)0 : B;
+      return c * sqrt(B / aa);
+    }
+    if (model == LES_PRODUCTION) return ss == (T)0 ? (T)0 : fmax((T)0, prod_x(c));
+    return (T)0;  // fakeSmagorinsky: nu_t = 0
+  };
+  T R[3][3];
+  if (l.model == LES_SANDP) {
+    const T nu = eddy_nu(l.smodel, (T)l.c);
+    T P[3][3];
+    lie_p(P);
+    const T beta = (T)l.cp * d2;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) R[i][j] = (l.smodel == LES_FAKE_SMAGORINSKY ? (T)0 : (T)2 * nu * S[i][j]) + beta * P[i][j];
+  } else if (l.model == LES_SILVIS) {
+    T sw[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) sw[i] = S[i][0] * w[0] + S[i][1] * w[1] + S[i][2] * w[2];
+    const T f = sqrt(sw[0] * sw[0] + sw[1] * sw[1] + sw[2] * sw[2]) /
+                (sqrt(ss) * sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]));  // fvs
+    const T c = (T)l.c, cp = (T)l.cp;
+    const T nut = c * c * d2 * f * f * f * sqrt(ss) / (T)2;
+    const T nup = cp * d2 * f * f * f * f / (T)4;
+    T P[3][3];
+    lie_p(P);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) R[i][j] = (T)2 * nut * S[i][j] + nup * P[i][j];
+  } else if (l.model == LES_STABLE_NL) {  // c Delta^2 traceless(W^2 - Lie(S, W) - S^2) + 2 mnu S, W = A
+    const T c = (T)l.c;
+    const T x = prod_x(c);
+    const T mnu = ss == (T)0 ? (T)0 : -fmin((T)0, x);
+    T M[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        T v = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          v += A[i][k] * A[k][j] - (S[i][k] * A[k][j] - A[i][k] * S[k][j]) - S[i][k] * S[k][j];
+        M[i][j] = v;
+      }
+    const T tr = (M[0][0] + M[1][1] + M[2][2]) / (T)3;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) R[i][j] = c * d2 * (M[i][j] - (i == j ? tr : (T)0)) + (T)2 * mnu * S[i][j];
+  } else {  // Vreman, productionViscosity, Smagorinsky
+    const T nu = eddy_nu(l.model, (T)l.c);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) R[i][j] = (T)2 * nu * S[i][j];
+  }
+  t[0] = R[0][0];
+  t[1] = R[0][1];
+  t[2] = R[0][2];
+  t[3] = R[1][1];
+  t[4] = R[1][2];
+}
+
 // Fused nonlinear x pass, general field set (LDS slots; any transform length).
 // Inputs (a2, y/z-transformed): u (0..2), w (3..5), [rho], [S xx xy xz yy yz].
 // Outputs (b2, x-forward transformed): u x w (0..2), [-rho u (3)], [tau =
@@ -503,7 +633,14 @@ __global__ void __launch_bounds__(256) k_x_gen(Geo g, const C<T>* __restrict__ a
       out[o++] = (-phi) * uy;
       out[o++] = (-phi) * uz;
     }
-    if (ops.les) {
+    if (ops.les && ops.lm.model != LES_SMAGORINSKY) {  // the models that read w (slots 3..5)
+      const T sv[5] = {in[i], in[i + 1], in[i + 2], in[i + 3], in[i + 4]};
+      const T wv[3] = {in[3], in[4], in[5]};
+      T tv[5];
+      les_stress<T>(ops.lm, sv, wv, tv);
+#pragma unroll
+      for (int e = 0; e < 5; ++e) out[o++] = tv[e];
+    } else if (ops.les) {
       const T sxx = in[i], sxy = in[i + 1], sxz = in[i + 2], syy = in[i + 3], syz = in[i + 4];
       const T szz = -sxx - syy;
       const T ss = sxx * sxx + syy * syy + szz * szz + (T)2 * (sxy * sxy + sxz * sxz + syz * syz);
@@ -875,8 +1012,9 @@ template <typename T, int N>
 static void run_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
                      unsigned long long* rhomax, const void* tw, hipStream_t s) {
   const bool plain = !ops.rho && !ops.les && !ops.phi;
+  const bool w_les = ops.les && les_needs_w(ops.lm);  // the general pass holds u, w and S of a row together
   if constexpr (xreg_ok<T, N>()) {
-    if (FFS_XNL_MODES || plain) {
+    if ((FFS_XNL_MODES || plain) && !w_les) {
       // u x w or traceless(-u u) (+ 1/2|u|^2, always the last output slot), then -rho u, -phi u and
       // tau as launches of their own
       XRegArgs xa{0, 0, ops.nout - 1, 0.0};
@@ -1372,6 +1510,25 @@ __global__ void __launch_bounds__(256) k_les_tau_real(T* __restrict__ s5, size_t
 template <typename T>
 void launch_les_tau_real(T* s5, size_t n_padded, double c2a, hipStream_t s) {
   k_les_tau_real<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(s5, n_padded, c2a);
+}
+
+// calc_les! of a model that reads w, on padded real S and w fields (S overwritten by t)
+template <typename T>
+__global__ void __launch_bounds__(256) k_les_model_real(T* __restrict__ s5, const T* __restrict__ w3, size_t n,
+                                                        LesArgs l) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T sv[5] = {s5[i], s5[n + i], s5[2 * n + i], s5[3 * n + i], s5[4 * n + i]};
+  const T wv[3] = {w3[i], w3[n + i], w3[2 * n + i]};
+  T tv[5];
+  les_stress<T>(l, sv, wv, tv);
+#pragma unroll
+  for (int e = 0; e < 5; ++e) s5[e * n + i] = tv[e];
+}
+
+template <typename T>
+void launch_les_model_real(T* s5, const T* w3, size_t n_padded, const LesArgs& l, hipStream_t s) {
+  k_les_model_real<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(s5, w3, n_padded, l);
 }
 
 // div!(rhs, uu) of the conservation form (models.jl:176-180): rhs_c = (i k) . uu, uu symmetric
@@ -2473,6 +2630,7 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   template void launch_project<T>(const Geo&, void*, hipStream_t);                                                   \
   template void launch_div_uu<T>(const Geo&, const void*, void*, hipStream_t);                                     \
   template void launch_les_tau_real<T>(T*, size_t, double, hipStream_t);                                          \
+  template void launch_les_model_real<T>(T*, const T*, size_t, const LesArgs&, hipStream_t);                      \
   template int launch_stats<T>(const Geo&, const void*, const void*, const void*, const StatsAux&, double*, double*,  \
                                hipStream_t);                                                                       \
   template void launch_spectrum<T>(const Geo&, const void*, double*, double*, double*, double*, double*, double*,     \

@@ -118,6 +118,7 @@ struct ffs_sim {
   bool les = false, dens = false, etd = false, pscal = false, nsform = false;
   int nin = 6, nfw = 3;     // physical inputs / forward outputs of the nonlinear term (without 1/2|u|^2)
   double les_alpha = 0.0;   // (c Delta)^2
+  LesArgs lesm{};           // the LES model (lesm.model = p.les_model)
   double alpha = 0.0, gz = 0.0, drhodz = 0.0;
   double palpha = 0.0;  // passive scalar diffusivity nu / scalarPr
   void* w1 = nullptr;
@@ -591,8 +592,14 @@ struct ffs_sim {
     p = in;
     if (p.size != sizeof(ffs_params)) throw FfsError(FFS_ERR_ARG, "ffs_params.size mismatch (ABI version)");
     if (p.integrator != FFS_AB3 && p.integrator != FFS_ETD3) throw FfsError(FFS_ERR_ARG, "unknown integrator");
-    if (p.les_model != FFS_LES_NONE && p.les_model != FFS_LES_SMAGORINSKY)
-      throw FfsError(FFS_ERR_ARG, "unsupported LES model (Smagorinsky is available)");
+    if (p.les_model < FFS_LES_NONE || p.les_model > FFS_LES_SANDP)
+      throw FfsError(FFS_ERR_ARG, "unknown LES model");
+    if (p.les_model == FFS_LES_FAKE_SMAGORINSKY)
+      throw FfsError(FFS_ERR_ARG, "fakeSmagorinsky is built only as the S model of SandP");
+    if (p.les_model == FFS_LES_SANDP &&
+        p.les_s_model != FFS_LES_SMAGORINSKY && p.les_s_model != FFS_LES_VREMAN &&
+        p.les_s_model != FFS_LES_PRODUCTION && p.les_s_model != FFS_LES_FAKE_SMAGORINSKY)
+      throw FfsError(FFS_ERR_ARG, "SandP's sLesModel: Smagorinsky, Vreman, productionViscosity or fakeSmagorinsky");
     if (p.forcing < FFS_FORCING_NONE || p.forcing > FFS_FORCING_A) throw FfsError(FFS_ERR_ARG, "unsupported forcing");
     plane = p.forcing >= FFS_FORCING_RF && p.forcing <= FFS_FORCING_A;
     if (plane && p.forcing != FFS_FORCING_QF && !(p.forcing_tf > 0))
@@ -608,7 +615,17 @@ struct ffs_sim {
       // (models.jl:298-300, hyperviscosity_types.jl:13-14)
       throw FfsError(FFS_ERR_ARG, "spectralBarrier needs velocityTimeStep ETD3rdO");
     etd = (p.integrator == FFS_ETD3);
-    les = (p.les_model == FFS_LES_SMAGORINSKY);
+    les = (p.les_model != FFS_LES_NONE);
+    if (std::isnan(p.smagorinsky_c)) {  // the constants' defaults (les_types.jl:364-404)
+      const int m = p.les_model == FFS_LES_SANDP ? p.les_s_model : p.les_model;
+      p.smagorinsky_c = m == FFS_LES_VREMAN ? 0.032
+                        : m == FFS_LES_SILVIS ? std::sqrt(0.17)
+                        : m == FFS_LES_PRODUCTION ? (p.les_model == FFS_LES_SANDP ? 0.1 : 0.4)
+                        : m == FFS_LES_STABLE_NL ? 0.1
+                        : m == FFS_LES_FAKE_SMAGORINSKY ? 0.0
+                                                        : 0.17;
+    }
+    if (std::isnan(p.les_cp)) p.les_cp = p.les_model == FFS_LES_SILVIS ? 5.0 : (p.les_model == FFS_LES_SANDP ? 0.09 : 0.0);
     dens = (p.density != 0);
     pscal = (p.passive_scalar != 0);
     if (p.equation != FFS_EQ_ROTATIONAL && p.equation != FFS_EQ_NS) throw FfsError(FFS_ERR_ARG, "unknown equation");
@@ -621,9 +638,19 @@ struct ffs_sim {
       palpha = p.nu / p.scalar_pr;  // nu / Inf = 0
     }
     if (les) {
-      // Delta default (2 pi lx / nx) / cutoff (les_types.jl:366)
-      const double delta = p.filter_width > 0 ? p.filter_width : (p.x_domain * 2 * M_PI / p.nx) / p.cutoff;
+      // Delta default (2 pi lx / nx) / cutoff, twice that for SandP's Smagorinsky (les_types.jl:366-401)
+      const bool twice = p.les_model == FFS_LES_SANDP && p.les_s_model == FFS_LES_SMAGORINSKY;
+      const double delta = p.filter_width > 0 ? p.filter_width
+                                              : (twice ? 2.0 : 1.0) * (p.x_domain * 2 * M_PI / p.nx) / p.cutoff;
       les_alpha = p.smagorinsky_c * p.smagorinsky_c * delta * delta;
+      lesm.model = p.les_model;
+      lesm.smodel = p.les_s_model;
+      lesm.c = p.smagorinsky_c;
+      lesm.cp = p.les_cp;
+      lesm.d2 = delta * delta;
+      // the models that read w take it from the rotational form's curl (need_vorticity, simulationtypes.jl:41)
+      if (les_needs_w(lesm) && p.equation == FFS_EQ_NS)
+        throw FfsError(FFS_ERR_ARG, "LES models other than Smagorinsky need equation = rotational here");
     }
     if (dens) {
       if (!(p.pr > 0) || !(p.reference_density != 0))
@@ -747,6 +774,7 @@ struct ffs_sim {
     ops.les = les ? 1 : 0;
     ops.ke = need_ke ? 1 : 0;
     ops.les_2alpha = 2 * les_alpha;
+    ops.lm = lesm;
     unsigned long long* um = p.variable_dt ? &st->umax : nullptr;
     unsigned long long* rm = (p.variable_dt && dens) ? &st->rhomax : nullptr;
     launch("x_nl", [&] { FFS_DISPATCH(launch_x_nl, g, w2, w1, ops, um, rm, tw_x, stream); });
@@ -1239,7 +1267,7 @@ struct ffs_sim {
     h.ny = p.ny;
     h.nz = p.nz;
     h.integrator = p.integrator;
-    h.les = les ? 1 : 0;
+    h.les = les ? p.les_model : 0;
     h.dens = dens ? 1 : 0;
     h.pscal = pscal ? 1 : 0;
     h.variable_dt = p.variable_dt;
@@ -1476,7 +1504,9 @@ int ffs_params_default(ffs_params* p) {
   p->dt = 0.0;
   p->dt_stat = p->dt_spec = p->write_time = 1 << 30;
   p->precision = FFS_FP64;
-  p->smagorinsky_c = 0.17;
+  p->smagorinsky_c = NAN;  // the model's own default (les_types.jl:364-404)
+  p->les_cp = NAN;
+  p->les_s_model = FFS_LES_SMAGORINSKY;
   p->filter_width = 0.0;
   p->pr = 1.0;
   p->reference_density = 1.0;
@@ -1684,14 +1714,32 @@ int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n) {
       const int s0 = (s->nsform ? 3 : 6) + (s->dens ? 1 : 0) + (s->pscal ? 1 : 0);
       const size_t S2 = (size_t)s->g.sx * s->g.ny * s->g.nz;
       void* a2 = s->cptr(s->w2, (size_t)s0 * S2);
+      const bool w_les = les_needs_w(s->lesm);
+      void* wreal = w_les ? s->dalloc(3 * padded * tb) : nullptr;  // w, for the models that read it
+      void* a2w = s->cptr(s->w2, (size_t)3 * S2);                   // w at slots 3..5 (rotational form)
       if (s->fp32) {
         s->launch("x_c2r", [&] { launch_x_c2r<float>(s->g, a2, (float*)real, 5, s->tw_x, s->stream); });
-        s->launch("les_tau_real", [&] { launch_les_tau_real<float>((float*)real, padded, 2 * s->les_alpha, s->stream); });
+        if (w_les) {
+          s->launch("x_c2r", [&] { launch_x_c2r<float>(s->g, a2w, (float*)wreal, 3, s->tw_x, s->stream); });
+          s->launch("les_model_real", [&] {
+            launch_les_model_real<float>((float*)real, (const float*)wreal, padded, s->lesm, s->stream);
+          });
+        } else {
+          s->launch("les_tau_real", [&] { launch_les_tau_real<float>((float*)real, padded, 2 * s->les_alpha, s->stream); });
+        }
       } else {
         s->launch("x_c2r", [&] { launch_x_c2r<double>(s->g, a2, (double*)real, 5, s->tw_x, s->stream); });
-        s->launch("les_tau_real", [&] { launch_les_tau_real<double>((double*)real, padded, 2 * s->les_alpha, s->stream); });
+        if (w_les) {
+          s->launch("x_c2r", [&] { launch_x_c2r<double>(s->g, a2w, (double*)wreal, 3, s->tw_x, s->stream); });
+          s->launch("les_model_real", [&] {
+            launch_les_model_real<double>((double*)real, (const double*)wreal, padded, s->lesm, s->stream);
+          });
+        } else {
+          s->launch("les_tau_real", [&] { launch_les_tau_real<double>((double*)real, padded, 2 * s->les_alpha, s->stream); });
+        }
       }
       fetch_real((const char*)real + (size_t)(field - FFS_FIELD_T11) * padded * tb);
+      if (wreal) release(wreal, 3 * padded * tb);
       release(real, 5 * padded * tb);
     } else {  // REAL (a state field) or FOURIER_TO_REAL: the stored transform back in real space
       void* real = s->dalloc(padded * tb);

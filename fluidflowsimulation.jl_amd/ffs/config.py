@@ -147,8 +147,11 @@ class Params:
     time_duration: float = math.inf
     threaded: bool = True
     precision: str = "fp64"
-    les_model: str = None
-    smagorinsky_c: float = 0.17
+    les_model: str = None           # lesModel: Smagorinsky, Vreman, Silvis, productionViscosity,
+                                    # stableNonLinear or SandP (les_types.jl:364-404)
+    smagorinsky_c: float = None     # the model's constant (None: its default; SandP: the S model's)
+    les_cp: float = None            # silvisPConstant / SandP's pConstant (None: 5.0 / 0.09)
+    les_s_model: str = None         # sLesModel of SandP
     filter_width: float = 0.0
     density: bool = False
     pr: float = 1.0
@@ -192,8 +195,11 @@ class Params:
         p.start_time = self.start_time
         p.dt_stat, p.dt_spec, p.write_time = self.dt_stat, self.dt_spec, self.write_time
         p.precision = _lib.FFS_FP32 if self.precision == "fp32" else _lib.FFS_FP64
-        p.les_model = {None: _lib.FFS_LES_NONE, "Smagorinsky": _lib.FFS_LES_SMAGORINSKY}.get(self.les_model, -1)
-        p.smagorinsky_c = self.smagorinsky_c
+        p.les_model = _lib.FFS_LES_NONE if self.les_model is None else _lib.LES_MODELS.get(self.les_model, -1)
+        p.smagorinsky_c = math.nan if self.smagorinsky_c is None else self.smagorinsky_c
+        p.les_cp = math.nan if self.les_cp is None else self.les_cp
+        if self.les_s_model is not None:
+            p.les_s_model = _lib.LES_MODELS.get(self.les_s_model, -1)
         p.filter_width = self.filter_width
         p.density = int(bool(self.density))
         p.pr, p.density_gradient = self.pr, self.density_gradient
@@ -263,9 +269,22 @@ def parameters(d):
     p.nt = opt("nt", int, None)
     p.time_duration = opt("timeDuration", float, math.inf)
     p.precision = opt("precision", str, "fp64")
-    if "lesModel" in d:
+    if "lesModel" in d:  # les_types (les_types.jl:360-404): each model reads its own constant keys
         p.les_model = d["lesModel"]
-        p.smagorinsky_c = opt("smagorinskyConstant", parse_expr, 0.17)
+        smodel = d.get("sLesModel") if p.les_model == "SandP" else None
+        known.add("sLesModel")
+        if p.les_model == "SandP" and smodel is None:
+            raise KeyError("sLesModel")
+        key = {"Smagorinsky": "smagorinskyConstant", "Vreman": "vremanConstant", "Silvis": "silvisConstant",
+               "productionViscosity": "productionConstant", "stableNonLinear": "nlConstant"}
+        ck = key.get(smodel if p.les_model == "SandP" else p.les_model)
+        if ck is not None:
+            p.smagorinsky_c = opt(ck, parse_expr, None)
+        if p.les_model == "Silvis":
+            p.les_cp = opt("silvisPConstant", parse_expr, None)
+        elif p.les_model == "SandP":
+            p.les_cp = opt("pConstant", parse_expr, None)
+            p.les_s_model = smodel
         p.filter_width = opt("filterWidth", parse_expr, 0.0)
     known.add("lesModel")
     if "densityStratification" in d:

@@ -63,6 +63,8 @@ mutable struct ffs_params
     scalar_gradient::Float64
     scalar_direction::Int32
     equation::Int32
+    les_cp::Float64
+    les_s_model::Int32
     ffs_params() = new()
 end
 
@@ -101,9 +103,20 @@ function params_from_global(d::AbstractDict)
     haskey(d, :variableTimestepFlag) && (p.variable_dt = parse(Bool, d[:variableTimestepFlag]))
     haskey(d, :cfl) && (p.cfl = parse(Float64, d[:cfl]))
     haskey(d, :dt) && (p.dt = num(:dt))
-    if haskey(d, :lesModel) && d[:lesModel] == "Smagorinsky"
-        p.les_model = Int32(1)
-        haskey(d, :smagorinskyConstant) && (p.smagorinsky_c = num(:smagorinskyConstant))
+    if haskey(d, :lesModel)   # les_types (les_types.jl:360-404); dynamic models are refused by ffs_create
+        codes = Dict("Smagorinsky" => 1, "Vreman" => 2, "Silvis" => 3, "productionViscosity" => 4,
+                     "stableNonLinear" => 5, "fakeSmagorinsky" => 6, "SandP" => 7)
+        ckey = Dict("Smagorinsky" => :smagorinskyConstant, "Vreman" => :vremanConstant,
+                    "Silvis" => :silvisConstant, "productionViscosity" => :productionConstant,
+                    "stableNonLinear" => :nlConstant)
+        m = d[:lesModel]
+        p.les_model = Int32(get(codes, m, -1))
+        sm = m == "SandP" ? d[:sLesModel] : m
+        m == "SandP" && (p.les_s_model = Int32(get(codes, sm, -1)))
+        k = get(ckey, sm, nothing)
+        k !== nothing && haskey(d, k) && (p.smagorinsky_c = num(k))
+        m == "Silvis" && haskey(d, :silvisPConstant) && (p.les_cp = num(:silvisPConstant))
+        m == "SandP" && haskey(d, :pConstant) && (p.les_cp = num(:pConstant))
         haskey(d, :filterWidth) && (p.filter_width = num(:filterWidth))
     end
     if haskey(d, :densityStratification)

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FFS_ABI_VERSION 3
+#define FFS_ABI_VERSION 4
 
 /* error codes */
 #define FFS_OK 0
@@ -44,9 +44,18 @@ extern "C" {
 #define FFS_FP64 0
 #define FFS_FP32 1 /* FP32 fields/FFTs; FP64 coefficients and reductions */
 
-/* LES models (`lesModel` key, LESmodels/les_types.jl:360) */
+/* LES models (`lesModel` key, LESmodels/les_types.jl:360-404).  The pointwise
+ * models of calc_les! (les_real_calc.jl:74-114); the dynamic ones
+ * (dynamicSmagorinsky, piomelliSmagorinsky, dynamicSandP, piomelliSandP) and a
+ * standalone fakeSmagorinsky are not built and are refused by ffs_create. */
 #define FFS_LES_NONE 0
 #define FFS_LES_SMAGORINSKY 1
+#define FFS_LES_VREMAN 2           /* VremanLESModel.jl */
+#define FFS_LES_SILVIS 3           /* silvismodel.jl */
+#define FFS_LES_PRODUCTION 4       /* productionViscosity, ProductionViscosityLESModel.jl */
+#define FFS_LES_STABLE_NL 5        /* stableNonLinear, stablenonlinearmodel.jl */
+#define FFS_LES_FAKE_SMAGORINSKY 6 /* fakeSmagorinsky: only as the S model of SandP */
+#define FFS_LES_SANDP 7            /* SandP: an S model (les_s_model) + cb Delta^2 Lie(S, W) */
 
 /* forcing (`forcing` key); FFS_FORCING_SHELL is the constant-power isotropic
  * low-wavenumber forcing f = (P / sum_{0<|k|<=kf} |u_k|^2) u_k, which has no
@@ -126,8 +135,13 @@ typedef struct ffs_params {
   int32_t dt_stat, dt_spec, write_time;     /* dtStat, dtSpec, writeTime */
   int32_t precision;                        /* FFS_FP64 / FFS_FP32 */
   int32_t les_model;                        /* FFS_LES_* */
-  double smagorinsky_c;                     /* smagorinskyConstant (0.17) */
-  double filter_width;                      /* filterWidth; <= 0 selects (2 pi L/nx)/cutoff */
+  double smagorinsky_c;                     /* the model's constant: smagorinskyConstant (0.17),
+                                               vremanConstant (0.032), silvisConstant (sqrt 0.17),
+                                               productionConstant (0.4; 0.1 under SandP) or
+                                               nlConstant (0.1); SandP: its S model's.  NaN (the
+                                               ffs_params_default value) selects that default */
+  double filter_width;                      /* filterWidth; <= 0 selects (2 pi L/nx)/cutoff
+                                               (twice that for SandP's Smagorinsky) */
   int32_t density;                          /* densityStratification present */
   double pr, density_gradient, z_acceleration, reference_density; /* Pr, densityGradient, zAcceleration, referenceDensity */
   int32_t forcing;                          /* FFS_FORCING_* */
@@ -147,6 +161,10 @@ typedef struct ffs_params {
   double scalar_gradient;                   /* scalarGradient: the mean gradient's source -G u_dir */
   int32_t scalar_direction;                 /* scalarDirection: 0 x, 1 y, 2 z (default z) */
   int32_t equation;                         /* FFS_EQ_*: equation (simulationtypes.jl:265-273) */
+  double les_cp;                            /* silvisPConstant (5.0) or SandP's pConstant cb (0.09);
+                                               NaN selects the default */
+  int32_t les_s_model;                      /* SandP's sLesModel: SMAGORINSKY, VREMAN, PRODUCTION or
+                                               FAKE_SMAGORINSKY */
 } ffs_params;
 
 typedef struct ffs_sim ffs_sim;

@@ -31,6 +31,8 @@ import os
 import numpy as np
 import scipy.fft as sfft
 
+from . import les_models
+
 __all__ = ["Grid", "OracleSim", "taylor_green", "workers"]
 
 
@@ -153,10 +155,13 @@ class OracleSim:
 
     def __init__(self, grid, nu, dt=0.0, variable_dt=False, cfl=0.18, dt_stat=1 << 30, dt_spec=1 << 30,
                  forcing_kf=None, forcing_power=0.1, start=0, start_time=0.0, integrator="AB3", les_c=None,
-                 les_delta=None, density=None, plane_forcing=None, hv=None, scalar=None, equation="rotational"):
+                 les_delta=None, density=None, plane_forcing=None, hv=None, scalar=None, equation="rotational",
+                 les_model="Smagorinsky", les_cp=None, les_smodel=None):
         """integrator: "AB3" (Adams_Bashforth3rdO) or "ETD3" (ETD3rdO);
-        les_c: Smagorinsky constant (None: no LES), les_delta: filter width
-        (default (2 pi L/nx)/cutoff, les_types.jl:366);
+        les_c: the LES model's constant (None: no LES), les_delta: filter width
+        (default (2 pi L/nx)/cutoff, les_types.jl:366); les_model: Smagorinsky or one of
+        oracle.les_models.MODELS, with les_cp (Silvis P constant, SandP pConstant) and les_smodel
+        (SandP's sLesModel);
         density: dict(Pr=, gradient=, g=) for the Boussinesq approximation;
         hv: dict(kind="hyper", nu=, m=) (HyperViscosity{nu_h, M}) or dict(kind="barrier",
         initk=, endk=) (SpectralBarrier), Hyperviscosity/hyperviscosity_types.jl;
@@ -167,9 +172,12 @@ class OracleSim:
         self.nu = nu
         self.integrator = integrator
         self.les_c = les_c
+        self.les_model, self.les_cp, self.les_smodel = les_model, les_cp, les_smodel
         if les_c is not None:
-            delta = les_delta if les_delta else (grid.LX * 2 * np.pi / grid.NRX) / grid.cutoff
+            delta = les_delta if les_delta else les_models.default_delta(les_model, les_smodel, grid.LX, grid.NRX,
+                                                                          grid.cutoff)
             self.les_alpha = les_c * les_c * delta * delta
+            self.les_d2 = delta * delta
         self.density = density
         if density is not None:
             self.alpha = nu / density["Pr"]
@@ -263,7 +271,11 @@ class OracleSim:
             # |S| = sqrt(2 S:S) with Szz = -Sxx - Syy
             szz = -Sr[0] - Sr[3]
             ss = Sr[0] ** 2 + Sr[3] ** 2 + szz**2 + 2 * (Sr[1] ** 2 + Sr[2] ** 2 + Sr[4] ** 2)
-            taur = 2 * (self.les_alpha * np.sqrt(2 * ss)) * Sr
+            if self.les_model == "Smagorinsky":
+                taur = 2 * (self.les_alpha * np.sqrt(2 * ss)) * Sr
+            else:  # the other pointwise models (les_real_calc.jl:80-114), from S and the vorticity
+                taur = les_models.stress(self.les_model, Sr, wr, self.les_c, self.les_d2, self.les_cp,
+                                         self.les_smodel)
             self.taur = taur  # the real-space stress writeoutput writes (T11 ... T23, output.jl:28-35)
         if self.equation == "NS":
             # traceless(symouter(u, -u)): xx xy xz yy yz, then div!(rhs, uu) = (i k) . uu

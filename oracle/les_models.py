@@ -1,0 +1,148 @@
+"""Test infrastructure: CPU restatement of the pointwise LES stresses of calc_les!
+(LESmodels/les_real_calc.jl:19-139) for the eddy-viscosity models other than
+Smagorinsky.  Only tests/, smoke() and bench.py's cpu_baseline leg use it, as the
+checker.
+
+Inputs are real-space fields: S = symouter(ik, u) in its five stored components
+(xx xy xz yy yz, zz = -xx - yy: the SymTrTen layout) and the vorticity w = ik x u
+(3 components).  The result t is stored into the same five components.
+
+The tensor algebra comes from the FluidTensors package, which is not vendored
+with the reference.  The conventions used here, each a standard one:
+  * AntiSymTen(v) is the cross-product matrix [v]x, AntiSymTen(v) . x = v x x.
+    Then AntiSymTen(-w/2) = -(grad u - grad u^T)/2 and S + AntiSymTen(-w/2) is
+    the transposed velocity gradient (grad u)^T.
+  * Lie(a, b) = a . b - b . a;  square(a) = a . a;  a:b = sum_ij a_ij b_ij;
+    traceless(a) = a - tr(a)/3 I;  a' is the transpose;  norm of a vector is its
+    Euclidean length.
+With them the stable non-linear model's tensor W^2 - Lie(S, W) - S^2 equals
+-(grad u)(grad u)^T (Clark's gradient model), which tests/test_les_models.py
+checks from the velocity gradient itself.  These conventions are parity
+unpinned: no reference test or fixture fixes them.
+"""
+import numpy as np
+
+MODELS = ("Smagorinsky", "Vreman", "Silvis", "productionViscosity", "stableNonLinear", "SandP")
+S_MODELS = ("Smagorinsky", "Vreman", "productionViscosity", "fakeSmagorinsky")
+
+
+def sym_full(s5):
+    """The 3x3 tensor (as [i][j] arrays) of the five stored components, zz = -xx - yy."""
+    xx, xy, xz, yy, yz = s5
+    zz = -xx - yy
+    return [[xx, xy, xz], [xy, yy, yz], [xz, yz, zz]]
+
+
+def cross_matrix(v):
+    """AntiSymTen(v): the matrix M with M . x = v x x, built from the cross products of the basis."""
+    m = [[None] * 3 for _ in range(3)]
+    for j in range(3):
+        e = [np.zeros_like(v[0]) for _ in range(3)]
+        e[j] = np.ones_like(v[0])
+        col = (v[1] * e[2] - v[2] * e[1], v[2] * e[0] - v[0] * e[2], v[0] * e[1] - v[1] * e[0])
+        for i in range(3):
+            m[i][j] = col[i]
+    return m
+
+
+def mat(a, b):
+    return [[sum(a[i][k] * b[k][j] for k in range(3)) for j in range(3)] for i in range(3)]
+
+
+def add(a, b, fb=1.0):
+    return [[a[i][j] + fb * b[i][j] for j in range(3)] for i in range(3)]
+
+
+def ddot(a, b):
+    return sum(a[i][j] * b[i][j] for i in range(3) for j in range(3))
+
+
+def five(m):
+    return np.array([m[0][0], m[0][1], m[0][2], m[1][1], m[1][2]])
+
+
+def scale(a, f):
+    return [[a[i][j] * f for j in range(3)] for i in range(3)]
+
+
+def traceless(a):
+    t = (a[0][0] + a[1][1] + a[2][2]) / 3
+    return [[a[i][j] - (t if i == j else 0) for j in range(3)] for i in range(3)]
+
+
+def vreman_nu(S, w, c, d2):
+    """Vreman_eddy_viscosity (VremanLESModel.jl:1-7): alpha = S + AntiSymTen(-w/2), beta = Delta^2
+    sym(alpha' alpha), B = b11 (b22 + b33) + b22 b33 - (b12^2 + b13^2 + b23^2) clipped at 0,
+    nu = c sqrt(B / alpha:alpha)."""
+    a = add(S, cross_matrix([-0.5 * x for x in w]))
+    at = [[a[j][i] for j in range(3)] for i in range(3)]
+    b = scale(mat(at, a), d2)
+    B = b[0][0] * (b[1][1] + b[2][2]) + b[1][1] * b[2][2] - (b[0][1] ** 2 + b[0][2] ** 2 + b[1][2] ** 2)
+    B = np.where(B < 0.0, 0.0, B)
+    return c * np.sqrt(B / ddot(a, a))
+
+
+def production_x(S, W, c, d2):
+    """c Delta^2 (W^2:S - S^2:S) / (2 S:S) (ProductionViscosityLESModel.jl, stablenonlinearmodel.jl:21-27),
+    and S:S; the models clip it at 0 (0 where S:S = 0)."""
+    a = ddot(S, S)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        x = c * d2 * (ddot(mat(W, W), S) - ddot(mat(S, S), S)) / (2 * a)
+    return x, a
+
+
+def stress(model, s5, w3, c, d2, cp=None, smodel=None):
+    """calc_les! (les_real_calc.jl:74-114) for one model: returns the five stored components of t.
+    model in MODELS; c the model constant (smagorinsky/vreman/silvis/production/nl constant); cp the
+    Silvis P constant or SandP's pConstant; smodel SandP's sLesModel."""
+    S = sym_full(s5)
+    if model == "SandP":
+        t = None if smodel == "fakeSmagorinsky" else stress(smodel, s5, w3, c, d2)
+        P = add(mat(S, cross_matrix([0.5 * x for x in w3])), mat(cross_matrix([0.5 * x for x in w3]), S), -1.0)
+        bp = five(scale(P, cp * d2))  # beta P, beta = cb Delta^2
+        return bp if t is None else t + bp
+    if model == "Smagorinsky":
+        nu = c * c * d2 * np.sqrt(2 * ddot(S, S))  # alpha norm(S), norm(S) = sqrt(2 S:S)
+        return five(scale(S, 2 * nu))
+    if model == "Vreman":
+        return five(scale(S, 2 * vreman_nu(S, w3, c, d2)))
+    if model == "productionViscosity":
+        x, a = production_x(S, cross_matrix([-0.5 * v for v in w3]), c, d2)
+        nu = np.where(a == 0.0, 0.0, np.maximum(0.0, x))
+        return five(scale(S, 2 * nu))
+    if model == "Silvis":
+        Sw = [sum(S[i][j] * w3[j] for j in range(3)) for i in range(3)]
+        nsw = np.sqrt(Sw[0] ** 2 + Sw[1] ** 2 + Sw[2] ** 2)
+        nw = np.sqrt(w3[0] ** 2 + w3[1] ** 2 + w3[2] ** 2)
+        ss = ddot(S, S)
+        f = nsw / (np.sqrt(ss) * nw)  # fvs (silvismodel.jl:24)
+        nut = c * c * d2 * f * f * f * np.sqrt(ss) / 2
+        nup = cp * d2 * f * f * f * f / 4
+        half = cross_matrix([0.5 * x for x in w3])
+        P = add(mat(S, half), mat(half, S), -1.0)  # Lie(S, AntiSymTen(w/2))
+        return five(add(scale(S, 2 * nut), scale(P, nup)))
+    if model == "stableNonLinear":
+        W = cross_matrix([-0.5 * x for x in w3])
+        lie = add(mat(S, W), mat(W, S), -1.0)
+        T = traceless(add(add(mat(W, W), lie, -1.0), mat(S, S), -1.0))
+        x, a = production_x(S, W, c, d2)
+        mnu = np.where(a == 0.0, 0.0, -np.minimum(0.0, x))
+        return five(add(scale(T, c * d2), scale(S, 2 * mnu)))
+    raise ValueError(f"LES model {model!r}")
+
+
+def default_constants(model, smodel=None):
+    """(c, cp) defaults of les_types (les_types.jl:364-404); cp is None where the model has none."""
+    base = {"Smagorinsky": 0.17, "Vreman": 0.032, "Silvis": np.sqrt(0.17), "productionViscosity": 0.4,
+            "stableNonLinear": 0.1, "fakeSmagorinsky": 0.0}
+    if model == "SandP":
+        # the S model's own constant; SandP's production default is 0.1 (les_types.jl:398)
+        c = 0.1 if smodel == "productionViscosity" else base[smodel]
+        return c, 0.09
+    return base[model], (5.0 if model == "Silvis" else None)
+
+
+def default_delta(model, smodel, lx, nx, cutoff):
+    """filterWidth default (les_types.jl:366-401): (2 pi lx / nx) / cutoff; SandP's Smagorinsky twice that."""
+    d = (lx * 2 * np.pi / nx) / cutoff
+    return 2 * d if (model == "SandP" and smodel == "Smagorinsky") else d

@@ -122,3 +122,26 @@ def test_plane_forcing_keys():
         # calculate_Zf: an unknown Zf is tanh for the shell form, the cutoff for aForcing's matrix form
         assert c.forcing_zf_tanh == (0 if name == "aForcing" else 1)
         assert config.parameters(dict(base, forcing=name, TF="1", alphac="1", kf="3")).to_c().forcing_zf_tanh == 0
+
+
+def test_les_model_keys():
+    """les_types (les_types.jl:360-404): each model reads its own constant key; unset constants stay
+    None (the library applies the model's default)."""
+    base = {"nx": "32", "ny": "32", "nz": "32", "xDomainSize": "1.0", "yDomainSize": "1.0", "zDomainSize": "1.0",
+            "kinematicViscosity": "1e-3", "dtStat": "10", "dtSpec": "100", "writeTime": "100"}
+    from ffs import _lib
+    v = config.parameters(dict(base, lesModel="Vreman", vremanConstant="0.05", smagorinskyConstant="0.3"))
+    assert (v.les_model, v.smagorinsky_c) == ("Vreman", 0.05) and v.to_c().les_model == _lib.FFS_LES_VREMAN
+    s = config.parameters(dict(base, lesModel="Silvis", silvisPConstant="4.0"))
+    assert s.smagorinsky_c is None and s.les_cp == 4.0
+    c = s.to_c()
+    assert math.isnan(c.smagorinsky_c) and c.les_cp == 4.0 and c.les_model == _lib.FFS_LES_SILVIS
+    q = config.parameters(dict(base, lesModel="SandP", sLesModel="productionViscosity", productionConstant="0.2",
+                               pConstant="0.05", filterWidth="0.1"))
+    assert (q.les_s_model, q.smagorinsky_c, q.les_cp, q.filter_width) == ("productionViscosity", 0.2, 0.05, 0.1)
+    assert q.to_c().les_s_model == _lib.FFS_LES_PRODUCTION and q.to_c().les_model == _lib.FFS_LES_SANDP
+    n = config.parameters(dict(base, lesModel="stableNonLinear", nlConstant="0.2"))
+    assert n.smagorinsky_c == 0.2
+    with pytest.raises(KeyError):
+        config.parameters(dict(base, lesModel="SandP"))  # sLesModel required (les_types.jl:391)
+    assert config.parameters(dict(base, lesModel="dynamicSmagorinsky")).to_c().les_model == -1  # refused at create
