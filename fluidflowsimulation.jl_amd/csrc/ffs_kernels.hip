@@ -122,6 +122,9 @@ template <typename T> __device__ __forceinline__ T nanmax(T a, T b) { return (a 
 template <typename T> __host__ __device__ constexpr bool lds_twiddles() {
   return sizeof(T) == 8 ? FFS_LDS_TW64 : FFS_LDS_TW32;
 }
+// LDS values a block reserves for the staged twiddle table (none when the table stays in global memory,
+// which leaves room for one more FP64 column block per CU)
+template <typename T> __host__ __device__ constexpr int tw_lds(int n) { return lds_twiddles<T>() ? n : 0; }
 template <typename T, int N>
 __device__ __forceinline__ const C<T>* stage_twiddles(C<T>* __restrict__ twl, const C<T>* __restrict__ tw) {
   if constexpr (!lds_twiddles<T>()) return tw;
@@ -314,7 +317,7 @@ static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, con
   constexpr int TX = col_tx<T>(N);
   constexpr int TP = col_tper<T, N, CURL>();
   static_assert(TX * TP <= kColThreads, "column block exceeds its launch bound");
-  const size_t lds = (size_t)(N * TX + N) * sizeof(C<T>);
+  const size_t lds = (size_t)(N * TX + tw_lds<T>(N)) * sizeof(C<T>);
   static bool attr = false;
   if (!attr) {
     hipFuncSetAttribute((const void*)&k_col<T, N, KIND, CURL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -714,7 +717,7 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
   static_assert(xreg_ok<T, N>(), "register-resident x pass needs an even plan");
   extern __shared__ unsigned char smem_raw[];
   C<T>* twl = reinterpret_cast<C<T>*>(smem_raw);
-  C<T>* lds = twl + N;
+  C<T>* lds = twl + tw_lds<T>(N);
   const C<T>* tw = stage_twiddles<T, N>(twl, tw_g);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -971,14 +974,14 @@ __global__ void __launch_bounds__(256) k_x_r2c(Geo g, const T* __restrict__ real
 
 template <typename T, int N>
 static size_t x_lds_bytes() {
-  return ((size_t)3 * xrows<N>() * (N + 1) + N) * sizeof(C<T>);
+  return ((size_t)3 * xrows<N>() * (N + 1) + tw_lds<T>(N)) * sizeof(C<T>);
 }
 
 template <typename T, int N, int NSL>
 static void run_x_gen(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
                       unsigned long long* rhomax, const void* tw, hipStream_t s) {
   constexpr int ROWS = xgen_rows<N, NSL>();
-  const size_t lds = ((size_t)ROWS * NSL * (N + 1) + N) * sizeof(C<T>);
+  const size_t lds = ((size_t)ROWS * NSL * (N + 1) + tw_lds<T>(N)) * sizeof(C<T>);
   static bool attr = false;
   if (!attr) {
     (void)hipFuncSetAttribute((const void*)&k_x_gen<T, N, NSL>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -996,7 +999,7 @@ static void launch_xreg(const Geo& g, const void* a2, void* b2, const void* tw, 
   constexpr int TP = N / xreg_ppt<T, N>();
   constexpr int WPB = 2;  // wavefronts per block
   constexpr int RPB = WPB * (64 / TP);
-  const size_t lds = ((size_t)RPB * xreg_ldx<T, N>() + N) * sizeof(C<T>);
+  const size_t lds = ((size_t)RPB * xreg_ldx<T, N>() + tw_lds<T>(N)) * sizeof(C<T>);
   static bool attr = false;
   if (!attr) {
     (void)hipFuncSetAttribute((const void*)&k_x_nl_reg<T, N, MODE, KE>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1577,7 +1580,7 @@ static void run_fwd_z_update(const Geo& g, const void* b1, void* u, void* rhs, c
                              const DevState* st, const UpdateArgs& a, double scale, const void* tw, hipStream_t s) {
   constexpr int TX = col_tx<T>(N);
   constexpr int TP = col_tper<T, N, false>();
-  const size_t lds = (size_t)(N * TX + N) * sizeof(C<T>);
+  const size_t lds = (size_t)(N * TX + tw_lds<T>(N)) * sizeof(C<T>);
   static bool attr = false;
   if (!attr) {
     (void)hipFuncSetAttribute((const void*)&k_fwd_z_update<T, N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
