@@ -548,6 +548,7 @@ def leg_features(args, n, nu):
         "ab3 aForcing": dict(forcing="aForcing", forcing_kf=3.0, forcing_tf=1.0, forcing_alpha=0.5),
         "ab3 smagorinsky": dict(les_model="Smagorinsky", smagorinsky_c=0.16),
         "ab3 vreman": dict(les_model="Vreman"),
+        "ab3 dynamic smagorinsky": dict(les_model="dynamicSmagorinsky"),
         "ab3 conservation form": dict(equation="NS"),
         "ab3 passive scalar": dict(passive_scalar=True, scalar_pr=1.0, scalar_gradient=1.0),
     }

@@ -127,7 +127,7 @@ void launch_inv_y(const Geo& g, const void* a1, void* a2, int nf, const void* tw
 // field set of the fused x pass
 // pointwise LES models of calc_les! (les_real_calc.jl:74-114); codes as FFS_LES_* of the C-ABI
 enum : int { LES_SMAGORINSKY = 1, LES_VREMAN = 2, LES_SILVIS = 3, LES_PRODUCTION = 4, LES_STABLE_NL = 5,
-             LES_FAKE_SMAGORINSKY = 6, LES_SANDP = 7 };
+             LES_FAKE_SMAGORINSKY = 6, LES_SANDP = 7, LES_DYNAMIC = 8 };
 struct LesArgs {
   int model;   // LES_*
   int smodel;  // SandP's S model
@@ -136,7 +136,7 @@ struct LesArgs {
   double d2;   // Delta^2
 };
 // whether the model reads the vorticity (every one but Smagorinsky)
-inline bool les_needs_w(const LesArgs& l) { return l.model != LES_SMAGORINSKY; }
+inline bool les_needs_w(const LesArgs& l) { return l.model != LES_SMAGORINSKY && l.model != LES_DYNAMIC; }
 
 struct XOps {
   int nin, nout;      // physical inputs (6 [+1 rho] [+1 phi] [+5 S]) and forward outputs (3 [+3] [+3] [+5] [+1 ke])
@@ -217,6 +217,28 @@ void launch_les_tau_real(T* s5, size_t n_padded, double c2a, hipStream_t s);
 // the same for a model that reads the vorticity: w3[3] padded real w fields
 template <typename T>
 void launch_les_model_real(T* s5, const T* w3, size_t n_padded, const LesArgs& l, hipStream_t s);
+
+// Dynamic Smagorinsky (LESmodels/dynamic_smagorinsky.jl).  Gaussian test filter G = exp(-td2 k^2 / 24)
+// (Gaussfilter, utils.jl:220-225).
+// fourierspacep1! (models.jl:98-108) on the compact modes: out = S (5), M = G S (5), uf = G u (3)
+template <typename T>
+void launch_dyn_prep(const Geo& g, const void* u, void* out, double td2, hipStream_t s);
+// realspace_dynamic_les_calculation! (dynamic_smagorinsky.jl:44-76) on padded real fields:
+// sa = Delta^2 |S| S (5), l = symouter(u, u) (6: xx xy xz yy yz zz)
+template <typename T>
+void launch_dyn_products(const T* s5, const T* u3, T* sa5, T* l6, size_t n_padded, double d2, hipStream_t s);
+// x G of a spectrum stored on the full mode set of g (fourierspace_dynamic_les_calculation!, average_c!)
+template <typename T>
+void launch_gauss(const Geo& g, void* spec, double td2, hipStream_t s);
+// coefficient_les_calculation! (dynamic_smagorinsky.jl:131-200, dynamicSmagorinsky branch): from the test-
+// filtered uf (3), M (5), L (6), sa (5): c = clamp(0.5 (L:M)/(M:M), -1, 1), L = traceless(uf uf - L),
+// M = td2e |M| M - sa
+template <typename T>
+void launch_dyn_coef(const T* uf3, const T* m5, const T* l6, const T* sa5, T* c, size_t n_padded, double td2e,
+                     hipStream_t s);
+// calc_les! dynamic branch (les_real_calc.jl:75-77): t = 2 max(cmin, min(0.4, c)) Delta^2 |S| S
+template <typename T>
+void launch_dyn_tau(const T* s5, const T* c, T* t5, size_t n_padded, double d2, double cmin, hipStream_t s);
 // conservation form: rhs = (i k) . uu (div!, vectorfunctions.jl:107-117 on the SymTrTen field)
 template <typename T>
 void launch_div_uu(const Geo& g, const void* uu, void* rhs, hipStream_t s);

@@ -1534,6 +1534,141 @@ void launch_les_model_real(T* s5, const T* w3, size_t n_padded, const LesArgs& l
   k_les_model_real<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(s5, w3, n_padded, l);
 }
 
+// ---- dynamic Smagorinsky ---------------------------------------------------------------------------
+__device__ __forceinline__ double gauss_filter(double td2, double k2) { return exp((-td2 / 24) * k2); }
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_dyn_prep(Geo g, const C<T>* __restrict__ u, C<T>* __restrict__ out,
+                                                  double td2) {
+  const size_t n = (size_t)g.sx * g.my * g.mz;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int cx = idx % g.sx;
+  if (cx >= g.mx) return;
+  const int cy = (idx / g.sx) % g.my;
+  const int cz = idx / ((size_t)g.sx * g.my);
+  const double kxd = g.kx[cx], kyd = g.ky[cy], kzd = g.kz[cz];
+  const T kx = (T)kxd, ky = (T)kyd, kz = (T)kzd;
+  const C<T> v[3] = {u[idx], u[n + idx], u[2 * n + idx]};
+  const T gf = (T)gauss_filter(td2, kxd * kxd + kyd * kyd + kzd * kzd);
+  // S_ab = (i k_a u_b + i k_b u_a) / 2 for ab = xx, xy, xz, yy, yz, as the curl pass forms it
+  const int A_[5] = {0, 0, 0, 1, 1}, B_[5] = {0, 1, 2, 1, 2};
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    const T ka = A_[e] == 0 ? kx : ky;
+    const T kb = B_[e] == 0 ? kx : (B_[e] == 1 ? ky : kz);
+    const C<T> ua = v[A_[e]], ub = v[B_[e]];
+    const C<T> sum = mk<T>((T)0.5 * (ka * ub.x + kb * ua.x), (T)0.5 * (ka * ub.y + kb * ua.y));
+    const C<T> S = mk<T>(-sum.y, sum.x);
+    out[e * n + idx] = S;
+    out[(5 + e) * n + idx] = mk<T>(gf * S.x, gf * S.y);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[(10 + c) * n + idx] = mk<T>(gf * v[c].x, gf * v[c].y);
+}
+
+template <typename T>
+void launch_dyn_prep(const Geo& g, const void* u, void* out, double td2, hipStream_t s) {
+  const size_t n = (size_t)g.sx * g.my * g.mz;
+  k_dyn_prep<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, (const C<T>*)u, (C<T>*)out, td2);
+}
+
+// |S| = sqrt(2 S:S) of a traceless symmetric tensor stored as xx xy xz yy yz
+template <typename T> __device__ __forceinline__ T norm_s5(T xx, T xy, T xz, T yy, T yz) {
+  const T zz = -xx - yy;
+  return sqrt((T)2 * (xx * xx + yy * yy + zz * zz + (T)2 * (xy * xy + xz * xz + yz * yz)));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_dyn_products(const T* __restrict__ s5, const T* __restrict__ u3,
+                                                      T* __restrict__ sa5, T* __restrict__ l6, size_t n, double d2) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T S[5] = {s5[i], s5[n + i], s5[2 * n + i], s5[3 * n + i], s5[4 * n + i]};
+  const T f = (T)d2 * norm_s5<T>(S[0], S[1], S[2], S[3], S[4]);
+#pragma unroll
+  for (int e = 0; e < 5; ++e) sa5[e * n + i] = f * S[e];
+  const T ux = u3[i], uy = u3[n + i], uz = u3[2 * n + i];
+  l6[i] = ux * ux;
+  l6[n + i] = ux * uy;
+  l6[2 * n + i] = ux * uz;
+  l6[3 * n + i] = uy * uy;
+  l6[4 * n + i] = uy * uz;
+  l6[5 * n + i] = uz * uz;
+}
+
+template <typename T>
+void launch_dyn_products(const T* s5, const T* u3, T* sa5, T* l6, size_t n_padded, double d2, hipStream_t s) {
+  k_dyn_products<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(s5, u3, sa5, l6, n_padded, d2);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_gauss(Geo g, C<T>* __restrict__ f, double td2) {
+  const size_t n = (size_t)g.sx * g.my * g.mz;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int cx = idx % g.sx;
+  if (cx >= g.mx) return;
+  const int cy = (idx / g.sx) % g.my;
+  const int cz = idx / ((size_t)g.sx * g.my);
+  const double kx = g.kx[cx], ky = g.ky[cy], kz = g.kz[cz];
+  const T gf = (T)gauss_filter(td2, kx * kx + ky * ky + kz * kz);
+  const C<T> v = f[idx];
+  f[idx] = mk<T>(gf * v.x, gf * v.y);
+}
+
+template <typename T>
+void launch_gauss(const Geo& g, void* spec, double td2, hipStream_t s) {
+  const size_t n = (size_t)g.sx * g.my * g.mz;
+  k_gauss<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, (C<T>*)spec, td2);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_dyn_coef(const T* __restrict__ uf3, const T* __restrict__ m5,
+                                                  const T* __restrict__ l6, const T* __restrict__ sa5,
+                                                  T* __restrict__ c, size_t n, double td2e) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T ux = uf3[i], uy = uf3[n + i], uz = uf3[2 * n + i];
+  // L = traceless(symouter(uf, uf) - L)
+  const T lxx = ux * ux - l6[i], lxy = ux * uy - l6[n + i], lxz = ux * uz - l6[2 * n + i];
+  const T lyy = uy * uy - l6[3 * n + i], lyz = uy * uz - l6[4 * n + i], lzz = uz * uz - l6[5 * n + i];
+  const T tr = (lxx + lyy + lzz) / (T)3;
+  const T Lxx = lxx - tr, Lyy = lyy - tr, Lzz = lzz - tr;
+  // M = td2e |Sh| Sh - sa (traceless symmetric, zz = -xx - yy)
+  const T h[5] = {m5[i], m5[n + i], m5[2 * n + i], m5[3 * n + i], m5[4 * n + i]};
+  const T f = (T)td2e * norm_s5<T>(h[0], h[1], h[2], h[3], h[4]);
+  T M[5];
+#pragma unroll
+  for (int e = 0; e < 5; ++e) M[e] = f * h[e] - sa5[e * n + i];
+  const T Mzz = -M[0] - M[3];
+  const T LM = Lxx * M[0] + Lyy * M[3] + Lzz * Mzz + (T)2 * (lxy * M[1] + lxz * M[2] + lyz * M[4]);
+  const T MM = M[0] * M[0] + M[3] * M[3] + Mzz * Mzz + (T)2 * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
+  c[i] = fmax((T)-1, fmin((T)1, (T)0.5 * (LM / MM)));
+}
+
+template <typename T>
+void launch_dyn_coef(const T* uf3, const T* m5, const T* l6, const T* sa5, T* c, size_t n_padded, double td2e,
+                     hipStream_t s) {
+  k_dyn_coef<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(uf3, m5, l6, sa5, c, n_padded, td2e);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_dyn_tau(const T* __restrict__ s5, const T* __restrict__ c,
+                                                 T* __restrict__ t5, size_t n, double d2, double cmin) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T S[5] = {s5[i], s5[n + i], s5[2 * n + i], s5[3 * n + i], s5[4 * n + i]};
+  const T nut = fmax((T)cmin, fmin((T)0.4, c[i])) * (T)d2 * norm_s5<T>(S[0], S[1], S[2], S[3], S[4]);
+#pragma unroll
+  for (int e = 0; e < 5; ++e) t5[e * n + i] = (T)2 * nut * S[e];
+}
+
+template <typename T>
+void launch_dyn_tau(const T* s5, const T* c, T* t5, size_t n_padded, double d2, double cmin, hipStream_t s) {
+  k_dyn_tau<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(s5, c, t5, n_padded, d2, cmin);
+}
+
 // div!(rhs, uu) of the conservation form (models.jl:176-180): rhs_c = (i k) . uu, uu symmetric
 // traceless (xx xy xz yy yz), as the LES (i k) . tau
 template <typename T>
@@ -2634,6 +2769,11 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   template void launch_div_uu<T>(const Geo&, const void*, void*, hipStream_t);                                     \
   template void launch_les_tau_real<T>(T*, size_t, double, hipStream_t);                                          \
   template void launch_les_model_real<T>(T*, const T*, size_t, const LesArgs&, hipStream_t);                      \
+  template void launch_dyn_prep<T>(const Geo&, const void*, void*, double, hipStream_t);                           \
+  template void launch_dyn_products<T>(const T*, const T*, T*, T*, size_t, double, hipStream_t);                  \
+  template void launch_gauss<T>(const Geo&, void*, double, hipStream_t);                                           \
+  template void launch_dyn_coef<T>(const T*, const T*, const T*, const T*, T*, size_t, double, hipStream_t);      \
+  template void launch_dyn_tau<T>(const T*, const T*, T*, size_t, double, double, hipStream_t);                   \
   template int launch_stats<T>(const Geo&, const void*, const void*, const void*, const StatsAux&, double*, double*,  \
                                hipStream_t);                                                                       \
   template void launch_spectrum<T>(const Geo&, const void*, double*, double*, double*, double*, double*, double*,     \

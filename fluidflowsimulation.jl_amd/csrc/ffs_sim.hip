@@ -119,6 +119,18 @@ struct ffs_sim {
   int nin = 6, nfw = 3;     // physical inputs / forward outputs of the nonlinear term (without 1/2|u|^2)
   double les_alpha = 0.0;   // (c Delta)^2
   LesArgs lesm{};           // the LES model (lesm.model = p.les_model)
+  // dynamicSmagorinsky: the stress is formed before the nonlinear pass (dynamic_les), through real
+  // fields and Gaussian test filters on the full (not dealiased) spectrum
+  bool dyn = false;
+  double dyn_td2 = 0.0, dyn_cmin = 0.0;
+  Geo gf{};                   // every mode of the grid (full spectrum layout)
+  void* dspec = nullptr;      // compact S (5), G S (5), G u (3)
+  void* dreal = nullptr;      // padded real fields, kDynReal of them
+  void* fspec = nullptr;      // one full spectrum
+  void* fw1 = nullptr;        // its pencil scratch
+  void* fw2 = nullptr;
+  // slots of dreal: u, then S, M = G S, uf = G u in dspec's order, then Delta^2 |S| S (later tau), L, c
+  enum { DR_U = 0, DR_S = 3, DR_M = 8, DR_UF = 13, DR_SA = 16, DR_L = 21, DR_C = 27, kDynReal = 28 };
   double alpha = 0.0, gz = 0.0, drhodz = 0.0;
   double palpha = 0.0;  // passive scalar diffusivity nu / scalarPr
   void* w1 = nullptr;
@@ -334,6 +346,21 @@ struct ffs_sim {
     g.kz = upload(kzc);
     scale = 1.0 / ((double)nrx * ny * nz);
     S = (size_t)g.sx * g.my * g.mz;
+    if (dyn) {  // every mode kept: identity maps, the full rfftfreq / fftfreq wavenumbers
+      gf = g;
+      gf.mx = nxh;
+      gf.my = gf.fy = ny;
+      gf.mz = gf.fz = nz;
+      gf.sx = (nxh + line - 1) / line * line;
+      std::vector<int> iy(ny), iz(nz);
+      for (int j = 0; j < ny; ++j) iy[j] = j;
+      for (int k = 0; k < nz; ++k) iz[k] = k;
+      gf.ymap = gf.yinv = upload(iy);
+      gf.zmap = gf.zinv = upload(iz);
+      gf.kx = upload(KX);
+      gf.ky = upload(KY);
+      gf.kz = upload(KZ);
+    }
   }
 
   template <typename T>
@@ -358,6 +385,15 @@ struct ffs_sim {
     if (les) {
       tau = dalloc(5 * S * csz);
       if (!dry) HIPCHECK(hipMemset(tau, 0, 5 * S * csz));
+    }
+    if (dyn) {
+      const size_t nf = (size_t)gf.sx * gf.ny * gf.nz;
+      dspec = dalloc(13 * S * csz);
+      dreal = dalloc(kDynReal * padded_len() * (csz / 2));
+      fspec = dalloc(nf * csz);
+      fw1 = dalloc(nf * csz);
+      fw2 = dalloc(nf * csz);
+      if (!dry) HIPCHECK(hipMemset(dspec, 0, 13 * S * csz));
     }
     if (dens) {
       flux = dalloc(3 * S * csz);
@@ -592,7 +628,7 @@ struct ffs_sim {
     p = in;
     if (p.size != sizeof(ffs_params)) throw FfsError(FFS_ERR_ARG, "ffs_params.size mismatch (ABI version)");
     if (p.integrator != FFS_AB3 && p.integrator != FFS_ETD3) throw FfsError(FFS_ERR_ARG, "unknown integrator");
-    if (p.les_model < FFS_LES_NONE || p.les_model > FFS_LES_SANDP)
+    if (p.les_model < FFS_LES_NONE || p.les_model > FFS_LES_DYNAMIC_SMAGORINSKY)
       throw FfsError(FFS_ERR_ARG, "unknown LES model");
     if (p.les_model == FFS_LES_FAKE_SMAGORINSKY)
       throw FfsError(FFS_ERR_ARG, "fakeSmagorinsky is built only as the S model of SandP");
@@ -648,6 +684,12 @@ struct ffs_sim {
       lesm.c = p.smagorinsky_c;
       lesm.cp = p.les_cp;
       lesm.d2 = delta * delta;
+      dyn = p.les_model == FFS_LES_DYNAMIC_SMAGORINSKY;
+      if (dyn) {  // DynamicSmagorinsky(Delta, TestFilterWidth = 2 Delta, dims, cmin) (les_types.jl:383-387)
+        const double td = p.les_test_width > 0 ? p.les_test_width : 2 * delta;
+        dyn_td2 = td * td;
+        dyn_cmin = p.les_cmin;
+      }
       // the models that read w take it from the rotational form's curl (need_vorticity, simulationtypes.jl:41)
       if (les_needs_w(lesm) && p.equation == FFS_EQ_NS)
         throw FfsError(FFS_ERR_ARG, "LES models other than Smagorinsky need equation = rotational here");
@@ -659,8 +701,10 @@ struct ffs_sim {
       gz = p.z_acceleration / p.reference_density;
       drhodz = p.density_gradient;
     }
-    nin = (nsform ? 3 : 6) + (dens ? 1 : 0) + (pscal ? 1 : 0) + (les ? 5 : 0);
-    nfw = (nsform ? 5 : 3) + (dens ? 3 : 0) + (pscal ? 3 : 0) + (les ? 5 : 0);
+    // the x pass carries S and tau for the pointwise models; the dynamic model forms tau beforehand
+    const bool xles = les && !dyn;
+    nin = (nsform ? 3 : 6) + (dens ? 1 : 0) + (pscal ? 1 : 0) + (xles ? 5 : 0);
+    nfw = (nsform ? 5 : 3) + (dens ? 3 : 0) + (pscal ? 3 : 0) + (xles ? 5 : 0);
     if (p.dt_stat <= 0 || p.dt_spec <= 0 || p.write_time <= 0)
       throw FfsError(FFS_ERR_ARG, "dtStat, dtSpec and writeTime must be positive");
     if (!p.variable_dt && !(p.dt > 0)) throw FfsError(FFS_ERR_ARG, "a fixed time step needs dt > 0");
@@ -750,6 +794,73 @@ struct ffs_sim {
     launch("fwd_z", [&] { FFS_DISPATCH(launch_fwd_z, g, w2, dst, nf, 0, scale, tw_z, stream); });
   }
 
+  size_t padded_len() const { return 2 * (size_t)(p.nx / 2 + 1) * p.ny * p.nz; }
+  char* rptr(void* base, size_t fields) { return (char*)base + fields * padded_len() * (csz / 2); }
+
+  // compact spectra -> padded real, in groups the pencil scratch holds
+  void inverse_many(void* spec, int nf, void* real) {
+    const int chunk = std::max(nin, nfw + 1);
+    for (int f0 = 0; f0 < nf; f0 += chunk) {
+      const int k = std::min(chunk, nf - f0);
+      inverse_to_real(cptr(spec, (size_t)f0 * S), k, rptr(real, f0));
+    }
+  }
+
+  // fullfourier! + x G + real! of one padded real field in place (dynamic_smagorinsky.jl:79-117,
+  // average_c! dynamic_piomelli.jl:13-26): the full spectrum, scaled by 1/N^3, is never dealiased
+  void filter_real(void* real) {
+    if (fp32) {
+      launch("dyn_filter", [&] { launch_x_r2c<float>(gf, (const float*)real, fw1, 1, tw_x, stream); });
+    } else {
+      launch("dyn_filter", [&] { launch_x_r2c<double>(gf, (const double*)real, fw1, 1, tw_x, stream); });
+    }
+    launch("dyn_filter", [&] { FFS_DISPATCH(launch_fwd_y, gf, fw1, fw2, 1, tw_y, stream); });
+    void* dst[kMaxFields] = {fspec};
+    launch("dyn_filter", [&] { FFS_DISPATCH(launch_fwd_z, gf, fw2, dst, 1, 0, scale, tw_z, stream); });
+    launch("dyn_filter", [&] { FFS_DISPATCH(launch_gauss, gf, fspec, dyn_td2, stream); });
+    launch("dyn_filter", [&] { FFS_DISPATCH(launch_inv_z, gf, fspec, fw1, 1, false, tw_z, stream); });
+    launch("dyn_filter", [&] { FFS_DISPATCH(launch_inv_y, gf, fw1, fw2, 1, tw_y, stream); });
+    if (fp32) launch("dyn_filter", [&] { launch_x_c2r<float>(gf, fw2, (float*)real, 1, tw_x, stream); });
+    else launch("dyn_filter", [&] { launch_x_c2r<double>(gf, fw2, (double*)real, 1, tw_x, stream); });
+  }
+
+  // the dynamic Smagorinsky stress of the current velocity (realspace_LES_calculation!, les_real_calc.jl:3-11)
+  // into dreal's slots DR_SA..DR_SA+4, and with store its dealiased transform into tau
+  void dynamic_les(bool store) {
+    launch("dyn_prep", [&] { FFS_DISPATCH(launch_dyn_prep, g, u, dspec, dyn_td2, stream); });
+    inverse_many(u, 3, rptr(dreal, DR_U));
+    inverse_many(dspec, 13, rptr(dreal, DR_S));  // S, M, uf
+    if (fp32) dyn_products<float>();
+    else dyn_products<double>();
+    for (int f = 0; f < 5; ++f) filter_real(rptr(dreal, DR_SA + f));
+    for (int f = 0; f < 6; ++f) filter_real(rptr(dreal, DR_L + f));
+    if (fp32) dyn_coef<float>();
+    else dyn_coef<double>();
+    filter_real(rptr(dreal, DR_C));  // average_c! (avg = true)
+    if (fp32) dyn_tau<float>();
+    else dyn_tau<double>();
+    if (store) forward_from_real(rptr(dreal, DR_SA), 5, tau);
+  }
+  template <typename T> T* rslot(int slot) { return (T*)rptr(dreal, slot); }
+  template <typename T> void dyn_products() {
+    launch("dyn_products", [&] {
+      launch_dyn_products<T>(rslot<T>(DR_S), rslot<T>(DR_U), rslot<T>(DR_SA), rslot<T>(DR_L), padded_len(), lesm.d2,
+                             stream);
+    });
+  }
+  template <typename T> void dyn_coef() {
+    // the filter widths add up in the coefficient (dynamic_smagorinsky.jl:155)
+    launch("dyn_coef", [&] {
+      launch_dyn_coef<T>(rslot<T>(DR_UF), rslot<T>(DR_M), rslot<T>(DR_L), rslot<T>(DR_SA), rslot<T>(DR_C),
+                         padded_len(), dyn_td2 + lesm.d2, stream);
+    });
+  }
+  template <typename T> void dyn_tau() {
+    launch("dyn_tau", [&] {
+      launch_dyn_tau<T>(rslot<T>(DR_S), rslot<T>(DR_C), rslot<T>(DR_SA), padded_len(), lesm.d2, dyn_cmin, stream);
+    });
+  }
+
   // ---------------------------------------------------------------- the step
   bool is_stats_time(int64_t it) const { return it % p.dt_stat == 0; }
   bool is_spec_time(int64_t it) const { return it % p.dt_spec == 0; }
@@ -761,6 +872,7 @@ struct ffs_sim {
   // fused = true (a plain step, fuse_update()): set_dt and the forcing sum run before the forward y pass,
   // and the forward z pass ends in the update, so the step is complete when this returns
   void nonlinear(bool need_ke, bool fused = false) {
+    if (dyn) dynamic_les(true);
     launch("inv_z_curl", [&] {
       FFS_DISPATCH(launch_inv_z, g, u, w1, nin, true, tw_z, stream, rho, phi, nsform ? 1 : 0);
     });
@@ -771,7 +883,7 @@ struct ffs_sim {
     ops.rho = dens ? 1 : 0;
     ops.phi = pscal ? 1 : 0;
     ops.ns = nsform ? 1 : 0;
-    ops.les = les ? 1 : 0;
+    ops.les = (les && !dyn) ? 1 : 0;
     ops.ke = need_ke ? 1 : 0;
     ops.les_2alpha = 2 * les_alpha;
     ops.lm = lesm;
@@ -800,7 +912,7 @@ struct ffs_sim {
       for (int c = 0; c < 3; ++c) dst[f++] = cptr(flux, c * S);
     if (pscal)
       for (int c = 0; c < 3; ++c) dst[f++] = cptr(pflux, c * S);
-    if (les)
+    if (les && !dyn)
       for (int c = 0; c < 5; ++c) dst[f++] = cptr(tau, c * S);
     if (need_ke) dst[f++] = ke;
     launch("fwd_z", [&] { FFS_DISPATCH(launch_fwd_z, g, w2, dst, nf, nsform ? 0 : 3, scale, tw_z, stream); });
@@ -1702,6 +1814,10 @@ int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n) {
             host[2 * dsti] = c[2 * d];
             host[2 * dsti + 1] = c[2 * d + 1];
           }
+    } else if (stress && space == FFS_SPACE_REAL && s->dyn) {
+      // the dynamic model's stress of the current velocity, as the step forms it
+      s->dynamic_les(false);
+      fetch_real(s->rptr(s->dreal, ffs_sim::DR_SA + (field - FFS_FIELD_T11)));
     } else if (stress && space == FFS_SPACE_REAL) {
       // realspace!: the curl pass forms S from the current velocity, the x pass's inverse transform
       // takes the five S fields to real space, then the pointwise product (les_real_calc.jl:77-79)

@@ -152,6 +152,8 @@ class Params:
     smagorinsky_c: float = None     # the model's constant (None: its default; SandP: the S model's)
     les_cp: float = None            # silvisPConstant / SandP's pConstant (None: 5.0 / 0.09)
     les_s_model: str = None         # sLesModel of SandP
+    les_test_width: float = 0.0     # TestFilterWidth of dynamicSmagorinsky (0: twice the filter width)
+    les_cmin: float = 0.0           # cmin of dynamicSmagorinsky
     filter_width: float = 0.0
     density: bool = False
     pr: float = 1.0
@@ -200,6 +202,7 @@ class Params:
         p.les_cp = math.nan if self.les_cp is None else self.les_cp
         if self.les_s_model is not None:
             p.les_s_model = _lib.LES_MODELS.get(self.les_s_model, -1)
+        p.les_test_width, p.les_cmin = self.les_test_width, self.les_cmin
         p.filter_width = self.filter_width
         p.density = int(bool(self.density))
         p.pr, p.density_gradient = self.pr, self.density_gradient
@@ -285,6 +288,9 @@ def parameters(d):
         elif p.les_model == "SandP":
             p.les_cp = opt("pConstant", parse_expr, None)
             p.les_s_model = smodel
+        elif p.les_model == "dynamicSmagorinsky":
+            p.les_test_width = opt("TestFilterWidth", parse_expr, 0.0)
+            p.les_cmin = opt("cmin", float, 0.0)
         p.filter_width = opt("filterWidth", parse_expr, 0.0)
     known.add("lesModel")
     if "densityStratification" in d:

@@ -156,12 +156,13 @@ class OracleSim:
     def __init__(self, grid, nu, dt=0.0, variable_dt=False, cfl=0.18, dt_stat=1 << 30, dt_spec=1 << 30,
                  forcing_kf=None, forcing_power=0.1, start=0, start_time=0.0, integrator="AB3", les_c=None,
                  les_delta=None, density=None, plane_forcing=None, hv=None, scalar=None, equation="rotational",
-                 les_model="Smagorinsky", les_cp=None, les_smodel=None):
+                 les_model="Smagorinsky", les_cp=None, les_smodel=None, les_test_width=None, les_cmin=0.0):
         """integrator: "AB3" (Adams_Bashforth3rdO) or "ETD3" (ETD3rdO);
         les_c: the LES model's constant (None: no LES), les_delta: filter width
         (default (2 pi L/nx)/cutoff, les_types.jl:366); les_model: Smagorinsky or one of
         oracle.les_models.MODELS, with les_cp (Silvis P constant, SandP pConstant) and les_smodel
-        (SandP's sLesModel);
+        (SandP's sLesModel); dynamicSmagorinsky takes les_test_width (default twice the filter
+        width) and les_cmin;
         density: dict(Pr=, gradient=, g=) for the Boussinesq approximation;
         hv: dict(kind="hyper", nu=, m=) (HyperViscosity{nu_h, M}) or dict(kind="barrier",
         initk=, endk=) (SpectralBarrier), Hyperviscosity/hyperviscosity_types.jl;
@@ -178,6 +179,8 @@ class OracleSim:
                                                                           grid.cutoff)
             self.les_alpha = les_c * les_c * delta * delta
             self.les_d2 = delta * delta
+            td = les_test_width if les_test_width else 2 * delta
+            self.les_td2, self.les_cmin = td * td, les_cmin
         self.density = density
         if density is not None:
             self.alpha = nu / density["Pr"]
@@ -273,6 +276,9 @@ class OracleSim:
             ss = Sr[0] ** 2 + Sr[3] ** 2 + szz**2 + 2 * (Sr[1] ** 2 + Sr[2] ** 2 + Sr[4] ** 2)
             if self.les_model == "Smagorinsky":
                 taur = 2 * (self.les_alpha * np.sqrt(2 * ss)) * Sr
+            elif self.les_model == "dynamicSmagorinsky":
+                taur, self.les_coef = les_models.dynamic_stress(g, u, S, Sr, ur, self.les_d2, self.les_td2,
+                                                                self.les_cmin)
             else:  # the other pointwise models (les_real_calc.jl:80-114), from S and the vorticity
                 taur = les_models.stress(self.les_model, Sr, wr, self.les_c, self.les_d2, self.les_cp,
                                          self.les_smodel)

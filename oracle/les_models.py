@@ -146,3 +146,41 @@ def default_delta(model, smodel, lx, nx, cutoff):
     """filterWidth default (les_types.jl:366-401): (2 pi lx / nx) / cutoff; SandP's Smagorinsky twice that."""
     d = (lx * 2 * np.pi / nx) / cutoff
     return 2 * d if (model == "SandP" and smodel == "Smagorinsky") else d
+
+
+def gauss(grid, td2):
+    """Gaussfilter(td2, k^2) = exp(-td2 k^2 / 24) on the half spectrum (utils.jl:220-225)."""
+    return np.exp((-td2 / 24) * grid.K2)
+
+
+def filter_real(grid, a, td2):
+    """fullfourier! (rfft, every mode scaled by 1/N^3, none dealiased), x G, real! (utils.jl:108-112,
+    dynamic_smagorinsky.jl:79-117, average_c! dynamic_piomelli.jl:13-26)."""
+    return grid.irfft(grid.rfft(a) * grid.scale * gauss(grid, td2))
+
+
+def dynamic_stress(grid, uh, S, Sr, ur, d2, td2, cmin):
+    """dynamicSmagorinsky with the averaged coefficient (avg = true): fourierspacep1! forms M = G S and
+    uf = G u on the loop range (models.jl:98-108); realspace_dynamic_les_calculation! sa = Delta^2 |S| S
+    and L = symouter(u, u) (dynamic_smagorinsky.jl:44-76), both test-filtered; the coefficient
+    c = clamp(0.5 (L:M)/(M:M), -1, 1) with L = traceless(uf uf - L), M = (td2 + Delta^2) |M| M - sa
+    (:131-200), averaged by one more filter; calc_les! t = 2 max(cmin, min(0.4, c)) Delta^2 |S| S
+    (les_real_calc.jl:75-77).  |X| = sqrt(2 X:X).  Returns the five stored components of t."""
+    G = gauss(grid, td2)
+    Mr = np.array([grid.irfft(np.where(grid.RANGE, G * c, 0)) for c in S])
+    ufr = np.array([grid.irfft(np.where(grid.RANGE, G * c, 0)) for c in uh])
+    Sf = sym_full(Sr)
+    nS = np.sqrt(2 * ddot(Sf, Sf))
+    sa = np.array([filter_real(grid, d2 * nS * x, td2) for x in Sr])
+    pairs = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
+    Lf = {ab: filter_real(grid, ur[ab[0]] * ur[ab[1]], td2) for ab in pairs}
+    L = [[ufr[i] * ufr[j] - Lf[(min(i, j), max(i, j))] for j in range(3)] for i in range(3)]
+    L = traceless(L)
+    Mh = sym_full(Mr)
+    nM = np.sqrt(2 * ddot(Mh, Mh))
+    M = add(scale(Mh, (td2 + d2) * nM), sym_full(sa), -1.0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        c = np.clip(0.5 * (ddot(L, M) / ddot(M, M)), -1.0, 1.0)
+    c = filter_real(grid, c, td2)
+    nut = np.maximum(cmin, np.minimum(0.4, c)) * d2 * nS
+    return np.array([2 * nut * x for x in Sr]), c

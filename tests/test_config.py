@@ -144,4 +144,6 @@ def test_les_model_keys():
     assert n.smagorinsky_c == 0.2
     with pytest.raises(KeyError):
         config.parameters(dict(base, lesModel="SandP"))  # sLesModel required (les_types.jl:391)
-    assert config.parameters(dict(base, lesModel="dynamicSmagorinsky")).to_c().les_model == -1  # refused at create
+    d = config.parameters(dict(base, lesModel="dynamicSmagorinsky", TestFilterWidth="0.3", cmin="-0.2"))
+    assert (d.les_test_width, d.les_cmin) == (0.3, -0.2) and d.to_c().les_model == _lib.FFS_LES_DYNAMIC_SMAGORINSKY
+    assert config.parameters(dict(base, lesModel="piomelliSmagorinsky")).to_c().les_model == -1  # refused at create

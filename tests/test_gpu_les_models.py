@@ -1,8 +1,9 @@
-"""GPU parity of the pointwise LES models other than Smagorinsky (Vreman, Silvis,
+"""GPU parity of the LES models other than Smagorinsky (Vreman, Silvis,
 productionViscosity, stableNonLinear, SandP with each S model; calc_les!,
-les_real_calc.jl:74-114) against the oracle, through the C ABI: steps of the
-rotational form with AB3 and ETD3, the LES stats columns, the real-space stress
-fields T11 ... T23, and the `global` keys each model reads.
+les_real_calc.jl:74-114; and dynamicSmagorinsky, dynamic_smagorinsky.jl) against
+the oracle, through the C ABI: steps of the rotational form with AB3 and ETD3, the
+LES stats columns, the real-space stress fields T11 ... T23, and the `global` keys
+each model reads.
 
 The oracle's tensor conventions (AntiSymTen, Lie) are pinned by the gradient
 model identity in tests/test_les_models.py; the package that defines them in the
@@ -142,9 +143,74 @@ def test_default_constants_from_the_library():
 
 
 def test_refused_combinations():
-    print_case("LES models refused: dynamic, standalone fakeSmagorinsky, w-models in the NS form")
+    print_case("LES models refused: Piomelli, standalone fakeSmagorinsky, w-models in the NS form")
     base = dict(nx=16, ny=16, nz=16, nu=1e-2, dt=1e-3, variable_dt=False)
-    for kw in (dict(les_model="dynamicSmagorinsky"), dict(les_model="fakeSmagorinsky"),
+    for kw in (dict(les_model="piomelliSmagorinsky"), dict(les_model="fakeSmagorinsky"),
                dict(les_model="Vreman", equation="NS"), dict(les_model="SandP", les_s_model="Silvis")):
         with pytest.raises(_lib.FfsError):
             ffs.Simulation(ffs.Params(**base, **kw))
+
+
+# ---- dynamicSmagorinsky ------------------------------------------------------------------------------
+def dyn_pair(shape, integrator="AB3", prec="fp64", variable_dt=True, seed=81, test_width=None, cmin=0.0):
+    nx, ny, nz = shape
+    g = Grid(nx, ny, nz)
+    u0 = random_spectrum(nx, ny, nz, seed=seed, k0=3.0)
+    ref = OracleSim(g, 2e-3, dt=1e-3, variable_dt=variable_dt, cfl=0.5, dt_stat=2, integrator=integrator,
+                    les_c=0.0, les_model="dynamicSmagorinsky", les_test_width=test_width, les_cmin=cmin)
+    ref.set_velocity(u0)
+    p = ffs.Params(nx=nx, ny=ny, nz=nz, nu=2e-3, dt=1e-3, variable_dt=variable_dt, cfl=0.5, dt_stat=2,
+                   dt_spec=1 << 30, write_time=1 << 30, precision=prec,
+                   integrator="Adams_Bashforth3rdO" if integrator == "AB3" else "ETD3rdO",
+                   les_model="dynamicSmagorinsky", les_test_width=test_width or 0.0, les_cmin=cmin)
+    s = ffs.Simulation(p)
+    s.set_velocity(*u0)
+    return g, ref, s
+
+
+DYN = [("AB3 variable dt 32^3", (32, 32, 32), dict()),
+       # nx >= ny: the spectra of iteration 0 bin ky into the kx-spaced shells (calculate_specHV)
+       ("ETD3 fixed dt 48x32x24", (48, 32, 24), dict(integrator="ETD3", variable_dt=False)),
+       ("AB3 test width 0.5, cmin -0.1, 48^3", (48, 48, 48), dict(test_width=0.5, cmin=-0.1))]
+
+
+@pytest.mark.parametrize("name,shape,kw", DYN, ids=[d[0] for d in DYN])
+def test_dynamic_smagorinsky_steps(name, shape, kw):
+    print_case(f"dynamicSmagorinsky {name}")
+    g, ref, s = dyn_pair(shape, **kw)
+    with s:
+        ref.initialize()
+        s.initialize()
+        for step in range(4):
+            ref.advance_in_time(1)
+            s.advance(1)
+            err = rel(s.velocity_hat(), ref.uh)
+            print(f"step {step} err {err:.2e}")
+            assert err < 1e-10, (step, err)
+        rows = s.pop_stats()
+    for (st, _, _), (ost, _, _) in zip(rows, ref.records):
+        assert np.allclose(st[-4:], ost[-4:], rtol=1e-9, atol=1e-12 * np.abs(ost[-4:]).max())
+
+
+def test_dynamic_smagorinsky_fp32_and_stress():
+    print_case("dynamicSmagorinsky fp32 steps; fp64 stress fields")
+    g, ref, s = dyn_pair((32, 32, 32), prec="fp32", variable_dt=False, seed=82)
+    with s:
+        ref.initialize()
+        s.initialize()
+        ref.advance_in_time(3)
+        s.advance(3)
+        assert rel(s.velocity_hat(), ref.uh) < 2e-4
+    g, ref, s = dyn_pair((32, 32, 32), variable_dt=False, seed=83)
+    with s:
+        ref.initialize()
+        s.initialize()
+        ref.advance_in_time(2)
+        s.advance(2)
+        ref.calculate_rhs(collect=False)
+        names = (_lib.FFS_FIELD_T11, _lib.FFS_FIELD_T12, _lib.FFS_FIELD_T13, _lib.FFS_FIELD_T22,
+                 _lib.FFS_FIELD_T23)
+        for k, f in enumerate(names):
+            got = s.get_field(f, _lib.FFS_SPACE_REAL)[:32]
+            want = ref.taur[k]
+            assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max(), (k, np.abs(got - want).max())

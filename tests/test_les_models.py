@@ -143,3 +143,62 @@ def test_defaults():
     d = (2 * np.pi / 32) / (2 / 3)
     assert lm.default_delta("SandP", "Smagorinsky", 1.0, 32, 2 / 3) == 2 * d
     assert lm.default_delta("Vreman", None, 1.0, 32, 2 / 3) == d
+
+
+# ---- dynamicSmagorinsky (dynamic_smagorinsky.jl) ------------------------------------------------------
+from oracle.ffs_oracle import Grid  # noqa: E402
+
+
+def _dyn_inputs(grid, seed, shift=(0.0, 0.0, 0.0), amp=1.0):
+    """A random dealiased solenoidal field (+ a constant velocity) with S, S real and u real."""
+    from ffs.initial import random_spectrum
+    u0 = random_spectrum(grid.NRX, seed=seed, k0=3.0)
+    uh = np.array([grid.myfourier(amp * c) for c in u0])
+    uh = grid.project(uh)
+    for c in range(3):
+        uh[c][0, 0, 0] = shift[c]
+    kv = (grid.kx, grid.ky, grid.kz)
+    pairs = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2))
+    S = np.array([np.where(grid.RANGE, 0.5 * (1j * kv[a] * uh[b] + 1j * kv[b] * uh[a]), 0) for a, b in pairs])
+    return uh, S, np.array([grid.irfft(c) for c in S]), np.array([grid.irfft(c) for c in uh])
+
+
+def test_test_filter_on_one_mode():
+    """The test filter multiplies cos(k.x) by exp(-td2 |k|^2 / 24) and keeps a constant."""
+    g = Grid(16, 16, 16)
+    x, y, z = g.coords()
+    td2 = 0.7
+    f = np.cos(3 * x + 2 * y - z) + 0.25
+    out = lm.filter_real(g, f, td2)
+    want = np.exp(-td2 * (9 + 4 + 1) / 24) * np.cos(3 * x + 2 * y - z) + 0.25
+    assert np.allclose(out, want, atol=1e-13)
+    # the Nyquist planes stay real and filtered: cos(8 x) -> exp(-td2 64/24) cos(8 x)
+    assert np.allclose(lm.filter_real(g, np.cos(8 * x), td2), np.exp(-td2 * 64 / 24) * np.cos(8 * x), atol=1e-13)
+
+
+def test_dynamic_coefficient_invariances():
+    """The dynamic procedure is scale invariant (c unchanged, t x lambda^2 when u -> lambda u) and
+    Galilean invariant (a constant velocity leaves L, M, c and t unchanged)."""
+    g = Grid(16, 16, 16)
+    d2, td2 = 0.09, 0.36
+    uh, S, Sr, ur = _dyn_inputs(g, 5)
+    t1, c1 = lm.dynamic_stress(g, uh, S, Sr, ur, d2, td2, 0.0)
+    uh2, S2, Sr2, ur2 = _dyn_inputs(g, 5, amp=3.0)
+    t2, c2 = lm.dynamic_stress(g, uh2, S2, Sr2, ur2, d2, td2, 0.0)
+    assert np.allclose(c2, c1, atol=1e-12) and np.allclose(t2, 9 * t1, rtol=1e-10, atol=1e-12 * np.abs(t2).max())
+    uh3, S3, Sr3, ur3 = _dyn_inputs(g, 5, shift=(0.7, -0.4, 1.1))
+    assert np.allclose(ur3, ur + np.array([0.7, -0.4, 1.1])[:, None, None, None], atol=1e-13)
+    t3, c3 = lm.dynamic_stress(g, uh3, S3, Sr3, ur3, d2, td2, 0.0)
+    assert np.allclose(c3, c1, atol=1e-11) and np.allclose(t3, t1, atol=1e-11 * np.abs(t1).max())
+    assert np.abs(c1).max() <= 1.0
+
+
+def test_dynamic_stress_limits():
+    """cmin floors the coefficient and 0.4 caps it: with cmin = 0.4 the model is Smagorinsky with
+    c^2 = 0.4 (t = 0.8 Delta^2 |S| S) whatever the coefficient."""
+    g = Grid(16, 16, 16)
+    d2, td2 = 0.09, 0.36
+    uh, S, Sr, ur = _dyn_inputs(g, 6)
+    t, _ = lm.dynamic_stress(g, uh, S, Sr, ur, d2, td2, 0.4)
+    smag = lm.stress("Smagorinsky", Sr, None, np.sqrt(0.4), d2)
+    assert np.allclose(t, smag, rtol=1e-12, atol=1e-14 * np.abs(smag).max())
