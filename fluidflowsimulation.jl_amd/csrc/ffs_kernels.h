@@ -226,7 +226,8 @@ void launch_dyn_prep(const Geo& g, const void* u, void* out, double td2, hipStre
 // realspace_dynamic_les_calculation! (dynamic_smagorinsky.jl:44-76) on padded real fields:
 // sa = Delta^2 |S| S (5), l = symouter(u, u) (6: xx xy xz yy yz zz)
 template <typename T>
-void launch_dyn_products(const T* s5, const T* u3, T* sa5, T* l6, size_t n_padded, double d2, hipStream_t s);
+void launch_dyn_products(const T* s5, const T* u3, T* sa5, T* l6, size_t n_padded, double d2, hipStream_t s,
+                         const T* cstar = nullptr);  // piomelliSmagorinsky: sa = 2 cstar Delta^2 |S| S
 // x G of a spectrum stored on the full mode set of g (fourierspace_dynamic_les_calculation!, average_c!)
 template <typename T>
 void launch_gauss(const Geo& g, void* spec, double td2, hipStream_t s);
@@ -235,7 +236,13 @@ void launch_gauss(const Geo& g, void* spec, double td2, hipStream_t s);
 // M = td2e |M| M - sa
 template <typename T>
 void launch_dyn_coef(const T* uf3, const T* m5, const T* l6, const T* sa5, T* c, size_t n_padded, double td2e,
-                     hipStream_t s);
+                     hipStream_t s, int piomelli = 0);  // piomelli: c = clamp((sa + L):M/(M:M)), M = 2 td2e |M| M
+// piomelliSmagorinsky's calc_cstar! (dynamic_piomelli.jl:1-11): c <- c + r (c - cm1), cm1 <- c, with
+// r = dt / dtm1 from the device dt (0 read as 1) and *dtm1 (0 read as dt); then *dtm1 <- dt
+template <typename T>
+void launch_piomelli_cstar(T* c, T* cm1, const DevState* st, double* dtm1, size_t n_padded, hipStream_t s);
+template <typename T>
+void launch_fill_real(T* a, size_t n, double v, hipStream_t s);
 // calc_les! dynamic branch (les_real_calc.jl:75-77): t = 2 max(cmin, min(0.4, c)) Delta^2 |S| S
 template <typename T>
 void launch_dyn_tau(const T* s5, const T* c, T* t5, size_t n_padded, double d2, double cmin, hipStream_t s);

@@ -1581,11 +1581,12 @@ template <typename T> __device__ __forceinline__ T norm_s5(T xx, T xy, T xz, T y
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_dyn_products(const T* __restrict__ s5, const T* __restrict__ u3,
-                                                      T* __restrict__ sa5, T* __restrict__ l6, size_t n, double d2) {
+                                                      T* __restrict__ sa5, T* __restrict__ l6, size_t n, double d2,
+                                                      const T* __restrict__ cstar) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const T S[5] = {s5[i], s5[n + i], s5[2 * n + i], s5[3 * n + i], s5[4 * n + i]};
-  const T f = (T)d2 * norm_s5<T>(S[0], S[1], S[2], S[3], S[4]);
+  const T f = (cstar ? (T)2 * cstar[i] * (T)d2 : (T)d2) * norm_s5<T>(S[0], S[1], S[2], S[3], S[4]);
 #pragma unroll
   for (int e = 0; e < 5; ++e) sa5[e * n + i] = f * S[e];
   const T ux = u3[i], uy = u3[n + i], uz = u3[2 * n + i];
@@ -1598,8 +1599,9 @@ __global__ void __launch_bounds__(256) k_dyn_products(const T* __restrict__ s5, 
 }
 
 template <typename T>
-void launch_dyn_products(const T* s5, const T* u3, T* sa5, T* l6, size_t n_padded, double d2, hipStream_t s) {
-  k_dyn_products<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(s5, u3, sa5, l6, n_padded, d2);
+void launch_dyn_products(const T* s5, const T* u3, T* sa5, T* l6, size_t n_padded, double d2, hipStream_t s,
+                         const T* cstar) {
+  k_dyn_products<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(s5, u3, sa5, l6, n_padded, d2, cstar);
 }
 
 template <typename T>
@@ -1626,7 +1628,7 @@ void launch_gauss(const Geo& g, void* spec, double td2, hipStream_t s) {
 template <typename T>
 __global__ void __launch_bounds__(256) k_dyn_coef(const T* __restrict__ uf3, const T* __restrict__ m5,
                                                   const T* __restrict__ l6, const T* __restrict__ sa5,
-                                                  T* __restrict__ c, size_t n, double td2e) {
+                                                  T* __restrict__ c, size_t n, double td2e, int piomelli) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const T ux = uf3[i], uy = uf3[n + i], uz = uf3[2 * n + i];
@@ -1635,8 +1637,26 @@ __global__ void __launch_bounds__(256) k_dyn_coef(const T* __restrict__ uf3, con
   const T lyy = uy * uy - l6[3 * n + i], lyz = uy * uz - l6[4 * n + i], lzz = uz * uz - l6[5 * n + i];
   const T tr = (lxx + lyy + lzz) / (T)3;
   const T Lxx = lxx - tr, Lyy = lyy - tr, Lzz = lzz - tr;
-  // M = td2e |Sh| Sh - sa (traceless symmetric, zz = -xx - yy)
   const T h[5] = {m5[i], m5[n + i], m5[2 * n + i], m5[3 * n + i], m5[4 * n + i]};
+  if (piomelli) {  // M = 2 td2e |Sh| Sh, Mn = M / (M:M), c = (sa + L):Mn (dynamic_smagorinsky.jl:168-182)
+    const T f = (T)2 * (T)td2e * norm_s5<T>(h[0], h[1], h[2], h[3], h[4]);
+    T M[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) M[e] = f * h[e];
+    const T Mzz = -M[0] - M[3];
+    const T MM = M[0] * M[0] + M[3] * M[3] + Mzz * Mzz + (T)2 * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
+    T Mn[6];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) Mn[e] = M[e] / MM;
+    Mn[5] = Mzz / MM;
+    const T a[5] = {sa5[i], sa5[n + i], sa5[2 * n + i], sa5[3 * n + i], sa5[4 * n + i]};
+    const T txx = a[0] + Lxx, tyy = a[3] + Lyy, tzz = (-a[0] - a[3]) + Lzz;
+    const T v = txx * Mn[0] + tyy * Mn[3] + tzz * Mn[5] +
+                (T)2 * ((a[1] + lxy) * Mn[1] + (a[2] + lxz) * Mn[2] + (a[4] + lyz) * Mn[4]);
+    c[i] = fmax((T)-1, fmin((T)1, v));
+    return;
+  }
+  // M = td2e |Sh| Sh - sa (traceless symmetric, zz = -xx - yy)
   const T f = (T)td2e * norm_s5<T>(h[0], h[1], h[2], h[3], h[4]);
   T M[5];
 #pragma unroll
@@ -1649,8 +1669,42 @@ __global__ void __launch_bounds__(256) k_dyn_coef(const T* __restrict__ uf3, con
 
 template <typename T>
 void launch_dyn_coef(const T* uf3, const T* m5, const T* l6, const T* sa5, T* c, size_t n_padded, double td2e,
-                     hipStream_t s) {
-  k_dyn_coef<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(uf3, m5, l6, sa5, c, n_padded, td2e);
+                     hipStream_t s, int piomelli) {
+  k_dyn_coef<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(uf3, m5, l6, sa5, c, n_padded, td2e, piomelli);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_piomelli_cstar(T* __restrict__ c, T* __restrict__ cm1,
+                                                        const DevState* __restrict__ st,
+                                                        const double* __restrict__ dtm1, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dt = st->dt, dtm = *dtm1;
+  const double dtf = dt == 0.0 ? 1.0 : dt;
+  const double dtm1f = dtm == 0.0 ? dtf : dtm;
+  const T r = (T)(dtf / dtm1f);
+  const T ci = c[i], pi = cm1[i];
+  cm1[i] = ci;
+  c[i] = fma(r, ci - pi, ci);
+}
+
+__global__ void k_store_dt(const DevState* __restrict__ st, double* __restrict__ dst) { *dst = st->dt; }
+
+template <typename T>
+void launch_piomelli_cstar(T* c, T* cm1, const DevState* st, double* dtm1, size_t n_padded, hipStream_t s) {
+  k_piomelli_cstar<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(c, cm1, st, dtm1, n_padded);
+  k_store_dt<<<1, 1, 0, s>>>(st, dtm1);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill_real(T* __restrict__ a, size_t n, T v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+template <typename T>
+void launch_fill_real(T* a, size_t n, double v, hipStream_t s) {
+  k_fill_real<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, n, (T)v);
 }
 
 template <typename T>
@@ -2770,9 +2824,11 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   template void launch_les_tau_real<T>(T*, size_t, double, hipStream_t);                                          \
   template void launch_les_model_real<T>(T*, const T*, size_t, const LesArgs&, hipStream_t);                      \
   template void launch_dyn_prep<T>(const Geo&, const void*, void*, double, hipStream_t);                           \
-  template void launch_dyn_products<T>(const T*, const T*, T*, T*, size_t, double, hipStream_t);                  \
+  template void launch_dyn_products<T>(const T*, const T*, T*, T*, size_t, double, hipStream_t, const T*);        \
   template void launch_gauss<T>(const Geo&, void*, double, hipStream_t);                                           \
-  template void launch_dyn_coef<T>(const T*, const T*, const T*, const T*, T*, size_t, double, hipStream_t);      \
+  template void launch_dyn_coef<T>(const T*, const T*, const T*, const T*, T*, size_t, double, hipStream_t, int); \
+  template void launch_piomelli_cstar<T>(T*, T*, const DevState*, double*, size_t, hipStream_t);                  \
+  template void launch_fill_real<T>(T*, size_t, double, hipStream_t);                                             \
   template void launch_dyn_tau<T>(const T*, const T*, T*, size_t, double, double, hipStream_t);                   \
   template int launch_stats<T>(const Geo&, const void*, const void*, const void*, const StatsAux&, double*, double*,  \
                                hipStream_t);                                                                       \

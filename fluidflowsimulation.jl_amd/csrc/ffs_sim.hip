@@ -80,6 +80,13 @@ struct Record {
 
 }  // namespace
 
+// for launchers whose pointers are the precision's real type
+#define FFS_DISPATCH_R(fn, ptr, ...)                       \
+  do {                                                     \
+    if (fp32) fn<float>((float*)(ptr), __VA_ARGS__);       \
+    else fn<double>((double*)(ptr), __VA_ARGS__);          \
+  } while (0)
+
 struct ffs_sim {
   ffs_params p{};
   bool fp32 = false;
@@ -122,6 +129,8 @@ struct ffs_sim {
   // dynamicSmagorinsky: the stress is formed before the nonlinear pass (dynamic_les), through real
   // fields and Gaussian test filters on the full (not dealiased) spectrum
   bool dyn = false;
+  bool pio = false;           // piomelliSmagorinsky: c and cm1 (DR_PC, DR_PCM1) and dtm1 persist across steps
+  double* pio_dtm1 = nullptr;
   double dyn_td2 = 0.0, dyn_cmin = 0.0;
   Geo gf{};                   // every mode of the grid (full spectrum layout)
   void* dspec = nullptr;      // compact S (5), G S (5), G u (3)
@@ -130,7 +139,9 @@ struct ffs_sim {
   void* fw1 = nullptr;        // its pencil scratch
   void* fw2 = nullptr;
   // slots of dreal: u, then S, M = G S, uf = G u in dspec's order, then Delta^2 |S| S (later tau), L, c
-  enum { DR_U = 0, DR_S = 3, DR_M = 8, DR_UF = 13, DR_SA = 16, DR_L = 21, DR_C = 27, kDynReal = 28 };
+  // (Piomelli: its state c, cm1 and their copies around a stress read-out)
+  enum { DR_U = 0, DR_S = 3, DR_M = 8, DR_UF = 13, DR_SA = 16, DR_L = 21, DR_C = 27, DR_PC = 28, DR_PCM1 = 29,
+         DR_SAVE = 30, kDynReal = 32 };
   double alpha = 0.0, gz = 0.0, drhodz = 0.0;
   double palpha = 0.0;  // passive scalar diffusivity nu / scalarPr
   void* w1 = nullptr;
@@ -394,6 +405,14 @@ struct ffs_sim {
       fw1 = dalloc(nf * csz);
       fw2 = dalloc(nf * csz);
       if (!dry) HIPCHECK(hipMemset(dspec, 0, 13 * S * csz));
+      if (pio) {  // c = cm1 = 0.0289, dtm1 = 0 (PiomelliSmagorinsky, les_types.jl:296-300)
+        pio_dtm1 = (double*)dalloc(sizeof(double));
+        if (!dry) {
+          HIPCHECK(hipMemset(pio_dtm1, 0, sizeof(double)));
+          FFS_DISPATCH_R(launch_fill_real, rptr(dreal, DR_PC), 2 * padded_len(), 0.0289, nullptr);
+          HIPCHECK(hipDeviceSynchronize());
+        }
+      }
     }
     if (dens) {
       flux = dalloc(3 * S * csz);
@@ -628,7 +647,7 @@ struct ffs_sim {
     p = in;
     if (p.size != sizeof(ffs_params)) throw FfsError(FFS_ERR_ARG, "ffs_params.size mismatch (ABI version)");
     if (p.integrator != FFS_AB3 && p.integrator != FFS_ETD3) throw FfsError(FFS_ERR_ARG, "unknown integrator");
-    if (p.les_model < FFS_LES_NONE || p.les_model > FFS_LES_DYNAMIC_SMAGORINSKY)
+    if (p.les_model < FFS_LES_NONE || p.les_model > FFS_LES_PIOMELLI_SMAGORINSKY)
       throw FfsError(FFS_ERR_ARG, "unknown LES model");
     if (p.les_model == FFS_LES_FAKE_SMAGORINSKY)
       throw FfsError(FFS_ERR_ARG, "fakeSmagorinsky is built only as the S model of SandP");
@@ -684,7 +703,8 @@ struct ffs_sim {
       lesm.c = p.smagorinsky_c;
       lesm.cp = p.les_cp;
       lesm.d2 = delta * delta;
-      dyn = p.les_model == FFS_LES_DYNAMIC_SMAGORINSKY;
+      pio = p.les_model == FFS_LES_PIOMELLI_SMAGORINSKY;
+      dyn = p.les_model == FFS_LES_DYNAMIC_SMAGORINSKY || pio;
       if (dyn) {  // DynamicSmagorinsky(Delta, TestFilterWidth = 2 Delta, dims, cmin) (les_types.jl:383-387)
         const double td = p.les_test_width > 0 ? p.les_test_width : 2 * delta;
         dyn_td2 = td * td;
@@ -827,6 +847,23 @@ struct ffs_sim {
   // the dynamic Smagorinsky stress of the current velocity (realspace_LES_calculation!, les_real_calc.jl:3-11)
   // into dreal's slots DR_SA..DR_SA+4, and with store its dealiased transform into tau
   void dynamic_les(bool store) {
+    const size_t rb = padded_len() * (csz / 2);
+    if (pio && !store) {  // a read-out leaves the coefficient state as the next step finds it
+      HIPCHECK(hipMemcpyAsync(rptr(dreal, DR_SAVE), rptr(dreal, DR_PC), 2 * rb, hipMemcpyDeviceToDevice, stream));
+      HIPCHECK(hipMemcpyAsync(&sd_save_dtm1, pio_dtm1, sizeof(double), hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+    }
+    if (pio) {  // realspace_dynamic_les_calculation!: cm1 = c at iteration 0, then calc_cstar!
+      if (iteration == 0)
+        HIPCHECK(hipMemcpyAsync(rptr(dreal, DR_PCM1), rptr(dreal, DR_PC), rb, hipMemcpyDeviceToDevice, stream));
+      launch("dyn_cstar", [&] {
+        if (fp32)
+          launch_piomelli_cstar<float>(rslot<float>(DR_PC), rslot<float>(DR_PCM1), st, pio_dtm1, padded_len(), stream);
+        else
+          launch_piomelli_cstar<double>(rslot<double>(DR_PC), rslot<double>(DR_PCM1), st, pio_dtm1, padded_len(),
+                                        stream);
+      });
+    }
     launch("dyn_prep", [&] { FFS_DISPATCH(launch_dyn_prep, g, u, dspec, dyn_td2, stream); });
     inverse_many(u, 3, rptr(dreal, DR_U));
     inverse_many(dspec, 13, rptr(dreal, DR_S));  // S, M, uf
@@ -836,28 +873,37 @@ struct ffs_sim {
     for (int f = 0; f < 6; ++f) filter_real(rptr(dreal, DR_L + f));
     if (fp32) dyn_coef<float>();
     else dyn_coef<double>();
-    filter_real(rptr(dreal, DR_C));  // average_c! (avg = true)
+    filter_real(rptr(dreal, coef_slot()));  // average_c! (avg = true)
     if (fp32) dyn_tau<float>();
     else dyn_tau<double>();
     if (store) forward_from_real(rptr(dreal, DR_SA), 5, tau);
+    if (pio && !store) {
+      HIPCHECK(hipMemcpyAsync(rptr(dreal, DR_PC), rptr(dreal, DR_SAVE), 2 * rb, hipMemcpyDeviceToDevice, stream));
+      HIPCHECK(hipMemcpyAsync(pio_dtm1, &sd_save_dtm1, sizeof(double), hipMemcpyHostToDevice, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+    }
   }
+  double sd_save_dtm1 = 0.0;
+  // the coefficient field: the step's own (dynamicSmagorinsky) or the persistent one (Piomelli)
+  int coef_slot() const { return pio ? DR_PC : DR_C; }
   template <typename T> T* rslot(int slot) { return (T*)rptr(dreal, slot); }
   template <typename T> void dyn_products() {
     launch("dyn_products", [&] {
       launch_dyn_products<T>(rslot<T>(DR_S), rslot<T>(DR_U), rslot<T>(DR_SA), rslot<T>(DR_L), padded_len(), lesm.d2,
-                             stream);
+                             stream, pio ? rslot<T>(DR_PC) : nullptr);
     });
   }
   template <typename T> void dyn_coef() {
     // the filter widths add up in the coefficient (dynamic_smagorinsky.jl:155)
     launch("dyn_coef", [&] {
-      launch_dyn_coef<T>(rslot<T>(DR_UF), rslot<T>(DR_M), rslot<T>(DR_L), rslot<T>(DR_SA), rslot<T>(DR_C),
-                         padded_len(), dyn_td2 + lesm.d2, stream);
+      launch_dyn_coef<T>(rslot<T>(DR_UF), rslot<T>(DR_M), rslot<T>(DR_L), rslot<T>(DR_SA), rslot<T>(coef_slot()),
+                         padded_len(), dyn_td2 + lesm.d2, stream, pio ? 1 : 0);
     });
   }
   template <typename T> void dyn_tau() {
     launch("dyn_tau", [&] {
-      launch_dyn_tau<T>(rslot<T>(DR_S), rslot<T>(DR_C), rslot<T>(DR_SA), padded_len(), lesm.d2, dyn_cmin, stream);
+      launch_dyn_tau<T>(rslot<T>(DR_S), rslot<T>(coef_slot()), rslot<T>(DR_SA), padded_len(), lesm.d2, dyn_cmin,
+                        stream);
     });
   }
 
@@ -1367,6 +1413,7 @@ struct ffs_sim {
     size_t n = sizeof(CkptHeader) + sizeof(DevState);
     for (void* b : ckpt_buffers()) n += ckpt_buffer_bytes(b);
     if (plane) n += 4 * pfd.nslot * sizeof(double);  // the forcing's R, Em, N, Nm1
+    if (pio) n += 2 * padded_len() * (csz / 2) + sizeof(double);  // Piomelli's c, cm1 and dtm1
     return n;
   }
 
@@ -1408,6 +1455,12 @@ struct ffs_sim {
         HIPCHECK(hipMemcpyAsync(out + off, b, pfd.nslot * sizeof(double), hipMemcpyDeviceToHost, stream));
         off += pfd.nslot * sizeof(double);
       }
+    if (pio) {
+      const size_t rb = 2 * padded_len() * (csz / 2);
+      HIPCHECK(hipMemcpyAsync(out + off, rptr(dreal, DR_PC), rb, hipMemcpyDeviceToHost, stream));
+      off += rb;
+      HIPCHECK(hipMemcpyAsync(out + off, pio_dtm1, sizeof(double), hipMemcpyDeviceToHost, stream));
+    }
     HIPCHECK(hipStreamSynchronize(stream));
   }
 
@@ -1436,6 +1489,12 @@ struct ffs_sim {
         HIPCHECK(hipMemcpy(b, in + off, pfd.nslot * sizeof(double), hipMemcpyHostToDevice));
         off += pfd.nslot * sizeof(double);
       }
+    if (pio) {
+      const size_t rb = 2 * padded_len() * (csz / 2);
+      HIPCHECK(hipMemcpy(rptr(dreal, DR_PC), in + off, rb, hipMemcpyHostToDevice));
+      off += rb;
+      HIPCHECK(hipMemcpy(pio_dtm1, in + off, sizeof(double), hipMemcpyHostToDevice));
+    }
     iteration = h.iteration;
     tit = h.tit;
     host_time = h.host_time;

@@ -152,8 +152,8 @@ class Params:
     smagorinsky_c: float = None     # the model's constant (None: its default; SandP: the S model's)
     les_cp: float = None            # silvisPConstant / SandP's pConstant (None: 5.0 / 0.09)
     les_s_model: str = None         # sLesModel of SandP
-    les_test_width: float = 0.0     # TestFilterWidth of dynamicSmagorinsky (0: twice the filter width)
-    les_cmin: float = 0.0           # cmin of dynamicSmagorinsky
+    les_test_width: float = 0.0     # TestFilterWidth of the dynamic models (0: twice the filter width)
+    les_cmin: float = 0.0           # cmin of the dynamic models
     filter_width: float = 0.0
     density: bool = False
     pr: float = 1.0
@@ -288,7 +288,7 @@ def parameters(d):
         elif p.les_model == "SandP":
             p.les_cp = opt("pConstant", parse_expr, None)
             p.les_s_model = smodel
-        elif p.les_model == "dynamicSmagorinsky":
+        elif p.les_model in ("dynamicSmagorinsky", "piomelliSmagorinsky"):
             p.les_test_width = opt("TestFilterWidth", parse_expr, 0.0)
             p.les_cmin = opt("cmin", float, 0.0)
         p.filter_width = opt("filterWidth", parse_expr, 0.0)

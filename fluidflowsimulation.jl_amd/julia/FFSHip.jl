@@ -107,7 +107,8 @@ function params_from_global(d::AbstractDict)
     haskey(d, :dt) && (p.dt = num(:dt))
     if haskey(d, :lesModel)   # les_types (les_types.jl:360-404); dynamic models are refused by ffs_create
         codes = Dict("Smagorinsky" => 1, "Vreman" => 2, "Silvis" => 3, "productionViscosity" => 4,
-                     "stableNonLinear" => 5, "fakeSmagorinsky" => 6, "SandP" => 7, "dynamicSmagorinsky" => 8)
+                     "stableNonLinear" => 5, "fakeSmagorinsky" => 6, "SandP" => 7, "dynamicSmagorinsky" => 8,
+                     "piomelliSmagorinsky" => 9)
         ckey = Dict("Smagorinsky" => :smagorinskyConstant, "Vreman" => :vremanConstant,
                     "Silvis" => :silvisConstant, "productionViscosity" => :productionConstant,
                     "stableNonLinear" => :nlConstant)
@@ -119,8 +120,9 @@ function params_from_global(d::AbstractDict)
         k !== nothing && haskey(d, k) && (p.smagorinsky_c = num(k))
         m == "Silvis" && haskey(d, :silvisPConstant) && (p.les_cp = num(:silvisPConstant))
         m == "SandP" && haskey(d, :pConstant) && (p.les_cp = num(:pConstant))
-        m == "dynamicSmagorinsky" && haskey(d, :TestFilterWidth) && (p.les_test_width = num(:TestFilterWidth))
-        m == "dynamicSmagorinsky" && haskey(d, :cmin) && (p.les_cmin = parse(Float64, d[:cmin]))
+        dynm = m == "dynamicSmagorinsky" || m == "piomelliSmagorinsky"
+        dynm && haskey(d, :TestFilterWidth) && (p.les_test_width = num(:TestFilterWidth))
+        dynm && haskey(d, :cmin) && (p.les_cmin = parse(Float64, d[:cmin]))
         haskey(d, :filterWidth) && (p.filter_width = num(:filterWidth))
     end
     if haskey(d, :densityStratification)

@@ -45,9 +45,10 @@ extern "C" {
 #define FFS_FP32 1 /* FP32 fields/FFTs; FP64 coefficients and reductions */
 
 /* LES models (`lesModel` key, LESmodels/les_types.jl:360-404).  The pointwise
- * models of calc_les! (les_real_calc.jl:74-114) and dynamicSmagorinsky
- * (dynamic_smagorinsky.jl); piomelliSmagorinsky, dynamicSandP, piomelliSandP
- * and a standalone fakeSmagorinsky are not built and are refused by ffs_create. */
+ * models of calc_les! (les_real_calc.jl:74-114), dynamicSmagorinsky and
+ * piomelliSmagorinsky (dynamic_smagorinsky.jl, dynamic_piomelli.jl);
+ * dynamicSandP, piomelliSandP and a standalone fakeSmagorinsky are not built and
+ * are refused by ffs_create. */
 #define FFS_LES_NONE 0
 #define FFS_LES_SMAGORINSKY 1
 #define FFS_LES_VREMAN 2           /* VremanLESModel.jl */
@@ -57,6 +58,7 @@ extern "C" {
 #define FFS_LES_FAKE_SMAGORINSKY 6 /* fakeSmagorinsky: only as the S model of SandP */
 #define FFS_LES_SANDP 7            /* SandP: an S model (les_s_model) + cb Delta^2 Lie(S, W) */
 #define FFS_LES_DYNAMIC_SMAGORINSKY 8 /* dynamicSmagorinsky: Gaussian test filter, averaged coefficient */
+#define FFS_LES_PIOMELLI_SMAGORINSKY 9 /* piomelliSmagorinsky: the coefficient extrapolated in time */
 
 /* forcing (`forcing` key); FFS_FORCING_SHELL is the constant-power isotropic
  * low-wavenumber forcing f = (P / sum_{0<|k|<=kf} |u_k|^2) u_k, which has no
@@ -166,9 +168,9 @@ typedef struct ffs_params {
                                                NaN selects the default */
   int32_t les_s_model;                      /* SandP's sLesModel: SMAGORINSKY, VREMAN, PRODUCTION or
                                                FAKE_SMAGORINSKY */
-  double les_test_width;                    /* TestFilterWidth of dynamicSmagorinsky; <= 0 selects
+  double les_test_width;                    /* TestFilterWidth of the dynamic models; <= 0 selects
                                                twice the filter width */
-  double les_cmin;                          /* cmin of dynamicSmagorinsky (0): the coefficient's floor */
+  double les_cmin;                          /* cmin of the dynamic models (0): the coefficient's floor */
 } ffs_params;
 
 typedef struct ffs_sim ffs_sim;

@@ -181,6 +181,9 @@ class OracleSim:
             self.les_d2 = delta * delta
             td = les_test_width if les_test_width else 2 * delta
             self.les_td2, self.les_cmin = td * td, les_cmin
+            if les_model == "piomelliSmagorinsky":  # c, cm1 start at 0.0289, dtm1 at 0 (les_types.jl:296-300)
+                shape = (grid.NRX, grid.NY, grid.NZ)
+                self.pio_c, self.pio_cm1, self.pio_dtm1 = np.full(shape, 0.0289), np.full(shape, 0.0289), 0.0
         self.density = density
         if density is not None:
             self.alpha = nu / density["Pr"]
@@ -276,6 +279,12 @@ class OracleSim:
             ss = Sr[0] ** 2 + Sr[3] ** 2 + szz**2 + 2 * (Sr[1] ** 2 + Sr[2] ** 2 + Sr[4] ** 2)
             if self.les_model == "Smagorinsky":
                 taur = 2 * (self.les_alpha * np.sqrt(2 * ss)) * Sr
+            elif self.les_model == "piomelliSmagorinsky":
+                cstar, self.pio_cm1 = les_models.piomelli_extrapolate(self.pio_c, self.pio_cm1, self.dt,
+                                                                      self.pio_dtm1, self.iteration)
+                self.pio_dtm1 = self.dt
+                taur, self.pio_c = les_models.dynamic_stress(g, u, S, Sr, ur, self.les_d2, self.les_td2,
+                                                             self.les_cmin, cstar=cstar)
             elif self.les_model == "dynamicSmagorinsky":
                 taur, self.les_coef = les_models.dynamic_stress(g, u, S, Sr, ur, self.les_d2, self.les_td2,
                                                                 self.les_cmin)

@@ -159,28 +159,50 @@ def filter_real(grid, a, td2):
     return grid.irfft(grid.rfft(a) * grid.scale * gauss(grid, td2))
 
 
-def dynamic_stress(grid, uh, S, Sr, ur, d2, td2, cmin):
+def piomelli_extrapolate(c, cm1, dt, dtm1, iteration):
+    """calc_cstar! as realspace_dynamic_les_calculation! calls it (dynamic_smagorinsky.jl:9-21,
+    dynamic_piomelli.jl:1-11): dt = 0 reads 1, a zero previous dt reads dt; at iteration 0 cm1 = c first.
+    Returns (cstar, the new cm1) = (muladd(r, c - cm1, c), c), r = dt / dtm1."""
+    dtf = 1.0 if dt == 0.0 else dt
+    dtm1f = dtf if dtm1 == 0.0 else dtm1
+    if iteration == 0:
+        cm1 = c.copy()
+    r = dtf / dtm1f
+    return r * (c - cm1) + c, c.copy()
+
+
+def dynamic_stress(grid, uh, S, Sr, ur, d2, td2, cmin, cstar=None):
     """dynamicSmagorinsky with the averaged coefficient (avg = true): fourierspacep1! forms M = G S and
     uf = G u on the loop range (models.jl:98-108); realspace_dynamic_les_calculation! sa = Delta^2 |S| S
     and L = symouter(u, u) (dynamic_smagorinsky.jl:44-76), both test-filtered; the coefficient
     c = clamp(0.5 (L:M)/(M:M), -1, 1) with L = traceless(uf uf - L), M = (td2 + Delta^2) |M| M - sa
     (:131-200), averaged by one more filter; calc_les! t = 2 max(cmin, min(0.4, c)) Delta^2 |S| S
-    (les_real_calc.jl:75-77).  |X| = sqrt(2 X:X).  Returns the five stored components of t."""
+    (les_real_calc.jl:75-77).  |X| = sqrt(2 X:X).  With cstar (piomelliSmagorinsky, the extrapolated
+    coefficient field) sa = 2 cstar Delta^2 |S| S and c = clamp((sa + L):M / (M:M), -1, 1) with
+    M = 2 (td2 + Delta^2) |M| M (:62-66, :168-182).  Returns (the five stored components of t, c)."""
     G = gauss(grid, td2)
     Mr = np.array([grid.irfft(np.where(grid.RANGE, G * c, 0)) for c in S])
     ufr = np.array([grid.irfft(np.where(grid.RANGE, G * c, 0)) for c in uh])
     Sf = sym_full(Sr)
     nS = np.sqrt(2 * ddot(Sf, Sf))
-    sa = np.array([filter_real(grid, d2 * nS * x, td2) for x in Sr])
+    if cstar is None:
+        sa = np.array([filter_real(grid, d2 * nS * x, td2) for x in Sr])
+    else:
+        sa = np.array([filter_real(grid, 2 * cstar * d2 * nS * x, td2) for x in Sr])
     pairs = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
     Lf = {ab: filter_real(grid, ur[ab[0]] * ur[ab[1]], td2) for ab in pairs}
     L = [[ufr[i] * ufr[j] - Lf[(min(i, j), max(i, j))] for j in range(3)] for i in range(3)]
     L = traceless(L)
     Mh = sym_full(Mr)
     nM = np.sqrt(2 * ddot(Mh, Mh))
-    M = add(scale(Mh, (td2 + d2) * nM), sym_full(sa), -1.0)
     with np.errstate(divide="ignore", invalid="ignore"):
-        c = np.clip(0.5 * (ddot(L, M) / ddot(M, M)), -1.0, 1.0)
+        if cstar is None:
+            M = add(scale(Mh, (td2 + d2) * nM), sym_full(sa), -1.0)
+            c = np.clip(0.5 * (ddot(L, M) / ddot(M, M)), -1.0, 1.0)
+        else:
+            M = scale(Mh, 2 * (td2 + d2) * nM)
+            Mn = scale(M, 1.0 / ddot(M, M))
+            c = np.clip(ddot(add(sym_full(sa), L), Mn), -1.0, 1.0)
     c = filter_real(grid, c, td2)
     nut = np.maximum(cmin, np.minimum(0.4, c)) * d2 * nS
     return np.array([2 * nut * x for x in Sr]), c

@@ -146,4 +146,6 @@ def test_les_model_keys():
         config.parameters(dict(base, lesModel="SandP"))  # sLesModel required (les_types.jl:391)
     d = config.parameters(dict(base, lesModel="dynamicSmagorinsky", TestFilterWidth="0.3", cmin="-0.2"))
     assert (d.les_test_width, d.les_cmin) == (0.3, -0.2) and d.to_c().les_model == _lib.FFS_LES_DYNAMIC_SMAGORINSKY
-    assert config.parameters(dict(base, lesModel="piomelliSmagorinsky")).to_c().les_model == -1  # refused at create
+    pm = config.parameters(dict(base, lesModel="piomelliSmagorinsky", cmin="0.01"))
+    assert pm.les_cmin == 0.01 and pm.to_c().les_model == _lib.FFS_LES_PIOMELLI_SMAGORINSKY
+    assert config.parameters(dict(base, lesModel="dynamicSandP", sLesModel="x")).to_c().les_model == -1  # refused

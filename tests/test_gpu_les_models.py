@@ -143,26 +143,27 @@ def test_default_constants_from_the_library():
 
 
 def test_refused_combinations():
-    print_case("LES models refused: Piomelli, standalone fakeSmagorinsky, w-models in the NS form")
+    print_case("LES models refused: piomelliSandP, standalone fakeSmagorinsky, w-models in the NS form")
     base = dict(nx=16, ny=16, nz=16, nu=1e-2, dt=1e-3, variable_dt=False)
-    for kw in (dict(les_model="piomelliSmagorinsky"), dict(les_model="fakeSmagorinsky"),
+    for kw in (dict(les_model="piomelliSandP"), dict(les_model="fakeSmagorinsky"),
                dict(les_model="Vreman", equation="NS"), dict(les_model="SandP", les_s_model="Silvis")):
         with pytest.raises(_lib.FfsError):
             ffs.Simulation(ffs.Params(**base, **kw))
 
 
 # ---- dynamicSmagorinsky ------------------------------------------------------------------------------
-def dyn_pair(shape, integrator="AB3", prec="fp64", variable_dt=True, seed=81, test_width=None, cmin=0.0):
+def dyn_pair(shape, integrator="AB3", prec="fp64", variable_dt=True, seed=81, test_width=None, cmin=0.0,
+             model="dynamicSmagorinsky"):
     nx, ny, nz = shape
     g = Grid(nx, ny, nz)
     u0 = random_spectrum(nx, ny, nz, seed=seed, k0=3.0)
     ref = OracleSim(g, 2e-3, dt=1e-3, variable_dt=variable_dt, cfl=0.5, dt_stat=2, integrator=integrator,
-                    les_c=0.0, les_model="dynamicSmagorinsky", les_test_width=test_width, les_cmin=cmin)
+                    les_c=0.0, les_model=model, les_test_width=test_width, les_cmin=cmin)
     ref.set_velocity(u0)
     p = ffs.Params(nx=nx, ny=ny, nz=nz, nu=2e-3, dt=1e-3, variable_dt=variable_dt, cfl=0.5, dt_stat=2,
                    dt_spec=1 << 30, write_time=1 << 30, precision=prec,
                    integrator="Adams_Bashforth3rdO" if integrator == "AB3" else "ETD3rdO",
-                   les_model="dynamicSmagorinsky", les_test_width=test_width or 0.0, les_cmin=cmin)
+                   les_model=model, les_test_width=test_width or 0.0, les_cmin=cmin)
     s = ffs.Simulation(p)
     s.set_velocity(*u0)
     return g, ref, s
@@ -214,3 +215,53 @@ def test_dynamic_smagorinsky_fp32_and_stress():
             got = s.get_field(f, _lib.FFS_SPACE_REAL)[:32]
             want = ref.taur[k]
             assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max(), (k, np.abs(got - want).max())
+
+
+PIO = [("AB3 variable dt 32^3", (32, 32, 32), dict()),
+       ("ETD3 fixed dt 48x32x24, cmin -0.05", (48, 32, 24), dict(integrator="ETD3", variable_dt=False, cmin=-0.05))]
+
+
+@pytest.mark.parametrize("name,shape,kw", PIO, ids=[d[0] for d in PIO])
+def test_piomelli_smagorinsky_steps(name, shape, kw):
+    """The extrapolated coefficient carries over the steps (dt ratio from the device dt), so parity
+    over several steps checks the state as well as the filter chain."""
+    print_case(f"piomelliSmagorinsky {name}")
+    g, ref, s = dyn_pair(shape, model="piomelliSmagorinsky", **kw)
+    with s:
+        ref.initialize()
+        s.initialize()
+        for step in range(5):
+            ref.advance_in_time(1)
+            s.advance(1)
+            err = rel(s.velocity_hat(), ref.uh)
+            print(f"step {step} err {err:.2e}")
+            assert err < 1e-10, (step, err)
+
+
+def test_piomelli_stress_readout_and_resume():
+    """Reading T11..T23 in real space does not move the coefficient state, and a checkpoint carries it:
+    a run that reads the stress and a run restored mid-way both stay bitwise with a plain run."""
+    print_case("piomelliSmagorinsky stress read-out and checkpoint resume")
+    _, ref, a = dyn_pair((32, 32, 32), model="piomelliSmagorinsky", variable_dt=True, seed=84)
+    _, _, b = dyn_pair((32, 32, 32), model="piomelliSmagorinsky", variable_dt=True, seed=84)
+    with a, b:
+        a.initialize()
+        b.initialize()
+        a.advance(3)
+        b.advance(3)
+        ref.initialize()
+        ref.advance_in_time(3)
+        ref.calculate_rhs(collect=False)
+        got = b.get_field(_lib.FFS_FIELD_T12, _lib.FFS_SPACE_REAL)[:32]
+        assert np.abs(got - ref.taur[1]).max() <= 1e-10 * np.abs(ref.taur[1]).max()
+        blob = b.checkpoint()
+        a.advance(3)
+        b.advance(3)
+        for x, y in zip(a.velocity_hat(), b.velocity_hat()):
+            assert np.array_equal(x, y)
+        _, _, c = dyn_pair((32, 32, 32), model="piomelliSmagorinsky", variable_dt=True, seed=84)
+        with c:
+            c.restore(blob)
+            c.advance(3)
+            for x, y in zip(a.velocity_hat(), c.velocity_hat()):
+                assert np.array_equal(x, y)

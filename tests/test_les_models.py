@@ -202,3 +202,34 @@ def test_dynamic_stress_limits():
     t, _ = lm.dynamic_stress(g, uh, S, Sr, ur, d2, td2, 0.4)
     smag = lm.stress("Smagorinsky", Sr, None, np.sqrt(0.4), d2)
     assert np.allclose(t, smag, rtol=1e-12, atol=1e-14 * np.abs(smag).max())
+
+
+def test_piomelli_extrapolation():
+    """calc_cstar!: c* = c + (dt/dt_prev)(c - c_prev), with dt = 0 read as 1, a zero previous dt read as
+    dt (so r = 1), and c_prev = c at iteration 0 (c* = c)."""
+    rng = np.random.default_rng(9)
+    c, cm1 = rng.standard_normal(50), rng.standard_normal(50)
+    cs, new = lm.piomelli_extrapolate(c, cm1, 0.02, 0.01, 5)
+    assert np.allclose(cs, 3 * c - 2 * cm1, atol=1e-15) and np.array_equal(new, c)
+    cs, _ = lm.piomelli_extrapolate(c, cm1, 0.02, 0.0, 5)
+    assert np.allclose(cs, 2 * c - cm1, atol=1e-15)
+    cs, _ = lm.piomelli_extrapolate(c, cm1, 0.0, 0.5, 5)
+    assert np.allclose(cs, c + 2 * (c - cm1), atol=1e-15)
+    cs, new = lm.piomelli_extrapolate(c, cm1, 0.02, 0.01, 0)
+    assert np.array_equal(cs, c) and np.array_equal(new, c)
+
+
+def test_piomelli_coefficient_invariances():
+    """With cstar the coefficient is (sa + L):M/(M:M), M = 2 (td2 + Delta^2)|M| M: under u -> lambda u it is
+    unchanged (sa and L scale as lambda^2, M as lambda^2), and it is Galilean invariant."""
+    g = Grid(16, 16, 16)
+    d2, td2 = 0.09, 0.36
+    cstar = 0.02 + 0.01 * np.cos(g.coords()[0])
+    uh, S, Sr, ur = _dyn_inputs(g, 7)
+    t1, c1 = lm.dynamic_stress(g, uh, S, Sr, ur, d2, td2, -1.0, cstar=cstar)
+    uh2, S2, Sr2, ur2 = _dyn_inputs(g, 7, amp=2.0)
+    t2, c2 = lm.dynamic_stress(g, uh2, S2, Sr2, ur2, d2, td2, -1.0, cstar=cstar)
+    assert np.allclose(c2, c1, atol=1e-12) and np.allclose(t2, 4 * t1, atol=1e-10 * np.abs(t2).max())
+    uh3, S3, Sr3, ur3 = _dyn_inputs(g, 7, shift=(0.3, 0.9, -0.5))
+    t3, c3 = lm.dynamic_stress(g, uh3, S3, Sr3, ur3, d2, td2, -1.0, cstar=cstar)
+    assert np.allclose(c3, c1, atol=1e-11)
