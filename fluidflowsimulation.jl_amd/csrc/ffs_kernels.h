@@ -222,12 +222,22 @@ void launch_les_model_real(T* s5, const T* w3, size_t n_padded, const LesArgs& l
 // (Gaussfilter, utils.jl:220-225).
 // fourierspacep1! (models.jl:98-108) on the compact modes: out = S (5), M = G S (5), uf = G u (3)
 template <typename T>
-void launch_dyn_prep(const Geo& g, const void* u, void* out, double td2, hipStream_t s);
+void launch_dyn_prep(const Geo& g, const void* u, void* out, double td2, hipStream_t s, int with_w = 0);
+// with_w (dynamicSandP, piomelliSandP): also w = ik x u (3) and G w (3) at out slots 13..18
 // realspace_dynamic_les_calculation! (dynamic_smagorinsky.jl:44-76) on padded real fields:
 // sa = Delta^2 |S| S (5), l = symouter(u, u) (6: xx xy xz yy yz zz)
+// The P part of dynamicSandP / piomelliSandP: w3 the vorticity, pa5 out P = [cpstar] Delta^2 Lie(S, AntiSymTen(w/2))
+struct DynP {
+  const void* w3 = nullptr;     // real vorticity (null: no P part)
+  const void* wf3 = nullptr;    // its test-filtered field
+  void* pa5 = nullptr;          // P (filtered between the products and the coefficient)
+  const void* cpstar = nullptr; // piomelliSandP's extrapolated coefficient (null: dynamicSandP)
+  void* cp = nullptr;           // the coefficient cp out
+};
 template <typename T>
 void launch_dyn_products(const T* s5, const T* u3, T* sa5, T* l6, size_t n_padded, double d2, hipStream_t s,
-                         const T* cstar = nullptr);  // piomelliSmagorinsky: sa = 2 cstar Delta^2 |S| S
+                         const T* cstar = nullptr,  // piomelliSmagorinsky: sa = 2 cstar Delta^2 |S| S
+                         const DynP& dp = DynP{});
 // x G of a spectrum stored on the full mode set of g (fourierspace_dynamic_les_calculation!, average_c!)
 template <typename T>
 void launch_gauss(const Geo& g, void* spec, double td2, hipStream_t s);
@@ -236,7 +246,8 @@ void launch_gauss(const Geo& g, void* spec, double td2, hipStream_t s);
 // M = td2e |M| M - sa
 template <typename T>
 void launch_dyn_coef(const T* uf3, const T* m5, const T* l6, const T* sa5, T* c, size_t n_padded, double td2e,
-                     hipStream_t s, int piomelli = 0);  // piomelli: c = clamp((sa + L):M/(M:M)), M = 2 td2e |M| M
+                     hipStream_t s, int piomelli = 0,  // piomelli: c = clamp((sa + L):M/(M:M)), M = 2 td2e |M| M
+                     const DynP& dp = DynP{});          // and cp of the SandP variants
 // piomelliSmagorinsky's calc_cstar! (dynamic_piomelli.jl:1-11): c <- c + r (c - cm1), cm1 <- c, with
 // r = dt / dtm1 from the device dt (0 read as 1) and *dtm1 (0 read as dt); then *dtm1 <- dt
 template <typename T>
@@ -245,7 +256,8 @@ template <typename T>
 void launch_fill_real(T* a, size_t n, double v, hipStream_t s);
 // calc_les! dynamic branch (les_real_calc.jl:75-77): t = 2 max(cmin, min(0.4, c)) Delta^2 |S| S
 template <typename T>
-void launch_dyn_tau(const T* s5, const T* c, T* t5, size_t n_padded, double d2, double cmin, hipStream_t s);
+void launch_dyn_tau(const T* s5, const T* c, T* t5, size_t n_padded, double d2, double cmin, hipStream_t s,
+                    const T* w3 = nullptr, const T* cp = nullptr);  // + clamp(cp, +-0.8) Delta^2 Lie(S, AntiSymTen(w/2))
 // conservation form: rhs = (i k) . uu (div!, vectorfunctions.jl:107-117 on the SymTrTen field)
 template <typename T>
 void launch_div_uu(const Geo& g, const void* uu, void* rhs, hipStream_t s);

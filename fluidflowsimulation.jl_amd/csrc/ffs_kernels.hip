@@ -1539,7 +1539,7 @@ __device__ __forceinline__ double gauss_filter(double td2, double k2) { return e
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_dyn_prep(Geo g, const C<T>* __restrict__ u, C<T>* __restrict__ out,
-                                                  double td2) {
+                                                  double td2, int with_w) {
   const size_t n = (size_t)g.sx * g.my * g.mz;
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n) return;
@@ -1565,12 +1565,42 @@ __global__ void __launch_bounds__(256) k_dyn_prep(Geo g, const C<T>* __restrict_
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) out[(10 + c) * n + idx] = mk<T>(gf * v[c].x, gf * v[c].y);
+  if (with_w) {  // w = i k x u as the curl pass forms it, then G w (models.jl:89-108)
+    const C<T> a[3] = {mk<T>(ky * v[2].x - kz * v[1].x, ky * v[2].y - kz * v[1].y),
+                       mk<T>(kz * v[0].x - kx * v[2].x, kz * v[0].y - kx * v[2].y),
+                       mk<T>(kx * v[1].x - ky * v[0].x, kx * v[1].y - ky * v[0].y)};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const C<T> w = mk<T>(-a[c].y, a[c].x);
+      out[(13 + c) * n + idx] = w;
+      out[(16 + c) * n + idx] = mk<T>(gf * w.x, gf * w.y);
+    }
+  }
 }
 
 template <typename T>
-void launch_dyn_prep(const Geo& g, const void* u, void* out, double td2, hipStream_t s) {
+void launch_dyn_prep(const Geo& g, const void* u, void* out, double td2, hipStream_t s, int with_w) {
   const size_t n = (size_t)g.sx * g.my * g.mz;
-  k_dyn_prep<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, (const C<T>*)u, (C<T>*)out, td2);
+  k_dyn_prep<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, (const C<T>*)u, (C<T>*)out, td2, with_w);
+}
+
+// Lie(S, AntiSymTen(w/2)) = A S - S A, A = AntiSymTen(-w/2), of S stored as xx xy xz yy yz (zz = -xx - yy);
+// the result is symmetric and traceless, returned in the same five components
+template <typename T> __device__ __forceinline__ void lie_s_w(const T s[5], T wx, T wy, T wz, T P[5]) {
+  const T S[3][3] = {{s[0], s[1], s[2]}, {s[1], s[3], s[4]}, {s[2], s[4], -s[0] - s[3]}};
+  const T hx = (T)0.5 * wx, hy = (T)0.5 * wy, hz = (T)0.5 * wz;
+  const T A[3][3] = {{0, hz, -hy}, {-hz, 0, hx}, {hy, -hx, 0}};
+  auto e = [&](int i, int j) {
+    T v = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v += A[i][k] * S[k][j] - S[i][k] * A[k][j];
+    return v;
+  };
+  P[0] = e(0, 0);
+  P[1] = e(0, 1);
+  P[2] = e(0, 2);
+  P[3] = e(1, 1);
+  P[4] = e(1, 2);
 }
 
 // |S| = sqrt(2 S:S) of a traceless symmetric tensor stored as xx xy xz yy yz
@@ -1582,7 +1612,8 @@ template <typename T> __device__ __forceinline__ T norm_s5(T xx, T xy, T xz, T y
 template <typename T>
 __global__ void __launch_bounds__(256) k_dyn_products(const T* __restrict__ s5, const T* __restrict__ u3,
                                                       T* __restrict__ sa5, T* __restrict__ l6, size_t n, double d2,
-                                                      const T* __restrict__ cstar) {
+                                                      const T* __restrict__ cstar, const T* __restrict__ w3,
+                                                      const T* __restrict__ cpstar, T* __restrict__ pa5) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const T S[5] = {s5[i], s5[n + i], s5[2 * n + i], s5[3 * n + i], s5[4 * n + i]};
@@ -1596,12 +1627,20 @@ __global__ void __launch_bounds__(256) k_dyn_products(const T* __restrict__ s5, 
   l6[3 * n + i] = uy * uy;
   l6[4 * n + i] = uy * uz;
   l6[5 * n + i] = uz * uz;
+  if (w3) {  // P = [cp] Delta^2 Lie(S, AntiSymTen(w/2)) (dynamic_smagorinsky.jl:71-75)
+    T P[5];
+    lie_s_w<T>(S, w3[i], w3[n + i], w3[2 * n + i], P);
+    const T fp = cpstar ? cpstar[i] * (T)d2 : (T)d2;
+#pragma unroll
+    for (int e = 0; e < 5; ++e) pa5[e * n + i] = fp * P[e];
+  }
 }
 
 template <typename T>
 void launch_dyn_products(const T* s5, const T* u3, T* sa5, T* l6, size_t n_padded, double d2, hipStream_t s,
-                         const T* cstar) {
-  k_dyn_products<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(s5, u3, sa5, l6, n_padded, d2, cstar);
+                         const T* cstar, const DynP& dp) {
+  k_dyn_products<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(
+      s5, u3, sa5, l6, n_padded, d2, cstar, (const T*)dp.w3, (const T*)dp.cpstar, (T*)dp.pa5);
 }
 
 template <typename T>
@@ -1628,7 +1667,9 @@ void launch_gauss(const Geo& g, void* spec, double td2, hipStream_t s) {
 template <typename T>
 __global__ void __launch_bounds__(256) k_dyn_coef(const T* __restrict__ uf3, const T* __restrict__ m5,
                                                   const T* __restrict__ l6, const T* __restrict__ sa5,
-                                                  T* __restrict__ c, size_t n, double td2e, int piomelli) {
+                                                  T* __restrict__ c, size_t n, double td2e, int piomelli,
+                                                  const T* __restrict__ wf3, const T* __restrict__ pa5,
+                                                  T* __restrict__ cp, int ppio) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const T ux = uf3[i], uy = uf3[n + i], uz = uf3[2 * n + i];
@@ -1638,39 +1679,75 @@ __global__ void __launch_bounds__(256) k_dyn_coef(const T* __restrict__ uf3, con
   const T tr = (lxx + lyy + lzz) / (T)3;
   const T Lxx = lxx - tr, Lyy = lyy - tr, Lzz = lzz - tr;
   const T h[5] = {m5[i], m5[n + i], m5[2 * n + i], m5[3 * n + i], m5[4 * n + i]};
-  if (piomelli) {  // M = 2 td2e |Sh| Sh, Mn = M / (M:M), c = (sa + L):Mn (dynamic_smagorinsky.jl:168-182)
+  const T a[5] = {sa5[i], sa5[n + i], sa5[2 * n + i], sa5[3 * n + i], sa5[4 * n + i]};
+  T pa[5] = {0, 0, 0, 0, 0};
+  if (pa5) {
+#pragma unroll
+    for (int e = 0; e < 5; ++e) pa[e] = pa5[e * n + i];
+  }
+  // X:Y of a traceless symmetric tau (5 stored) + L (traceless, full) with a traceless symmetric Y (5 stored)
+  auto tl_dot = [&](const T t[5], const T y[5]) {
+    const T yzz = -y[0] - y[3];
+    return (t[0] + Lxx) * y[0] + (t[3] + Lyy) * y[3] + ((-t[0] - t[3]) + Lzz) * yzz +
+           (T)2 * ((t[1] + lxy) * y[1] + (t[2] + lxz) * y[2] + (t[4] + lyz) * y[4]);
+  };
+  auto dot5 = [&](const T x[5], const T y[5]) {
+    const T xzz = -x[0] - x[3], yzz = -y[0] - y[3];
+    return x[0] * y[0] + x[3] * y[3] + xzz * yzz + (T)2 * (x[1] * y[1] + x[2] * y[2] + x[4] * y[4]);
+  };
+  if (piomelli) {  // M = 2 td2e |Sh| Sh, Mn = M / (M:M), c = (tau + L):Mn, tau = sa (+ P under piomelliSandP)
     const T f = (T)2 * (T)td2e * norm_s5<T>(h[0], h[1], h[2], h[3], h[4]);
-    T M[5];
+    T M[5], Mn[5], tau[5];
 #pragma unroll
     for (int e = 0; e < 5; ++e) M[e] = f * h[e];
+    const T MM = dot5(M, M);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      Mn[e] = M[e] / MM;
+      tau[e] = ppio == 2 ? a[e] + pa[e] : a[e];
+    }
+    c[i] = fmax((T)-1, fmin((T)1, tl_dot(tau, Mn)));
+  } else {  // M = td2e |Sh| Sh - sa, c = 0.5 (L:M)/(M:M)
+    const T f = (T)td2e * norm_s5<T>(h[0], h[1], h[2], h[3], h[4]);
+    T M[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) M[e] = f * h[e] - a[e];
     const T Mzz = -M[0] - M[3];
+    const T LM = Lxx * M[0] + Lyy * M[3] + Lzz * Mzz + (T)2 * (lxy * M[1] + lxz * M[2] + lyz * M[4]);
     const T MM = M[0] * M[0] + M[3] * M[3] + Mzz * Mzz + (T)2 * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
-    T Mn[6];
-#pragma unroll
-    for (int e = 0; e < 5; ++e) Mn[e] = M[e] / MM;
-    Mn[5] = Mzz / MM;
-    const T a[5] = {sa5[i], sa5[n + i], sa5[2 * n + i], sa5[3 * n + i], sa5[4 * n + i]};
-    const T txx = a[0] + Lxx, tyy = a[3] + Lyy, tzz = (-a[0] - a[3]) + Lzz;
-    const T v = txx * Mn[0] + tyy * Mn[3] + tzz * Mn[5] +
-                (T)2 * ((a[1] + lxy) * Mn[1] + (a[2] + lxz) * Mn[2] + (a[4] + lyz) * Mn[4]);
-    c[i] = fmax((T)-1, fmin((T)1, v));
-    return;
+    c[i] = fmax((T)-1, fmin((T)1, (T)0.5 * (LM / MM)));
   }
-  // M = td2e |Sh| Sh - sa (traceless symmetric, zz = -xx - yy)
-  const T f = (T)td2e * norm_s5<T>(h[0], h[1], h[2], h[3], h[4]);
-  T M[5];
+  if (ppio) {  // the SandP coefficient from Ph = Lie(Sh, AntiSymTen(wf/2)) (dynamic_smagorinsky.jl:185-199)
+    T Ph[5];
+    lie_s_w<T>(h, wf3[i], wf3[n + i], wf3[2 * n + i], Ph);
+    const T zero[5] = {0, 0, 0, 0, 0};
+    T Mp[5];
+    if (ppio == 1) {  // dynamicSandP: Mp = td2e Ph - P, cp = L:Mp / Mp:Mp
 #pragma unroll
-  for (int e = 0; e < 5; ++e) M[e] = f * h[e] - sa5[e * n + i];
-  const T Mzz = -M[0] - M[3];
-  const T LM = Lxx * M[0] + Lyy * M[3] + Lzz * Mzz + (T)2 * (lxy * M[1] + lxz * M[2] + lyz * M[4]);
-  const T MM = M[0] * M[0] + M[3] * M[3] + Mzz * Mzz + (T)2 * (M[1] * M[1] + M[2] * M[2] + M[4] * M[4]);
-  c[i] = fmax((T)-1, fmin((T)1, (T)0.5 * (LM / MM)));
+      for (int e = 0; e < 5; ++e) Mp[e] = (T)td2e * Ph[e] - pa[e];
+      cp[i] = fmax((T)-1, fmin((T)1, tl_dot(zero, Mp) / dot5(Mp, Mp)));
+    } else {  // piomelliSandP: Mp = td2e Ph, cp = (tau + L):(Mp / Mp:Mp), tau = sa + P or P
+#pragma unroll
+      for (int e = 0; e < 5; ++e) Mp[e] = (T)td2e * Ph[e];
+      const T MM = dot5(Mp, Mp);
+      T Mpn[5], tau[5];
+#pragma unroll
+      for (int e = 0; e < 5; ++e) {
+        Mpn[e] = Mp[e] / MM;
+        tau[e] = piomelli ? a[e] + pa[e] : pa[e];
+      }
+      cp[i] = fmax((T)-1, fmin((T)1, tl_dot(tau, Mpn)));
+    }
+  }
 }
 
 template <typename T>
 void launch_dyn_coef(const T* uf3, const T* m5, const T* l6, const T* sa5, T* c, size_t n_padded, double td2e,
-                     hipStream_t s, int piomelli) {
-  k_dyn_coef<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(uf3, m5, l6, sa5, c, n_padded, td2e, piomelli);
+                     hipStream_t s, int piomelli, const DynP& dp) {
+  const int ppio = dp.wf3 ? (dp.cpstar ? 2 : 1) : 0;
+  k_dyn_coef<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(uf3, m5, l6, sa5, c, n_padded, td2e, piomelli,
+                                                                   (const T*)dp.wf3, (const T*)dp.pa5, (T*)dp.cp,
+                                                                   ppio);
 }
 
 template <typename T>
@@ -1709,18 +1786,30 @@ void launch_fill_real(T* a, size_t n, double v, hipStream_t s) {
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_dyn_tau(const T* __restrict__ s5, const T* __restrict__ c,
-                                                 T* __restrict__ t5, size_t n, double d2, double cmin) {
+                                                 T* __restrict__ t5, size_t n, double d2, double cmin,
+                                                 const T* __restrict__ w3, const T* __restrict__ cp) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const T S[5] = {s5[i], s5[n + i], s5[2 * n + i], s5[3 * n + i], s5[4 * n + i]};
   const T nut = fmax((T)cmin, fmin((T)0.4, c[i])) * (T)d2 * norm_s5<T>(S[0], S[1], S[2], S[3], S[4]);
+  T t[5];
 #pragma unroll
-  for (int e = 0; e < 5; ++e) t5[e * n + i] = (T)2 * nut * S[e];
+  for (int e = 0; e < 5; ++e) t[e] = (T)2 * nut * S[e];
+  if (cp) {  // + clamp(cp, -0.8, 0.8) Delta^2 Lie(S, AntiSymTen(w/2)) (les_real_calc.jl:116-118)
+    T P[5];
+    lie_s_w<T>(S, w3[i], w3[n + i], w3[2 * n + i], P);
+    const T fp = fmax((T)-0.8, fmin((T)0.8, cp[i])) * (T)d2;
+#pragma unroll
+    for (int e = 0; e < 5; ++e) t[e] += fp * P[e];
+  }
+#pragma unroll
+  for (int e = 0; e < 5; ++e) t5[e * n + i] = t[e];
 }
 
 template <typename T>
-void launch_dyn_tau(const T* s5, const T* c, T* t5, size_t n_padded, double d2, double cmin, hipStream_t s) {
-  k_dyn_tau<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(s5, c, t5, n_padded, d2, cmin);
+void launch_dyn_tau(const T* s5, const T* c, T* t5, size_t n_padded, double d2, double cmin, hipStream_t s,
+                    const T* w3, const T* cp) {
+  k_dyn_tau<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(s5, c, t5, n_padded, d2, cmin, w3, cp);
 }
 
 // div!(rhs, uu) of the conservation form (models.jl:176-180): rhs_c = (i k) . uu, uu symmetric
@@ -2823,13 +2912,15 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   template void launch_div_uu<T>(const Geo&, const void*, void*, hipStream_t);                                     \
   template void launch_les_tau_real<T>(T*, size_t, double, hipStream_t);                                          \
   template void launch_les_model_real<T>(T*, const T*, size_t, const LesArgs&, hipStream_t);                      \
-  template void launch_dyn_prep<T>(const Geo&, const void*, void*, double, hipStream_t);                           \
-  template void launch_dyn_products<T>(const T*, const T*, T*, T*, size_t, double, hipStream_t, const T*);        \
+  template void launch_dyn_prep<T>(const Geo&, const void*, void*, double, hipStream_t, int);                      \
+  template void launch_dyn_products<T>(const T*, const T*, T*, T*, size_t, double, hipStream_t, const T*,          \
+                                       const DynP&);                                                               \
   template void launch_gauss<T>(const Geo&, void*, double, hipStream_t);                                           \
-  template void launch_dyn_coef<T>(const T*, const T*, const T*, const T*, T*, size_t, double, hipStream_t, int); \
+  template void launch_dyn_coef<T>(const T*, const T*, const T*, const T*, T*, size_t, double, hipStream_t, int,   \
+                                   const DynP&);                                                                   \
   template void launch_piomelli_cstar<T>(T*, T*, const DevState*, double*, size_t, hipStream_t);                  \
   template void launch_fill_real<T>(T*, size_t, double, hipStream_t);                                             \
-  template void launch_dyn_tau<T>(const T*, const T*, T*, size_t, double, double, hipStream_t);                   \
+  template void launch_dyn_tau<T>(const T*, const T*, T*, size_t, double, double, hipStream_t, const T*, const T*); \
   template int launch_stats<T>(const Geo&, const void*, const void*, const void*, const StatsAux&, double*, double*,  \
                                hipStream_t);                                                                       \
   template void launch_spectrum<T>(const Geo&, const void*, double*, double*, double*, double*, double*, double*,     \

@@ -131,6 +131,9 @@ struct ffs_sim {
   bool dyn = false;
   bool pio = false;           // piomelliSmagorinsky: c and cm1 (DR_PC, DR_PCM1) and dtm1 persist across steps
   double* pio_dtm1 = nullptr;
+  bool dynp = false;          // dynamicSandP / piomelliSandP: the P part (w, G w, P, cp)
+  bool piop = false;          // piomelliSandP: cp and cpm1 (DR_CP, DR_CPM1) and dtpm1 persist
+  double* pio_dtpm1 = nullptr;
   double dyn_td2 = 0.0, dyn_cmin = 0.0;
   Geo gf{};                   // every mode of the grid (full spectrum layout)
   void* dspec = nullptr;      // compact S (5), G S (5), G u (3)
@@ -140,8 +143,9 @@ struct ffs_sim {
   void* fw2 = nullptr;
   // slots of dreal: u, then S, M = G S, uf = G u in dspec's order, then Delta^2 |S| S (later tau), L, c
   // (Piomelli: its state c, cm1 and their copies around a stress read-out)
+  // (the SandP variants: w, G w, P, cp, cpm1 and the copies of cp, cpm1)
   enum { DR_U = 0, DR_S = 3, DR_M = 8, DR_UF = 13, DR_SA = 16, DR_L = 21, DR_C = 27, DR_PC = 28, DR_PCM1 = 29,
-         DR_SAVE = 30, kDynReal = 32 };
+         DR_SAVE = 30, DR_W = 32, DR_WF = 35, DR_PA = 38, DR_CP = 43, DR_CPM1 = 44, DR_SAVEP = 45, kDynReal = 47 };
   double alpha = 0.0, gz = 0.0, drhodz = 0.0;
   double palpha = 0.0;  // passive scalar diffusivity nu / scalarPr
   void* w1 = nullptr;
@@ -399,17 +403,25 @@ struct ffs_sim {
     }
     if (dyn) {
       const size_t nf = (size_t)gf.sx * gf.ny * gf.nz;
-      dspec = dalloc(13 * S * csz);
-      dreal = dalloc(kDynReal * padded_len() * (csz / 2));
+      dspec = dalloc(dyn_spec_fields() * S * csz);
+      dreal = dalloc(dyn_real_fields() * padded_len() * (csz / 2));
       fspec = dalloc(nf * csz);
       fw1 = dalloc(nf * csz);
       fw2 = dalloc(nf * csz);
-      if (!dry) HIPCHECK(hipMemset(dspec, 0, 13 * S * csz));
+      if (!dry) HIPCHECK(hipMemset(dspec, 0, dyn_spec_fields() * S * csz));
       if (pio) {  // c = cm1 = 0.0289, dtm1 = 0 (PiomelliSmagorinsky, les_types.jl:296-300)
         pio_dtm1 = (double*)dalloc(sizeof(double));
         if (!dry) {
           HIPCHECK(hipMemset(pio_dtm1, 0, sizeof(double)));
           FFS_DISPATCH_R(launch_fill_real, rptr(dreal, DR_PC), 2 * padded_len(), 0.0289, nullptr);
+          HIPCHECK(hipDeviceSynchronize());
+        }
+      }
+      if (piop) {  // cp = cpm1 = 0.09, dtpm1 = 0 (PiomelliSandP, les_types.jl:320-329)
+        pio_dtpm1 = (double*)dalloc(sizeof(double));
+        if (!dry) {
+          HIPCHECK(hipMemset(pio_dtpm1, 0, sizeof(double)));
+          FFS_DISPATCH_R(launch_fill_real, rptr(dreal, DR_CP), 2 * padded_len(), 0.09, nullptr);
           HIPCHECK(hipDeviceSynchronize());
         }
       }
@@ -647,7 +659,7 @@ struct ffs_sim {
     p = in;
     if (p.size != sizeof(ffs_params)) throw FfsError(FFS_ERR_ARG, "ffs_params.size mismatch (ABI version)");
     if (p.integrator != FFS_AB3 && p.integrator != FFS_ETD3) throw FfsError(FFS_ERR_ARG, "unknown integrator");
-    if (p.les_model < FFS_LES_NONE || p.les_model > FFS_LES_PIOMELLI_SMAGORINSKY)
+    if (p.les_model < FFS_LES_NONE || p.les_model > FFS_LES_PIOMELLI_SANDP)
       throw FfsError(FFS_ERR_ARG, "unknown LES model");
     if (p.les_model == FFS_LES_FAKE_SMAGORINSKY)
       throw FfsError(FFS_ERR_ARG, "fakeSmagorinsky is built only as the S model of SandP");
@@ -655,6 +667,9 @@ struct ffs_sim {
         p.les_s_model != FFS_LES_SMAGORINSKY && p.les_s_model != FFS_LES_VREMAN &&
         p.les_s_model != FFS_LES_PRODUCTION && p.les_s_model != FFS_LES_FAKE_SMAGORINSKY)
       throw FfsError(FFS_ERR_ARG, "SandP's sLesModel: Smagorinsky, Vreman, productionViscosity or fakeSmagorinsky");
+    if ((p.les_model == FFS_LES_DYNAMIC_SANDP || p.les_model == FFS_LES_PIOMELLI_SANDP) &&
+        p.les_s_model != FFS_LES_DYNAMIC_SMAGORINSKY && p.les_s_model != FFS_LES_PIOMELLI_SMAGORINSKY)
+      throw FfsError(FFS_ERR_ARG, "the dynamic SandP models' sLesModel: dynamicSmagorinsky or piomelliSmagorinsky");
     if (p.forcing < FFS_FORCING_NONE || p.forcing > FFS_FORCING_A) throw FfsError(FFS_ERR_ARG, "unsupported forcing");
     plane = p.forcing >= FFS_FORCING_RF && p.forcing <= FFS_FORCING_A;
     if (plane && p.forcing != FFS_FORCING_QF && !(p.forcing_tf > 0))
@@ -703,8 +718,12 @@ struct ffs_sim {
       lesm.c = p.smagorinsky_c;
       lesm.cp = p.les_cp;
       lesm.d2 = delta * delta;
-      pio = p.les_model == FFS_LES_PIOMELLI_SMAGORINSKY;
-      dyn = p.les_model == FFS_LES_DYNAMIC_SMAGORINSKY || pio;
+      dynp = p.les_model == FFS_LES_DYNAMIC_SANDP || p.les_model == FFS_LES_PIOMELLI_SANDP;
+      piop = p.les_model == FFS_LES_PIOMELLI_SANDP;
+      pio = p.les_model == FFS_LES_PIOMELLI_SMAGORINSKY || (dynp && p.les_s_model == FFS_LES_PIOMELLI_SMAGORINSKY);
+      dyn = p.les_model == FFS_LES_DYNAMIC_SMAGORINSKY || pio || dynp;
+      if (dynp && p.equation == FFS_EQ_NS)  // P reads the vorticity the rotational form's rhs holds
+        throw FfsError(FFS_ERR_ARG, "the dynamic SandP models need equation = rotational here");
       if (dyn) {  // DynamicSmagorinsky(Delta, TestFilterWidth = 2 Delta, dims, cmin) (les_types.jl:383-387)
         const double td = p.les_test_width > 0 ? p.les_test_width : 2 * delta;
         dyn_td2 = td * td;
@@ -815,6 +834,8 @@ struct ffs_sim {
   }
 
   size_t padded_len() const { return 2 * (size_t)(p.nx / 2 + 1) * p.ny * p.nz; }
+  int dyn_spec_fields() const { return dynp ? 19 : 13; }
+  int dyn_real_fields() const { return dynp ? kDynReal : DR_W; }
   char* rptr(void* base, size_t fields) { return (char*)base + fields * padded_len() * (csz / 2); }
 
   // compact spectra -> padded real, in groups the pencil scratch holds
@@ -848,41 +869,72 @@ struct ffs_sim {
   // into dreal's slots DR_SA..DR_SA+4, and with store its dealiased transform into tau
   void dynamic_les(bool store) {
     const size_t rb = padded_len() * (csz / 2);
-    if (pio && !store) {  // a read-out leaves the coefficient state as the next step finds it
-      HIPCHECK(hipMemcpyAsync(rptr(dreal, DR_SAVE), rptr(dreal, DR_PC), 2 * rb, hipMemcpyDeviceToDevice, stream));
-      HIPCHECK(hipMemcpyAsync(&sd_save_dtm1, pio_dtm1, sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (!store) {  // a read-out leaves the coefficient state as the next step finds it
+      if (pio) {
+        HIPCHECK(hipMemcpyAsync(rptr(dreal, DR_SAVE), rptr(dreal, DR_PC), 2 * rb, hipMemcpyDeviceToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(&sd_save_dtm1, pio_dtm1, sizeof(double), hipMemcpyDeviceToHost, stream));
+      }
+      if (piop) {
+        HIPCHECK(hipMemcpyAsync(rptr(dreal, DR_SAVEP), rptr(dreal, DR_CP), 2 * rb, hipMemcpyDeviceToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(&sd_save_dtpm1, pio_dtpm1, sizeof(double), hipMemcpyDeviceToHost, stream));
+      }
       HIPCHECK(hipStreamSynchronize(stream));
     }
-    if (pio) {  // realspace_dynamic_les_calculation!: cm1 = c at iteration 0, then calc_cstar!
+    // realspace_dynamic_les_calculation!: cm1 = c at iteration 0, then calc_cstar!; the same for cp
+    if (pio) {
       if (iteration == 0)
         HIPCHECK(hipMemcpyAsync(rptr(dreal, DR_PCM1), rptr(dreal, DR_PC), rb, hipMemcpyDeviceToDevice, stream));
-      launch("dyn_cstar", [&] {
-        if (fp32)
-          launch_piomelli_cstar<float>(rslot<float>(DR_PC), rslot<float>(DR_PCM1), st, pio_dtm1, padded_len(), stream);
-        else
-          launch_piomelli_cstar<double>(rslot<double>(DR_PC), rslot<double>(DR_PCM1), st, pio_dtm1, padded_len(),
-                                        stream);
-      });
+      launch("dyn_cstar", [&] { piomelli_cstar(DR_PC, DR_PCM1, pio_dtm1); });
     }
-    launch("dyn_prep", [&] { FFS_DISPATCH(launch_dyn_prep, g, u, dspec, dyn_td2, stream); });
+    if (piop) {
+      if (iteration == 0)
+        HIPCHECK(hipMemcpyAsync(rptr(dreal, DR_CPM1), rptr(dreal, DR_CP), rb, hipMemcpyDeviceToDevice, stream));
+      launch("dyn_cstar", [&] { piomelli_cstar(DR_CP, DR_CPM1, pio_dtpm1); });
+    }
+    launch("dyn_prep", [&] { FFS_DISPATCH(launch_dyn_prep, g, u, dspec, dyn_td2, stream, dynp ? 1 : 0); });
     inverse_many(u, 3, rptr(dreal, DR_U));
     inverse_many(dspec, 13, rptr(dreal, DR_S));  // S, M, uf
+    if (dynp) inverse_many(cptr(dspec, 13 * S), 6, rptr(dreal, DR_W));  // w, G w
     if (fp32) dyn_products<float>();
     else dyn_products<double>();
     for (int f = 0; f < 5; ++f) filter_real(rptr(dreal, DR_SA + f));
     for (int f = 0; f < 6; ++f) filter_real(rptr(dreal, DR_L + f));
+    if (dynp)
+      for (int f = 0; f < 5; ++f) filter_real(rptr(dreal, DR_PA + f));
     if (fp32) dyn_coef<float>();
     else dyn_coef<double>();
     filter_real(rptr(dreal, coef_slot()));  // average_c! (avg = true)
+    if (dynp) filter_real(rptr(dreal, DR_CP));
     if (fp32) dyn_tau<float>();
     else dyn_tau<double>();
     if (store) forward_from_real(rptr(dreal, DR_SA), 5, tau);
-    if (pio && !store) {
-      HIPCHECK(hipMemcpyAsync(rptr(dreal, DR_PC), rptr(dreal, DR_SAVE), 2 * rb, hipMemcpyDeviceToDevice, stream));
-      HIPCHECK(hipMemcpyAsync(pio_dtm1, &sd_save_dtm1, sizeof(double), hipMemcpyHostToDevice, stream));
+    if (!store) {
+      if (pio) {
+        HIPCHECK(hipMemcpyAsync(rptr(dreal, DR_PC), rptr(dreal, DR_SAVE), 2 * rb, hipMemcpyDeviceToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(pio_dtm1, &sd_save_dtm1, sizeof(double), hipMemcpyHostToDevice, stream));
+      }
+      if (piop) {
+        HIPCHECK(hipMemcpyAsync(rptr(dreal, DR_CP), rptr(dreal, DR_SAVEP), 2 * rb, hipMemcpyDeviceToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(pio_dtpm1, &sd_save_dtpm1, sizeof(double), hipMemcpyHostToDevice, stream));
+      }
       HIPCHECK(hipStreamSynchronize(stream));
     }
   }
+  void piomelli_cstar(int c, int cm1, double* dtm1) {
+    if (fp32) launch_piomelli_cstar<float>(rslot<float>(c), rslot<float>(cm1), st, dtm1, padded_len(), stream);
+    else launch_piomelli_cstar<double>(rslot<double>(c), rslot<double>(cm1), st, dtm1, padded_len(), stream);
+  }
+  DynP dyn_p() {
+    DynP d;
+    if (!dynp) return d;
+    d.w3 = rptr(dreal, DR_W);
+    d.wf3 = rptr(dreal, DR_WF);
+    d.pa5 = rptr(dreal, DR_PA);
+    d.cpstar = piop ? rptr(dreal, DR_CP) : nullptr;
+    d.cp = rptr(dreal, DR_CP);
+    return d;
+  }
+  double sd_save_dtpm1 = 0.0;
   double sd_save_dtm1 = 0.0;
   // the coefficient field: the step's own (dynamicSmagorinsky) or the persistent one (Piomelli)
   int coef_slot() const { return pio ? DR_PC : DR_C; }
@@ -890,20 +942,20 @@ struct ffs_sim {
   template <typename T> void dyn_products() {
     launch("dyn_products", [&] {
       launch_dyn_products<T>(rslot<T>(DR_S), rslot<T>(DR_U), rslot<T>(DR_SA), rslot<T>(DR_L), padded_len(), lesm.d2,
-                             stream, pio ? rslot<T>(DR_PC) : nullptr);
+                             stream, pio ? rslot<T>(DR_PC) : nullptr, dyn_p());
     });
   }
   template <typename T> void dyn_coef() {
     // the filter widths add up in the coefficient (dynamic_smagorinsky.jl:155)
     launch("dyn_coef", [&] {
       launch_dyn_coef<T>(rslot<T>(DR_UF), rslot<T>(DR_M), rslot<T>(DR_L), rslot<T>(DR_SA), rslot<T>(coef_slot()),
-                         padded_len(), dyn_td2 + lesm.d2, stream, pio ? 1 : 0);
+                         padded_len(), dyn_td2 + lesm.d2, stream, pio ? 1 : 0, dyn_p());
     });
   }
   template <typename T> void dyn_tau() {
     launch("dyn_tau", [&] {
       launch_dyn_tau<T>(rslot<T>(DR_S), rslot<T>(coef_slot()), rslot<T>(DR_SA), padded_len(), lesm.d2, dyn_cmin,
-                        stream);
+                        stream, dynp ? rslot<T>(DR_W) : nullptr, dynp ? rslot<T>(DR_CP) : nullptr);
     });
   }
 
@@ -1414,6 +1466,7 @@ struct ffs_sim {
     for (void* b : ckpt_buffers()) n += ckpt_buffer_bytes(b);
     if (plane) n += 4 * pfd.nslot * sizeof(double);  // the forcing's R, Em, N, Nm1
     if (pio) n += 2 * padded_len() * (csz / 2) + sizeof(double);  // Piomelli's c, cm1 and dtm1
+    if (piop) n += 2 * padded_len() * (csz / 2) + sizeof(double);  // and cp, cpm1, dtpm1
     return n;
   }
 
@@ -1460,6 +1513,13 @@ struct ffs_sim {
       HIPCHECK(hipMemcpyAsync(out + off, rptr(dreal, DR_PC), rb, hipMemcpyDeviceToHost, stream));
       off += rb;
       HIPCHECK(hipMemcpyAsync(out + off, pio_dtm1, sizeof(double), hipMemcpyDeviceToHost, stream));
+      off += sizeof(double);
+    }
+    if (piop) {
+      const size_t rb = 2 * padded_len() * (csz / 2);
+      HIPCHECK(hipMemcpyAsync(out + off, rptr(dreal, DR_CP), rb, hipMemcpyDeviceToHost, stream));
+      off += rb;
+      HIPCHECK(hipMemcpyAsync(out + off, pio_dtpm1, sizeof(double), hipMemcpyDeviceToHost, stream));
     }
     HIPCHECK(hipStreamSynchronize(stream));
   }
@@ -1494,6 +1554,13 @@ struct ffs_sim {
       HIPCHECK(hipMemcpy(rptr(dreal, DR_PC), in + off, rb, hipMemcpyHostToDevice));
       off += rb;
       HIPCHECK(hipMemcpy(pio_dtm1, in + off, sizeof(double), hipMemcpyHostToDevice));
+      off += sizeof(double);
+    }
+    if (piop) {
+      const size_t rb = 2 * padded_len() * (csz / 2);
+      HIPCHECK(hipMemcpy(rptr(dreal, DR_CP), in + off, rb, hipMemcpyHostToDevice));
+      off += rb;
+      HIPCHECK(hipMemcpy(pio_dtpm1, in + off, sizeof(double), hipMemcpyHostToDevice));
     }
     iteration = h.iteration;
     tit = h.tit;

@@ -24,12 +24,14 @@ FFS_FP64, FFS_FP32 = 0, 1
 FFS_LES_NONE, FFS_LES_SMAGORINSKY = 0, 1
 FFS_LES_VREMAN, FFS_LES_SILVIS, FFS_LES_PRODUCTION, FFS_LES_STABLE_NL = 2, 3, 4, 5
 FFS_LES_FAKE_SMAGORINSKY, FFS_LES_SANDP, FFS_LES_DYNAMIC_SMAGORINSKY, FFS_LES_PIOMELLI_SMAGORINSKY = 6, 7, 8, 9
+FFS_LES_DYNAMIC_SANDP, FFS_LES_PIOMELLI_SANDP = 10, 11
 # lesModel / sLesModel values of the `global` file (les_types.jl:364-404)
 LES_MODELS = {"Smagorinsky": FFS_LES_SMAGORINSKY, "Vreman": FFS_LES_VREMAN, "Silvis": FFS_LES_SILVIS,
               "productionViscosity": FFS_LES_PRODUCTION, "stableNonLinear": FFS_LES_STABLE_NL,
               "fakeSmagorinsky": FFS_LES_FAKE_SMAGORINSKY, "SandP": FFS_LES_SANDP,
               "dynamicSmagorinsky": FFS_LES_DYNAMIC_SMAGORINSKY,
-              "piomelliSmagorinsky": FFS_LES_PIOMELLI_SMAGORINSKY}
+              "piomelliSmagorinsky": FFS_LES_PIOMELLI_SMAGORINSKY, "dynamicSandP": FFS_LES_DYNAMIC_SANDP,
+              "piomelliSandP": FFS_LES_PIOMELLI_SANDP}
 FFS_FORCING_NONE, FFS_FORCING_SHELL, FFS_FORCING_RF, FFS_FORCING_QF = 0, 1, 2, 3
 FFS_FORCING_CRF, FFS_FORCING_NRF, FFS_FORCING_CNRF, FFS_FORCING_A = 4, 5, 6, 7
 FFS_HV_NONE, FFS_HV_HYPER, FFS_HV_BARRIER = 0, 1, 2

@@ -274,9 +274,10 @@ def parameters(d):
     p.precision = opt("precision", str, "fp64")
     if "lesModel" in d:  # les_types (les_types.jl:360-404): each model reads its own constant keys
         p.les_model = d["lesModel"]
-        smodel = d.get("sLesModel") if p.les_model == "SandP" else None
+        sandp = p.les_model in ("SandP", "dynamicSandP", "piomelliSandP")
+        smodel = d.get("sLesModel") if sandp else None
         known.add("sLesModel")
-        if p.les_model == "SandP" and smodel is None:
+        if sandp and smodel is None:
             raise KeyError("sLesModel")
         key = {"Smagorinsky": "smagorinskyConstant", "Vreman": "vremanConstant", "Silvis": "silvisConstant",
                "productionViscosity": "productionConstant", "stableNonLinear": "nlConstant"}
@@ -288,7 +289,9 @@ def parameters(d):
         elif p.les_model == "SandP":
             p.les_cp = opt("pConstant", parse_expr, None)
             p.les_s_model = smodel
-        elif p.les_model in ("dynamicSmagorinsky", "piomelliSmagorinsky"):
+        elif p.les_model in ("dynamicSmagorinsky", "piomelliSmagorinsky", "dynamicSandP", "piomelliSandP"):
+            if sandp:
+                p.les_s_model = smodel
             p.les_test_width = opt("TestFilterWidth", parse_expr, 0.0)
             p.les_cmin = opt("cmin", float, 0.0)
         p.filter_width = opt("filterWidth", parse_expr, 0.0)

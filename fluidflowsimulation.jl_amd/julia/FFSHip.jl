@@ -108,19 +108,20 @@ function params_from_global(d::AbstractDict)
     if haskey(d, :lesModel)   # les_types (les_types.jl:360-404); dynamic models are refused by ffs_create
         codes = Dict("Smagorinsky" => 1, "Vreman" => 2, "Silvis" => 3, "productionViscosity" => 4,
                      "stableNonLinear" => 5, "fakeSmagorinsky" => 6, "SandP" => 7, "dynamicSmagorinsky" => 8,
-                     "piomelliSmagorinsky" => 9)
+                     "piomelliSmagorinsky" => 9, "dynamicSandP" => 10, "piomelliSandP" => 11)
         ckey = Dict("Smagorinsky" => :smagorinskyConstant, "Vreman" => :vremanConstant,
                     "Silvis" => :silvisConstant, "productionViscosity" => :productionConstant,
                     "stableNonLinear" => :nlConstant)
         m = d[:lesModel]
         p.les_model = Int32(get(codes, m, -1))
-        sm = m == "SandP" ? d[:sLesModel] : m
-        m == "SandP" && (p.les_s_model = Int32(get(codes, sm, -1)))
+        sandp = m in ("SandP", "dynamicSandP", "piomelliSandP")
+        sm = sandp ? d[:sLesModel] : m
+        sandp && (p.les_s_model = Int32(get(codes, sm, -1)))
         k = get(ckey, sm, nothing)
         k !== nothing && haskey(d, k) && (p.smagorinsky_c = num(k))
         m == "Silvis" && haskey(d, :silvisPConstant) && (p.les_cp = num(:silvisPConstant))
         m == "SandP" && haskey(d, :pConstant) && (p.les_cp = num(:pConstant))
-        dynm = m == "dynamicSmagorinsky" || m == "piomelliSmagorinsky"
+        dynm = m in ("dynamicSmagorinsky", "piomelliSmagorinsky", "dynamicSandP", "piomelliSandP")
         dynm && haskey(d, :TestFilterWidth) && (p.les_test_width = num(:TestFilterWidth))
         dynm && haskey(d, :cmin) && (p.les_cmin = parse(Float64, d[:cmin]))
         haskey(d, :filterWidth) && (p.filter_width = num(:filterWidth))

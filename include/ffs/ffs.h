@@ -45,10 +45,9 @@ extern "C" {
 #define FFS_FP32 1 /* FP32 fields/FFTs; FP64 coefficients and reductions */
 
 /* LES models (`lesModel` key, LESmodels/les_types.jl:360-404).  The pointwise
- * models of calc_les! (les_real_calc.jl:74-114), dynamicSmagorinsky and
- * piomelliSmagorinsky (dynamic_smagorinsky.jl, dynamic_piomelli.jl);
- * dynamicSandP, piomelliSandP and a standalone fakeSmagorinsky are not built and
- * are refused by ffs_create. */
+ * models of calc_les! (les_real_calc.jl:74-114) and the dynamic ones
+ * (dynamic_smagorinsky.jl, dynamic_piomelli.jl).  A standalone fakeSmagorinsky is
+ * not built and is refused by ffs_create. */
 #define FFS_LES_NONE 0
 #define FFS_LES_SMAGORINSKY 1
 #define FFS_LES_VREMAN 2           /* VremanLESModel.jl */
@@ -59,6 +58,8 @@ extern "C" {
 #define FFS_LES_SANDP 7            /* SandP: an S model (les_s_model) + cb Delta^2 Lie(S, W) */
 #define FFS_LES_DYNAMIC_SMAGORINSKY 8 /* dynamicSmagorinsky: Gaussian test filter, averaged coefficient */
 #define FFS_LES_PIOMELLI_SMAGORINSKY 9 /* piomelliSmagorinsky: the coefficient extrapolated in time */
+#define FFS_LES_DYNAMIC_SANDP 10  /* dynamicSandP: a dynamic S model (les_s_model) + a dynamic P part */
+#define FFS_LES_PIOMELLI_SANDP 11 /* piomelliSandP: the same with the P coefficient extrapolated in time */
 
 /* forcing (`forcing` key); FFS_FORCING_SHELL is the constant-power isotropic
  * low-wavenumber forcing f = (P / sum_{0<|k|<=kf} |u_k|^2) u_k, which has no
@@ -166,8 +167,9 @@ typedef struct ffs_params {
   int32_t equation;                         /* FFS_EQ_*: equation (simulationtypes.jl:265-273) */
   double les_cp;                            /* silvisPConstant (5.0) or SandP's pConstant cb (0.09);
                                                NaN selects the default */
-  int32_t les_s_model;                      /* SandP's sLesModel: SMAGORINSKY, VREMAN, PRODUCTION or
-                                               FAKE_SMAGORINSKY */
+  int32_t les_s_model;                      /* sLesModel: SandP's SMAGORINSKY, VREMAN, PRODUCTION or
+                                               FAKE_SMAGORINSKY; dynamicSandP's and piomelliSandP's
+                                               DYNAMIC_SMAGORINSKY or PIOMELLI_SMAGORINSKY */
   double les_test_width;                    /* TestFilterWidth of the dynamic models; <= 0 selects
                                                twice the filter width */
   double les_cmin;                          /* cmin of the dynamic models (0): the coefficient's floor */

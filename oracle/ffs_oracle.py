@@ -181,9 +181,14 @@ class OracleSim:
             self.les_d2 = delta * delta
             td = les_test_width if les_test_width else 2 * delta
             self.les_td2, self.les_cmin = td * td, les_cmin
-            if les_model == "piomelliSmagorinsky":  # c, cm1 start at 0.0289, dtm1 at 0 (les_types.jl:296-300)
-                shape = (grid.NRX, grid.NY, grid.NZ)
+            # dynamicSandP / piomelliSandP carry their eddy-viscosity model in les_smodel
+            dmodel = les_smodel if les_model in ("dynamicSandP", "piomelliSandP") else les_model
+            shape = (grid.NRX, grid.NY, grid.NZ)
+            if dmodel == "piomelliSmagorinsky":  # c, cm1 start at 0.0289, dtm1 at 0 (les_types.jl:296-300)
                 self.pio_c, self.pio_cm1, self.pio_dtm1 = np.full(shape, 0.0289), np.full(shape, 0.0289), 0.0
+            if les_model == "piomelliSandP":  # cp, cpm1 start at 0.09, dtpm1 at 0 (les_types.jl:320-329)
+                self.pio_cp, self.pio_cpm1, self.pio_dtpm1 = np.full(shape, 0.09), np.full(shape, 0.09), 0.0
+            self.les_dmodel = dmodel
         self.density = density
         if density is not None:
             self.alpha = nu / density["Pr"]
@@ -279,15 +284,26 @@ class OracleSim:
             ss = Sr[0] ** 2 + Sr[3] ** 2 + szz**2 + 2 * (Sr[1] ** 2 + Sr[2] ** 2 + Sr[4] ** 2)
             if self.les_model == "Smagorinsky":
                 taur = 2 * (self.les_alpha * np.sqrt(2 * ss)) * Sr
-            elif self.les_model == "piomelliSmagorinsky":
-                cstar, self.pio_cm1 = les_models.piomelli_extrapolate(self.pio_c, self.pio_cm1, self.dt,
-                                                                      self.pio_dtm1, self.iteration)
-                self.pio_dtm1 = self.dt
-                taur, self.pio_c = les_models.dynamic_stress(g, u, S, Sr, ur, self.les_d2, self.les_td2,
-                                                             self.les_cmin, cstar=cstar)
-            elif self.les_model == "dynamicSmagorinsky":
-                taur, self.les_coef = les_models.dynamic_stress(g, u, S, Sr, ur, self.les_d2, self.les_td2,
-                                                                self.les_cmin)
+            elif self.les_model in ("dynamicSmagorinsky", "piomelliSmagorinsky", "dynamicSandP", "piomelliSandP"):
+                # realspace_dynamic_les_calculation! (dynamic_smagorinsky.jl:3-41): the Piomelli
+                # coefficients are extrapolated first, c then cp, each with its own previous dt
+                cstar = cpstar = None
+                if self.les_dmodel == "piomelliSmagorinsky":
+                    cstar, self.pio_cm1 = les_models.piomelli_extrapolate(self.pio_c, self.pio_cm1, self.dt,
+                                                                          self.pio_dtm1, self.iteration)
+                    self.pio_dtm1 = self.dt
+                if self.les_model == "piomelliSandP":
+                    cpstar, self.pio_cpm1 = les_models.piomelli_extrapolate(self.pio_cp, self.pio_cpm1, self.dt,
+                                                                            self.pio_dtpm1, self.iteration)
+                    self.pio_dtpm1 = self.dt
+                pmode = {"dynamicSandP": "dyn", "piomelliSandP": "pio"}.get(self.les_model)
+                taur, c, cp = les_models.dynamic_stress(g, u, S, Sr, ur, self.les_d2, self.les_td2, self.les_cmin,
+                                                        cstar=cstar, wh=w, wr=wr, pmode=pmode, cpstar=cpstar)
+                self.les_coef = c
+                if cstar is not None:
+                    self.pio_c = c
+                if cpstar is not None:
+                    self.pio_cp = cp
             else:  # the other pointwise models (les_real_calc.jl:80-114), from S and the vorticity
                 taur = les_models.stress(self.les_model, Sr, wr, self.les_c, self.les_d2, self.les_cp,
                                          self.les_smodel)

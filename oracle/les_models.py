@@ -171,7 +171,7 @@ def piomelli_extrapolate(c, cm1, dt, dtm1, iteration):
     return r * (c - cm1) + c, c.copy()
 
 
-def dynamic_stress(grid, uh, S, Sr, ur, d2, td2, cmin, cstar=None):
+def dynamic_stress(grid, uh, S, Sr, ur, d2, td2, cmin, cstar=None, wh=None, wr=None, pmode=None, cpstar=None):
     """dynamicSmagorinsky with the averaged coefficient (avg = true): fourierspacep1! forms M = G S and
     uf = G u on the loop range (models.jl:98-108); realspace_dynamic_les_calculation! sa = Delta^2 |S| S
     and L = symouter(u, u) (dynamic_smagorinsky.jl:44-76), both test-filtered; the coefficient
@@ -179,7 +179,15 @@ def dynamic_stress(grid, uh, S, Sr, ur, d2, td2, cmin, cstar=None):
     (:131-200), averaged by one more filter; calc_les! t = 2 max(cmin, min(0.4, c)) Delta^2 |S| S
     (les_real_calc.jl:75-77).  |X| = sqrt(2 X:X).  With cstar (piomelliSmagorinsky, the extrapolated
     coefficient field) sa = 2 cstar Delta^2 |S| S and c = clamp((sa + L):M / (M:M), -1, 1) with
-    M = 2 (td2 + Delta^2) |M| M (:62-66, :168-182).  Returns (the five stored components of t, c)."""
+    M = 2 (td2 + Delta^2) |M| M (:62-66, :168-182).
+
+    pmode "dyn" (dynamicSandP) or "pio" (piomelliSandP, with cpstar its extrapolated coefficient) adds
+    the P part (les_types.jl:196-218, 302-337): P = [cpstar] Delta^2 Lie(S, AntiSymTen(w/2)), filtered;
+    wf = G w; Ph = Lie(M, AntiSymTen(wf/2)); "dyn": Mp = (td2 + Delta^2) Ph - P, cp = clamp(L:Mp / Mp:Mp);
+    "pio": Mp = (td2 + Delta^2) Ph, cp = clamp((tau + L):Mp / Mp:Mp) with tau = sa + P under
+    piomelliSmagorinsky, P otherwise (and then c's tau is sa + P too); cp averaged; calc_les! adds
+    clamp(cp, -0.8, 0.8) Delta^2 Lie(S, AntiSymTen(w/2)) (les_real_calc.jl:116-118).
+    Returns (the five stored components of t, c, cp or None)."""
     G = gauss(grid, td2)
     Mr = np.array([grid.irfft(np.where(grid.RANGE, G * c, 0)) for c in S])
     ufr = np.array([grid.irfft(np.where(grid.RANGE, G * c, 0)) for c in uh])
@@ -195,14 +203,38 @@ def dynamic_stress(grid, uh, S, Sr, ur, d2, td2, cmin, cstar=None):
     L = traceless(L)
     Mh = sym_full(Mr)
     nM = np.sqrt(2 * ddot(Mh, Mh))
+    pa = None
+    if pmode is not None:
+        half = cross_matrix([0.5 * x for x in wr])
+        lie = add(mat(Sf, half), mat(half, Sf), -1.0)  # Lie(S, AntiSymTen(w/2))
+        fac = cpstar * d2 if pmode == "pio" else d2
+        pa = np.array([filter_real(grid, x, td2) for x in five(scale(lie, fac))])
     with np.errstate(divide="ignore", invalid="ignore"):
         if cstar is None:
             M = add(scale(Mh, (td2 + d2) * nM), sym_full(sa), -1.0)
             c = np.clip(0.5 * (ddot(L, M) / ddot(M, M)), -1.0, 1.0)
         else:
+            tau = sym_full(sa + pa) if pmode == "pio" else sym_full(sa)
             M = scale(Mh, 2 * (td2 + d2) * nM)
             Mn = scale(M, 1.0 / ddot(M, M))
-            c = np.clip(ddot(add(sym_full(sa), L), Mn), -1.0, 1.0)
+            c = np.clip(ddot(add(tau, L), Mn), -1.0, 1.0)
+        cp = None
+        if pmode is not None:
+            wfr = np.array([grid.irfft(np.where(grid.RANGE, G * x, 0)) for x in wh])
+            hf = cross_matrix([0.5 * x for x in wfr])
+            Ph = add(mat(Mh, hf), mat(hf, Mh), -1.0)  # Lie(Sh, AntiSymTen(wf/2))
+            if pmode == "dyn":
+                Mp = add(scale(Ph, td2 + d2), sym_full(pa), -1.0)
+                cp = np.clip(ddot(L, Mp) / ddot(Mp, Mp), -1.0, 1.0)
+            else:
+                tau = sym_full(sa + pa) if cstar is not None else sym_full(pa)
+                Mp = scale(Ph, td2 + d2)
+                Mpn = scale(Mp, 1.0 / ddot(Mp, Mp))
+                cp = np.clip(ddot(add(tau, L), Mpn), -1.0, 1.0)
     c = filter_real(grid, c, td2)
     nut = np.maximum(cmin, np.minimum(0.4, c)) * d2 * nS
-    return np.array([2 * nut * x for x in Sr]), c
+    t = np.array([2 * nut * x for x in Sr])
+    if cp is not None:
+        cp = filter_real(grid, cp, td2)
+        t = t + five(scale(lie, np.maximum(-0.8, np.minimum(0.8, cp)) * d2))
+    return t, c, cp

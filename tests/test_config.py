@@ -148,4 +148,7 @@ def test_les_model_keys():
     assert (d.les_test_width, d.les_cmin) == (0.3, -0.2) and d.to_c().les_model == _lib.FFS_LES_DYNAMIC_SMAGORINSKY
     pm = config.parameters(dict(base, lesModel="piomelliSmagorinsky", cmin="0.01"))
     assert pm.les_cmin == 0.01 and pm.to_c().les_model == _lib.FFS_LES_PIOMELLI_SMAGORINSKY
-    assert config.parameters(dict(base, lesModel="dynamicSandP", sLesModel="x")).to_c().les_model == -1  # refused
+    ds = config.parameters(dict(base, lesModel="piomelliSandP", sLesModel="dynamicSmagorinsky", cmin="0.02"))
+    assert (ds.les_s_model, ds.les_cmin) == ("dynamicSmagorinsky", 0.02)
+    assert ds.to_c().les_model == _lib.FFS_LES_PIOMELLI_SANDP and ds.to_c().les_s_model == _lib.FFS_LES_DYNAMIC_SMAGORINSKY
+    assert config.parameters(dict(base, lesModel="fancyModel")).to_c().les_model == -1  # refused at create

@@ -143,9 +143,9 @@ def test_default_constants_from_the_library():
 
 
 def test_refused_combinations():
-    print_case("LES models refused: piomelliSandP, standalone fakeSmagorinsky, w-models in the NS form")
+    print_case("LES models refused: piomelliSandP over Vreman, standalone fakeSmagorinsky, w-models in the NS form")
     base = dict(nx=16, ny=16, nz=16, nu=1e-2, dt=1e-3, variable_dt=False)
-    for kw in (dict(les_model="piomelliSandP"), dict(les_model="fakeSmagorinsky"),
+    for kw in (dict(les_model="piomelliSandP", les_s_model="Vreman"), dict(les_model="fakeSmagorinsky"),
                dict(les_model="Vreman", equation="NS"), dict(les_model="SandP", les_s_model="Silvis")):
         with pytest.raises(_lib.FfsError):
             ffs.Simulation(ffs.Params(**base, **kw))
@@ -153,17 +153,17 @@ def test_refused_combinations():
 
 # ---- dynamicSmagorinsky ------------------------------------------------------------------------------
 def dyn_pair(shape, integrator="AB3", prec="fp64", variable_dt=True, seed=81, test_width=None, cmin=0.0,
-             model="dynamicSmagorinsky"):
+             model="dynamicSmagorinsky", smodel=None):
     nx, ny, nz = shape
     g = Grid(nx, ny, nz)
     u0 = random_spectrum(nx, ny, nz, seed=seed, k0=3.0)
     ref = OracleSim(g, 2e-3, dt=1e-3, variable_dt=variable_dt, cfl=0.5, dt_stat=2, integrator=integrator,
-                    les_c=0.0, les_model=model, les_test_width=test_width, les_cmin=cmin)
+                    les_c=0.0, les_model=model, les_test_width=test_width, les_cmin=cmin, les_smodel=smodel)
     ref.set_velocity(u0)
     p = ffs.Params(nx=nx, ny=ny, nz=nz, nu=2e-3, dt=1e-3, variable_dt=variable_dt, cfl=0.5, dt_stat=2,
                    dt_spec=1 << 30, write_time=1 << 30, precision=prec,
                    integrator="Adams_Bashforth3rdO" if integrator == "AB3" else "ETD3rdO",
-                   les_model=model, les_test_width=test_width or 0.0, les_cmin=cmin)
+                   les_model=model, les_test_width=test_width or 0.0, les_cmin=cmin, les_s_model=smodel)
     s = ffs.Simulation(p)
     s.set_velocity(*u0)
     return g, ref, s
@@ -260,6 +260,54 @@ def test_piomelli_stress_readout_and_resume():
         for x, y in zip(a.velocity_hat(), b.velocity_hat()):
             assert np.array_equal(x, y)
         _, _, c = dyn_pair((32, 32, 32), model="piomelliSmagorinsky", variable_dt=True, seed=84)
+        with c:
+            c.restore(blob)
+            c.advance(3)
+            for x, y in zip(a.velocity_hat(), c.velocity_hat()):
+                assert np.array_equal(x, y)
+
+
+SANDP = [("dynamicSandP", "dynamicSmagorinsky"), ("dynamicSandP", "piomelliSmagorinsky"),
+         ("piomelliSandP", "dynamicSmagorinsky"), ("piomelliSandP", "piomelliSmagorinsky")]
+
+
+@pytest.mark.parametrize("model,smodel", SANDP, ids=[f"{m}-{s}" for m, s in SANDP])
+def test_dynamic_sandp_steps(model, smodel):
+    """The P part's coefficient (per step, or extrapolated for piomelliSandP) over several steps."""
+    print_case(f"{model} over {smodel}")
+    g, ref, s = dyn_pair((32, 32, 32), model=model, smodel=smodel, seed=85)
+    with s:
+        ref.initialize()
+        s.initialize()
+        for step in range(5):
+            ref.advance_in_time(1)
+            s.advance(1)
+            err = rel(s.velocity_hat(), ref.uh)
+            print(f"step {step} err {err:.2e}")
+            assert err < 1e-10, (step, err)
+
+
+def test_piomelli_sandp_readout_and_resume():
+    print_case("piomelliSandP over piomelliSmagorinsky: stress read-out and checkpoint resume")
+    kw = dict(model="piomelliSandP", smodel="piomelliSmagorinsky", seed=86)
+    _, ref, a = dyn_pair((32, 32, 32), **kw)
+    _, _, b = dyn_pair((32, 32, 32), **kw)
+    with a, b:
+        a.initialize()
+        b.initialize()
+        a.advance(3)
+        b.advance(3)
+        ref.initialize()
+        ref.advance_in_time(3)
+        ref.calculate_rhs(collect=False)
+        got = b.get_field(_lib.FFS_FIELD_T13, _lib.FFS_SPACE_REAL)[:32]
+        assert np.abs(got - ref.taur[2]).max() <= 1e-10 * np.abs(ref.taur[2]).max()
+        blob = b.checkpoint()
+        a.advance(3)
+        b.advance(3)
+        for x, y in zip(a.velocity_hat(), b.velocity_hat()):
+            assert np.array_equal(x, y)
+        _, _, c = dyn_pair((32, 32, 32), **kw)
         with c:
             c.restore(blob)
             c.advance(3)

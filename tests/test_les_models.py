@@ -182,13 +182,13 @@ def test_dynamic_coefficient_invariances():
     g = Grid(16, 16, 16)
     d2, td2 = 0.09, 0.36
     uh, S, Sr, ur = _dyn_inputs(g, 5)
-    t1, c1 = lm.dynamic_stress(g, uh, S, Sr, ur, d2, td2, 0.0)
+    t1, c1, _ = lm.dynamic_stress(g, uh, S, Sr, ur, d2, td2, 0.0)
     uh2, S2, Sr2, ur2 = _dyn_inputs(g, 5, amp=3.0)
-    t2, c2 = lm.dynamic_stress(g, uh2, S2, Sr2, ur2, d2, td2, 0.0)
+    t2, c2, _ = lm.dynamic_stress(g, uh2, S2, Sr2, ur2, d2, td2, 0.0)
     assert np.allclose(c2, c1, atol=1e-12) and np.allclose(t2, 9 * t1, rtol=1e-10, atol=1e-12 * np.abs(t2).max())
     uh3, S3, Sr3, ur3 = _dyn_inputs(g, 5, shift=(0.7, -0.4, 1.1))
     assert np.allclose(ur3, ur + np.array([0.7, -0.4, 1.1])[:, None, None, None], atol=1e-13)
-    t3, c3 = lm.dynamic_stress(g, uh3, S3, Sr3, ur3, d2, td2, 0.0)
+    t3, c3, _ = lm.dynamic_stress(g, uh3, S3, Sr3, ur3, d2, td2, 0.0)
     assert np.allclose(c3, c1, atol=1e-11) and np.allclose(t3, t1, atol=1e-11 * np.abs(t1).max())
     assert np.abs(c1).max() <= 1.0
 
@@ -199,7 +199,7 @@ def test_dynamic_stress_limits():
     g = Grid(16, 16, 16)
     d2, td2 = 0.09, 0.36
     uh, S, Sr, ur = _dyn_inputs(g, 6)
-    t, _ = lm.dynamic_stress(g, uh, S, Sr, ur, d2, td2, 0.4)
+    t, _, _ = lm.dynamic_stress(g, uh, S, Sr, ur, d2, td2, 0.4)
     smag = lm.stress("Smagorinsky", Sr, None, np.sqrt(0.4), d2)
     assert np.allclose(t, smag, rtol=1e-12, atol=1e-14 * np.abs(smag).max())
 
@@ -226,10 +226,50 @@ def test_piomelli_coefficient_invariances():
     d2, td2 = 0.09, 0.36
     cstar = 0.02 + 0.01 * np.cos(g.coords()[0])
     uh, S, Sr, ur = _dyn_inputs(g, 7)
-    t1, c1 = lm.dynamic_stress(g, uh, S, Sr, ur, d2, td2, -1.0, cstar=cstar)
+    t1, c1, _ = lm.dynamic_stress(g, uh, S, Sr, ur, d2, td2, -1.0, cstar=cstar)
     uh2, S2, Sr2, ur2 = _dyn_inputs(g, 7, amp=2.0)
-    t2, c2 = lm.dynamic_stress(g, uh2, S2, Sr2, ur2, d2, td2, -1.0, cstar=cstar)
+    t2, c2, _ = lm.dynamic_stress(g, uh2, S2, Sr2, ur2, d2, td2, -1.0, cstar=cstar)
     assert np.allclose(c2, c1, atol=1e-12) and np.allclose(t2, 4 * t1, atol=1e-10 * np.abs(t2).max())
     uh3, S3, Sr3, ur3 = _dyn_inputs(g, 7, shift=(0.3, 0.9, -0.5))
-    t3, c3 = lm.dynamic_stress(g, uh3, S3, Sr3, ur3, d2, td2, -1.0, cstar=cstar)
+    t3, c3, _ = lm.dynamic_stress(g, uh3, S3, Sr3, ur3, d2, td2, -1.0, cstar=cstar)
     assert np.allclose(c3, c1, atol=1e-11)
+
+
+@pytest.mark.parametrize("pmode,smodel", [("dyn", None), ("dyn", "pio"), ("pio", None), ("pio", "pio")])
+def test_dynamic_sandp_cp_invariances(pmode, smodel):
+    """dynamicSandP / piomelliSandP: cp is scale invariant (L, P, Ph all scale as lambda^2) and Galilean
+    invariant (the vorticity and the Leonard stress ignore a constant velocity), and the P part of t is
+    clamp(cp, -0.8, 0.8) Delta^2 (S W - W S) with W the rotation tensor."""
+    g = Grid(16, 16, 16)
+    d2, td2 = 0.09, 0.36
+    x = g.coords()[0]
+    cstar = 0.02 + 0.01 * np.cos(x) if smodel == "pio" else None
+    cpstar = 0.09 + 0.02 * np.sin(x) if pmode == "pio" else None
+
+    def run(amp=1.0, shift=(0.0, 0.0, 0.0)):
+        uh, S, Sr, ur = _dyn_inputs(g, 11, shift=shift, amp=amp)
+        kv = (g.kx, g.ky, g.kz)
+        w = np.array([1j * (kv[1] * uh[2] - kv[2] * uh[1]), 1j * (kv[2] * uh[0] - kv[0] * uh[2]),
+                      1j * (kv[0] * uh[1] - kv[1] * uh[0])])
+        w = np.where(g.RANGE, w, 0)
+        wr = np.array([g.irfft(c) for c in w])
+        return lm.dynamic_stress(g, uh, S, Sr, ur, d2, td2, -1.0, cstar=cstar, wh=w, wr=wr, pmode=pmode,
+                                 cpstar=cpstar), Sr, wr
+
+    (t1, c1, cp1), Sr, wr = run()
+    (t2, c2, cp2), _, _ = run(amp=1.5)
+    (t3, c3, cp3), _, _ = run(shift=(0.4, -0.2, 0.9))
+    assert np.allclose(cp2, cp1, atol=1e-11) and np.allclose(cp3, cp1, atol=1e-11)
+    assert np.allclose(c2, c1, atol=1e-11) and np.allclose(c3, c1, atol=1e-11)
+    assert np.allclose(t2, 2.25 * t1, atol=1e-10 * np.abs(t2).max())
+    # the P part from the rotation tensor W = (grad u - grad u^T)/2 = AntiSymTen(w/2)
+    eddy = np.array([2 * np.maximum(-1.0, np.minimum(0.4, c1)) * d2 * np.sqrt(
+        2 * lm.ddot(lm.sym_full(Sr), lm.sym_full(Sr))) * s for s in Sr])
+    Sm = np.moveaxis(np.array(lm.sym_full(Sr)), (0, 1), (-2, -1))
+    Wm = np.zeros_like(Sm)
+    Wm[..., 0, 1], Wm[..., 0, 2], Wm[..., 1, 2] = -0.5 * wr[2], 0.5 * wr[1], -0.5 * wr[0]
+    Wm[..., 1, 0], Wm[..., 2, 0], Wm[..., 2, 1] = 0.5 * wr[2], -0.5 * wr[1], 0.5 * wr[0]
+    P = Sm @ Wm - Wm @ Sm
+    f = np.clip(cp1, -0.8, 0.8) * d2
+    want = np.array([P[..., 0, 0], P[..., 0, 1], P[..., 0, 2], P[..., 1, 1], P[..., 1, 2]]) * f
+    assert np.allclose(t1 - eddy, want, atol=1e-12 * np.abs(t1).max())
