@@ -189,6 +189,7 @@ struct UpdateArgs {
   double gz;           // g_z = zAcceleration / referenceDensity
   double* time;        // s.time[] (DevState::time): advanced by dt once per step, or null
   HvArgs hv;
+  HvArgs hvd;          // the density's own hyperviscosity (its ETD3 operator)
   // passive scalar (PassiveScalar, simulationtypes.jl:198-224; models.jl:353-357, 470)
   void* phi;           // updated in place, or null
   void* prhs;          // its rhs out
@@ -275,6 +276,7 @@ struct StatsAux {
   const void* rho;  // density or null
   double gz;        // g_z for the buoyancy term of the pressure statistics
   HvArgs hv;        // hyperviscous dissipation sums (slots 30, 31)
+  HvArgs hvd;       // the density's (slot 37)
   const void* phi;  // passive scalar or null (slots 32-36)
 };
 template <typename T>

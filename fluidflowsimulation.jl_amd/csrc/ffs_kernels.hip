@@ -1360,8 +1360,8 @@ __device__ __forceinline__ void update_point(const Geo& g, C<T>* __restrict__ u,
         else if (a.step == 2) U[c][p] = fma(e.E, U[c][p], fma(e.A, R[c][p], e.B * M1[c][p]));
         else U[c][p] = fma(e.A, R[c][p], fma(e.B, M1[c][p], fma(e.Cc, M2[c][p], e.E * U[c][p])));
       }
-    if (FULL && a.rho) {
-      const EtdCoef d = etd_coef(etd_c(a.hv, K2, kv3, -a.alpha), a.step, st);
+    if (FULL && a.rho) {  // the density's own hyperviscosity (density_stratification_types.jl:168-174)
+      const EtdCoef d = etd_coef(etd_c(a.hvd, K2, kv3, -a.alpha), a.step, st);
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
         if (a.step == 1) RH[p] = fma(d.E, RH[p], d.A * RR[p]);
@@ -1928,7 +1928,8 @@ void launch_project(const Geo& g, void* u, hipStream_t s) {
 //  density (scalar_stats :221-248, buoyancy_flux density_stratification_types.jl:192-211):
 //  22 sum w|rho|^2  23..25 sum w k_i^2 |rho|^2  26 sum w proj(uz, rho)  29 Re rho(k=0)
 //  LES (les_stats :679-705): 27 sum w proj((ik.tau)_{x,y}, u_{x,y})  28 the same for z
-enum { kStatsN = 37, kStatsBlocks = 512, kStatsThreads = 128 };
+//  the density's hyperviscous dissipation (scalar_hvis_stats :1086-1170): 37 sum w f_hvd |rho|^2
+enum { kStatsN = 38, kStatsBlocks = 512, kStatsThreads = 128 };
 static_assert(kStatsN <= kStatsSlots, "a stats row of partial sums fits its slot");
 
 __device__ __forceinline__ double proj2(double ax, double ay, double bx, double by) { return ax * bx + ay * by; }
@@ -2018,6 +2019,10 @@ __global__ void __launch_bounds__(kStatsThreads) k_stats(Geo g, const C<T>* __re
       acc[25] += w * kz * kz * er;
       acc[26] += w * proj2(uz[0], uz[1], rr[0], rr[1]);
       if (idx == 0) acc[29] += rr[0];
+      if (aux.hvd.kind) {
+        const double kv[3] = {kx, ky, kz};
+        acc[37] += w * hv_weight(aux.hvd, kv) * er;
+      }
     }
     if (nl) {
       const C<T> n0 = nl[idx], n1 = nl[n + idx], n2 = nl[2 * n + idx];

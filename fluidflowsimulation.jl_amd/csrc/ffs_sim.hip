@@ -677,6 +677,11 @@ struct ffs_sim {
     if (p.forcing == FFS_FORCING_QF && !(p.alvelius_c != 0))
       throw FfsError(FFS_ERR_ARG, "QfForcing needs alveliusC != 0");
     if (p.precision != FFS_FP64 && p.precision != FFS_FP32) throw FfsError(FFS_ERR_ARG, "unknown precision");
+    if (p.density_hv < FFS_HV_NONE || p.density_hv > FFS_HV_BARRIER)
+      throw FfsError(FFS_ERR_ARG, "unknown densityHyperViscosity");
+    if (p.density_hv == FFS_HV_HYPER && p.density_hv_m < 1)
+      throw FfsError(FFS_ERR_ARG, "densityHyperViscosityM must be >= 1");
+    if (!p.density) p.density_hv = FFS_HV_NONE;  // no density, nothing to act on
     if (p.hyperviscosity < FFS_HV_NONE || p.hyperviscosity > FFS_HV_BARRIER)
       throw FfsError(FFS_ERR_ARG, "unknown hyperviscosity");
     if (p.hyperviscosity == FFS_HV_HYPER && p.hv_m < 1) throw FfsError(FFS_ERR_ARG, "hyperViscosityM must be >= 1");
@@ -1152,7 +1157,8 @@ struct ffs_sim {
         }
     }
     auto& sl = slots[slot];
-    StatsAux aux{les ? tau : nullptr, dens ? rho : nullptr, gz, hv_args(), pscal ? phi : nullptr};
+    StatsAux aux{les ? tau : nullptr, dens ? rho : nullptr, gz, hv_args(), dens ? hvd_args() : HvArgs{},
+                 pscal ? phi : nullptr};
     launch("stats", [&] { FFS_DISPATCH(launch_stats, g, u, rhs, ke, aux, partials, red, stream); });
     HIPCHECK(hipMemcpyAsync(sl.red, red, kStatsSlots * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipMemcpyAsync(sl.st, st, sizeof(DevState), hipMemcpyDeviceToHost, stream));
@@ -1200,7 +1206,14 @@ struct ffs_sim {
     }
     if (dens) {  // stats(::AbstractDensityStratification), density_stratification_types.jl:46-87
       const double rhodiss = alpha * (a[23] + a[24] + a[25]);
-      other.insert(other.end(), {a[29], a[22], a[23], a[24], a[25], rhodiss, rhodiss, -gz * a[26]});
+      double trhodiss = rhodiss;
+      double rhohdiss = 0.0;
+      if (p.density_hv) {  // scalar_hvis_stats (stats.jl:1086-1170): + rhohdiss
+        rhohdiss = (p.density_hv == FFS_HV_HYPER ? p.density_hv_nu : -1.0) * a[37];
+        trhodiss += rhohdiss;
+      }
+      other.insert(other.end(), {a[29], a[22], a[23], a[24], a[25], rhodiss, trhodiss, -gz * a[26]});
+      if (p.density_hv) other.push_back(rhohdiss);
     }
     if (les) {  // les_stats, stats.jl:679-705
       const double prh = -a[27], prv = -a[28];
@@ -1253,20 +1266,26 @@ struct ffs_sim {
   }
 
   // SpectralBarrier defaults (simulationtypes.jl:294-296): initk = cutoff 2 KX[end] / 3, endk = sqrt(3 cutoff) KX[end]
-  HvArgs hv_args() const {
+  HvArgs hv_args() const { return make_hv(p.hyperviscosity, p.hv_m, p.hv_nu, p.sb_initk, p.sb_endk); }
+  // the density's own (densityHyperViscosity, density_stratification_types.jl:137-149; same defaults)
+  HvArgs hvd_args() const {
+    return make_hv(p.density_hv, p.density_hv_m, p.density_hv_nu, p.density_sb_initk, p.density_sb_endk);
+  }
+  HvArgs make_hv(int kind, int m, double nuh, double initk, double endk) const {
     HvArgs h{};
-    h.kind = p.hyperviscosity;
-    h.m = p.hv_m;
-    h.nuh = p.hv_nu;
+    h.kind = kind;
+    h.m = m;
+    h.nuh = nuh;
     const double kxe = KX.empty() ? 0.0 : KX.back();
-    h.sb_i = p.sb_initk > 0 ? p.sb_initk : p.cutoff * 2 * kxe / 3;
-    h.sb_p = p.sb_endk > 0 ? p.sb_endk : std::sqrt(p.cutoff * 3) * kxe;
+    h.sb_i = initk > 0 ? initk : p.cutoff * 2 * kxe / 3;
+    h.sb_p = endk > 0 ? endk : std::sqrt(p.cutoff * 3) * kxe;
     return h;
   }
 
   UpdateArgs update_args(int step) const {
     UpdateArgs a{};
     a.hv = hv_args();
+    a.hvd = hvd_args();
     a.integrator = etd ? 1 : 0;
     a.step = step;
     a.nu = p.nu;
@@ -1765,6 +1784,8 @@ int ffs_params_default(ffs_params* p) {
   p->scalar_gradient = 0.0;
   p->scalar_direction = 2;
   p->equation = FFS_EQ_ROTATIONAL;
+  p->density_hv = FFS_HV_NONE;
+  p->density_hv_m = 2;
   return FFS_OK;
 }
 
@@ -2114,6 +2135,7 @@ int ffs_stats_header(const ffs_sim* s, int table, char* buf, size_t cap) {
     h = "iteration,time,u1,u2,u3,u1p2,u2p2,u3p2,kh,kv,kaniso,k,dissh,dissv,diss,nlh,nlv,pressh,pressv,tnlh,tnlv,tdiss";
     if (s->pscal) h += ",scalar,scalarp2,dscalardxp2,dscalardyp2,dscalardzp2,scalardiss";
     if (s->dens) h += ",rho,rhop2,drhodxp2,drhodyp2,drhodzp2,rhodiss,trhodiss,bflux";
+    if (s->dens && s->p.density_hv) h += ",rhohdiss";
     if (s->les) h += ",prh,prv,praniso,pr";
     if (s->plane) h += s->p.forcing == FFS_FORCING_A ? ",hp,vp" : ",hp,vp,tp";
     if (s->p.hyperviscosity) h += ",hdissh,hdissv,hdiss";

@@ -170,6 +170,11 @@ class Params:
     alvelius_c: float = 1.0
     alvelius_kf: float = 0.0
     hyperviscosity: str = None      # hyperViscosity: "spectralBarrier", or any other value: HyperViscosity
+    density_hyperviscosity: str = None  # densityHyperViscosity: the density's own, as hyperviscosity
+    density_hv_nu: float = 0.0      # densityHyperViscosityCoefficient
+    density_hv_m: int = 2           # densityHyperViscosityM
+    density_sb_initk: float = 0.0   # densityInitk / densityEndk (0: the defaults)
+    density_sb_endk: float = 0.0
     hv_nu: float = 0.0              # hyperViscosityCoefficient
     hv_m: int = 2                   # hyperViscosityM
     sb_initk: float = 0.0           # initk / endk of the spectral barrier (0: the reference's defaults)
@@ -220,14 +225,15 @@ class Params:
         else:
             p.forcing_zf_tanh = int(self.forcing_zf != "cutoff")
         p.alvelius_p, p.alvelius_c, p.alvelius_kf = self.alvelius_p, self.alvelius_c, self.alvelius_kf
-        if self.hyperviscosity is None:
-            p.hyperviscosity = _lib.FFS_HV_NONE
-        elif self.hyperviscosity == "spectralBarrier":
-            p.hyperviscosity = _lib.FFS_HV_BARRIER
-        else:
-            p.hyperviscosity = _lib.FFS_HV_HYPER
+        def hv_code(v):
+            return _lib.FFS_HV_NONE if v is None else (_lib.FFS_HV_BARRIER if v == "spectralBarrier"
+                                                       else _lib.FFS_HV_HYPER)
+        p.hyperviscosity = hv_code(self.hyperviscosity)
         p.hv_nu, p.hv_m = self.hv_nu, self.hv_m
         p.sb_initk, p.sb_endk = self.sb_initk, self.sb_endk
+        p.density_hv = hv_code(self.density_hyperviscosity)
+        p.density_hv_nu, p.density_hv_m = self.density_hv_nu, self.density_hv_m
+        p.density_sb_initk, p.density_sb_endk = self.density_sb_initk, self.density_sb_endk
         p.passive_scalar = int(bool(self.passive_scalar))
         p.scalar_pr, p.scalar_gradient = self.scalar_pr, self.scalar_gradient
         p.scalar_direction = {"x": 0, "y": 1, "z": 2}[self.scalar_direction]
@@ -312,6 +318,15 @@ def parameters(d):
         p.scalar_direction = opt("scalarDirection", str, "z")
         if p.scalar_direction not in ("x", "y", "z"):
             raise ValueError(f"scalarDirection {p.scalar_direction!r}: x, y or z")
+    if "densityHyperViscosity" in d:  # density_stratification_types.jl:137-149
+        known.add("densityHyperViscosity")
+        p.density_hyperviscosity = d["densityHyperViscosity"]
+        if p.density_hyperviscosity == "spectralBarrier":
+            p.density_sb_initk = opt("densityInitk", float, 0.0)
+            p.density_sb_endk = opt("densityEndk", float, 0.0)
+        else:
+            p.density_hv_nu = float(req("densityHyperViscosityCoefficient"))
+            p.density_hv_m = opt("densityHyperViscosityM", int, 2)
     if "hyperViscosity" in d:  # simulationtypes.jl:293-305
         known.add("hyperViscosity")
         p.hyperviscosity = d["hyperViscosity"]

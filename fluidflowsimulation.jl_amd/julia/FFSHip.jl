@@ -67,6 +67,11 @@ mutable struct ffs_params
     les_s_model::Int32
     les_test_width::Float64
     les_cmin::Float64
+    density_hv::Int32
+    density_hv_nu::Float64
+    density_hv_m::Int32
+    density_sb_initk::Float64
+    density_sb_endk::Float64
     ffs_params() = new()
 end
 
@@ -149,6 +154,18 @@ function params_from_global(d::AbstractDict)
             p.hyperviscosity = Int32(1)
             p.hv_nu = parse(Float64, d[:hyperViscosityCoefficient])
             haskey(d, :hyperViscosityM) && (p.hv_m = parse(Int32, d[:hyperViscosityM]))
+        end
+    end
+    # the density's own hyperviscosity (density_stratification_types.jl:137-149)
+    if haskey(d, :densityHyperViscosity)
+        if d[:densityHyperViscosity] == "spectralBarrier"
+            p.density_hv = Int32(2)
+            haskey(d, :densityInitk) && (p.density_sb_initk = parse(Float64, d[:densityInitk]))
+            haskey(d, :densityEndk) && (p.density_sb_endk = parse(Float64, d[:densityEndk]))
+        else
+            p.density_hv = Int32(1)
+            p.density_hv_nu = parse(Float64, d[:densityHyperViscosityCoefficient])
+            haskey(d, :densityHyperViscosityM) && (p.density_hv_m = parse(Int32, d[:densityHyperViscosityM]))
         end
     end
     # forcing_model (forcing_types.jl:338-446): the kz = 0-plane schemes

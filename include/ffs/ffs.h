@@ -173,6 +173,15 @@ typedef struct ffs_params {
   double les_test_width;                    /* TestFilterWidth of the dynamic models; <= 0 selects
                                                twice the filter width */
   double les_cmin;                          /* cmin of the dynamic models (0): the coefficient's floor */
+  /* the density's own hyperviscosity (densityHyperViscosity, density_stratification_types.jl:137-149;
+   * the velocity's hyperViscosity does not reach the density, while the passive scalar shares it):
+   * FFS_HV_*, densityHyperViscosityCoefficient, densityHyperViscosityM (2), densityInitk,
+   * densityEndk (<= 0: the defaults of initk / endk).  It acts through the density's ETD3 operator
+   * and adds the rhohdiss stats column. */
+  int32_t density_hv;
+  double density_hv_nu;
+  int32_t density_hv_m;
+  double density_sb_initk, density_sb_endk;
 } ffs_params;
 
 typedef struct ffs_sim ffs_sim;

@@ -163,7 +163,8 @@ class OracleSim:
         oracle.les_models.MODELS, with les_cp (Silvis P constant, SandP pConstant) and les_smodel
         (SandP's sLesModel); dynamicSmagorinsky takes les_test_width (default twice the filter
         width) and les_cmin;
-        density: dict(Pr=, gradient=, g=) for the Boussinesq approximation;
+        density: dict(Pr=, gradient=, g=[, hv=]) for the Boussinesq approximation, hv its own
+        hyperviscosity (densityHyperViscosity, as hv below; none by default);
         hv: dict(kind="hyper", nu=, m=) (HyperViscosity{nu_h, M}) or dict(kind="barrier",
         initk=, endk=) (SpectralBarrier), Hyperviscosity/hyperviscosity_types.jl;
         scalar: dict(Pr=, gradient=, dir=0|1|2) for the passive scalar (PassiveScalar,
@@ -453,7 +454,7 @@ class OracleSim:
             if self.integrator == "AB3":
                 r = self.ab3(self.rhoh, self.rhorhs, "rfm1", "rfm2")
             else:
-                r = self.etd(self.rhoh, self.rhorhs, "rfm1", "rfm2", -self.alpha)
+                r = self.etd(self.rhoh, self.rhorhs, "rfm1", "rfm2", -self.alpha, self.density.get("hv"))
             self.rhoh = np.where(g.RANGE, r, 0)
 
     def ab3(self, u, rhs, n1, n2):
@@ -517,44 +518,49 @@ class OracleSim:
             r = r * x
         return r
 
-    def barrier(self, k):
+    def barrier(self, k, hv=None):
         """SpectralBarrier's f(k) = log(esp(k, initk, endk, 1/4)) (hyperviscosity_types.jl:44-48)."""
-        i, p = self.hv["initk"], self.hv["endk"]
+        hv = self.hv if hv is None else hv
+        i, p = hv["initk"], hv["endk"]
         ak = np.abs(k)
         with np.errstate(all="ignore"):
             e = np.where(ak <= i, 1.0, np.where(ak < p, np.exp(-((ak - i) ** 2) / (p ** 0.25 - ak ** 0.25)), 0.0))
             return np.log(e)
 
-    def etd_c(self, mnu):
+    def etd_c(self, mnu, hv="velocity"):
         """init_c! (ETD.jl:56-95): c = -nu k^2, plus -nu_h k^2M (HyperViscosity) or f(|k|)
-        (SpectralBarrier, -Inf beyond endk)."""
+        (SpectralBarrier, -Inf beyond endk).  hv: the velocity's model (also the passive scalar's,
+        simulationtypes.jl:331) or a field's own (the density's densityHyperViscosity,
+        density_stratification_types.jl:137-149)."""
         g = self.g
-        if self.hv is None:
+        hv = self.hv if hv == "velocity" else hv
+        if hv is None:
             c = g.K2 * mnu
-        elif self.hv["kind"] == "hyper":
+        elif hv["kind"] == "hyper":
             mk = self.modk()
-            c = mk * mnu + self.ipow(mk, self.hv["m"]) * (-self.hv["nu"])
+            c = mk * mnu + self.ipow(mk, hv["m"]) * (-hv["nu"])
         else:
             mk = self.modk()
-            c = mk * mnu + self.barrier(np.sqrt(mk))
+            c = mk * mnu + self.barrier(np.sqrt(mk), hv)
         return np.where(g.RANGE, c, 0.0)
 
-    def hv_weight(self):
+    def hv_weight(self, hv=None):
         """Per-mode factor of the hyperviscous dissipation: nu_h k^2M (hyperviscosity_stats,
         stats.jl:532-561) or -f(|k|) with f = -Inf read as 0 (spectral_barrier_stats,
         hyperviscosity_types.jl:50-79)."""
+        hv = self.hv if hv is None else hv
         mk = self.modk()
-        if self.hv["kind"] == "hyper":
-            return self.hv["nu"] * self.ipow(mk, self.hv["m"])
-        f = self.barrier(np.sqrt(mk))
+        if hv["kind"] == "hyper":
+            return hv["nu"] * self.ipow(mk, hv["m"])
+        f = self.barrier(np.sqrt(mk), hv)
         return -np.where(f == -np.inf, 0.0, f)
 
-    def etd(self, u, rhs, n1, n2, mnu):
+    def etd(self, u, rhs, n1, n2, mnu, hv="velocity"):
         """ETD3rdO functor (ETD.jl:229-247) with ETD1stO!/ETD2ndO! (ETD1stand2ndO.jl)
         for the first two steps; c from init_c! (ETD.jl:56-95); c = -Inf gives zero
         coefficients (the test2 branches)."""
         g = self.g
-        c = self.etd_c(mnu)
+        c = self.etd_c(mnu, hv)
         inf = c == -np.inf
         dt = self.dt
         fm1, fm2 = getattr(self, n1), getattr(self, n2)
@@ -712,7 +718,14 @@ class OracleSim:
             drd = [float(np.sum(W * kk * kk * er)) for kk in (g.kx, g.ky, g.kz)]
             rhodiss = self.alpha * (drd[0] + drd[1] + drd[2])
             bf = -self.gz * float(np.sum(W * proj(u[2], r)))  # buoyancy_flux (:192-211)
-            other += [rho, rho2, *drd, rhodiss, rhodiss, bf]
+            trhodiss = rhodiss
+            extra = []
+            if self.density.get("hv") is not None:  # scalar_hvis_stats (stats.jl:1086-1170): rhohdiss
+                wgt = np.where(g.RANGE, g.W * self.hv_weight(self.density["hv"]), 0.0)
+                rhohdiss = float(np.sum(wgt * np.abs(self.rhoh) ** 2))
+                trhodiss += rhohdiss
+                extra = [rhohdiss]
+            other += [rho, rho2, *drd, rhodiss, trhodiss, bf, *extra]
         if self.les_c is not None:
             # les_stats (stats.jl:679-705)
             d = self.div_tau()

@@ -152,3 +152,20 @@ def test_les_model_keys():
     assert (ds.les_s_model, ds.les_cmin) == ("dynamicSmagorinsky", 0.02)
     assert ds.to_c().les_model == _lib.FFS_LES_PIOMELLI_SANDP and ds.to_c().les_s_model == _lib.FFS_LES_DYNAMIC_SMAGORINSKY
     assert config.parameters(dict(base, lesModel="fancyModel")).to_c().les_model == -1  # refused at create
+
+
+def test_density_hyperviscosity_keys():
+    """densityHyperViscosity (density_stratification_types.jl:137-149): its own coefficient and M, or
+    the spectral barrier with densityInitk / densityEndk."""
+    from ffs import _lib
+    base = {"nx": "32", "ny": "32", "nz": "32", "xDomainSize": "1.0", "yDomainSize": "1.0", "zDomainSize": "1.0",
+            "kinematicViscosity": "1e-3", "dtStat": "10", "dtSpec": "100", "writeTime": "100"}
+    h = config.parameters(dict(base, densityHyperViscosity="hyperViscosity", densityHyperViscosityCoefficient="1e-8",
+                               densityHyperViscosityM="3"))
+    c = h.to_c()
+    assert (c.density_hv, c.density_hv_nu, c.density_hv_m, c.hyperviscosity) == (_lib.FFS_HV_HYPER, 1e-8, 3,
+                                                                                 _lib.FFS_HV_NONE)
+    b = config.parameters(dict(base, densityHyperViscosity="spectralBarrier", densityInitk="5", densityEndk="9"))
+    assert (b.to_c().density_hv, b.density_sb_initk, b.density_sb_endk) == (_lib.FFS_HV_BARRIER, 5.0, 9.0)
+    with pytest.raises(KeyError):
+        config.parameters(dict(base, densityHyperViscosity="hyperViscosity"))
