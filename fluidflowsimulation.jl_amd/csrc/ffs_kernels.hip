@@ -1200,7 +1200,7 @@ __device__ __forceinline__ EtdCoef etd_coef(double c, int step, const DevState* 
 template <typename T, bool FULL = true>
 __device__ __forceinline__ void form_rhs(const UpdateArgs& a, size_t idx, size_t n, const double kv3[3], double K2,
                                          bool origin, const double U[3][2], double R[3][2], double RH[2]) {
-  if (a.integrator == 0) {  // explicit viscosity (is_implicit false): (K2 mnu + K2^M mnuh) v
+  if (a.integrator != 1) {  // explicit viscosity (is_implicit false: AB3, Euler): (K2 mnu + K2^M mnuh) v
     double kvis;
     {
 #pragma clang fp contract(off)
@@ -1305,77 +1305,90 @@ __device__ __forceinline__ void update_point(const Geo& g, C<T>* __restrict__ u,
     ((C<T>*)a.prhs)[idx] = mk<T>((T)PR[0], (T)PR[1]);
   }
   if (a.step == 0) return;  // initialisation: final rhs only (initialize!, FluidFlowSimulation.jl:82)
-
-  double M1[3][2], M2[3][2];
+  if (a.integrator == 2) {  // Euller (Euller.jl:14-25): u = muladd(dt, rhs, u), no history
+    const double dt = st->dt;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const C<T> h1 = fm1[c * n + idx], h2 = fm2[c * n + idx];
-    M1[c][0] = h1.x; M1[c][1] = h1.y;
-    M2[c][0] = h2.x; M2[c][1] = h2.y;
-  }
-  double HR1[2] = {0, 0}, HR2[2] = {0, 0};
-  if (FULL && a.rho) {
-    const C<T> h1 = ((const C<T>*)a.rfm1)[idx], h2 = ((const C<T>*)a.rfm2)[idx];
-    HR1[0] = h1.x; HR1[1] = h1.y;
-    HR2[0] = h2.x; HR2[1] = h2.y;
-  }
-  double HP1[2] = {0, 0}, HP2[2] = {0, 0};
-  if (FULL && a.phi) {
-    const C<T> h1 = ((const C<T>*)a.pfm1)[idx], h2 = ((const C<T>*)a.pfm2)[idx];
-    HP1[0] = h1.x; HP1[1] = h1.y;
-    HP2[0] = h2.x; HP2[1] = h2.y;
-  }
-  if (a.integrator == 0) {
-    const double At = st->At, Bt = st->Bt, Ct = st->Ct;
-    if (a.step == 2) {  // fix_fm2!: fm2 = -(dt3+dt2)(rhs - fm1)/dt2 + rhs
-      const double dt2 = st->dt2, dt3 = st->dt3;
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) U[c][p] = fma(dt, R[c][p], U[c][p]);
+    if constexpr (FULL) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p) RH[p] = fma(dt, RR[p], RH[p]);
+#pragma unroll
+      for (int p = 0; p < 2; ++p) PH[p] = fma(dt, PR[p], PH[p]);
+    }
+  } else {
+    double M1[3][2], M2[3][2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const C<T> h1 = fm1[c * n + idx], h2 = fm2[c * n + idx];
+      M1[c][0] = h1.x; M1[c][1] = h1.y;
+      M2[c][0] = h2.x; M2[c][1] = h2.y;
+    }
+    double HR1[2] = {0, 0}, HR2[2] = {0, 0};
+    if (FULL && a.rho) {
+      const C<T> h1 = ((const C<T>*)a.rfm1)[idx], h2 = ((const C<T>*)a.rfm2)[idx];
+      HR1[0] = h1.x; HR1[1] = h1.y;
+      HR2[0] = h2.x; HR2[1] = h2.y;
+    }
+    double HP1[2] = {0, 0}, HP2[2] = {0, 0};
+    if (FULL && a.phi) {
+      const C<T> h1 = ((const C<T>*)a.pfm1)[idx], h2 = ((const C<T>*)a.pfm2)[idx];
+      HP1[0] = h1.x; HP1[1] = h1.y;
+      HP2[0] = h2.x; HP2[1] = h2.y;
+    }
+    if (a.integrator == 0) {
+      const double At = st->At, Bt = st->Bt, Ct = st->Ct;
+      if (a.step == 2) {  // fix_fm2!: fm2 = -(dt3+dt2)(rhs - fm1)/dt2 + rhs
+        const double dt2 = st->dt2, dt3 = st->dt3;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int p = 0; p < 2; ++p) M2[c][p] = -(dt3 + dt2) * (R[c][p] - M1[c][p]) / dt2 + R[c][p];
+        if constexpr (FULL) {
+#pragma unroll
+          for (int p = 0; p < 2; ++p) HR2[p] = -(dt3 + dt2) * (RR[p] - HR1[p]) / dt2 + RR[p];
+#pragma unroll
+          for (int p = 0; p < 2; ++p) HP2[p] = -(dt3 + dt2) * (PR[p] - HP1[p]) / dt2 + PR[p];
+        }
+      }
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int p = 0; p < 2; ++p) M2[c][p] = -(dt3 + dt2) * (R[c][p] - M1[c][p]) / dt2 + R[c][p];
+        for (int p = 0; p < 2; ++p) U[c][p] = fma(At, R[c][p], fma(Bt, M1[c][p], fma(Ct, M2[c][p], U[c][p])));
       if constexpr (FULL) {
 #pragma unroll
-        for (int p = 0; p < 2; ++p) HR2[p] = -(dt3 + dt2) * (RR[p] - HR1[p]) / dt2 + RR[p];
+        for (int p = 0; p < 2; ++p) RH[p] = fma(At, RR[p], fma(Bt, HR1[p], fma(Ct, HR2[p], RH[p])));
 #pragma unroll
-        for (int p = 0; p < 2; ++p) HP2[p] = -(dt3 + dt2) * (PR[p] - HP1[p]) / dt2 + PR[p];
+        for (int p = 0; p < 2; ++p) PH[p] = fma(At, PR[p], fma(Bt, HP1[p], fma(Ct, HP2[p], PH[p])));
       }
-    }
+    } else {
+      const EtdCoef e = etd_coef(FULL ? etd_c(a.hv, K2, kv3, -a.nu) : K2 * (-a.nu), a.step, st);
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+      for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int p = 0; p < 2; ++p) U[c][p] = fma(At, R[c][p], fma(Bt, M1[c][p], fma(Ct, M2[c][p], U[c][p])));
-    if constexpr (FULL) {
+        for (int p = 0; p < 2; ++p) {
+          if (a.step == 1) U[c][p] = fma(e.E, U[c][p], e.A * R[c][p]);
+          else if (a.step == 2) U[c][p] = fma(e.E, U[c][p], fma(e.A, R[c][p], e.B * M1[c][p]));
+          else U[c][p] = fma(e.A, R[c][p], fma(e.B, M1[c][p], fma(e.Cc, M2[c][p], e.E * U[c][p])));
+        }
+      if (FULL && a.rho) {  // the density's own hyperviscosity (density_stratification_types.jl:168-174)
+        const EtdCoef d = etd_coef(etd_c(a.hvd, K2, kv3, -a.alpha), a.step, st);
 #pragma unroll
-      for (int p = 0; p < 2; ++p) RH[p] = fma(At, RR[p], fma(Bt, HR1[p], fma(Ct, HR2[p], RH[p])));
-#pragma unroll
-      for (int p = 0; p < 2; ++p) PH[p] = fma(At, PR[p], fma(Bt, HP1[p], fma(Ct, HP2[p], PH[p])));
-    }
-  } else {
-    const EtdCoef e = etd_coef(FULL ? etd_c(a.hv, K2, kv3, -a.nu) : K2 * (-a.nu), a.step, st);
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        if (a.step == 1) U[c][p] = fma(e.E, U[c][p], e.A * R[c][p]);
-        else if (a.step == 2) U[c][p] = fma(e.E, U[c][p], fma(e.A, R[c][p], e.B * M1[c][p]));
-        else U[c][p] = fma(e.A, R[c][p], fma(e.B, M1[c][p], fma(e.Cc, M2[c][p], e.E * U[c][p])));
+        for (int p = 0; p < 2; ++p) {
+          if (a.step == 1) RH[p] = fma(d.E, RH[p], d.A * RR[p]);
+          else if (a.step == 2) RH[p] = fma(d.E, RH[p], fma(d.A, RR[p], d.B * HR1[p]));
+          else RH[p] = fma(d.A, RR[p], fma(d.B, HR1[p], fma(d.Cc, HR2[p], d.E * RH[p])));
+        }
       }
-    if (FULL && a.rho) {  // the density's own hyperviscosity (density_stratification_types.jl:168-174)
-      const EtdCoef d = etd_coef(etd_c(a.hvd, K2, kv3, -a.alpha), a.step, st);
+      if (FULL && a.phi) {
+        const EtdCoef d = etd_coef(etd_c(a.hv, K2, kv3, -a.palpha), a.step, st);
 #pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        if (a.step == 1) RH[p] = fma(d.E, RH[p], d.A * RR[p]);
-        else if (a.step == 2) RH[p] = fma(d.E, RH[p], fma(d.A, RR[p], d.B * HR1[p]));
-        else RH[p] = fma(d.A, RR[p], fma(d.B, HR1[p], fma(d.Cc, HR2[p], d.E * RH[p])));
-      }
-    }
-    if (FULL && a.phi) {
-      const EtdCoef d = etd_coef(etd_c(a.hv, K2, kv3, -a.palpha), a.step, st);
-#pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        if (a.step == 1) PH[p] = fma(d.E, PH[p], d.A * PR[p]);
-        else if (a.step == 2) PH[p] = fma(d.E, PH[p], fma(d.A, PR[p], d.B * HP1[p]));
-        else PH[p] = fma(d.A, PR[p], fma(d.B, HP1[p], fma(d.Cc, HP2[p], d.E * PH[p])));
+        for (int p = 0; p < 2; ++p) {
+          if (a.step == 1) PH[p] = fma(d.E, PH[p], d.A * PR[p]);
+          else if (a.step == 2) PH[p] = fma(d.E, PH[p], fma(d.A, PR[p], d.B * HP1[p]));
+          else PH[p] = fma(d.A, PR[p], fma(d.B, HP1[p], fma(d.Cc, HP2[p], d.E * PH[p])));
+        }
       }
     }
   }

@@ -658,7 +658,8 @@ struct ffs_sim {
   void create(const ffs_params& in) {
     p = in;
     if (p.size != sizeof(ffs_params)) throw FfsError(FFS_ERR_ARG, "ffs_params.size mismatch (ABI version)");
-    if (p.integrator != FFS_AB3 && p.integrator != FFS_ETD3) throw FfsError(FFS_ERR_ARG, "unknown integrator");
+    if (p.integrator != FFS_AB3 && p.integrator != FFS_ETD3 && p.integrator != FFS_EULER)
+      throw FfsError(FFS_ERR_ARG, "unknown integrator");
     if (p.les_model < FFS_LES_NONE || p.les_model > FFS_LES_PIOMELLI_SANDP)
       throw FfsError(FFS_ERR_ARG, "unknown LES model");
     if (p.les_model == FFS_LES_FAKE_SMAGORINSKY)
@@ -1030,7 +1031,7 @@ struct ffs_sim {
     // it matters only through it <= 3 (cfl ramp) and tit through tit == 0 (AB3 halving)
     a.it = (int)std::min<int64_t>(iteration, 4);
     a.tit = (int)std::min<int64_t>(tit, 1);
-    a.etd = etd ? 1 : 0;
+    a.etd = (etd || p.integrator == FFS_EULER) ? 1 : 0;  // no first-step halving (Euller set_dt!, Euller.jl:12)
     a.cfl = p.cfl;
     a.dx = 2 * M_PI * p.y_domain / p.ny;
     a.gnorm = dens ? std::fabs(gz) : 0.0;
@@ -1286,7 +1287,7 @@ struct ffs_sim {
     UpdateArgs a{};
     a.hv = hv_args();
     a.hvd = hvd_args();
-    a.integrator = etd ? 1 : 0;
+    a.integrator = etd ? 1 : (p.integrator == FFS_EULER ? 2 : 0);
     a.step = step;
     a.nu = p.nu;
     a.forcing = (p.forcing == FFS_FORCING_SHELL && nband > 0) ? 1 : 0;

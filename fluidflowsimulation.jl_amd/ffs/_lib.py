@@ -19,7 +19,7 @@ ERRORS = {
     -5: "FFS_ERR_HIP",
     -6: "FFS_ERR_STATE",
 }
-FFS_AB3, FFS_ETD3 = 0, 1
+FFS_AB3, FFS_ETD3, FFS_EULER = 0, 1, 2
 FFS_FP64, FFS_FP32 = 0, 1
 FFS_LES_NONE, FFS_LES_SMAGORINSKY = 0, 1
 FFS_LES_VREMAN, FFS_LES_SILVIS, FFS_LES_PRODUCTION, FFS_LES_STABLE_NL = 2, 3, 4, 5

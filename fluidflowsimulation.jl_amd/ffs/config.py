@@ -194,7 +194,9 @@ class Params:
         p.x_domain, p.y_domain, p.z_domain = self.x_domain, self.y_domain, self.z_domain
         p.nu = self.nu
         p.cutoff = self.cutoff
-        p.integrator = _lib.FFS_AB3 if self.integrator == "Adams_Bashforth3rdO" else _lib.FFS_ETD3
+        # velocityTimeStep (simulationtypes.jl:308-318): any value but these two selects ETD3rdO
+        p.integrator = {"Adams_Bashforth3rdO": _lib.FFS_AB3, "Euller": _lib.FFS_EULER}.get(self.integrator,
+                                                                                            _lib.FFS_ETD3)
         p.variable_dt = int(bool(self.variable_dt))
         p.cfl = self.cfl
         p.dt = self.dt

@@ -77,6 +77,7 @@ end
 
 const FFS_AB3 = Int32(0)
 const FFS_ETD3 = Int32(1)
+const FFS_EULER = Int32(2)
 const FFS_SPACE_REAL = Int32(0)
 const FFS_SPACE_FOURIER = Int32(1)
 const FFS_EVENT_OUTPUT = Int32(1)
@@ -106,7 +107,8 @@ function params_from_global(d::AbstractDict)
     haskey(d, :start) && (p.start = parse(Int64, d[:start]))
     haskey(d, :startTime) && (p.start_time = parse(Float64, d[:startTime]))
     haskey(d, :cutoff) && (p.cutoff = num(:cutoff))
-    p.integrator = get(d, :velocityTimeStep, "ETD3rdO") == "Adams_Bashforth3rdO" ? FFS_AB3 : FFS_ETD3
+    vts = get(d, :velocityTimeStep, "ETD3rdO")   # any other value selects ETD3rdO (simulationtypes.jl:308-318)
+    p.integrator = vts == "Adams_Bashforth3rdO" ? FFS_AB3 : (vts == "Euller" ? FFS_EULER : FFS_ETD3)
     haskey(d, :variableTimestepFlag) && (p.variable_dt = parse(Bool, d[:variableTimestepFlag]))
     haskey(d, :cfl) && (p.cfl = parse(Float64, d[:cfl]))
     haskey(d, :dt) && (p.dt = num(:dt))

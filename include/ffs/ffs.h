@@ -39,6 +39,7 @@ extern "C" {
 /* integrators: `velocityTimeStep` key (simulationtypes.jl:308) */
 #define FFS_AB3 0  /* Adams_Bashforth3rdO */
 #define FFS_ETD3 1 /* ETD3rdO (reference default) */
+#define FFS_EULER 2 /* Euller (time_step_functions/Euller.jl): u += dt rhs, explicit viscosity */
 
 /* precision of the bulk arithmetic */
 #define FFS_FP64 0

@@ -157,7 +157,7 @@ class OracleSim:
                  forcing_kf=None, forcing_power=0.1, start=0, start_time=0.0, integrator="AB3", les_c=None,
                  les_delta=None, density=None, plane_forcing=None, hv=None, scalar=None, equation="rotational",
                  les_model="Smagorinsky", les_cp=None, les_smodel=None, les_test_width=None, les_cmin=0.0):
-        """integrator: "AB3" (Adams_Bashforth3rdO) or "ETD3" (ETD3rdO);
+        """integrator: "AB3" (Adams_Bashforth3rdO), "Euler" (Euller) or "ETD3" (ETD3rdO);
         les_c: the LES model's constant (None: no LES), les_delta: filter width
         (default (2 pi L/nx)/cutoff, les_types.jl:366); les_model: Smagorinsky or one of
         oracle.les_models.MODELS, with les_cp (Silvis P constant, SandP pConstant) and les_smodel
@@ -353,8 +353,8 @@ class OracleSim:
         if collect and self.is_stats_time():
             self.nlstats = self.non_linear_stats(rhs)
             self.pressstats = self.pressure_stats(rhs)
-        # fourierspacep2_velocity! (models.jl:292-341)
-        if self.integrator == "AB3":
+        # fourierspacep2_velocity! (models.jl:292-341): explicit viscosity unless the integrator is implicit
+        if self.integrator in ("AB3", "Euler"):
             if self.hv is not None:  # (K2 mnu + K2^M mnuh) v, models.jl:320-322
                 rhs = rhs + (g.K2 * (-self.nu) + self.ipow(g.K2, self.hv["m"]) * (-self.hv["nu"])) * self.uh
             else:
@@ -435,6 +435,8 @@ class OracleSim:
         self.tit += 1
         if self.integrator == "AB3":
             u = self.ab3(self.uh, self.rhs, "fm1", "fm2")
+        elif self.integrator == "Euler":  # Euller functor (Euller.jl:14-25): u = muladd(dt, rhs, u)
+            u = self.dt * self.rhs + self.uh
         else:
             u = self.etd(self.uh, self.rhs, "fm1", "fm2", -self.nu)
         u = np.where(g.RANGE, u, 0)
@@ -447,12 +449,16 @@ class OracleSim:
         if self.scalar is not None:  # the passive scalar's integrator (models.jl:470)
             if self.integrator == "AB3":
                 r = self.ab3(self.phih, self.phirhs, "pfm1", "pfm2")
+            elif self.integrator == "Euler":
+                r = self.dt * self.phirhs + self.phih
             else:
                 r = self.etd(self.phih, self.phirhs, "pfm1", "pfm2", -self.palpha)
             self.phih = np.where(g.RANGE, r, 0)
         if self.density is not None:
             if self.integrator == "AB3":
                 r = self.ab3(self.rhoh, self.rhorhs, "rfm1", "rfm2")
+            elif self.integrator == "Euler":
+                r = self.dt * self.rhorhs + self.rhoh
             else:
                 r = self.etd(self.rhoh, self.rhorhs, "rfm1", "rfm2", -self.alpha, self.density.get("hv"))
             self.rhoh = np.where(g.RANGE, r, 0)

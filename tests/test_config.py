@@ -169,3 +169,14 @@ def test_density_hyperviscosity_keys():
     assert (b.to_c().density_hv, b.density_sb_initk, b.density_sb_endk) == (_lib.FFS_HV_BARRIER, 5.0, 9.0)
     with pytest.raises(KeyError):
         config.parameters(dict(base, densityHyperViscosity="hyperViscosity"))
+
+
+def test_velocity_time_step_values():
+    """velocityTimeStep (simulationtypes.jl:308-318): Euller, Adams_Bashforth3rdO, anything else ETD3rdO."""
+    from ffs import _lib
+    base = {"nx": "16", "ny": "16", "nz": "16", "xDomainSize": "1.0", "yDomainSize": "1.0", "zDomainSize": "1.0",
+            "kinematicViscosity": "1e-3", "dtStat": "10", "dtSpec": "100", "writeTime": "100"}
+    for v, code in (("Euller", _lib.FFS_EULER), ("Adams_Bashforth3rdO", _lib.FFS_AB3), ("ETD3rdO", _lib.FFS_ETD3),
+                    ("RK4", _lib.FFS_ETD3)):
+        assert config.parameters(dict(base, velocityTimeStep=v)).to_c().integrator == code
+    assert config.parameters(base).to_c().integrator == _lib.FFS_ETD3
