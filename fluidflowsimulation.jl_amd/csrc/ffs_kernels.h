@@ -186,7 +186,9 @@ struct UpdateArgs {
   const void* flux;    // F(-rho u), 3 compact components
   double alpha;        // density diffusivity nu / Pr (ETD only, as in the reference)
   double drhodz;       // mean density gradient
-  double gz;           // g_z = zAcceleration / referenceDensity
+  double gz;           // unused by the update (see gval)
+  int gdir;            // gravityDirection: the axis of g and of the density's mean gradient (0 x, 1 y, 2 z)
+  double gval;         // zAcceleration / referenceDensity, the component of g along gdir
   double* time;        // s.time[] (DevState::time): advanced by dt once per step, or null
   HvArgs hv;
   HvArgs hvd;          // the density's own hyperviscosity (its ETD3 operator)
@@ -274,7 +276,9 @@ void launch_project(const Geo& g, void* u, hipStream_t s);
 struct StatsAux {
   const void* tau;  // Smagorinsky stress (5 compact components) or null
   const void* rho;  // density or null
-  double gz;        // g_z for the buoyancy term of the pressure statistics
+  double gz;        // unused (see gval)
+  int gdir;         // the axis of g (buoyancy in the pressure statistics, the buoyancy flux)
+  double gval;      // g along gdir
   HvArgs hv;        // hyperviscous dissipation sums (slots 30, 31)
   HvArgs hvd;       // the density's (slot 37)
   const void* phi;  // passive scalar or null (slots 32-36)
@@ -299,7 +303,9 @@ struct SpecArgs {
   const void* flux;   // F(-rho u) (3 components) or null
   const double* pf_factor;  // plane forcing: per kz = 0 element factor, or null (no forcing_h)
   int pf_kz0;               // plane forcing: the vertical modes are pf_kz0 <= |kz| <= 5
-  double nu, alpha, gz;
+  double nu, alpha, gz;  // gz = g.z (spectrum_buoyancy reads g.z only)
+  int gdir;              // the axis of g for the pressure's buoyancy term
+  double gval;
   const DevState* st;
   HvArgs hv;  // hvis: spectrum_hyperviscosity / spectrum_spectral_barrier
 };

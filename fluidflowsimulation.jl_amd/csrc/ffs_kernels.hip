@@ -1226,12 +1226,12 @@ __device__ __forceinline__ void form_rhs(const UpdateArgs& a, size_t idx, size_t
       R[c][1] += sr;
     }
   }
-  if (FULL && a.rho) {  // buoyancy rhs += rho g, g = (0, 0, g_z)
+  if (FULL && a.rho) {  // buoyancy rhs += rho g, g = g e_gdir (models.jl:334-336)
     const C<T> r0 = ((const C<T>*)a.rho)[idx];
     RH[0] = r0.x;
     RH[1] = r0.y;
-    R[2][0] += RH[0] * a.gz;
-    R[2][1] += RH[1] * a.gz;
+    R[a.gdir][0] += RH[0] * a.gval;
+    R[a.gdir][1] += RH[1] * a.gval;
   }
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
@@ -1284,8 +1284,8 @@ __device__ __forceinline__ void update_point(const Geo& g, C<T>* __restrict__ u,
     const C<T> fx = f[idx], fy = f[n + idx], fz = f[2 * n + idx];
     const double kf_r = fma(kv3[0], (double)fx.x, fma(kv3[1], (double)fy.x, kv3[2] * fz.x));
     const double kf_i = fma(kv3[0], (double)fx.y, fma(kv3[1], (double)fy.y, kv3[2] * fz.y));
-    RR[0] = -kf_i + (-a.drhodz) * U[2][0];
-    RR[1] = kf_r + (-a.drhodz) * U[2][1];
+    RR[0] = -kf_i + (-a.drhodz) * U[a.gdir][0];  // div!(rhs, flux, u_gdir, -drhodz) (models.jl:358-360)
+    RR[1] = kf_r + (-a.drhodz) * U[a.gdir][1];
     ((C<T>*)a.rrhs)[idx] = mk<T>((T)RR[0], (T)RR[1]);
   }
   // passive scalar rhs: div!(rhs, flux, u_dir, -scalarGradient) (models.jl:353-355) from the pre-step u
@@ -2030,7 +2030,8 @@ __global__ void __launch_bounds__(kStatsThreads) k_stats(Geo g, const C<T>* __re
       acc[23] += w * kx * kx * er;
       acc[24] += w * ky * ky * er;
       acc[25] += w * kz * kz * er;
-      acc[26] += w * proj2(uz[0], uz[1], rr[0], rr[1]);
+      const double* ug = aux.gdir == 0 ? ux : (aux.gdir == 1 ? uy : uz);  // buoyancy_flux's u_gdir
+      acc[26] += w * proj2(ug[0], ug[1], rr[0], rr[1]);
       if (idx == 0) acc[29] += rr[0];
       if (aux.hvd.kind) {
         const double kv[3] = {kx, ky, kz};
@@ -2049,9 +2050,11 @@ __global__ void __launch_bounds__(kStatsThreads) k_stats(Geo g, const C<T>* __re
       acc[6] += w * proj2(qz[0], qz[1], uz[0], uz[1]);
       if (idx != 0) {
         // pressure rhs: + (i k).tau (LES) + rho g (density)
-        const double px[2] = {qx[0] + dtau[0][0], qx[1] + dtau[0][1]};
-        const double py[2] = {qy[0] + dtau[1][0], qy[1] + dtau[1][1]};
-        const double pz[2] = {qz[0] + dtau[2][0] + rr[0] * aux.gz, qz[1] + dtau[2][1] + rr[1] * aux.gz};
+        const double gx = aux.gdir == 0 ? aux.gval : 0.0, gy = aux.gdir == 1 ? aux.gval : 0.0,
+                     gzz = aux.gdir == 2 ? aux.gval : 0.0;
+        const double px[2] = {qx[0] + dtau[0][0] + rr[0] * gx, qx[1] + dtau[0][1] + rr[1] * gx};
+        const double py[2] = {qy[0] + dtau[1][0] + rr[0] * gy, qy[1] + dtau[1][1] + rr[1] * gy};
+        const double pz[2] = {qz[0] + dtau[2][0] + rr[0] * gzz, qz[1] + dtau[2][1] + rr[1] * gzz};
         const double p1r = -(kx * px[0] + ky * py[0] + kz * pz[0]) / K2;
         const double p1i = -(kx * px[1] + ky * py[1] + kz * pz[1]) / K2;
         acc[7] += w * (proj2(ux[0], ux[1], p1r * kx, p1i * kx) + proj2(uy[0], uy[1], p1r * ky, p1i * ky));
@@ -2297,8 +2300,8 @@ __device__ __forceinline__ void spec_values(const Geo& g, const C<T>* __restrict
     const C<T> r0 = ((const C<T>*)a.rho)[idx];
     rr = r0.x;
     ri = r0.y;
-    R[2][0] += rr * a.gz;
-    R[2][1] += ri * a.gz;
+    R[a.gdir][0] += rr * a.gval;
+    R[a.gdir][1] += ri * a.gval;
   }
   double P[2];
 #pragma unroll

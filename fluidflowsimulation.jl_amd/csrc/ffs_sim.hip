@@ -146,7 +146,9 @@ struct ffs_sim {
   // (the SandP variants: w, G w, P, cp, cpm1 and the copies of cp, cpm1)
   enum { DR_U = 0, DR_S = 3, DR_M = 8, DR_UF = 13, DR_SA = 16, DR_L = 21, DR_C = 27, DR_PC = 28, DR_PCM1 = 29,
          DR_SAVE = 30, DR_W = 32, DR_WF = 35, DR_PA = 38, DR_CP = 43, DR_CPM1 = 44, DR_SAVEP = 45, kDynReal = 47 };
-  double alpha = 0.0, gz = 0.0, drhodz = 0.0;
+  // gravity g = gval e_gdir (gravityDirection); gz = g.z, what calculate_N and spectrum_buoyancy read
+  double alpha = 0.0, gz = 0.0, gval = 0.0, drhodz = 0.0;
+  int gdir = 2;
   double palpha = 0.0;  // passive scalar diffusivity nu / scalarPr
   void* w1 = nullptr;
   void* w2 = nullptr;
@@ -743,7 +745,11 @@ struct ffs_sim {
       if (!(p.pr > 0) || !(p.reference_density != 0))
         throw FfsError(FFS_ERR_ARG, "density needs Pr > 0 and referenceDensity != 0");
       alpha = p.nu / p.pr;
-      gz = p.z_acceleration / p.reference_density;
+      if (p.gravity_direction < 0 || p.gravity_direction > 2)
+        throw FfsError(FFS_ERR_ARG, "gravityDirection is x, y or z");
+      gdir = p.gravity_direction;
+      gval = p.z_acceleration / p.reference_density;
+      gz = gdir == 2 ? gval : 0.0;
       drhodz = p.density_gradient;
     }
     // the x pass carries S and tau for the pointwise models; the dynamic model forms tau beforehand
@@ -1034,7 +1040,7 @@ struct ffs_sim {
     a.etd = (etd || p.integrator == FFS_EULER) ? 1 : 0;  // no first-step halving (Euller set_dt!, Euller.jl:12)
     a.cfl = p.cfl;
     a.dx = 2 * M_PI * p.y_domain / p.ny;
-    a.gnorm = dens ? std::fabs(gz) : 0.0;
+    a.gnorm = dens ? std::fabs(gval) : 0.0;  // norm(gravity(s))
     const bool force = do_force && p.forcing == FFS_FORCING_SHELL && nband > 0;
     a.band = band;
     a.nband = force ? nband : 0;
@@ -1111,6 +1117,8 @@ struct ffs_sim {
     a.nu = p.nu;
     a.alpha = dens ? alpha : 0.0;
     a.gz = dens ? gz : 0.0;
+    a.gdir = gdir;
+    a.gval = dens ? gval : 0.0;
     a.st = st;
     a.hv = hv_args();
     launch("spectra", [&] { FFS_DISPATCH(launch_spectra, g, u, a, spec_part, spec_out, stream); });
@@ -1158,7 +1166,8 @@ struct ffs_sim {
         }
     }
     auto& sl = slots[slot];
-    StatsAux aux{les ? tau : nullptr, dens ? rho : nullptr, gz, hv_args(), dens ? hvd_args() : HvArgs{},
+    StatsAux aux{les ? tau : nullptr, dens ? rho : nullptr, gz, gdir, dens ? gval : 0.0, hv_args(),
+                 dens ? hvd_args() : HvArgs{},
                  pscal ? phi : nullptr};
     launch("stats", [&] { FFS_DISPATCH(launch_stats, g, u, rhs, ke, aux, partials, red, stream); });
     HIPCHECK(hipMemcpyAsync(sl.red, red, kStatsSlots * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -1213,7 +1222,7 @@ struct ffs_sim {
         rhohdiss = (p.density_hv == FFS_HV_HYPER ? p.density_hv_nu : -1.0) * a[37];
         trhodiss += rhohdiss;
       }
-      other.insert(other.end(), {a[29], a[22], a[23], a[24], a[25], rhodiss, trhodiss, -gz * a[26]});
+      other.insert(other.end(), {a[29], a[22], a[23], a[24], a[25], rhodiss, trhodiss, -gval * a[26]});
       if (p.density_hv) other.push_back(rhohdiss);
     }
     if (les) {  // les_stats, stats.jl:679-705
@@ -1307,6 +1316,8 @@ struct ffs_sim {
       a.alpha = alpha;
       a.drhodz = drhodz;
       a.gz = gz;
+      a.gdir = gdir;
+      a.gval = gval;
     }
     if (pscal) {
       a.phi = phi;
@@ -1787,6 +1798,7 @@ int ffs_params_default(ffs_params* p) {
   p->equation = FFS_EQ_ROTATIONAL;
   p->density_hv = FFS_HV_NONE;
   p->density_hv_m = 2;
+  p->gravity_direction = 2;
   return FFS_OK;
 }
 

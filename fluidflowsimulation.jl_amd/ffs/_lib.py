@@ -103,6 +103,7 @@ class FfsParams(ctypes.Structure):
         ("les_s_model", ctypes.c_int32),
         ("les_test_width", ctypes.c_double),
         ("les_cmin", ctypes.c_double),
+        ("gravity_direction", ctypes.c_int32),
         ("density_hv", ctypes.c_int32),
         ("density_hv_nu", ctypes.c_double),
         ("density_hv_m", ctypes.c_int32),

@@ -170,6 +170,7 @@ class Params:
     alvelius_c: float = 1.0
     alvelius_kf: float = 0.0
     hyperviscosity: str = None      # hyperViscosity: "spectralBarrier", or any other value: HyperViscosity
+    gravity_direction: str = "z"    # gravityDirection: the axis of g and of the density's mean gradient
     density_hyperviscosity: str = None  # densityHyperViscosity: the density's own, as hyperviscosity
     density_hv_nu: float = 0.0      # densityHyperViscosityCoefficient
     density_hv_m: int = 2           # densityHyperViscosityM
@@ -234,6 +235,7 @@ class Params:
         p.hv_nu, p.hv_m = self.hv_nu, self.hv_m
         p.sb_initk, p.sb_endk = self.sb_initk, self.sb_endk
         p.density_hv = hv_code(self.density_hyperviscosity)
+        p.gravity_direction = {"x": 0, "y": 1, "z": 2}[self.gravity_direction]
         p.density_hv_nu, p.density_hv_m = self.density_hv_nu, self.density_hv_m
         p.density_sb_initk, p.density_sb_endk = self.density_sb_initk, self.density_sb_endk
         p.passive_scalar = int(bool(self.passive_scalar))
@@ -311,6 +313,9 @@ def parameters(d):
         p.density_gradient = parse_expr(req("densityGradient"))
         p.z_acceleration = parse_expr(req("zAcceleration"))
         p.reference_density = parse_expr(req("referenceDensity"))
+        p.gravity_direction = opt("gravityDirection", str, "z")  # density_stratification_types.jl:152
+        if p.gravity_direction not in ("x", "y", "z"):
+            raise ValueError(f"gravityDirection {p.gravity_direction!r}: x, y or z")
     p.equation = opt("equation", str, None)  # simulationtypes.jl:265-273
     if "passiveScalar" in d:  # simulationtypes.jl:320-352
         known.add("passiveScalar")

@@ -67,6 +67,7 @@ mutable struct ffs_params
     les_s_model::Int32
     les_test_width::Float64
     les_cmin::Float64
+    gravity_direction::Int32
     density_hv::Int32
     density_hv_nu::Float64
     density_hv_m::Int32
@@ -137,6 +138,7 @@ function params_from_global(d::AbstractDict)
         p.density = Int32(1)
         p.pr, p.density_gradient = num(:Pr), num(:densityGradient)
         p.z_acceleration, p.reference_density = num(:zAcceleration), num(:referenceDensity)
+        p.gravity_direction = Int32(Dict("x" => 0, "y" => 1, "z" => 2)[get(d, :gravityDirection, "z")])
     end
     # equation (simulationtypes.jl:265-273): "NS" selects the conservation form
     p.equation = Int32(get(d, :equation, "") == "NS" ? 1 : 0)

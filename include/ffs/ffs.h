@@ -179,6 +179,8 @@ typedef struct ffs_params {
    * FFS_HV_*, densityHyperViscosityCoefficient, densityHyperViscosityM (2), densityInitk,
    * densityEndk (<= 0: the defaults of initk / endk).  It acts through the density's ETD3 operator
    * and adds the rhohdiss stats column. */
+  int32_t gravity_direction;                /* gravityDirection: 0 x, 1 y, 2 z (default): the axis of g and
+                                               of the density's mean gradient */
   int32_t density_hv;
   double density_hv_nu;
   int32_t density_hv_m;

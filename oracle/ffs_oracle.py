@@ -163,7 +163,8 @@ class OracleSim:
         oracle.les_models.MODELS, with les_cp (Silvis P constant, SandP pConstant) and les_smodel
         (SandP's sLesModel); dynamicSmagorinsky takes les_test_width (default twice the filter
         width) and les_cmin;
-        density: dict(Pr=, gradient=, g=[, hv=]) for the Boussinesq approximation, hv its own
+        density: dict(Pr=, gradient=, g=[, dir=][, hv=]) for the Boussinesq approximation, dir the
+        gravity's axis (gravityDirection, 0 1 2, default 2), hv its own
         hyperviscosity (densityHyperViscosity, as hv below; none by default);
         hv: dict(kind="hyper", nu=, m=) (HyperViscosity{nu_h, M}) or dict(kind="barrier",
         initk=, endk=) (SpectralBarrier), Hyperviscosity/hyperviscosity_types.jl;
@@ -194,7 +195,11 @@ class OracleSim:
         if density is not None:
             self.alpha = nu / density["Pr"]
             self.drhodz = density["gradient"]
-            self.gz = density["g"]
+            # gravityDirection (density_stratification_types.jl:152-162): g = g e_dir, the mean gradient's
+            # direction too; calculate_N and spectrum_buoyancy read g.z, which is 0 off the z axis
+            self.gdir = density.get("dir", 2)
+            self.gval = density["g"]
+            self.gz = self.gval if self.gdir == 2 else 0.0
         self.variable_dt = variable_dt
         self.cfl = cfl
         self.dt = self.dt2 = self.dt3 = dt
@@ -363,7 +368,7 @@ class OracleSim:
             rhs = rhs + self.div_tau()
         if self.density is not None:
             rhs = rhs.copy()
-            rhs[2] = rhs[2] + self.rhoh * self.gz
+            rhs[self.gdir] = rhs[self.gdir] + self.rhoh * self.gval
         rhs = np.where(g.RANGE, rhs, 0)
         self.rhs = g.project(rhs)
         if self.scalar is not None:
@@ -374,7 +379,7 @@ class OracleSim:
         if self.density is not None:
             # div!(rhorhs, flux, u_z, -drhodz) (vectorfunctions.jl:93-103); explicit diffusion is
             # absent in the reference (models.jl:361), so AB3 density has none
-            r = 1j * (kx * fluxh[0] + (ky * fluxh[1] + kz * fluxh[2])) + (-self.drhodz) * self.uh[2]
+            r = 1j * (kx * fluxh[0] + (ky * fluxh[1] + kz * fluxh[2])) + (-self.drhodz) * self.uh[self.gdir]
             self.rhorhs = np.where(g.RANGE, r, 0)
 
     def div_tau(self):
@@ -395,7 +400,7 @@ class OracleSim:
         cfl *= {0: 0.1, 1: 0.2, 2: 0.4, 3: 0.8}.get(it, 1.0)
         newdt = cfl * dx / self.umax
         if self.density is not None:
-            newdt = min(newdt, cfl * math.sqrt(dx / (self.rhomax * abs(self.gz))))
+            newdt = min(newdt, cfl * math.sqrt(dx / (self.rhomax * abs(self.gval))))
         if it > 3:
             dt = self.dt
             newdt = max(dt / 1.25, newdt) if newdt < dt else min(1.25 * dt, newdt)
@@ -637,7 +642,7 @@ class OracleSim:
             dt_ = self.div_tau()
             r = [r[c] + dt_[c] for c in range(3)]
         if self.density is not None:
-            r[2] = r[2] + self.rhoh * self.gz
+            r[self.gdir] = r[self.gdir] + self.rhoh * self.gval
         with np.errstate(invalid="ignore", divide="ignore"):
             p1 = -(g.kx * r[0] + g.ky * r[1] + g.kz * r[2]) / g.K2
         p1[0, 0, 0] = 0
@@ -723,7 +728,7 @@ class OracleSim:
             rho2 = float(np.sum(W * er))
             drd = [float(np.sum(W * kk * kk * er)) for kk in (g.kx, g.ky, g.kz)]
             rhodiss = self.alpha * (drd[0] + drd[1] + drd[2])
-            bf = -self.gz * float(np.sum(W * proj(u[2], r)))  # buoyancy_flux (:192-211)
+            bf = -self.gval * float(np.sum(W * proj(u[self.gdir], r)))  # buoyancy_flux (:192-211)
             trhodiss = rhodiss
             extra = []
             if self.density.get("hv") is not None:  # scalar_hvis_stats (stats.jl:1086-1170): rhohdiss
@@ -835,7 +840,7 @@ class OracleSim:
             d = self.div_tau()
             r = [r[c] + d[c] for c in range(3)]
         if self.density is not None:
-            r[2] = r[2] + self.rhoh * self.gz
+            r[self.gdir] = r[self.gdir] + self.rhoh * self.gval
         with np.errstate(invalid="ignore", divide="ignore"):
             p1 = -(g.kx * r[0] + g.ky * r[1] + g.kz * r[2]) / g.K2
         p1[0, 0, 0] = 0

@@ -180,3 +180,16 @@ def test_velocity_time_step_values():
                     ("RK4", _lib.FFS_ETD3)):
         assert config.parameters(dict(base, velocityTimeStep=v)).to_c().integrator == code
     assert config.parameters(base).to_c().integrator == _lib.FFS_ETD3
+
+
+def test_gravity_direction_key():
+    from ffs import _lib
+    base = {"nx": "16", "ny": "16", "nz": "16", "xDomainSize": "1.0", "yDomainSize": "1.0", "zDomainSize": "1.0",
+            "kinematicViscosity": "1e-3", "dtStat": "10", "dtSpec": "100", "writeTime": "100",
+            "densityStratification": "true", "Pr": "1", "densityGradient": "1", "zAcceleration": "-1",
+            "referenceDensity": "1"}
+    assert config.parameters(base).to_c().gravity_direction == 2
+    assert config.parameters(dict(base, gravityDirection="y")).to_c().gravity_direction == 1
+    with pytest.raises(ValueError):
+        config.parameters(dict(base, gravityDirection="w"))
+    assert _lib.default_params().gravity_direction == 2
