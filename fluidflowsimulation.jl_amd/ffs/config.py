@@ -317,6 +317,12 @@ def parameters(d):
         if p.gravity_direction not in ("x", "y", "z"):
             raise ValueError(f"gravityDirection {p.gravity_direction!r}: x, y or z")
     p.equation = opt("equation", str, None)  # simulationtypes.jl:265-273
+    # keys whose features this build does not have: refused rather than ignored
+    for k, what in (("specFil", "the filtered statistics and spectra (Stats_fil.txt, *_fil spectra)"),
+                    ("densityLesModel", "the density's LES flux (EddyDiffusion, VorticityDiffusion)"),
+                    ("scalarLesModel", "the passive scalar's LES flux (EddyDiffusion, VorticityDiffusion)")):
+        if k in d:
+            raise NotImplementedError(f"{k}: {what} is not built")
     if "passiveScalar" in d:  # simulationtypes.jl:320-352
         known.add("passiveScalar")
         p.passive_scalar = True

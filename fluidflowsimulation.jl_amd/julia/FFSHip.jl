@@ -160,6 +160,10 @@ function params_from_global(d::AbstractDict)
             haskey(d, :hyperViscosityM) && (p.hv_m = parse(Int32, d[:hyperViscosityM]))
         end
     end
+    # keys whose features this build does not have: refused rather than ignored
+    for k in (:specFil, :densityLesModel, :scalarLesModel)
+        haskey(d, k) && error("$(k): not built (filtered statistics / scalar LES fluxes)")
+    end
     # the density's own hyperviscosity (density_stratification_types.jl:137-149)
     if haskey(d, :densityHyperViscosity)
         if d[:densityHyperViscosity] == "spectralBarrier"

@@ -193,3 +193,12 @@ def test_gravity_direction_key():
     with pytest.raises(ValueError):
         config.parameters(dict(base, gravityDirection="w"))
     assert _lib.default_params().gravity_direction == 2
+
+
+@pytest.mark.parametrize("key,value", [("specFil", "2"), ("densityLesModel", "eddyDiffusion"),
+                                       ("scalarLesModel", "vorticityDiffusion")])
+def test_unbuilt_keys_are_refused(key, value):
+    base = {"nx": "16", "ny": "16", "nz": "16", "xDomainSize": "1.0", "yDomainSize": "1.0", "zDomainSize": "1.0",
+            "kinematicViscosity": "1e-3", "dtStat": "10", "dtSpec": "100", "writeTime": "100"}
+    with pytest.raises(NotImplementedError, match=key):
+        config.parameters(dict(base, **{key: value}))
