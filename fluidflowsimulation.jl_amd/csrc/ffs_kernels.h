@@ -113,6 +113,9 @@ template <typename T> struct ColAux {
   const typename std::conditional<sizeof(T) == 8, double2, float2>::type* rho = nullptr;  // inverse z: density
   const typename std::conditional<sizeof(T) == 8, double2, float2>::type* phi = nullptr;  // and passive scalar
   int nsform = 0;  // conservation form: the inputs have no vorticity (u, then rho, phi, S)
+  // the scalars' LES fluxes: i k s of these scalars follow S (3 fields each), in this order
+  const typename std::conditional<sizeof(T) == 8, double2, float2>::type* grad1 = nullptr;
+  const typename std::conditional<sizeof(T) == 8, double2, float2>::type* grad2 = nullptr;
   typename std::conditional<sizeof(T) == 8, double2, float2>::type* dst[kMaxFields] = {};  // forward z outputs
 };
 
@@ -121,7 +124,8 @@ template <typename T> struct ColAux {
 // (u, w, [rho], [S]) are formed on the fly from u (and rho)
 template <typename T>
 void launch_inv_z(const Geo& g, const void* u, void* a1, int nf, bool curl, const void* tw, hipStream_t s,
-                  const void* rho = nullptr, const void* phi = nullptr, int nsform = 0);
+                  const void* rho = nullptr, const void* phi = nullptr, int nsform = 0,
+                  const void* grad1 = nullptr, const void* grad2 = nullptr);  // i k s after S, per scalar
 template <typename T>
 void launch_inv_y(const Geo& g, const void* a1, void* a2, int nf, const void* tw, hipStream_t s);
 // field set of the fused x pass
@@ -145,6 +149,11 @@ struct XOps {
   int ns;             // conservation form: inputs u (no w), outputs uu = traceless(-u u) (5) in place of u x w
   double les_2alpha;  // 2 (c Delta)^2 of the Smagorinsky stress
   LesArgs lm;         // the LES model (lm.model != LES_SMAGORINSKY: the general x pass, with w at slots 3..5)
+  // the scalars' LES fluxes (les_real_calc.jl:120-136): 0 none, 1 EddyDiffusion nu_t grad s, 2 also
+  // (c Delta^2 / 2) w x grad s (VorticityDiffusion); their gradients are inputs after S, rho's first
+  int rho_les = 0, phi_les = 0;
+  double rho_cv = 0.0, phi_cv = 0.0;  // c Delta^2 of VorticityDiffusion
+  double les_alpha = 0.0;             // c^2 Delta^2 (Smagorinsky's nu_t = alpha |S|)
 };
 template <typename T>
 void launch_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
@@ -282,6 +291,7 @@ struct StatsAux {
   HvArgs hv;        // hyperviscous dissipation sums (slots 30, 31)
   HvArgs hvd;       // the density's (slot 37)
   const void* phi;  // passive scalar or null (slots 32-36)
+  const void* rflux = nullptr;  // the density's total flux, with an LES flux (rhopr, slot 38)
 };
 template <typename T>
 int launch_stats(const Geo& g, const void* u, const void* nl, const void* ke, const StatsAux& aux, double* partials,

@@ -240,6 +240,13 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
           if (e == 0) return aux.phi[idx];
           --e;
         }
+        if (e >= 5) {  // i k s of the scalars with an LES flux (fourierspacep1!, models.jl:110-116)
+          e -= 5;
+          const C<T> sv = e < 3 ? aux.grad1[idx] : aux.grad2[idx];
+          const int d = e < 3 ? e : e - 3;
+          const T kd = d == 0 ? kx : (d == 1 ? ky : kz);
+          return mk<T>(-kd * sv.y, kd * sv.x);
+        }
         // S_ab = (i k_a u_b + i k_b u_a) / 2 for ab = xx, xy, xz, yy, yz (symouter, models.jl:98)
         const int a_ = (e < 3) ? 0 : 1;
         const int b_ = (e < 3) ? e : e - 2;
@@ -338,11 +345,13 @@ static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, con
 
 template <typename T>
 void launch_inv_z(const Geo& g, const void* u, void* a1, int nf, bool curl, const void* tw, hipStream_t s,
-                  const void* rho, const void* phi, int nsform) {
+                  const void* rho, const void* phi, int nsform, const void* grad1, const void* grad2) {
   ColAux<T> aux{};
   aux.rho = (const C<T>*)rho;
   aux.phi = (const C<T>*)phi;
   aux.nsform = nsform;
+  aux.grad1 = (const C<T>*)grad1;
+  aux.grad2 = (const C<T>*)grad2;
   switch (g.nz) {
 #define X(n)                                                                                    \
   case n:                                                                                       \
@@ -420,7 +429,7 @@ __device__ __forceinline__ void split_pair(const C<T>* Z, int k, C<T>& A, C<T>& 
 // (AntiSymTen(v) x = v x x), so A = AntiSymTen(-w/2) has A_xy = w_z/2, A_xz = -w_y/2, A_yz = w_x/2 and
 // S + A is the transposed velocity gradient; Lie(a, b) = ab - ba; a:b sums every product a_ij b_ij.
 template <typename T>
-__device__ __forceinline__ void les_stress(const LesArgs& l, const T s[5], const T w[3], T t[5]) {
+__device__ __forceinline__ void les_stress(const LesArgs& l, const T s[5], const T w[3], T t[5], T* nut_out = nullptr) {
   const T S[3][3] = {{s[0], s[1], s[2]}, {s[1], s[3], s[4]}, {s[2], s[4], -s[0] - s[3]}};
   const T hx = (T)0.5 * w[0], hy = (T)0.5 * w[1], hz = (T)0.5 * w[2];
   const T A[3][3] = {{0, hz, -hy}, {-hz, 0, hx}, {hy, -hx, 0}};  // AntiSymTen(-w/2)
@@ -487,8 +496,10 @@ __device__ __forceinline__ void les_stress(const LesArgs& l, const T s[5], const
     return (T)0;  // fakeSmagorinsky: nu_t = 0
   };
   T R[3][3];
+  T nut = 0;  // the eddy viscosity the scalars' LES fluxes read
   if (l.model == LES_SANDP) {
     const T nu = eddy_nu(l.smodel, (T)l.c);
+    nut = nu;
     T P[3][3];
     lie_p(P);
     const T beta = (T)l.cp * d2;
@@ -503,7 +514,7 @@ __device__ __forceinline__ void les_stress(const LesArgs& l, const T s[5], const
     const T f = sqrt(sw[0] * sw[0] + sw[1] * sw[1] + sw[2] * sw[2]) /
                 (sqrt(ss) * sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]));  // fvs
     const T c = (T)l.c, cp = (T)l.cp;
-    const T nut = c * c * d2 * f * f * f * sqrt(ss) / (T)2;
+    nut = c * c * d2 * f * f * f * sqrt(ss) / (T)2;
     const T nup = cp * d2 * f * f * f * f / (T)4;
     T P[3][3];
     lie_p(P);
@@ -515,6 +526,7 @@ __device__ __forceinline__ void les_stress(const LesArgs& l, const T s[5], const
     const T c = (T)l.c;
     const T x = prod_x(c);
     const T mnu = ss == (T)0 ? (T)0 : -fmin((T)0, x);
+    nut = ss == (T)0 ? (T)0 : fmax((T)0, x);
     T M[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -533,6 +545,7 @@ __device__ __forceinline__ void les_stress(const LesArgs& l, const T s[5], const
       for (int j = 0; j < 3; ++j) R[i][j] = c * d2 * (M[i][j] - (i == j ? tr : (T)0)) + (T)2 * mnu * S[i][j];
   } else {  // Vreman, productionViscosity, Smagorinsky
     const T nu = eddy_nu(l.model, (T)l.c);
+    nut = nu;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -543,6 +556,28 @@ __device__ __forceinline__ void les_stress(const LesArgs& l, const T s[5], const
   t[2] = R[0][2];
   t[3] = R[1][1];
   t[4] = R[1][2];
+  if (nut_out) *nut_out = nut;
+}
+
+// a scalar's flux with its LES part: -s u + nu_t grad s [+ (cv/2) w x grad s] (les_real_calc.jl:120-136,
+// models.jl:264-273)
+template <typename T>
+__device__ __forceinline__ void scalar_flux(T s, T ux, T uy, T uz, int les, T nut, const T* g, const T* w, T cv,
+                                            T* out) {
+  T f[3] = {(-s) * ux, (-s) * uy, (-s) * uz};
+  if (les) {
+    T l[3] = {nut * g[0], nut * g[1], nut * g[2]};
+    if (les == 2) {
+      const T h = cv / (T)2;
+      l[0] += h * (w[1] * g[2] - w[2] * g[1]);
+      l[1] += h * (w[2] * g[0] - w[0] * g[2]);
+      l[2] += h * (w[0] * g[1] - w[1] * g[0]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[c] += l[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[c] = f[c];
 }
 
 // Fused nonlinear x pass, general field set (LDS slots; any transform length).
@@ -623,18 +658,33 @@ __global__ void __launch_bounds__(256) k_x_gen(Geo g, const C<T>* __restrict__ a
       o = 3;
       i = 6;
     }
+    // the eddy viscosity and the gradients of the scalars' LES fluxes (they follow S in the inputs)
+    T nut = 0;
+    const T wv3[3] = {in[3], in[4], in[5]};
+    const int i_s = i + (ops.rho ? 1 : 0) + (ops.phi ? 1 : 0);
+    if (ops.rho_les || ops.phi_les) {
+      const T sv[5] = {in[i_s], in[i_s + 1], in[i_s + 2], in[i_s + 3], in[i_s + 4]};
+      if (ops.lm.model == LES_SMAGORINSKY) {  // nu_t = alpha |S| (les_real_calc.jl:77-79)
+        const T szz = -sv[0] - sv[3];
+        const T ss = sv[0] * sv[0] + sv[3] * sv[3] + szz * szz + (T)2 * (sv[1] * sv[1] + sv[2] * sv[2] + sv[4] * sv[4]);
+        nut = (T)ops.les_alpha * sqrt((T)2 * ss);
+      } else {
+        T tv[5];
+        les_stress<T>(ops.lm, sv, wv3, tv, &nut);
+      }
+    }
+    const T* g_rho = in + i_s + 5;
+    const T* g_phi = in + i_s + 5 + (ops.rho_les ? 3 : 0);
     if (ops.rho) {
       const T rho = in[i++];
-      out[o++] = (-rho) * ux;
-      out[o++] = (-rho) * uy;
-      out[o++] = (-rho) * uz;
+      scalar_flux<T>(rho, ux, uy, uz, ops.rho_les, nut, g_rho, wv3, (T)ops.rho_cv, out + o);
+      o += 3;
       if (row0 + r < nrows) rmax = nanmax(rmax, (double)fabs(rho));
     }
     if (ops.phi) {  // -phi u (the passive scalar enters no time-step bound)
       const T phi = in[i++];
-      out[o++] = (-phi) * ux;
-      out[o++] = (-phi) * uy;
-      out[o++] = (-phi) * uz;
+      scalar_flux<T>(phi, ux, uy, uz, ops.phi_les, nut, g_phi, wv3, (T)ops.phi_cv, out + o);
+      o += 3;
     }
     if (ops.les && ops.lm.model != LES_SMAGORINSKY) {  // the models that read w (slots 3..5)
       const T sv[5] = {in[i], in[i + 1], in[i + 2], in[i + 3], in[i + 4]};
@@ -981,6 +1031,9 @@ template <typename T, int N, int NSL>
 static void run_x_gen(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
                       unsigned long long* rhomax, const void* tw, hipStream_t s) {
   constexpr int ROWS = xgen_rows<N, NSL>();
+  if (ROWS * NSL * tper<N>() > 256)  // the kernel's launch bound
+    throw std::runtime_error("nonlinear x pass: " + std::to_string(NSL) + " field pairs of length " + std::to_string(N) +
+                             " exceed one block");
   const size_t lds = ((size_t)ROWS * NSL * (N + 1) + tw_lds<T>(N)) * sizeof(C<T>);
   static bool attr = false;
   if (!attr) {
@@ -1015,7 +1068,8 @@ template <typename T, int N>
 static void run_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
                      unsigned long long* rhomax, const void* tw, hipStream_t s) {
   const bool plain = !ops.rho && !ops.les && !ops.phi;
-  const bool w_les = ops.les && les_needs_w(ops.lm);  // the general pass holds u, w and S of a row together
+  // the general pass holds u, w, S and the scalars' gradients of a row together
+  const bool w_les = (ops.les && les_needs_w(ops.lm)) || ops.rho_les || ops.phi_les;
   if constexpr (xreg_ok<T, N>()) {
     if ((FFS_XNL_MODES || plain) && !w_les) {
       // u x w or traceless(-u u) (+ 1/2|u|^2, always the last output slot), then -rho u, -phi u and
@@ -1046,9 +1100,12 @@ static void run_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, un
     }
   }
   const int nsl = (std::max(ops.nin, ops.nout) + 1) / 2;
+  if (nsl > 10) throw std::runtime_error("nonlinear x pass: more fields than the general pass holds");
   if (nsl <= 3) run_x_gen<T, N, 3>(g, a2, b2, ops, umax, rhomax, tw, s);
   else if (nsl <= 4) run_x_gen<T, N, 4>(g, a2, b2, ops, umax, rhomax, tw, s);
-  else run_x_gen<T, N, 6>(g, a2, b2, ops, umax, rhomax, tw, s);
+  else if (nsl <= 6) run_x_gen<T, N, 6>(g, a2, b2, ops, umax, rhomax, tw, s);
+  else if (nsl <= 8) run_x_gen<T, N, 8>(g, a2, b2, ops, umax, rhomax, tw, s);
+  else run_x_gen<T, N, 10>(g, a2, b2, ops, umax, rhomax, tw, s);
 }
 
 template <typename T>
@@ -1942,7 +1999,8 @@ void launch_project(const Geo& g, void* u, hipStream_t s) {
 //  22 sum w|rho|^2  23..25 sum w k_i^2 |rho|^2  26 sum w proj(uz, rho)  29 Re rho(k=0)
 //  LES (les_stats :679-705): 27 sum w proj((ik.tau)_{x,y}, u_{x,y})  28 the same for z
 //  the density's hyperviscous dissipation (scalar_hvis_stats :1086-1170): 37 sum w f_hvd |rho|^2
-enum { kStatsN = 38, kStatsBlocks = 512, kStatsThreads = 128 };
+//  the density's LES flux (scalar_les_stats :1032-1053): 38 sum w proj((i k).F, rho)
+enum { kStatsN = 39, kStatsBlocks = 512, kStatsThreads = 128 };
 static_assert(kStatsN <= kStatsSlots, "a stats row of partial sums fits its slot");
 
 __device__ __forceinline__ double proj2(double ax, double ay, double bx, double by) { return ax * bx + ay * by; }
@@ -2036,6 +2094,12 @@ __global__ void __launch_bounds__(kStatsThreads) k_stats(Geo g, const C<T>* __re
       if (aux.hvd.kind) {
         const double kv[3] = {kx, ky, kz};
         acc[37] += w * hv_weight(aux.hvd, kv) * er;
+      }
+      if (aux.rflux) {  // scalar_les_stats (stats.jl:1032-1053): proj((i k).F, rho) of the total flux F
+        const C<T>* fl = (const C<T>*)aux.rflux;
+        const C<T> fx = fl[idx], fy = fl[n + idx], fz = fl[2 * n + idx];
+        const double dr = kx * fx.x + ky * fy.x + kz * fz.x, di = kx * fx.y + ky * fy.y + kz * fz.y;
+        acc[38] += w * proj2(-di, dr, rr[0], rr[1]);
       }
     }
     if (nl) {
@@ -2917,7 +2981,7 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
 // ---- explicit instantiations ----
 #define FFS_INST(T)                                                                                                   \
   template void launch_inv_z<T>(const Geo&, const void*, void*, int, bool, const void*, hipStream_t, const void*,      \
-                                const void*, int);                                                                 \
+                                const void*, int, const void*, const void*);                                       \
   template void launch_inv_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t);                       \
   template void launch_x_nl<T>(const Geo&, const void*, void*, const XOps&, unsigned long long*, unsigned long long*, \
                                const void*, hipStream_t);                                                           \

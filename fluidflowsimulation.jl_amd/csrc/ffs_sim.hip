@@ -123,6 +123,8 @@ struct ffs_sim {
   void* pflux = nullptr;
   void* uu = nullptr;     // conservation form: F(traceless(-u u)), 5 components
   bool les = false, dens = false, etd = false, pscal = false, nsform = false;
+  int sles = 0, dles = 0;   // the passive scalar's and the density's LES flux (0 none, 1 eddy, 2 vorticity)
+  double sles_cv = 0.0, dles_cv = 0.0;  // c Delta^2 of VorticityDiffusion
   int nin = 6, nfw = 3;     // physical inputs / forward outputs of the nonlinear term (without 1/2|u|^2)
   double les_alpha = 0.0;   // (c Delta)^2
   LesArgs lesm{};           // the LES model (lesm.model = p.les_model)
@@ -710,7 +712,9 @@ struct ffs_sim {
     nsform = (p.equation == FFS_EQ_NS);
     if (pscal) {
       // with an LES model the reference gives the scalar an eddy-diffusion flux (simulationtypes.jl:335-348)
-      if (les) throw FfsError(FFS_ERR_ARG, "the passive scalar's LES models (EddyDiffusion, VorticityDiffusion) are not built");
+      // with an LES model the reference gives the scalar an LES flux, EddyDiffusion unless
+      // scalarLesModel is vorticityDiffusion (simulationtypes.jl:334-346)
+      if (les) sles = p.scalar_les == 2 ? 2 : 1;
       if (!(p.scalar_pr > 0)) throw FfsError(FFS_ERR_ARG, "scalarPr must be > 0 (Inf: no diffusion)");
       if (p.scalar_direction < 0 || p.scalar_direction > 2) throw FfsError(FFS_ERR_ARG, "scalarDirection is x, y or z");
       palpha = p.nu / p.scalar_pr;  // nu / Inf = 0
@@ -752,9 +756,20 @@ struct ffs_sim {
       gz = gdir == 2 ? gval : 0.0;
       drhodz = p.density_gradient;
     }
+    if (dens && p.density_les) {  // densityLesModel (density_stratification_types.jl:176-184)
+      if (p.density_les < 0 || p.density_les > 2) throw FfsError(FFS_ERR_ARG, "unknown densityLesModel");
+      if (!les) throw FfsError(FFS_ERR_ARG, "densityLesModel needs an LES model");
+      dles = p.density_les;
+    }
+    if (sles || dles) {
+      if (dyn) throw FfsError(FFS_ERR_ARG, "the scalars' LES fluxes are built for the pointwise LES models only");
+      if (nsform) throw FfsError(FFS_ERR_ARG, "the scalars' LES fluxes need equation = rotational here");
+      sles_cv = (std::isnan(p.scalar_les_c) ? 1.0 : p.scalar_les_c) * lesm.d2;
+      dles_cv = (std::isnan(p.density_les_c) ? 0.1 : p.density_les_c) * lesm.d2;
+    }
     // the x pass carries S and tau for the pointwise models; the dynamic model forms tau beforehand
     const bool xles = les && !dyn;
-    nin = (nsform ? 3 : 6) + (dens ? 1 : 0) + (pscal ? 1 : 0) + (xles ? 5 : 0);
+    nin = (nsform ? 3 : 6) + (dens ? 1 : 0) + (pscal ? 1 : 0) + (xles ? 5 : 0) + 3 * ((dles ? 1 : 0) + (sles ? 1 : 0));
     nfw = (nsform ? 5 : 3) + (dens ? 3 : 0) + (pscal ? 3 : 0) + (xles ? 5 : 0);
     if (p.dt_stat <= 0 || p.dt_spec <= 0 || p.write_time <= 0)
       throw FfsError(FFS_ERR_ARG, "dtStat, dtSpec and writeTime must be positive");
@@ -846,6 +861,9 @@ struct ffs_sim {
   }
 
   size_t padded_len() const { return 2 * (size_t)(p.nx / 2 + 1) * p.ny * p.nz; }
+  // the scalars whose i k s follow S in the curl pass's inputs: the density's first
+  const void* grad1() const { return dles ? rho : (sles ? phi : nullptr); }
+  const void* grad2() const { return (dles && sles) ? phi : nullptr; }
   int dyn_spec_fields() const { return dynp ? 19 : 13; }
   int dyn_real_fields() const { return dynp ? kDynReal : DR_W; }
   char* rptr(void* base, size_t fields) { return (char*)base + fields * padded_len() * (csz / 2); }
@@ -984,7 +1002,7 @@ struct ffs_sim {
   void nonlinear(bool need_ke, bool fused = false) {
     if (dyn) dynamic_les(true);
     launch("inv_z_curl", [&] {
-      FFS_DISPATCH(launch_inv_z, g, u, w1, nin, true, tw_z, stream, rho, phi, nsform ? 1 : 0);
+      FFS_DISPATCH(launch_inv_z, g, u, w1, nin, true, tw_z, stream, rho, phi, nsform ? 1 : 0, grad1(), grad2());
     });
     launch("inv_y", [&] { FFS_DISPATCH(launch_inv_y, g, w1, w2, nin, tw_y, stream); });
     XOps ops{};
@@ -997,6 +1015,11 @@ struct ffs_sim {
     ops.ke = need_ke ? 1 : 0;
     ops.les_2alpha = 2 * les_alpha;
     ops.lm = lesm;
+    ops.rho_les = dles;
+    ops.phi_les = sles;
+    ops.rho_cv = dles_cv;
+    ops.phi_cv = sles_cv;
+    ops.les_alpha = les_alpha;
     unsigned long long* um = p.variable_dt ? &st->umax : nullptr;
     unsigned long long* rm = (p.variable_dt && dens) ? &st->rhomax : nullptr;
     launch("x_nl", [&] { FFS_DISPATCH(launch_x_nl, g, w2, w1, ops, um, rm, tw_x, stream); });
@@ -1168,7 +1191,7 @@ struct ffs_sim {
     auto& sl = slots[slot];
     StatsAux aux{les ? tau : nullptr, dens ? rho : nullptr, gz, gdir, dens ? gval : 0.0, hv_args(),
                  dens ? hvd_args() : HvArgs{},
-                 pscal ? phi : nullptr};
+                 pscal ? phi : nullptr, dles ? flux : nullptr};
     launch("stats", [&] { FFS_DISPATCH(launch_stats, g, u, rhs, ke, aux, partials, red, stream); });
     HIPCHECK(hipMemcpyAsync(sl.red, red, kStatsSlots * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipMemcpyAsync(sl.st, st, sizeof(DevState), hipMemcpyDeviceToHost, stream));
@@ -1217,12 +1240,17 @@ struct ffs_sim {
     if (dens) {  // stats(::AbstractDensityStratification), density_stratification_types.jl:46-87
       const double rhodiss = alpha * (a[23] + a[24] + a[25]);
       double trhodiss = rhodiss;
-      double rhohdiss = 0.0;
+      double rhohdiss = 0.0, rhopr = 0.0;
+      if (dles) {  // scalar_les_stats of the total flux (density_stratification_types.jl:52-56)
+        rhopr = -a[38];
+        trhodiss += rhopr;
+      }
       if (p.density_hv) {  // scalar_hvis_stats (stats.jl:1086-1170): + rhohdiss
         rhohdiss = (p.density_hv == FFS_HV_HYPER ? p.density_hv_nu : -1.0) * a[37];
         trhodiss += rhohdiss;
       }
       other.insert(other.end(), {a[29], a[22], a[23], a[24], a[25], rhodiss, trhodiss, -gval * a[26]});
+      if (dles) other.push_back(rhopr);
       if (p.density_hv) other.push_back(rhohdiss);
     }
     if (les) {  // les_stats, stats.jl:679-705
@@ -1799,6 +1827,8 @@ int ffs_params_default(ffs_params* p) {
   p->density_hv = FFS_HV_NONE;
   p->density_hv_m = 2;
   p->gravity_direction = 2;
+  p->scalar_les_c = NAN;
+  p->density_les_c = NAN;
   return FFS_OK;
 }
 
@@ -1984,7 +2014,7 @@ int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n) {
       void* real = s->dalloc(5 * padded * tb);
       s->launch("inv_z_curl", [&] {
         FFS_DISPATCH_S(launch_inv_z, s->g, s->u, s->w1, s->nin, true, s->tw_z, s->stream, s->rho, s->phi,
-                       s->nsform ? 1 : 0);
+                       s->nsform ? 1 : 0, s->grad1(), s->grad2());
       });
       s->launch("inv_y", [&] { FFS_DISPATCH_S(launch_inv_y, s->g, s->w1, s->w2, s->nin, s->tw_y, s->stream); });
       const int s0 = (s->nsform ? 3 : 6) + (s->dens ? 1 : 0) + (s->pscal ? 1 : 0);
@@ -2148,6 +2178,7 @@ int ffs_stats_header(const ffs_sim* s, int table, char* buf, size_t cap) {
     h = "iteration,time,u1,u2,u3,u1p2,u2p2,u3p2,kh,kv,kaniso,k,dissh,dissv,diss,nlh,nlv,pressh,pressv,tnlh,tnlv,tdiss";
     if (s->pscal) h += ",scalar,scalarp2,dscalardxp2,dscalardyp2,dscalardzp2,scalardiss";
     if (s->dens) h += ",rho,rhop2,drhodxp2,drhodyp2,drhodzp2,rhodiss,trhodiss,bflux";
+    if (s->dens && s->dles) h += ",rhopr";
     if (s->dens && s->p.density_hv) h += ",rhohdiss";
     if (s->les) h += ",prh,prv,praniso,pr";
     if (s->plane) h += s->p.forcing == FFS_FORCING_A ? ",hp,vp" : ",hp,vp,tp";

@@ -103,6 +103,10 @@ class FfsParams(ctypes.Structure):
         ("les_s_model", ctypes.c_int32),
         ("les_test_width", ctypes.c_double),
         ("les_cmin", ctypes.c_double),
+        ("scalar_les", ctypes.c_int32),
+        ("scalar_les_c", ctypes.c_double),
+        ("density_les", ctypes.c_int32),
+        ("density_les_c", ctypes.c_double),
         ("gravity_direction", ctypes.c_int32),
         ("density_hv", ctypes.c_int32),
         ("density_hv_nu", ctypes.c_double),

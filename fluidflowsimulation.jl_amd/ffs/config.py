@@ -170,6 +170,10 @@ class Params:
     alvelius_c: float = 1.0
     alvelius_kf: float = 0.0
     hyperviscosity: str = None      # hyperViscosity: "spectralBarrier", or any other value: HyperViscosity
+    scalar_les: str = None          # scalarLesModel: "vorticityDiffusion", else EddyDiffusion (with lesModel)
+    scalar_les_c: float = None      # scalarVorticityCoefficient (None: 1.0)
+    density_les: str = None         # densityLesModel: "vorticityDiffusion" or any other value: EddyDiffusion
+    density_les_c: float = None     # densityVorticityCoefficient (None: 0.1)
     gravity_direction: str = "z"    # gravityDirection: the axis of g and of the density's mean gradient
     density_hyperviscosity: str = None  # densityHyperViscosity: the density's own, as hyperviscosity
     density_hv_nu: float = 0.0      # densityHyperViscosityCoefficient
@@ -236,6 +240,10 @@ class Params:
         p.sb_initk, p.sb_endk = self.sb_initk, self.sb_endk
         p.density_hv = hv_code(self.density_hyperviscosity)
         p.gravity_direction = {"x": 0, "y": 1, "z": 2}[self.gravity_direction]
+        p.scalar_les = 2 if self.scalar_les == "vorticityDiffusion" else (1 if self.scalar_les is not None else 0)
+        p.density_les = 0 if self.density_les is None else (2 if self.density_les == "vorticityDiffusion" else 1)
+        p.scalar_les_c = math.nan if self.scalar_les_c is None else self.scalar_les_c
+        p.density_les_c = math.nan if self.density_les_c is None else self.density_les_c
         p.density_hv_nu, p.density_hv_m = self.density_hv_nu, self.density_hv_m
         p.density_sb_initk, p.density_sb_endk = self.density_sb_initk, self.density_sb_endk
         p.passive_scalar = int(bool(self.passive_scalar))
@@ -318,11 +326,19 @@ def parameters(d):
             raise ValueError(f"gravityDirection {p.gravity_direction!r}: x, y or z")
     p.equation = opt("equation", str, None)  # simulationtypes.jl:265-273
     # keys whose features this build does not have: refused rather than ignored
-    for k, what in (("specFil", "the filtered statistics and spectra (Stats_fil.txt, *_fil spectra)"),
-                    ("densityLesModel", "the density's LES flux (EddyDiffusion, VorticityDiffusion)"),
-                    ("scalarLesModel", "the passive scalar's LES flux (EddyDiffusion, VorticityDiffusion)")):
-        if k in d:
-            raise NotImplementedError(f"{k}: {what} is not built")
+    if "specFil" in d:
+        raise NotImplementedError("specFil: the filtered statistics and spectra (Stats_fil.txt, *_fil spectra) "
+                                  "are not built")
+    # the scalars' LES fluxes (simulationtypes.jl:334-346, density_stratification_types.jl:176-184)
+    if "scalarLesModel" in d:
+        p.scalar_les = d["scalarLesModel"]
+        if p.scalar_les == "vorticityDiffusion":
+            p.scalar_les_c = opt("scalarVorticityCoefficient", float, None)
+    if "densityLesModel" in d:
+        p.density_les = d["densityLesModel"]
+        if p.density_les == "vorticityDiffusion":
+            p.density_les_c = opt("densityVorticityCoefficient", float, None)
+    known.update(("scalarLesModel", "densityLesModel"))
     if "passiveScalar" in d:  # simulationtypes.jl:320-352
         known.add("passiveScalar")
         p.passive_scalar = True

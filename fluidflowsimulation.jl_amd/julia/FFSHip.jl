@@ -67,6 +67,10 @@ mutable struct ffs_params
     les_s_model::Int32
     les_test_width::Float64
     les_cmin::Float64
+    scalar_les::Int32
+    scalar_les_c::Float64
+    density_les::Int32
+    density_les_c::Float64
     gravity_direction::Int32
     density_hv::Int32
     density_hv_nu::Float64
@@ -161,8 +165,15 @@ function params_from_global(d::AbstractDict)
         end
     end
     # keys whose features this build does not have: refused rather than ignored
-    for k in (:specFil, :densityLesModel, :scalarLesModel)
-        haskey(d, k) && error("$(k): not built (filtered statistics / scalar LES fluxes)")
+    haskey(d, :specFil) && error("specFil: the filtered statistics and spectra are not built")
+    # the scalars' LES fluxes (simulationtypes.jl:334-346, density_stratification_types.jl:176-184)
+    if haskey(d, :scalarLesModel) && d[:scalarLesModel] == "vorticityDiffusion"
+        p.scalar_les = Int32(2)
+        haskey(d, :scalarVorticityCoefficient) && (p.scalar_les_c = parse(Float64, d[:scalarVorticityCoefficient]))
+    end
+    if haskey(d, :densityLesModel)
+        p.density_les = Int32(d[:densityLesModel] == "vorticityDiffusion" ? 2 : 1)
+        haskey(d, :densityVorticityCoefficient) && (p.density_les_c = parse(Float64, d[:densityVorticityCoefficient]))
     end
     # the density's own hyperviscosity (density_stratification_types.jl:137-149)
     if haskey(d, :densityHyperViscosity)

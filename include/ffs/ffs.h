@@ -179,6 +179,15 @@ typedef struct ffs_params {
    * FFS_HV_*, densityHyperViscosityCoefficient, densityHyperViscosityM (2), densityInitk,
    * densityEndk (<= 0: the defaults of initk / endk).  It acts through the density's ETD3 operator
    * and adds the rhohdiss stats column. */
+  /* the scalars' LES fluxes (scalarLesModel / densityLesModel, les_real_calc.jl:120-136): 0 none,
+   * 1 EddyDiffusion (nu_t grad s), 2 VorticityDiffusion (+ (c Delta^2 / 2) w x grad s).  With an
+   * LES model the passive scalar always has one (0 reads as 1, as in the reference); the density's
+   * needs an LES model.  The coefficients: NaN selects scalarVorticityCoefficient's 1.0 and
+   * densityVorticityCoefficient's 0.1.  Pointwise LES models and the rotational form only. */
+  int32_t scalar_les;
+  double scalar_les_c;
+  int32_t density_les;
+  double density_les_c;
   int32_t gravity_direction;                /* gravityDirection: 0 x, 1 y, 2 z (default): the axis of g and
                                                of the density's mean gradient */
   int32_t density_hv;

@@ -156,13 +156,15 @@ class OracleSim:
     def __init__(self, grid, nu, dt=0.0, variable_dt=False, cfl=0.18, dt_stat=1 << 30, dt_spec=1 << 30,
                  forcing_kf=None, forcing_power=0.1, start=0, start_time=0.0, integrator="AB3", les_c=None,
                  les_delta=None, density=None, plane_forcing=None, hv=None, scalar=None, equation="rotational",
-                 les_model="Smagorinsky", les_cp=None, les_smodel=None, les_test_width=None, les_cmin=0.0):
+                 les_model="Smagorinsky", les_cp=None, les_smodel=None, les_test_width=None, les_cmin=0.0,
+                 density_les=None, scalar_les=None):
         """integrator: "AB3" (Adams_Bashforth3rdO), "Euler" (Euller) or "ETD3" (ETD3rdO);
         les_c: the LES model's constant (None: no LES), les_delta: filter width
         (default (2 pi L/nx)/cutoff, les_types.jl:366); les_model: Smagorinsky or one of
         oracle.les_models.MODELS, with les_cp (Silvis P constant, SandP pConstant) and les_smodel
         (SandP's sLesModel); dynamicSmagorinsky takes les_test_width (default twice the filter
-        width) and les_cmin;
+        width) and les_cmin; density_les / scalar_les: None, "eddy" or ("vorticity", c), the scalars'
+        LES fluxes (EddyDiffusion, VorticityDiffusion(c); les_real_calc.jl:120-136);
         density: dict(Pr=, gradient=, g=[, dir=][, hv=]) for the Boussinesq approximation, dir the
         gravity's axis (gravityDirection, 0 1 2, default 2), hv its own
         hyperviscosity (densityHyperViscosity, as hv below; none by default);
@@ -175,6 +177,7 @@ class OracleSim:
         self.nu = nu
         self.integrator = integrator
         self.les_c = les_c
+        self.density_les, self.scalar_les = density_les, scalar_les
         self.les_model, self.les_cp, self.les_smodel = les_model, les_cp, les_smodel
         if les_c is not None:
             delta = les_delta if les_delta else les_models.default_delta(les_model, les_smodel, grid.LX, grid.NRX,
@@ -322,10 +325,19 @@ class OracleSim:
         else:
             cross = np.array([ur[1] * wr[2] - ur[2] * wr[1], ur[2] * wr[0] - ur[0] * wr[2],
                               ur[0] * wr[1] - ur[1] * wr[0]])
+        def les_flux(spec, hat):  # the scalar's LES flux of calc_les! (les_real_calc.jl:120-136)
+            grad = [g.irfft(np.where(g.RANGE, 1j * kk * hat, 0)) for kk in (kx, ky, kz)]
+            nut = les_models.eddy_nu(self.les_model, Sr, wr, self.les_c, self.les_d2, self.les_smodel)
+            cv = None if spec == "eddy" else spec[1] * self.les_d2
+            return les_models.scalar_les_flux(nut, grad, wr, cv)
         if self.density is not None:
             fluxr = np.array([(-rhor) * ur[c] for c in range(3)])
+            if self.density_les is not None:
+                fluxr = fluxr + les_flux(self.density_les, self.rhoh)
         if self.scalar is not None:  # realspacecalculation!: flux = -phi u (models.jl:245-273)
             pfluxr = np.array([(-phir) * ur[c] for c in range(3)])
+            if self.scalar_les is not None:
+                pfluxr = pfluxr + les_flux(self.scalar_les, self.phih)
         if self.variable_dt:
             self.umax = float(np.max(np.abs(ur)))
             if self.density is not None:
@@ -731,11 +743,17 @@ class OracleSim:
             bf = -self.gval * float(np.sum(W * proj(u[self.gdir], r)))  # buoyancy_flux (:192-211)
             trhodiss = rhodiss
             extra = []
+            if self.density_les is not None:  # scalar_les_stats (stats.jl:1032-1053) of the total flux
+                f = self.fluxh
+                dv = 1j * (g.kx * f[0] + g.ky * f[1] + g.kz * f[2])
+                rhopr = -float(np.sum(np.where(g.RANGE, W * proj(dv, r), 0.0)))
+                trhodiss += rhopr
+                extra.append(rhopr)
             if self.density.get("hv") is not None:  # scalar_hvis_stats (stats.jl:1086-1170): rhohdiss
                 wgt = np.where(g.RANGE, g.W * self.hv_weight(self.density["hv"]), 0.0)
                 rhohdiss = float(np.sum(wgt * np.abs(self.rhoh) ** 2))
                 trhodiss += rhohdiss
-                extra = [rhohdiss]
+                extra.append(rhohdiss)
             other += [rho, rho2, *drd, rhodiss, trhodiss, bf, *extra]
         if self.les_c is not None:
             # les_stats (stats.jl:679-705)

@@ -238,3 +238,35 @@ def dynamic_stress(grid, uh, S, Sr, ur, d2, td2, cmin, cstar=None, wh=None, wr=N
         cp = filter_real(grid, cp, td2)
         t = t + five(scale(lie, np.maximum(-0.8, np.minimum(0.8, cp)) * d2))
     return t, c, cp
+
+
+def eddy_nu(model, s5, w3, c, d2, smodel=None):
+    """The eddy viscosity nu_t that calc_les! leaves for the scalars' LES fluxes (les_real_calc.jl:74-91,
+    120-136): Smagorinsky alpha |S| (alpha = c c Delta^2), Vreman's, productionViscosity's, Silvis's nu_t
+    (not its P coefficient), stableNonLinear's nu (not mnu); SandP its S model's, 0 for fakeSmagorinsky."""
+    S = sym_full(s5)
+    if model == "SandP":
+        return 0.0 * s5[0] if smodel == "fakeSmagorinsky" else eddy_nu(smodel, s5, w3, c, d2)
+    if model == "Smagorinsky":
+        return (c * c * d2) * np.sqrt(2 * ddot(S, S))
+    if model == "Vreman":
+        return vreman_nu(S, w3, c, d2)
+    if model in ("productionViscosity", "stableNonLinear"):
+        x, a = production_x(S, cross_matrix([-0.5 * v for v in w3]), c, d2)
+        return np.where(a == 0.0, 0.0, np.maximum(0.0, x))
+    if model == "Silvis":
+        Sw = [sum(S[i][j] * w3[j] for j in range(3)) for i in range(3)]
+        ss = ddot(S, S)
+        f = np.sqrt(Sw[0] ** 2 + Sw[1] ** 2 + Sw[2] ** 2) / (np.sqrt(ss) * np.sqrt(w3[0] ** 2 + w3[1] ** 2 + w3[2] ** 2))
+        return c * c * d2 * f * f * f * np.sqrt(ss) / 2
+    raise ValueError(f"LES model {model!r} has no pointwise eddy viscosity")
+
+
+def scalar_les_flux(nut, grad, w3, cv):
+    """EddyDiffusion nu_t grad s, plus (c Delta^2 / 2) w x grad s for VorticityDiffusion (cv = c Delta^2,
+    None for EddyDiffusion) (les_real_calc.jl:120-136)."""
+    f = [nut * g for g in grad]
+    if cv is not None:
+        cr = (w3[1] * grad[2] - w3[2] * grad[1], w3[2] * grad[0] - w3[0] * grad[2], w3[0] * grad[1] - w3[1] * grad[0])
+        f = [f[i] + (cv / 2) * cr[i] for i in range(3)]
+    return np.array(f)

@@ -195,10 +195,21 @@ def test_gravity_direction_key():
     assert _lib.default_params().gravity_direction == 2
 
 
-@pytest.mark.parametrize("key,value", [("specFil", "2"), ("densityLesModel", "eddyDiffusion"),
-                                       ("scalarLesModel", "vorticityDiffusion")])
+@pytest.mark.parametrize("key,value", [("specFil", "2")])
 def test_unbuilt_keys_are_refused(key, value):
     base = {"nx": "16", "ny": "16", "nz": "16", "xDomainSize": "1.0", "yDomainSize": "1.0", "zDomainSize": "1.0",
             "kinematicViscosity": "1e-3", "dtStat": "10", "dtSpec": "100", "writeTime": "100"}
     with pytest.raises(NotImplementedError, match=key):
         config.parameters(dict(base, **{key: value}))
+
+
+def test_scalar_les_keys():
+    from ffs import _lib
+    base = {"nx": "16", "ny": "16", "nz": "16", "xDomainSize": "1.0", "yDomainSize": "1.0", "zDomainSize": "1.0",
+            "kinematicViscosity": "1e-3", "dtStat": "10", "dtSpec": "100", "writeTime": "100", "lesModel": "Vreman"}
+    c = config.parameters(dict(base, scalarLesModel="vorticityDiffusion", scalarVorticityCoefficient="1.3",
+                               densityLesModel="eddyDiffusion")).to_c()
+    assert (c.scalar_les, c.scalar_les_c, c.density_les) == (2, 1.3, 1) and math.isnan(c.density_les_c)
+    c = config.parameters(dict(base, densityLesModel="vorticityDiffusion")).to_c()
+    assert (c.scalar_les, c.density_les) == (0, 2) and math.isnan(c.density_les_c)
+    assert _lib.default_params().scalar_les == 0 and math.isnan(_lib.default_params().scalar_les_c)

@@ -130,10 +130,12 @@ def test_passive_scalar_graph_replay_and_resume_bitwise():
                     assert np.array_equal(x, y)
 
 
-def test_passive_scalar_with_les_is_refused():
+def test_passive_scalar_with_dynamic_les_is_refused():
+    """With an LES model the scalar takes an eddy-diffusion flux (tests/test_gpu_scalar_les.py); the dynamic
+    models' eddy viscosity is not carried to the scalars."""
     p = ffs.Params(nx=16, ny=16, nz=16, nu=5e-3, dt=1e-3, variable_dt=False, dt_stat=10, dt_spec=10, write_time=10,
-                   passive_scalar=True, scalar_pr=1.0, les_model="Smagorinsky")
-    with pytest.raises(_lib.FfsError, match="passive scalar's LES"):
+                   passive_scalar=True, scalar_pr=1.0, les_model="dynamicSmagorinsky")
+    with pytest.raises(_lib.FfsError, match="pointwise LES models only"):
         ffs.Simulation(p)
 
 
