@@ -226,6 +226,11 @@ void launch_fwd_z_update(const Geo& g, const void* b1, void* u, void* rhs, const
 // the Smagorinsky stress in real space: s[5] padded real S fields (xx xy xz yy yz) -> c2a |S| S in place
 template <typename T>
 void launch_les_tau_real(T* s5, size_t n_padded, double c2a, hipStream_t s);
+// the density's LES flux alone (its real-space field at writeoutput, output.jl:22-26): nu_t grad rho
+// [+ (cv/2) w x grad rho] from S (5), w (3) and grad rho (3) padded real fields into out3
+template <typename T>
+void launch_les_scalar_flux_real(const T* s5, const T* w3, const T* g3, T* out3, size_t n_padded, const LesArgs& l,
+                                 double alpha, int les, double cv, hipStream_t s);
 // the same for a model that reads the vorticity: w3[3] padded real w fields
 template <typename T>
 void launch_les_model_real(T* s5, const T* w3, size_t n_padded, const LesArgs& l, hipStream_t s);

@@ -1585,6 +1585,37 @@ void launch_les_tau_real(T* s5, size_t n_padded, double c2a, hipStream_t s) {
   k_les_tau_real<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(s5, n_padded, c2a);
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) k_les_scalar_flux_real(const T* __restrict__ s5, const T* __restrict__ w3,
+                                                              const T* __restrict__ g3, T* __restrict__ out3,
+                                                              size_t n, LesArgs l, double alpha, int les, double cv) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T sv[5] = {s5[i], s5[n + i], s5[2 * n + i], s5[3 * n + i], s5[4 * n + i]};
+  const T wv[3] = {w3[i], w3[n + i], w3[2 * n + i]};
+  const T gv[3] = {g3[i], g3[n + i], g3[2 * n + i]};
+  T nut;
+  if (l.model == LES_SMAGORINSKY) {
+    const T szz = -sv[0] - sv[3];
+    const T ss = sv[0] * sv[0] + sv[3] * sv[3] + szz * szz + (T)2 * (sv[1] * sv[1] + sv[2] * sv[2] + sv[4] * sv[4]);
+    nut = (T)alpha * sqrt((T)2 * ss);
+  } else {
+    T tv[5];
+    les_stress<T>(l, sv, wv, tv, &nut);
+  }
+  T f[3];
+  scalar_flux<T>((T)0, (T)0, (T)0, (T)0, les, nut, gv, wv, (T)cv, f);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out3[c * n + i] = f[c];
+}
+
+template <typename T>
+void launch_les_scalar_flux_real(const T* s5, const T* w3, const T* g3, T* out3, size_t n_padded, const LesArgs& l,
+                                 double alpha, int les, double cv, hipStream_t s) {
+  k_les_scalar_flux_real<T><<<(unsigned)((n_padded + 255) / 256), 256, 0, s>>>(s5, w3, g3, out3, n_padded, l, alpha,
+                                                                               les, cv);
+}
+
 // calc_les! of a model that reads w, on padded real S and w fields (S overwritten by t)
 template <typename T>
 __global__ void __launch_bounds__(256) k_les_model_real(T* __restrict__ s5, const T* __restrict__ w3, size_t n,
@@ -2997,6 +3028,8 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   template void launch_div_uu<T>(const Geo&, const void*, void*, hipStream_t);                                     \
   template void launch_les_tau_real<T>(T*, size_t, double, hipStream_t);                                          \
   template void launch_les_model_real<T>(T*, const T*, size_t, const LesArgs&, hipStream_t);                      \
+  template void launch_les_scalar_flux_real<T>(const T*, const T*, const T*, T*, size_t, const LesArgs&, double,   \
+                                               int, double, hipStream_t);                                          \
   template void launch_dyn_prep<T>(const Geo&, const void*, void*, double, hipStream_t, int);                      \
   template void launch_dyn_products<T>(const T*, const T*, T*, T*, size_t, double, hipStream_t, const T*,          \
                                        const DynP&);                                                               \

@@ -939,6 +939,9 @@ struct ffs_sim {
     else dyn_tau<double>();
     if (store) forward_from_real(rptr(dreal, DR_SA), 5, tau);
     if (!store) {
+      // the read-out's coefficients: c at DR_C, cp at DR_M (free once the coefficient is formed)
+      if (pio) HIPCHECK(hipMemcpyAsync(rptr(dreal, DR_C), rptr(dreal, DR_PC), rb, hipMemcpyDeviceToDevice, stream));
+      if (dynp) HIPCHECK(hipMemcpyAsync(rptr(dreal, DR_M), rptr(dreal, DR_CP), rb, hipMemcpyDeviceToDevice, stream));
       if (pio) {
         HIPCHECK(hipMemcpyAsync(rptr(dreal, DR_PC), rptr(dreal, DR_SAVE), 2 * rb, hipMemcpyDeviceToDevice, stream));
         HIPCHECK(hipMemcpyAsync(pio_dtm1, &sd_save_dtm1, sizeof(double), hipMemcpyHostToDevice, stream));
@@ -1948,10 +1951,16 @@ int ffs_seed_density(ffs_sim* s, uint64_t seed, double k0, double rms) {
 int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n) {
   if (!s || !host) return FFS_ERR_ARG;
   return guard(s, [&] {
-    if (field < FFS_FIELD_U1 || field > FFS_FIELD_T23) throw FfsError(FFS_ERR_ARG, "unknown field");
+    if (field < FFS_FIELD_U1 || field > FFS_FIELD_CP) throw FfsError(FFS_ERR_ARG, "unknown field");
+    if (field >= FFS_FIELD_DFLUX1) {  // real-space read-outs formed from the current state
+      if (space != FFS_SPACE_REAL) throw FfsError(FFS_ERR_ARG, "dflux, cs and cp are real-space fields");
+      if (field <= FFS_FIELD_DFLUX3 && !s->dles) throw FfsError(FFS_ERR_ARG, "the run has no density LES flux");
+      if (field == FFS_FIELD_CS && !s->dyn) throw FfsError(FFS_ERR_ARG, "the run has no dynamic LES model");
+      if (field == FFS_FIELD_CP && !s->dynp) throw FfsError(FFS_ERR_ARG, "the run has no dynamic P coefficient");
+    }
     if (field == FFS_FIELD_RHO && !s->dens) throw FfsError(FFS_ERR_ARG, "the run has no density field");
     if (field == FFS_FIELD_SCALAR && !s->pscal) throw FfsError(FFS_ERR_ARG, "the run has no passive scalar");
-    const bool stress = field >= FFS_FIELD_T11;
+    const bool stress = field >= FFS_FIELD_T11 && field <= FFS_FIELD_T23;
     if (stress && !s->les) throw FfsError(FFS_ERR_ARG, "the run has no LES stress");
     if (space != FFS_SPACE_REAL && space != FFS_SPACE_FOURIER && space != FFS_SPACE_FOURIER_TO_REAL)
       throw FfsError(FFS_ERR_ARG, "unknown space");
@@ -2004,6 +2013,44 @@ int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n) {
             host[2 * dsti] = c[2 * d];
             host[2 * dsti + 1] = c[2 * d + 1];
           }
+    } else if (field == FFS_FIELD_CS || field == FFS_FIELD_CP) {
+      s->dynamic_les(false);
+      fetch_real(s->rptr(s->dreal, field == FFS_FIELD_CS ? ffs_sim::DR_C : ffs_sim::DR_M));
+    } else if (field >= FFS_FIELD_DFLUX1) {
+      // realspace!: S, w and i k rho to real space, then the LES flux alone (les_real_calc.jl:120-127)
+      s->launch("inv_z_curl", [&] {
+        FFS_DISPATCH_S(launch_inv_z, s->g, s->u, s->w1, s->nin, true, s->tw_z, s->stream, s->rho, s->phi,
+                       s->nsform ? 1 : 0, s->grad1(), s->grad2());
+      });
+      s->launch("inv_y", [&] { FFS_DISPATCH_S(launch_inv_y, s->g, s->w1, s->w2, s->nin, s->tw_y, s->stream); });
+      const int s0 = 6 + 1 + (s->pscal ? 1 : 0);  // S, then i k rho (the density's gradient comes first)
+      const size_t S2 = (size_t)s->g.sx * s->g.ny * s->g.nz;
+      void* real = s->dalloc(14 * padded * tb);  // S (5), w (3), grad rho (3), the flux (3)
+      auto R = [&](int k) { return (char*)real + (size_t)k * padded * tb; };
+      if (s->fp32) {
+        s->launch("x_c2r", [&] { launch_x_c2r<float>(s->g, s->cptr(s->w2, s0 * S2), (float*)R(0), 5, s->tw_x, s->stream); });
+        s->launch("x_c2r", [&] { launch_x_c2r<float>(s->g, s->cptr(s->w2, 3 * S2), (float*)R(5), 3, s->tw_x, s->stream); });
+        s->launch("x_c2r", [&] {
+          launch_x_c2r<float>(s->g, s->cptr(s->w2, (s0 + 5) * S2), (float*)R(8), 3, s->tw_x, s->stream);
+        });
+        s->launch("les_flux_real", [&] {
+          launch_les_scalar_flux_real<float>((const float*)R(0), (const float*)R(5), (const float*)R(8), (float*)R(11),
+                                             padded, s->lesm, s->les_alpha, s->dles, s->dles_cv, s->stream);
+        });
+      } else {
+        s->launch("x_c2r", [&] { launch_x_c2r<double>(s->g, s->cptr(s->w2, s0 * S2), (double*)R(0), 5, s->tw_x, s->stream); });
+        s->launch("x_c2r", [&] { launch_x_c2r<double>(s->g, s->cptr(s->w2, 3 * S2), (double*)R(5), 3, s->tw_x, s->stream); });
+        s->launch("x_c2r", [&] {
+          launch_x_c2r<double>(s->g, s->cptr(s->w2, (s0 + 5) * S2), (double*)R(8), 3, s->tw_x, s->stream);
+        });
+        s->launch("les_flux_real", [&] {
+          launch_les_scalar_flux_real<double>((const double*)R(0), (const double*)R(5), (const double*)R(8),
+                                              (double*)R(11), padded, s->lesm, s->les_alpha, s->dles, s->dles_cv,
+                                              s->stream);
+        });
+      }
+      fetch_real(R(11 + (field - FFS_FIELD_DFLUX1)));
+      release(real, 14 * padded * tb);
     } else if (stress && space == FFS_SPACE_REAL && s->dyn) {
       // the dynamic model's stress of the current velocity, as the step forms it
       s->dynamic_les(false);

@@ -39,6 +39,7 @@ FFS_EQ_ROTATIONAL, FFS_EQ_NS = 0, 1
 FFS_FSTATE_R, FFS_FSTATE_EM, FFS_FSTATE_KH, FFS_FSTATE_EF, FFS_FSTATE_N = 0, 1, 2, 3, 4
 FFS_FIELD_U1, FFS_FIELD_U2, FFS_FIELD_U3, FFS_FIELD_RHO, FFS_FIELD_SCALAR = 0, 1, 2, 3, 4
 FFS_FIELD_T11, FFS_FIELD_T12, FFS_FIELD_T13, FFS_FIELD_T22, FFS_FIELD_T23 = 5, 6, 7, 8, 9
+FFS_FIELD_DFLUX1, FFS_FIELD_DFLUX2, FFS_FIELD_DFLUX3, FFS_FIELD_CS, FFS_FIELD_CP = 10, 11, 12, 13, 14
 FFS_SPACE_REAL, FFS_SPACE_FOURIER, FFS_SPACE_FOURIER_TO_REAL = 0, 1, 2
 FFS_SCALAR_ITERATION, FFS_SCALAR_TIME, FFS_SCALAR_DT, FFS_SCALAR_UMAX = 0, 1, 2, 3
 FFS_TABLE_STATS, FFS_TABLE_SCALES, FFS_TABLE_NUMBERS = 0, 1, 2

@@ -324,7 +324,9 @@ class Simulation:
 
     def writeoutput(self, directory=".", closing=False):
         """writeoutput (output.jl:1-67): u1.$it u2.$it u3.$it (padded real), scalar.$it with the passive
-        scalar, rho.$it with density, and
+        scalar, rho.$it with density, dflux1..3.$it with a density LES model (its real-space LES flux
+        nu_t grad rho (+ the vorticity term) of the current state), cs.$it (and cp.$it) with a dynamic
+        model (the averaged coefficient the current velocity gives, with Piomelli's extrapolation), and
         the forcing state: R.$it (the oscillators), N.$it (nRf, cNRf), Eh.$it = (mean |k_h|, Ef)
         rows (rf, cRf, nRf, cNRf); with Smagorinsky the stress T11 T12 T13 T22 T23: the real-space
         product of the current velocity, or with `closing` the stored transform back in real space
@@ -337,11 +339,21 @@ class Simulation:
                         self.get_field(_lib.FFS_FIELD_SCALAR, _lib.FFS_SPACE_REAL))
         if self.params.density:
             write_field(os.path.join(directory, f"rho.{it}"), self.get_field(_lib.FFS_FIELD_RHO, _lib.FFS_SPACE_REAL))
+            if self.params.density_les is not None and self.params.les_model not in (None, "", "noLesModel"):
+                for c in range(3):
+                    write_field(os.path.join(directory, f"dflux{c + 1}.{it}"),
+                                self.get_field(_lib.FFS_FIELD_DFLUX1 + c, _lib.FFS_SPACE_REAL))
         if self.params.les_model not in (None, "", "noLesModel"):
             space = _lib.FFS_SPACE_FOURIER_TO_REAL if closing else _lib.FFS_SPACE_REAL
             for name, f in (("T11", _lib.FFS_FIELD_T11), ("T12", _lib.FFS_FIELD_T12), ("T13", _lib.FFS_FIELD_T13),
                             ("T22", _lib.FFS_FIELD_T22), ("T23", _lib.FFS_FIELD_T23)):
                 write_field(os.path.join(directory, f"{name}.{it}"), self.get_field(f, space))
+            lm = self.params.les_model
+            if lm in ("dynamicSmagorinsky", "piomelliSmagorinsky", "dynamicSandP", "piomelliSandP"):
+                write_field(os.path.join(directory, f"cs.{it}"), self.get_field(_lib.FFS_FIELD_CS, _lib.FFS_SPACE_REAL))
+                if lm in ("dynamicSandP", "piomelliSandP"):
+                    write_field(os.path.join(directory, f"cp.{it}"),
+                                self.get_field(_lib.FFS_FIELD_CP, _lib.FFS_SPACE_REAL))
         forcing = self.params.forcing
         if forcing in RF_FAMILY:
             R = self.forcing_state(_lib.FFS_FSTATE_R)

@@ -451,10 +451,17 @@ function writeoutput(s::FFSSimulation; closing::Bool = false)
     end
     s.p.passive_scalar != 0 && write("scalar.$it", get_field(s, 4))
     s.p.density != 0 && write("rho.$it", get_field(s, 3))
+    if s.p.density != 0 && s.p.density_les != 0 && s.p.les_model != 0   # the density's LES flux
+        for c in 1:3
+            write("dflux$c.$it", get_field(s, 9 + c))
+        end
+    end
     if s.p.les_model != 0   # the stress: real-space product, or real!(tau) at the closing output
         for (name, f) in (("T11", 5), ("T12", 6), ("T13", 7), ("T22", 8), ("T23", 9))
             write("$name.$it", get_field(s, f; space = closing ? 2 : 0))
         end
+        s.p.les_model >= 8 && write("cs.$it", get_field(s, 13))    # the dynamic models' coefficient
+        s.p.les_model >= 10 && write("cp.$it", get_field(s, 14))   # and SandP's P coefficient
     end
     # the forcing state files (output.jl:44-66): R, N (nRf, cNRf), Eh = (mean |k_h|, Ef) (rf, cRf, nRf, cNRf)
     f = s.p.forcing

@@ -102,6 +102,13 @@ extern "C" {
 #define FFS_FIELD_T13 7
 #define FFS_FIELD_T22 8
 #define FFS_FIELD_T23 9
+/* real space only, formed from the current state as the next step forms them (writeoutput,
+ * output.jl:22-40): the density's LES flux (densityLesModel) and the dynamic models' coefficients */
+#define FFS_FIELD_DFLUX1 10
+#define FFS_FIELD_DFLUX2 11
+#define FFS_FIELD_DFLUX3 12
+#define FFS_FIELD_CS 13 /* the dynamic models' averaged coefficient c */
+#define FFS_FIELD_CP 14 /* dynamicSandP / piomelliSandP: the P coefficient */
 
 /* spaces for get/set_field */
 #define FFS_SPACE_REAL 0    /* padded real (2*(nx/2+1), ny, nz), column major; or unpadded (nx,ny,nz) on input */

@@ -308,7 +308,7 @@ class OracleSim:
                 pmode = {"dynamicSandP": "dyn", "piomelliSandP": "pio"}.get(self.les_model)
                 taur, c, cp = les_models.dynamic_stress(g, u, S, Sr, ur, self.les_d2, self.les_td2, self.les_cmin,
                                                         cstar=cstar, wh=w, wr=wr, pmode=pmode, cpstar=cpstar)
-                self.les_coef = c
+                self.les_coef, self.les_pcoef = c, cp  # cs and cp of writeoutput (output.jl:36-41)
                 if cstar is not None:
                     self.pio_c = c
                 if cpstar is not None:
@@ -333,7 +333,8 @@ class OracleSim:
         if self.density is not None:
             fluxr = np.array([(-rhor) * ur[c] for c in range(3)])
             if self.density_les is not None:
-                fluxr = fluxr + les_flux(self.density_les, self.rhoh)
+                self.dfluxr = les_flux(self.density_les, self.rhoh)  # dflux1..3 (output.jl:22-26)
+                fluxr = fluxr + self.dfluxr
         if self.scalar is not None:  # realspacecalculation!: flux = -phi u (models.jl:245-273)
             pfluxr = np.array([(-phir) * ur[c] for c in range(3)])
             if self.scalar_les is not None:
