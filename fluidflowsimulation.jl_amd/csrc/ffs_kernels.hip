@@ -155,7 +155,9 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) 
 // column; LDS holds one [N][TX] tile.
 enum ColKind { INV_Z = 0, INV_Y = 1, FWD_Y = 2, FWD_Z = 3 };
 
-template <typename T, int N, int KIND, bool CURL>
+// GRAD: the curl pass also forms i k s of the scalars with an LES flux (a separate instantiation,
+// so the plain step's curl pass carries none of that selection in its loads)
+template <typename T, int N, int KIND, bool CURL, bool GRAD = false>
 __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ out,
                                             ColAux<T> aux, const C<T>* __restrict__ tw_g, int nzero, T scale) {
   constexpr int TX = col_tx<T>(N);
@@ -240,7 +242,7 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
           if (e == 0) return aux.phi[idx];
           --e;
         }
-        if (e >= 5) {  // i k s of the scalars with an LES flux (fourierspacep1!, models.jl:110-116)
+        if (GRAD && e >= 5) {  // i k s of the scalars with an LES flux (fourierspacep1!, models.jl:110-116)
           e -= 5;
           const C<T> sv = e < 3 ? aux.grad1[idx] : aux.grad2[idx];
           const int d = e < 3 ? e : e - 3;
@@ -318,7 +320,7 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
   }
 }
 
-template <typename T, int N, int KIND, bool CURL>
+template <typename T, int N, int KIND, bool CURL, bool GRAD = false>
 static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, const void* tw, int nzero, double scale,
                     hipStream_t s, const ColAux<T>& aux = ColAux<T>{}) {
   constexpr int TX = col_tx<T>(N);
@@ -327,13 +329,14 @@ static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, con
   const size_t lds = (size_t)(N * TX + tw_lds<T>(N)) * sizeof(C<T>);
   static bool attr = false;
   if (!attr) {
-    hipFuncSetAttribute((const void*)&k_col<T, N, KIND, CURL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipFuncSetAttribute((const void*)&k_col<T, N, KIND, CURL, GRAD>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)lds);
     attr = true;
   }
   const int ntile = (g.mx + TX - 1) / TX * nb;
   const dim3 grid = (CURL && FFS_CURL_SPLIT) ? dim3((unsigned)(nzero * ((ntile + kXcd - 1) / kXcd * kXcd)))
                                              : dim3((g.mx + TX - 1) / TX, nb, CURL ? 1 : nf);
-  k_col<T, N, KIND, CURL><<<grid, TX * TP, lds, s>>>(g, (const C<T>*)in, (C<T>*)out, aux, (const C<T>*)tw, nzero,
+  k_col<T, N, KIND, CURL, GRAD><<<grid, TX * TP, lds, s>>>(g, (const C<T>*)in, (C<T>*)out, aux, (const C<T>*)tw, nzero,
                                                     (T)scale);
 }
 
@@ -355,7 +358,8 @@ void launch_inv_z(const Geo& g, const void* u, void* a1, int nf, bool curl, cons
   switch (g.nz) {
 #define X(n)                                                                                    \
   case n:                                                                                       \
-    if (curl) run_col<T, n, INV_Z, true>(g, u, a1, 1, g.my, tw, nf, 1.0, s, aux);               \
+    if (curl && grad1) run_col<T, n, INV_Z, true, true>(g, u, a1, 1, g.my, tw, nf, 1.0, s, aux); \
+    else if (curl) run_col<T, n, INV_Z, true>(g, u, a1, 1, g.my, tw, nf, 1.0, s, aux);          \
     else run_col<T, n, INV_Z, false>(g, u, a1, nf, g.my, tw, 0, 1.0, s, aux);                   \
     break;
     FFS_SIZES(X)
