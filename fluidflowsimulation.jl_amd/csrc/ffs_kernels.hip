@@ -633,41 +633,49 @@ __global__ void __launch_bounds__(256) k_x_gen(Geo g, const C<T>* __restrict__ a
   for (int w = tid; w < ROWS * N; w += NTHR) {
     const int r = w / N, n = w % N;
     C<T>* Z = lds + (r * NSL) * LD;
-    T in[2 * NSL], out[2 * NSL];
+    // real slot k of this point is component k & 1 of Z[(k >> 1) * LD + n]; the slots a field occupies
+    // depend on the run's options, so values are read from and written to LDS at those slots directly
+    // (a register array indexed at run time would live in private memory).  Every input is read
+    // before the first output is written: inputs and outputs share the slots.
+    T* zr = reinterpret_cast<T*>(Z);
+    auto IN = [&](int k) -> T { return zr[2 * ((k >> 1) * LD + n) + (k & 1)]; };
+    auto OUT = [&](int k, T v) { zr[2 * ((k >> 1) * LD + n) + (k & 1)] = v; };
+    const T ux = IN(0), uy = IN(1), uz = IN(2);
+    const int i0 = ops.ns ? 3 : 6;  // rho, phi, then S, then the scalars' gradients
+    const T wv3[3] = {ops.ns ? (T)0 : IN(3), ops.ns ? (T)0 : IN(4), ops.ns ? (T)0 : IN(5)};
+    const int i_rho = i0, i_phi = i0 + (ops.rho ? 1 : 0);
+    const int i_s = i_phi + (ops.phi ? 1 : 0);
+    const T rho = ops.rho ? IN(i_rho) : (T)0;
+    const T phi = ops.phi ? IN(i_phi) : (T)0;
+    const bool need_s = ops.les || ops.rho_les || ops.phi_les;
+    T sv[5];
 #pragma unroll
-    for (int j = 0; j < NSL; ++j) {
-      const C<T> z = Z[j * LD + n];
-      in[2 * j] = z.x;
-      in[2 * j + 1] = z.y;
-      out[2 * j] = 0;
-      out[2 * j + 1] = 0;
+    for (int e = 0; e < 5; ++e) sv[e] = need_s ? IN(i_s + e) : (T)0;
+    T g_rho[3], g_phi[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      g_rho[c] = ops.rho_les ? IN(i_s + 5 + c) : (T)0;
+      g_phi[c] = ops.phi_les ? IN(i_s + 5 + (ops.rho_les ? 3 : 0) + c) : (T)0;
     }
-    const T ux = in[0], uy = in[1], uz = in[2];
-    int o, i;
+    int o;
     if (ops.ns) {  // traceless(symouter(u, -u)) (models.jl:262): xx xy xz yy yz
       const T xx = -(ux * ux), yy = -(uy * uy), zz = -(uz * uz);
       const T t = (xx + yy + zz) / (T)3;
-      out[0] = xx - t;
-      out[1] = -(ux * uy);
-      out[2] = -(ux * uz);
-      out[3] = yy - t;
-      out[4] = -(uy * uz);
+      OUT(0, xx - t);
+      OUT(1, -(ux * uy));
+      OUT(2, -(ux * uz));
+      OUT(3, yy - t);
+      OUT(4, -(uy * uz));
       o = 5;
-      i = 3;
     } else {
-      const T wx = in[3], wy = in[4], wz = in[5];
-      out[0] = uy * wz - uz * wy;
-      out[1] = uz * wx - ux * wz;
-      out[2] = ux * wy - uy * wx;
+      OUT(0, uy * wv3[2] - uz * wv3[1]);
+      OUT(1, uz * wv3[0] - ux * wv3[2]);
+      OUT(2, ux * wv3[1] - uy * wv3[0]);
       o = 3;
-      i = 6;
     }
-    // the eddy viscosity and the gradients of the scalars' LES fluxes (they follow S in the inputs)
+    // the eddy viscosity of the scalars' LES fluxes
     T nut = 0;
-    const T wv3[3] = {in[3], in[4], in[5]};
-    const int i_s = i + (ops.rho ? 1 : 0) + (ops.phi ? 1 : 0);
     if (ops.rho_les || ops.phi_les) {
-      const T sv[5] = {in[i_s], in[i_s + 1], in[i_s + 2], in[i_s + 3], in[i_s + 4]};
       if (ops.lm.model == LES_SMAGORINSKY) {  // nu_t = alpha |S| (les_real_calc.jl:77-79)
         const T szz = -sv[0] - sv[3];
         const T ss = sv[0] * sv[0] + sv[3] * sv[3] + szz * szz + (T)2 * (sv[1] * sv[1] + sv[2] * sv[2] + sv[4] * sv[4]);
@@ -677,40 +685,37 @@ __global__ void __launch_bounds__(256) k_x_gen(Geo g, const C<T>* __restrict__ a
         les_stress<T>(ops.lm, sv, wv3, tv, &nut);
       }
     }
-    const T* g_rho = in + i_s + 5;
-    const T* g_phi = in + i_s + 5 + (ops.rho_les ? 3 : 0);
     if (ops.rho) {
-      const T rho = in[i++];
-      scalar_flux<T>(rho, ux, uy, uz, ops.rho_les, nut, g_rho, wv3, (T)ops.rho_cv, out + o);
+      T f[3];
+      scalar_flux<T>(rho, ux, uy, uz, ops.rho_les, nut, g_rho, wv3, (T)ops.rho_cv, f);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) OUT(o + c, f[c]);
       o += 3;
       if (row0 + r < nrows) rmax = nanmax(rmax, (double)fabs(rho));
     }
     if (ops.phi) {  // -phi u (the passive scalar enters no time-step bound)
-      const T phi = in[i++];
-      scalar_flux<T>(phi, ux, uy, uz, ops.phi_les, nut, g_phi, wv3, (T)ops.phi_cv, out + o);
+      T f[3];
+      scalar_flux<T>(phi, ux, uy, uz, ops.phi_les, nut, g_phi, wv3, (T)ops.phi_cv, f);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) OUT(o + c, f[c]);
       o += 3;
     }
     if (ops.les && ops.lm.model != LES_SMAGORINSKY) {  // the models that read w (slots 3..5)
-      const T sv[5] = {in[i], in[i + 1], in[i + 2], in[i + 3], in[i + 4]};
-      const T wv[3] = {in[3], in[4], in[5]};
       T tv[5];
-      les_stress<T>(ops.lm, sv, wv, tv);
+      les_stress<T>(ops.lm, sv, wv3, tv);
 #pragma unroll
-      for (int e = 0; e < 5; ++e) out[o++] = tv[e];
+      for (int e = 0; e < 5; ++e) OUT(o + e, tv[e]);
+      o += 5;
     } else if (ops.les) {
-      const T sxx = in[i], sxy = in[i + 1], sxz = in[i + 2], syy = in[i + 3], syz = in[i + 4];
-      const T szz = -sxx - syy;
-      const T ss = sxx * sxx + syy * syy + szz * szz + (T)2 * (sxy * sxy + sxz * sxz + syz * syz);
+      const T szz = -sv[0] - sv[3];
+      const T ss = sv[0] * sv[0] + sv[3] * sv[3] + szz * szz + (T)2 * (sv[1] * sv[1] + sv[2] * sv[2] + sv[4] * sv[4]);
       const T f = (T)ops.les_2alpha * sqrt((T)2 * ss);
-      out[o++] = f * sxx;
-      out[o++] = f * sxy;
-      out[o++] = f * sxz;
-      out[o++] = f * syy;
-      out[o++] = f * syz;
-    }
-    if (ops.ke) out[o++] = (T)0.5 * (ux * ux + uy * uy + uz * uz);
 #pragma unroll
-    for (int j = 0; j < NSL; ++j) Z[j * LD + n] = mk<T>(out[2 * j], out[2 * j + 1]);
+      for (int e = 0; e < 5; ++e) OUT(o + e, f * sv[e]);
+      o += 5;
+    }
+    if (ops.ke) OUT(o++, (T)0.5 * (ux * ux + uy * uy + uz * uz));
+    for (int k = o; k < 2 * NSL; ++k) OUT(k, (T)0);  // the unused slots transform as zeros
     if (row0 + r < nrows) lmax = nanmax(lmax, (double)nanmax(fabs(ux), nanmax(fabs(uy), fabs(uz))));
   }
   __syncthreads();
