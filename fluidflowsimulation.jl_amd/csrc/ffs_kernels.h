@@ -106,7 +106,8 @@ struct Tables {  // per-length twiddle tables (device pointers), by precision
   const void* tw;
 };
 
-enum : int { kMaxFields = 16 };
+// forward-z outputs at most: uu (5) + density and scalar fluxes (3 + 3) + tau (5) + 1/2|u|^2
+enum : int { kMaxFields = 17 };
 
 // extra operands of the column kernels
 template <typename T> struct ColAux {

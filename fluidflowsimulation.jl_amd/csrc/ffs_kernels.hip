@@ -394,7 +394,8 @@ template <typename T>
 void launch_fwd_z(const Geo& g, const void* b1, void* const* dst, int nf, int nzero, double scale, const void* tw,
                   hipStream_t s) {
   ColAux<T> aux{};
-  for (int f = 0; f < nf && f < kMaxFields; ++f) aux.dst[f] = (C<T>*)dst[f];
+  if (nf > kMaxFields) throw std::runtime_error("forward z pass: more outputs than kMaxFields");
+  for (int f = 0; f < nf; ++f) aux.dst[f] = (C<T>*)dst[f];
   switch (g.nz) {
 #define X(n)                                                                        \
   case n: run_col<T, n, FWD_Z, false>(g, b1, nullptr, nf, g.my, tw, nzero, scale, s, aux); break;

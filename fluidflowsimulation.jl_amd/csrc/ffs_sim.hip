@@ -763,7 +763,9 @@ struct ffs_sim {
     }
     if (sles || dles) {
       if (dyn) throw FfsError(FFS_ERR_ARG, "the scalars' LES fluxes are built for the pointwise LES models only");
-      if (nsform) throw FfsError(FFS_ERR_ARG, "the scalars' LES fluxes need equation = rotational here");
+      // the conservation form carries no vorticity: eddy diffusion only there
+      if (nsform && (sles == 2 || dles == 2))
+        throw FfsError(FFS_ERR_ARG, "vorticityDiffusion needs the vorticity: equation = rotational");
       sles_cv = (std::isnan(p.scalar_les_c) ? 1.0 : p.scalar_les_c) * lesm.d2;
       dles_cv = (std::isnan(p.density_les_c) ? 0.1 : p.density_les_c) * lesm.d2;
     }
@@ -1051,6 +1053,7 @@ struct ffs_sim {
     if (les && !dyn)
       for (int c = 0; c < 5; ++c) dst[f++] = cptr(tau, c * S);
     if (need_ke) dst[f++] = ke;
+    if (f != nf || nf > kMaxFields) throw FfsError(FFS_ERR_ARG, "forward z outputs do not match the x pass");
     launch("fwd_z", [&] { FFS_DISPATCH(launch_fwd_z, g, w2, dst, nf, nsform ? 0 : 3, scale, tw_z, stream); });
     // conservation form: rhs = div!(uu) (models.jl:176-180)
     if (nsform) launch("div_uu", [&] { FFS_DISPATCH(launch_div_uu, g, uu, rhs, stream); });
@@ -2023,7 +2026,9 @@ int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n) {
                        s->nsform ? 1 : 0, s->grad1(), s->grad2());
       });
       s->launch("inv_y", [&] { FFS_DISPATCH_S(launch_inv_y, s->g, s->w1, s->w2, s->nin, s->tw_y, s->stream); });
-      const int s0 = 6 + 1 + (s->pscal ? 1 : 0);  // S, then i k rho (the density's gradient comes first)
+      // S, then i k rho (the density's gradient comes first); w (slots 3..5) only in the rotational form,
+      // and only the vorticity term reads it
+      const int s0 = (s->nsform ? 3 : 6) + 1 + (s->pscal ? 1 : 0);
       const size_t S2 = (size_t)s->g.sx * s->g.ny * s->g.nz;
       void* real = s->dalloc(14 * padded * tb);  // S (5), w (3), grad rho (3), the flux (3)
       auto R = [&](int k) { return (char*)real + (size_t)k * padded * tb; };

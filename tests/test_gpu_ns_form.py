@@ -30,7 +30,22 @@ CASES = [
           z_acceleration=-1.0, reference_density=1.0, passive_scalar=True, scalar_pr=1.0, scalar_gradient=0.5),
      dict(density={"Pr": 0.7, "gradient": 1.0, "g": -1.0}, scalar=dict(Pr=1.0, gradient=0.5, dir=2)), "fp64", 1e-12),
     ("ab3 fp32", dict(integrator="Adams_Bashforth3rdO", variable_dt=True, cfl=0.3, precision="fp32"), {}, "fp32", 2e-5),
+    # the scalars' eddy-diffusion LES fluxes (les_real_calc.jl:120-136) need no vorticity
+    ("ab3 fixed dt, smagorinsky with the density's and the scalar's eddy fluxes",
+     dict(integrator="Adams_Bashforth3rdO", variable_dt=False, dt=2e-3, les_model="Smagorinsky", smagorinsky_c=0.16,
+          density=True, pr=0.7, density_gradient=1.0, z_acceleration=-1.0, reference_density=1.0,
+          density_les="eddyDiffusion", passive_scalar=True, scalar_pr=1.0, scalar_gradient=0.5),
+     dict(les_c=0.16, density={"Pr": 0.7, "gradient": 1.0, "g": -1.0}, density_les="eddy",
+          scalar=dict(Pr=1.0, gradient=0.5, dir=2), scalar_les="eddy"), "fp64", 1e-12),
 ]
+
+
+def test_ns_form_vorticity_diffusion_is_refused():
+    p = ffs.Params(nx=16, ny=16, nz=16, nu=5e-3, dt=1e-3, variable_dt=False, dt_stat=10, dt_spec=10, write_time=10,
+                   equation="NS", les_model="Smagorinsky", passive_scalar=True, scalar_pr=1.0,
+                   scalar_les="vorticityDiffusion")
+    with pytest.raises(_lib.FfsError, match="vorticityDiffusion needs the vorticity"):
+        ffs.Simulation(p)
 
 
 @pytest.mark.parametrize("name,kw,okw,prec,tol", CASES, ids=[c[0] for c in CASES])

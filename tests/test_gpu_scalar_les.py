@@ -92,11 +92,11 @@ def test_scalar_les_flux(name, integ, model, smodel, sles, dles):
 
 
 def test_scalar_les_refusals():
-    print_case("scalar LES refusals: densityLesModel without lesModel, dynamic models, the NS form")
+    print_case("scalar LES refusals: densityLesModel without lesModel, dynamic models, vorticity diffusion in NS")
     base = dict(nx=16, ny=16, nz=16, nu=1e-2, dt=1e-3, variable_dt=False)
     dens = dict(density=True, pr=1.0, density_gradient=1.0, z_acceleration=-1.0, reference_density=1.0)
     for kw in (dict(dens, density_les="eddyDiffusion"),
                dict(passive_scalar=True, les_model="dynamicSmagorinsky"),
-               dict(passive_scalar=True, les_model="Smagorinsky", equation="NS")):
+               dict(passive_scalar=True, les_model="Smagorinsky", equation="NS", scalar_les="vorticityDiffusion")):
         with pytest.raises(_lib.FfsError):
             ffs.Simulation(ffs.Params(**base, **kw))
