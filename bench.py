@@ -583,6 +583,20 @@ def leg_features(args, n, nu):
     return out
 
 
+def cpu_model():
+    """The host CPU's model string (/proc/cpuinfo) and its logical core count."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, os.cpu_count()
+
+
 def cpu_baseline(n, nu, args):
     """The FP64 oracle (numpy + multi-threaded scipy.fft) on the host cores:
     initialize + `cpu_steps` steps of the same workload."""
@@ -601,7 +615,9 @@ def cpu_baseline(n, nu, args):
     el = time.perf_counter() - t0
     v = n**3 * args.cpu_steps / el
     log(f"cpu baseline: {el / args.cpu_steps:.2f} s/step, {v:.4e} N^3*steps/s")
-    return {"value": v, "unit": "N^3*steps/s", "cores": cores, "kind": "port",
+    model, ncpu = cpu_model()
+    return {"value": v, "unit": "N^3*steps/s", "cores": cores, "kind": "port", "cpu_model": model,
+            "host_logical_cpus": ncpu,
             "sample": f"{args.cpu_steps} steps of the same {n}^3 FP64 workload after initialize (numpy/scipy.fft "
                       f"oracle, {el:.1f}s)",
             "s_per_step": el / args.cpu_steps}

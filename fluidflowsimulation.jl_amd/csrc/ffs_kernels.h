@@ -1,5 +1,6 @@
 // Kernel launch interface of the FFS HIP solver (internal to libffs_hip.so).
 #pragma once
+#include <string>
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <type_traits>
@@ -156,6 +157,11 @@ struct XOps {
   double rho_cv = 0.0, phi_cv = 0.0;  // c Delta^2 of VorticityDiffusion
   double les_alpha = 0.0;             // c^2 Delta^2 (Smagorinsky's nu_t = alpha |S|)
 };
+// whether n is a compiled transform length, and (empty: yes) whether the x pass of length n can run
+// with these options -- checked when a run is created, before anything is allocated
+bool length_supported(int n);
+template <typename T>
+std::string x_pass_check(int n, const XOps& ops);
 template <typename T>
 void launch_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
                  unsigned long long* rhomax, const void* tw, hipStream_t s);

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -148,6 +149,18 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) 
   }
   // `red` is one array per kernel: a following call must not overwrite it before thread 0 has read it
   __syncthreads();
+}
+
+// Raises a kernel's dynamic LDS limit to `bytes`, once per process and kernel.  Handles may be
+// driven from several host threads, so the first call is serialised by the kernel's once_flag; a
+// failure is reported (and retried by the next call) rather than surfacing as a failed launch.
+static void lds_limit_once(std::once_flag& once, const void* fn, size_t bytes, const char* what) {
+  std::call_once(once, [&] {
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess)
+      throw std::runtime_error(std::string("raising the LDS limit of ") + what + " to " + std::to_string(bytes) +
+                               " bytes: " + hipGetErrorString(e));
+  });
 }
 
 // ===========================================================================
@@ -327,12 +340,8 @@ static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, con
   constexpr int TP = col_tper<T, N, CURL>();
   static_assert(TX * TP <= kColThreads, "column block exceeds its launch bound");
   const size_t lds = (size_t)(N * TX + tw_lds<T>(N)) * sizeof(C<T>);
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void*)&k_col<T, N, KIND, CURL, GRAD>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)lds);
-    attr = true;
-  }
+  static std::once_flag once;
+  lds_limit_once(once, (const void*)&k_col<T, N, KIND, CURL, GRAD>, lds, "a column pass");
   const int ntile = (g.mx + TX - 1) / TX * nb;
   const dim3 grid = (CURL && FFS_CURL_SPLIT) ? dim3((unsigned)(nzero * ((ntile + kXcd - 1) / kXcd * kXcd)))
                                              : dim3((g.mx + TX - 1) / TX, nb, CURL ? 1 : nf);
@@ -1045,12 +1054,8 @@ static void run_x_gen(const Geo& g, const void* a2, void* b2, const XOps& ops, u
     throw std::runtime_error("nonlinear x pass: " + std::to_string(NSL) + " field pairs of length " + std::to_string(N) +
                              " exceed one block");
   const size_t lds = ((size_t)ROWS * NSL * (N + 1) + tw_lds<T>(N)) * sizeof(C<T>);
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)&k_x_gen<T, N, NSL>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
-    attr = true;
-  }
+  static std::once_flag once;
+  lds_limit_once(once, (const void*)&k_x_gen<T, N, NSL>, lds, "the general x pass");
   const int nrows = g.ny * g.nz;
   k_x_gen<T, N, NSL><<<(nrows + ROWS - 1) / ROWS, ROWS * NSL * tper<N>(), lds, s>>>(
       g, (const C<T>*)a2, (C<T>*)b2, (const C<T>*)tw, ops, umax, rhomax);
@@ -1063,25 +1068,72 @@ static void launch_xreg(const Geo& g, const void* a2, void* b2, const void* tw, 
   constexpr int WPB = 2;  // wavefronts per block
   constexpr int RPB = WPB * (64 / TP);
   const size_t lds = ((size_t)RPB * xreg_ldx<T, N>() + tw_lds<T>(N)) * sizeof(C<T>);
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)&k_x_nl_reg<T, N, MODE, KE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
-    attr = true;
-  }
+  static std::once_flag once;
+  lds_limit_once(once, (const void*)&k_x_nl_reg<T, N, MODE, KE>, lds, "the x pass");
   const int nrows = g.ny * g.nz;
   k_x_nl_reg<T, N, MODE, KE><<<(nrows + RPB - 1) / RPB, 64 * WPB, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
                                                                           (const C<T>*)tw, vmax, xa);
 }
 
+// The x pass a run's options select: 0 for the register pass (one launch per product mode), else
+// the field-pair slots per row of the general LDS pass, which holds u, w, S and the scalars'
+// gradients of a row together
+template <typename T, int N>
+static int x_general_slots(const XOps& ops) {
+  const bool plain = !ops.rho && !ops.les && !ops.phi;
+  const bool w_les = (ops.les && les_needs_w(ops.lm)) || ops.rho_les || ops.phi_les;
+  if constexpr (xreg_ok<T, N>()) {
+    if ((FFS_XNL_MODES || plain) && !w_les) return 0;
+  }
+  return (std::max(ops.nin, ops.nout) + 1) / 2;
+}
+
+// the general pass's slot counts are instantiated as 3, 4, 6, 8 and 10; one row's slots must fit the
+// kernel's 256-thread launch bound
+static int x_gen_bucket(int nsl) {
+  return nsl <= 3 ? 3 : nsl <= 4 ? 4 : nsl <= 6 ? 6 : nsl <= 8 ? 8 : nsl <= 10 ? 10 : -1;
+}
+
+template <typename T, int N>
+static std::string x_check(const XOps& ops) {
+  const int nsl = x_general_slots<T, N>(ops);
+  if (nsl == 0) return "";
+  const int b = x_gen_bucket(nsl);
+  if (b < 0) return "the x pass holds at most 10 field pairs per row (" + std::to_string(nsl) + " needed)";
+  if (b * tper<N>() > 256)
+    return "the x pass of length " + std::to_string(N) + " holds at most " + std::to_string(256 / tper<N>()) +
+           " field pairs per row (" + std::to_string(nsl) + " needed with these options)";
+  return "";
+}
+
+bool length_supported(int n) {
+  switch (n) {
+#define X(m) case m:
+    FFS_SIZES(X)
+#undef X
+    return true;
+    default: return false;
+  }
+}
+
+template <typename T>
+std::string x_pass_check(int n, const XOps& ops) {
+  switch (n) {
+#define X(m) \
+  case m: return x_check<T, m>(ops);
+    FFS_SIZES(X)
+#undef X
+    default: return "unsupported transform length " + std::to_string(n);
+  }
+}
+template std::string x_pass_check<float>(int, const XOps&);
+template std::string x_pass_check<double>(int, const XOps&);
+
 template <typename T, int N>
 static void run_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
                      unsigned long long* rhomax, const void* tw, hipStream_t s) {
-  const bool plain = !ops.rho && !ops.les && !ops.phi;
-  // the general pass holds u, w, S and the scalars' gradients of a row together
-  const bool w_les = (ops.les && les_needs_w(ops.lm)) || ops.rho_les || ops.phi_les;
   if constexpr (xreg_ok<T, N>()) {
-    if ((FFS_XNL_MODES || plain) && !w_les) {
+    if (x_general_slots<T, N>(ops) == 0) {
       // u x w or traceless(-u u) (+ 1/2|u|^2, always the last output slot), then -rho u, -phi u and
       // tau as launches of their own
       XRegArgs xa{0, 0, ops.nout - 1, 0.0};
@@ -1109,7 +1161,7 @@ static void run_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, un
       return;
     }
   }
-  const int nsl = (std::max(ops.nin, ops.nout) + 1) / 2;
+  const int nsl = x_general_slots<T, N>(ops);
   if (nsl > 10) throw std::runtime_error("nonlinear x pass: more fields than the general pass holds");
   if (nsl <= 3) run_x_gen<T, N, 3>(g, a2, b2, ops, umax, rhomax, tw, s);
   else if (nsl <= 4) run_x_gen<T, N, 4>(g, a2, b2, ops, umax, rhomax, tw, s);
@@ -1970,11 +2022,8 @@ static void run_fwd_z_update(const Geo& g, const void* b1, void* u, void* rhs, c
   constexpr int TX = col_tx<T>(N);
   constexpr int TP = col_tper<T, N, false>();
   const size_t lds = (size_t)(N * TX + tw_lds<T>(N)) * sizeof(C<T>);
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)&k_fwd_z_update<T, N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr = true;
-  }
+  static std::once_flag once;
+  lds_limit_once(once, (const void*)&k_fwd_z_update<T, N>, lds, "the fused forward z + update");
   const dim3 grid((g.mx + TX - 1) / TX, g.my);
   k_fwd_z_update<T, N><<<grid, TX * TP, lds, s>>>(g, (const C<T>*)b1, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1,
                                                   (const C<T>*)fm2, st, a, (const C<T>*)tw, (T)scale);

@@ -741,9 +741,10 @@ struct ffs_sim {
         dyn_td2 = td * td;
         dyn_cmin = p.les_cmin;
       }
-      // the models that read w take it from the rotational form's curl (need_vorticity, simulationtypes.jl:41)
+      // the models that read w take it from the rotational form's curl pass; the reference also forms w
+      // for NS with LES (need_vorticity, simulationtypes.jl:41-42), which this build does not implement
       if (les_needs_w(lesm) && p.equation == FFS_EQ_NS)
-        throw FfsError(FFS_ERR_ARG, "LES models other than Smagorinsky need equation = rotational here");
+        throw FfsError(FFS_ERR_ARG, "LES models other than Smagorinsky with equation = NS are not implemented");
     }
     if (dens) {
       if (!(p.pr > 0) || !(p.reference_density != 0))
@@ -763,9 +764,10 @@ struct ffs_sim {
     }
     if (sles || dles) {
       if (dyn) throw FfsError(FFS_ERR_ARG, "the scalars' LES fluxes are built for the pointwise LES models only");
-      // the conservation form carries no vorticity: eddy diffusion only there
+      // this build forms w only in the rotational form's curl pass (the reference also forms it for NS
+      // with LES, need_vorticity, simulationtypes.jl:41-42)
       if (nsform && (sles == 2 || dles == 2))
-        throw FfsError(FFS_ERR_ARG, "vorticityDiffusion needs the vorticity: equation = rotational");
+        throw FfsError(FFS_ERR_ARG, "vorticityDiffusion with equation = NS is not implemented (use rotational)");
       sles_cv = (std::isnan(p.scalar_les_c) ? 1.0 : p.scalar_les_c) * lesm.d2;
       dles_cv = (std::isnan(p.density_les_c) ? 0.1 : p.density_les_c) * lesm.d2;
     }
@@ -779,6 +781,7 @@ struct ffs_sim {
     fp32 = (p.precision == FFS_FP32);
     // the FP32 tile (16 columns x 32 threads) holds too many registers to fuse profitably
     fuse_ok = !fp32;
+    check_lengths();
     if (dry) {  // footprint: the same allocations, counted, plus the transient staging of set_field
       setup_grid();
       setup_buffers();
@@ -997,6 +1000,29 @@ struct ffs_sim {
   // ---------------------------------------------------------------- the step
   bool is_stats_time(int64_t it) const { return it % p.dt_stat == 0; }
   bool is_spec_time(int64_t it) const { return it % p.dt_spec == 0; }
+  // every transform length must be compiled and the x pass must hold a row's fields with these options;
+  // checked before anything is allocated, so an unsupported run fails at creation, not mid-step
+  void check_lengths() const {
+    const char* ax[3] = {"nx", "ny", "nz"};
+    const int n[3] = {p.nx, p.ny, p.nz};
+    for (int i = 0; i < 3; ++i)
+      if (!length_supported(n[i]))
+        throw FfsError(FFS_ERR_ARG, std::string(ax[i]) + " = " + std::to_string(n[i]) +
+                                        " is not a compiled transform length (16 24 32 48 64 96 128 192 256 512)");
+    XOps ops{};
+    ops.nin = nin;
+    ops.nout = nfw + 1;  // a stats step adds 1/2|u|^2
+    ops.rho = dens ? 1 : 0;
+    ops.phi = pscal ? 1 : 0;
+    ops.ns = nsform ? 1 : 0;
+    ops.les = (les && !dyn) ? 1 : 0;
+    ops.lm = lesm;
+    ops.rho_les = dles;
+    ops.phi_les = sles;
+    const std::string e = fp32 ? x_pass_check<float>(p.nx, ops) : x_pass_check<double>(p.nx, ops);
+    if (!e.empty()) throw FfsError(FFS_ERR_ARG, e);
+  }
+
   bool is_output_time(int64_t it) const { return tit != 0 && it % p.write_time == 0; }
 
   // calculate_rhs! up to (not including) fourierspacep2!: rhs <- dealias(F(u x w))/N^3,

@@ -86,6 +86,8 @@ const FFS_EULER = Int32(2)
 const FFS_SPACE_REAL = Int32(0)
 const FFS_SPACE_FOURIER = Int32(1)
 const FFS_EVENT_OUTPUT = Int32(1)
+const FFS_SCALAR_ITERATION = Int32(0)
+const FFS_SCALAR_TIME = Int32(1)
 
 function check(rc::Integer, h::Ptr{Cvoid})
     rc < 0 || return rc
@@ -494,16 +496,22 @@ function run_simulation(d::AbstractDict)
         set_field!(s, 3, a)
     end
     nt = haskey(d, :nt) ? parse(Int, d[:nt]) : typemax(Int) - p.start
+    lengthtime = haskey(d, :timeDuration) ? parse(Float64, d[:timeDuration]) : Inf
     if p.start == 0
         for (name, t) in (("Stats.txt", 0), ("Scales.txt", 1), ("Numbers.txt", 2))
             open(f -> write(f, stats_header(s, t), "\n"), name, "w")
         end
     end
     initialize!(s)
-    left = nt
-    while left > 0
+    @assert nt + Int(scalar(s, FFS_SCALAR_ITERATION)) >= p.write_time
+    @assert nt + Int(scalar(s, FFS_SCALAR_ITERATION)) >= p.dt_stat
+    finalstep = Int(scalar(s, FFS_SCALAR_ITERATION)) + nt
+    finaltime = scalar(s, FFS_SCALAR_TIME) + lengthtime
+    # the loop of FluidFlowSimulation.jl:41-43: stop at finalstep or once the time reaches finaltime;
+    # with a finite duration the time is checked after every step, else the library runs to the next event
+    while Int(scalar(s, FFS_SCALAR_ITERATION)) < finalstep && scalar(s, FFS_SCALAR_TIME) < finaltime
+        left = isinf(finaltime) ? finalstep - Int(scalar(s, FFS_SCALAR_ITERATION)) : 1
         done, ev = advance!(s, left)
-        left -= done
         ev == FFS_EVENT_OUTPUT && writeoutput(s)
         writestats(s)
         writespectra(s)

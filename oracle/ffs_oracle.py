@@ -405,7 +405,11 @@ class OracleSim:
                          1j * (g.kx * txz + g.ky * tyz + g.kz * tzz)])
 
     def set_dt(self):
-        """set_dt! (time_step_functions.jl:86-129) with the intended fresh max|u_i|."""
+        """set_dt! (time_step_functions.jl:86-129) with the intended fresh max|u_i|.
+
+        pinned by: steady inviscid states with known max|u_i| and max|rho| stepped through the CFL
+        ramp, AB3's first-step halving, both x1.25 clamps and the density limit
+        (tests/test_oracle_pins.py::test_set_dt_*)."""
         g = self.g
         dx = 2 * math.pi * g.LY / g.NY
         cfl = self.cfl
@@ -665,7 +669,12 @@ class OracleSim:
         return ph, pv
 
     def stats(self):
-        """stats(s) (stats.jl:64-165): returns (Stats row, Scales row, Numbers row)."""
+        """stats(s) (stats.jl:64-165): returns (Stats row, Scales row, Numbers row).
+
+        pinned by: Taylor-Green t = 0 closed forms and the physical-space means of an asymmetric
+        analytic field on a non-cubic box for every Scales and Numbers entry
+        (tests/test_oracle_pins.py::test_scales_numbers_*); the Stats columns by the tests of
+        tests/test_oracle.py and tests/test_spectra.py."""
         g = self.g
         u = self.uh
         nu = self.nu

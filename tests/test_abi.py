@@ -89,3 +89,31 @@ def test_bad_params_fail_cleanly_without_device(lib):
     rc = lib.ffs_create(ctypes.byref(p), ctypes.byref(h))
     assert rc == -1 and not h.value
     assert b"size" in lib.ffs_last_error(None)
+
+
+def test_unsupported_sizes_fail_at_creation(lib):
+    """An uncompiled transform length, or an x pass that cannot hold a row's fields, fails in
+    ffs_create / ffs_footprint with FFS_ERR_ARG before any device work (so this runs without a GPU)."""
+    import pytest
+
+    import ffs
+
+    base = dict(nx=32, ny=32, nz=32, nu=0.01, dt=0.01, variable_dt=False)
+    assert ffs.footprint(ffs.Params(**base)) > 0
+    for key in ("nx", "ny", "nz"):
+        p = ffs.Params(**{**base, key: 40})
+        with pytest.raises(RuntimeError) as e:
+            ffs.footprint(p)
+        assert "not a compiled transform length" in str(e.value), str(e.value)
+        with pytest.raises(RuntimeError) as e:
+            ffs.Simulation(p)
+        assert "not a compiled transform length" in str(e.value), str(e.value)
+        assert e.value.code == -1  # FFS_ERR_ARG, before any HIP call
+    # Vreman (reads w) with the density's and the passive scalar's LES fluxes: 10 field pairs per row,
+    # more than the general x pass of length 512 holds (8); at length 256 it fits
+    heavy = dict(les_model="Vreman", density=True, pr=1.0, density_gradient=1.0, z_acceleration=1.0,
+                 reference_density=1.0, density_les="eddy", passive_scalar=True, scalar_pr=1.0, scalar_les="eddy")
+    with pytest.raises(RuntimeError) as e:
+        ffs.footprint(ffs.Params(**{**base, "nx": 512, **heavy}))
+    assert "field pairs per row" in str(e.value), str(e.value)
+    assert ffs.footprint(ffs.Params(**{**base, "nx": 256, **heavy})) > 0

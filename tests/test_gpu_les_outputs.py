@@ -59,12 +59,13 @@ def test_dynamic_coefficient_read_out(model, smodel):
             assert np.array_equal(a, b)
 
 
-FLUX = [("Smagorinsky, eddy diffusion", "Smagorinsky", "eddyDiffusion", None),
-        ("Vreman, vorticity diffusion c=0.3", "Vreman", "vorticityDiffusion", 0.3)]
+FLUX = [("Smagorinsky, eddy diffusion", "Smagorinsky", "eddyDiffusion", None, "rotational"),
+        ("Vreman, vorticity diffusion c=0.3", "Vreman", "vorticityDiffusion", 0.3, "rotational"),
+        ("Smagorinsky, eddy diffusion, conservation form", "Smagorinsky", "eddyDiffusion", None, "NS")]
 
 
-@pytest.mark.parametrize("name,model,dles,cv", FLUX, ids=[f[0] for f in FLUX])
-def test_density_les_flux_read_out(tmp_path, name, model, dles, cv):
+@pytest.mark.parametrize("name,model,dles,cv,eq", FLUX, ids=[f[0] for f in FLUX])
+def test_density_les_flux_read_out(tmp_path, name, model, dles, cv, eq):
     print_case(f"dflux read-out: {name}")
     from oracle import les_models
     g = Grid(N, N, N)
@@ -73,13 +74,13 @@ def test_density_les_flux_read_out(tmp_path, name, model, dles, cv):
     c, cp = les_models.default_constants(model)
     spec = "eddy" if dles == "eddyDiffusion" else ("vorticity", cv)
     ref = OracleSim(g, 3e-3, dt=2e-3, variable_dt=False, dt_stat=3, les_c=c, les_model=model, les_cp=cp,
-                    density={"Pr": 0.9, "gradient": 1.0, "g": -1.0}, density_les=spec)
+                    density={"Pr": 0.9, "gradient": 1.0, "g": -1.0}, density_les=spec, equation=eq)
     ref.set_velocity(u0)
     ref.set_density(rho0)
     p = ffs.Params(nx=N, ny=N, nz=N, nu=3e-3, dt=2e-3, variable_dt=False, dt_stat=3, dt_spec=1 << 30,
                    write_time=1 << 30, integrator="Adams_Bashforth3rdO", les_model=model, density=True, pr=0.9,
                    density_gradient=1.0, z_acceleration=-1.0, reference_density=1.0, density_les=dles,
-                   density_les_c=cv)
+                   density_les_c=cv, equation=eq)
     with ffs.Simulation(p) as s:
         s.set_velocity(*u0)
         s.set_density(rho0)

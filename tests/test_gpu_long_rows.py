@@ -19,8 +19,12 @@ pytestmark = pytest.mark.gpu
 
 # (nx, ny, nz, x_domain): the horizontal spectrum bins ky on the x wavenumbers unless kx is the finer
 # grid (Globals.jl:148-152), so the long-y case takes a longer x box
+# (the 256 and 128 column lengths are the ones the 256^3 and 128^3 runs use; on plain FP64 steps the z
+# length also selects the fused forward-z + update kernel of that length)
 SHAPES = [(512, 16, 16, 1.0), (256, 16, 16, 1.0), (16, 512, 16, 2.0), (16, 16, 512, 1.0), (192, 16, 16, 1.0),
-          (96, 24, 16, 1.0)]
+          (96, 24, 16, 1.0), (16, 256, 16, 2.0), (16, 16, 256, 1.0), (128, 16, 16, 1.0), (16, 128, 16, 2.0),
+          (16, 16, 128, 1.0), (16, 192, 16, 2.0), (16, 16, 192, 1.0), (16, 96, 16, 2.0), (16, 16, 96, 1.0),
+          (16, 48, 16, 2.0), (16, 16, 48, 1.0)]
 MODELS = ["plain", "les", "density"]
 
 

@@ -55,3 +55,19 @@ def test_params_struct_mirror():
     jbody = re.search(r"mutable struct ffs_params\n(.*?)\n    ffs_params\(\) = new\(\)", jl, flags=re.S).group(1)
     jfields = [tuple(x.strip().split("::")) for x in jbody.strip().splitlines()]
     assert jfields == cfields
+
+
+def test_run_simulation_keeps_the_reference_loop():
+    """run_simulation of the shim reads timeDuration, asserts nt against writeTime and dtStat and
+    stops at the final step or the final time, as FluidFlowSimulation.jl:23-44 does."""
+    src = open(SHIM).read()
+    body = src[src.index("function run_simulation(d::AbstractDict)"):]
+    assert "haskey(d, :timeDuration)" in body and "Inf" in body
+    assert "@assert nt + Int(scalar(s, FFS_SCALAR_ITERATION)) >= p.write_time" in body
+    assert "@assert nt + Int(scalar(s, FFS_SCALAR_ITERATION)) >= p.dt_stat" in body
+    assert "scalar(s, FFS_SCALAR_TIME) < finaltime" in body and "< finalstep" in body
+    hdr = open(HEADER).read()
+    for name in ("FFS_SCALAR_ITERATION", "FFS_SCALAR_TIME", "FFS_EVENT_OUTPUT"):
+        c = re.search(rf"#define {name} (-?\d+)", hdr).group(1)
+        j = re.search(rf"const {name} = Int32\((-?\d+)\)", src).group(1)
+        assert c == j, name
