@@ -20,7 +20,7 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("shape,dom,prec", [((32, 32, 32), (1.0, 1.0, 1.0), "fp64"),
-                                            ((24, 32, 40), (1.0, 1.5, 0.75), "fp64"),
+                                            ((24, 32, 48), (1.0, 1.5, 0.75), "fp64"),
                                             ((48, 48, 48), (1.0, 1.0, 1.0), "fp32")])
 def test_seeded_velocity_matches_host_twin(shape, dom, prec):
     print_case(f"seeded velocity {shape} {dom} {prec}")
