@@ -167,6 +167,10 @@ void launch_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, unsign
                  unsigned long long* rhomax, const void* tw, hipStream_t s);
 template <typename T>
 void launch_x_c2r(const Geo& g, const void* a2, T* real, int nf, const void* tw, hipStream_t s);
+// the same C2R for the rows [row0, row0 + nrow) of one field (row0 even), written as doubles at
+// real[(row - row0) * 2 nxh + x]
+template <typename T>
+void launch_x_c2r_rows(const Geo& g, const void* a2, double* real, int row0, int nrow, const void* tw, hipStream_t s);
 template <typename T>
 void launch_x_r2c(const Geo& g, const T* real, void* b2, int nf, const void* tw, hipStream_t s);
 template <typename T>

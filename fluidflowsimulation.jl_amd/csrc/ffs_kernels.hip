@@ -942,10 +942,12 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
 }
 
 // C2R of nf fields (rows paired 2m, 2m+1) into the padded real layout
-// (2*nxh, ny, nz), padding entries zeroed.
-template <typename T, int N>
-__global__ void __launch_bounds__(256) k_x_c2r(Geo g, const C<T>* __restrict__ a2, T* __restrict__ real,
-                                              const C<T>* __restrict__ tw_g, int nf) {
+// (2*nxh, ny, nz), padding entries zeroed.  Only the rows [row0, row0 + nrow) of each field are
+// transformed and written (row0 even; row = y + ny z), at row - row0 of an output of nrow rows per
+// field, as values of type O.
+template <typename T, int N, typename O = T>
+__global__ void __launch_bounds__(256) k_x_c2r(Geo g, const C<T>* __restrict__ a2, O* __restrict__ real,
+                                              const C<T>* __restrict__ tw_g, int nf, int row0, int nrow) {
   constexpr int TP = tper<N>();
   constexpr int NSLOT = xrows<N>() * 3;
   constexpr int LD = N + 1;
@@ -955,11 +957,11 @@ __global__ void __launch_bounds__(256) k_x_c2r(Geo g, const C<T>* __restrict__ a
   const C<T>* tw = stage_twiddles<T, N>(lds + NSLOT * LD, tw_g);  // published by the first __syncthreads below
   const int tid = threadIdx.x;
   const int nrows = g.ny * g.nz;
-  const int npairs = (nrows + 1) / 2;
+  const int npairs = (nrow + 1) / 2;
   const int mx = g.mx;
   const size_t S2 = (size_t)g.sx * nrows;
   const int rl = 2 * g.nxh;
-  const size_t RS = (size_t)rl * nrows;
+  const size_t RS = (size_t)rl * nrow;
   const int p0 = blockIdx.x * NSLOT;
 
   for (int w = tid; w < NSLOT * (N / 2 + 1); w += NTHR) {
@@ -967,9 +969,9 @@ __global__ void __launch_bounds__(256) k_x_c2r(Geo g, const C<T>* __restrict__ a
     const int pidx = p0 + sl;
     C<T> A = mk<T>(0, 0), B = mk<T>(0, 0);
     if (pidx < nf * npairs && k < mx) {
-      const int f = pidx / npairs, row = 2 * (pidx % npairs);
+      const int f = pidx / npairs, row = row0 + 2 * (pidx % npairs);
       A = a2[f * S2 + (size_t)row * g.sx + k];
-      if (row + 1 < nrows) B = a2[f * S2 + (size_t)(row + 1) * g.sx + k];
+      if (row + 1 < row0 + nrow) B = a2[f * S2 + (size_t)(row + 1) * g.sx + k];
     }
     build_pair<T, N>(lds + sl * LD, k, A, B);
   }
@@ -984,10 +986,10 @@ __global__ void __launch_bounds__(256) k_x_c2r(Geo g, const C<T>* __restrict__ a
     const int sl2 = w / rl, n = w % rl;
     const int pidx = p0 + sl2;
     if (pidx >= nf * npairs) continue;
-    const int f = pidx / npairs, row = 2 * (pidx % npairs);
+    const int f = pidx / npairs, row = 2 * (pidx % npairs);  // relative to row0
     const C<T> z = (n < N) ? lds[sl2 * LD + n] : mk<T>(0, 0);
-    real[f * RS + (size_t)row * rl + n] = z.x;
-    if (row + 1 < nrows) real[f * RS + (size_t)(row + 1) * rl + n] = z.y;
+    real[f * RS + (size_t)row * rl + n] = (O)z.x;
+    if (row + 1 < nrow) real[f * RS + (size_t)(row + 1) * rl + n] = (O)z.y;
   }
 }
 
@@ -1190,8 +1192,29 @@ void launch_x_c2r(const Geo& g, const void* a2, T* real, int nf, const void* tw,
   case n: {                                                                                       \
     constexpr int NS = 3 * xrows<n>();                                                            \
     dim3 grid((nf * npairs + NS - 1) / NS);                                                       \
-    k_x_c2r<T, n><<<grid, NS * tper<n>(), x_lds_bytes<T, n>(), s>>>(g, (const C<T>*)a2, real, (const C<T>*)tw, nf); \
+    k_x_c2r<T, n><<<grid, NS * tper<n>(), x_lds_bytes<T, n>(), s>>>(g, (const C<T>*)a2, real, (const C<T>*)tw, nf, \
+                                                                      0, g.ny * g.nz);                               \
     break;                                                                                        \
+  }
+    FFS_SIZES(X)
+#undef X
+    default: bad_size("x C2R pass", g.nrx);
+  }
+}
+
+template <typename T>
+void launch_x_c2r_rows(const Geo& g, const void* a2, double* real, int row0, int nrow, const void* tw, hipStream_t s) {
+  if (row0 < 0 || row0 % 2 || nrow < 1 || row0 + nrow > g.ny * g.nz)
+    throw std::runtime_error("x C2R pass: row range outside the field or not starting at an even row");
+  const int npairs = (nrow + 1) / 2;
+  switch (g.nrx) {
+#define X(n)                                                                                            \
+  case n: {                                                                                             \
+    constexpr int NS = 3 * xrows<n>();                                                                  \
+    dim3 grid((npairs + NS - 1) / NS);                                                                  \
+    k_x_c2r<T, n, double><<<grid, NS * tper<n>(), x_lds_bytes<T, n>(), s>>>(g, (const C<T>*)a2, real,    \
+                                                                           (const C<T>*)tw, 1, row0, nrow); \
+    break;                                                                                              \
   }
     FFS_SIZES(X)
 #undef X
@@ -3076,6 +3099,7 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   template void launch_x_nl<T>(const Geo&, const void*, void*, const XOps&, unsigned long long*, unsigned long long*, \
                                const void*, hipStream_t);                                                           \
   template void launch_x_c2r<T>(const Geo&, const void*, T*, int, const void*, hipStream_t);                          \
+  template void launch_x_c2r_rows<T>(const Geo&, const void*, double*, int, int, const void*, hipStream_t);         \
   template void launch_x_r2c<T>(const Geo&, const T*, void*, int, const void*, hipStream_t);                          \
   template void launch_fwd_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t);                       \
   template void launch_fwd_z<T>(const Geo&, const void*, void* const*, int, int, double, const void*, hipStream_t);   \

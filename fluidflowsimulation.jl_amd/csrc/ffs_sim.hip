@@ -1817,6 +1817,38 @@ int guard(ffs_sim* s, F&& f) {
 
 }  // namespace
 
+// ---- the allocation-free real-space read-out (ffs_get_field_noalloc)
+// One file-scope staging array per GPU, of kReadoutScratchBytes: the x pass writes slabs of whole z
+// planes of the padded real field into it as doubles and each slab is copied to the caller's buffer.
+// Host code reaches a device's copy only through the address hipGetSymbolAddress returns for it
+// (`scratch` itself names a host-side stand-in).  Every read-out on a GPU shares that GPU's array.
+constexpr size_t kReadoutScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) double scratch[kReadoutScratchBytes / sizeof(double)];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_addr;  // device -> its copy of `scratch`
+
+// the device's copy of `scratch`, looked up once per device with that device current
+double* readout_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  const auto it = g_scratch_addr.find(device);
+  if (it != g_scratch_addr.end()) return it->second;
+  int cur = -1;
+  HIPCHECK(hipGetDevice(&cur));
+  if (cur != device) HIPCHECK(hipSetDevice(device));
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (cur != device) HIPCHECK(hipSetDevice(cur));
+  if (e != hipSuccess || !ptr)
+    throw FfsError(FFS_ERR_HIP, std::string("hipGetSymbolAddress of the read-out scratch: ") + hipGetErrorString(e));
+  g_scratch_addr[device] = (double*)ptr;
+  return (double*)ptr;
+}
+
+}  // namespace
+
 extern "C" {
 
 int ffs_abi_version(void) { return FFS_ABI_VERSION; }
@@ -2131,6 +2163,41 @@ int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n) {
       fetch_real(real);
       release(real, padded * tb);
     }
+  });
+}
+
+int ffs_get_field_noalloc(ffs_sim* s, int field, double* host, size_t n) {
+  if (!s || !host) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    if (field < FFS_FIELD_U1 || field > FFS_FIELD_SCALAR)
+      throw FfsError(FFS_ERR_ARG, "the allocation-free read-out serves the state fields u1 u2 u3 rho scalar");
+    if (field == FFS_FIELD_RHO && !s->dens) throw FfsError(FFS_ERR_ARG, "the run has no density field");
+    if (field == FFS_FIELD_SCALAR && !s->pscal) throw FfsError(FFS_ERR_ARG, "the run has no passive scalar");
+    const size_t nxh = s->g.nxh, ny = s->g.ny, nz = s->g.nz;
+    const size_t plane = 2 * nxh * ny;  // doubles per z plane of the padded real layout
+    if (n != plane * nz) throw FfsError(FFS_ERR_IO, "output buffer must hold 2*(nx/2+1)*ny*nz doubles");
+    const size_t cap = kReadoutScratchBytes / sizeof(double);
+    if (plane > cap)
+      throw FfsError(FFS_ERR_ARG, "one z plane of the padded real field (" + std::to_string(plane * sizeof(double)) +
+                                      " bytes) exceeds the read-out scratch (" +
+                                      std::to_string(kReadoutScratchBytes) + " bytes)");
+    double* buf = readout_scratch(s->device);
+    void* src = field == FFS_FIELD_RHO ? s->rho : field == FFS_FIELD_SCALAR ? s->phi : s->cptr(s->u, (size_t)field * s->S);
+    // the inverse z and y passes into the run's own pencil workspace, then the x pass slab by slab
+    FFS_DISPATCH_S(launch_inv_z, s->g, src, s->w1, 1, false, s->tw_z, s->stream);
+    ffs_sim::check_launch("inv_z");
+    FFS_DISPATCH_S(launch_inv_y, s->g, s->w1, s->w2, 1, s->tw_y, s->stream);
+    ffs_sim::check_launch("inv_y");
+    const int zper = (int)std::min<size_t>(nz, cap / plane);
+    for (int z0 = 0; z0 < (int)nz; z0 += zper) {
+      const int nzs = std::min(zper, (int)nz - z0);
+      FFS_DISPATCH_S(launch_x_c2r_rows, s->g, s->w2, buf, z0 * (int)ny, nzs * (int)ny, s->tw_x, s->stream);
+      ffs_sim::check_launch("x_c2r");
+      // stream order keeps the next slab's x pass behind this copy
+      HIPCHECK(hipMemcpyAsync(host + (size_t)z0 * plane, buf, (size_t)nzs * plane * sizeof(double),
+                              hipMemcpyDeviceToHost, s->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(s->stream));
   });
 }
 

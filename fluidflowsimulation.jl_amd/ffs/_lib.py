@@ -128,6 +128,7 @@ SIGNATURES = {
     "ffs_create": (ctypes.c_int, [ctypes.POINTER(FfsParams), ctypes.POINTER(_P)]),
     "ffs_set_field": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D, ctypes.c_size_t]),
     "ffs_get_field": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D, ctypes.c_size_t]),
+    "ffs_get_field_noalloc": (ctypes.c_int, [_P, ctypes.c_int, _D, ctypes.c_size_t]),
     "ffs_seed_velocity": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]),
     "ffs_seed_density": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]),
     "ffs_initialize": (ctypes.c_int, [_P]),

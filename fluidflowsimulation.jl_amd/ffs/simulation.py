@@ -89,6 +89,16 @@ class Simulation:
         c = buf[0::2] + 1j * buf[1::2]
         return c.reshape((self.nx // 2 + 1, self.ny, self.nz), order="F")
 
+    def get_field_noalloc(self, field, out=None):
+        """The real-space read-out of a state field (u1 u2 u3 rho scalar) through
+        ffs_get_field_noalloc, which allocates no device memory; `out` (float64, the padded size)
+        is filled in place when given.  Returns the padded Fortran array (2*(nx//2+1), ny, nz)."""
+        buf = np.empty(self.padded_size, dtype=np.float64) if out is None else out
+        if buf.dtype != np.float64 or buf.size != self.padded_size or not buf.flags.c_contiguous:
+            raise ValueError("out must be a contiguous float64 array of the padded size")
+        self._check(self._lib.ffs_get_field_noalloc(self._h, field, _dptr(buf), buf.size))
+        return buf.reshape(padded_shape(self.nx, self.ny, self.nz), order="F")
+
     def seed_velocity(self, seed, k0=4.0, energy=0.5):
         """Seeded random-phase solenoidal velocity generated on the device (see ffs_seed_velocity)."""
         self._check(self._lib.ffs_seed_velocity(self._h, int(seed) & (2**64 - 1), float(k0), float(energy)))

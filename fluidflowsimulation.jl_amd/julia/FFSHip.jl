@@ -239,6 +239,14 @@ function get_field(s::FFSSimulation, field::Integer; space = FFS_SPACE_REAL)
     return a
 end
 
+# the real-space read-out without device allocation, into a caller-owned buffer (for callers that
+# cannot allocate device memory); serialise calls on one GPU
+function get_field_noalloc!(a::Vector{Float64}, s::FFSSimulation, field::Integer)
+    GC.@preserve a check(ccall((:ffs_get_field_noalloc, libffs), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Csize_t),
+                               s.h, field, a, length(a)), s.h)
+    return a
+end
+
 initialize!(s::FFSSimulation) = check(ccall((:ffs_initialize, libffs), Cint, (Ptr{Cvoid},), s.h), s.h)
 
 function advance!(s::FFSSimulation, n::Integer)

@@ -233,6 +233,16 @@ int ffs_seed_density(ffs_sim* s, uint64_t seed, double k0, double rms);
  * complex views of the state: copies one component out in the padded layout. */
 int ffs_get_field(ffs_sim* s, int field, int space, double* host, size_t n_doubles);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the real-space
+ * read-out of ffs_get_field(s, field, FFS_SPACE_REAL, host, n) for the state fields u1 u2 u3 rho
+ * scalar, with the same result, calling no hipMalloc / hipFree.  The inverse transforms use the
+ * run's own pencil workspace and the last pass stages slabs of whole z planes through one
+ * library-wide 32 MiB device array per GPU; a z plane (2*(nx/2+1)*ny doubles) larger than that is
+ * rejected with FFS_ERR_ARG (no compiled size reaches it).  Calls that overlap on one GPU (two
+ * host threads, any handles) race on that array: serialise them.  Synchronous, like ffs_get_field. */
+int ffs_get_field_noalloc(ffs_sim* s, int field, double* host, size_t n_doubles);
+
 /* Replaces initialize!(s) (FluidFlowSimulation.jl:67-91). */
 int ffs_initialize(ffs_sim* s);
 
