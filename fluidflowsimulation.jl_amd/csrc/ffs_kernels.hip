@@ -349,7 +349,9 @@ static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, con
                                                     (T)scale);
 }
 
+#ifndef FFS_SIZES
 #define FFS_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(512)
+#endif
 
 [[noreturn]] static void bad_size(const char* what, int n) {
   throw std::runtime_error(std::string("unsupported transform length ") + std::to_string(n) + " in " + what);
@@ -941,6 +943,208 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
   if (MODE != 1 && vmax) block_max_to(lmax, vmax);
 }
 
+// ---------------------------------------------------------------------------
+// The x pass with mirror-paired lanes (FP64, 8 points per lane, TP = N/8 lanes per row).  The
+// inverse transforms start with a radix-4 stage and the forward transforms end with one (the same
+// plan run in reverse order), and those stages deal their N/4 butterflies to the lanes in mirror
+// pairs: lane q owns butterflies q and N/4 - q (lane 0: 0 and N/8), i.e. the positions
+// p(t, r) = b_t + r N/4, which hold Z[p] and Z[N - p] together.  So each lane loads just the
+// retained modes k < mx among its own positions -- every mode of a row is read once, no load for the
+// 2/3 zero band -- and builds both Z[k] and Z[N - k] from them, and after the forward transforms it
+// splits Z[k], Z[N - k] into the two half spectra in registers, without the LDS round trip of a
+// split by position.  Lo legs r < 2 lie below N/2; the hi legs are their mirrors:
+//   q > 0: (t, r) <-> (1 - t, 3 - r);  q = 0: (0, r) <-> (0, (4 - r) % 4), (1, r) <-> (1, 3 - r).
+#ifndef FFS_XNL_PAIR
+#define FFS_XNL_PAIR 1
+#endif
+template <int N> __host__ __device__ constexpr int xpair_maxr() { return plan_r(N, 0, 8) == 4 ? 8 : 4; }
+template <typename T, int N> __host__ __device__ constexpr bool xpair_ok() {
+  return FFS_XNL_PAIR && sizeof(T) == 8 && N % 8 == 0 && 64 % (N / 8) == 0 &&
+         plan_r(N, 0, xpair_maxr<N>()) == 4 && plan_even(N, N / 8, xpair_maxr<N>());
+}
+
+#ifndef FFS_XPAIR_WAVES
+#define FFS_XPAIR_WAVES 2  // minimum waves per SIMD the mirror-paired pass's registers must allow (3 spills)
+#endif
+// The same pointer, opaque to the optimiser: each transform of a row then reads its own twiddles instead
+// of one set being computed once and kept live, in registers, across all the row's transforms.
+template <typename P> __device__ __forceinline__ P* opaque(P* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+template <typename T, int N, int MODE, bool KE>
+__global__ void __launch_bounds__(128, FFS_XPAIR_WAVES) k_x_pair(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
+                                               const C<T>* __restrict__ tw_g, unsigned long long* __restrict__ vmax,
+                                               XRegArgs xa) {
+  constexpr int TP = N / 8;
+  constexpr int GPW = 64 / TP;  // rows per wavefront
+  constexpr int MAXR = xpair_maxr<N>();
+  constexpr int LDX = xreg_ldx<T, N>();
+  constexpr int NSTG = plan_ns(N, MAXR);
+  constexpr int M = N / 4;  // butterflies of the radix-4 end stages
+  // register stride of the inverse's last-stage legs = of the forward's first-stage legs
+  constexpr int ML = (N / plan_r(N, NSTG - 1, MAXR)) / TP;
+  static_assert(ML == (N / plan_r(N, 0, MAXR, true)) / TP, "the two plans meet at the same positions");
+  constexpr int NPI = xreg_npin<MODE>();
+  constexpr int NPO = xreg_npout<MODE, KE>();
+  static_assert(xpair_ok<T, N>(), "mirror-paired x pass: FP64, a radix-4 end stage, an even plan");
+  extern __shared__ unsigned char smem_raw[];
+  C<T>* twl = reinterpret_cast<C<T>*>(smem_raw);
+  C<T>* lds = twl + tw_lds<T>(N);
+  const C<T>* tw = stage_twiddles<T, N>(twl, tw_g);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int grp = lane / TP, q = lane % TP;
+  const int nrows = g.ny * g.nz;
+  const int row = (blockIdx.x * (int)(blockDim.x >> 6) + wave) * GPW + grp;
+  const bool row_ok = row < nrows;
+  const int mx = g.mx;
+  const size_t S2 = (size_t)g.sx * nrows;
+  const size_t roff = (size_t)(row_ok ? row : 0) * g.sx;
+  C<T>* X = lds + (size_t)(wave * GPW + grp) * LDX;
+  auto xref = [&](int p) -> C<T>& {
+    if constexpr (FFS_XNL_PAD_SHIFT > 0) return X[p + (p >> FFS_XNL_PAD_SHIFT)];
+    else return X[p];
+  };
+  auto slot_a = [&](int j) -> int {
+    if constexpr (MODE == 1) return xa.in0 + 2 * j;
+    else return 2 * j;
+  };
+  auto slot_b = [&](int j) -> int {
+    if constexpr (MODE == 0) return 2 * j + 1;
+    else if constexpr (MODE == 1) return j < 2 ? xa.in0 + 2 * j + 1 : -1;
+    else if constexpr (MODE == 3) return j == 0 ? 1 : -1;
+    else return j == 0 ? 1 : xa.in0;
+  };
+  const bool q0 = q == 0;
+  const int bt1 = q0 ? M / 2 : M - q;
+  auto pos = [&](int t, int r) { return (t == 0 ? q : bt1) + r * M; };
+  const C<T> zero = mk<T>(0, 0);
+
+  // the modes a pair needs: A[k], B[k] at the lo legs (k < mx), read with 32-bit offsets from the
+  // fields' uniform bases; pair j + 1's loads are issued before pair j's transform
+  const unsigned rbase = (unsigned)(row_ok ? row : 0) * (unsigned)g.sx;
+  C<T> la[2][2][2], lb[2][2][2];  // [buffer][t][r]
+  auto fetch = [&](int j, C<T> (&fa)[2][2], C<T> (&fb)[2][2]) {
+    const C<T>* A = a2 + (size_t)slot_a(j) * S2;
+    const int sb = slot_b(j);
+    const C<T>* B = a2 + (size_t)(sb < 0 ? 0 : sb) * S2;
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int k = pos(t, r);
+        const unsigned o = rbase + (unsigned)((row_ok && k < mx) ? k : 0);  // every lane loads
+        fa[t][r] = A[o];
+        fb[t][r] = sb < 0 ? zero : B[o];
+      }
+  };
+  C<T> R[3][8];
+  fetch(0, la[0], lb[0]);
+#pragma unroll
+  for (int j = 0; j < NPI; ++j) {
+    if (j + 1 < NPI) fetch(j + 1, la[(j + 1) & 1], lb[(j + 1) & 1]);
+    // Z of the pair at the lane's 8 positions: A + iB at the lo legs, conj(A) + i conj(B) mirrored
+    C<T> Z[2][4], mh[2][2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int k = pos(t, r);
+        const bool in = row_ok && k < mx;
+        const C<T> a = la[j & 1][t][r], b = lb[j & 1][t][r];
+        Z[t][r] = in ? (k == 0 ? mk<T>(a.x, b.x) : mk<T>(a.x - b.y, a.y + b.x)) : zero;
+        mh[t][r] = in ? mk<T>(a.x + b.y, b.x - a.y) : zero;
+      }
+    Z[0][2] = q0 ? zero : mh[1][1];  // lane 0: the Nyquist position, zero (N/2 >= mx)
+    Z[0][3] = q0 ? mh[0][1] : mh[1][0];
+    Z[1][2] = q0 ? mh[1][1] : mh[0][1];
+    Z[1][3] = q0 ? mh[1][0] : mh[0][0];
+    auto ld = [&](int, int t, int r) -> C<T> { return Z[t][r]; };
+    auto st = [&](int, int t, int r, C<T> v) { R[j][t + r * ML] = v; };
+    if (j > 0) __syncwarp();
+    Fft<T, N, +1, TP, false, false, SYNC_WAVE, MAXR, 0, false, 0>::run(q, opaque(tw), ld, st, xref);
+  }
+  // pointwise products at positions q + TP m
+  double lmax = 0.0;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    if constexpr (MODE == 0) {  // u x w, 1/2|u|^2, max|u_i|
+      const T ux = R[0][m].x, uy = R[0][m].y, uz = R[1][m].x, wx = R[1][m].y, wy = R[2][m].x, wz = R[2][m].y;
+      T ke = 0;
+      if constexpr (KE) ke = (T)0.5 * (ux * ux + uy * uy + uz * uz);
+      R[0][m] = mk<T>(uy * wz - uz * wy, uz * wx - ux * wz);
+      R[1][m] = mk<T>(ux * wy - uy * wx, ke);
+      if (row_ok) lmax = nanmax(lmax, (double)nanmax(fabs(ux), nanmax(fabs(uy), fabs(uz))));
+    } else if constexpr (MODE == 1) {  // tau = c2a |S| S
+      const T sxx = R[0][m].x, sxy = R[0][m].y, sxz = R[1][m].x, syy = R[1][m].y, syz = R[2][m].x;
+      const T szz = -sxx - syy;
+      const T ss = sxx * sxx + syy * syy + szz * szz + (T)2 * (sxy * sxy + sxz * sxz + syz * syz);
+      const T f = (T)xa.c2a * sqrt((T)2 * ss);
+      R[0][m] = mk<T>(f * sxx, f * sxy);
+      R[1][m] = mk<T>(f * sxz, f * syy);
+      R[2][m] = mk<T>(f * syz, (T)0);
+    } else if constexpr (MODE == 3) {  // traceless(symouter(u, -u)) (models.jl:262), 1/2|u|^2, max|u_i|
+      const T ux = R[0][m].x, uy = R[0][m].y, uz = R[1][m].x;
+      const T xx = -(ux * ux), yy = -(uy * uy), zz = -(uz * uz);
+      const T t = (xx + yy + zz) / (T)3;
+      T ke = 0;
+      if constexpr (KE) ke = (T)0.5 * (ux * ux + uy * uy + uz * uz);
+      R[0][m] = mk<T>(xx - t, -(ux * uy));
+      R[1][m] = mk<T>(-(ux * uz), yy - t);
+      R[2][m] = mk<T>(-(uy * uz), ke);
+      if (row_ok) lmax = nanmax(lmax, (double)nanmax(fabs(ux), nanmax(fabs(uy), fabs(uz))));
+    } else {  // -rho u, max|rho|
+      const T ux = R[0][m].x, uy = R[0][m].y, uz = R[1][m].x, rho = R[1][m].y;
+      R[0][m] = mk<T>((-rho) * ux, (-rho) * uy);
+      R[1][m] = mk<T>((-rho) * uz, (T)0);
+      if (row_ok) lmax = nanmax(lmax, (double)fabs(rho));
+    }
+  }
+  // forward transforms (the plan reversed, mirror pairs in its last stage), split in registers
+#pragma unroll
+  for (int j = 0; j < NPO; ++j) {
+    C<T> W[2][4];
+    auto ld = [&](int, int t, int r) { return R[j][t + r * ML]; };
+    auto st = [&](int, int t, int r, C<T> v) { W[t][r] = v; };
+    __syncwarp();
+    Fft<T, N, -1, TP, false, false, SYNC_WAVE, MAXR, 0, true, NSTG - 1>::run(q, opaque(tw), ld, st, xref);
+    int oa, ob;
+    if constexpr (MODE == 0) {
+      oa = 2 * j;
+      ob = j == 0 ? 1 : (KE ? xa.ke_slot : -1);
+    } else if constexpr (MODE == 1) {
+      oa = xa.out0 + 2 * j;
+      ob = j < 2 ? xa.out0 + 2 * j + 1 : -1;
+    } else if constexpr (MODE == 3) {
+      oa = 2 * j;
+      ob = j < 2 ? 2 * j + 1 : (KE ? xa.ke_slot : -1);
+    } else {
+      oa = xa.out0 + 2 * j;
+      ob = j == 0 ? xa.out0 + 1 : -1;
+    }
+    C<T> zn[2][2];  // Z[N - k] of the lo legs
+    zn[0][0] = q0 ? W[0][0] : W[1][3];
+    zn[0][1] = q0 ? W[0][3] : W[1][2];
+    zn[1][0] = q0 ? W[1][3] : W[0][3];
+    zn[1][1] = q0 ? W[1][2] : W[0][2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int k = pos(t, r);
+        if (row_ok && k < mx) {
+          const C<T> zk = W[t][r], zm = zn[t][r];
+          const unsigned o = rbase + (unsigned)k;
+          (b2 + (size_t)oa * S2)[o] = mk<T>((T)0.5 * (zk.x + zm.x), (T)0.5 * (zk.y - zm.y));
+          if (ob >= 0) (b2 + (size_t)ob * S2)[o] = mk<T>((T)0.5 * (zk.y + zm.y), (T)0.5 * (zm.x - zk.x));
+        }
+      }
+  }
+  if (MODE != 1 && vmax) block_max_to(lmax, vmax);
+}
+
 // C2R of nf fields (rows paired 2m, 2m+1) into the padded real layout
 // (2*nxh, ny, nz), padding entries zeroed.  Only the rows [row0, row0 + nrow) of each field are
 // transformed and written (row0 even; row = y + ny z), at row - row0 of an output of nrow rows per
@@ -1066,15 +1270,23 @@ static void run_x_gen(const Geo& g, const void* a2, void* b2, const XOps& ops, u
 template <typename T, int N, int MODE, bool KE>
 static void launch_xreg(const Geo& g, const void* a2, void* b2, const void* tw, unsigned long long* vmax,
                         const XRegArgs& xa, hipStream_t s) {
-  constexpr int TP = N / xreg_ppt<T, N>();
+  constexpr bool PAIR = xpair_ok<T, N>();
+  constexpr int TP = PAIR ? N / 8 : N / xreg_ppt<T, N>();
   constexpr int WPB = 2;  // wavefronts per block
   constexpr int RPB = WPB * (64 / TP);
   const size_t lds = ((size_t)RPB * xreg_ldx<T, N>() + tw_lds<T>(N)) * sizeof(C<T>);
-  static std::once_flag once;
-  lds_limit_once(once, (const void*)&k_x_nl_reg<T, N, MODE, KE>, lds, "the x pass");
   const int nrows = g.ny * g.nz;
-  k_x_nl_reg<T, N, MODE, KE><<<(nrows + RPB - 1) / RPB, 64 * WPB, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
+  if constexpr (PAIR) {
+    static std::once_flag once;
+    lds_limit_once(once, (const void*)&k_x_pair<T, N, MODE, KE>, lds, "the x pass");
+    k_x_pair<T, N, MODE, KE><<<(nrows + RPB - 1) / RPB, 64 * WPB, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
                                                                           (const C<T>*)tw, vmax, xa);
+  } else {
+    static std::once_flag once;
+    lds_limit_once(once, (const void*)&k_x_nl_reg<T, N, MODE, KE>, lds, "the x pass");
+    k_x_nl_reg<T, N, MODE, KE><<<(nrows + RPB - 1) / RPB, 64 * WPB, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
+                                                                            (const C<T>*)tw, vmax, xa);
+  }
 }
 
 // The x pass a run's options select: 0 for the register pass (one launch per product mode), else

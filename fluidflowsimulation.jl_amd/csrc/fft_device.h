@@ -215,10 +215,13 @@ __host__ __device__ constexpr Plan make_plan(int N, int maxr = 16) {
 }
 
 __host__ __device__ constexpr int plan_ns(int N, int maxr = 16) { return make_plan(N, maxr).ns; }
-__host__ __device__ constexpr int plan_r(int N, int s, int maxr = 16) { return make_plan(N, maxr).r[s]; }
-__host__ __device__ constexpr int plan_nsprev(int N, int s, int maxr = 16) {
+// REV: the plan's stages in reverse order (the small radix last)
+__host__ __device__ constexpr int plan_r(int N, int s, int maxr = 16, bool rev = false) {
+  return make_plan(N, maxr).r[rev ? make_plan(N, maxr).ns - 1 - s : s];
+}
+__host__ __device__ constexpr int plan_nsprev(int N, int s, int maxr = 16, bool rev = false) {
   int v = 1;
-  for (int i = 0; i < s; ++i) v *= make_plan(N, maxr).r[i];
+  for (int i = 0; i < s; ++i) v *= plan_r(N, i, maxr, rev);
   return v;
 }
 // points handled per thread: 16 if it divides N, else 8, 12, ... (always >= the largest radix of
@@ -239,18 +242,32 @@ __host__ __device__ constexpr int default_ppt(int N) {
 #define FFS_TWIDDLE_POWERS 1
 #endif
 // TWIDDLE_POWERS: derive W^r (r >= 2) from W^1 instead of reading every leg's twiddle
-template <typename T, int N, int DIR, int STAGE, int TPER, int MAXR = 16, bool TWIDDLE_POWERS = FFS_TWIDDLE_POWERS>
+//
+// MIR: the stage's 2 TPER butterflies are dealt to the threads in mirror pairs, thread q owning b = q
+// and b = NBF - q (thread 0: 0 and NBF/2).  A first stage (NS = 1) reads positions b + r NBF, a last
+// stage (NS = NBF) writes them, and the mirror of b + r NBF is (NBF - b) + (R - 1 - r) NBF: each
+// thread then holds x[p] and x[N - p] together, which is what a pair of real rows packed into one
+// complex sequence needs to be built from, or split into, its half spectra.
+template <typename T, int N, int DIR, int STAGE, int TPER, int MAXR = 16, bool TWIDDLE_POWERS = FFS_TWIDDLE_POWERS,
+          bool REV = false, bool MIR = false>
 struct Stage {
-  static constexpr int R = plan_r(N, STAGE, MAXR);
-  static constexpr int NS = plan_nsprev(N, STAGE, MAXR);
+  static constexpr int R = plan_r(N, STAGE, MAXR, REV);
+  static constexpr int NS = plan_nsprev(N, STAGE, MAXR, REV);
   static constexpr int NBF = N / R;                       // butterflies
   static constexpr int NB = (NBF + TPER - 1) / TPER;      // per thread
+  static_assert(!MIR || (NB == 2 && NBF == 2 * TPER && (NS == 1 || NS == NBF)),
+                "mirror-paired butterflies: two per thread, in a first or last stage");
+
+  static __device__ __forceinline__ int bfly(int q, int t) {
+    if constexpr (MIR) return t == 0 ? q : (q == 0 ? NBF / 2 : NBF - q);
+    else return q + t * TPER;
+  }
 
   template <class Load>
   static __device__ __forceinline__ void load(C<T> (&v)[NB][R], int q, const Load& ld) {
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
-      const int b = q + t * TPER;
+      const int b = bfly(q, t);
       if (NB * TPER == NBF || b < NBF) {
 #pragma unroll
         for (int r = 0; r < R; ++r) v[t][r] = ld(b + r * NBF, t, r);
@@ -261,7 +278,7 @@ struct Stage {
   static __device__ __forceinline__ void compute(C<T> (&v)[NB][R], int q, const C<T>* __restrict__ tw) {
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
-      const int b = q + t * TPER;
+      const int b = bfly(q, t);
       if (NB * TPER == NBF || b < NBF) {
         if constexpr (NS > 1) {
           const int j = b % NS;
@@ -291,7 +308,7 @@ struct Stage {
   static __device__ __forceinline__ void store(C<T> (&v)[NB][R], int q, const Store& st) {
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
-      const int b = q + t * TPER;
+      const int b = bfly(q, t);
       if (NB * TPER == NBF || b < NBF) {
         const int base = (b / NS) * NS * R + (b % NS);
 #pragma unroll
@@ -319,8 +336,10 @@ template <int SYNC> __device__ __forceinline__ void fft_sync() {
 // The caller synchronises before the call if the LDS region was in use by a
 // previous transform, and after it before reading OUT_LDS results.  Every
 // thread of the synchronisation scope must call run().
+// REV runs the plan's stages in reverse order; stage MIRS (if >= 0) deals its butterflies in mirror
+// pairs (see Stage).
 template <typename T, int N, int DIR, int TPER, bool IN_LDS = false, bool OUT_LDS = false, int SYNC = SYNC_BLOCK,
-          int MAXR = 16, int STAGE = 0>
+          int MAXR = 16, int STAGE = 0, bool REV = false, int MIRS = -1>
 struct Fft {
   static constexpr int NSTG = plan_ns(N, MAXR);
   static_assert(NSTG > 0, "FFT length must be 2^a 3^b 5^c");
@@ -328,7 +347,7 @@ struct Fft {
   template <class Load, class Store, class Lds>
   static __device__ __forceinline__ void run(int q, const C<T>* __restrict__ tw, const Load& ld, const Store& st,
                                              const Lds& lds) {
-    using S = Stage<T, N, DIR, STAGE, TPER, MAXR>;
+    using S = Stage<T, N, DIR, STAGE, TPER, MAXR, FFS_TWIDDLE_POWERS, REV, STAGE == MIRS>;
     C<T> v[S::NB][S::R];
     S::load(v, q, ld);
     S::compute(v, q, tw);
@@ -340,7 +359,7 @@ struct Fft {
       if constexpr (reads_lds) fft_sync<SYNC>();  // everyone has read the previous contents
       S::store(v, q, [&](int p, int, int, C<T> x) { lds(p) = x; });
       fft_sync<SYNC>();
-      Fft<T, N, DIR, TPER, IN_LDS, OUT_LDS, SYNC, MAXR, STAGE + 1>::run(
+      Fft<T, N, DIR, TPER, IN_LDS, OUT_LDS, SYNC, MAXR, STAGE + 1, REV, MIRS>::run(
           q, tw, [&](int p, int, int) { return lds(p); }, st, lds);
     }
   }
