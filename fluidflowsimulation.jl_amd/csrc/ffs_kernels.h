@@ -294,6 +294,10 @@ void launch_div_uu(const Geo& g, const void* uu, void* rhs, hipStream_t s);
 template <typename T>
 void launch_plane_forcing(const Geo& g, const void* u, const void* rhs, DevState* st, const PlaneForcing& pf,
                           const UpdateArgs& ua, int rhs_final, int it, hipStream_t s);
+// the u (rho, phi) round trip of the reference on the kx = 0 and kx = N/2 planes: Hermitian part, then
+// the projection of u; needed only when a Nyquist mode is retained
+template <typename T>
+void launch_hermitian_planes(const Geo& g, void* u, void* rho, void* phi, hipStream_t s);
 template <typename T>
 void launch_project(const Geo& g, void* u, hipStream_t s);
 

@@ -54,9 +54,17 @@ constexpr int kXcd = 8;
 template <int N> __host__ __device__ constexpr bool ppt8_ok() { return N % 8 == 0 && plan_even(N, N / 8, 8); }
 // the column kernels run TX x TP threads per block, at most kColThreads (their launch bound)
 constexpr int kColThreads = 512;
+// columns per tile of the curl pass (the inverse z pass that forms w): FFS_ZCURL_TXMUL lines of x values
+#ifndef FFS_ZCURL_TXMUL
+#define FFS_ZCURL_TXMUL 1
+#endif
+template <typename T, int N, bool CURL> __host__ __device__ constexpr int col_txc() {
+  return (CURL && FFS_ZCURL_TXMUL * col_tx<T>(N) * tper<N>() <= kColThreads) ? FFS_ZCURL_TXMUL * col_tx<T>(N)
+                                                                              : col_tx<T>(N);
+}
 template <typename T, int N, bool CURL> __host__ __device__ constexpr bool col_ppt8() {
   return ppt8_ok<N>() && ((FFS_COL_PPT8 == 0 && CURL) || FFS_COL_PPT8 == 1) &&
-         col_tx<T>(N) * (N / 8) <= kColThreads;
+         col_txc<T, N, CURL>() * (N / 8) <= kColThreads;
 }
 template <typename T, int N, bool CURL> __host__ __device__ constexpr int col_tper() {
   return col_ppt8<T, N, CURL>() ? N / 8 : tper<N>();
@@ -173,7 +181,7 @@ enum ColKind { INV_Z = 0, INV_Y = 1, FWD_Y = 2, FWD_Z = 3 };
 template <typename T, int N, int KIND, bool CURL, bool GRAD = false>
 __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ out,
                                             ColAux<T> aux, const C<T>* __restrict__ tw_g, int nzero, T scale) {
-  constexpr int TX = col_tx<T>(N);
+  constexpr int TX = col_txc<T, N, CURL>();
   constexpr int TP = col_tper<T, N, CURL>();
   constexpr int MAXR = col_maxr<T, N, CURL>();
   static_assert(TX * TP <= kColThreads, "column block exceeds its launch bound");
@@ -336,7 +344,7 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
 template <typename T, int N, int KIND, bool CURL, bool GRAD = false>
 static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, const void* tw, int nzero, double scale,
                     hipStream_t s, const ColAux<T>& aux = ColAux<T>{}) {
-  constexpr int TX = col_tx<T>(N);
+  constexpr int TX = col_txc<T, N, CURL>();
   constexpr int TP = col_tper<T, N, CURL>();
   static_assert(TX * TP <= kColThreads, "column block exceeds its launch bound");
   const size_t lds = (size_t)(N * TX + tw_lds<T>(N)) * sizeof(C<T>);
@@ -958,6 +966,8 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
 #define FFS_XNL_PAIR 1
 #endif
 template <int N> __host__ __device__ constexpr int xpair_maxr() { return plan_r(N, 0, 8) == 4 ? 8 : 4; }
+// FP64 only: in FP32 the register x pass's 16 points per lane and one exchange per transform are faster
+// than 8 points and two exchanges (measured at 256^3: 189 against 214 us)
 template <typename T, int N> __host__ __device__ constexpr bool xpair_ok() {
   return FFS_XNL_PAIR && sizeof(T) == 8 && N % 8 == 0 && 64 % (N / 8) == 0 &&
          plan_r(N, 0, xpair_maxr<N>()) == 4 && plan_even(N, N / 8, xpair_maxr<N>());
@@ -1268,10 +1278,32 @@ static void run_x_gen(const Geo& g, const void* a2, void* b2, const XOps& ops, u
 }
 
 template <typename T, int N, int MODE, bool KE>
+static void launch_xreg_plain(const Geo& g, const void* a2, void* b2, const void* tw, unsigned long long* vmax,
+                              const XRegArgs& xa, hipStream_t s) {
+  constexpr int TP = N / xreg_ppt<T, N>();
+  constexpr int RPB = 2 * (64 / TP);
+  const size_t lds = ((size_t)RPB * xreg_ldx<T, N>() + tw_lds<T>(N)) * sizeof(C<T>);
+  static std::once_flag once;
+  lds_limit_once(once, (const void*)&k_x_nl_reg<T, N, MODE, KE>, lds, "the x pass");
+  const int nrows = g.ny * g.nz;
+  k_x_nl_reg<T, N, MODE, KE><<<(nrows + RPB - 1) / RPB, 128, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
+                                                                       (const C<T>*)tw, vmax, xa);
+}
+
+template <typename T, int N, int MODE, bool KE>
 static void launch_xreg(const Geo& g, const void* a2, void* b2, const void* tw, unsigned long long* vmax,
                         const XRegArgs& xa, hipStream_t s) {
   constexpr bool PAIR = xpair_ok<T, N>();
-  constexpr int TP = PAIR ? N / 8 : N / xreg_ppt<T, N>();
+  // the mirror-paired pass assumes the x Nyquist mode is dealiased (mx <= N/2; cutoff < 1)
+  if (g.mx > N / 2 || !PAIR) {
+    if constexpr (xreg_ok<T, N>()) {
+      launch_xreg_plain<T, N, MODE, KE>(g, a2, b2, tw, vmax, xa, s);
+      return;
+    } else {
+      throw std::runtime_error("x pass: no register variant for this length");
+    }
+  }
+  constexpr int TP = N / 8;
   constexpr int WPB = 2;  // wavefronts per block
   constexpr int RPB = WPB * (64 / TP);
   const size_t lds = ((size_t)RPB * xreg_ldx<T, N>() + tw_lds<T>(N)) * sizeof(C<T>);
@@ -1281,11 +1313,6 @@ static void launch_xreg(const Geo& g, const void* a2, void* b2, const void* tw, 
     lds_limit_once(once, (const void*)&k_x_pair<T, N, MODE, KE>, lds, "the x pass");
     k_x_pair<T, N, MODE, KE><<<(nrows + RPB - 1) / RPB, 64 * WPB, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
                                                                           (const C<T>*)tw, vmax, xa);
-  } else {
-    static std::once_flag once;
-    lds_limit_once(once, (const void*)&k_x_nl_reg<T, N, MODE, KE>, lds, "the x pass");
-    k_x_nl_reg<T, N, MODE, KE><<<(nrows + RPB - 1) / RPB, 64 * WPB, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
-                                                                            (const C<T>*)tw, vmax, xa);
   }
 }
 
@@ -2275,6 +2302,65 @@ void launch_fwd_z_update(const Geo& g, const void* b1, void* u, void* rhs, const
 #undef X
     default: bad_size("forward z pass", g.nz);
   }
+}
+
+// The reference carries the velocity (and rho, phi) through real space every step (real!(u), then
+// myfourier!(u) and pressure_projection!(u), models.jl:122-193).  That round trip is the identity except on
+// the kx = 0 and kx = N/2 planes, where the C2R keeps only the Hermitian part of each pair of modes
+// k, -k: f(k) <- (f(k) + conj f(-k)) / 2.  This kernel applies it, and the projection after it to the
+// velocity (its first three fields), once per step.  It matters only when a Nyquist mode is retained
+// (cutoff >= 1): the curl, the divergence and the projection at ky or kz = -N/2, whose partner index is
+// itself, are not Hermitian there; with the Nyquist rows dealiased the planes already are.
+// Thread (plane, cy, cz) with its partner (cy', cz') = (-ky, -kz): the lower index writes both.
+template <typename T>
+__global__ void __launch_bounds__(256) k_hermitian_planes(Geo g, C<T>* __restrict__ u, C<T>* __restrict__ rho,
+                                                          C<T>* __restrict__ phi, int nplane) {
+  const size_t n = (size_t)g.sx * g.my * g.mz;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = g.my * g.mz;
+  if (t >= nplane * per) return;
+  const int plane = t / per, cy = (t % per) % g.my, cz = (t % per) / g.my;
+  const int cx = plane == 0 ? 0 : g.mx - 1;  // the second plane is kx = N/2 (retained only when mx = N/2 + 1)
+  const int py = g.yinv[(g.ny - g.ymap[cy]) % g.ny], pz = g.zinv[(g.nz - g.zmap[cz]) % g.nz];
+  if (py < 0 || pz < 0) return;
+  const size_t a = cx + (size_t)g.sx * (cy + (size_t)g.my * cz);
+  const size_t b = cx + (size_t)g.sx * (py + (size_t)g.my * pz);
+  if (b < a) return;
+  auto herm = [&](C<T>* f) {
+    const C<T> fa = f[a], fb = f[b];
+    const C<T> h = mk<T>((T)0.5 * (fa.x + fb.x), (T)0.5 * (fa.y - fb.y));
+    f[a] = h;
+    f[b] = mk<T>(h.x, -h.y);
+  };
+  for (int c = 0; c < 3; ++c) herm(u + c * n);
+  if (rho) herm(rho);
+  if (phi) herm(phi);
+  const size_t mode[2] = {a, b};
+  const int my_[2] = {cy, py}, mz_[2] = {cz, pz};
+  for (int m = 0; m < (b == a ? 1 : 2); ++m) {
+    const size_t idx = mode[m];
+    if (idx == 0) continue;
+    const double kx = g.kx[cx], ky = g.ky[my_[m]], kz = g.kz[mz_[m]];
+    const double K2 = kx * kx + ky * ky + kz * kz;
+    const C<T> v0 = u[idx], v1 = u[n + idx], v2 = u[2 * n + idx];
+    double ux[2] = {v0.x, v0.y}, uy[2] = {v1.x, v1.y}, uz[2] = {v2.x, v2.y};
+    for (int c = 0; c < 2; ++c) {
+      const double p1 = -(kx * ux[c] + ky * uy[c] + kz * uz[c]) / K2;
+      ux[c] = p1 * kx + ux[c];
+      uy[c] = p1 * ky + uy[c];
+      uz[c] = p1 * kz + uz[c];
+    }
+    u[idx] = mk<T>((T)ux[0], (T)ux[1]);
+    u[n + idx] = mk<T>((T)uy[0], (T)uy[1]);
+    u[2 * n + idx] = mk<T>((T)uz[0], (T)uz[1]);
+  }
+}
+
+template <typename T>
+void launch_hermitian_planes(const Geo& g, void* u, void* rho, void* phi, hipStream_t s) {
+  const int nplane = g.mx == g.nxh ? 2 : 1;
+  const int nt = nplane * g.my * g.mz;
+  k_hermitian_planes<T><<<(nt + 255) / 256, 256, 0, s>>>(g, (C<T>*)u, (C<T>*)rho, (C<T>*)phi, nplane);
 }
 
 // Leray projection of a spectral vector field (pressure_projection!, models.jl:443-456)
@@ -3320,6 +3406,7 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   template void launch_fwd_z_update<T>(const Geo&, const void*, void*, void*, const void*, const void*,              \
                                        const DevState*, const UpdateArgs&, double, const void*, hipStream_t);       \
   template void launch_project<T>(const Geo&, void*, hipStream_t);                                                   \
+  template void launch_hermitian_planes<T>(const Geo&, void*, void*, void*, hipStream_t);                           \
   template void launch_div_uu<T>(const Geo&, const void*, void*, hipStream_t);                                     \
   template void launch_les_tau_real<T>(T*, size_t, double, hipStream_t);                                          \
   template void launch_les_model_real<T>(T*, const T*, size_t, const LesArgs&, hipStream_t);                      \

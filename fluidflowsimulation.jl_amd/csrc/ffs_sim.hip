@@ -1023,6 +1023,11 @@ struct ffs_sim {
     if (!e.empty()) throw FfsError(FFS_ERR_ARG, e);
   }
 
+  // a Nyquist index is retained in some direction (cutoff >= 1): the Hermitian planes need restoring
+  bool nyquist_kept() const {
+    return g.mx == g.nxh || (g.ny % 2 == 0 && g.my == g.ny) || (g.nz % 2 == 0 && g.mz == g.nz);
+  }
+
   bool is_output_time(int64_t it) const { return tit != 0 && it % p.write_time == 0; }
 
   // calculate_rhs! up to (not including) fourierspacep2!: rhs <- dealias(F(u x w))/N^3,
@@ -1054,6 +1059,10 @@ struct ffs_sim {
     unsigned long long* um = p.variable_dt ? &st->umax : nullptr;
     unsigned long long* rm = (p.variable_dt && dens) ? &st->rhomax : nullptr;
     launch("x_nl", [&] { FFS_DISPATCH(launch_x_nl, g, w2, w1, ops, um, rm, tw_x, stream); });
+    // the state's round trip through real space (its inverse transforms above are done)
+    if (nyquist_kept())
+      launch("hermitian", [&] { FFS_DISPATCH(launch_hermitian_planes, g, u, dens ? rho : nullptr,
+                                             pscal ? phi : nullptr, stream); });
     const int nf = ops.nout;
     if (fused) {
       step_scalars(true, true);  // reads u and the maximum the x pass left, as after the forward passes

@@ -68,3 +68,30 @@ def test_long_transforms_match_oracle(shape, model, prec):
         if rho0 is not None:
             rerr = rel(s.density_hat(), ref.rhoh)
             assert rerr < tol, ("density", rerr)
+
+
+CUTS = [(256, 0.9), (256, 1.0), (64, 1.0), (32, 0.8)]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("nx,cutoff", CUTS, ids=[f"{n}-cut{c}" for n, c in CUTS])
+def test_other_cutoffs_match_oracle(nx, cutoff, prec):
+    """A wider retained set (cutoff 0.8, 0.9) and no dealiasing at all (cutoff 1: the x Nyquist mode is
+    kept, so the x pass takes its register variant that builds and splits rows by position)."""
+    print_case(f"cutoff {cutoff} nx={nx} {prec}")
+    ny = nz = 16
+    g = Grid(nx, ny, nz, 1.0, 1.0, 1.0, cutoff=cutoff)
+    u0 = random_spectrum(nx, ny, nz, seed=9, cutoff=cutoff)
+    ref = OracleSim(g, 2e-3, dt=1e-3, variable_dt=False, dt_stat=1 << 30)
+    ref.set_velocity(u0)
+    p = ffs.Params(nx=nx, ny=ny, nz=nz, nu=2e-3, integrator="Adams_Bashforth3rdO", variable_dt=False, dt=1e-3,
+                   dt_stat=1 << 30, dt_spec=1 << 30, write_time=1 << 30, precision=prec, cutoff=cutoff)
+    with ffs.Simulation(p) as s:
+        s.set_velocity(*u0)
+        ref.initialize()
+        s.initialize()
+        ref.advance_in_time(3)
+        s.advance(3)
+        got = s.velocity_hat()
+        err = max(rel(got[c], ref.uh[c]) for c in range(3))
+        assert err < (1e-11 if prec == "fp64" else 2e-5), err
