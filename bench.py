@@ -387,7 +387,8 @@ def leg_fp32(args, n, nu, fp64_value):
 def leg_ensemble(args):
     """Batched LES ensemble: `ensemble_runs` Smagorinsky (c=0.16) runs, sizes drawn uniformly from
     {32,48,64,96,128}^3, nu log-uniform in [1e-4, 1e-2], own seeded field, shell forcing, FP32,
-    AB3 CFL 0.5, advanced in lockstep (one graph launch per steady step)."""
+    AB3 CFL 0.5, advanced in lockstep (one graph launch per steady step; runs of one size share one launch
+    per pass).  The same runs timed as one graph branch per run are reported beside it."""
     import numpy as np
 
     import ffs
@@ -402,12 +403,24 @@ def leg_ensemble(args):
     pts = float(sum(n**3 for n in sizes))
     K = args.ensemble_steps
     with ffs.Ensemble(sims) as ens:
+        # the same runs with one graph branch per run (no merged launches), over a shorter stretch
+        ens.set_merge(False)
+        ens.advance_in_time(args.warmup)
+        ens.synchronize()
+        kb = max(1, K // 4)
+        t0 = time.perf_counter()
+        ens.advance_in_time(kb)
+        ens.synchronize()
+        el_branches = time.perf_counter() - t0
+        branch_info = ens.launch_info()
+        ens.set_merge(True)
         ens.advance_in_time(args.warmup)
         ens.synchronize()
         t0 = time.perf_counter()
         ens.advance_in_time(K)
         ens.synchronize()
         el = time.perf_counter() - t0
+        merged_info = ens.launch_info()
         lat = []
         for _ in range(args.latency_samples):
             a = time.perf_counter()
@@ -421,9 +434,16 @@ def leg_ensemble(args):
     out = {"runs": R, "steps": K, "value": pts * K / el, "unit": "N^3*steps/s (sum over runs)",
            "ms_per_lockstep_step": 1e3 * el / K, "step_ms_p50": float(np.percentile(lat, 50)),
            "step_ms_p99": float(np.percentile(lat, 99)), "latency_samples": len(lat), "finite_runs": finite,
-           "device_bytes": fp, "size_counts": {str(n): sizes.count(n) for n in sorted(set(sizes))}}
+           "device_bytes": fp, "size_counts": {str(n): sizes.count(n) for n in sorted(set(sizes))},
+           "launches_per_step": merged_info[0] if merged_info else None,
+           "merged_groups": merged_info[1] if merged_info else None,
+           "merged_runs": merged_info[2] if merged_info else None,
+           "branches": {"ms_per_lockstep_step": 1e3 * el_branches / kb, "steps": kb,
+                        "launches_per_step": branch_info[0] if branch_info else None}}
     log(f"ensemble: {1e3 * el / K:.3f} ms/lockstep step, {pts * K / el:.4e} N^3*steps/s, "
-        f"p50 {out['step_ms_p50']:.3f} ms p99 {out['step_ms_p99']:.3f} ms")
+        f"p50 {out['step_ms_p50']:.3f} ms p99 {out['step_ms_p99']:.3f} ms; launches/step {out['launches_per_step']} "
+        f"in {out['merged_groups']} groups ({out['merged_runs']} runs) vs {out['branches']['launches_per_step']} "
+        f"as branches at {out['branches']['ms_per_lockstep_step']:.3f} ms/step")
     return out
 
 

@@ -1,6 +1,7 @@
 // Kernel launch interface of the FFS HIP solver (internal to libffs_hip.so).
 #pragma once
 #include <string>
+#include <vector>
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <type_traits>
@@ -120,6 +121,24 @@ template <typename T> struct ColAux {
   const typename std::conditional<sizeof(T) == 8, double2, float2>::type* grad2 = nullptr;
   typename std::conditional<sizeof(T) == 8, double2, float2>::type* dst[kMaxFields] = {};  // forward z outputs
 };
+
+// ---- launch records: batching the launches of several runs' steps ----
+// A recorded launch of a batchable kernel: its batch form (an array of call records with the run index
+// folded into grid z), the launch shape and the record's bytes.
+struct KLaunch {
+  const void* batch = nullptr;
+  dim3 grid, block;
+  size_t lds = 0;
+  const char* name = "";
+  std::vector<unsigned char> call;
+};
+struct KRecorder {
+  std::vector<KLaunch> launches;
+};
+// set on a thread, batchable launches are recorded there instead of issued
+extern thread_local KRecorder* g_recorder;
+// one launch of k.batch over nrun runs whose call records (k.call's layout) are at `calls` (device memory)
+void issue_batch(const KLaunch& k, const void* calls, int nrun, hipStream_t s);
 
 // ---- launchers (T = double or float) ----
 // inverse z pass; curl = true: the nf physical-space inputs of the nonlinear term

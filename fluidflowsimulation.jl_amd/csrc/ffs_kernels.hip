@@ -54,17 +54,9 @@ constexpr int kXcd = 8;
 template <int N> __host__ __device__ constexpr bool ppt8_ok() { return N % 8 == 0 && plan_even(N, N / 8, 8); }
 // the column kernels run TX x TP threads per block, at most kColThreads (their launch bound)
 constexpr int kColThreads = 512;
-// columns per tile of the curl pass (the inverse z pass that forms w): FFS_ZCURL_TXMUL lines of x values
-#ifndef FFS_ZCURL_TXMUL
-#define FFS_ZCURL_TXMUL 1
-#endif
-template <typename T, int N, bool CURL> __host__ __device__ constexpr int col_txc() {
-  return (CURL && FFS_ZCURL_TXMUL * col_tx<T>(N) * tper<N>() <= kColThreads) ? FFS_ZCURL_TXMUL * col_tx<T>(N)
-                                                                              : col_tx<T>(N);
-}
 template <typename T, int N, bool CURL> __host__ __device__ constexpr bool col_ppt8() {
   return ppt8_ok<N>() && ((FFS_COL_PPT8 == 0 && CURL) || FFS_COL_PPT8 == 1) &&
-         col_txc<T, N, CURL>() * (N / 8) <= kColThreads;
+         col_tx<T>(N) * (N / 8) <= kColThreads;
 }
 template <typename T, int N, bool CURL> __host__ __device__ constexpr int col_tper() {
   return col_ppt8<T, N, CURL>() ? N / 8 : tper<N>();
@@ -171,6 +163,97 @@ static void lds_limit_once(std::once_flag& once, const void* fn, size_t bytes, c
   });
 }
 
+// Launches of the batchable kernels go through launch_call: a kernel's single form takes its arguments as
+// before, its batch form an array of call records (one per run, all of one launch shape) with the run index
+// folded into grid z; both run the same body.  With a recorder set on the calling thread the launch is recorded, not issued, so
+// the launches of several runs' steps can be merged into one launch per pass (issue_batch).
+thread_local KRecorder* g_recorder = nullptr;
+// a batch form's call records: read-only, the same for every thread of a block
+#define FFS_CALLS(T) const T* __restrict__
+
+// A struct that no thread of the kernel writes, at an address that is the same for the whole block, read
+// word by word through the constant address space: every word comes in with a scalar load and the copy
+// lives in SGPRs, as kernel arguments do.  (Read through a pointer the compiler cannot prove unwritten,
+// fields used late would be fetched with vector loads into VGPRs.)
+template <class S> __device__ __forceinline__ S load_const(const S* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  static_assert(sizeof(S) % 4 == 0 && alignof(S) >= 4, "read as whole words");
+  typedef const __attribute__((address_space(4))) unsigned cword;
+  cword* src = (cword*)p;
+  S c;
+  unsigned* d = reinterpret_cast<unsigned*>(&c);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(S) / 4); ++k) d[k] = src[k];
+  return c;
+#else
+  return *p;
+#endif
+}
+// run i's call record
+template <class Call> __device__ __forceinline__ Call load_call(const Call* calls, int i) { return load_const(calls + i); }
+
+// Every buffer the kernels touch is global memory.  The compiler knows it of pointer kernel arguments but
+// not of pointers read from memory (a call record, a struct argument), whose loads and stores it would
+// otherwise issue as flat instructions; glb() states it for a pointer that is the same across the block
+// (it passes through an SGPR pair).  The empty asm hands the compiler a global-address-space pointer it
+// cannot fold back into the generic one, so the accesses through it become global instructions.
+template <class P> __device__ __forceinline__ P* glb(P* p) {
+  __attribute__((address_space(1))) P* q;
+  asm("" : "=s"(q) : "0"(p));
+  return (P*)q;
+}
+__device__ __forceinline__ Geo glb(Geo g) {
+  g.ymap = glb(g.ymap);
+  g.zmap = glb(g.zmap);
+  g.yinv = glb(g.yinv);
+  g.zinv = glb(g.zinv);
+  g.kx = glb(g.kx);
+  g.ky = glb(g.ky);
+  g.kz = glb(g.kz);
+  return g;
+}
+__device__ __forceinline__ UpdateArgs glb(UpdateArgs a) {
+  a.pf_factor = glb(a.pf_factor);
+  a.tau = glb(a.tau);
+  a.rho = glb(a.rho);
+  a.rrhs = glb(a.rrhs);
+  a.rfm1 = glb(a.rfm1);
+  a.rfm2 = glb(a.rfm2);
+  a.flux = glb(a.flux);
+  a.time = glb(a.time);
+  a.phi = glb(a.phi);
+  a.prhs = glb(a.prhs);
+  a.pfm1 = glb(a.pfm1);
+  a.pfm2 = glb(a.pfm2);
+  a.pflux = glb(a.pflux);
+  return a;
+}
+
+template <class Call, class F>
+static void launch_call(const void* batch, dim3 grid, dim3 block, size_t lds, const Call& c, const char* name,
+                        F&& single) {
+  if (g_recorder) {
+    KLaunch k;
+    k.batch = batch;
+    k.grid = grid;
+    k.block = block;
+    k.lds = lds;
+    k.name = name;
+    k.call.assign(reinterpret_cast<const unsigned char*>(&c), reinterpret_cast<const unsigned char*>(&c) + sizeof(Call));
+    g_recorder->launches.push_back(std::move(k));
+    return;
+  }
+  single();
+}
+
+void issue_batch(const KLaunch& k, const void* calls, int nrun, hipStream_t s) {
+  dim3 grid = k.grid;
+  grid.z *= (unsigned)nrun;
+  void* args[] = {const_cast<void**>(&calls), &nrun};
+  const hipError_t e = hipLaunchKernel(k.batch, grid, k.block, args, k.lds, s);
+  if (e != hipSuccess) throw std::runtime_error(std::string("batched launch of ") + k.name + ": " + hipGetErrorString(e));
+}
+
 // ===========================================================================
 // Column (y / z) passes.  Block = TX consecutive x columns x TPER threads per
 // column; LDS holds one [N][TX] tile.
@@ -178,10 +261,21 @@ enum ColKind { INV_Z = 0, INV_Y = 1, FWD_Y = 2, FWD_Z = 3 };
 
 // GRAD: the curl pass also forms i k s of the scalars with an LES flux (a separate instantiation,
 // so the plain step's curl pass carries none of that selection in its loads)
-template <typename T, int N, int KIND, bool CURL, bool GRAD = false>
-__global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ out,
-                                            ColAux<T> aux, const C<T>* __restrict__ tw_g, int nzero, T scale) {
-  constexpr int TX = col_txc<T, N, CURL>();
+template <typename T> struct ColCall {
+  Geo g;
+  const C<T>* in;
+  C<T>* out;
+  ColAux<T> aux;
+  const C<T>* tw;
+  int nzero;
+  T scale;
+};
+
+template <typename T, int N, int KIND, bool CURL, bool GRAD>
+__device__ __forceinline__ void col_body(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ out, const ColAux<T>& aux,
+                                         const C<T>* __restrict__ tw_g, int nzero, T scale, int bX, int bY, int bZ) {
+  g = glb(g);
+  constexpr int TX = col_tx<T>(N);
   constexpr int TP = col_tper<T, N, CURL>();
   constexpr int MAXR = col_maxr<T, N, CURL>();
   static_assert(TX * TP <= kColThreads, "column block exceeds its launch bound");
@@ -193,15 +287,15 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
   int bx, bb, bf = 0;
   if constexpr (SPLIT) {
     const int ntx = (g.mx + TX - 1) / TX;
-    const int L = blockIdx.x, sq = L / kXcd;
+    const int L = bX, sq = L / kXcd;
     bf = sq % nzero;
     const int t = (sq / nzero) * kXcd + L % kXcd;
     if (t >= ntx * g.my) return;  // whole block: the grid is padded to a multiple of kXcd tiles
     bx = t % ntx;
     bb = t / ntx;
   } else {
-    bx = blockIdx.x;
-    bb = blockIdx.y;
+    bx = bX;
+    bb = bY;
   }
   extern __shared__ unsigned char smem_raw[];
   C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
@@ -224,8 +318,8 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
     // nonlinear term -- u (0..2), w = ik x u (3..5) (fourierspacep1!,
     // models.jl:88-95), rho (if aux.rho), phi (if aux.phi), S = sym(ik (x) u) (xx,xy,xz,yy,yz; if
     // aux.les) -- so the u tile is fetched from HBM once and re-read from L2
-    const int f0 = SPLIT ? bf : (CURL ? 0 : blockIdx.z);
-    const int f1 = SPLIT ? bf + 1 : (CURL ? nzero : blockIdx.z + 1);
+    const int f0 = SPLIT ? bf : (CURL ? 0 : bZ);
+    const int f1 = SPLIT ? bf + 1 : (CURL ? nzero : bZ + 1);
     const T kx = valid ? (T)g.kx[cx] : (T)0, ky = (T)g.ky[cy];
     for (int f = f0; f < f1; ++f) {
       if (f != f0) __syncthreads();
@@ -256,16 +350,16 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
         }
         int e = f - (aux.nsform ? 3 : 6);
         if (aux.rho) {
-          if (e == 0) return aux.rho[idx];
+          if (e == 0) return glb(aux.rho)[idx];
           --e;
         }
         if (aux.phi) {
-          if (e == 0) return aux.phi[idx];
+          if (e == 0) return glb(aux.phi)[idx];
           --e;
         }
         if (GRAD && e >= 5) {  // i k s of the scalars with an LES flux (fourierspacep1!, models.jl:110-116)
           e -= 5;
-          const C<T> sv = e < 3 ? aux.grad1[idx] : aux.grad2[idx];
+          const C<T> sv = e < 3 ? glb(aux.grad1)[idx] : glb(aux.grad2)[idx];
           const int d = e < 3 ? e : e - 3;
           const T kd = d == 0 ? kx : (d == 1 ? ky : kz);
           return mk<T>(-kd * sv.y, kd * sv.x);
@@ -296,7 +390,7 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
       Fft<T, N, +1, TP, false, false, SYNC_BLOCK, MAXR>::run(q, tw, ld, st, ldsf);
     }
   } else if constexpr (KIND == INV_Y) {
-    const int z = b, f = blockIdx.z;
+    const int z = b, f = bZ;
     auto ld = [&](int p, int, int) -> C<T> {
       const int cy = y_compact(g, p);
 #if FFS_COL_BRANCHFREE
@@ -312,7 +406,7 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
     };
     Fft<T, N, +1, TP, false, false, SYNC_BLOCK, MAXR>::run(q, tw, ld, st, ldsf);
   } else if constexpr (KIND == FWD_Y) {
-    const int z = b, f = blockIdx.z;
+    const int z = b, f = bZ;
     auto ld = [&](int p, int, int) -> C<T> {
       if (!valid) return mk<T>(0, 0);
       return in[f * S2 + cx + sx * (p + (size_t)g.ny * z)];
@@ -323,7 +417,7 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
     };
     Fft<T, N, -1, TP, false, false, SYNC_BLOCK, MAXR>::run(q, tw, ld, st, ldsf);
   } else {  // FWD_Z
-    const int cy = b, f = blockIdx.z;
+    const int cy = b, f = bZ;
     auto ld = [&](int p, int, int) -> C<T> {
       if (!valid) return mk<T>(0, 0);
       return in[f * S1 + cx + sx * (cy + (size_t)g.my * p)];
@@ -334,7 +428,7 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
         C<T> r = cscale<T>(v, scale);
         if (f < nzero && cx == 0 && cy == 0 && cz == 0) r = mk<T>(0, 0);
         const size_t o = cx + sx * (cy + (size_t)g.my * cz);
-        aux.dst[f][o] = r;  // rhs components, density flux, tau, 1/2|u|^2 ...
+        glb(aux.dst[f])[o] = r;  // rhs components, density flux, tau, 1/2|u|^2 ...
       }
     };
     Fft<T, N, -1, TP, false, false, SYNC_BLOCK, MAXR>::run(q, tw, ld, st, ldsf);
@@ -342,19 +436,36 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const
 }
 
 template <typename T, int N, int KIND, bool CURL, bool GRAD = false>
+__global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ out,
+                                            ColAux<T> aux, const C<T>* __restrict__ tw_g, int nzero, T scale) {
+  col_body<T, N, KIND, CURL, GRAD>(g, in, out, aux, tw_g, nzero, scale, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+template <typename T, int N, int KIND, bool CURL, bool GRAD = false>
+__global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col_b(FFS_CALLS(ColCall<T>) calls, int nrun) {
+  const int nz = gridDim.z / nrun;
+  const ColCall<T>& c = calls[blockIdx.z / nz];  // (aux.dst is indexed by the field: read in place)
+  col_body<T, N, KIND, CURL, GRAD>(c.g, glb(c.in), glb(c.out), c.aux, glb(c.tw), c.nzero, c.scale, blockIdx.x,
+                                   blockIdx.y, blockIdx.z % nz);
+}
+
+template <typename T, int N, int KIND, bool CURL, bool GRAD = false>
 static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, const void* tw, int nzero, double scale,
                     hipStream_t s, const ColAux<T>& aux = ColAux<T>{}) {
-  constexpr int TX = col_txc<T, N, CURL>();
+  constexpr int TX = col_tx<T>(N);
   constexpr int TP = col_tper<T, N, CURL>();
   static_assert(TX * TP <= kColThreads, "column block exceeds its launch bound");
   const size_t lds = (size_t)(N * TX + tw_lds<T>(N)) * sizeof(C<T>);
   static std::once_flag once;
   lds_limit_once(once, (const void*)&k_col<T, N, KIND, CURL, GRAD>, lds, "a column pass");
+  static std::once_flag once_b;
+  lds_limit_once(once_b, (const void*)&k_col_b<T, N, KIND, CURL, GRAD>, lds, "a batched column pass");
   const int ntile = (g.mx + TX - 1) / TX * nb;
   const dim3 grid = (CURL && FFS_CURL_SPLIT) ? dim3((unsigned)(nzero * ((ntile + kXcd - 1) / kXcd * kXcd)))
                                              : dim3((g.mx + TX - 1) / TX, nb, CURL ? 1 : nf);
-  k_col<T, N, KIND, CURL, GRAD><<<grid, TX * TP, lds, s>>>(g, (const C<T>*)in, (C<T>*)out, aux, (const C<T>*)tw, nzero,
-                                                    (T)scale);
+  const ColCall<T> c{g, (const C<T>*)in, (C<T>*)out, aux, (const C<T>*)tw, nzero, (T)scale};
+  launch_call((const void*)&k_col_b<T, N, KIND, CURL, GRAD>, grid, dim3(TX * TP), lds, c, "column pass", [&] {
+    k_col<T, N, KIND, CURL, GRAD><<<grid, TX * TP, lds, s>>>(c.g, c.in, c.out, c.aux, c.tw, c.nzero, c.scale);
+  });
 }
 
 #ifndef FFS_SIZES
@@ -610,11 +721,20 @@ __device__ __forceinline__ void scalar_flux(T s, T ux, T uy, T uz, int les, T nu
 // 2 (c Delta)^2 |S| S (5), calc_les! les_real_calc.jl:77-79 with
 // |S| = sqrt(2 S:S)], [1/2|u|^2].  Each row holds NSL slots of two real
 // sequences; outputs overwrite inputs slot by slot after the pointwise phase.
+template <typename T> struct XGCall {
+  Geo g;
+  const C<T>* a2;
+  C<T>* b2;
+  const C<T>* tw;
+  XOps ops;
+  unsigned long long* umax;
+  unsigned long long* rhomax;
+};
 template <typename T, int N, int NSL>
-__global__ void __launch_bounds__(256) k_x_gen(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
-                                              const C<T>* __restrict__ tw_g, XOps ops,
-                                              unsigned long long* __restrict__ umax,
-                                              unsigned long long* __restrict__ rhomax) {
+__device__ __forceinline__ void x_gen_body(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
+                                           const C<T>* __restrict__ tw_g, XOps ops, unsigned long long* __restrict__ umax,
+                                           unsigned long long* __restrict__ rhomax, int bX) {
+  g = glb(g);
   constexpr int TP = tper<N>();
   constexpr int ROWS = xgen_rows<N, NSL>();
   constexpr int LD = N + 1;
@@ -624,7 +744,7 @@ __global__ void __launch_bounds__(256) k_x_gen(Geo g, const C<T>* __restrict__ a
   const C<T>* tw = stage_twiddles<T, N>(lds + ROWS * NSL * LD, tw_g);  // published by the barrier below
   const int tid = threadIdx.x;
   const int nrows = g.ny * g.nz;
-  const int row0 = blockIdx.x * ROWS;
+  const int row0 = bX * ROWS;
   const int mx = g.mx;
   const size_t S2 = (size_t)g.sx * nrows;
   const int nin = ops.nin, nout = ops.nout;
@@ -757,6 +877,18 @@ __global__ void __launch_bounds__(256) k_x_gen(Geo g, const C<T>* __restrict__ a
   if (umax) block_max_to(lmax, umax);
   if (rhomax) block_max_to(rmax, rhomax);
 }
+template <typename T, int N, int NSL>
+__global__ void __launch_bounds__(256) k_x_gen(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
+                                              const C<T>* __restrict__ tw_g, XOps ops,
+                                              unsigned long long* __restrict__ umax,
+                                              unsigned long long* __restrict__ rhomax) {
+  x_gen_body<T, N, NSL>(g, a2, b2, tw_g, ops, umax, rhomax, blockIdx.x);
+}
+template <typename T, int N, int NSL>
+__global__ void __launch_bounds__(256) k_x_gen_b(FFS_CALLS(XGCall<T>) calls, int nrun) {
+  const XGCall<T> c = load_call(calls, blockIdx.z);
+  x_gen_body<T, N, NSL>(c.g, glb(c.a2), glb(c.b2), glb(c.tw), c.ops, glb(c.umax), glb(c.rhomax), blockIdx.x);
+}
 
 // Register-resident variant of the fused x pass (used when every FFT stage
 // splits evenly over the TPER lanes of a row, i.e. power-of-two nx): TPER
@@ -773,6 +905,14 @@ struct XRegArgs {
   int in0, out0, ke_slot;
   double c2a;
 };
+template <typename T> struct XCall {
+  Geo g;
+  const C<T>* a2;
+  C<T>* b2;
+  const C<T>* tw;
+  unsigned long long* vmax;
+  XRegArgs xa;
+};
 // MODE 0: u x w (+ 1/2|u|^2); 1: the Smagorinsky stress; 2: -s u of a scalar s; 3: the conservation
 // form's traceless(-u u) (+ 1/2|u|^2)
 template <int MODE> __host__ __device__ constexpr int xreg_npin() { return MODE == 2 || MODE == 3 ? 2 : 3; }
@@ -781,9 +921,9 @@ template <int MODE, bool KE> __host__ __device__ constexpr int xreg_npout() {
 }
 
 template <typename T, int N, int MODE, bool KE>
-__global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
-                                                 const C<T>* __restrict__ tw_g, unsigned long long* __restrict__ vmax,
-                                                 XRegArgs xa) {
+__device__ __forceinline__ void x_reg_body(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2, const C<T>* __restrict__ tw_g,
+                                           unsigned long long* __restrict__ vmax, XRegArgs xa, int bX) {
+  g = glb(g);
   constexpr int PPT = xreg_ppt<T, N>();
   constexpr int TP = N / PPT;
   constexpr int GPW = 64 / TP;  // rows per wavefront
@@ -802,7 +942,7 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int grp = lane / TP, q = lane % TP;
   const int nrows = g.ny * g.nz;
-  const int row = (blockIdx.x * (int)(blockDim.x >> 6) + wave) * GPW + grp;
+  const int row = (bX * (int)(blockDim.x >> 6) + wave) * GPW + grp;
   const bool row_ok = row < nrows;
   const int mx = g.mx;
   const size_t S2 = (size_t)g.sx * nrows;
@@ -951,6 +1091,18 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<
   if (MODE != 1 && vmax) block_max_to(lmax, vmax);
 }
 
+template <typename T, int N, int MODE, bool KE>
+__global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
+                                                   const C<T>* __restrict__ tw_g, unsigned long long* __restrict__ vmax,
+                                                   XRegArgs xa) {
+  x_reg_body<T, N, MODE, KE>(g, a2, b2, tw_g, vmax, xa, blockIdx.x);
+}
+template <typename T, int N, int MODE, bool KE>
+__global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg_b(FFS_CALLS(XCall<T>) calls, int nrun) {
+  const XCall<T> c = load_call(calls, blockIdx.z);
+  x_reg_body<T, N, MODE, KE>(c.g, glb(c.a2), glb(c.b2), glb(c.tw), glb(c.vmax), c.xa, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // The x pass with mirror-paired lanes (FP64, 8 points per lane, TP = N/8 lanes per row).  The
 // inverse transforms start with a radix-4 stage and the forward transforms end with one (the same
@@ -976,17 +1128,24 @@ template <typename T, int N> __host__ __device__ constexpr bool xpair_ok() {
 #ifndef FFS_XPAIR_WAVES
 #define FFS_XPAIR_WAVES 2  // minimum waves per SIMD the mirror-paired pass's registers must allow (3 spills)
 #endif
-// The same pointer, opaque to the optimiser: each transform of a row then reads its own twiddles instead
-// of one set being computed once and kept live, in registers, across all the row's transforms.
-template <typename P> __device__ __forceinline__ P* opaque(P* p) {
-  asm volatile("" : "+s"(p));
-  return p;
+// The twiddle table's pointer, opaque to the optimiser: each transform of a row then reads its own
+// twiddles instead of one set being computed once and kept live, in registers, across all the row's
+// transforms.  A table left in global memory comes back as a global pointer (as glb()).
+template <typename T> __device__ __forceinline__ const C<T>* opaque_tw(const C<T>* p) {
+  if constexpr (lds_twiddles<T>()) {
+    asm volatile("" : "+s"(p));
+    return p;
+  } else {
+    __attribute__((address_space(1))) const C<T>* q;
+    asm volatile("" : "=s"(q) : "0"(p));
+    return (const C<T>*)q;
+  }
 }
 
 template <typename T, int N, int MODE, bool KE>
-__global__ void __launch_bounds__(128, FFS_XPAIR_WAVES) k_x_pair(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
-                                               const C<T>* __restrict__ tw_g, unsigned long long* __restrict__ vmax,
-                                               XRegArgs xa) {
+__device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2, const C<T>* __restrict__ tw_g,
+                                           unsigned long long* __restrict__ vmax, XRegArgs xa, int bX) {
+  g = glb(g);
   constexpr int TP = N / 8;
   constexpr int GPW = 64 / TP;  // rows per wavefront
   constexpr int MAXR = xpair_maxr<N>();
@@ -1007,7 +1166,7 @@ __global__ void __launch_bounds__(128, FFS_XPAIR_WAVES) k_x_pair(Geo g, const C<
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int grp = lane / TP, q = lane % TP;
   const int nrows = g.ny * g.nz;
-  const int row = (blockIdx.x * (int)(blockDim.x >> 6) + wave) * GPW + grp;
+  const int row = (bX * (int)(blockDim.x >> 6) + wave) * GPW + grp;
   const bool row_ok = row < nrows;
   const int mx = g.mx;
   const size_t S2 = (size_t)g.sx * nrows;
@@ -1074,7 +1233,7 @@ __global__ void __launch_bounds__(128, FFS_XPAIR_WAVES) k_x_pair(Geo g, const C<
     auto ld = [&](int, int t, int r) -> C<T> { return Z[t][r]; };
     auto st = [&](int, int t, int r, C<T> v) { R[j][t + r * ML] = v; };
     if (j > 0) __syncwarp();
-    Fft<T, N, +1, TP, false, false, SYNC_WAVE, MAXR, 0, false, 0>::run(q, opaque(tw), ld, st, xref);
+    Fft<T, N, +1, TP, false, false, SYNC_WAVE, MAXR, 0, false, 0>::run(q, opaque_tw<T>(tw), ld, st, xref);
   }
   // pointwise products at positions q + TP m
   double lmax = 0.0;
@@ -1119,7 +1278,7 @@ __global__ void __launch_bounds__(128, FFS_XPAIR_WAVES) k_x_pair(Geo g, const C<
     auto ld = [&](int, int t, int r) { return R[j][t + r * ML]; };
     auto st = [&](int, int t, int r, C<T> v) { W[t][r] = v; };
     __syncwarp();
-    Fft<T, N, -1, TP, false, false, SYNC_WAVE, MAXR, 0, true, NSTG - 1>::run(q, opaque(tw), ld, st, xref);
+    Fft<T, N, -1, TP, false, false, SYNC_WAVE, MAXR, 0, true, NSTG - 1>::run(q, opaque_tw<T>(tw), ld, st, xref);
     int oa, ob;
     if constexpr (MODE == 0) {
       oa = 2 * j;
@@ -1153,6 +1312,18 @@ __global__ void __launch_bounds__(128, FFS_XPAIR_WAVES) k_x_pair(Geo g, const C<
       }
   }
   if (MODE != 1 && vmax) block_max_to(lmax, vmax);
+}
+
+template <typename T, int N, int MODE, bool KE>
+__global__ void __launch_bounds__(128, FFS_XPAIR_WAVES) k_x_pair(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
+                                                   const C<T>* __restrict__ tw_g, unsigned long long* __restrict__ vmax,
+                                                   XRegArgs xa) {
+  x_pair_body<T, N, MODE, KE>(g, a2, b2, tw_g, vmax, xa, blockIdx.x);
+}
+template <typename T, int N, int MODE, bool KE>
+__global__ void __launch_bounds__(128, FFS_XPAIR_WAVES) k_x_pair_b(FFS_CALLS(XCall<T>) calls, int nrun) {
+  const XCall<T> c = load_call(calls, blockIdx.z);
+  x_pair_body<T, N, MODE, KE>(c.g, glb(c.a2), glb(c.b2), glb(c.tw), glb(c.vmax), c.xa, blockIdx.x);
 }
 
 // C2R of nf fields (rows paired 2m, 2m+1) into the padded real layout
@@ -1272,9 +1443,14 @@ static void run_x_gen(const Geo& g, const void* a2, void* b2, const XOps& ops, u
   const size_t lds = ((size_t)ROWS * NSL * (N + 1) + tw_lds<T>(N)) * sizeof(C<T>);
   static std::once_flag once;
   lds_limit_once(once, (const void*)&k_x_gen<T, N, NSL>, lds, "the general x pass");
+  static std::once_flag once_b;
+  lds_limit_once(once_b, (const void*)&k_x_gen_b<T, N, NSL>, lds, "the batched general x pass");
   const int nrows = g.ny * g.nz;
-  k_x_gen<T, N, NSL><<<(nrows + ROWS - 1) / ROWS, ROWS * NSL * tper<N>(), lds, s>>>(
-      g, (const C<T>*)a2, (C<T>*)b2, (const C<T>*)tw, ops, umax, rhomax);
+  const XGCall<T> c{g, (const C<T>*)a2, (C<T>*)b2, (const C<T>*)tw, ops, umax, rhomax};
+  const dim3 grid((nrows + ROWS - 1) / ROWS), block(ROWS * NSL * tper<N>());
+  launch_call((const void*)&k_x_gen_b<T, N, NSL>, grid, block, lds, c, "general x pass", [&] {
+    k_x_gen<T, N, NSL><<<grid, block, lds, s>>>(c.g, c.a2, c.b2, c.tw, c.ops, c.umax, c.rhomax);
+  });
 }
 
 template <typename T, int N, int MODE, bool KE>
@@ -1285,9 +1461,14 @@ static void launch_xreg_plain(const Geo& g, const void* a2, void* b2, const void
   const size_t lds = ((size_t)RPB * xreg_ldx<T, N>() + tw_lds<T>(N)) * sizeof(C<T>);
   static std::once_flag once;
   lds_limit_once(once, (const void*)&k_x_nl_reg<T, N, MODE, KE>, lds, "the x pass");
+  static std::once_flag once_b;
+  lds_limit_once(once_b, (const void*)&k_x_nl_reg_b<T, N, MODE, KE>, lds, "the batched x pass");
   const int nrows = g.ny * g.nz;
-  k_x_nl_reg<T, N, MODE, KE><<<(nrows + RPB - 1) / RPB, 128, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
-                                                                       (const C<T>*)tw, vmax, xa);
+  const XCall<T> c{g, (const C<T>*)a2, (C<T>*)b2, (const C<T>*)tw, vmax, xa};
+  const dim3 grid((nrows + RPB - 1) / RPB);
+  launch_call((const void*)&k_x_nl_reg_b<T, N, MODE, KE>, grid, dim3(128), lds, c, "x pass", [&] {
+    k_x_nl_reg<T, N, MODE, KE><<<grid, 128, lds, s>>>(c.g, c.a2, c.b2, c.tw, c.vmax, c.xa);
+  });
 }
 
 template <typename T, int N, int MODE, bool KE>
@@ -1311,8 +1492,13 @@ static void launch_xreg(const Geo& g, const void* a2, void* b2, const void* tw, 
   if constexpr (PAIR) {
     static std::once_flag once;
     lds_limit_once(once, (const void*)&k_x_pair<T, N, MODE, KE>, lds, "the x pass");
-    k_x_pair<T, N, MODE, KE><<<(nrows + RPB - 1) / RPB, 64 * WPB, lds, s>>>(g, (const C<T>*)a2, (C<T>*)b2,
-                                                                          (const C<T>*)tw, vmax, xa);
+    static std::once_flag once_b;
+    lds_limit_once(once_b, (const void*)&k_x_pair_b<T, N, MODE, KE>, lds, "the batched x pass");
+    const XCall<T> c{g, (const C<T>*)a2, (C<T>*)b2, (const C<T>*)tw, vmax, xa};
+    const dim3 grid((nrows + RPB - 1) / RPB);
+    launch_call((const void*)&k_x_pair_b<T, N, MODE, KE>, grid, dim3(64 * WPB), lds, c, "x pass", [&] {
+      k_x_pair<T, N, MODE, KE><<<grid, 64 * WPB, lds, s>>>(c.g, c.a2, c.b2, c.tw, c.vmax, c.xa);
+    });
   }
 }
 
@@ -1590,7 +1776,7 @@ __device__ __forceinline__ void form_rhs(const UpdateArgs& a, size_t idx, size_t
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int p = 0; p < 2; ++p) R[c][p] += kvis * U[c][p];
+      for (int p = 0; p < 2; ++p) R[c][p] = fma(kvis, U[c][p], R[c][p]);
   }
   if (FULL && a.tau) {  // rhs += (i k) . tau, tau symmetric traceless (xx xy xz yy yz)
     const C<T>* t = (const C<T>*)a.tau;
@@ -1601,8 +1787,8 @@ __device__ __forceinline__ void form_rhs(const UpdateArgs& a, size_t idx, size_t
     const int row[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double sr = kv3[0] * T6[row[c][0]][0] + kv3[1] * T6[row[c][1]][0] + kv3[2] * T6[row[c][2]][0];
-      const double si = kv3[0] * T6[row[c][0]][1] + kv3[1] * T6[row[c][1]][1] + kv3[2] * T6[row[c][2]][1];
+      const double sr = fma(kv3[2], T6[row[c][2]][0], fma(kv3[1], T6[row[c][1]][0], kv3[0] * T6[row[c][0]][0]));
+      const double si = fma(kv3[2], T6[row[c][2]][1], fma(kv3[1], T6[row[c][1]][1], kv3[0] * T6[row[c][0]][1]));
       R[c][0] += -si;
       R[c][1] += sr;
     }
@@ -1611,14 +1797,14 @@ __device__ __forceinline__ void form_rhs(const UpdateArgs& a, size_t idx, size_t
     const C<T> r0 = ((const C<T>*)a.rho)[idx];
     RH[0] = r0.x;
     RH[1] = r0.y;
-    R[a.gdir][0] += RH[0] * a.gval;
-    R[a.gdir][1] += RH[1] * a.gval;
+    R[a.gdir][0] = fma(RH[0], a.gval, R[a.gdir][0]);
+    R[a.gdir][1] = fma(RH[1], a.gval, R[a.gdir][1]);
   }
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
-    const double p1 = origin ? 0.0 : -(kv3[0] * R[0][p] + kv3[1] * R[1][p] + kv3[2] * R[2][p]) / K2;
+    const double p1 = origin ? 0.0 : -fma(kv3[2], R[2][p], fma(kv3[1], R[1][p], kv3[0] * R[0][p])) / K2;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) R[c][p] = p1 * kv3[c] + R[c][p];
+    for (int c = 0; c < 3; ++c) R[c][p] = fma(p1, kv3[c], R[c][p]);
   }
 }
 
@@ -1630,7 +1816,9 @@ __device__ __forceinline__ void update_point(const Geo& g, C<T>* __restrict__ u,
                                              const DevState* __restrict__ st, const UpdateArgs& a, size_t idx,
                                              size_t n, int cx, int cy, int cz, double (&R)[3][2]) {
   const double kv3[3] = {g.kx[cx], g.ky[cy], g.kz[cz]};
-  const double K2 = kv3[0] * kv3[0] + kv3[1] * kv3[1] + kv3[2] * kv3[2];
+  // the sums of products are written as explicit fma chains, so every kernel that inlines the update
+  // rounds them alike whatever its compiler contracts around them
+  const double K2 = fma(kv3[2], kv3[2], fma(kv3[1], kv3[1], kv3[0] * kv3[0]));
   const bool origin = (idx == 0);
   double U[3][2];
 #pragma unroll
@@ -1780,7 +1968,7 @@ __device__ __forceinline__ void update_point(const Geo& g, C<T>* __restrict__ u,
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int p = 0; p < 2; ++p) U[c][p] += fac * U[c][p];
+      for (int p = 0; p < 2; ++p) U[c][p] = fma(fac, U[c][p], U[c][p]);
   }
   // plane forcing: u_h += force (add_Rf_forcing!, forcing_types.jl:54-64)
   if constexpr (FULL) {
@@ -1791,9 +1979,9 @@ __device__ __forceinline__ void update_point(const Geo& g, C<T>* __restrict__ u,
   }
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
-    const double p1 = origin ? 0.0 : -(kv3[0] * U[0][p] + kv3[1] * U[1][p] + kv3[2] * U[2][p]) / K2;
+    const double p1 = origin ? 0.0 : -fma(kv3[2], U[2][p], fma(kv3[1], U[1][p], kv3[0] * U[0][p])) / K2;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) U[c][p] = p1 * kv3[c] + U[c][p];
+    for (int c = 0; c < 3; ++c) U[c][p] = fma(p1, kv3[c], U[c][p]);
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) u[c * n + idx] = mk<T>((T)U[c][0], (T)U[c][1]);
@@ -1801,14 +1989,28 @@ __device__ __forceinline__ void update_point(const Geo& g, C<T>* __restrict__ u,
   if (FULL && a.phi) ((C<T>*)a.phi)[idx] = mk<T>((T)PH[0], (T)PH[1]);
 }
 
+template <typename T> struct UCall {
+  Geo g;
+  C<T>* u;
+  C<T>* rhs;
+  const C<T>* fm1;
+  const C<T>* fm2;
+  const DevState* st;
+  UpdateArgs a;
+};
+
 template <typename T, bool FULL>
-__global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T>* __restrict__ rhs,
-                                               const C<T>* __restrict__ fm1, const C<T>* __restrict__ fm2,
-                                               const DevState* __restrict__ st, UpdateArgs a) {
+__device__ __forceinline__ void update_body(Geo g, C<T>* __restrict__ u, C<T>* __restrict__ rhs,
+                                            const C<T>* __restrict__ fm1, const C<T>* __restrict__ fm2,
+                                            const DevState* __restrict__ st, UpdateArgs a, int bX) {
+  g = glb(g);
+  a = glb(a);
+  // the step's scalars (no thread reads the time this kernel advances)
+  const DevState sv = load_const(st);
   const size_t n = (size_t)g.sx * g.my * g.mz;
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t idx = (size_t)bX * blockDim.x + threadIdx.x;
   // s.time[] += dt, once per step (no kernel reads the time during the update)
-  if (a.time && idx == 0) *a.time += st->dt;
+  if (a.time && idx == 0) *a.time += sv.dt;
   if (idx >= n) return;
   const int cx = idx % g.sx;
   if (cx >= g.mx) return;  // row padding
@@ -1820,7 +2022,18 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
     const C<T> r = rhs[c * n + idx];
     R[c][0] = r.x; R[c][1] = r.y;
   }
-  update_point<T, FULL>(g, u, rhs, fm1, fm2, st, a, idx, n, cx, cy, cz, R);
+  update_point<T, FULL>(g, u, rhs, fm1, fm2, &sv, a, idx, n, cx, cy, cz, R);
+}
+template <typename T, bool FULL>
+__global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T>* __restrict__ rhs,
+                                               const C<T>* __restrict__ fm1, const C<T>* __restrict__ fm2,
+                                               const DevState* __restrict__ st, UpdateArgs a) {
+  update_body<T, FULL>(g, u, rhs, fm1, fm2, st, a, blockIdx.x);
+}
+template <typename T, bool FULL>
+__global__ void __launch_bounds__(256) k_update_b(FFS_CALLS(UCall<T>) calls, int nrun) {
+  const UCall<T> c = load_call(calls, blockIdx.z);
+  update_body<T, FULL>(c.g, glb(c.u), glb(c.rhs), glb(c.fm1), glb(c.fm2), glb(c.st), c.a, blockIdx.x);
 }
 
 // The forward z pass of the three nonlinear components fused with the update of the plain step (no
@@ -1828,11 +2041,25 @@ __global__ void __launch_bounds__(256) k_update(Geo g, C<T>* __restrict__ u, C<T
 // its TX x-columns at one cy for all three components, keeps the results in registers and updates
 // those modes, so the nonlinear term never goes through rhs in HBM.  The values are those of the
 // forward z pass followed by the update kernel, bit for bit.
+template <typename T> struct FzuCall {
+  Geo g;
+  const C<T>* in;
+  C<T>* u;
+  C<T>* rhs;
+  const C<T>* fm1;
+  const C<T>* fm2;
+  const DevState* st;
+  UpdateArgs a;
+  const C<T>* tw;
+  T scale;
+};
 template <typename T, int N>
-__global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES)
-    k_fwd_z_update(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ u, C<T>* __restrict__ rhs,
-                   const C<T>* __restrict__ fm1, const C<T>* __restrict__ fm2, const DevState* __restrict__ st,
-                   UpdateArgs a, const C<T>* __restrict__ tw_g, T scale) {
+__device__ __forceinline__ void fwd_z_update_body(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ u,
+                                                  C<T>* __restrict__ rhs, const C<T>* __restrict__ fm1,
+                                                  const C<T>* __restrict__ fm2, const DevState* __restrict__ st,
+                                                  UpdateArgs a, const C<T>* __restrict__ tw_g, T scale, int bX, int bY) {
+  g = glb(g);
+  a = glb(a);
   constexpr int TX = col_tx<T>(N);
   constexpr int TP = col_tper<T, N, false>();
   constexpr int MAXR = col_maxr<T, N, false>();
@@ -1841,12 +2068,12 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES)
   extern __shared__ unsigned char smem_raw[];
   C<T>* lds = reinterpret_cast<C<T>*>(smem_raw);
   const C<T>* tw = stage_twiddles<T, N>(lds + N * TX, tw_g);
-  if (a.time && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *a.time += st->dt;
+  if (a.time && bX == 0 && bY == 0 && threadIdx.x == 0) *a.time += st->dt;
   __syncthreads();
   const int c = threadIdx.x % TX;
   const int q = threadIdx.x / TX;
-  const int cx = blockIdx.x * TX + c;
-  const int cy = blockIdx.y;
+  const int cx = bX * TX + c;
+  const int cy = bY;
   const bool valid = cx < g.mx;
   const size_t sx = g.sx;
   const size_t n = sx * g.my * g.mz;
@@ -1879,12 +2106,29 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES)
     double R[3][2];
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-      const C<T> r = zero ? mk<T>(0, 0) : cscale<T>(V[f][m], scale);
-      R[f][0] = r.x;
-      R[f][1] = r.y;
+      // rounded as the forward z pass stores it: the product is not contracted into the update's sums
+#pragma clang fp contract(off)
+      const T rx = V[f][m].x * scale, ry = V[f][m].y * scale;
+      R[f][0] = zero ? 0.0 : (double)rx;
+      R[f][1] = zero ? 0.0 : (double)ry;
     }
     update_point<T, false>(g, u, rhs, fm1, fm2, st, a, idx, n, cx, cy, cz, R);
   }
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES)
+    k_fwd_z_update(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ u, C<T>* __restrict__ rhs,
+                   const C<T>* __restrict__ fm1, const C<T>* __restrict__ fm2, const DevState* __restrict__ st,
+                   UpdateArgs a, const C<T>* __restrict__ tw_g, T scale) {
+  fwd_z_update_body<T, N>(g, in, u, rhs, fm1, fm2, st, a, tw_g, scale, blockIdx.x, blockIdx.y);
+}
+template <typename T, int N>
+__global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_fwd_z_update_b(FFS_CALLS(FzuCall<T>) calls,
+                                                                              int nrun) {
+  const FzuCall<T> c = load_call(calls, blockIdx.z);
+  fwd_z_update_body<T, N>(c.g, glb(c.in), glb(c.u), glb(c.rhs), glb(c.fm1), glb(c.fm2), glb(c.st), c.a, glb(c.tw),
+                          c.scale, blockIdx.x, blockIdx.y);
 }
 
 // calc_les! Smagorinsky branch (les_real_calc.jl:77-79) on padded real S fields: tau = 2 (c Delta)^2
@@ -2271,11 +2515,16 @@ template <typename T>
 void launch_update(const Geo& g, void* u, void* rhs, const void* fm1, const void* fm2, const DevState* st,
                    const UpdateArgs& a, hipStream_t s) {
   const size_t n = (size_t)g.sx * g.my * g.mz;
-  const bool full = update_full(a);
-  if (full)
-    k_update<T, true><<<(n + 255) / 256, 256, 0, s>>>(g, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1, (const C<T>*)fm2, st, a);
+  const UCall<T> c{g, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1, (const C<T>*)fm2, st, a};
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if (update_full(a))
+    launch_call((const void*)&k_update_b<T, true>, grid, dim3(256), 0, c, "update", [&] {
+      k_update<T, true><<<grid, 256, 0, s>>>(c.g, c.u, c.rhs, c.fm1, c.fm2, c.st, c.a);
+    });
   else
-    k_update<T, false><<<(n + 255) / 256, 256, 0, s>>>(g, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1, (const C<T>*)fm2, st, a);
+    launch_call((const void*)&k_update_b<T, false>, grid, dim3(256), 0, c, "update", [&] {
+      k_update<T, false><<<grid, 256, 0, s>>>(c.g, c.u, c.rhs, c.fm1, c.fm2, c.st, c.a);
+    });
 }
 
 template <typename T, int N>
@@ -2286,9 +2535,14 @@ static void run_fwd_z_update(const Geo& g, const void* b1, void* u, void* rhs, c
   const size_t lds = (size_t)(N * TX + tw_lds<T>(N)) * sizeof(C<T>);
   static std::once_flag once;
   lds_limit_once(once, (const void*)&k_fwd_z_update<T, N>, lds, "the fused forward z + update");
+  static std::once_flag once_b;
+  lds_limit_once(once_b, (const void*)&k_fwd_z_update_b<T, N>, lds, "the batched fused forward z + update");
   const dim3 grid((g.mx + TX - 1) / TX, g.my);
-  k_fwd_z_update<T, N><<<grid, TX * TP, lds, s>>>(g, (const C<T>*)b1, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1,
-                                                  (const C<T>*)fm2, st, a, (const C<T>*)tw, (T)scale);
+  const FzuCall<T> c{g, (const C<T>*)b1, (C<T>*)u, (C<T>*)rhs, (const C<T>*)fm1, (const C<T>*)fm2, st, a,
+                     (const C<T>*)tw, (T)scale};
+  launch_call((const void*)&k_fwd_z_update_b<T, N>, grid, dim3(TX * TP), lds, c, "fwd_z_update", [&] {
+    k_fwd_z_update<T, N><<<grid, TX * TP, lds, s>>>(c.g, c.in, c.u, c.rhs, c.fm1, c.fm2, c.st, c.a, c.tw, c.scale);
+  });
 }
 
 template <typename T>
@@ -3214,8 +3468,16 @@ void launch_plane_forcing(const Geo& g, const void* u, const void* rhs, DevState
 }
 
 // one block of 64 threads: the forcing band sum (as k_forcing_sum) and, on thread 0, set_dt
+template <typename T> struct SCall {
+  Geo g;
+  const C<T>* u;
+  DevState* st;
+  ScalarArgs a;
+};
 template <typename T>
-__global__ void k_step_scalars(Geo g, const C<T>* __restrict__ u, DevState* st, ScalarArgs a) {
+__device__ __forceinline__ void step_scalars_body(Geo g, const C<T>* __restrict__ u, DevState* st, ScalarArgs a) {
+  g = glb(g);
+  a.band = glb(a.band);
   if (a.nband > 0) {
     const size_t n = (size_t)g.sx * g.my * g.mz;
     double acc = 0;
@@ -3239,10 +3501,21 @@ __global__ void k_step_scalars(Geo g, const C<T>* __restrict__ u, DevState* st, 
   }
   if (a.do_dt && threadIdx.x == 0) set_dt_body(st, a.it, a.tit, a.cfl, a.dx, a.etd, a.gnorm);
 }
+template <typename T>
+__global__ void __launch_bounds__(64) k_step_scalars(Geo g, const C<T>* __restrict__ u, DevState* st, ScalarArgs a) {
+  step_scalars_body<T>(g, u, st, a);
+}
+template <typename T>
+__global__ void __launch_bounds__(64) k_step_scalars_b(FFS_CALLS(SCall<T>) calls, int nrun) {
+  const SCall<T> c = load_call(calls, blockIdx.z);
+  step_scalars_body<T>(c.g, glb(c.u), glb(c.st), c.a);
+}
 
 template <typename T>
 void launch_step_scalars(const Geo& g, const void* u, DevState* st, const ScalarArgs& a, hipStream_t s) {
-  k_step_scalars<T><<<1, 64, 0, s>>>(g, (const C<T>*)u, st, a);
+  const SCall<T> c{g, (const C<T>*)u, st, a};
+  launch_call((const void*)&k_step_scalars_b<T>, dim3(1), dim3(64), 0, c, "step_scalars",
+              [&] { k_step_scalars<T><<<1, 64, 0, s>>>(c.g, c.u, c.st, c.a); });
 }
 
 void launch_set_dt(DevState* st, int it, int tit, double cfl, double dx, int etd, double gnorm, hipStream_t s) {

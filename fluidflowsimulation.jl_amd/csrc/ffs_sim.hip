@@ -10,6 +10,7 @@
 #include "ffs_kernels.h"
 #include "ffs/ffs.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -1671,10 +1672,14 @@ struct ffs_sim {
 };
 
 // An ensemble advanced in lockstep.  Steps in which every run is steady replay one graph whose
-// branches (forked from `lead` by an event, joined back by one event per run) are the runs' steady
+// branches (forked from `lead` by an event, joined back by one event each) are the runs' steady
 // steps; the branch structure is fixed by the runs' rotation phases, so graphs are cached per phase
-// signature.  Ordering against each run's own stream: before a graph launch `lead` waits for every
-// run's stream, and before a run's stream is used again it waits for `lead`.
+// signature.  Runs whose steady steps launch the same kernels with the same launch shapes (same
+// transform lengths, precision and model) form a group, and a group's step is one launch per pass of
+// the kernels' batch forms over a device table of the runs' call records (the run index in grid z),
+// so a group costs as many launches as one run.  A run whose step holds work that has no batch form
+// keeps a branch of its own.  Ordering against each run's own stream: before a graph launch `lead`
+// waits for every run's stream, and before a run's stream is used again it waits for `lead`.
 struct ffs_batch {
   std::vector<ffs_sim*> runs;
   hipStream_t lead = nullptr;
@@ -1684,8 +1689,14 @@ struct ffs_batch {
     std::vector<int> sig;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    void* calls = nullptr;  // the groups' call-record tables (device memory)
+    int launches = 0;       // kernel launches per step
+    int groups = 0;         // merged groups (of two runs or more)
+    int merged_runs = 0;    // runs inside those groups
   };
   std::vector<Cached> cache;
+  bool merge = true;
+  int last_graph = -1;  // cache index of the last graph launched
   bool lead_behind = true;    // lead must wait for the runs' streams before the next graph
   bool runs_behind = false;   // the runs' streams must wait for lead before their next work
   std::string err;
@@ -1712,11 +1723,7 @@ struct ffs_batch {
 
   void destroy() {
     if (lead) hipStreamSynchronize(lead);
-    for (auto& c : cache) {
-      if (c.exec) hipGraphExecDestroy(c.exec);
-      if (c.graph) hipGraphDestroy(c.graph);
-    }
-    cache.clear();
+    drop_cache();
     for (auto e : joins)
       if (e) hipEventDestroy(e);
     if (fork) hipEventDestroy(fork);
@@ -1731,22 +1738,164 @@ struct ffs_batch {
     return sig;
   }
 
+  void drop_cache() {
+    for (auto& c : cache) {
+      if (c.exec) hipGraphExecDestroy(c.exec);
+      if (c.graph) hipGraphDestroy(c.graph);
+      if (c.calls) hipFree(c.calls);
+    }
+    cache.clear();
+    last_graph = -1;
+  }
+
+  // The launches of run r's steady step, recorded instead of issued.  The step is enqueued on r's
+  // stream under capture with the recorder set: a launch with a batch form is recorded, anything else
+  // lands in the captured graph, so an empty graph means the whole step was recorded.
+  static bool record_step(ffs_sim* r, KRecorder& rec) {
+    hipGraph_t gr = nullptr;
+    HIPCHECK(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+    g_recorder = &rec;
+    try {
+      r->enqueue_steady_step();
+    } catch (...) {
+      g_recorder = nullptr;
+      if (hipStreamEndCapture(r->stream, &gr) == hipSuccess && gr) hipGraphDestroy(gr);
+      throw;
+    }
+    g_recorder = nullptr;
+    HIPCHECK(hipStreamEndCapture(r->stream, &gr));
+    size_t nodes = 0;
+    const hipError_t e = gr ? hipGraphGetNodes(gr, nullptr, &nodes) : hipSuccess;
+    if (gr) HIPCHECK(hipGraphDestroy(gr));
+    HIPCHECK(e);
+    return nodes == 0 && !rec.launches.empty();
+  }
+
+  // two recorded steps can share launches when every launch has the same kernel and shape
+  static bool same_shape(const KRecorder& a, const KRecorder& b) {
+    if (a.launches.size() != b.launches.size()) return false;
+    for (size_t j = 0; j < a.launches.size(); ++j) {
+      const KLaunch &x = a.launches[j], &y = b.launches[j];
+      if (x.batch != y.batch || x.lds != y.lds || x.call.size() != y.call.size() || x.grid.x != y.grid.x ||
+          x.grid.y != y.grid.y || x.grid.z != y.grid.z || x.block.x != y.block.x || x.block.y != y.block.y ||
+          x.block.z != y.block.z)
+        return false;
+    }
+    return true;
+  }
+
   hipGraphExec_t graph_for(const std::vector<int>& sig) {
-    for (auto& c : cache)
-      if (c.sig == sig) return c.exec;
+    for (size_t k = 0; k < cache.size(); ++k)
+      if (cache[k].sig == sig) {
+        last_graph = (int)k;
+        return cache[k].exec;
+      }
     Cached c;
     c.sig = sig;
-    HIPCHECK(hipStreamBeginCapture(lead, hipStreamCaptureModeThreadLocal));
-    HIPCHECK(hipEventRecord(fork, lead));
-    for (size_t i = 0; i < runs.size(); ++i) {
-      HIPCHECK(hipStreamWaitEvent(runs[i]->stream, fork, 0));
-      runs[i]->enqueue_steady_step();
-      HIPCHECK(hipEventRecord(joins[i], runs[i]->stream));
-      HIPCHECK(hipStreamWaitEvent(lead, joins[i], 0));
+    const size_t n = runs.size();
+    // groups of runs whose steps were recorded whole and share their launch shapes
+    std::vector<KRecorder> rec(n);
+    std::vector<std::vector<int>> groups;
+    std::vector<int> alone;
+    for (size_t i = 0; i < n; ++i) {
+      if (!merge || !record_step(runs[i], rec[i])) {
+        alone.push_back((int)i);
+        continue;
+      }
+      bool placed = false;
+      for (auto& gr : groups)
+        if (same_shape(rec[gr[0]], rec[i])) {
+          gr.push_back((int)i);
+          placed = true;
+          break;
+        }
+      if (!placed) groups.push_back({(int)i});
     }
-    HIPCHECK(hipStreamEndCapture(lead, &c.graph));
-    HIPCHECK(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
+    // a group of one runs its own step; larger groups are cut so that grid z stays within 65535
+    std::vector<std::vector<int>> merged;
+    for (auto& gr : groups) {
+      if (gr.size() == 1) {
+        alone.push_back(gr[0]);
+        continue;
+      }
+      unsigned zmax = 1;
+      for (const KLaunch& k : rec[gr[0]].launches) zmax = std::max(zmax, k.grid.z * k.block.z);
+      const size_t per = std::max<size_t>(1, 65535 / zmax);
+      for (size_t a = 0; a < gr.size(); a += per) {
+        std::vector<int> part(gr.begin() + a, gr.begin() + std::min(gr.size(), a + per));
+        if (part.size() == 1)
+          alone.push_back(part[0]);
+        else
+          merged.push_back(std::move(part));
+      }
+    }
+    std::sort(alone.begin(), alone.end());
+    // the call-record tables: for each merged group and launch, the group's records back to back
+    std::vector<std::vector<size_t>> offs(merged.size());
+    size_t bytes = 0;
+    for (size_t m = 0; m < merged.size(); ++m)
+      for (const KLaunch& k : rec[merged[m][0]].launches) {
+        offs[m].push_back(bytes);
+        bytes += (k.call.size() * merged[m].size() + 255) / 256 * 256;
+      }
+    if (bytes) {
+      std::vector<unsigned char> host(bytes, 0);
+      for (size_t m = 0; m < merged.size(); ++m)
+        for (size_t j = 0; j < offs[m].size(); ++j)
+          for (size_t r = 0; r < merged[m].size(); ++r) {
+            const std::vector<unsigned char>& call = rec[merged[m][r]].launches[j].call;
+            std::memcpy(host.data() + offs[m][j] + r * call.size(), call.data(), call.size());
+          }
+      HIPCHECK(hipMalloc(&c.calls, bytes));
+      HIPCHECK(hipMemcpy(c.calls, host.data(), bytes, hipMemcpyHostToDevice));
+    }
+    try {
+      HIPCHECK(hipStreamBeginCapture(lead, hipStreamCaptureModeThreadLocal));
+      HIPCHECK(hipEventRecord(fork, lead));
+      for (size_t m = 0; m < merged.size(); ++m) {
+        const int first = merged[m][0];
+        hipStream_t st = runs[first]->stream;  // the group's branch
+        HIPCHECK(hipStreamWaitEvent(st, fork, 0));
+        const auto& ls = rec[first].launches;
+        for (size_t j = 0; j < ls.size(); ++j)
+          issue_batch(ls[j], (const char*)c.calls + offs[m][j], (int)merged[m].size(), st);
+        HIPCHECK(hipEventRecord(joins[first], st));
+        HIPCHECK(hipStreamWaitEvent(lead, joins[first], 0));
+        c.merged_runs += (int)merged[m].size();
+      }
+      c.groups = (int)merged.size();
+      for (int i : alone) {
+        HIPCHECK(hipStreamWaitEvent(runs[i]->stream, fork, 0));
+        runs[i]->enqueue_steady_step();
+        HIPCHECK(hipEventRecord(joins[i], runs[i]->stream));
+        HIPCHECK(hipStreamWaitEvent(lead, joins[i], 0));
+      }
+      HIPCHECK(hipStreamEndCapture(lead, &c.graph));
+      HIPCHECK(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
+    } catch (...) {
+      hipGraph_t gr = nullptr;
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      if (hipStreamIsCapturing(lead, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone &&
+          hipStreamEndCapture(lead, &gr) == hipSuccess && gr)
+        hipGraphDestroy(gr);
+      if (c.graph) hipGraphDestroy(c.graph);
+      if (c.calls) hipFree(c.calls);
+      throw;
+    }
+    {  // kernel launches per step, counted in the graph
+      size_t nodes = 0;
+      HIPCHECK(hipGraphGetNodes(c.graph, nullptr, &nodes));
+      std::vector<hipGraphNode_t> nv(nodes);
+      if (nodes) HIPCHECK(hipGraphGetNodes(c.graph, nv.data(), &nodes));
+      c.launches = 0;
+      for (auto nd : nv) {
+        hipGraphNodeType t;
+        HIPCHECK(hipGraphNodeGetType(nd, &t));
+        if (t == hipGraphNodeTypeKernel) ++c.launches;
+      }
+    }
     cache.push_back(c);
+    last_graph = (int)cache.size() - 1;
     return cache.back().exec;
   }
 
@@ -2582,6 +2731,30 @@ int ffs_batch_synchronize(ffs_batch* b) {
     b->err = e.what();
     return e.code;
   }
+}
+
+int ffs_batch_set_merge(ffs_batch* b, int on) {
+  if (!b) return FFS_ERR_ARG;
+  try {
+    if (b->merge == (on != 0)) return FFS_OK;
+    HIPCHECK(hipStreamSynchronize(b->lead));
+    b->drop_cache();  // the cached graphs hold the other launch structure
+    b->merge = on != 0;
+    return FFS_OK;
+  } catch (const FfsError& e) {
+    b->err = e.what();
+    return e.code;
+  }
+}
+
+int ffs_batch_launch_info(const ffs_batch* b, int* launches, int* groups, int* merged_runs) {
+  if (!b) return FFS_ERR_ARG;
+  const bool any = b->last_graph >= 0;
+  const auto* c = any ? &b->cache[b->last_graph] : nullptr;
+  if (launches) *launches = any ? c->launches : 0;
+  if (groups) *groups = any ? c->groups : 0;
+  if (merged_runs) *merged_runs = any ? c->merged_runs : 0;
+  return any ? FFS_OK : FFS_ERR_STATE;
 }
 
 const char* ffs_batch_last_error(const ffs_batch* b) {

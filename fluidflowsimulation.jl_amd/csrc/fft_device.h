@@ -12,8 +12,15 @@
 // (unnormalised, FFTW_FORWARD), DIR = +1 the backward one (unnormalised,
 // FFTW_BACKWARD), which is what FluidFields' rfft!/real! call
 // (utils.jl:101-112, models.jl:122-211).
+//
+// Rounding: the arithmetic below is compiled without FMA contraction, and every fused multiply-add is
+// written out (fmaT), so a transform rounds the same way in every kernel that inlines it, whatever
+// the compiler could contract around it there.  Two kernels that transform the same data therefore
+// agree bit for bit (a fused pass and its unfused pair, a batched launch and a single one).
 #pragma once
 #include <hip/hip_runtime.h>
+
+#pragma clang fp contract(off)
 
 namespace fsk {
 
@@ -23,14 +30,17 @@ template <> struct cvec<float> { using type = float2; };
 template <typename T> using C = typename cvec<T>::type;
 
 template <typename T> __device__ __forceinline__ C<T> mk(T x, T y) { C<T> r; r.x = x; r.y = y; return r; }
+template <typename T> __device__ __forceinline__ T fmaT(T a, T b, T c);
+template <> __device__ __forceinline__ double fmaT<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <> __device__ __forceinline__ float fmaT<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 template <typename T> __device__ __forceinline__ C<T> cadd(C<T> a, C<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
 template <typename T> __device__ __forceinline__ C<T> csub(C<T> a, C<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
 template <typename T> __device__ __forceinline__ C<T> cmul(C<T> a, C<T> b) {
-  return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return mk<T>(fmaT<T>(a.x, b.x, -(a.y * b.y)), fmaT<T>(a.x, b.y, a.y * b.x));
 }
 // a * conj(b)
 template <typename T> __device__ __forceinline__ C<T> cmulc(C<T> a, C<T> b) {
-  return mk<T>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+  return mk<T>(fmaT<T>(a.x, b.x, a.y * b.y), fmaT<T>(a.y, b.x, -(a.x * b.y)));
 }
 template <typename T> __device__ __forceinline__ C<T> cconj(C<T> a) { return mk<T>(a.x, -a.y); }
 template <typename T> __device__ __forceinline__ C<T> cscale(C<T> a, T s) { return mk<T>(a.x * s, a.y * s); }
@@ -75,7 +85,7 @@ template <int S, int M, typename T> __device__ __forceinline__ C<T> tw16(C<T> a)
   } else {
     const T c = (T)cos16(m);
     const T s = (T)(S * sin16(m));
-    return mk<T>(a.x * c - a.y * s, a.x * s + a.y * c);
+    return mk<T>(fmaT<T>(a.x, c, -(a.y * s)), fmaT<T>(a.x, s, a.y * c));
   }
 }
 
@@ -97,13 +107,13 @@ template <int S, typename T> __device__ __forceinline__ void dft4(C<T>& a0, C<T>
 }
 
 template <int S, typename T> __device__ __forceinline__ void dft3(C<T>& a0, C<T>& a1, C<T>& a2) {
-  const T h = (T)0.86602540378443864676;  // sin(2 pi / 3)
+  const T h = (T)(S * 0.86602540378443864676);  // S sin(2 pi / 3)
   C<T> s = cadd<T>(a1, a2);
-  C<T> d = mul_si<S, T>(cscale<T>(csub<T>(a1, a2), h));
-  C<T> m = mk<T>(a0.x - (T)0.5 * s.x, a0.y - (T)0.5 * s.y);
+  C<T> e = csub<T>(a1, a2);  // a1, a2 = m +- S i h e
+  C<T> m = mk<T>(fmaT<T>((T)-0.5, s.x, a0.x), fmaT<T>((T)-0.5, s.y, a0.y));
   a0 = cadd<T>(a0, s);
-  a1 = cadd<T>(m, d);
-  a2 = csub<T>(m, d);
+  a1 = mk<T>(fmaT<T>(-h, e.y, m.x), fmaT<T>(h, e.x, m.y));
+  a2 = mk<T>(fmaT<T>(h, e.y, m.x), fmaT<T>(-h, e.x, m.y));
 }
 
 template <int S, typename T>
@@ -112,10 +122,10 @@ __device__ __forceinline__ void dft5(C<T>& a0, C<T>& a1, C<T>& a2, C<T>& a3, C<T
   const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
   C<T> b1 = cadd<T>(a1, a4), b2 = cadd<T>(a2, a3);
   C<T> d1 = csub<T>(a1, a4), d2 = csub<T>(a2, a3);
-  C<T> r1 = mk<T>(a0.x + c1 * b1.x + c2 * b2.x, a0.y + c1 * b1.y + c2 * b2.y);
-  C<T> r2 = mk<T>(a0.x + c2 * b1.x + c1 * b2.x, a0.y + c2 * b1.y + c1 * b2.y);
-  C<T> i1 = mul_si<S, T>(mk<T>(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y));
-  C<T> i2 = mul_si<S, T>(mk<T>(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y));
+  C<T> r1 = mk<T>(fmaT<T>(c2, b2.x, fmaT<T>(c1, b1.x, a0.x)), fmaT<T>(c2, b2.y, fmaT<T>(c1, b1.y, a0.y)));
+  C<T> r2 = mk<T>(fmaT<T>(c1, b2.x, fmaT<T>(c2, b1.x, a0.x)), fmaT<T>(c1, b2.y, fmaT<T>(c2, b1.y, a0.y)));
+  C<T> i1 = mul_si<S, T>(mk<T>(fmaT<T>(s2, d2.x, s1 * d1.x), fmaT<T>(s2, d2.y, s1 * d1.y)));
+  C<T> i2 = mul_si<S, T>(mk<T>(fmaT<T>(-s1, d2.x, s2 * d1.x), fmaT<T>(-s1, d2.y, s2 * d1.y)));
   a0 = mk<T>(a0.x + b1.x + b2.x, a0.y + b1.y + b2.y);
   a1 = cadd<T>(r1, i1);
   a4 = csub<T>(r1, i1);
@@ -375,3 +385,5 @@ __host__ __device__ constexpr bool plan_even(int N, int tper, int maxr = 16) {
 }
 
 }  // namespace fsk
+
+#pragma clang fp contract(fast)

@@ -156,6 +156,8 @@ SIGNATURES = {
     "ffs_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(_P)]),
     "ffs_batch_advance": (ctypes.c_int, [_P, ctypes.c_int64, _I64, _I, _I]),
     "ffs_batch_synchronize": (ctypes.c_int, [_P]),
+    "ffs_batch_set_merge": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ffs_batch_launch_info": (ctypes.c_int, [_P, _I, _I, _I]),
     "ffs_batch_last_error": (ctypes.c_char_p, [_P]),
     "ffs_batch_destroy": (None, [_P]),
     "ffs_last_error": (ctypes.c_char_p, [_P]),

@@ -16,7 +16,7 @@ __all__ = ["Ensemble"]
 class Ensemble:
     """Lockstep advance of initialized `Simulation`s (which it does not own)."""
 
-    def __init__(self, sims):
+    def __init__(self, sims, merge=True):
         self.sims = list(sims)
         self._lib = _lib.load()
         arr = (ctypes.c_void_p * len(self.sims))(*[s._h.value for s in self.sims])
@@ -24,6 +24,20 @@ class Ensemble:
         rc = self._lib.ffs_batch_create(arr, len(self.sims), ctypes.byref(h))
         self._check(rc, None)
         self._h = h
+        if not merge:
+            self.set_merge(False)
+
+    def set_merge(self, on):
+        """Merge the launches of runs with the same launch shapes into one launch per pass (default on)."""
+        self._check(self._lib.ffs_batch_set_merge(self._h, 1 if on else 0), self._h)
+
+    def launch_info(self):
+        """(kernel launches per step, merged groups, runs in them) of the last steady-step graph, or None."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        rc = self._lib.ffs_batch_launch_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        if rc < 0:
+            return None
+        return a.value, b.value, c.value
 
     def _check(self, rc, h=None):
         if rc is not None and rc < 0:

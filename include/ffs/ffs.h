@@ -384,11 +384,14 @@ int ffs_set_fusion(ffs_sim* s, int on);
 
 /* An ensemble: up to FFS_MAX_BATCH initialized handles on the current device
  * advanced in lockstep.  A step in which every run is steady is one graph
- * launch whose branches are the runs' steps (independent, so they overlap on
- * the device); other steps run each handle's own path.  Each run's results
- * are identical to running it alone.  The batch does not own the handles:
- * destroy it before any of them.  No counterpart in the reference (one
- * simulation per process). */
+ * launch whose branches are independent, so they overlap on the device: runs
+ * whose steps launch the same kernels with the same shapes (same transform
+ * lengths, precision and model) are merged into one group, whose step is one
+ * launch per pass over all its runs; any other run is a branch of its own.
+ * Other steps run each handle's own path.  Each run's results are identical
+ * to running it alone.  The batch does not own the handles: destroy it before
+ * any of them.  No counterpart in the reference (one simulation per
+ * process). */
 #define FFS_MAX_BATCH 256
 typedef struct ffs_batch ffs_batch;
 int ffs_batch_create(ffs_sim* const* sims, int n, ffs_batch** out);
@@ -399,6 +402,14 @@ int ffs_batch_create(ffs_sim* const* sims, int n, ffs_batch** out);
 int ffs_batch_advance(ffs_batch* b, int64_t nsteps, int64_t* done, int* event, int* who);
 /* Blocks until every run's queued work is done. */
 int ffs_batch_synchronize(ffs_batch* b);
+/* on = 0: every run keeps a branch of its own (no merged launches); on = 1
+ * (default) merges runs of the same launch shapes.  Bitwise the same results
+ * either way. */
+int ffs_batch_set_merge(ffs_batch* b, int on);
+/* The last steady-step graph launched: its kernel launches per step, its
+ * merged groups and the runs inside them.  FFS_ERR_STATE before the first
+ * steady step. */
+int ffs_batch_launch_info(const ffs_batch* b, int* launches, int* groups, int* merged_runs);
 const char* ffs_batch_last_error(const ffs_batch* b);
 void ffs_batch_destroy(ffs_batch* b);
 
