@@ -451,8 +451,9 @@ def leg_serve(args):
     """Serving: Poisson arrivals (mean `serve_rate`/s) of `serve_requests` solves, size from
     {64,128,192,256}^3 weighted 4:3:2:1, FP32:FP64 3:1, nu log-uniform in [1e-4, 1e-2], horizon
     uniform in [200, 2000] steps, Smagorinsky LES with probability 1/2, admitted under an HBM budget
-    with state resident on the device; continuous batching in lockstep.  Each request's initial field
-    is generated on the device from its own seed."""
+    with state resident on the device; continuous batching in lanes of one size and precision that run
+    side by side.  Step latency comes from device timestamps around each lane's chunks.  Each request's
+    initial field is generated on the device from its own seed."""
     import numpy as np
 
     import ffs
@@ -484,8 +485,9 @@ def leg_serve(args):
     pts_steps = float(sum(j.params.nx**3 * j.nsteps for j in done))
     lat = srv.latency()
     out = {"requests": R, "rate_per_s": args.serve_rate, "wall_s": el, "value": pts_steps / el,
-           "unit": "N^3*steps/s (sum over requests)", **lat, "max_active": srv.max_active,
-           "budget_bytes": srv.budget, "peak_bytes": srv.peak_used,
+           "unit": "N^3*steps/s (sum over requests)", **lat, "latency_clock": "device timestamps per lane chunk",
+           "max_active": srv.max_active, "budget_bytes": srv.budget, "peak_bytes": srv.peak_used,
+           "memory_releases": srv.releases,
            "finite": sum(1 for j in done if all(np.all(np.isfinite(r[0][2:12])) for r in j.stats))}
     log(f"serving: {R} requests in {el:.1f}s, {pts_steps / el:.4e} N^3*steps/s, step p50 "
         f"{lat.get('step_ms_p50', 0):.3f} ms p99 {lat.get('step_ms_p99', 0):.3f} ms, max active {srv.max_active}")

@@ -1700,6 +1700,10 @@ struct ffs_batch {
   bool lead_behind = true;    // lead must wait for the runs' streams before the next graph
   bool runs_behind = false;   // the runs' streams must wait for lead before their next work
   std::string err;
+  // device timestamps around the work of the last advance call that took a step: t0 after every run's
+  // earlier work, t1 after all of this call's work (both on lead)
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  int64_t timed_steps = 0;
 
   void create(ffs_sim* const* sims, int n) {
     if (n <= 0 || n > FFS_MAX_BATCH) throw FfsError(FFS_ERR_ARG, "batch size must be 1.." + std::to_string(FFS_MAX_BATCH));
@@ -1719,15 +1723,20 @@ struct ffs_batch {
     HIPCHECK(hipEventCreateWithFlags(&join_lead, hipEventDisableTiming));
     joins.resize(n);
     for (auto& e : joins) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHECK(hipEventCreate(&t0));
+    HIPCHECK(hipEventCreate(&t1));
   }
 
   void destroy() {
     if (lead) hipStreamSynchronize(lead);
     drop_cache();
+    if (lead) hipStreamSynchronize(lead);  // the tables' release
     for (auto e : joins)
       if (e) hipEventDestroy(e);
     if (fork) hipEventDestroy(fork);
     if (join_lead) hipEventDestroy(join_lead);
+    if (t0) hipEventDestroy(t0);
+    if (t1) hipEventDestroy(t1);
     if (lead) hipStreamDestroy(lead);
     lead = nullptr;
   }
@@ -1742,7 +1751,7 @@ struct ffs_batch {
     for (auto& c : cache) {
       if (c.exec) hipGraphExecDestroy(c.exec);
       if (c.graph) hipGraphDestroy(c.graph);
-      if (c.calls) hipFree(c.calls);
+      if (c.calls) hipFreeAsync(c.calls, lead);
     }
     cache.clear();
     last_graph = -1;
@@ -1846,8 +1855,11 @@ struct ffs_batch {
             const std::vector<unsigned char>& call = rec[merged[m][r]].launches[j].call;
             std::memcpy(host.data() + offs[m][j] + r * call.size(), call.data(), call.size());
           }
-      HIPCHECK(hipMalloc(&c.calls, bytes));
-      HIPCHECK(hipMemcpy(c.calls, host.data(), bytes, hipMemcpyHostToDevice));
+      // stream-ordered, so that neither this allocation nor its release (drop_cache) waits for the device
+      // as a whole: a serving loop rebuilds ensembles while others run
+      HIPCHECK(hipMallocAsync(&c.calls, bytes, lead));
+      HIPCHECK(hipMemcpyAsync(c.calls, host.data(), bytes, hipMemcpyHostToDevice, lead));
+      HIPCHECK(hipStreamSynchronize(lead));  // `host` goes out of scope
     }
     try {
       HIPCHECK(hipStreamBeginCapture(lead, hipStreamCaptureModeThreadLocal));
@@ -1879,7 +1891,7 @@ struct ffs_batch {
           hipStreamEndCapture(lead, &gr) == hipSuccess && gr)
         hipGraphDestroy(gr);
       if (c.graph) hipGraphDestroy(c.graph);
-      if (c.calls) hipFree(c.calls);
+      if (c.calls) hipFreeAsync(c.calls, lead);
       throw;
     }
     {  // kernel launches per step, counted in the graph
@@ -1917,18 +1929,38 @@ struct ffs_batch {
 
   void advance(int64_t nsteps, int64_t* done, int* event, int* who) {
     lead_behind = true;  // the runs may have been used on their own since the last call
+    int64_t taken = 0;
+    bool marked = false;
+    // the start mark goes in before the first step, after every run's earlier work
+    auto mark_start = [&] {
+      if (marked) return;
+      lead_waits_runs();
+      HIPCHECK(hipEventRecord(t0, lead));
+      runs_behind = true;
+      marked = true;
+    };
+    // the end mark, after every run's work of this call
+    auto mark_end = [&] {
+      if (!marked) return;
+      lead_waits_runs();
+      HIPCHECK(hipEventRecord(t1, lead));
+      runs_behind = true;
+      timed_steps = taken;
+    };
     for (int64_t k = 0; k < nsteps; ++k) {
       for (size_t i = 0; i < runs.size(); ++i) {
         ffs_sim* r = runs[i];
         if (r->is_output_time(r->iteration) && r->output_served != r->iteration) {
           for (auto* q : runs)  // every run due at this step is served by the one event
             if (q->is_output_time(q->iteration)) q->output_served = q->iteration;
+          mark_end();
           if (runs_behind) runs_wait_lead();
           if (event) *event = FFS_EVENT_OUTPUT;
           if (who) *who = (int)i;
           return;
         }
       }
+      mark_start();
       bool all_steady = true;
       for (auto* r : runs) all_steady = all_steady && r->steady();
       if (all_steady) {
@@ -1942,8 +1974,23 @@ struct ffs_batch {
         lead_behind = true;
       }
       if (done) *done += 1;
+      ++taken;
     }
+    mark_end();
     if (runs_behind) runs_wait_lead();
+  }
+
+  // 1: the work of every advance call so far has finished on the device; 0: some is still queued
+  int idle() const {
+    hipError_t e = hipStreamQuery(lead);
+    if (e == hipErrorNotReady) return 0;
+    HIPCHECK(e);
+    for (auto* r : runs) {
+      e = hipStreamQuery(r->stream);
+      if (e == hipErrorNotReady) return 0;
+      HIPCHECK(e);
+    }
+    return 1;
   }
 };
 
@@ -2726,6 +2773,34 @@ int ffs_batch_synchronize(ffs_batch* b) {
   try {
     HIPCHECK(hipStreamSynchronize(b->lead));
     for (auto* r : b->runs) HIPCHECK(hipStreamSynchronize(r->stream));
+    return FFS_OK;
+  } catch (const FfsError& e) {
+    b->err = e.what();
+    return e.code;
+  }
+}
+
+int ffs_batch_query(ffs_batch* b) {
+  if (!b) return FFS_ERR_ARG;
+  try {
+    return b->idle();
+  } catch (const FfsError& e) {
+    b->err = e.what();
+    return e.code;
+  }
+}
+
+int ffs_batch_last_advance_ms(ffs_batch* b, double* ms, int64_t* steps) {
+  if (!b || !ms) return FFS_ERR_ARG;
+  try {
+    if (b->timed_steps <= 0) throw FfsError(FFS_ERR_STATE, "no advance call has taken a step yet");
+    const hipError_t e = hipEventQuery(b->t1);
+    if (e == hipErrorNotReady) return 1;
+    HIPCHECK(e);
+    float f = 0;
+    HIPCHECK(hipEventElapsedTime(&f, b->t0, b->t1));
+    *ms = f;
+    if (steps) *steps = b->timed_steps;
     return FFS_OK;
   } catch (const FfsError& e) {
     b->err = e.what();

@@ -157,6 +157,8 @@ SIGNATURES = {
     "ffs_batch_advance": (ctypes.c_int, [_P, ctypes.c_int64, _I64, _I, _I]),
     "ffs_batch_synchronize": (ctypes.c_int, [_P]),
     "ffs_batch_set_merge": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ffs_batch_query": (ctypes.c_int, [_P]),
+    "ffs_batch_last_advance_ms": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), _I64]),
     "ffs_batch_launch_info": (ctypes.c_int, [_P, _I, _I, _I]),
     "ffs_batch_last_error": (ctypes.c_char_p, [_P]),
     "ffs_batch_destroy": (None, [_P]),

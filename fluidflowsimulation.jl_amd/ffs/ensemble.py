@@ -69,6 +69,16 @@ class Ensemble:
     def synchronize(self):
         self._check(self._lib.ffs_batch_synchronize(self._h), self._h)
 
+    def idle(self):
+        """True when every run's queued work has finished on the device (does not block)."""
+        return self._check(self._lib.ffs_batch_query(self._h), self._h) == 1
+
+    def last_advance_ms(self):
+        """(device ms, steps) of the last advance call that took a step, or None while it still runs."""
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        rc = self._check(self._lib.ffs_batch_last_advance_ms(self._h, ctypes.byref(ms), ctypes.byref(n)), self._h)
+        return None if rc == 1 else (ms.value, n.value)
+
     def close(self):
         if getattr(self, "_h", None):
             self._lib.ffs_batch_destroy(self._h)

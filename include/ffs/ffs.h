@@ -402,6 +402,15 @@ int ffs_batch_create(ffs_sim* const* sims, int n, ffs_batch** out);
 int ffs_batch_advance(ffs_batch* b, int64_t nsteps, int64_t* done, int* event, int* who);
 /* Blocks until every run's queued work is done. */
 int ffs_batch_synchronize(ffs_batch* b);
+/* Without blocking: 1 when every run's queued work is done, 0 while some is
+ * still running, or an FFS_ERR_* code. */
+int ffs_batch_query(ffs_batch* b);
+/* Device time of the last ffs_batch_advance call that took a step, from a
+ * timestamp taken after every run's earlier work to one after all of the
+ * call's work (so it counts what the runs waited for each other and for
+ * other work on the device), and the steps it took.  Returns 1 while that
+ * work is still running, FFS_ERR_STATE before any step. */
+int ffs_batch_last_advance_ms(ffs_batch* b, double* ms, int64_t* steps);
 /* on = 0: every run keeps a branch of its own (no merged launches); on = 1
  * (default) merges runs of the same launch shapes.  Bitwise the same results
  * either way. */
