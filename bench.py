@@ -89,11 +89,15 @@ def main():
                     help="first time a plain 1 GiB device copy (the box's measured copy bandwidth, beside the 8 TB/s "
                          "peak, and the calibration of the traffic counters); on by default")
     ap.add_argument("--no-copy-probe", dest="copy_probe", action="store_false")
+    ap.add_argument("--plan", choices=["auto", "default"], default="auto",
+                    help="kernel plans: tuned per workload on this device (cached, see ffs/tune.py) or the defaults")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(runs shard as independent jobs; see the design notes)", file=sys.stderr)
         sys.exit(2)
+    global PLAN_MODE
+    PLAN_MODE = "auto" if args.plan == "auto" else None
     raise SystemExit(bench(args))
 
 
@@ -154,6 +158,18 @@ def copy_probe():
     return gbs
 
 
+# the kernel plan of every run the bench creates: "auto" (ffs.tune: timed on a twin before the run is
+# created, then cached) or None (the library's defaults); PLANS records what each plan key ran with
+PLAN_MODE = "auto"
+PLANS = {}
+
+
+def record_plan(s):
+    from ffs import tune
+
+    PLANS[tune.plan_key(s.params)] = s.plan
+
+
 def make_forced(n, nu, prec, dt_stat, seed, les=False):
     """One run of the forced workload, its initial field generated on the device from `seed`."""
     import ffs
@@ -162,7 +178,8 @@ def make_forced(n, nu, prec, dt_stat, seed, les=False):
                    dt_stat=dt_stat, dt_spec=1 << 30, write_time=1 << 30, precision=prec,
                    forcing="shellForcing", forcing_kf=2.5, forcing_power=0.1,
                    les_model="Smagorinsky" if les else None, smagorinsky_c=0.16)
-    s = ffs.Simulation(p)
+    s = ffs.Simulation(p, plan=PLAN_MODE)
+    record_plan(s)
     s.seed_velocity(seed, k0=4.0, energy=0.5)
     s.initialize()
     return s
@@ -202,6 +219,7 @@ def bench(args):
     b = 4 if args.fp32 else 8
     log(f"workload n={n} nu={nu:.6e} precision={prec}")
     s = make_forced(n, nu, prec, args.dt_stat, args.seed)
+    main_plan = s.plan
     if args.no_fusion:
         s.set_fusion(False)
     s.synchronize()
@@ -308,6 +326,7 @@ def bench(args):
         "kernels": kernels,
         "copy_probe_GBs": copy_gbs,
     }
+    out["config"]["plan"] = main_plan
     legs = [x for x in args.legs.split(",") if x]
     if "fp32" in legs and not args.fp32:
         out["fp32"] = leg_fp32(args, n, nu, value)
@@ -321,6 +340,9 @@ def bench(args):
         out["features"] = leg_features(args, n, nu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, nu, args)
+    from ffs import tune
+
+    out["plans"] = {"mode": args.plan, "cache": tune.cache_path() if PLAN_MODE else None, "by_key": PLANS}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -478,9 +500,21 @@ def leg_serve(args):
         return Job(p, horizons[i], lambda s, k=args.seed + 500 + i: s.seed_velocity(k), name=f"req{i}",
                    keep_fields=False)
 
-    srv = Server(budget=int(args.serve_budget_gb * 1e9), max_runs=args.serve_max_runs, chunk=args.serve_chunk)
+    jobs = [job(i) for i in range(R)]
+    if PLAN_MODE == "auto":  # tune every plan key before the clock starts (later lookups hit the cache)
+        from ffs import tune
+
+        t0 = time.perf_counter()
+        keys = {}
+        for j in jobs:
+            keys.setdefault(tune.plan_key(j.params), j.params)
+        for k, p in keys.items():
+            PLANS[k] = tune.tuned_plan(p)
+        log(f"serving: {len(keys)} plan keys tuned or read from the cache in {time.perf_counter() - t0:.1f}s")
+    srv = Server(budget=int(args.serve_budget_gb * 1e9), max_runs=args.serve_max_runs, chunk=args.serve_chunk,
+                 plan=PLAN_MODE)
     t0 = time.perf_counter()
-    done = srv.run(arrivals=[(float(arrivals[i]), job(i)) for i in range(R)])
+    done = srv.run(arrivals=[(float(arrivals[i]), jobs[i]) for i in range(R)])
     el = time.perf_counter() - t0
     pts_steps = float(sum(j.params.nx**3 * j.nsteps for j in done))
     lat = srv.latency()
@@ -514,7 +548,8 @@ def leg_strat(args):
                           reference_density=1.0)
 
     def seeded(p):
-        s = ffs.Simulation(p)
+        s = ffs.Simulation(p, plan=PLAN_MODE)
+        record_plan(s)
         s.seed_velocity(args.seed, k0=4.0, energy=0.5)
         s.seed_density(args.seed + 1, k0=4.0, rms=0.01)
         s.initialize()

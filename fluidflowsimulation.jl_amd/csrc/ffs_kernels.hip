@@ -84,9 +84,6 @@ template <int N, int NSL> __host__ __device__ constexpr int xgen_rows() {
 #ifndef FFS_XNL_BRANCHFREE
 #define FFS_XNL_BRANCHFREE 1  // 1: the x pass loads its inputs without branches
 #endif
-#ifndef FFS_XNL_MODES
-#define FFS_XNL_MODES 1  // 1: density flux and LES stress through the register x pass too
-#endif
 #ifndef FFS_XNL_PREFETCH
 #define FFS_XNL_PREFETCH 1  // 1: request every input of a row before its first transform
 #endif
@@ -1455,9 +1452,10 @@ static void run_x_gen(const Geo& g, const void* a2, void* b2, const XOps& ops, u
 
 template <typename T, int N, int MODE, bool KE>
 static void launch_xreg_plain(const Geo& g, const void* a2, void* b2, const void* tw, unsigned long long* vmax,
-                              const XRegArgs& xa, hipStream_t s) {
+                              const XRegArgs& xa, hipStream_t s, int waves) {
   constexpr int TP = N / xreg_ppt<T, N>();
-  constexpr int RPB = 2 * (64 / TP);
+  if (waves != 1 && waves != 2) throw std::runtime_error("x pass: 1 or 2 wavefronts per block");
+  const int RPB = waves * (64 / TP);
   const size_t lds = ((size_t)RPB * xreg_ldx<T, N>() + tw_lds<T>(N)) * sizeof(C<T>);
   static std::once_flag once;
   lds_limit_once(once, (const void*)&k_x_nl_reg<T, N, MODE, KE>, lds, "the x pass");
@@ -1466,27 +1464,28 @@ static void launch_xreg_plain(const Geo& g, const void* a2, void* b2, const void
   const int nrows = g.ny * g.nz;
   const XCall<T> c{g, (const C<T>*)a2, (C<T>*)b2, (const C<T>*)tw, vmax, xa};
   const dim3 grid((nrows + RPB - 1) / RPB);
-  launch_call((const void*)&k_x_nl_reg_b<T, N, MODE, KE>, grid, dim3(128), lds, c, "x pass", [&] {
-    k_x_nl_reg<T, N, MODE, KE><<<grid, 128, lds, s>>>(c.g, c.a2, c.b2, c.tw, c.vmax, c.xa);
+  launch_call((const void*)&k_x_nl_reg_b<T, N, MODE, KE>, grid, dim3(64 * waves), lds, c, "x pass", [&] {
+    k_x_nl_reg<T, N, MODE, KE><<<grid, 64 * waves, lds, s>>>(c.g, c.a2, c.b2, c.tw, c.vmax, c.xa);
   });
 }
 
 template <typename T, int N, int MODE, bool KE>
 static void launch_xreg(const Geo& g, const void* a2, void* b2, const void* tw, unsigned long long* vmax,
-                        const XRegArgs& xa, hipStream_t s) {
+                        const XRegArgs& xa, hipStream_t s, const XOps& ops) {
   constexpr bool PAIR = xpair_ok<T, N>();
   // the mirror-paired pass assumes the x Nyquist mode is dealiased (mx <= N/2; cutoff < 1)
-  if (g.mx > N / 2 || !PAIR) {
+  if (g.mx > N / 2 || !PAIR || !ops.pair) {
     if constexpr (xreg_ok<T, N>()) {
-      launch_xreg_plain<T, N, MODE, KE>(g, a2, b2, tw, vmax, xa, s);
+      launch_xreg_plain<T, N, MODE, KE>(g, a2, b2, tw, vmax, xa, s, ops.waves);
       return;
     } else {
       throw std::runtime_error("x pass: no register variant for this length");
     }
   }
   constexpr int TP = N / 8;
-  constexpr int WPB = 2;  // wavefronts per block
-  constexpr int RPB = WPB * (64 / TP);
+  const int WPB = ops.waves;  // wavefronts per block
+  if (WPB != 1 && WPB != 2) throw std::runtime_error("x pass: 1 or 2 wavefronts per block");
+  const int RPB = WPB * (64 / TP);
   const size_t lds = ((size_t)RPB * xreg_ldx<T, N>() + tw_lds<T>(N)) * sizeof(C<T>);
   const int nrows = g.ny * g.nz;
   if constexpr (PAIR) {
@@ -1510,7 +1509,7 @@ static int x_general_slots(const XOps& ops) {
   const bool plain = !ops.rho && !ops.les && !ops.phi;
   const bool w_les = (ops.les && les_needs_w(ops.lm)) || ops.rho_les || ops.phi_les;
   if constexpr (xreg_ok<T, N>()) {
-    if ((FFS_XNL_MODES || plain) && !w_les) return 0;
+    if ((ops.split || plain) && !w_les) return 0;
   }
   return (std::max(ops.nin, ops.nout) + 1) / 2;
 }
@@ -1557,6 +1556,30 @@ template std::string x_pass_check<float>(int, const XOps&);
 template std::string x_pass_check<double>(int, const XOps&);
 
 template <typename T, int N>
+static void x_plan_options_n(int mx, const XOps& ops, int* pair, int* waves, int* split) {
+  XOps a = ops, b = ops;
+  a.split = 1;
+  b.split = 0;
+  const bool reg_split = x_general_slots<T, N>(a) == 0, reg_joint = x_general_slots<T, N>(b) == 0;
+  *split = (reg_split != reg_joint && x_check<T, N>(b).empty()) ? 1 : 0;
+  const bool reg = x_general_slots<T, N>(ops) == 0;
+  *waves = reg ? 1 : 0;
+  *pair = (reg && xpair_ok<T, N>() && mx <= N / 2) ? 1 : 0;
+}
+template <typename T>
+void x_plan_options(int n, int mx, const XOps& ops, int* pair, int* waves, int* split) {
+  switch (n) {
+#define X(m) \
+  case m: return x_plan_options_n<T, m>(mx, ops, pair, waves, split);
+    FFS_SIZES(X)
+#undef X
+    default: *pair = *waves = *split = 0;
+  }
+}
+template void x_plan_options<float>(int, int, const XOps&, int*, int*, int*);
+template void x_plan_options<double>(int, int, const XOps&, int*, int*, int*);
+
+template <typename T, int N>
 static void run_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
                      unsigned long long* rhomax, const void* tw, hipStream_t s) {
   if constexpr (xreg_ok<T, N>()) {
@@ -1565,25 +1588,25 @@ static void run_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, un
       // tau as launches of their own
       XRegArgs xa{0, 0, ops.nout - 1, 0.0};
       if (ops.ns) {
-        if (ops.ke) launch_xreg<T, N, 3, true>(g, a2, b2, tw, umax, xa, s);
-        else launch_xreg<T, N, 3, false>(g, a2, b2, tw, umax, xa, s);
+        if (ops.ke) launch_xreg<T, N, 3, true>(g, a2, b2, tw, umax, xa, s, ops);
+        else launch_xreg<T, N, 3, false>(g, a2, b2, tw, umax, xa, s, ops);
       } else {
-        if (ops.ke) launch_xreg<T, N, 0, true>(g, a2, b2, tw, umax, xa, s);
-        else launch_xreg<T, N, 0, false>(g, a2, b2, tw, umax, xa, s);
+        if (ops.ke) launch_xreg<T, N, 0, true>(g, a2, b2, tw, umax, xa, s, ops);
+        else launch_xreg<T, N, 0, false>(g, a2, b2, tw, umax, xa, s, ops);
       }
       const int ns = (ops.rho ? 1 : 0) + (ops.phi ? 1 : 0);
       const int i0 = ops.ns ? 3 : 6, o0 = ops.ns ? 5 : 3;  // first scalar input / output
       if (ops.rho) {
         XRegArgs xr{i0, o0, 0, 0.0};
-        launch_xreg<T, N, 2, false>(g, a2, b2, tw, rhomax, xr, s);
+        launch_xreg<T, N, 2, false>(g, a2, b2, tw, rhomax, xr, s, ops);
       }
       if (ops.phi) {  // -phi u through the same kernel, without a maximum
         XRegArgs xp{i0 + (ops.rho ? 1 : 0), o0 + (ops.rho ? 3 : 0), 0, 0.0};
-        launch_xreg<T, N, 2, false>(g, a2, b2, tw, nullptr, xp, s);
+        launch_xreg<T, N, 2, false>(g, a2, b2, tw, nullptr, xp, s, ops);
       }
       if (ops.les) {
         XRegArgs xl{i0 + ns, o0 + 3 * ns, 0, ops.les_2alpha};
-        launch_xreg<T, N, 1, false>(g, a2, b2, tw, nullptr, xl, s);
+        launch_xreg<T, N, 1, false>(g, a2, b2, tw, nullptr, xl, s, ops);
       }
       return;
     }

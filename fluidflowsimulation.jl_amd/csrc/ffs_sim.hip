@@ -201,6 +201,8 @@ struct ffs_sim {
 
   // plain steps run the forward z pass and the update as one kernel (ffs_set_fusion; default FP64 only)
   bool fuse_ok = true;
+  // the x pass's plan (ffs_set_plan): mirror-paired rows, wavefronts per block, split product launches
+  int x_pair = 1, x_waves = 2, x_split = 1;
   // steady-state graphs: one per history-rotation phase
   bool graph_on = true;
   int phase = 0;  // history rotations mod 3
@@ -1020,8 +1022,31 @@ struct ffs_sim {
     ops.lm = lesm;
     ops.rho_les = dles;
     ops.phi_les = sles;
+    ops.pair = x_pair;
+    ops.waves = x_waves;
+    ops.split = x_split;
     const std::string e = fp32 ? x_pass_check<float>(p.nx, ops) : x_pass_check<double>(p.nx, ops);
     if (!e.empty()) throw FfsError(FFS_ERR_ARG, e);
+  }
+
+  // which of the plan's choices change this run's kernels
+  void plan_options(int* pair, int* waves, int* split, int* fusion) const {
+    XOps ops{};
+    ops.nin = nin;
+    ops.nout = nfw;
+    ops.rho = dens ? 1 : 0;
+    ops.phi = pscal ? 1 : 0;
+    ops.ns = nsform ? 1 : 0;
+    ops.les = (les && !dyn) ? 1 : 0;
+    ops.lm = lesm;
+    ops.rho_les = dles;
+    ops.phi_les = sles;
+    ops.pair = x_pair;
+    ops.waves = x_waves;
+    ops.split = x_split;
+    if (fp32) x_plan_options<float>(p.nx, g.mx, ops, pair, waves, split);
+    else x_plan_options<double>(p.nx, g.mx, ops, pair, waves, split);
+    *fusion = (!nsform && !dens && !pscal && !les && !plane && p.hyperviscosity == FFS_HV_NONE) ? 1 : 0;
   }
 
   // a Nyquist index is retained in some direction (cutoff >= 1): the Hermitian planes need restoring
@@ -1057,6 +1082,9 @@ struct ffs_sim {
     ops.rho_cv = dles_cv;
     ops.phi_cv = sles_cv;
     ops.les_alpha = les_alpha;
+    ops.pair = x_pair;
+    ops.waves = x_waves;
+    ops.split = x_split;
     unsigned long long* um = p.variable_dt ? &st->umax : nullptr;
     unsigned long long* rm = (p.variable_dt && dens) ? &st->rhomax : nullptr;
     launch("x_nl", [&] { FFS_DISPATCH(launch_x_nl, g, w2, w1, ops, um, rm, tw_x, stream); });
@@ -2730,6 +2758,56 @@ int ffs_set_fusion(ffs_sim* s, int on) {
     s->drop_graphs();  // the captured steps hold the other kernel sequence
     s->fuse_ok = on != 0;
   });
+}
+
+int ffs_device_name(char* buf, size_t n) {
+  if (!buf || n == 0) return FFS_ERR_ARG;
+  int dev = 0;
+  hipDeviceProp_t prop{};
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return FFS_ERR_HIP;
+  const std::string name = std::string(prop.name) + " " + prop.gcnArchName + " " +
+                           std::to_string(prop.multiProcessorCount) + "CU";
+  std::snprintf(buf, n, "%s", name.c_str());
+  return FFS_OK;
+}
+
+int ffs_get_plan(const ffs_sim* s, ffs_plan* out) {
+  if (!s || !out) return FFS_ERR_ARG;
+  *out = ffs_plan{};
+  out->x_pair = s->x_pair;
+  out->x_waves = s->x_waves;
+  out->x_split = s->x_split;
+  out->fusion = s->fuse_ok ? 1 : 0;
+  return FFS_OK;
+}
+
+int ffs_set_plan(ffs_sim* s, const ffs_plan* plan) {
+  if (!s || !plan) return FFS_ERR_ARG;
+  return guard(s, [&] {
+    if (plan->x_waves != 1 && plan->x_waves != 2) throw FfsError(FFS_ERR_ARG, "x_waves must be 1 or 2");
+    const int old[3] = {s->x_pair, s->x_waves, s->x_split};
+    s->x_pair = plan->x_pair != 0;
+    s->x_waves = plan->x_waves;
+    s->x_split = plan->x_split != 0;
+    try {
+      s->check_lengths();  // the general pass must hold the row if the products are not split
+    } catch (...) {
+      s->x_pair = old[0];
+      s->x_waves = old[1];
+      s->x_split = old[2];
+      throw;
+    }
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    s->drop_graphs();  // the captured steps hold the old plan's launches
+    s->fuse_ok = plan->fusion != 0;
+  });
+}
+
+int ffs_plan_options(const ffs_sim* s, ffs_plan* matters) {
+  if (!s || !matters) return FFS_ERR_ARG;
+  *matters = ffs_plan{};
+  s->plan_options(&matters->x_pair, &matters->x_waves, &matters->x_split, &matters->fusion);
+  return FFS_OK;
 }
 
 int ffs_batch_create(ffs_sim* const* sims, int n, ffs_batch** out) {

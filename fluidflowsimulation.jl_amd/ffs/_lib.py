@@ -117,6 +117,13 @@ class FfsParams(ctypes.Structure):
     ]
 
 
+class FfsPlan(ctypes.Structure):
+    """ffs_plan of include/ffs/ffs.h."""
+
+    _fields_ = [("x_pair", ctypes.c_int), ("x_waves", ctypes.c_int), ("x_split", ctypes.c_int),
+                ("fusion", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/ffs/ffs.h
 _P = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
@@ -153,6 +160,10 @@ SIGNATURES = {
     "ffs_device_memory": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "ffs_set_graph": (ctypes.c_int, [_P, ctypes.c_int]),
     "ffs_set_fusion": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ffs_get_plan": (ctypes.c_int, [_P, ctypes.POINTER(FfsPlan)]),
+    "ffs_set_plan": (ctypes.c_int, [_P, ctypes.POINTER(FfsPlan)]),
+    "ffs_plan_options": (ctypes.c_int, [_P, ctypes.POINTER(FfsPlan)]),
+    "ffs_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "ffs_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(_P)]),
     "ffs_batch_advance": (ctypes.c_int, [_P, ctypes.c_int64, _I64, _I, _I]),
     "ffs_batch_synchronize": (ctypes.c_int, [_P]),

@@ -100,10 +100,11 @@ class Server:
     budget: device bytes the runs may hold (default: 90% of the free memory at
     construction).  max_runs: cap on concurrently admitted runs.  chunk: most
     lockstep steps a lane is given at a time (its admission and retirement
-    points).  A retired run's device memory is released when a queued job needs
-    it or when the loop ends (a release waits for the whole device)."""
+    points).  plan: every run's kernel plan (see Simulation).  A retired run's
+    device memory is released when a queued job needs it or when the loop ends
+    (a release waits for the whole device)."""
 
-    def __init__(self, budget=None, max_runs=64, chunk=50, poll_s=5e-5):
+    def __init__(self, budget=None, max_runs=64, chunk=50, poll_s=5e-5, plan=None):
         if budget is None:
             free, _ = device_memory()
             budget = int(0.9 * free)
@@ -111,6 +112,7 @@ class Server:
         self.max_runs = int(max_runs)
         self.chunk = int(chunk)
         self.poll_s = float(poll_s)
+        self.plan = plan  # passed to every run's Simulation ("auto": the tuned plan of its parameters)
         self.queue = deque()
         self.lanes = {}
         self.done = []
@@ -152,7 +154,7 @@ class Server:
             n = plan_admission(fps, self.used, self.budget, n_active, self.max_runs)
         for _ in range(n):
             job = self.queue.popleft()
-            sim = Simulation(job.params)
+            sim = Simulation(job.params, plan=self.plan)
             job.init(sim)
             sim.initialize()
             job.started = time.perf_counter()

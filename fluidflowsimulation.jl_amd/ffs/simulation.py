@@ -26,7 +26,9 @@ def _dptr(a):
 class Simulation:
     """One run on the GPU (an `ffs_sim` handle)."""
 
-    def __init__(self, params: Params):
+    def __init__(self, params: Params, plan=None):
+        """plan: None keeps the library's default kernel plan, "auto" takes the tuned plan of these
+        parameters on this device (ffs.tune: timed once, then cached), a dict sets the fields it names."""
         self.params = params
         self._lib = _lib.load()
         cp = params.to_c()
@@ -35,6 +37,12 @@ class Simulation:
         self._h = h
         self.nx, self.ny, self.nz = params.nx, params.ny, params.nz
         self.stats_rows = []
+        if plan is not None:
+            if plan == "auto":
+                from .tune import tuned_plan
+
+                plan = tuned_plan(params)
+            self.set_plan(plan)
 
     # ---------------------------------------------------------------- lifetime
     def close(self):
@@ -194,6 +202,31 @@ class Simulation:
         """Run the forward z pass and the update of plain steps as one kernel (default on for FP64, off
         for FP32; bitwise identical to the two kernels)."""
         self._check(self._lib.ffs_set_fusion(self._h, int(on)))
+
+    PLAN_FIELDS = ("x_pair", "x_waves", "x_split", "fusion")
+
+    @property
+    def plan(self):
+        """The run's kernel plan (ffs_plan) as a dict."""
+        p = _lib.FfsPlan()
+        self._check(self._lib.ffs_get_plan(self._h, ctypes.byref(p)))
+        return {k: getattr(p, k) for k in self.PLAN_FIELDS}
+
+    def set_plan(self, plan):
+        """Applies a plan (a dict with any of x_pair, x_waves, x_split, fusion; the rest kept)."""
+        cur = self.plan
+        unknown = set(plan) - set(cur)
+        if unknown:
+            raise KeyError(f"unknown plan fields {sorted(unknown)}")
+        cur.update(plan)
+        p = _lib.FfsPlan(**{k: int(v) for k, v in cur.items()})
+        self._check(self._lib.ffs_set_plan(self._h, ctypes.byref(p)))
+
+    def plan_options(self):
+        """Which plan fields change this run's kernels: dict of field -> bool."""
+        p = _lib.FfsPlan()
+        self._check(self._lib.ffs_plan_options(self._h, ctypes.byref(p)))
+        return {k: bool(getattr(p, k)) for k in self.PLAN_FIELDS}
 
     # ---------------------------------------------------------------- diagnostics
     def pop_stats(self):

@@ -382,6 +382,28 @@ int ffs_set_graph(ffs_sim* s, int on);
  * reference. */
 int ffs_set_fusion(ffs_sim* s, int on);
 
+/* A run's plan: the kernel choices that do not change what is computed, only
+ * how (results agree with the oracle to the same tolerance under every plan,
+ * and are bitwise reproducible under one plan).  An autotuner times the
+ * candidates (ffs/tune.py) and keeps the fastest per device, grid, precision
+ * and model.  No counterpart in the reference (FFTW picks its own plans). */
+typedef struct ffs_plan {
+  int x_pair;  /* 1: FP64 power-of-two x passes use the mirror-paired kernel */
+  int x_waves; /* wavefronts per block of the register x passes: 1 or 2 */
+  int x_split; /* 1: density / scalar fluxes and the Smagorinsky stress in
+                * register x launches of their own; 0: one general x pass */
+  int fusion;  /* as ffs_set_fusion */
+  int reserved[4];
+} ffs_plan;
+int ffs_get_plan(const ffs_sim* s, ffs_plan* out);
+/* The current device as "<name> <arch> <CUs>CU" (the tuner's cache key). */
+int ffs_device_name(char* buf, size_t n);
+/* Applies a plan (FFS_ERR_ARG, the plan unchanged, if the x pass cannot hold
+ * the run's rows under it); waits for the run's queued work. */
+int ffs_set_plan(ffs_sim* s, const ffs_plan* plan);
+/* Which fields of a plan change this run's kernels (1) and which do not (0). */
+int ffs_plan_options(const ffs_sim* s, ffs_plan* matters);
+
 /* An ensemble: up to FFS_MAX_BATCH initialized handles on the current device
  * advanced in lockstep.  A step in which every run is steady is one graph
  * launch whose branches are independent, so they overlap on the device: runs

@@ -1,0 +1,107 @@
+"""Every kernel plan a run admits (ffs_plan: mirror-paired or plain x pass, waves per x block, split or
+joint product launches, fused or separate forward z + update) against the oracle, and the tuner end to
+end on the device: it picks one of the candidates, persists it, and Simulation(plan="auto") applies it.
+Tolerances as the parity tests: FP64 fields 1e-11 relative to the component's max after three steps,
+FP32 2e-5."""
+import numpy as np
+import pytest
+
+import ffs
+from ffs import tune
+from ffs.initial import random_scalar, random_spectrum
+from oracle.ffs_oracle import Grid, OracleSim
+
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+DENS = {"Pr": 0.7, "gradient": 1.0, "g": -1.0}
+# (name, n, precision, les, density, shell forcing)
+CASES = [
+    ("plain fp64 64^3, shell forcing", 64, "fp64", False, False, True),
+    ("smagorinsky fp32 64^3", 64, "fp32", True, False, False),
+    ("smagorinsky + boussinesq fp64 32^3", 32, "fp64", True, True, False),
+    ("smagorinsky fp64 32^3", 32, "fp64", True, False, False),
+    ("plain fp32 32^3", 32, "fp32", False, False, False),
+]
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def make_params(n, prec, les, dens, forcing):
+    p = ffs.Params(nx=n, ny=n, nz=n, nu=2e-3, integrator="Adams_Bashforth3rdO", variable_dt=False, dt=1e-3,
+                   dt_stat=1 << 30, dt_spec=1 << 30, write_time=1 << 30, precision=prec)
+    if les:
+        p.les_model, p.smagorinsky_c = "Smagorinsky", 0.16
+    if dens:
+        p.density, p.pr, p.density_gradient, p.z_acceleration, p.reference_density = True, 0.7, 1.0, -1.0, 1.0
+    if forcing:
+        p.forcing, p.forcing_kf, p.forcing_power = "shellForcing", 2.5, 0.1
+    return p
+
+
+def run(p, plan, u0, rho0, steps):
+    with ffs.Simulation(p, plan=plan) as s:
+        assert all(s.plan[k] == v for k, v in plan.items()), (s.plan, plan)
+        s.set_velocity(*u0)
+        if rho0 is not None:
+            s.set_density(rho0)
+        s.initialize()
+        s.advance(steps)
+        out = [s.velocity_hat()[c] for c in range(3)]
+        if rho0 is not None:
+            out.append(s.density_hat())
+        return out
+
+
+@pytest.mark.parametrize("name,n,prec,les,dens,forcing", CASES, ids=[c[0] for c in CASES])
+def test_every_candidate_plan_matches_oracle(name, n, prec, les, dens, forcing):
+    p = make_params(n, prec, les, dens, forcing)
+    with ffs.Simulation(p) as probe:
+        options, base = probe.plan_options(), probe.plan
+    cands = tune.candidates(options, base)
+    print_case(f"{name}: plan options {options}, {len(cands)} candidates")
+    g = Grid(n, n, n)
+    u0 = random_spectrum(n, n, n, seed=21)
+    rho0 = random_scalar(n, n, n, seed=22) if dens else None
+    ref = OracleSim(g, 2e-3, dt=1e-3, variable_dt=False, dt_stat=1 << 30, les_c=0.16 if les else None,
+                    density=DENS if dens else None, forcing_kf=2.5 if forcing else None, forcing_power=0.1)
+    ref.set_velocity(u0)
+    if dens:
+        ref.set_density(rho0)
+    ref.initialize()
+    ref.advance_in_time(3)
+    want = [ref.uh[c] for c in range(3)] + ([ref.rhoh] if dens else [])
+    tol = 1e-11 if prec == "fp64" else 2e-5
+    first = None
+    for c in cands:
+        print_case(f"{name}: plan {c}")
+        got = run(p, c, u0, rho0, 3)
+        for i, (a, b) in enumerate(zip(got, want)):
+            assert rel(a, b) < tol, (c, i, rel(a, b))
+        if first is None:
+            first = (c, got)
+    # a plan's results are reproducible bit for bit
+    again = run(p, first[0], u0, rho0, 3)
+    for a, b in zip(again, first[1]):
+        assert np.array_equal(a, b)
+
+
+def test_tuner_on_the_device(tmp_path, monkeypatch):
+    path = str(tmp_path / "plans.json")
+    monkeypatch.setenv(tune.CACHE_ENV, path)
+    p = make_params(32, "fp32", True, False, False)
+    print_case("tuning 32^3 fp32 smagorinsky into a fresh cache")
+    best = tune.tune(p, min_ms=5.0)
+    with ffs.Simulation(p) as probe:
+        cands = tune.candidates(probe.plan_options(), probe.plan)
+    assert best in cands
+    with ffs.Simulation(p, plan="auto") as s:  # read back from the cache
+        assert s.plan == best
+    import json
+
+    data = json.load(open(path))
+    entry = data[tune.device_name()][tune.library_id()][tune.plan_key(p)]
+    assert entry["plan"] == best and len(entry["ms_per_step"]) == len(cands)
