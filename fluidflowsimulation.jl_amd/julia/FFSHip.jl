@@ -423,6 +423,33 @@ end
 set_graph!(s::FFSSimulation, on::Bool) = check(ccall((:ffs_set_graph, libffs), Cint, (Ptr{Cvoid}, Cint), s.h, on), s.h)
 set_fusion!(s::FFSSimulation, on::Bool) = check(ccall((:ffs_set_fusion, libffs), Cint, (Ptr{Cvoid}, Cint), s.h, on), s.h)
 
+# ---- kernel plans (ffs_plan): the choices an autotuner makes; the tuner itself is ffs/tune.py ----
+mutable struct ffs_plan
+    x_pair::Cint
+    x_waves::Cint
+    x_split::Cint
+    fusion::Cint
+    reserved::NTuple{4,Cint}
+    ffs_plan() = new(0, 0, 0, 0, (0, 0, 0, 0))
+end
+
+function get_plan(s::FFSSimulation)
+    p = ffs_plan()
+    check(ccall((:ffs_get_plan, libffs), Cint, (Ptr{Cvoid}, Ref{ffs_plan}), s.h, p), s.h)
+    return p
+end
+set_plan!(s::FFSSimulation, p::ffs_plan) = check(ccall((:ffs_set_plan, libffs), Cint, (Ptr{Cvoid}, Ref{ffs_plan}), s.h, p), s.h)
+function plan_options(s::FFSSimulation)
+    p = ffs_plan()
+    check(ccall((:ffs_plan_options, libffs), Cint, (Ptr{Cvoid}, Ref{ffs_plan}), s.h, p), s.h)
+    return p
+end
+function device_name()
+    buf = Vector{UInt8}(undef, 256)
+    check(ccall((:ffs_device_name, libffs), Cint, (Ptr{UInt8}, Csize_t), buf, length(buf)), C_NULL)
+    return unsafe_string(pointer(buf))
+end
+
 # ---- ensembles: runs advanced in lockstep, one graph launch per steady step ----
 function batch_check(rc::Integer, b::Ptr{Cvoid})
     rc < 0 || return rc
@@ -452,6 +479,25 @@ function advance!(b::FFSBatch, n::Integer)
 end
 
 synchronize(b::FFSBatch) = batch_check(ccall((:ffs_batch_synchronize, libffs), Cint, (Ptr{Cvoid},), b.h), b.h)
+# true when every run's queued work has finished (does not block)
+isidle(b::FFSBatch) = batch_check(ccall((:ffs_batch_query, libffs), Cint, (Ptr{Cvoid},), b.h), b.h) == 1
+set_merge!(b::FFSBatch, on::Bool) = batch_check(ccall((:ffs_batch_set_merge, libffs), Cint, (Ptr{Cvoid}, Cint), b.h, on), b.h)
+
+# device ms and steps of the last advance! that took a step; nothing while it still runs
+function last_advance_ms(b::FFSBatch)
+    ms, n = Ref{Float64}(0), Ref{Int64}(0)
+    rc = batch_check(ccall((:ffs_batch_last_advance_ms, libffs), Cint, (Ptr{Cvoid}, Ref{Float64}, Ref{Int64}),
+                           b.h, ms, n), b.h)
+    return rc == 1 ? nothing : (ms[], n[])
+end
+
+# kernel launches per steady step, merged groups and the runs in them
+function launch_info(b::FFSBatch)
+    l, g, r = Ref{Cint}(0), Ref{Cint}(0), Ref{Cint}(0)
+    batch_check(ccall((:ffs_batch_launch_info, libffs), Cint, (Ptr{Cvoid}, Ref{Cint}, Ref{Cint}, Ref{Cint}),
+                      b.h, l, g, r), b.h)
+    return Int(l[]), Int(g[]), Int(r[])
+end
 
 # writeoutput (output.jl:1-8): u1.$it u2.$it u3.$it in the padded real layout
 function writeoutput(s::FFSSimulation; closing::Bool = false)
