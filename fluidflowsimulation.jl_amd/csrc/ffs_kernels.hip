@@ -25,6 +25,15 @@
 #include <stdexcept>
 #include <string>
 
+// The library compiles this file several times in parallel, once per part (FFS_PART = 0 .. 7): every part
+// sees all of the templates, but each instantiates only its own share of the launch functions below, and part
+// 0 alone holds the definitions that are not templates.  Without FFS_PART one object holds everything.
+#ifdef FFS_PART
+#define FFS_IN_PART(p) (FFS_PART == (p))
+#else
+#define FFS_IN_PART(p) 1
+#endif
+
 namespace fsk {
 
 // columns per tile of the y/z passes: one 128-byte line of x-consecutive values
@@ -164,7 +173,9 @@ static void lds_limit_once(std::once_flag& once, const void* fn, size_t bytes, c
 // before, its batch form an array of call records (one per run, all of one launch shape) with the run index
 // folded into grid z; both run the same body.  With a recorder set on the calling thread the launch is recorded, not issued, so
 // the launches of several runs' steps can be merged into one launch per pass (issue_batch).
+#if FFS_IN_PART(0)
 thread_local KRecorder* g_recorder = nullptr;
+#endif
 // a batch form's call records: read-only, the same for every thread of a block
 #define FFS_CALLS(T) const T* __restrict__
 
@@ -243,6 +254,7 @@ static void launch_call(const void* batch, dim3 grid, dim3 block, size_t lds, co
   single();
 }
 
+#if FFS_IN_PART(0)
 void issue_batch(const KLaunch& k, const void* calls, int nrun, hipStream_t s) {
   dim3 grid = k.grid;
   grid.z *= (unsigned)nrun;
@@ -250,6 +262,7 @@ void issue_batch(const KLaunch& k, const void* calls, int nrun, hipStream_t s) {
   const hipError_t e = hipLaunchKernel(k.batch, grid, k.block, args, k.lds, s);
   if (e != hipSuccess) throw std::runtime_error(std::string("batched launch of ") + k.name + ": " + hipGetErrorString(e));
 }
+#endif
 
 // ===========================================================================
 // Column (y / z) passes.  Block = TX consecutive x columns x TPER threads per
@@ -1532,6 +1545,7 @@ static std::string x_check(const XOps& ops) {
   return "";
 }
 
+#if FFS_IN_PART(0)
 bool length_supported(int n) {
   switch (n) {
 #define X(m) case m:
@@ -1541,6 +1555,7 @@ bool length_supported(int n) {
     default: return false;
   }
 }
+#endif
 
 template <typename T>
 std::string x_pass_check(int n, const XOps& ops) {
@@ -1552,8 +1567,10 @@ std::string x_pass_check(int n, const XOps& ops) {
     default: return "unsupported transform length " + std::to_string(n);
   }
 }
+#if FFS_IN_PART(0)
 template std::string x_pass_check<float>(int, const XOps&);
 template std::string x_pass_check<double>(int, const XOps&);
+#endif
 
 template <typename T, int N>
 static void x_plan_options_n(int mx, const XOps& ops, int* pair, int* waves, int* split) {
@@ -1576,8 +1593,10 @@ void x_plan_options(int n, int mx, const XOps& ops, int* pair, int* waves, int* 
     default: *pair = *waves = *split = 0;
   }
 }
+#if FFS_IN_PART(0)
 template void x_plan_options<float>(int, int, const XOps&, int*, int*, int*);
 template void x_plan_options<double>(int, int, const XOps&, int*, int*, int*);
+#endif
 
 template <typename T, int N>
 static void run_x_nl(const Geo& g, const void* a2, void* b2, const XOps& ops, unsigned long long* umax,
@@ -2457,7 +2476,11 @@ __global__ void __launch_bounds__(256) k_piomelli_cstar(T* __restrict__ c, T* __
   c[i] = fma(r, ci - pi, ci);
 }
 
+#if FFS_IN_PART(0)
 __global__ void k_store_dt(const DevState* __restrict__ st, double* __restrict__ dst) { *dst = st->dt; }
+#else
+__global__ void k_store_dt(const DevState* __restrict__ st, double* __restrict__ dst);
+#endif
 
 template <typename T>
 void launch_piomelli_cstar(T* c, T* cm1, const DevState* st, double* dtm1, size_t n_padded, hipStream_t s) {
@@ -2830,6 +2853,9 @@ __global__ void __launch_bounds__(kStatsThreads) k_stats(Geo g, const C<T>* __re
 }
 
 // sums nblk rows of `partials` (each kStatsSlots wide) column-wise in a fixed order
+#if !FFS_IN_PART(0)
+__global__ void k_reduce_partials(const double* __restrict__ partials, int nblk, int ncols, double* __restrict__ out);
+#else
 __global__ void k_reduce_partials(const double* __restrict__ partials, int nblk, int ncols, double* __restrict__ out) {
   __shared__ double red[256];
   for (int col = 0; col < ncols; ++col) {
@@ -2845,6 +2871,7 @@ __global__ void k_reduce_partials(const double* __restrict__ partials, int nblk,
     __syncthreads();
   }
 }
+#endif
 
 template <typename T>
 int launch_stats(const Geo& g, const void* u, const void* nl, const void* ke, const StatsAux& aux, double* partials,
@@ -3163,6 +3190,9 @@ __global__ void __launch_bounds__(256) k_spectra_planes(Geo g, const C<T>* __res
 }
 
 // out[q] = X[mx] | Y0[my] | Y[my] | Z0[mz] | Z[mz], the plane partials summed over cz in order
+#if !FFS_IN_PART(0)
+__global__ void k_spectra_finish(Geo g, const double* __restrict__ part, double* __restrict__ out);
+#else
 __global__ void k_spectra_finish(Geo g, const double* __restrict__ part, double* __restrict__ out) {
   const size_t per = (size_t)g.mz;
   const double* px = part;
@@ -3190,6 +3220,7 @@ __global__ void k_spectra_finish(Geo g, const double* __restrict__ part, double*
   }
   out[t] = acc;
 }
+#endif
 
 template <typename T>
 void launch_spectra(const Geo& g, const void* u, const SpecArgs& a, double* partials, double* out, hipStream_t s) {
@@ -3309,9 +3340,11 @@ __device__ void set_dt_body(DevState* st, int it, int tit, double cfl, double dx
   etd_scalars(st);
 }
 
+#if FFS_IN_PART(0)
 __global__ void k_set_dt(DevState* st, int it, int tit, double cfl, double dx, int etd, double gnorm) {
   set_dt_body(st, it, tit, cfl, dx, etd, gnorm);
 }
+#endif
 
 // kz = 0-plane forcing, one block (Forcing_methods/forcing.jl, cforcing.jl, nrforcing.jl,
 // cnrforcing.jl, aforcing.jl, qfforcing.jl).  Thread s < nslot sums the spectra of its slot over its
@@ -3541,6 +3574,7 @@ void launch_step_scalars(const Geo& g, const void* u, DevState* st, const Scalar
               [&] { k_step_scalars<T><<<1, 64, 0, s>>>(c.g, c.u, c.st, c.a); });
 }
 
+#if FFS_IN_PART(0)
 void launch_set_dt(DevState* st, int it, int tit, double cfl, double dx, int etd, double gnorm, hipStream_t s) {
   k_set_dt<<<1, 1, 0, s>>>(st, it, tit, cfl, dx, etd, gnorm);
 }
@@ -3552,6 +3586,7 @@ void host_step_coefficients(DevState* st) {
 
 __global__ void k_end_step(DevState* st) { st->time += st->dt; }
 void launch_end_step(DevState* st, hipStream_t s) { k_end_step<<<1, 1, 0, s>>>(st); }
+#endif
 
 template <typename S, typename D>
 __global__ void k_cast_copy(const S* __restrict__ a, D* __restrict__ b, size_t n) {
@@ -3685,22 +3720,27 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   k_cast_copy<double, T><<<(n + 255) / 256, 256, 0, s>>>((const double*)src, (T*)dst, n);
 }
 
-// ---- explicit instantiations ----
-#define FFS_INST(T)                                                                                                   \
-  template void launch_inv_z<T>(const Geo&, const void*, void*, int, bool, const void*, hipStream_t, const void*,      \
-                                const void*, int, const void*, const void*);                                       \
-  template void launch_inv_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t);                       \
+// ---- explicit instantiations, by part ----
+#define FFS_INST_INV_Z(T)                                                                                         \
+  template void launch_inv_z<T>(const Geo&, const void*, void*, int, bool, const void*, hipStream_t, const void*,  \
+                                const void*, int, const void*, const void*);
+#define FFS_INST_COLS(T)                                                                                            \
+  template void launch_inv_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t);                   \
+  template void launch_fwd_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t);                   \
+  template void launch_fwd_z<T>(const Geo&, const void*, void* const*, int, int, double, const void*, hipStream_t);
+#define FFS_INST_X(T)                                                                                               \
   template void launch_x_nl<T>(const Geo&, const void*, void*, const XOps&, unsigned long long*, unsigned long long*, \
-                               const void*, hipStream_t);                                                           \
+                               const void*, hipStream_t);
+#define FFS_INST_XIO(T)                                                                                             \
   template void launch_x_c2r<T>(const Geo&, const void*, T*, int, const void*, hipStream_t);                          \
   template void launch_x_c2r_rows<T>(const Geo&, const void*, double*, int, int, const void*, hipStream_t);         \
-  template void launch_x_r2c<T>(const Geo&, const T*, void*, int, const void*, hipStream_t);                          \
-  template void launch_fwd_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t);                       \
-  template void launch_fwd_z<T>(const Geo&, const void*, void* const*, int, int, double, const void*, hipStream_t);   \
+  template void launch_x_r2c<T>(const Geo&, const T*, void*, int, const void*, hipStream_t);
+#define FFS_INST_FZU(T)                                                                                             \
+  template void launch_fwd_z_update<T>(const Geo&, const void*, void*, void*, const void*, const void*,           \
+                                       const DevState*, const UpdateArgs&, double, const void*, hipStream_t);
+#define FFS_INST_REST(T)                                                                                              \
   template void launch_update<T>(const Geo&, void*, void*, const void*, const void*, const DevState*,                 \
                                  const UpdateArgs&, hipStream_t);                                                     \
-  template void launch_fwd_z_update<T>(const Geo&, const void*, void*, void*, const void*, const void*,              \
-                                       const DevState*, const UpdateArgs&, double, const void*, hipStream_t);       \
   template void launch_project<T>(const Geo&, void*, hipStream_t);                                                   \
   template void launch_hermitian_planes<T>(const Geo&, void*, void*, void*, hipStream_t);                           \
   template void launch_div_uu<T>(const Geo&, const void*, void*, hipStream_t);                                     \
@@ -3722,7 +3762,7 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   template void launch_spectrum<T>(const Geo&, const void*, double*, double*, double*, double*, double*, double*,     \
                                    hipStream_t);                                                                      \
   template void launch_spectra<T>(const Geo&, const void*, const SpecArgs&, double*, double*, hipStream_t);       \
-  template void launch_forcing_sum<T>(const Geo&, const void*, const int*, int, DevState*, double, hipStream_t);               \
+  template void launch_forcing_sum<T>(const Geo&, const void*, const int*, int, DevState*, double, hipStream_t);  \
   template void launch_cast_copy<T>(const void*, void*, size_t, hipStream_t);                                       \
   template void launch_step_scalars<T>(const Geo&, const void*, DevState*, const ScalarArgs&, hipStream_t);          \
   template void launch_plane_forcing<T>(const Geo&, const void*, const void*, DevState*, const PlaneForcing&,          \
@@ -3730,7 +3770,33 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   template void launch_seed<T>(const Geo&, void*, int, unsigned long long, double, double, double*, double*,           \
                                hipStream_t);
 
-FFS_INST(double)
-FFS_INST(float)
+#if FFS_IN_PART(0)
+FFS_INST_REST(double)
+FFS_INST_REST(float)
+FFS_INST_XIO(double)
+FFS_INST_XIO(float)
+#endif
+#if FFS_IN_PART(1)
+FFS_INST_INV_Z(double)
+#endif
+#if FFS_IN_PART(2)
+FFS_INST_INV_Z(float)
+#endif
+#if FFS_IN_PART(3)
+FFS_INST_COLS(double)
+#endif
+#if FFS_IN_PART(4)
+FFS_INST_COLS(float)
+#endif
+#if FFS_IN_PART(5)
+FFS_INST_X(double)
+#endif
+#if FFS_IN_PART(6)
+FFS_INST_X(float)
+#endif
+#if FFS_IN_PART(7)
+FFS_INST_FZU(double)
+FFS_INST_FZU(float)
+#endif
 
 }  // namespace fsk

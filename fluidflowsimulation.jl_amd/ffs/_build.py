@@ -15,7 +15,12 @@ REPO = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(HERE, "libffs_hip.so")
-SOURCES = ["ffs_kernels.hip", "ffs_sim.hip"]
+# (source, defines, object): ffs_kernels.hip compiles in KERNEL_PARTS parts that run in parallel, each with its
+# own share of the template instantiations (FFS_PART in that file)
+KERNEL_PARTS = 8
+UNITS = [("ffs_kernels.hip", [f"-DFFS_PART={p}"], f"ffs_kernels_p{p}.o") for p in range(KERNEL_PARTS)] + [
+    ("ffs_sim.hip", [], "ffs_sim.o")
+]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FFS_OFFLOAD_ARCH", "gfx950")
@@ -47,11 +52,11 @@ def build(verbose=False):
     os.makedirs(bdir, exist_ok=True)
     objs = []
     cmds = []
-    for s in SOURCES:
-        o = os.path.join(bdir, os.path.splitext(s)[0] + ".o")
+    for src, defs, obj in UNITS:
+        o = os.path.join(bdir, obj)
         objs.append(o)
-        cmds.append([HIPCC, *CFLAGS, "-c", os.path.join(CSRC, s), "-o", o])
-    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        cmds.append([HIPCC, *CFLAGS, *defs, "-c", os.path.join(CSRC, src), "-o", o])
+    with ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1)) as ex:
         for out in ex.map(_run, cmds):
             if verbose and out.strip():
                 print(out, file=sys.stderr)
