@@ -52,6 +52,19 @@ __host__ __device__ __forceinline__ double kz_of(const Geo& g, int cz) {
   return (double)(j < (g.nz + 1) / 2 ? j : j - g.nz) / g.lz;
 }
 
+// The x pass's maxima (max|u_i| for the CFL, max|rho|) are gathered per block and combined with one
+// atomicMax per block on the bit pattern of a non-negative double.  The atomics go to kMaxSlots slots, one
+// 128-byte line apart (block b uses slot b mod kMaxSlots), so thousands of blocks do not queue on one
+// address; set_dt takes the maximum over the slots and clears them.
+constexpr int kMaxSlots = 32;
+constexpr int kMaxSlotStride = 16;  // unsigned long longs between slots
+struct alignas(128) MaxSlots {
+  unsigned long long v[kMaxSlots * kMaxSlotStride];
+};
+__device__ __forceinline__ unsigned long long* max_slot(unsigned long long* slots, int block) {
+  return slots + (block & (kMaxSlots - 1)) * kMaxSlotStride;
+}
+
 // Device-resident scalar state of a run (so variable-dt runs never stall the
 // host): mirrors the Refs of Adams_Bashforth3rdO / ETD3rdO and Simulation.
 struct DevState {
@@ -59,8 +72,8 @@ struct DevState {
   double At, Bt, Ct;       // AB3 coefficients for the current step
   double time;             // s.time[]
   double umax_d;           // last max|u_i| (decoded)
-  unsigned long long umax; // bit pattern of max |u_i| (atomicMax on non-negative doubles)
-  unsigned long long rhomax;
+  MaxSlots umax;           // bit patterns of max|u_i| per slot
+  MaxSlots rhomax;         // ... of max|rho|
   double force_factor;     // isotropic forcing: P / sum |u_lowk|^2
   double rhomax_d;         // last max|rho|
   double etd[20];          // ETD3 scalars a0..a3 b0..b2 c0..c2 e1..e9 of set_ABCt! (ETD.jl:142-227)

@@ -140,7 +140,7 @@ __device__ __forceinline__ const C<T>* stage_twiddles(C<T>* __restrict__ twl, co
 }
 
 // wave64 + cross-wave max, then one atomic per block on the IEEE bit pattern
-// (monotone for non-negative values; NaN sorts above +inf so it propagates).
+// (monotone for non-negative values; NaN sorts above +inf so it propagates), into the block's slot.
 __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) {
   __shared__ double red[16];
   for (int off = 32; off > 0; off >>= 1) v = nanmax(v, __shfl_xor(v, off, 64));
@@ -151,7 +151,7 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) 
     double m = red[0];
     const int nw = (blockDim.x + 63) >> 6;
     for (int i = 1; i < nw; ++i) m = nanmax(m, red[i]);
-    atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+    atomicMax(max_slot(dst, blockIdx.x), (unsigned long long)__double_as_longlong(m));
   }
   // `red` is one array per kernel: a following call must not overwrite it before thread 0 has read it
   __syncthreads();
@@ -3308,18 +3308,30 @@ __host__ __device__ inline void etd_scalars(DevState* st) {
 }
 
 __device__ void set_dt_body(DevState* st, int it, int tit, double cfl, double dx, int etd, double gnorm) {
-  const double umax = __longlong_as_double((long long)st->umax);
+  unsigned long long ub = 0ull;
+#pragma unroll
+  for (int k = 0; k < kMaxSlots; ++k) {
+    unsigned long long* slot = max_slot(st->umax.v, k);
+    ub = *slot > ub ? *slot : ub;
+    *slot = 0ull;  // cleared for the next step's atomicMax
+  }
+  const double umax = __longlong_as_double((long long)ub);
   st->umax_d = umax;
-  st->umax = 0ull;  // cleared for the next step's atomicMax
   if (it == 0) cfl *= 0.1;
   else if (it == 1) cfl *= 0.2;
   else if (it == 2) cfl *= 0.4;
   else if (it == 3) cfl *= 0.8;
   double newdt = cfl * dx / umax;
   if (gnorm > 0) {  // density: newdt = min(newdt, cfl sqrt(dx / (rho_max g)))
-    const double rmax = __longlong_as_double((long long)st->rhomax);
+    unsigned long long rb = 0ull;
+#pragma unroll
+    for (int k = 0; k < kMaxSlots; ++k) {
+      unsigned long long* slot = max_slot(st->rhomax.v, k);
+      rb = *slot > rb ? *slot : rb;
+      *slot = 0ull;
+    }
+    const double rmax = __longlong_as_double((long long)rb);
     st->rhomax_d = rmax;
-    st->rhomax = 0ull;
     newdt = fmin(newdt, cfl * sqrt(dx / (rmax * gnorm)));
   }
   if (it > 3) {

@@ -1085,8 +1085,8 @@ struct ffs_sim {
     ops.pair = x_pair;
     ops.waves = x_waves;
     ops.split = x_split;
-    unsigned long long* um = p.variable_dt ? &st->umax : nullptr;
-    unsigned long long* rm = (p.variable_dt && dens) ? &st->rhomax : nullptr;
+    unsigned long long* um = p.variable_dt ? st->umax.v : nullptr;
+    unsigned long long* rm = (p.variable_dt && dens) ? st->rhomax.v : nullptr;
     launch("x_nl", [&] { FFS_DISPATCH(launch_x_nl, g, w2, w1, ops, um, rm, tw_x, stream); });
     // the state's round trip through real space (its inverse transforms above are done)
     if (nyquist_kept())
@@ -1602,7 +1602,7 @@ struct ffs_sim {
   CkptHeader ckpt_header() const {
     CkptHeader h{};
     std::memcpy(h.magic, "FFSCKPT1", 8);
-    h.version = 1;
+    h.version = 2;  // 2: the state carries the per-slot maxima
     h.precision = fp32 ? 1 : 0;
     h.nx = p.nx;
     h.ny = p.ny;
