@@ -253,7 +253,13 @@ def bench(args):
         # those steps only when the plain steps are fused)
         ke_frac = min(1.0, n_ke / cnt) if name == "fwd_z" else n_ke / args.steps
         kb = kernel_bytes(g, name, 2 * b, ke_frac)
+        # with z slabs the inverse y, x and forward y passes launch once per slab, each on its share of planes
+        slabs = max(1, round(cnt / args.steps)) if name in ("inv_y", "x_nl", "fwd_y") else 1
+        if kb:
+            kb /= slabs
         kernels[name] = {"launches": cnt, "avg_us": round(avg_us, 3), "share": round(ms / (1e3 * el_prof), 4)}
+        if slabs > 1:
+            kernels[name]["launches_per_step"] = slabs
         if kb:
             kernels[name]["alg_bytes"] = int(kb)
             kernels[name]["GBs"] = round(kb / (avg_us * 1e-6) / 1e9, 1)

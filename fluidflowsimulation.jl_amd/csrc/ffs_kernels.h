@@ -161,7 +161,9 @@ void launch_inv_z(const Geo& g, const void* u, void* a1, int nf, bool curl, cons
                   const void* rho = nullptr, const void* phi = nullptr, int nsform = 0,
                   const void* grad1 = nullptr, const void* grad2 = nullptr);  // i k s after S, per scalar
 template <typename T>
-void launch_inv_y(const Geo& g, const void* a1, void* a2, int nf, const void* tw, hipStream_t s);
+// z0, nzc: only the planes [z0, z0 + nzc) (default: all), written as a slab of nzc planes
+void launch_inv_y(const Geo& g, const void* a1, void* a2, int nf, const void* tw, hipStream_t s, int z0 = 0,
+                  int nzc = -1);
 // field set of the fused x pass
 // pointwise LES models of calc_les! (les_real_calc.jl:74-114); codes as FFS_LES_* of the C-ABI
 enum : int { LES_SMAGORINSKY = 1, LES_VREMAN = 2, LES_SILVIS = 3, LES_PRODUCTION = 4, LES_STABLE_NL = 5,
@@ -215,7 +217,9 @@ void launch_x_c2r_rows(const Geo& g, const void* a2, double* real, int row0, int
 template <typename T>
 void launch_x_r2c(const Geo& g, const T* real, void* b2, int nf, const void* tw, hipStream_t s);
 template <typename T>
-void launch_fwd_y(const Geo& g, const void* b2, void* b1, int nf, const void* tw, hipStream_t s);
+// z0, nzc: the input is a slab of nzc planes, written to the planes [z0, z0 + nzc) (default: all)
+void launch_fwd_y(const Geo& g, const void* b2, void* b1, int nf, const void* tw, hipStream_t s, int z0 = 0,
+                  int nzc = -1);
 // forward z pass: field f goes to dst[f] (compact spectral), scaled; the k=0 mode
 // of the first nzero_origin fields is zeroed
 template <typename T>

@@ -279,11 +279,13 @@ template <typename T> struct ColCall {
   const C<T>* tw;
   int nzero;
   T scale;
+  int z0, nzc;  // y passes: the z planes [z0, z0 + nzc), their (sx, ny) side in a buffer of nzc planes
 };
 
 template <typename T, int N, int KIND, bool CURL, bool GRAD>
 __device__ __forceinline__ void col_body(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ out, const ColAux<T>& aux,
-                                         const C<T>* __restrict__ tw_g, int nzero, T scale, int bX, int bY, int bZ) {
+                                         const C<T>* __restrict__ tw_g, int nzero, T scale, int bX, int bY, int bZ,
+                                         int z0, int nzc) {
   g = glb(g);
   constexpr int TX = col_tx<T>(N);
   constexpr int TP = col_tper<T, N, CURL>();
@@ -400,30 +402,34 @@ __device__ __forceinline__ void col_body(Geo g, const C<T>* __restrict__ in, C<T
       Fft<T, N, +1, TP, false, false, SYNC_BLOCK, MAXR>::run(q, tw, ld, st, ldsf);
     }
   } else if constexpr (KIND == INV_Y) {
-    const int z = b, f = bZ;
+    // plane z0 + z of the (sx, my, nz) input into plane z of an output of nzc planes (a z slab)
+    const int z = b, zg = z0 + b, f = bZ;
+    const size_t S2c = sx * g.ny * nzc;
     auto ld = [&](int p, int, int) -> C<T> {
       const int cy = y_compact(g, p);
 #if FFS_COL_BRANCHFREE
-      const C<T> v = in[f * S1 + cx + sx * ((cy < 0 ? 0 : cy) + (size_t)g.my * z)];
+      const C<T> v = in[f * S1 + cx + sx * ((cy < 0 ? 0 : cy) + (size_t)g.my * zg)];
       return (valid && cy >= 0) ? v : mk<T>(0, 0);
 #else
       if (!valid || cy < 0) return mk<T>(0, 0);
-      return in[f * S1 + cx + sx * (cy + (size_t)g.my * z)];
+      return in[f * S1 + cx + sx * (cy + (size_t)g.my * zg)];
 #endif
     };
     auto st = [&](int p, int, int, C<T> v) {
-      if (valid) out[f * S2 + cx + sx * (p + (size_t)g.ny * z)] = v;
+      if (valid) out[f * S2c + cx + sx * (p + (size_t)g.ny * z)] = v;
     };
     Fft<T, N, +1, TP, false, false, SYNC_BLOCK, MAXR>::run(q, tw, ld, st, ldsf);
   } else if constexpr (KIND == FWD_Y) {
-    const int z = b, f = bZ;
+    // plane z of an input of nzc planes (a z slab) into plane z0 + z of the (sx, my, nz) output
+    const int z = b, zg = z0 + b, f = bZ;
+    const size_t S2c = sx * g.ny * nzc;
     auto ld = [&](int p, int, int) -> C<T> {
       if (!valid) return mk<T>(0, 0);
-      return in[f * S2 + cx + sx * (p + (size_t)g.ny * z)];
+      return in[f * S2c + cx + sx * (p + (size_t)g.ny * z)];
     };
     auto st = [&](int p, int, int, C<T> v) {
       const int cy = y_compact(g, p);
-      if (valid && cy >= 0) out[f * S1 + cx + sx * (cy + (size_t)g.my * z)] = v;
+      if (valid && cy >= 0) out[f * S1 + cx + sx * (cy + (size_t)g.my * zg)] = v;
     };
     Fft<T, N, -1, TP, false, false, SYNC_BLOCK, MAXR>::run(q, tw, ld, st, ldsf);
   } else {  // FWD_Z
@@ -447,20 +453,21 @@ __device__ __forceinline__ void col_body(Geo g, const C<T>* __restrict__ in, C<T
 
 template <typename T, int N, int KIND, bool CURL, bool GRAD = false>
 __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col(Geo g, const C<T>* __restrict__ in, C<T>* __restrict__ out,
-                                            ColAux<T> aux, const C<T>* __restrict__ tw_g, int nzero, T scale) {
-  col_body<T, N, KIND, CURL, GRAD>(g, in, out, aux, tw_g, nzero, scale, blockIdx.x, blockIdx.y, blockIdx.z);
+                                            ColAux<T> aux, const C<T>* __restrict__ tw_g, int nzero, T scale,
+                                            int z0, int nzc) {
+  col_body<T, N, KIND, CURL, GRAD>(g, in, out, aux, tw_g, nzero, scale, blockIdx.x, blockIdx.y, blockIdx.z, z0, nzc);
 }
 template <typename T, int N, int KIND, bool CURL, bool GRAD = false>
 __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col_b(FFS_CALLS(ColCall<T>) calls, int nrun) {
   const int nz = gridDim.z / nrun;
   const ColCall<T>& c = calls[blockIdx.z / nz];  // (aux.dst is indexed by the field: read in place)
   col_body<T, N, KIND, CURL, GRAD>(c.g, glb(c.in), glb(c.out), c.aux, glb(c.tw), c.nzero, c.scale, blockIdx.x,
-                                   blockIdx.y, blockIdx.z % nz);
+                                   blockIdx.y, blockIdx.z % nz, c.z0, c.nzc);
 }
 
 template <typename T, int N, int KIND, bool CURL, bool GRAD = false>
 static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, const void* tw, int nzero, double scale,
-                    hipStream_t s, const ColAux<T>& aux = ColAux<T>{}) {
+                    hipStream_t s, const ColAux<T>& aux = ColAux<T>{}, int z0 = 0) {
   constexpr int TX = col_tx<T>(N);
   constexpr int TP = col_tper<T, N, CURL>();
   static_assert(TX * TP <= kColThreads, "column block exceeds its launch bound");
@@ -472,9 +479,11 @@ static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, con
   const int ntile = (g.mx + TX - 1) / TX * nb;
   const dim3 grid = (CURL && FFS_CURL_SPLIT) ? dim3((unsigned)(nzero * ((ntile + kXcd - 1) / kXcd * kXcd)))
                                              : dim3((g.mx + TX - 1) / TX, nb, CURL ? 1 : nf);
-  const ColCall<T> c{g, (const C<T>*)in, (C<T>*)out, aux, (const C<T>*)tw, nzero, (T)scale};
+  // the y passes' nb planes from z0 (a slab of nb planes, or all of them)
+  const ColCall<T> c{g, (const C<T>*)in, (C<T>*)out, aux, (const C<T>*)tw, nzero, (T)scale, z0, nb};
   launch_call((const void*)&k_col_b<T, N, KIND, CURL, GRAD>, grid, dim3(TX * TP), lds, c, "column pass", [&] {
-    k_col<T, N, KIND, CURL, GRAD><<<grid, TX * TP, lds, s>>>(c.g, c.in, c.out, c.aux, c.tw, c.nzero, c.scale);
+    k_col<T, N, KIND, CURL, GRAD><<<grid, TX * TP, lds, s>>>(c.g, c.in, c.out, c.aux, c.tw, c.nzero, c.scale, c.z0,
+                                                               c.nzc);
   });
 }
 
@@ -509,10 +518,12 @@ void launch_inv_z(const Geo& g, const void* u, void* a1, int nf, bool curl, cons
 }
 
 template <typename T>
-void launch_inv_y(const Geo& g, const void* a1, void* a2, int nf, const void* tw, hipStream_t s) {
+void launch_inv_y(const Geo& g, const void* a1, void* a2, int nf, const void* tw, hipStream_t s, int z0, int nzc) {
+  if (nzc < 0) nzc = g.nz - z0;
+  if (z0 < 0 || nzc <= 0 || z0 + nzc > g.nz) throw std::runtime_error("inverse y pass: z slab out of range");
   switch (g.ny) {
 #define X(n)                                                            \
-  case n: run_col<T, n, INV_Y, false>(g, a1, a2, nf, g.nz, tw, 0, 1.0, s); break;
+  case n: run_col<T, n, INV_Y, false>(g, a1, a2, nf, nzc, tw, 0, 1.0, s, ColAux<T>{}, z0); break;
     FFS_SIZES(X)
 #undef X
     default: bad_size("inverse y pass", g.ny);
@@ -520,10 +531,12 @@ void launch_inv_y(const Geo& g, const void* a1, void* a2, int nf, const void* tw
 }
 
 template <typename T>
-void launch_fwd_y(const Geo& g, const void* b2, void* b1, int nf, const void* tw, hipStream_t s) {
+void launch_fwd_y(const Geo& g, const void* b2, void* b1, int nf, const void* tw, hipStream_t s, int z0, int nzc) {
+  if (nzc < 0) nzc = g.nz - z0;
+  if (z0 < 0 || nzc <= 0 || z0 + nzc > g.nz) throw std::runtime_error("forward y pass: z slab out of range");
   switch (g.ny) {
 #define X(n)                                                            \
-  case n: run_col<T, n, FWD_Y, false>(g, b2, b1, nf, g.nz, tw, 0, 1.0, s); break;
+  case n: run_col<T, n, FWD_Y, false>(g, b2, b1, nf, nzc, tw, 0, 1.0, s, ColAux<T>{}, z0); break;
     FFS_SIZES(X)
 #undef X
     default: bad_size("forward y pass", g.ny);
@@ -3737,8 +3750,8 @@ void launch_cast_copy(const void* src, void* dst, size_t n, hipStream_t s) {
   template void launch_inv_z<T>(const Geo&, const void*, void*, int, bool, const void*, hipStream_t, const void*,  \
                                 const void*, int, const void*, const void*);
 #define FFS_INST_COLS(T)                                                                                            \
-  template void launch_inv_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t);                   \
-  template void launch_fwd_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t);                   \
+  template void launch_inv_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t, int, int);         \
+  template void launch_fwd_y<T>(const Geo&, const void*, void*, int, const void*, hipStream_t, int, int);         \
   template void launch_fwd_z<T>(const Geo&, const void*, void* const*, int, int, double, const void*, hipStream_t);
 #define FFS_INST_X(T)                                                                                               \
   template void launch_x_nl<T>(const Geo&, const void*, void*, const XOps&, unsigned long long*, unsigned long long*, \

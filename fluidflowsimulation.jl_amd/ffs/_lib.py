@@ -121,7 +121,7 @@ class FfsPlan(ctypes.Structure):
     """ffs_plan of include/ffs/ffs.h."""
 
     _fields_ = [("x_pair", ctypes.c_int), ("x_waves", ctypes.c_int), ("x_split", ctypes.c_int),
-                ("fusion", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
+                ("fusion", ctypes.c_int), ("z_slab", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/ffs/ffs.h

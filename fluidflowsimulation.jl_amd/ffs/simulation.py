@@ -203,7 +203,7 @@ class Simulation:
         for FP32; bitwise identical to the two kernels)."""
         self._check(self._lib.ffs_set_fusion(self._h, int(on)))
 
-    PLAN_FIELDS = ("x_pair", "x_waves", "x_split", "fusion")
+    PLAN_FIELDS = ("x_pair", "x_waves", "x_split", "fusion", "z_slab")
 
     @property
     def plan(self):
@@ -213,7 +213,7 @@ class Simulation:
         return {k: getattr(p, k) for k in self.PLAN_FIELDS}
 
     def set_plan(self, plan):
-        """Applies a plan (a dict with any of x_pair, x_waves, x_split, fusion; the rest kept)."""
+        """Applies a plan (a dict with any of x_pair, x_waves, x_split, fusion, z_slab; the rest kept)."""
         cur = self.plan
         unknown = set(plan) - set(cur)
         if unknown:
@@ -223,10 +223,11 @@ class Simulation:
         self._check(self._lib.ffs_set_plan(self._h, ctypes.byref(p)))
 
     def plan_options(self):
-        """Which plan fields change this run's kernels: dict of field -> bool."""
+        """Which plan fields change this run's kernels: dict of field -> bool; z_slab -> the run's default
+        planes per z slab (0: slabs do not apply)."""
         p = _lib.FfsPlan()
         self._check(self._lib.ffs_plan_options(self._h, ctypes.byref(p)))
-        return {k: bool(getattr(p, k)) for k in self.PLAN_FIELDS}
+        return {k: (int(getattr(p, k)) if k == "z_slab" else bool(getattr(p, k))) for k in self.PLAN_FIELDS}
 
     # ---------------------------------------------------------------- diagnostics
     def pop_stats(self):

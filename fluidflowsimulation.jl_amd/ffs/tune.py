@@ -4,7 +4,8 @@ A plan (ffs_plan, include/ffs/ffs.h) picks among kernel variants that compute
 the same step: the mirror-paired or the plain register x pass for FP64
 power-of-two rows, one or two wavefronts per x-pass block, the density /
 scalar / LES products as register launches of their own or in one general
-LDS pass, and the forward z pass fused with the update or not.  Which is
+LDS pass, the forward z pass fused with the update or not, and the inverse
+y, x and forward y passes run slab by slab along z or whole.  Which is
 fastest depends on the grid, the precision, the model and the device, so the
 tuner times every candidate the run admits on a twin of the run (same
 parameters, seeded field, no output) and keeps the fastest.  Results are
@@ -29,7 +30,7 @@ from .config import Params
 __all__ = ["plan_key", "candidates", "tune", "tuned_plan", "cache_path", "device_name", "library_id"]
 
 CACHE_ENV = "FFS_PLAN_CACHE"
-FIELDS = ("x_pair", "x_waves", "x_split", "fusion")
+FIELDS = ("x_pair", "x_waves", "x_split", "fusion", "z_slab")
 _LIB_ID = None
 
 
@@ -69,15 +70,24 @@ def plan_key(params: Params):
 
 def candidates(options, base):
     """The plans worth timing: every combination of the fields that change the run's kernels
-    (options: field -> bool, as Simulation.plan_options), the other fields as in `base`."""
+    (options: field -> bool, as Simulation.plan_options; z_slab -> the run's default slab planes, 0 when
+    slabs do not apply), the other fields as in `base`."""
     out = []
+    zs = int(options.get("z_slab", 0) or 0)
+    slabs = [zs, 2 * zs, 0] if zs > 0 else None  # the default slab, twice as thick, and none
     for split in ([1, 0] if options["x_split"] else [base["x_split"]]):
         reg = split == 1 or not options["x_split"]
         pairs = [1, 0] if (options["x_pair"] and reg) else [base["x_pair"]]
         waves = [2, 1] if (options["x_waves"] and reg) else [base["x_waves"]]
         fusion = [1, 0] if options["fusion"] else [base["fusion"]]
         for pr, w, f in itertools.product(pairs, waves, fusion):
-            out.append({"x_pair": pr, "x_waves": w, "x_split": split, "fusion": f})
+            c = {"x_pair": pr, "x_waves": w, "x_split": split, "fusion": f}
+            if slabs is None:
+                if "z_slab" in base:
+                    c["z_slab"] = base["z_slab"]
+                out.append(c)
+            else:
+                out.extend({**c, "z_slab": z} for z in slabs)
     return out
 
 

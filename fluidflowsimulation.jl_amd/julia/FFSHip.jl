@@ -429,8 +429,9 @@ mutable struct ffs_plan
     x_waves::Cint
     x_split::Cint
     fusion::Cint
-    reserved::NTuple{4,Cint}
-    ffs_plan() = new(0, 0, 0, 0, (0, 0, 0, 0))
+    z_slab::Cint
+    reserved::NTuple{3,Cint}
+    ffs_plan() = new(0, 0, 0, 0, 0, (0, 0, 0))
 end
 
 function get_plan(s::FFSSimulation)

@@ -393,7 +393,10 @@ typedef struct ffs_plan {
   int x_split; /* 1: density / scalar fluxes and the Smagorinsky stress in
                 * register x launches of their own; 0: one general x pass */
   int fusion;  /* as ffs_set_fusion */
-  int reserved[4];
+  int z_slab;  /* planes per z slab of the nonlinear term's inverse y, x and
+                * forward y passes (0: whole grid per launch); ffs_plan_options
+                * reports the default slab size of the run here (0: no slabs) */
+  int reserved[3];
 } ffs_plan;
 int ffs_get_plan(const ffs_sim* s, ffs_plan* out);
 /* The current device as "<name> <arch> <CUs>CU" (the tuner's cache key). */

@@ -105,3 +105,70 @@ def test_tuner_on_the_device(tmp_path, monkeypatch):
     data = json.load(open(path))
     entry = data[tune.device_name()][tune.library_id()][tune.plan_key(p)]
     assert entry["plan"] == best and len(entry["ms_per_step"]) == len(cands)
+
+
+# (name, (nx, ny, nz), precision, options): runs whose inverse y, x and forward y passes go slab by slab along z
+SLAB_CASES = [
+    ("plain fp64 32^3, fused update, stats steps", (32, 32, 32), "fp64", {}),
+    ("smagorinsky + boussinesq fp32 32^3", (32, 32, 32), "fp32", {"les": True, "dens": True}),
+    ("conservation form + passive scalar fp64 24x32x48", (24, 32, 48), "fp64", {"ns": True, "scalar": True}),
+    ("plain fp32 48x32x48, shell forcing", (48, 32, 48), "fp32", {"forcing": True}),
+]
+
+
+def slab_params(shape, prec, opts):
+    nx, ny, nz = shape
+    p = ffs.Params(nx=nx, ny=ny, nz=nz, nu=2e-3, integrator="Adams_Bashforth3rdO", variable_dt=True, cfl=0.3,
+                   dt_stat=4, dt_spec=1 << 30, write_time=1 << 30, precision=prec)
+    if opts.get("les"):
+        p.les_model, p.smagorinsky_c = "Smagorinsky", 0.16
+    if opts.get("dens"):
+        p.density, p.pr, p.density_gradient, p.z_acceleration, p.reference_density = True, 0.7, 1.0, -1.0, 1.0
+    if opts.get("ns"):
+        p.equation = "NS"
+    if opts.get("scalar"):
+        p.passive_scalar, p.scalar_pr, p.scalar_gradient = True, 1.0, 1.0
+    if opts.get("forcing"):
+        p.forcing, p.forcing_kf, p.forcing_power = "shellForcing", 2.5, 0.1
+    return p
+
+
+@pytest.mark.parametrize("name,shape,prec,opts", SLAB_CASES, ids=[c[0] for c in SLAB_CASES])
+def test_z_slabs_match_the_whole_grid_bitwise(name, shape, prec, opts):
+    """Slabs change which launch does a plane's work, not the work: fields and stats rows equal the
+    whole-grid run's bit for bit, for slabs that divide nz and for a thinner last slab, graph-replayed."""
+    p = slab_params(shape, prec, opts)
+    nz = shape[2]
+    results = {}
+    for zs in (0, 8, 5, nz - 1):
+        print_case(f"{name}: z_slab {zs}")
+        with ffs.Simulation(p) as s:
+            s.set_plan({"z_slab": zs})
+            assert s.plan["z_slab"] == zs
+            s.seed_velocity(31)
+            if opts.get("dens"):
+                s.seed_density(32)
+            s.initialize()
+            s.advance(13)
+            fields = [s.get_field(c, ffs._lib.FFS_SPACE_FOURIER) for c in range(3)]
+            results[zs] = (fields, s.pop_stats())
+    f0, r0 = results[0]
+    assert len(r0) > 0
+    for zs, (f, r) in results.items():
+        for c in range(3):
+            assert np.array_equal(f[c], f0[c]), (zs, c)
+        assert len(r) == len(r0)
+        for a, b in zip(r, r0):
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y, equal_nan=True), zs
+
+
+def test_slab_defaults_and_plan_options():
+    """A 256^3 FP64 run defaults to slabs (its y-x-y buffers exceed the cache); a 32^3 run does not."""
+    big = make_params(256, "fp64", False, False, True)
+    with ffs.Simulation(big) as s:
+        opts, plan = s.plan_options(), s.plan
+        print_case(f"256^3 fp64: options {opts}, plan {plan}")
+        assert opts["z_slab"] > 0 and plan["z_slab"] == opts["z_slab"] and 256 % plan["z_slab"] == 0
+    with ffs.Simulation(make_params(32, "fp64", False, False, False)) as s:
+        assert s.plan_options()["z_slab"] == 0 and s.plan["z_slab"] == 0

@@ -27,6 +27,12 @@ def test_candidates_cover_the_fields_that_matter():
     c = tune.candidates(fp32_plain, {**BASE, "x_pair": 0, "fusion": 0})
     assert sorted((x["x_waves"], x["fusion"]) for x in c) == [(1, 0), (1, 1), (2, 0), (2, 1)]
     assert all(x["x_pair"] == 0 for x in c)
+    # z slabs: each combination with the run's default slab size and without slabs
+    c = tune.candidates({**fp32_plain, "z_slab": 32}, {**BASE, "x_pair": 0, "fusion": 0, "z_slab": 32})
+    assert sorted((x["x_waves"], x["fusion"], x["z_slab"]) for x in c) == sorted(
+        (w, f, z) for w in (1, 2) for f in (0, 1) for z in (0, 32, 64))
+    c = tune.candidates({**fp32_plain, "z_slab": 0}, {**BASE, "z_slab": 0})
+    assert len(c) == 4 and all(x["z_slab"] == 0 for x in c)
 
 
 def test_plan_key_separates_what_changes_the_kernels():
