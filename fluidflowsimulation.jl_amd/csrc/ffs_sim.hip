@@ -208,6 +208,8 @@ struct ffs_sim {
   int z_slab = 0, slab_cap = 0;
   void* sa2 = nullptr;  // inputs of the x pass for one slab: nin fields of (sx, ny, z_slab)
   void* sb2 = nullptr;  // its outputs: nfw + 1 fields of (sx, ny, z_slab)
+  bool slab_own = false;  // sa2/sb2 allocated for themselves (else they lie in w2, after the forward y output)
+  size_t w2_bytes = 0;
   // steady-state graphs: one per history-rotation phase
   bool graph_on = true;
   int phase = 0;  // history rotations mod 3
@@ -475,6 +477,7 @@ struct ffs_sim {
     const size_t a2 = nmax * g.sx * g.ny * g.nz, b1 = nmax * g.sx * g.my * g.nz;
     w1 = dalloc(std::max(a1, b2) * csz);
     w2 = dalloc(std::max(a2, b1) * csz);
+    w2_bytes = std::max(a2, b1) * csz;
     z_slab = default_slab();
     alloc_slabs(z_slab);
     // the spectra partials live in w1 (free between the nonlinear term and the next step) when it fits
@@ -1035,15 +1038,28 @@ struct ffs_sim {
       if (g.nz % k == 0 && 2 * k > zs) return k;  // the largest power of two that divides nz and fits
     return zs;
   }
-  // slab buffers for slabs of up to `planes` planes (0: none)
+  // slab buffers for slabs of up to `planes` planes (0: none).  With slabs, w2 holds only the forward y
+  // output (nfw + 1 fields of (sx, my, nz)) during the nonlinear term, so the slab buffers go after it when
+  // they fit there; otherwise they get allocations of their own.
   void alloc_slabs(int planes) {
     if (planes <= slab_cap) return;
-    dfree(sa2);
-    dfree(sb2);
+    if (slab_own) {
+      dfree(sa2);
+      dfree(sb2);
+    }
     sa2 = sb2 = nullptr;
+    slab_own = false;
     slab_cap = 0;
-    sa2 = dalloc((size_t)nin * g.sx * g.ny * planes * csz);
-    sb2 = dalloc((size_t)(nfw + 1) * g.sx * g.ny * planes * csz);
+    const size_t a = (size_t)nin * g.sx * g.ny * planes * csz, b = (size_t)(nfw + 1) * g.sx * g.ny * planes * csz;
+    const size_t b1 = ((size_t)(nfw + 1) * g.sx * g.my * g.nz * csz + 255) / 256 * 256;
+    if (b1 + a + b <= w2_bytes) {
+      sa2 = (char*)w2 + b1;
+      sb2 = (char*)w2 + b1 + a;
+    } else {
+      sa2 = dalloc(a);
+      sb2 = dalloc(b);
+      slab_own = true;
+    }
     slab_cap = planes;
   }
   int slab_planes() const { return (z_slab > 0 && z_slab < g.nz) ? z_slab : 0; }
