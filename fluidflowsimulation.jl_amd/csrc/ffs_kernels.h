@@ -202,6 +202,8 @@ template <typename T>
 void x_plan_options(int n, int mx, const XOps& ops, int* pair, int* waves, int* split);
 // whether n is a compiled transform length, and (empty: yes) whether the x pass of length n can run
 // with these options -- checked when a run is created, before anything is allocated
+// the compiled transform lengths, space-separated
+std::string compiled_lengths();
 bool length_supported(int n);
 template <typename T>
 std::string x_pass_check(int n, const XOps& ops);

@@ -37,8 +37,16 @@
 namespace fsk {
 
 // columns per tile of the y/z passes: one 128-byte line of x-consecutive values
-template <typename T> __host__ __device__ constexpr int col_tx(int) { return 128 / (2 * (int)sizeof(T)); }
 template <int N> __host__ __device__ constexpr int tper() { return N / default_ppt(N); }
+// the column kernels run TX x TP threads per block, at most kColThreads (their launch bound)
+constexpr int kColThreads = 512;
+// columns per tile: one 128-byte line of x-consecutive values, halved while a tile of columns of length n (at
+// 16 points per thread) would exceed the block (FP32 columns of 640 and 768)
+template <typename T> __host__ __device__ constexpr int col_tx(int n) {
+  int tx = 128 / (2 * (int)sizeof(T));
+  while (tx > 1 && tx * (n / default_ppt(n)) > kColThreads) tx /= 2;
+  return tx;
+}
 // The inverse-z pass that forms w = ik x u (and rho, S) on load issues two loads per point in its
 // first stage; 8 points per thread (radix <= 8) instead of 16 halves the registers those loads hold.
 // FFS_COL_PPT8: 0 = that variant only, 1 = every column pass, -1 = none.
@@ -61,8 +69,6 @@ constexpr int kXcd = 8;
 #define FFS_ZMAP_ARITH 1
 #endif
 template <int N> __host__ __device__ constexpr bool ppt8_ok() { return N % 8 == 0 && plan_even(N, N / 8, 8); }
-// the column kernels run TX x TP threads per block, at most kColThreads (their launch bound)
-constexpr int kColThreads = 512;
 template <typename T, int N, bool CURL> __host__ __device__ constexpr bool col_ppt8() {
   return ppt8_ok<N>() && ((FFS_COL_PPT8 == 0 && CURL) || FFS_COL_PPT8 == 1) &&
          col_tx<T>(N) * (N / 8) <= kColThreads;
@@ -488,7 +494,9 @@ static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, con
 }
 
 #ifndef FFS_SIZES
-#define FFS_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(512)
+#define FFS_SIZES(X)                                                                                      \
+  X(16) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) X(320) X(384) X(512) X(640) \
+  X(768)
 #endif
 
 [[noreturn]] static void bad_size(const char* what, int n) {
@@ -1567,6 +1575,13 @@ bool length_supported(int n) {
     return true;
     default: return false;
   }
+}
+std::string compiled_lengths() {
+  std::string s;
+#define X(m) s += (s.empty() ? "" : " ") + std::to_string(m);
+  FFS_SIZES(X)
+#undef X
+  return s;
 }
 #endif
 

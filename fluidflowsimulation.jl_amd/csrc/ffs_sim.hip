@@ -1075,7 +1075,7 @@ struct ffs_sim {
     for (int i = 0; i < 3; ++i)
       if (!length_supported(n[i]))
         throw FfsError(FFS_ERR_ARG, std::string(ax[i]) + " = " + std::to_string(n[i]) +
-                                        " is not a compiled transform length (16 24 32 48 64 96 128 192 256 512)");
+                                        " is not a compiled transform length (" + compiled_lengths() + ")");
     XOps ops{};
     ops.nin = nin;
     ops.nout = nfw + 1;  // a stats step adds 1/2|u|^2

@@ -100,8 +100,10 @@ def test_unsupported_sizes_fail_at_creation(lib):
 
     base = dict(nx=32, ny=32, nz=32, nu=0.01, dt=0.01, variable_dt=False)
     assert ffs.footprint(ffs.Params(**base)) > 0
-    for key in ("nx", "ny", "nz"):
-        p = ffs.Params(**{**base, key: 40})
+    # 2^a 3^b 5^c lengths are compiled from a list (16 ... 768); 7 and 1024 are not on it
+    for key, bad in (("nx", 56), ("ny", 1024), ("nz", 100)):
+        assert ffs.footprint(ffs.Params(**{**base, key: 80})) > 0
+        p = ffs.Params(**{**base, key: bad})
         with pytest.raises(RuntimeError) as e:
             ffs.footprint(p)
         assert "not a compiled transform length" in str(e.value), str(e.value)
