@@ -32,6 +32,27 @@ SHAPES += [(n, 16, 16, 1.0) for n in NEW_LENGTHS] + [(16, n, 16, 2.0) for n in N
 MODELS = ["plain", "les", "density"]
 
 
+DT = 1e-3
+
+
+def stable_nu(nx, ny, nz, lx, nu=2e-3):
+    """nu = 2e-3, or less where the longest axis's highest retained mode would pass explicit AB3's viscous
+    limit at dt = 1e-3 (nu k^2 dt < ~0.545): beyond it AB3 amplifies roundoff by orders of magnitude in three
+    steps, and FP32 runs (LES most, whose stress fills the high modes) then differ from the FP64 oracle by that
+    noise.  dt stays 1e-3, so the nonlinear term moves the field as much as on the short axes."""
+    kmax = max(2 * np.pi * (n // 3) / L for n, L in ((nx, lx), (ny, 1.0), (nz, 1.0)))
+    return min(nu, 0.25 / (DT * kmax**2))
+
+
+def les_dt(nx, ny, nz):
+    """The LES stress is explicit: its eddy viscosity (c Delta)^2 |S| with the Delta of a thin, anisotropic grid
+    passes the explicit limit on the high modes of an axis longer than 512, which then amplify roundoff in both
+    precisions alike (measured at y length 768, dt = 1e-3: FP64 1.1e-12, FP32 1.3e-4 against the oracle; at 512:
+    3.7e-14 and 6.6e-6).  The LES runs there take dt scaled by (512 / n)^2."""
+    n = max(nx, ny, nz)
+    return DT * min(1.0, (512.0 / n) ** 2)
+
+
 def x_pass_fits(nx, model):
     """The general x pass holds a row's field pairs in one block of at most 256 threads (nx / 16 per pair
     at 16 points per thread): with LES (6 pairs) a row of 768 does not fit, and creating that run fails."""
@@ -52,9 +73,11 @@ def test_long_transforms_match_oracle(shape, model, prec):
     u0 = random_spectrum(nx, ny, nz, seed=5, lx=lx)
     les_c = 0.16 if model == "les" else None
     dens = {"Pr": 0.7, "gradient": 1.0, "g": -1.0} if model == "density" else None
-    ref = OracleSim(g, 2e-3, dt=1e-3, variable_dt=False, dt_stat=1 << 30, les_c=les_c, density=dens)
+    nu = stable_nu(nx, ny, nz, lx)
+    dt = les_dt(nx, ny, nz) if model == "les" else DT
+    ref = OracleSim(g, nu, dt=dt, variable_dt=False, dt_stat=1 << 30, les_c=les_c, density=dens)
     ref.set_velocity(u0)
-    p = ffs.Params(nx=nx, ny=ny, nz=nz, x_domain=lx, nu=2e-3, integrator="Adams_Bashforth3rdO", variable_dt=False, dt=1e-3,
+    p = ffs.Params(nx=nx, ny=ny, nz=nz, x_domain=lx, nu=nu, integrator="Adams_Bashforth3rdO", variable_dt=False, dt=dt,
                    dt_stat=1 << 30, dt_spec=1 << 30, write_time=1 << 30, precision=prec)
     if les_c is not None:
         p.les_model, p.smagorinsky_c = "Smagorinsky", les_c
