@@ -496,7 +496,7 @@ static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, con
 #ifndef FFS_SIZES
 #define FFS_SIZES(X)                                                                                      \
   X(16) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) X(320) X(384) X(512) X(640) \
-  X(768)
+  X(768) X(1024)
 #endif
 
 [[noreturn]] static void bad_size(const char* what, int n) {

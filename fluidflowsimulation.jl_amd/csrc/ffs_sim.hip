@@ -802,8 +802,10 @@ struct ffs_sim {
       throw FfsError(FFS_ERR_ARG, "dtStat, dtSpec and writeTime must be positive");
     if (!p.variable_dt && !(p.dt > 0)) throw FfsError(FFS_ERR_ARG, "a fixed time step needs dt > 0");
     fp32 = (p.precision == FFS_FP32);
-    // the FP32 tile (16 columns x 32 threads) holds too many registers to fuse profitably
-    fuse_ok = !fp32;
+    // the FP32 tile (16 columns x 32 threads) holds too many registers to fuse profitably, and the fused tile
+    // of a z length that is not a power of two, or longer than 512, is not register-resident (private memory or
+    // spills): those runs default to the two kernels (ffs_set_fusion / the plan can still choose fusion)
+    fuse_ok = !fp32 && (p.nz & (p.nz - 1)) == 0 && p.nz <= 512;
     check_lengths();
     if (dry) {  // footprint: the same allocations, counted, plus the transient staging of set_field
       setup_grid();

@@ -100,8 +100,8 @@ def test_unsupported_sizes_fail_at_creation(lib):
 
     base = dict(nx=32, ny=32, nz=32, nu=0.01, dt=0.01, variable_dt=False)
     assert ffs.footprint(ffs.Params(**base)) > 0
-    # 2^a 3^b 5^c lengths are compiled from a list (16 ... 768); 7 and 1024 are not on it
-    for key, bad in (("nx", 56), ("ny", 1024), ("nz", 100)):
+    # 2^a 3^b 5^c lengths are compiled from a list (16 ... 1024); 56, 2048 and 100 are not on it
+    for key, bad in (("nx", 56), ("ny", 2048), ("nz", 100)):
         assert ffs.footprint(ffs.Params(**{**base, key: 80})) > 0
         p = ffs.Params(**{**base, key: bad})
         with pytest.raises(RuntimeError) as e:
