@@ -2845,8 +2845,10 @@ int ffs_device_name(char* buf, size_t n) {
   int dev = 0;
   hipDeviceProp_t prop{};
   if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return FFS_ERR_HIP;
-  const std::string name = std::string(prop.name) + " " + prop.gcnArchName + " " +
-                           std::to_string(prop.multiProcessorCount) + "CU";
+  // only what the device reports the same in every process: its marketing name comes from a lookup table
+  // that some environments lack (the name is then empty), so it is left out
+  const std::string name = std::string(prop.gcnArchName) + " " + std::to_string(prop.multiProcessorCount) + "CU " +
+                           std::to_string((prop.totalGlobalMem + (1ull << 29)) >> 30) + "GiB";
   std::snprintf(buf, n, "%s", name.c_str());
   return FFS_OK;
 }

@@ -39,7 +39,7 @@ def cache_path():
 
 
 def device_name():
-    """The current device as the library reports it (name, architecture, compute units)."""
+    """The current device as the library reports it (architecture, compute units, memory)."""
     lib = _lib.load()
     buf = ctypes.create_string_buffer(256)
     _lib.check(lib.ffs_device_name(buf, len(buf)))

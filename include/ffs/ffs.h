@@ -399,7 +399,7 @@ typedef struct ffs_plan {
   int reserved[3];
 } ffs_plan;
 int ffs_get_plan(const ffs_sim* s, ffs_plan* out);
-/* The current device as "<name> <arch> <CUs>CU" (the tuner's cache key). */
+/* The current device as "<arch> <CUs>CU <memory>GiB" (the tuner's cache key: fields every process sees the same). */
 int ffs_device_name(char* buf, size_t n);
 /* Applies a plan (FFS_ERR_ARG, the plan unchanged, if the x pass cannot hold
  * the run's rows under it); waits for the run's queued work. */
