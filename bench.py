@@ -119,16 +119,17 @@ def reference_ranges(n, cutoff=2 / 3):
     return mx, 2 * f + 2, 2 * f + 2
 
 
-def measured_traffic(kernel, prec, n):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-    of this workload (profiles/step_kernels_<n>_<prec>.md), or None when none is recorded."""
+def measured_traffic(kernel, prec, n, launches_per_step=1):
+    """HBM bytes per launch of `kernel`: its bytes per step from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (profiles/step_kernels_<n>_<prec>.md) over this run's launches per
+    step (z slabs), or None when none is recorded."""
     path = os.path.join(REPO, "profiles", f"step_kernels_{n}_{prec}.md")
     if not os.path.exists(path):
         return None
-    key = f"traffic_bytes_per_launch {kernel} {prec} {n} ="
+    key = f"traffic_bytes_per_step {kernel} {prec} {n} ="
     for line in open(path):
         if line.startswith(key):
-            return int(float(line.split("=")[1]))
+            return int(float(line.split("=")[1]) / launches_per_step)
     return None
 
 
@@ -308,7 +309,7 @@ def bench(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom, prec, n),
+            "traffic": measured_traffic(dom, prec, n, d.get("launches_per_step", 1)),
             "traffic_source": f"profiles/step_kernels_{n}_{prec}.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, calibrated "
                               "by a 1 GiB copy and fill)",
             "alg_bytes_per_launch": d["alg_bytes"],
