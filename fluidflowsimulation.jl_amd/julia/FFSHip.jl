@@ -531,8 +531,9 @@ function writeoutput(s::FFSSimulation; closing::Bool = false)
     (f == 2 || f == 4 || f == 5 || f == 6) && writedlm("Eh.$it", zip(forcing_state(s, 2), forcing_state(s, 3)))
 end
 
-# run_simulation() (FluidFlowSimulation.jl:23-65) over the library
-function run_simulation(d::AbstractDict)
+# parameters(d) (simulationtypes.jl:242-365) over the library: the run of the `global` file's keys with
+# its u1.$start..u3.$start (and scalar.$start, rho.$start) loaded
+function load_simulation(d::AbstractDict)
     p = params_from_global(d)
     s = FFSSimulation(p)
     for f in 0:2
@@ -550,13 +551,23 @@ function run_simulation(d::AbstractDict)
         read!("rho.$(p.start)", a)
         set_field!(s, 3, a)
     end
+    return s
+end
+
+# the Stats/Scales/Numbers headers (writeheader, stats.jl:1-30), written when a run starts at iteration 0
+function writeheader(s::FFSSimulation)
+    for (name, t) in (("Stats.txt", 0), ("Scales.txt", 1), ("Numbers.txt", 2))
+        open(f -> write(f, stats_header(s, t), "\n"), name, "w")
+    end
+end
+
+# run_simulation() (FluidFlowSimulation.jl:23-65) over the library
+function run_simulation(d::AbstractDict)
+    s = load_simulation(d)
+    p = s.p
     nt = haskey(d, :nt) ? parse(Int, d[:nt]) : typemax(Int) - p.start
     lengthtime = haskey(d, :timeDuration) ? parse(Float64, d[:timeDuration]) : Inf
-    if p.start == 0
-        for (name, t) in (("Stats.txt", 0), ("Scales.txt", 1), ("Numbers.txt", 2))
-            open(f -> write(f, stats_header(s, t), "\n"), name, "w")
-        end
-    end
+    p.start == 0 && writeheader(s)
     initialize!(s)
     @assert nt + Int(scalar(s, FFS_SCALAR_ITERATION)) >= p.write_time
     @assert nt + Int(scalar(s, FFS_SCALAR_ITERATION)) >= p.dt_stat

@@ -71,3 +71,36 @@ def test_run_simulation_keeps_the_reference_loop():
         c = re.search(rf"#define {name} (-?\d+)", hdr).group(1)
         j = re.search(rf"const {name} = Int32\((-?\d+)\)", src).group(1)
         assert c == j, name
+
+
+DRIVER = os.path.join(PKG, "julia", "FFSHipDriver.jl")
+# AbstractSimulation's type parameters (sim_par, macros.jl:35) and what run_simulation(s, nt, T)
+# (FluidFlowSimulation.jl:31-65) reads of a simulation and calls on it
+SIM_PAR = ("EquationType", "VelocityTimeStepType", "PassiveScalarType", "DensityStratificationType",
+           "LESModelType", "ForcingType", "HyperViscosityType")
+DRIVER_FIELDS = ("iteration", "time", "dtoutput", "dtstats", "u", "passivescalar", "densitystratification",
+                 "lesmodel", "forcing", "hyperviscosity")
+DRIVER_CALLS = ("initialize!", "advance_in_time!", "calculate_rhs!", "writestats", "writeoutput")
+
+
+def test_driver_subtype_answers_the_reference_loop():
+    """FFSHipSimulation subtypes the reference's AbstractSimulation with its seven type parameters, holds
+    every field run_simulation reads, extends each function it calls (imported from FluidFlowSimulation, so
+    the reference's generic driver dispatches to them), and calls only FFSHip functions that exist."""
+    src = open(DRIVER).read()
+    m = re.search(r"mutable struct FFSHipSimulation\{([^}]*)\} <: FluidFlowSimulation\.AbstractSimulation\{([^}]*)\}",
+                  src)
+    assert m and m.group(1) == m.group(2) and len(m.group(1).split(",")) == len(SIM_PAR)
+    body = src[m.end():src.index("\nend", m.end())]
+    fields = [ln.strip().split("::")[0] for ln in body.strip().splitlines()]
+    assert set(DRIVER_FIELDS) <= set(fields), set(DRIVER_FIELDS) - set(fields)
+    imported = re.search(r"import FluidFlowSimulation: ([^\n]*)", src).group(1)
+    for f in DRIVER_CALLS:
+        assert f in [x.strip() for x in imported.split(",")], f
+        assert re.search(rf"^(function )?{re.escape(f)}\(s::FFSHipSimulation\)", src, flags=re.M), f
+    assert "FluidFlowSimulation.real!(::FFSFields) = nothing" in src
+    shim = open(SHIM).read()
+    defined = set(re.findall(r"^function (\w+!?)\(", shim, flags=re.M)) | set(
+        re.findall(r"^(\w+!?)\([^)]*\) =", shim, flags=re.M)) | set(re.findall(r"^const (\w+)", shim, flags=re.M))
+    for name in set(re.findall(r"FFSHip\.(\w+!?)", src)) - {"jl", "FFSSimulation"}:
+        assert name in defined, name
