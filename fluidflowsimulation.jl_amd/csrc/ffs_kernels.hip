@@ -1149,10 +1149,13 @@ __global__ void __launch_bounds__(128, FFS_XNL_WAVES) k_x_nl_reg_b(FFS_CALLS(XCa
 #define FFS_XNL_PAIR 1
 #endif
 template <int N> __host__ __device__ constexpr int xpair_maxr() { return plan_r(N, 0, 8) == 4 ? 8 : 4; }
-// FP64 only: in FP32 the register x pass's 16 points per lane and one exchange per transform are faster
-// than 8 points and two exchanges (measured at 256^3: 189 against 214 us)
+// FP32 too: at 256^3 the mirror-paired pass (102 VGPRs, four waves per SIMD) takes 62.8 us per slab where the
+// register pass (16 points per lane, 231 VGPRs, two waves) takes 87.9 (FP32 step 0.515 -> 0.476 ms)
+#ifndef FFS_XPAIR_F32
+#define FFS_XPAIR_F32 1
+#endif
 template <typename T, int N> __host__ __device__ constexpr bool xpair_ok() {
-  return FFS_XNL_PAIR && sizeof(T) == 8 && N % 8 == 0 && 64 % (N / 8) == 0 &&
+  return FFS_XNL_PAIR && (sizeof(T) == 8 || FFS_XPAIR_F32) && N % 8 == 0 && 64 % (N / 8) == 0 &&
          plan_r(N, 0, xpair_maxr<N>()) == 4 && plan_even(N, N / 8, xpair_maxr<N>());
 }
 
@@ -1188,7 +1191,7 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
   static_assert(ML == (N / plan_r(N, 0, MAXR, true)) / TP, "the two plans meet at the same positions");
   constexpr int NPI = xreg_npin<MODE>();
   constexpr int NPO = xreg_npout<MODE, KE>();
-  static_assert(xpair_ok<T, N>(), "mirror-paired x pass: FP64, a radix-4 end stage, an even plan");
+  static_assert(xpair_ok<T, N>(), "mirror-paired x pass: a radix-4 end stage, an even plan");
   extern __shared__ unsigned char smem_raw[];
   C<T>* twl = reinterpret_cast<C<T>*>(smem_raw);
   C<T>* lds = twl + tw_lds<T>(N);

@@ -1,7 +1,7 @@
 """Run-time autotuner for a run's kernel plan, with a persisted cache.
 
 A plan (ffs_plan, include/ffs/ffs.h) picks among kernel variants that compute
-the same step: the mirror-paired or the plain register x pass for FP64
+the same step: the mirror-paired or the plain register x pass for
 power-of-two rows, one or two wavefronts per x-pass block, the density /
 scalar / LES products as register launches of their own or in one general
 LDS pass, the forward z pass fused with the update or not, and the inverse
