@@ -374,7 +374,8 @@ def leg_fp32(args, n, nu, fp64_value):
     s = make_forced(n, nu, "fp32", 1 << 30, args.seed)
     s.advance_in_time(args.warmup)
     el = timed_steps(s, args.steps)
-    out["single"] = {"value": n**3 * args.steps / el, "ms_per_step": 1e3 * el / args.steps}
+    out["single"] = {"value": n**3 * args.steps / el, "ms_per_step": 1e3 * el / args.steps,
+                     "vs_fp64_value": n**3 * args.steps / el / fp64_value}
     s.close()
     log(f"fp32 single: {1e3 * el / args.steps:.3f} ms/step, {n**3 * args.steps / el:.4e} N^3*steps/s")
     nb = args.fp32_batch
