@@ -74,7 +74,9 @@ def candidates(options, base):
     slabs do not apply), the other fields as in `base`."""
     out = []
     zs = int(options.get("z_slab", 0) or 0)
-    slabs = [zs, 2 * zs, 0] if zs > 0 else None  # the default slab, twice as thick, and none
+    # the default slab, half as thick, and none: never thicker than the default, whose buffers ffs_footprint
+    # counts, so a tuned run holds no more device memory than its admission assumed
+    slabs = [zs, zs // 2, 0] if zs >= 16 else ([zs, 0] if zs > 0 else None)
     for split in ([1, 0] if options["x_split"] else [base["x_split"]]):
         reg = split == 1 or not options["x_split"]
         pairs = [1, 0] if (options["x_pair"] and reg) else [base["x_pair"]]

@@ -395,7 +395,9 @@ typedef struct ffs_plan {
   int fusion;  /* as ffs_set_fusion */
   int z_slab;  /* planes per z slab of the nonlinear term's inverse y, x and
                 * forward y passes (0: whole grid per launch); ffs_plan_options
-                * reports the default slab size of the run here (0: no slabs) */
+                * reports the default slab size of the run here (0: no slabs).
+                * Slabs thicker than the default may allocate device memory
+                * that ffs_footprint does not count */
   int reserved[3];
 } ffs_plan;
 int ffs_get_plan(const ffs_sim* s, ffs_plan* out);
