@@ -69,3 +69,12 @@ def test_les_128_fp32_matches_oracle():
     for step, err, dt, rdt, t, rt in run_pair(128, "fp32", 3, les_c=0.16):
         assert abs(dt - rdt) <= 1e-5 * rdt, (step, dt, rdt)
         assert err < 2e-5, (step, err)
+
+
+def test_forced_256_fp32_matches_oracle():
+    """The FP32 leg's own run: 256^3 with the library's default plan (mirror-paired x pass, 128-plane z
+    slabs), against the FP64 oracle."""
+    print_case("forced isotropic turbulence 256^3 FP32, AB3 CFL 0.5, shell forcing, 3 steps vs FP64 oracle")
+    for step, err, dt, rdt, t, rt in run_pair(256, "fp32", 3):
+        assert abs(dt - rdt) <= 1e-5 * rdt, (step, dt, rdt)
+        assert err < 2e-5, (step, err)
