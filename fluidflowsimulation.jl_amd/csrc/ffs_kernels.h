@@ -190,14 +190,16 @@ struct XOps {
   int rho_les = 0, phi_les = 0;
   double rho_cv = 0.0, phi_cv = 0.0;  // c Delta^2 of VorticityDiffusion
   double les_alpha = 0.0;             // c^2 Delta^2 (Smagorinsky's nu_t = alpha |S|)
-  // the run's plan (ffs_set_plan).  pair: FP64 power-of-two rows may take the mirror-paired pass; waves:
+  // the run's plan (ffs_set_plan).  pair: power-of-two rows may take the mirror-paired pass (2: FP64 u x w
+  // rows its looping form, which copies each next row group's first field pair into LDS ahead); waves:
   // wavefronts per block of the register passes (1 or 2); split: with a register-capable length, the
   // density / scalar fluxes and the Smagorinsky stress run as register launches of their own (0: all
   // products in one general LDS pass)
   int pair = 1, waves = 2, split = 1;
 };
-// whether the plan's choices change anything for a run of x length n with these products (pair: the
-// mirror-paired pass applies; split: both the register launches and the general pass can hold the row)
+// whether the plan's choices change anything for a run of x length n with these products (pair: 1 the
+// mirror-paired pass applies, 2 its looping form too; split: both the register launches and the general
+// pass can hold the row)
 template <typename T>
 void x_plan_options(int n, int mx, const XOps& ops, int* pair, int* waves, int* split);
 // whether n is a compiled transform length, and (empty: yes) whether the x pass of length n can run

@@ -20,6 +20,7 @@
 #include "fft_device.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <mutex>
 #include <stdexcept>
@@ -162,6 +163,28 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) 
   }
   // `red` is one array per kernel: a following call must not overwrite it before thread 0 has read it
   __syncthreads();
+}
+
+// Compute units of the current device (read once per device).
+static int device_cus() {
+  static std::atomic<int> cus[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("hipGetDevice failed");
+  int v = cus[dev & 63].load(std::memory_order_relaxed);
+  if (v == 0) {
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+      throw std::runtime_error("reading the device's compute unit count failed");
+    cus[dev & 63].store(v, std::memory_order_relaxed);
+  }
+  return v;
+}
+
+// Blocks of `threads` threads of a kernel (no dynamic LDS) that one compute unit holds at once.
+static int resident_blocks(const void* fn, int threads) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, threads, 0) != hipSuccess || nb <= 0)
+    throw std::runtime_error("reading a kernel's occupancy failed");
+  return nb;
 }
 
 // Raises a kernel's dynamic LDS limit to `bytes`, once per process and kernel.  Handles may be
@@ -1176,8 +1199,27 @@ template <typename T> __device__ __forceinline__ const C<T>* opaque_tw(const C<T
     return (const C<T>*)q;
   }
 }
+// The same for a table in a static LDS array: the pointer stays visibly derived from the array (an opaque zero
+// offset is added instead), so the compiler still tells its reads apart from LDS written by asynchronous
+// global-to-LDS loads and sets no wait for those in front of them.
+template <typename T> __device__ __forceinline__ const C<T>* opaque_tw_static(const C<T>* p) {
+  int z = 0;
+  asm volatile("" : "+s"(z));
+  return p + z;
+}
 
-template <typename T, int N, int MODE, bool KE>
+// The looping form (plan x_pair 2; FP64 u x w rows; DW wavefronts per block): each block runs row groups bX,
+// bX + gridDim.x, ... over a grid of resident blocks and, while a group's last inverse transform, products and
+// forward transforms run, its wavefronts copy the next group's first field pair from global memory straight
+// into LDS (global_load_lds: no registers held), so that pair is there when the next group starts.  Its LDS
+// is static, so the compiler keeps the exchange and twiddle reads apart from the copies in flight.  At 256^3
+// (64-plane slabs) the pass takes 208 us per step where the one-group form takes 225.
+template <typename T, int N> __host__ __device__ constexpr bool xdma_ok() {
+  return sizeof(T) == 8 && xpair_ok<T, N>();
+}
+constexpr int kXLoopWaves = 2;  // wavefronts per block of the looping form
+
+template <typename T, int N, int MODE, bool KE, int DW = 0>
 __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2, const C<T>* __restrict__ tw_g,
                                            unsigned long long* __restrict__ vmax, XRegArgs xa, int bX) {
   g = glb(g);
@@ -1193,23 +1235,43 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
   constexpr int NPI = xreg_npin<MODE>();
   constexpr int NPO = xreg_npout<MODE, KE>();
   static_assert(xpair_ok<T, N>(), "mirror-paired x pass: a radix-4 end stage, an even plan");
-  extern __shared__ unsigned char smem_raw[];
-  C<T>* twl = reinterpret_cast<C<T>*>(smem_raw);
-  C<T>* lds = twl + tw_lds<T>(N);
+  constexpr bool DMA = DW > 0;
+  static_assert(!DMA || sizeof(C<T>) == 16, "the staged copies move one complex FP64 value per lane");
+  // staged field pair of a wavefront: [field][chunk c][lane], chunk c holding modes c TP + (lane % TP) of
+  // row lane / TP; modes k < N/2 take 4 chunks
+  constexpr int SCH = 4 * 64;
+  C<T>* twl;
+  C<T>* lds;
+  C<T>* stg = nullptr;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if constexpr (DMA) {
+    __shared__ C<T> s_tw[tw_lds<T>(N) > 0 ? tw_lds<T>(N) : 1];
+    __shared__ C<T> s_x[DW * GPW * LDX];
+    __shared__ C<T> s_stage[DW * 2 * SCH];
+    twl = s_tw;
+    lds = s_x;
+    stg = s_stage + __builtin_amdgcn_readfirstlane(wave) * (2 * SCH);
+  } else {
+    extern __shared__ unsigned char smem_raw[];
+    twl = reinterpret_cast<C<T>*>(smem_raw);
+    lds = twl + tw_lds<T>(N);
+  }
   const C<T>* tw = stage_twiddles<T, N>(twl, tw_g);
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int grp = lane / TP, q = lane % TP;
   const int nrows = g.ny * g.nz;
-  const int row = (bX * (int)(blockDim.x >> 6) + wave) * GPW + grp;
-  const bool row_ok = row < nrows;
+  const int wpb = DMA ? DW : (int)(blockDim.x >> 6);
+  const int nbg = (nrows + wpb * GPW - 1) / (wpb * GPW);  // row groups of wpb GPW rows
   const int mx = g.mx;
   const size_t S2 = (size_t)g.sx * nrows;
-  const size_t roff = (size_t)(row_ok ? row : 0) * g.sx;
   C<T>* X = lds + (size_t)(wave * GPW + grp) * LDX;
   auto xref = [&](int p) -> C<T>& {
     if constexpr (FFS_XNL_PAD_SHIFT > 0) return X[p + (p >> FFS_XNL_PAD_SHIFT)];
     else return X[p];
+  };
+  auto twp = [&]() {
+    if constexpr (DMA && lds_twiddles<T>()) return opaque_tw_static<T>(tw);
+    else return opaque_tw<T>(tw);
   };
   auto slot_a = [&](int j) -> int {
     if constexpr (MODE == 1) return xa.in0 + 2 * j;
@@ -1225,6 +1287,34 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
   const int bt1 = q0 ? M / 2 : M - q;
   auto pos = [&](int t, int r) { return (t == 0 ? q : bt1) + r * M; };
   const C<T> zero = mk<T>(0, 0);
+  // a group's first field pair into this wavefront's stage: all 4 chunks of both fields, lanes past mx reading
+  // the row's last retained mode and a missing second field repeating the first (never used), so that the
+  // number of copies in flight is fixed and the compiler's waits for older loads can count past them
+  auto stage_pair0 = [&](int bgn) {
+    if constexpr (DMA) {
+      const int rown = (bgn * DW + wave) * GPW + grp;
+      const unsigned rb = (unsigned)(rown < nrows ? rown : 0) * (unsigned)g.sx;
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+        const int sl = f == 0 || slot_b(0) < 0 ? slot_a(0) : slot_b(0);
+        const C<T>* F = a2 + (size_t)sl * S2;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int k = c * TP + q;
+          const C<T>* src = F + rb + (unsigned)(k < mx ? k : mx - 1);
+          __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                           (__attribute__((address_space(3))) void*)(stg + f * SCH + c * 64), 16, 0, 0);
+        }
+      }
+    }
+  };
+  if constexpr (DMA) {
+    if (bX < nbg) stage_pair0(bX);
+  }
+  double lmax = 0.0;
+  auto group = [&](int bg) {  // the rows of row group bg
+  const int row = (bg * wpb + wave) * GPW + grp;
+  const bool row_ok = row < nrows;
 
   // the modes a pair needs: A[k], B[k] at the lo legs (k < mx), read with 32-bit offsets from the
   // fields' uniform bases; pair j + 1's loads are issued before pair j's transform
@@ -1245,10 +1335,33 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
       }
   };
   C<T> R[3][8];
-  fetch(0, la[0], lb[0]);
+  if constexpr (DMA) {  // the first pair from the stage
+    __syncwarp();
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int k = pos(t, r);
+        const int kk = (row_ok && k < mx) ? k : 0;
+        const int si = (kk / TP) * 64 + grp * TP + kk % TP;
+        la[0][t][r] = stg[si];
+        lb[0][t][r] = slot_b(0) < 0 ? zero : stg[SCH + si];
+      }
+  } else {
+    fetch(0, la[0], lb[0]);
+  }
 #pragma unroll
   for (int j = 0; j < NPI; ++j) {
     if (j + 1 < NPI) fetch(j + 1, la[(j + 1) & 1], lb[(j + 1) & 1]);
+    // the next group's first pair (the last group: its own again, unused), once the group's register loads have
+    // all arrived (vmcnt(0), which the last pair's products wait for anyway): the compiler does not count later
+    // waits past a copy into LDS, so a register load still outstanding behind one would wait for the copy too
+    if constexpr (DMA) {
+      if (j == NPI - 1) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        stage_pair0(bg + (int)gridDim.x < nbg ? bg + (int)gridDim.x : bg);
+      }
+    }
     // Z of the pair at the lane's 8 positions: A + iB at the lo legs, conj(A) + i conj(B) mirrored
     C<T> Z[2][4], mh[2][2];
 #pragma unroll
@@ -1267,11 +1380,10 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
     Z[1][3] = q0 ? mh[1][0] : mh[0][0];
     auto ld = [&](int, int t, int r) -> C<T> { return Z[t][r]; };
     auto st = [&](int, int t, int r, C<T> v) { R[j][t + r * ML] = v; };
-    if (j > 0) __syncwarp();
-    Fft<T, N, +1, TP, false, false, SYNC_WAVE, MAXR, 0, false, 0>::run(q, opaque_tw<T>(tw), ld, st, xref);
+    if (j > 0 || DMA) __syncwarp();
+    Fft<T, N, +1, TP, false, false, SYNC_WAVE, MAXR, 0, false, 0>::run(q, twp(), ld, st, xref);
   }
   // pointwise products at positions q + TP m
-  double lmax = 0.0;
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     if constexpr (MODE == 0) {  // u x w, 1/2|u|^2, max|u_i|
@@ -1313,7 +1425,7 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
     auto ld = [&](int, int t, int r) { return R[j][t + r * ML]; };
     auto st = [&](int, int t, int r, C<T> v) { W[t][r] = v; };
     __syncwarp();
-    Fft<T, N, -1, TP, false, false, SYNC_WAVE, MAXR, 0, true, NSTG - 1>::run(q, opaque_tw<T>(tw), ld, st, xref);
+    Fft<T, N, -1, TP, false, false, SYNC_WAVE, MAXR, 0, true, NSTG - 1>::run(q, twp(), ld, st, xref);
     int oa, ob;
     if constexpr (MODE == 0) {
       oa = 2 * j;
@@ -1346,6 +1458,13 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
         }
       }
   }
+  };
+  if constexpr (DMA) {
+    for (int bg = bX; bg < nbg; bg += (int)gridDim.x) group(bg);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): no copy into LDS outlives the block
+  } else {
+    group(bX);
+  }
   if (MODE != 1 && vmax) block_max_to(lmax, vmax);
 }
 
@@ -1359,6 +1478,20 @@ template <typename T, int N, int MODE, bool KE>
 __global__ void __launch_bounds__(128, FFS_XPAIR_WAVES) k_x_pair_b(FFS_CALLS(XCall<T>) calls, int nrun) {
   const XCall<T> c = load_call(calls, blockIdx.z);
   x_pair_body<T, N, MODE, KE>(c.g, glb(c.a2), glb(c.b2), glb(c.tw), glb(c.vmax), c.xa, blockIdx.x);
+}
+// the looping form: DW wavefronts per block, static LDS
+template <typename T, int N, int MODE, bool KE, int DW>
+__global__ void __launch_bounds__(64 * DW, FFS_XPAIR_WAVES) k_x_pair_l(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2,
+                                                        const C<T>* __restrict__ tw_g, unsigned long long* __restrict__ vmax,
+                                                        XRegArgs xa) {
+  if constexpr (xdma_ok<T, N>()) x_pair_body<T, N, MODE, KE, DW>(g, a2, b2, tw_g, vmax, xa, blockIdx.x);
+}
+template <typename T, int N, int MODE, bool KE, int DW>
+__global__ void __launch_bounds__(64 * DW, FFS_XPAIR_WAVES) k_x_pair_lb(FFS_CALLS(XCall<T>) calls, int nrun) {
+  if constexpr (xdma_ok<T, N>()) {
+    const XCall<T> c = load_call(calls, blockIdx.z);
+    x_pair_body<T, N, MODE, KE, DW>(c.g, glb(c.a2), glb(c.b2), glb(c.tw), glb(c.vmax), c.xa, blockIdx.x);
+  }
 }
 
 // C2R of nf fields (rows paired 2m, 2m+1) into the padded real layout
@@ -1526,6 +1659,19 @@ static void launch_xreg(const Geo& g, const void* a2, void* b2, const void* tw, 
   const int RPB = WPB * (64 / TP);
   const size_t lds = ((size_t)RPB * xreg_ldx<T, N>() + tw_lds<T>(N)) * sizeof(C<T>);
   const int nrows = g.ny * g.nz;
+  if constexpr (MODE == 0 && xdma_ok<T, N>()) {
+    if (ops.pair == 2) {  // the looping form: static LDS, a grid of resident blocks (x_waves does not apply)
+      constexpr int DW = kXLoopWaves;
+      const int nbg = (nrows + DW * (64 / TP) - 1) / (DW * (64 / TP));
+      const XCall<T> c{g, (const C<T>*)a2, (C<T>*)b2, (const C<T>*)tw, vmax, xa};
+      static const int resident = resident_blocks((const void*)&k_x_pair_l<T, N, MODE, KE, DW>, 64 * DW);
+      const dim3 grid(std::max(1, std::min(nbg, device_cus() * resident)));
+      launch_call((const void*)&k_x_pair_lb<T, N, MODE, KE, DW>, grid, dim3(64 * DW), 0, c, "x pass", [&] {
+        k_x_pair_l<T, N, MODE, KE, DW><<<grid, 64 * DW, 0, s>>>(c.g, c.a2, c.b2, c.tw, c.vmax, c.xa);
+      });
+      return;
+    }
+  }
   if constexpr (PAIR) {
     static std::once_flag once;
     lds_limit_once(once, (const void*)&k_x_pair<T, N, MODE, KE>, lds, "the x pass");
@@ -1613,7 +1759,7 @@ static void x_plan_options_n(int mx, const XOps& ops, int* pair, int* waves, int
   *split = (reg_split != reg_joint && x_check<T, N>(b).empty()) ? 1 : 0;
   const bool reg = x_general_slots<T, N>(ops) == 0;
   *waves = reg ? 1 : 0;
-  *pair = (reg && xpair_ok<T, N>() && mx <= N / 2) ? 1 : 0;
+  *pair = (reg && xpair_ok<T, N>() && mx <= N / 2) ? ((xdma_ok<T, N>() && !ops.ns) ? 2 : 1) : 0;
 }
 template <typename T>
 void x_plan_options(int n, int mx, const XOps& ops, int* pair, int* waves, int* split) {

@@ -202,7 +202,7 @@ struct ffs_sim {
   // plain steps run the forward z pass and the update as one kernel (ffs_set_fusion; default FP64 only)
   bool fuse_ok = true;
   // the x pass's plan (ffs_set_plan): mirror-paired rows, wavefronts per block, split product launches
-  int x_pair = 1, x_waves = 2, x_split = 1;
+  int x_pair = 2, x_waves = 2, x_split = 1;
   // z slabs of the nonlinear term's inverse y, x and forward y passes (ffs_set_plan): planes per slab (0: the
   // whole grid in one launch each) and the slab buffers (their capacity in planes)
   int z_slab = 0, slab_cap = 0;
@@ -2868,9 +2868,10 @@ int ffs_set_plan(ffs_sim* s, const ffs_plan* plan) {
   if (!s || !plan) return FFS_ERR_ARG;
   return guard(s, [&] {
     if (plan->x_waves != 1 && plan->x_waves != 2) throw FfsError(FFS_ERR_ARG, "x_waves must be 1 or 2");
+    if (plan->x_pair < 0 || plan->x_pair > 2) throw FfsError(FFS_ERR_ARG, "x_pair must be 0, 1 or 2");
     if (plan->z_slab < 0) throw FfsError(FFS_ERR_ARG, "z_slab must be >= 0");
     const int old[3] = {s->x_pair, s->x_waves, s->x_split};
-    s->x_pair = plan->x_pair != 0;
+    s->x_pair = plan->x_pair;
     s->x_waves = plan->x_waves;
     s->x_split = plan->x_split != 0;
     try {

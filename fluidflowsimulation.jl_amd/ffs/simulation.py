@@ -223,11 +223,13 @@ class Simulation:
         self._check(self._lib.ffs_set_plan(self._h, ctypes.byref(p)))
 
     def plan_options(self):
-        """Which plan fields change this run's kernels: dict of field -> bool; z_slab -> the run's default
-        planes per z slab (0: slabs do not apply)."""
+        """Which plan fields change this run's kernels: dict of field -> bool; x_pair -> the largest value that
+        applies (0: none, 1: the mirror-paired pass, 2: its looping form too); z_slab -> the run's default planes
+        per z slab (0: slabs do not apply)."""
         p = _lib.FfsPlan()
         self._check(self._lib.ffs_plan_options(self._h, ctypes.byref(p)))
-        return {k: (int(getattr(p, k)) if k == "z_slab" else bool(getattr(p, k))) for k in self.PLAN_FIELDS}
+        return {k: (int(getattr(p, k)) if k in ("z_slab", "x_pair") else bool(getattr(p, k)))
+                for k in self.PLAN_FIELDS}
 
     # ---------------------------------------------------------------- diagnostics
     def pop_stats(self):

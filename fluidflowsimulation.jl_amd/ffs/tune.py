@@ -79,10 +79,12 @@ def candidates(options, base):
     slabs = [zs, zs // 2, 0] if zs >= 16 else ([zs, 0] if zs > 0 else None)
     for split in ([1, 0] if options["x_split"] else [base["x_split"]]):
         reg = split == 1 or not options["x_split"]
-        pairs = [1, 0] if (options["x_pair"] and reg) else [base["x_pair"]]
+        pairs = list(range(int(options["x_pair"]), -1, -1)) if (options["x_pair"] and reg) else [base["x_pair"]]
         waves = [2, 1] if (options["x_waves"] and reg) else [base["x_waves"]]
         fusion = [1, 0] if options["fusion"] else [base["fusion"]]
         for pr, w, f in itertools.product(pairs, waves, fusion):
+            if pr == 2 and int(options["x_pair"]) == 2 and w != waves[0]:
+                continue  # the looping form has blocks of its own shape: x_waves does not change it
             c = {"x_pair": pr, "x_waves": w, "x_split": split, "fusion": f}
             if slabs is None:
                 if "z_slab" in base:

@@ -388,7 +388,10 @@ int ffs_set_fusion(ffs_sim* s, int on);
  * candidates (ffs/tune.py) and keeps the fastest per device, grid, precision
  * and model.  No counterpart in the reference (FFTW picks its own plans). */
 typedef struct ffs_plan {
-  int x_pair;  /* 1: power-of-two x passes (up to 512) use the mirror-paired kernel */
+  int x_pair;  /* 1: power-of-two x passes (up to 512) use the mirror-paired kernel;
+                * 2 (the default): FP64 u x w passes its looping form, which copies
+                * each next row group's first field pair into LDS ahead (other
+                * passes as 1); ffs_plan_options reports the largest that applies */
   int x_waves; /* wavefronts per block of the register x passes: 1 or 2 */
   int x_split; /* 1: density / scalar fluxes and the Smagorinsky stress in
                 * register x launches of their own; 0: one general x pass */

@@ -172,3 +172,47 @@ def test_slab_defaults_and_plan_options():
         assert opts["z_slab"] > 0 and plan["z_slab"] == opts["z_slab"] and 256 % plan["z_slab"] == 0
     with ffs.Simulation(make_params(32, "fp64", False, False, False)) as s:
         assert s.plan_options()["z_slab"] == 0 and s.plan["z_slab"] == 0
+
+
+# (name, (nx, ny, nz), z_slab): grids with more row groups than resident blocks, so every block of the looping
+# x pass runs several groups (length 256: 4 rows a group; length 64: 16)
+LOOP_CASES = [
+    ("256x64x128 whole grid", (256, 64, 128), 0),
+    ("256x64x128 slabs of 40 planes", (256, 64, 128), 40),
+    ("64x128x256 whole grid", (64, 128, 256), 0),
+]
+
+
+@pytest.mark.parametrize("name,shape,zs", LOOP_CASES, ids=[c[0] for c in LOOP_CASES])
+def test_looping_x_pass_matches_the_one_group_pass_bitwise(name, shape, zs):
+    """x_pair 2 (the FP64 u x w pass looping over row groups, each next group's first field pair copied into
+    LDS ahead) does the same arithmetic as x_pair 1: fields and stats rows (the 1/2|u|^2 variant) equal bit
+    for bit, graph-replayed."""
+    p = slab_params(shape, "fp64", {"forcing": True})
+    results = {}
+    for pair in (1, 2):
+        print_case(f"{name}: x_pair {pair}")
+        with ffs.Simulation(p) as s:
+            assert s.plan_options()["x_pair"] == 2
+            s.set_plan({"x_pair": pair, "z_slab": zs})
+            assert s.plan["x_pair"] == pair and s.plan["z_slab"] == zs
+            s.seed_velocity(41)
+            s.initialize()
+            s.advance(9)
+            fields = [s.get_field(c, ffs._lib.FFS_SPACE_FOURIER) for c in range(3)]
+            results[pair] = (fields, s.pop_stats())
+    (f1, r1), (f2, r2) = results[1], results[2]
+    assert len(r1) > 0 and len(r1) == len(r2)
+    for c in range(3):
+        assert np.array_equal(f1[c], f2[c]), c
+    for a, b in zip(r1, r2):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y, equal_nan=True)
+
+
+def test_x_pair_options():
+    """The looping x pass applies to FP64 u x w rows (x_pair option 2, the default plan); FP32 runs report 1."""
+    with ffs.Simulation(make_params(64, "fp64", False, False, False)) as s:
+        assert s.plan_options()["x_pair"] == 2 and s.plan["x_pair"] == 2
+    with ffs.Simulation(make_params(64, "fp32", False, False, False)) as s:
+        assert s.plan_options()["x_pair"] == 1

@@ -35,6 +35,16 @@ def test_candidates_cover_the_fields_that_matter():
     assert len(c) == 4 and all(x["z_slab"] == 0 for x in c)
 
 
+def test_candidates_with_the_looping_x_pass():
+    """x_pair option 2: candidates 2, 1 and 0; the looping form (2) once per fusion choice, whatever x_waves."""
+    opts = {"x_pair": 2, "x_waves": True, "x_split": False, "fusion": True}
+    c = tune.candidates(opts, {**BASE, "x_pair": 2})
+    assert len({json.dumps(x, sort_keys=True) for x in c}) == len(c)
+    assert sorted((x["x_pair"], x["x_waves"]) for x in c if x["fusion"] == 1) == [(0, 1), (0, 2), (1, 1), (1, 2),
+                                                                                 (2, 2)]
+    assert sum(x["x_pair"] == 2 for x in c) == 2
+
+
 def test_plan_key_separates_what_changes_the_kernels():
     k = tune.plan_key(params())
     assert k != tune.plan_key(params(precision="fp32"))
