@@ -1311,7 +1311,13 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
   if constexpr (DMA) {
     if (bX < nbg) stage_pair0(bX);
   }
-  double lmax = 0.0;
+  // max|.| of the lane's values (fmax drops NaN) and whether any was NaN: together the NaN-propagating maximum
+  T vmax_l = 0;
+  bool nan_l = false;
+  auto track = [&](T a, T b, T c) {
+    vmax_l = fmax(vmax_l, fmax(fabs(a), fmax(fabs(b), fabs(c))));
+    nan_l = nan_l || a != a || b != b || c != c;
+  };
   auto group = [&](int bg) {  // the rows of row group bg
   const int row = (bg * wpb + wave) * GPW + grp;
   const bool row_ok = row < nrows;
@@ -1392,7 +1398,7 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
       if constexpr (KE) ke = (T)0.5 * (ux * ux + uy * uy + uz * uz);
       R[0][m] = mk<T>(uy * wz - uz * wy, uz * wx - ux * wz);
       R[1][m] = mk<T>(ux * wy - uy * wx, ke);
-      if (row_ok) lmax = nanmax(lmax, (double)nanmax(fabs(ux), nanmax(fabs(uy), fabs(uz))));
+      if (row_ok) track(ux, uy, uz);
     } else if constexpr (MODE == 1) {  // tau = c2a |S| S
       const T sxx = R[0][m].x, sxy = R[0][m].y, sxz = R[1][m].x, syy = R[1][m].y, syz = R[2][m].x;
       const T szz = -sxx - syy;
@@ -1410,12 +1416,12 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
       R[0][m] = mk<T>(xx - t, -(ux * uy));
       R[1][m] = mk<T>(-(ux * uz), yy - t);
       R[2][m] = mk<T>(-(uy * uz), ke);
-      if (row_ok) lmax = nanmax(lmax, (double)nanmax(fabs(ux), nanmax(fabs(uy), fabs(uz))));
+      if (row_ok) track(ux, uy, uz);
     } else {  // -rho u, max|rho|
       const T ux = R[0][m].x, uy = R[0][m].y, uz = R[1][m].x, rho = R[1][m].y;
       R[0][m] = mk<T>((-rho) * ux, (-rho) * uy);
       R[1][m] = mk<T>((-rho) * uz, (T)0);
-      if (row_ok) lmax = nanmax(lmax, (double)fabs(rho));
+      if (row_ok) track(rho, rho, rho);
     }
   }
   // forward transforms (the plan reversed, mirror pairs in its last stage), split in registers
@@ -1465,7 +1471,7 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
   } else {
     group(bX);
   }
-  if (MODE != 1 && vmax) block_max_to(lmax, vmax);
+  if (MODE != 1 && vmax) block_max_to(nan_l ? __builtin_nan("") : (double)vmax_l, vmax);
 }
 
 template <typename T, int N, int MODE, bool KE>

@@ -59,6 +59,32 @@ def test_roundtrip_transform():
                 assert err_r < 1e-13, (shape, c, err_r)
 
 
+@pytest.mark.parametrize("prec,pair", [("fp64", 2), ("fp64", 1), ("fp64", 0), ("fp32", 1)])
+def test_umax_is_the_nan_propagating_maximum(prec, pair):
+    """find_max (models.jl: maximum(abs, u), which propagates NaN) as the x pass reduces it for set_dt (variable
+    dt): on a random field max|u_i| equals the maximum over the real-space components; one NaN makes it NaN."""
+    n = 32
+    p = ffs.Params(nx=n, ny=n, nz=n, nu=1e-3, integrator="Adams_Bashforth3rdO", variable_dt=True, cfl=0.5,
+                   dt_stat=1 << 30, dt_spec=1 << 30, write_time=1 << 30, precision=prec)
+    u0 = random_spectrum(n, n, n, seed=5)
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    for plant in (False, True):
+        print_case(f"{prec} x_pair {pair} {'one NaN' if plant else 'finite'}")
+        with ffs.Simulation(p) as s:
+            s.set_plan({"x_pair": pair})
+            u = [np.array(c, copy=True) for c in u0]
+            if plant:
+                u[1][3, 2, 1] = np.nan
+            s.set_velocity(*u)
+            s.initialize()
+            umax = s.scalar(_lib.FFS_SCALAR_UMAX)
+            if plant:
+                assert np.isnan(umax)
+            else:
+                want = max(np.abs(s.get_field(c)[:n]).max() for c in range(3))
+                assert abs(umax - want) <= tol * want, (umax, want)
+
+
 def test_unpadded_and_padded_inputs_agree():
     n = 16
     u0 = random_spectrum(n, seed=3)
