@@ -1219,6 +1219,30 @@ template <typename T, int N> __host__ __device__ constexpr bool xdma_ok() {
 }
 constexpr int kXLoopWaves = 2;  // wavefronts per block of the looping form
 
+// A wavefront's rows of a field as a buffer resource (raw, stride 0): loads through it check the offset
+// against the range and read zeros past it, so positions outside the retained modes and rows past the grid
+// need no select on the loaded values.  kBufOut is an offset past every range.
+constexpr unsigned kBufOut = 0xFFFFFFF0u;
+#if defined(__HIP_DEVICE_COMPILE__)
+using BufRsrc = __amdgpu_buffer_rsrc_t;
+template <typename T> __device__ __forceinline__ BufRsrc rows_rsrc(const C<T>* rows, int nvals) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)rows, (short)0, nvals * (int)sizeof(C<T>), 0x00020000);
+}
+template <typename T> __device__ __forceinline__ C<T> buf_load(BufRsrc r, unsigned off) {
+  if constexpr (sizeof(C<T>) == 16) return __builtin_bit_cast(C<T>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  else return __builtin_bit_cast(C<T>, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+// 16 bytes per lane into LDS at dst + 16 * lane (dst the same for the whole wavefront)
+__device__ __forceinline__ void buf_load_lds16(BufRsrc r, void* dst, unsigned off) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)dst, 16, off, 0, 0, 0);
+}
+#else  // the host pass parses the device bodies but never runs them
+struct BufRsrc {};
+template <typename T> BufRsrc rows_rsrc(const C<T>*, int) { return {}; }
+template <typename T> C<T> buf_load(BufRsrc, unsigned) { return {}; }
+inline void buf_load_lds16(BufRsrc, void*, unsigned) {}
+#endif
+
 template <typename T, int N, int MODE, bool KE, int DW = 0>
 __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, C<T>* __restrict__ b2, const C<T>* __restrict__ tw_g,
                                            unsigned long long* __restrict__ vmax, XRegArgs xa, int bX) {
@@ -1243,14 +1267,14 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
   C<T>* twl;
   C<T>* lds;
   C<T>* stg = nullptr;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if constexpr (DMA) {
     __shared__ C<T> s_tw[tw_lds<T>(N) > 0 ? tw_lds<T>(N) : 1];
     __shared__ C<T> s_x[DW * GPW * LDX];
     __shared__ C<T> s_stage[DW * 2 * SCH];
     twl = s_tw;
     lds = s_x;
-    stg = s_stage + __builtin_amdgcn_readfirstlane(wave) * (2 * SCH);
+    stg = s_stage + wave * (2 * SCH);
   } else {
     extern __shared__ unsigned char smem_raw[];
     twl = reinterpret_cast<C<T>*>(smem_raw);
@@ -1264,6 +1288,7 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
   const int nbg = (nrows + wpb * GPW - 1) / (wpb * GPW);  // row groups of wpb GPW rows
   const int mx = g.mx;
   const size_t S2 = (size_t)g.sx * nrows;
+  const unsigned lro = (unsigned)(grp * g.sx);  // the lane's row within its wavefront's GPW rows
   C<T>* X = lds + (size_t)(wave * GPW + grp) * LDX;
   auto xref = [&](int p) -> C<T>& {
     if constexpr (FFS_XNL_PAD_SHIFT > 0) return X[p + (p >> FFS_XNL_PAD_SHIFT)];
@@ -1287,23 +1312,23 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
   const int bt1 = q0 ? M / 2 : M - q;
   auto pos = [&](int t, int r) { return (t == 0 ? q : bt1) + r * M; };
   const C<T> zero = mk<T>(0, 0);
-  // a group's first field pair into this wavefront's stage: all 4 chunks of both fields, lanes past mx reading
-  // the row's last retained mode and a missing second field repeating the first (never used), so that the
-  // number of copies in flight is fixed and the compiler's waits for older loads can count past them
+  // a group's first field pair into this wavefront's stage: all 4 chunks of both fields (modes past mx and
+  // rows past the grid read as zeros through the range check; a missing second field repeats the first, never
+  // used), so that the number of copies in flight is fixed and the compiler's waits for older loads can count
+  // past them
   auto stage_pair0 = [&](int bgn) {
     if constexpr (DMA) {
-      const int rown = (bgn * DW + wave) * GPW + grp;
-      const unsigned rb = (unsigned)(rown < nrows ? rown : 0) * (unsigned)g.sx;
+      const int row0n = (bgn * DW + wave) * GPW;
+      const bool okn = row0n + grp < nrows;
 #pragma unroll
       for (int f = 0; f < 2; ++f) {
         const int sl = f == 0 || slot_b(0) < 0 ? slot_a(0) : slot_b(0);
-        const C<T>* F = a2 + (size_t)sl * S2;
+        const auto RF = rows_rsrc<T>(a2 + (size_t)sl * S2 + (size_t)row0n * g.sx, GPW * g.sx);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           const int k = c * TP + q;
-          const C<T>* src = F + rb + (unsigned)(k < mx ? k : mx - 1);
-          __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                           (__attribute__((address_space(3))) void*)(stg + f * SCH + c * 64), 16, 0, 0);
+          const unsigned o = (okn && k < mx) ? (lro + (unsigned)k) * (unsigned)sizeof(C<T>) : kBufOut;
+          buf_load_lds16(RF, stg + f * SCH + c * 64, o);
         }
       }
     }
@@ -1322,22 +1347,23 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
   const int row = (bg * wpb + wave) * GPW + grp;
   const bool row_ok = row < nrows;
 
-  // the modes a pair needs: A[k], B[k] at the lo legs (k < mx), read with 32-bit offsets from the
-  // fields' uniform bases; pair j + 1's loads are issued before pair j's transform
+  // the modes a pair needs: A[k], B[k] at the lo legs, read through buffer resources over the wavefront's
+  // rows (zeros for k >= mx and rows past the grid); pair j + 1's loads are issued before pair j's transform
+  const int row0 = (bg * wpb + wave) * GPW;
   const unsigned rbase = (unsigned)(row_ok ? row : 0) * (unsigned)g.sx;
   C<T> la[2][2][2], lb[2][2][2];  // [buffer][t][r]
   auto fetch = [&](int j, C<T> (&fa)[2][2], C<T> (&fb)[2][2]) {
-    const C<T>* A = a2 + (size_t)slot_a(j) * S2;
     const int sb = slot_b(j);
-    const C<T>* B = a2 + (size_t)(sb < 0 ? 0 : sb) * S2;
+    const auto RA = rows_rsrc<T>(a2 + (size_t)slot_a(j) * S2 + (size_t)row0 * g.sx, GPW * g.sx);
+    const auto RB = rows_rsrc<T>(a2 + (size_t)(sb < 0 ? 0 : sb) * S2 + (size_t)row0 * g.sx, GPW * g.sx);
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
       for (int r = 0; r < 2; ++r) {
         const int k = pos(t, r);
-        const unsigned o = rbase + (unsigned)((row_ok && k < mx) ? k : 0);  // every lane loads
-        fa[t][r] = A[o];
-        fb[t][r] = sb < 0 ? zero : B[o];
+        const unsigned o = (row_ok && k < mx) ? (lro + (unsigned)k) * (unsigned)sizeof(C<T>) : kBufOut;
+        fa[t][r] = buf_load<T>(RA, o);
+        fb[t][r] = sb < 0 ? zero : buf_load<T>(RB, o);
       }
   };
   C<T> R[3][8];
@@ -1347,9 +1373,8 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
     for (int t = 0; t < 2; ++t)
 #pragma unroll
       for (int r = 0; r < 2; ++r) {
-        const int k = pos(t, r);
-        const int kk = (row_ok && k < mx) ? k : 0;
-        const int si = (kk / TP) * 64 + grp * TP + kk % TP;
+        const int k = pos(t, r);  // < N/2: within the 4 chunks
+        const int si = (k / TP) * 64 + grp * TP + k % TP;
         la[0][t][r] = stg[si];
         lb[0][t][r] = slot_b(0) < 0 ? zero : stg[SCH + si];
       }
@@ -1374,11 +1399,11 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
     for (int t = 0; t < 2; ++t)
 #pragma unroll
       for (int r = 0; r < 2; ++r) {
-        const int k = pos(t, r);
-        const bool in = row_ok && k < mx;
+        // positions outside the retained modes hold loaded zeros, so Z and mh are +0 there without a select;
+        // the mean mode (lane 0's first position) takes the real parts alone
         const C<T> a = la[j & 1][t][r], b = lb[j & 1][t][r];
-        Z[t][r] = in ? (k == 0 ? mk<T>(a.x, b.x) : mk<T>(a.x - b.y, a.y + b.x)) : zero;
-        mh[t][r] = in ? mk<T>(a.x + b.y, b.x - a.y) : zero;
+        Z[t][r] = (t == 0 && r == 0 && q0) ? mk<T>(a.x, b.x) : mk<T>(a.x - b.y, a.y + b.x);
+        mh[t][r] = mk<T>(a.x + b.y, b.x - a.y);
       }
     Z[0][2] = q0 ? zero : mh[1][1];  // lane 0: the Nyquist position, zero (N/2 >= mx)
     Z[0][3] = q0 ? mh[0][1] : mh[1][0];
