@@ -43,10 +43,19 @@ template <int N> __host__ __device__ constexpr int tper() { return N / default_p
 constexpr int kColThreads = 512;
 // columns per tile: one 128-byte line of x-consecutive values, halved while a tile of columns of length n (at
 // 16 points per thread) would exceed the block (FP32 columns of 640 and 768)
-template <typename T> __host__ __device__ constexpr int col_tx(int n) {
-  int tx = 128 / (2 * (int)sizeof(T));
+template <typename T> __host__ __device__ constexpr int col_tx(int n, int line = 128) {
+  int tx = line / (2 * (int)sizeof(T));
   while (tx > 1 && tx * (n / default_ppt(n)) > kColThreads) tx /= 2;
   return tx;
+}
+// the inverse-z pass that forms w: bytes per tile line (each block writes one line per z position).  Lines of
+// 256 bytes (twice the columns per block, half the blocks per CU) took 161.4 us per 256^3 FP64 step against
+// 152.8, FP32 89.1 against 81.1
+#ifndef FFS_CURL_LINE
+#define FFS_CURL_LINE 128
+#endif
+template <typename T, int N, bool CURL> __host__ __device__ constexpr int col_txk() {
+  return CURL ? col_tx<T>(N, FFS_CURL_LINE) : col_tx<T>(N);
 }
 // The inverse-z pass that forms w = ik x u (and rho, S) on load issues two loads per point in its
 // first stage; 8 points per thread (radix <= 8) instead of 16 halves the registers those loads hold.
@@ -72,7 +81,7 @@ constexpr int kXcd = 8;
 template <int N> __host__ __device__ constexpr bool ppt8_ok() { return N % 8 == 0 && plan_even(N, N / 8, 8); }
 template <typename T, int N, bool CURL> __host__ __device__ constexpr bool col_ppt8() {
   return ppt8_ok<N>() && ((FFS_COL_PPT8 == 0 && CURL) || FFS_COL_PPT8 == 1) &&
-         col_tx<T>(N) * (N / 8) <= kColThreads;
+         col_txk<T, N, CURL>() * (N / 8) <= kColThreads;
 }
 template <typename T, int N, bool CURL> __host__ __device__ constexpr int col_tper() {
   return col_ppt8<T, N, CURL>() ? N / 8 : tper<N>();
@@ -317,7 +326,7 @@ __device__ __forceinline__ void col_body(Geo g, const C<T>* __restrict__ in, C<T
                                          const C<T>* __restrict__ tw_g, int nzero, T scale, int bX, int bY, int bZ,
                                          int z0, int nzc) {
   g = glb(g);
-  constexpr int TX = col_tx<T>(N);
+  constexpr int TX = col_txk<T, N, CURL>();
   constexpr int TP = col_tper<T, N, CURL>();
   constexpr int MAXR = col_maxr<T, N, CURL>();
   static_assert(TX * TP <= kColThreads, "column block exceeds its launch bound");
@@ -498,7 +507,7 @@ __global__ void __launch_bounds__(kColThreads, FFS_COL_WAVES) k_col_b(FFS_CALLS(
 template <typename T, int N, int KIND, bool CURL, bool GRAD = false>
 static void run_col(const Geo& g, const void* in, void* out, int nf, int nb, const void* tw, int nzero, double scale,
                     hipStream_t s, const ColAux<T>& aux = ColAux<T>{}, int z0 = 0) {
-  constexpr int TX = col_tx<T>(N);
+  constexpr int TX = col_txk<T, N, CURL>();
   constexpr int TP = col_tper<T, N, CURL>();
   static_assert(TX * TP <= kColThreads, "column block exceeds its launch bound");
   const size_t lds = (size_t)(N * TX + tw_lds<T>(N)) * sizeof(C<T>);
