@@ -1228,6 +1228,19 @@ template <typename T, int N> __host__ __device__ constexpr bool xdma_ok() {
 }
 constexpr int kXLoopWaves = 2;  // wavefronts per block of the looping form
 
+// Exchange layout of the paired pass: position p of a row at p plus pad slots, added (not XOR-ed), so the
+// compiler folds a stage's per-position constants into the LDS instructions' offsets.  Measured on the device
+// (SQ_LDS_BANK_CONFLICT of the 256-point row's exchanges replayed per candidate, then the x pass itself): FP64
+// keeps one pad per 8 values (2.0 conflict cycles per access; no two-term pad did better; one per 16: 5.3, the
+// x pass +3 %; XOR swizzles halve the conflicts but spill the looping form's registers, +12 %); FP32 takes
+// three per 32 (fewer LDS instructions and registers, five waves per SIMD: x pass 125.3 -> 119-120 us per
+// 256^3 step, although its accesses conflict more).
+template <typename T> __host__ __device__ constexpr int xpair_slot(int p) {
+  return sizeof(C<T>) == 16 ? p + (p >> 3) : p + 3 * (p >> 5);
+}
+// values per row: the slot of the last position, plus one
+template <typename T, int N> __host__ __device__ constexpr int xpair_ldx() { return xpair_slot<T>(N - 1) + 1; }
+
 // A wavefront's rows of a field as a buffer resource (raw, stride 0): loads through it check the offset
 // against the range and read zeros past it, so positions outside the retained modes and rows past the grid
 // need no select on the loaded values.  kBufOut is an offset past every range.
@@ -1259,7 +1272,7 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
   constexpr int TP = N / 8;
   constexpr int GPW = 64 / TP;  // rows per wavefront
   constexpr int MAXR = xpair_maxr<N>();
-  constexpr int LDX = xreg_ldx<T, N>();
+  constexpr int LDX = xpair_ldx<T, N>();
   constexpr int NSTG = plan_ns(N, MAXR);
   constexpr int M = N / 4;  // butterflies of the radix-4 end stages
   // register stride of the inverse's last-stage legs = of the forward's first-stage legs
@@ -1300,8 +1313,7 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
   const unsigned lro = (unsigned)(grp * g.sx);  // the lane's row within its wavefront's GPW rows
   C<T>* X = lds + (size_t)(wave * GPW + grp) * LDX;
   auto xref = [&](int p) -> C<T>& {
-    if constexpr (FFS_XNL_PAD_SHIFT > 0) return X[p + (p >> FFS_XNL_PAD_SHIFT)];
-    else return X[p];
+    return X[xpair_slot<T>(p)];
   };
   auto twp = [&]() {
     if constexpr (DMA && lds_twiddles<T>()) return opaque_tw_static<T>(tw);
@@ -1697,7 +1709,7 @@ static void launch_xreg(const Geo& g, const void* a2, void* b2, const void* tw, 
   const int WPB = ops.waves;  // wavefronts per block
   if (WPB != 1 && WPB != 2) throw std::runtime_error("x pass: 1 or 2 wavefronts per block");
   const int RPB = WPB * (64 / TP);
-  const size_t lds = ((size_t)RPB * xreg_ldx<T, N>() + tw_lds<T>(N)) * sizeof(C<T>);
+  const size_t lds = ((size_t)RPB * xpair_ldx<T, N>() + tw_lds<T>(N)) * sizeof(C<T>);
   const int nrows = g.ny * g.nz;
   if constexpr (MODE == 0 && xdma_ok<T, N>()) {
     if (ops.pair == 2) {  // the looping form: static LDS, a grid of resident blocks (x_waves does not apply)
