@@ -1228,31 +1228,18 @@ template <typename T, int N> __host__ __device__ constexpr bool xdma_ok() {
 }
 constexpr int kXLoopWaves = 2;  // wavefronts per block of the looping form
 
-// Exchange layouts of the paired pass: position p of a row at p plus pad slots, added (not XOR-ed, so the
-// compiler folds a stage's per-position constants into the LDS instructions' offsets; XOR swizzles spilled the
-// looping form's registers).  INV: the inverse transform's exchange, else the forward's; XI: the storing stage.
-// From SQ_LDS_BANK_CONFLICT measured on the device for each of the 256-point row's store and load patterns
-// (radix 4, 8, 8 and back, 32 lanes a row, two rows a wavefront) under 179 pad layouts, FP64 takes the best
-// layout per exchange within the row's current LDS: 2.37 -> 1.50 conflict cycles per LDS access in the x pass,
-// its time about -1 %.  FP32 keeps three pads per 32 values in every exchange: per-exchange layouts cut its
-// conflicts from 2.58 to 0.57 per access but hold more registers (106 against 94: four waves per SIMD, not
-// five), and the pass took 124-126 us per 256^3 step against 119-121.
-template <typename T, bool INV, int XI> __host__ __device__ constexpr int xpair_slot(int p) {
-  if constexpr (sizeof(C<T>) == 16) {
-    if constexpr (XI == 0) return p + (p >> 3);
-    else return INV ? p + 4 * (p >> 5) : p + (p >> 7);
-  } else {
-    return p + 3 * (p >> 5);
-  }
+// Exchange layout of the paired pass: position p of a row at p plus pad slots, added (not XOR-ed), so the
+// compiler folds a stage's per-position constants into the LDS instructions' offsets.  Measured on the device
+// (SQ_LDS_BANK_CONFLICT of the 256-point row's exchanges replayed per candidate, then the x pass itself): FP64
+// keeps one pad per 8 values (2.0 conflict cycles per access; no two-term pad did better; one per 16: 5.3, the
+// x pass +3 %; XOR swizzles halve the conflicts but spill the looping form's registers, +12 %); FP32 takes
+// three per 32 (fewer LDS instructions and registers, five waves per SIMD: x pass 125.3 -> 119-120 us per
+// 256^3 step, although its accesses conflict more).
+template <typename T> __host__ __device__ constexpr int xpair_slot(int p) {
+  return sizeof(C<T>) == 16 ? p + (p >> 3) : p + 3 * (p >> 5);
 }
-// values per row: the largest slot of the four layouts (each grows with p), plus one
-template <typename T, int N> __host__ __device__ constexpr int xpair_ldx() {
-  int m = xpair_slot<T, true, 0>(N - 1);
-  m = xpair_slot<T, true, 1>(N - 1) > m ? xpair_slot<T, true, 1>(N - 1) : m;
-  m = xpair_slot<T, false, 0>(N - 1) > m ? xpair_slot<T, false, 0>(N - 1) : m;
-  m = xpair_slot<T, false, 1>(N - 1) > m ? xpair_slot<T, false, 1>(N - 1) : m;
-  return m + 1;
-}
+// values per row: the slot of the last position, plus one
+template <typename T, int N> __host__ __device__ constexpr int xpair_ldx() { return xpair_slot<T>(N - 1) + 1; }
 
 // A wavefront's rows of a field as a buffer resource (raw, stride 0): loads through it check the offset
 // against the range and read zeros past it, so positions outside the retained modes and rows past the grid
@@ -1325,9 +1312,9 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
   const size_t S2 = (size_t)g.sx * nrows;
   const unsigned lro = (unsigned)(grp * g.sx);  // the lane's row within its wavefront's GPW rows
   C<T>* X = lds + (size_t)(wave * GPW + grp) * LDX;
-  // the inverse and the forward transforms' exchange accessors (layout per exchange)
-  auto xinv = [&](int p, auto xi) -> C<T>& { return X[xpair_slot<T, true, decltype(xi)::value>(p)]; };
-  auto xfwd = [&](int p, auto xi) -> C<T>& { return X[xpair_slot<T, false, decltype(xi)::value>(p)]; };
+  auto xref = [&](int p) -> C<T>& {
+    return X[xpair_slot<T>(p)];
+  };
   auto twp = [&]() {
     if constexpr (DMA && lds_twiddles<T>()) return opaque_tw_static<T>(tw);
     else return opaque_tw<T>(tw);
@@ -1446,7 +1433,7 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
     auto ld = [&](int, int t, int r) -> C<T> { return Z[t][r]; };
     auto st = [&](int, int t, int r, C<T> v) { R[j][t + r * ML] = v; };
     if (j > 0 || DMA) __syncwarp();
-    Fft<T, N, +1, TP, false, false, SYNC_WAVE, MAXR, 0, false, 0>::run(q, twp(), ld, st, xinv);
+    Fft<T, N, +1, TP, false, false, SYNC_WAVE, MAXR, 0, false, 0>::run(q, twp(), ld, st, xref);
   }
   // pointwise products at positions q + TP m
 #pragma unroll
@@ -1490,7 +1477,7 @@ __device__ __forceinline__ void x_pair_body(Geo g, const C<T>* __restrict__ a2, 
     auto ld = [&](int, int t, int r) { return R[j][t + r * ML]; };
     auto st = [&](int, int t, int r, C<T> v) { W[t][r] = v; };
     __syncwarp();
-    Fft<T, N, -1, TP, false, false, SYNC_WAVE, MAXR, 0, true, NSTG - 1>::run(q, twp(), ld, st, xfwd);
+    Fft<T, N, -1, TP, false, false, SYNC_WAVE, MAXR, 0, true, NSTG - 1>::run(q, twp(), ld, st, xref);
     int oa, ob;
     if constexpr (MODE == 0) {
       oa = 2 * j;

@@ -18,7 +18,6 @@
 // the compiler could contract around it there.  Two kernels that transform the same data therefore
 // agree bit for bit (a fused pass and its unfused pair, a batched launch and a single one).
 #pragma once
-#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #pragma clang fp contract(off)
@@ -329,15 +328,6 @@ struct Stage {
   }
 };
 
-// The exchange accessor `lds` may take, as a second, compile-time argument, the index of the exchange (the stage
-// that stores into it), so that each exchange can place its values in a layout of its own.
-template <int X, class Lds> __device__ __forceinline__ decltype(auto) lds_x(const Lds& lds, int p) {
-  if constexpr (std::is_invocable_v<const Lds&, int, std::integral_constant<int, X>>)
-    return lds(p, std::integral_constant<int, X>{});
-  else
-    return lds(p);
-}
-
 // Exchange synchronisation: the whole block, or only the wavefront (when all
 // TPER threads of a transform live in one wavefront).
 enum { SYNC_BLOCK = 0, SYNC_WAVE = 1 };
@@ -377,10 +367,10 @@ struct Fft {
       S::store(v, q, st);
     } else {
       if constexpr (reads_lds) fft_sync<SYNC>();  // everyone has read the previous contents
-      S::store(v, q, [&](int p, int, int, C<T> x) { lds_x<STAGE>(lds, p) = x; });
+      S::store(v, q, [&](int p, int, int, C<T> x) { lds(p) = x; });
       fft_sync<SYNC>();
       Fft<T, N, DIR, TPER, IN_LDS, OUT_LDS, SYNC, MAXR, STAGE + 1, REV, MIRS>::run(
-          q, tw, [&](int p, int, int) { return lds_x<STAGE>(lds, p); }, st, lds);
+          q, tw, [&](int p, int, int) { return lds(p); }, st, lds);
     }
   }
 };
