@@ -91,6 +91,7 @@ def main():
     ap.add_argument("--no-copy-probe", dest="copy_probe", action="store_false")
     ap.add_argument("--plan", choices=["auto", "default"], default="auto",
                     help="kernel plans: tuned per workload on this device (cached, see ffs/tune.py) or the defaults")
+    ap.add_argument("--leg-child", default=None, help=argparse.SUPPRESS)  # run one leg, print its JSON (internal)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -98,6 +99,11 @@ def main():
         sys.exit(2)
     global PLAN_MODE
     PLAN_MODE = "auto" if args.plan == "auto" else None
+    if args.leg_child == "serve":  # the serving leg in a process of its own (see leg_in_child)
+        res = leg_serve(args)
+        res["_plans"] = PLANS
+        print(json.dumps(res), flush=True)
+        return 0
     raise SystemExit(bench(args))
 
 
@@ -340,7 +346,7 @@ def bench(args):
     if "ensemble" in legs:
         out["ensemble"] = leg_ensemble(args)
     if "serve" in legs:
-        out["serving"] = leg_serve(args)
+        out["serving"] = leg_in_child("serve")
     if "strat" in legs:
         out["stratified"] = leg_strat(args)
     if "features" in legs:
@@ -475,6 +481,23 @@ def leg_ensemble(args):
         f"in {out['merged_groups']} groups ({out['merged_runs']} runs) vs {out['branches']['launches_per_step']} "
         f"as branches at {out['branches']['ms_per_lockstep_step']:.3f} ms/step")
     return out
+
+
+def leg_in_child(name):
+    """Runs one auxiliary leg in a child process (the same arguments, the leg's JSON on the child's last stdout
+    line), so that a crash in that leg cannot take the bench's line down; a failed child is reported as
+    {"error": ...}.  The child is started as a new process (nothing is exec-replaced)."""
+    import subprocess
+
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), *sys.argv[1:], "--leg-child", name]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+    lines = (r.stdout or "").strip().splitlines()
+    if r.returncode != 0 or not lines:
+        log(f"{name}: the leg's process exited with status {r.returncode}")
+        return {"error": f"the leg's process exited with status {r.returncode}"}
+    res = json.loads(lines[-1])
+    PLANS.update(res.pop("_plans", {}))
+    return res
 
 
 def leg_serve(args):
