@@ -216,3 +216,13 @@ def test_x_pair_options():
         assert s.plan_options()["x_pair"] == 2 and s.plan["x_pair"] == 2
     with ffs.Simulation(make_params(64, "fp32", False, False, False)) as s:
         assert s.plan_options()["x_pair"] == 1
+
+
+def test_set_plan_rejects_an_unknown_x_pass_form():
+    """x_pair takes 0 (register pass), 1 (mirror-paired) or 2 (its looping form); anything else is an argument
+    error and leaves the plan as it was."""
+    with ffs.Simulation(make_params(32, "fp64", False, False, False)) as s:
+        before = dict(s.plan)
+        with pytest.raises(ffs._lib.FfsError):
+            s.set_plan({"x_pair": 3})
+        assert s.plan == before
