@@ -174,6 +174,27 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) 
   __syncthreads();
 }
 
+// A store marked non-temporal (streamed past the caches: the value is not read again soon).  The outputs read
+// back only a whole pass later are written so: the inverse z pass's (the inverse y pass reads them slab by
+// slab), the forward y pass's (read by the forward z pass after the last slab) and the update's u and
+// right-hand side (read at the next step), while the slab buffers, which the next pass reads at once from the
+// last-level cache, keep ordinary stores.  At 256^3, measured side by side: FP64 step 0.823 -> 0.758 ms (the
+// inverse z pass 153 -> 134 us and the inverse y pass 193 -> 178 with the first alone), FP32 0.482 -> 0.462.
+#ifndef FFS_CURL_NT
+#define FFS_CURL_NT 1  // the inverse-z pass that forms w
+#endif
+#ifndef FFS_NT_FWDY
+#define FFS_NT_FWDY 1  // the forward y pass
+#endif
+#ifndef FFS_NT_UPDATE
+#define FFS_NT_UPDATE 1  // the update's u and right-hand side
+#endif
+template <typename T> __device__ __forceinline__ void store_nt(C<T>* p, C<T> v) {
+  typedef T V2 __attribute__((ext_vector_type(2)));
+  const V2 w = {v.x, v.y};
+  __builtin_nontemporal_store(w, reinterpret_cast<V2*>(p));
+}
+
 // Compute units of the current device (read once per device).
 static int device_cus() {
   static std::atomic<int> cus[64];
@@ -436,7 +457,10 @@ __device__ __forceinline__ void col_body(Geo g, const C<T>* __restrict__ in, C<T
 #endif
       };
       auto st = [&](int p, int, int, C<T> v) {
-        if (valid) out[f * S1 + cx + sx * (cy + (size_t)g.my * p)] = v;
+        if (valid) {
+          if constexpr (FFS_CURL_NT && CURL) store_nt<T>(&out[f * S1 + cx + sx * (cy + (size_t)g.my * p)], v);
+          else out[f * S1 + cx + sx * (cy + (size_t)g.my * p)] = v;
+        }
       };
       Fft<T, N, +1, TP, false, false, SYNC_BLOCK, MAXR>::run(q, tw, ld, st, ldsf);
     }
@@ -468,7 +492,10 @@ __device__ __forceinline__ void col_body(Geo g, const C<T>* __restrict__ in, C<T
     };
     auto st = [&](int p, int, int, C<T> v) {
       const int cy = y_compact(g, p);
-      if (valid && cy >= 0) out[f * S1 + cx + sx * (cy + (size_t)g.my * zg)] = v;
+      if (valid && cy >= 0) {
+        if constexpr (FFS_NT_FWDY) store_nt<T>(&out[f * S1 + cx + sx * (cy + (size_t)g.my * zg)], v);
+        else out[f * S1 + cx + sx * (cy + (size_t)g.my * zg)] = v;
+      }
     };
     Fft<T, N, -1, TP, false, false, SYNC_BLOCK, MAXR>::run(q, tw, ld, st, ldsf);
   } else {  // FWD_Z
@@ -2120,7 +2147,10 @@ __device__ __forceinline__ void update_point(const Geo& g, C<T>* __restrict__ u,
   double RH[2] = {0, 0};
   form_rhs<T, FULL>(a, idx, n, kv3, K2, origin, U, R, RH);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) rhs[c * n + idx] = mk<T>((T)R[c][0], (T)R[c][1]);
+  for (int c = 0; c < 3; ++c) {
+    if constexpr (FFS_NT_UPDATE) store_nt<T>(&rhs[c * n + idx], mk<T>((T)R[c][0], (T)R[c][1]));
+    else rhs[c * n + idx] = mk<T>((T)R[c][0], (T)R[c][1]);
+  }
   // density rhs from the pre-step vertical velocity
   double RR[2] = {0, 0};
   if (FULL && a.rho) {
@@ -2259,7 +2289,10 @@ __device__ __forceinline__ void update_point(const Geo& g, C<T>* __restrict__ u,
     for (int c = 0; c < 3; ++c) U[c][p] = fma(p1, kv3[c], U[c][p]);
   }
 #pragma unroll
-  for (int c = 0; c < 3; ++c) u[c * n + idx] = mk<T>((T)U[c][0], (T)U[c][1]);
+  for (int c = 0; c < 3; ++c) {
+    if constexpr (FFS_NT_UPDATE) store_nt<T>(&u[c * n + idx], mk<T>((T)U[c][0], (T)U[c][1]));
+    else u[c * n + idx] = mk<T>((T)U[c][0], (T)U[c][1]);
+  }
   if (FULL && a.rho) ((C<T>*)a.rho)[idx] = mk<T>((T)RH[0], (T)RH[1]);
   if (FULL && a.phi) ((C<T>*)a.phi)[idx] = mk<T>((T)PH[0], (T)PH[1]);
 }
