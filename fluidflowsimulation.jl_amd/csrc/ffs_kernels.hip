@@ -176,9 +176,9 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) 
 
 // A store marked non-temporal (streamed past the caches: the value is not read again soon).  The outputs read
 // back only a whole pass later are written so: the inverse z pass's (the inverse y pass reads them slab by
-// slab), the forward y pass's (read by the forward z pass after the last slab) and the update's u and
-// right-hand side (read at the next step), while the slab buffers, which the next pass reads at once from the
-// last-level cache, keep ordinary stores.  At 256^3, measured side by side: FP64 step 0.823 -> 0.758 ms (the
+// slab), the forward y pass's (read by the forward z pass after the last slab) and the update's right-hand
+// side (read at the next step's update), while the slab buffers, which the next pass reads at once from the
+// last-level cache, and u, which the next step's inverse z pass reads first, keep ordinary stores.  At 256^3, measured side by side: FP64 step 0.823 -> 0.758 ms (the
 // inverse z pass 153 -> 134 us and the inverse y pass 193 -> 178 with the first alone), FP32 0.482 -> 0.462.
 #ifndef FFS_CURL_NT
 #define FFS_CURL_NT 1  // the inverse-z pass that forms w
@@ -187,7 +187,10 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* dst) 
 #define FFS_NT_FWDY 1  // the forward y pass
 #endif
 #ifndef FFS_NT_UPDATE
-#define FFS_NT_UPDATE 1  // the update's u and right-hand side
+#define FFS_NT_UPDATE 1  // the update's right-hand side (and u with FFS_NT_U)
+#endif
+#ifndef FFS_NT_U
+#define FFS_NT_U 0  // 0: u with ordinary stores (the next step's inverse z pass reads it first; FP32 62 vs 72 us)
 #endif
 template <typename T> __device__ __forceinline__ void store_nt(C<T>* p, C<T> v) {
   typedef T V2 __attribute__((ext_vector_type(2)));
@@ -2290,7 +2293,7 @@ __device__ __forceinline__ void update_point(const Geo& g, C<T>* __restrict__ u,
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    if constexpr (FFS_NT_UPDATE) store_nt<T>(&u[c * n + idx], mk<T>((T)U[c][0], (T)U[c][1]));
+    if constexpr (FFS_NT_UPDATE && FFS_NT_U) store_nt<T>(&u[c * n + idx], mk<T>((T)U[c][0], (T)U[c][1]));
     else u[c * n + idx] = mk<T>((T)U[c][0], (T)U[c][1]);
   }
   if (FULL && a.rho) ((C<T>*)a.rho)[idx] = mk<T>((T)RH[0], (T)RH[1]);
